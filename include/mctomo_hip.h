@@ -1,0 +1,129 @@
+/*
+ * mctomo_hip.h -- C ABI of libmctomo_hip.so, the MI355X (gfx950) likelihood
+ * forward model that sits under MCTomo's Fortran m_likelihood surface.
+ *
+ * Conventions (the reference's Fortran storage, passed with c_loc as in
+ * src/fastMarching_wrapper.f90:51-74):
+ *   grid arrays are Fortran (nz,ny,nx): flat index k + nz*(j + ny*i);
+ *   points/params are (3,ncells); locations (4,nsrc); rev (3,nrev);
+ *   sites_id holds 1-based site numbers.
+ * Host pointers unless the name ends in _dev (device pointers, asynchronous on
+ * the context's stream; call mct_sync before reading results).
+ *
+ * Status codes: 0 ok; >0 soft reject (the caller sets like = huge and bumps
+ * RTI%num_bad_model / num_bad_ray as src/likelihood_surf.F90:161-164,212-224,
+ * 341-345 do); <0 fatal (the caller raises exception_raiseError,
+ * src/m_exception.f90:223-228).  No C++ exception crosses this boundary.
+ */
+#ifndef MCTOMO_HIP_H
+#define MCTOMO_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MCT_OK = 0,
+    MCT_REJECT_MODEL = 1,     /* check_model / dispersion failure -> num_bad_model */
+    MCT_REJECT_RAY = 2,       /* crazy ray -> num_bad_ray */
+    MCT_ERR_ARG = -1,         /* invalid argument or size */
+    MCT_ERR_HIP = -2,         /* HIP runtime error (no device, launch failure) */
+    MCT_ERR_NOMEM = -3,       /* device allocation failed */
+    MCT_ERR_SIGMA = -4,       /* noise level below EPS ('The noise level is 0!') */
+    MCT_ERR_NOCONV = -5,      /* eikonal solver hit its iteration cap */
+    MCT_ERR_OUTSIDE = -6      /* source or receiver outside the grid */
+};
+
+/* T_GRID, src/settings.f90:20-28.  zmin/dz are the SCALED values held by the
+ * reference after grid_setup (src/settings.f90:100-123). */
+typedef struct {
+    int nx, ny, nz;
+    double xmin, ymin, zmin;
+    double dx, dy, dz;
+    double scaling;
+} mct_grid;
+
+/* ---- (a) Voronoi -> grid ------------------------------------------------- */
+/* Replaces kdtree_to_grid (src/mcmc_loc2.f90:2002-2082): for every grid node
+ * inside box (NULL = whole grid; {x0,y0,z0,x1,y1,z1}) writes the 1-based index
+ * of the nearest site with kdtree2's tie-breaking (src/kdtree2.f90:1369-1599)
+ * and copies params(:,id) into vp, vs, rho.  With pm != NULL ({vp,vs,rho})
+ * only nodes whose old (vp,vs) equal pm within 1e-8 are updated (:2055-2062). */
+int mct_voronoi_to_grid(const mct_grid *grid, const double *points, const double *params,
+                        int ncells, const double *box, const double *pm,
+                        int32_t *sites_id, double *vp, double *vs, double *rho);
+
+/* ---- (c) 3-D eikonal: link-level drop-in ---------------------------------- */
+/* Same symbol, signature and semantics as the reference C ABI
+ * (fastmarching/fastMarching.cpp:26-30; Fortran interface
+ * src/fastMarching_wrapper.f90:18-29).  vel/tfield are Fortran (nz,ny,nx);
+ * tfield may be NULL.  Returns 0, or a negative MCT_ERR_* code. */
+int fastMarching3D(int nx, int ny, int nz, double dx, double dy, double dz,
+                   double xmin, double ymin, double zmin, double *vel,
+                   const double *src, const double *rev, int nrev, int order,
+                   double *ttime, double *tfield);
+
+/* ---- body-wave likelihood context ---------------------------------------- */
+typedef struct mct_ctx mct_ctx;
+
+/* The immutable part of T_DATA (src/likelihood_settings.f90:19-29) for one
+ * body-wave data set and the T_LIKE_SET knobs that the body path reads
+ * (src/likelihood_settings.f90:47-79). */
+typedef struct {
+    int datatype;             /* 0: P only (np=1), 1: P & S (np=2) */
+    int order;                /* 0 first order, 1 mixed order (LIKE_SET%ORDER) */
+    int sigdep;               /* MCMC_SET%SIGDEP: sigma = b0*t + b1 when != 0 */
+    int nsrc, nrev;
+    const double *rev;        /* (3,nrev) receiver coordinates */
+    const double *ttime;      /* (nsrc*nrev,3,np) observed time, sigma, unused */
+    const int32_t *raystat;   /* (nsrc*nrev,2,np) 1 = datum present */
+    int nrays;                /* dat%nrays */
+    int max_cells;            /* largest Voronoi site count a call may pass */
+    int max_chains;           /* largest batch for the _batch_dev entry */
+} mct_body_setup;
+
+/* like_setup for body data (src/likelihood.f90:31-55): uploads the data to
+ * HBM and returns an opaque context.  The context keeps no per-chain state
+ * between calls, so the sampler's like_copy / restore on reject
+ * (src/mcmc_loc2.f90:201,554-567) stays correct. */
+int mct_body_create(const mct_grid *grid, const mct_body_setup *setup, mct_ctx **out);
+int mct_destroy(mct_ctx *ctx);
+
+/* likelihood() for body waves (src/likelihood.f90:57-93 ->
+ * src/likelihood_body.F90:151-337) on one chain's Voronoi model:
+ * nearest-site rasterisation, one 3-D eikonal per source (per wave type),
+ * receiver picks, Gaussian misfit.  locations (4,nsrc) (x,y,z,t0);
+ * bnoise0/bnoise1 (np).  out = {like, misfit, unweighted_misfit}.
+ * phase_time (nrev,nsrc,np), srdist and sigma (nsrc*nrev,np) may be NULL;
+ * when given they receive like%phaseTime / like%srdist / like%sigma. */
+int mct_body_likelihood(mct_ctx *ctx, const double *points, const double *params, int ncells,
+                        const double *locations, const double *bnoise0, const double *bnoise1,
+                        double *out, double *phase_time, double *srdist, double *sigma);
+
+/* Batched form for many chains whose models are already in HBM (the serving
+ * and throughput path).  points_dev/params_dev: (nchain, max_cells, 3);
+ * ncells_dev (nchain) int32; locations_dev (nchain, nsrc, 4);
+ * noise_dev (nchain, 2, np) = (bnoise0, bnoise1); out_dev (nchain, 3).
+ * Asynchronous on the context stream; status_dev (nchain) int32 receives the
+ * per-chain status. */
+int mct_body_likelihood_batch_dev(mct_ctx *ctx, int nchain, const double *points_dev,
+                                  const double *params_dev, const int32_t *ncells_dev,
+                                  const double *locations_dev, const double *noise_dev,
+                                  double *out_dev, int32_t *status_dev);
+
+/* Wait for the context's stream. */
+int mct_sync(mct_ctx *ctx);
+
+/* Per-kernel timing of the last synchronised call, in milliseconds, measured
+ * with HIP events on the context's stream: [0] nearest-site, [1] eikonal
+ * (all rounds), [2] picks+misfit, [3] number of eikonal rounds. */
+int mct_last_timings(mct_ctx *ctx, double *ms4);
+
+/* Library version string and the device name the library runs on. */
+const char *mct_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,228 @@
+"""ctypes bindings of libmctomo_hip.so and the Python mirror of the reference's
+likelihood interface for the hot path (see the package docstring)."""
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+OK = 0
+REJECT_MODEL = 1
+REJECT_RAY = 2
+_ERRORS = {
+    -1: "invalid argument",
+    -2: "HIP runtime error",
+    -3: "device allocation failed",
+    -4: "The noise level is 0!",
+    -5: "eikonal solver did not converge",
+    -6: "source or receiver outside the grid",
+}
+
+
+class MCTError(RuntimeError):
+    """Fatal status from the library (the reference raises exception_raiseError)."""
+
+    def __init__(self, code, where=""):
+        self.code = code
+        super().__init__("%s: %s (status %d)" % (where, _ERRORS.get(code, "error"), code))
+
+
+def lib_path():
+    return os.path.join(HERE, "libmctomo_hip.so")
+
+
+class _Grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("xmin", ctypes.c_double), ("ymin", ctypes.c_double), ("zmin", ctypes.c_double),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+                ("scaling", ctypes.c_double)]
+
+
+class _BodySetup(ctypes.Structure):
+    _fields_ = [("datatype", ctypes.c_int), ("order", ctypes.c_int), ("sigdep", ctypes.c_int),
+                ("nsrc", ctypes.c_int), ("nrev", ctypes.c_int),
+                ("rev", ctypes.c_void_p), ("ttime", ctypes.c_void_p), ("raystat", ctypes.c_void_p),
+                ("nrays", ctypes.c_int), ("max_cells", ctypes.c_int), ("max_chains", ctypes.c_int)]
+
+
+def lib():
+    """Load libmctomo_hip.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        p = lib_path()
+        if not os.path.exists(p):
+            raise RuntimeError("libmctomo_hip.so is not built: run `python -m mctomo_amd._build`")
+        L = ctypes.CDLL(p)
+        L.mct_version.restype = ctypes.c_char_p
+        for name in ("fastMarching3D", "mct_voronoi_to_grid", "mct_body_create", "mct_destroy",
+                     "mct_body_likelihood", "mct_body_likelihood_batch_dev", "mct_sync",
+                     "mct_last_timings"):
+            getattr(L, name).restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def version():
+    return lib().mct_version().decode()
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _check(rc, where):
+    if rc < 0:
+        raise MCTError(rc, where)
+    return rc
+
+
+@dataclass
+class Grid:
+    """T_GRID (src/settings.f90:20-28); zmin and dz are the scaled values."""
+    nx: int
+    ny: int
+    nz: int
+    xmin: float
+    ymin: float
+    zmin: float
+    dx: float
+    dy: float
+    dz: float
+    scaling: float = 1.0
+
+    def c(self):
+        return _Grid(self.nx, self.ny, self.nz, self.xmin, self.ymin, self.zmin,
+                     self.dx, self.dy, self.dz, self.scaling)
+
+    @classmethod
+    def from_bounds(cls, nx, ny, nz, xmin, xmax, ymin, ymax, zmin, zmax, scaling=1.0):
+        """grid_setup (src/settings.f90:100-123): scale z, then the spacings."""
+        zmin = zmin * scaling
+        zmax = zmax * scaling
+        return cls(nx, ny, nz, xmin, ymin, zmin, (xmax - xmin) / (nx - 1),
+                   (ymax - ymin) / (ny - 1), (zmax - zmin) / (nz - 1), scaling)
+
+
+def fast_marching_3d(vel, dx, dy, dz, xmin, ymin, zmin, src, rev, order, field=False):
+    """fastMarching3D: vel (nx,ny,nz) C array (= Fortran (nz,ny,nx)), src (3,),
+    rev (nrev,3).  Returns traveltimes (nrev,) [and the (nx,ny,nz) field]."""
+    vel = _f64(vel)
+    nx, ny, nz = vel.shape
+    rev = _f64(rev).reshape(-1, 3)
+    tt = np.zeros(rev.shape[0])
+    tf = np.zeros_like(vel) if field else None
+    rc = lib().fastMarching3D(nx, ny, nz, ctypes.c_double(dx), ctypes.c_double(dy),
+                              ctypes.c_double(dz), ctypes.c_double(xmin), ctypes.c_double(ymin),
+                              ctypes.c_double(zmin), _p(vel), _p(_f64(src)), _p(rev),
+                              rev.shape[0], int(order), _p(tt), _p(tf))
+    _check(rc, "fastMarching3D")
+    return (tt, tf) if field else tt
+
+
+def kdtree_to_grid(grid, points, params, box=None, pm=None, vp=None, vs=None, rho=None):
+    """kdtree_to_grid: nearest Voronoi site of each node (1-based ids) and the
+    site's (vp, vs, rho).  Arrays are (nx,ny,nz) C (= Fortran (nz,ny,nx))."""
+    g = grid.c()
+    pts = _f64(points)
+    prm = _f64(params)
+    shape = (grid.nx, grid.ny, grid.nz)
+    sid = np.zeros(shape, np.int32)
+    vp = np.ones(shape) if vp is None else _f64(vp).copy()
+    vs = np.ones(shape) if vs is None else _f64(vs).copy()
+    rho = np.ones(shape) if rho is None else _f64(rho).copy()
+    rc = lib().mct_voronoi_to_grid(ctypes.byref(g), _p(pts), _p(prm), pts.shape[0],
+                                   _p(None if box is None else _f64(box)),
+                                   _p(None if pm is None else _f64(pm)),
+                                   _p(sid), _p(vp), _p(vs), _p(rho))
+    _check(rc, "mct_voronoi_to_grid")
+    return sid, vp, vs, rho
+
+
+class BodyLikelihood:
+    """Body-wave likelihood context: like_setup + likelihood for datatype 0/1.
+
+    rev (nrev,3); ttime (np,3,nsrc*nrev) = (time, sigma, -) per pair with the
+    receiver index fastest (nrr = k + (j-1)*nrev); raystat (np,2,nsrc*nrev).
+    """
+
+    def __init__(self, grid, rev, ttime, raystat, nsrc, datatype=0, order=0, sigdep=0,
+                 nrays=None, max_cells=1024, max_chains=1):
+        self.grid = grid
+        self.np = 1 if datatype == 0 else 2
+        self.nsrc = int(nsrc)
+        self.rev = _f64(rev).reshape(-1, 3)
+        self.nrev = self.rev.shape[0]
+        self.ttime = _f64(ttime).reshape(self.np, 3, self.nsrc * self.nrev)
+        self.raystat = np.ascontiguousarray(raystat, dtype=np.int32).reshape(self.np, 2, -1)
+        if nrays is None:
+            nrays = int((self.raystat[:, 0, :] == 1).sum())
+        self.nrays = nrays
+        self.sigdep = sigdep
+        self.max_cells = max_cells
+        self.max_chains = max_chains
+        st = _BodySetup(datatype, order, sigdep, self.nsrc, self.nrev,
+                        self.rev.ctypes.data, self.ttime.ctypes.data, self.raystat.ctypes.data,
+                        nrays, max_cells, max_chains)
+        self._g = grid.c()
+        h = ctypes.c_void_p()
+        _check(lib().mct_body_create(ctypes.byref(self._g), ctypes.byref(st), ctypes.byref(h)),
+               "mct_body_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mct_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def likelihood(self, points, params, locations, bnoise0=None, bnoise1=None, arrays=False):
+        """One chain.  Returns (status, (like, misfit, unweighted)[, phase_time
+        (np,nsrc,nrev), srdist (np,npair), sigma (np,npair)])."""
+        pts = _f64(points)
+        prm = _f64(params)
+        loc = _f64(locations)
+        b0 = None if bnoise0 is None else _f64(np.atleast_1d(bnoise0))
+        b1 = None if bnoise1 is None else _f64(np.atleast_1d(bnoise1))
+        out = np.zeros(3)
+        pt = sd = sg = None
+        if arrays:
+            npair = self.nsrc * self.nrev
+            pt = np.zeros((self.np, self.nsrc, self.nrev))
+            sd = np.zeros((self.np, npair))
+            sg = np.zeros((self.np, npair))
+        rc = lib().mct_body_likelihood(self._h, _p(pts), _p(prm), pts.shape[0], _p(loc),
+                                       _p(b0), _p(b1), _p(out), _p(pt), _p(sd), _p(sg))
+        _check(rc, "mct_body_likelihood")
+        if arrays:
+            return rc, out, pt, sd, sg
+        return rc, out
+
+    def likelihood_batch_dev(self, nchain, points_ptr, params_ptr, ncells_ptr, locations_ptr,
+                             noise_ptr, out_ptr, status_ptr):
+        """Batched call on device pointers (e.g. torch tensors' data_ptr())."""
+        rc = lib().mct_body_likelihood_batch_dev(
+            self._h, int(nchain), ctypes.c_void_p(points_ptr), ctypes.c_void_p(params_ptr),
+            ctypes.c_void_p(ncells_ptr), ctypes.c_void_p(locations_ptr),
+            ctypes.c_void_p(noise_ptr) if noise_ptr else None, ctypes.c_void_p(out_ptr),
+            ctypes.c_void_p(status_ptr) if status_ptr else None)
+        return _check(rc, "mct_body_likelihood_batch_dev")
+
+    def sync(self):
+        _check(lib().mct_sync(self._h), "mct_sync")
+
+    def timings(self):
+        t = np.zeros(4)
+        lib().mct_last_timings(self._h, _p(t))
+        return {"nearest_ms": t[0], "eikonal_ms": t[1], "misfit_ms": t[2], "rounds": int(t[3])}

@@ -1,0 +1,344 @@
+// capi.hip -- the C ABI of libmctomo_hip.so (declared in include/mctomo_hip.h).
+//
+// Host orchestration of the body-wave likelihood (src/likelihood.f90:57-93 ->
+// src/likelihood_body.F90:151-337): nearest-site rasterisation, 3-D eikonal
+// rounds, picks and misfit, all on one HIP stream per context.  Device memory
+// is owned by the context; host arrays stay owned by the caller (Fortran) and
+// are only read or written during a call, as with c_loc in
+// src/fastMarching_wrapper.f90:51-74.
+#include "mct_internal.h"
+#include <stdlib.h>
+#include <string.h>
+#include <mutex>
+#include <new>
+
+#define MCT_VERSION "mctomo-hip 0.1 (gfx950)"
+
+namespace {
+
+struct dbuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+int dalloc(dbuf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    if (b.n >= bytes) return MCT_OK;
+    if (b.p) hipFree(b.p);
+    b.p = nullptr; b.n = 0;
+    if (hipMalloc(&b.p, bytes) != hipSuccess) { b.p = nullptr; return MCT_ERR_NOMEM; }
+    b.n = bytes;
+    return MCT_OK;
+}
+
+void dfree(dbuf &b)
+{
+    if (b.p) hipFree(b.p);
+    b.p = nullptr; b.n = 0;
+}
+
+template <class T> T *P(dbuf &b) { return (T *)b.p; }
+
+// Eikonal workspace for a set of problems on one geometry.
+struct eik_ws {
+    dbuf T, list0, list1, flags0, flags1, counts;
+    uint32_t *h_count = nullptr;     // pinned
+    int alloc(const mct_geom &G, long nprob)
+    {
+        long nt = nprob * G.ntiles;
+        int rc;
+        if ((rc = dalloc(T, sizeof(float) * nt * MCT_TILE_N))) return rc;
+        if ((rc = dalloc(list0, sizeof(uint32_t) * nt))) return rc;
+        if ((rc = dalloc(list1, sizeof(uint32_t) * nt))) return rc;
+        if ((rc = dalloc(flags0, (size_t)((nt + 3) & ~3L)))) return rc;
+        if ((rc = dalloc(flags1, (size_t)((nt + 3) & ~3L)))) return rc;
+        if ((rc = dalloc(counts, 4 * sizeof(uint32_t)))) return rc;
+        if (!h_count && hipHostMalloc((void **)&h_count, 4 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+            return MCT_ERR_NOMEM;
+        return MCT_OK;
+    }
+    void release()
+    {
+        dfree(T); dfree(list0); dfree(list1); dfree(flags0); dfree(flags1); dfree(counts);
+        if (h_count) hipHostFree(h_count);
+        h_count = nullptr;
+    }
+};
+
+int round_grid_blocks()
+{
+    static int blocks = 0;
+    if (!blocks) {
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess) {
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+        }
+        blocks = cus * 32;          // 32 single-wave tile workers per CU
+    }
+    return blocks;
+}
+
+// Run eikonal rounds until no tile is active.  Returns rounds used or <0.
+int eik_solve(const mct_eik_problem_set &Pset, eik_ws &ws, float hx, float hy, float hz,
+              hipStream_t s, int *rounds_out)
+{
+    const mct_geom &G = Pset.G;
+    const int max_rounds = 8 * (G.ntx + G.nty + G.ntz) + 256;
+    const int K = 8;
+    int blocks = round_grid_blocks();
+    int r = 0;
+    for (;;) {
+        for (int k = 0; k < K; k++, r++) {
+            int rc = mct_launch_eik_round(Pset, r, hx, hy, hz, blocks, s);
+            if (rc) return rc;
+        }
+        // counts[r % 3] holds the size of the list the next round would read
+        MCT_CHECK(hipMemcpyAsync(ws.h_count, Pset.counts + (r % 3), sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, s));
+        MCT_CHECK(hipStreamSynchronize(s));
+        if (*ws.h_count == 0) break;
+        if (r >= max_rounds) return MCT_ERR_NOCONV;
+    }
+    if (rounds_out) *rounds_out = r;
+    return MCT_OK;
+}
+
+} // namespace
+
+struct mct_ctx {
+    mct_geom G;
+    mct_body_setup S;
+    int nwave, nsrc_eff, npick, reciprocal;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double ms[4] = {0, 0, 0, 0};
+    // immutable data
+    dbuf rev, ttime, raystat;
+    // per-chain inputs (host entry point stages here)
+    dbuf points, params, locations, noise, out, status;
+    // stage buffers
+    dbuf vel, flag_count, flag_nodes, kd_nodes, kd_ind, kd_rdata, picks;
+    dbuf phase_time, srdist, sigma;
+    eik_ws eik;
+    double *h_out = nullptr;
+    int32_t *h_status = nullptr;
+};
+
+static int body_pipeline(mct_ctx *C, int nchain, const double *d_points, const double *d_params,
+                         const int32_t *d_ncells, int ncells_const, const double *d_locations,
+                         const double *d_noise, double *d_out, int32_t *d_status, bool want_arrays)
+{
+    const mct_geom &G = C->G;
+    hipStream_t s = C->stream;
+    int rc;
+    MCT_CHECK(hipEventRecord(C->ev[0], s));
+    // (a) nearest site -> FP32 velocity tiles (vp, and vs for P&S)
+    mct_nearest_args na;
+    memset(&na, 0, sizeof na);
+    na.G = G; na.nchain = nchain; na.max_cells = C->S.max_cells;
+    na.points = d_points; na.params = d_params; na.ncells = d_ncells; na.ncells_const = ncells_const;
+    na.ix0 = 1; na.ix1 = G.nx; na.iy0 = 1; na.iy1 = G.ny; na.iz0 = 1; na.iz1 = G.nz;
+    na.vel_tiles = P<float>(C->vel); na.nwave = C->nwave;
+    na.flag_count = P<int32_t>(C->flag_count); na.flag_nodes = P<int32_t>(C->flag_nodes);
+    na.kd_nodes = P<mct_kd_node>(C->kd_nodes); na.kd_ind = P<int32_t>(C->kd_ind);
+    na.kd_rdata = P<double>(C->kd_rdata);
+    if ((rc = mct_launch_nearest(na, s))) return rc;
+    MCT_CHECK(hipEventRecord(C->ev[1], s));
+
+    // (c) eikonal: one problem per (chain, wave, traced point)
+    const double lz = G.dz / G.scaling, zmin = G.zmin / G.scaling;   // likelihood_body.F90:209-210
+    mct_eik_problem_set E;
+    memset(&E, 0, sizeof E);
+    E.G = G; E.nsrc_eff = C->nsrc_eff; E.nwave = C->nwave; E.order = C->S.order;
+    E.nprob = nchain * C->nwave * C->nsrc_eff;
+    E.T = P<float>(C->eik.T); E.vel = P<float>(C->vel);
+    E.list[0] = P<uint32_t>(C->eik.list0); E.list[1] = P<uint32_t>(C->eik.list1);
+    E.flags[0] = P<uint8_t>(C->eik.flags0); E.flags[1] = P<uint8_t>(C->eik.flags1);
+    E.counts = P<uint32_t>(C->eik.counts);
+    const double *src_pts; int sstride; long schain;
+    const double *pick_pts; int pstride; int pchain;
+    if (!C->reciprocal) {
+        src_pts = d_locations; sstride = 4; schain = (long)C->S.nsrc * 4;
+        pick_pts = P<double>(C->rev); pstride = 3; pchain = 0;
+    } else {   // nsrc > nrev: trace from receivers (likelihood_body.F90:226-249)
+        src_pts = P<double>(C->rev); sstride = 3; schain = 0;
+        pick_pts = d_locations; pstride = 4; pchain = C->S.nsrc * 4;
+    }
+    if ((rc = mct_launch_eik_init(E, src_pts, sstride, schain, G.xmin, G.ymin, zmin, G.dx, G.dy, lz, s)))
+        return rc;
+    int rounds = 0;
+    if ((rc = eik_solve(E, C->eik, (float)G.dx, (float)G.dy, (float)lz, s, &rounds))) return rc;
+    MCT_CHECK(hipEventRecord(C->ev[2], s));
+
+    // (d) picks + misfit
+    mct_pick_args pa;
+    pa.G = G; pa.T = E.T; pa.nchain = nchain; pa.nwave = C->nwave; pa.nsrc_eff = C->nsrc_eff;
+    pa.npick = C->npick; pa.pick_pts = pick_pts; pa.pick_stride_chain = pchain;
+    pa.pick_point_stride = pstride; pa.xmin = G.xmin; pa.ymin = G.ymin; pa.zmin = zmin;
+    pa.lx = G.dx; pa.ly = G.dy; pa.lz = lz; pa.out = P<double>(C->picks);
+    if ((rc = mct_launch_picks(pa, s))) return rc;
+    mct_misfit_args ma;
+    ma.nchain = nchain; ma.nsrc = C->S.nsrc; ma.nrev = C->S.nrev; ma.np = C->nwave;
+    ma.reciprocal = C->reciprocal; ma.picks = P<double>(C->picks);
+    ma.ttime = P<double>(C->ttime); ma.raystat = P<int32_t>(C->raystat);
+    ma.locations = d_locations; ma.noise = d_noise; ma.sigdep = C->S.sigdep; ma.nrays = C->S.nrays;
+    ma.out = d_out;
+    ma.phase_time = want_arrays ? P<double>(C->phase_time) : nullptr;
+    ma.srdist = want_arrays ? P<double>(C->srdist) : nullptr;
+    ma.sigma = want_arrays ? P<double>(C->sigma) : nullptr;
+    ma.status = d_status;
+    if ((rc = mct_launch_misfit(ma, s))) return rc;
+    MCT_CHECK(hipEventRecord(C->ev[3], s));
+    C->ms[3] = rounds;
+    return MCT_OK;
+}
+
+extern "C" {
+
+const char *mct_version(void) { return MCT_VERSION; }
+
+int mct_body_create(const mct_grid *grid, const mct_body_setup *setup, mct_ctx **out)
+{
+    if (!grid || !setup || !out) return MCT_ERR_ARG;
+    if (grid->nx < 2 || grid->ny < 2 || grid->nz < 2 || setup->nsrc < 1 || setup->nrev < 1 ||
+        setup->max_cells < 1 || setup->max_chains < 1 || !setup->rev || !setup->ttime || !setup->raystat)
+        return MCT_ERR_ARG;
+    if (setup->datatype != 0 && setup->datatype != 1) return MCT_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MCT_ERR_HIP;
+    mct_ctx *C = new (std::nothrow) mct_ctx();
+    if (!C) return MCT_ERR_NOMEM;
+    C->G = mct_make_geom(grid);
+    C->S = *setup;
+    C->S.rev = nullptr; C->S.ttime = nullptr; C->S.raystat = nullptr;
+    C->nwave = setup->datatype == 0 ? 1 : 2;
+    C->reciprocal = setup->nsrc > setup->nrev;
+    C->nsrc_eff = C->reciprocal ? setup->nrev : setup->nsrc;
+    C->npick = C->reciprocal ? setup->nsrc : setup->nrev;
+    int rc = MCT_OK;
+    const long npair = (long)setup->nsrc * setup->nrev;
+    const int np = C->nwave, M = setup->max_chains, S = setup->max_cells;
+    const long gsz = (long)C->G.ntiles * MCT_TILE_N;
+    if (hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking) != hipSuccess) { delete C; return MCT_ERR_HIP; }
+    for (int i = 0; i < 4; i++)
+        if (hipEventCreate(&C->ev[i]) != hipSuccess) { mct_destroy(C); return MCT_ERR_HIP; }
+#define ALLOC(buf, bytes) if (!rc) rc = dalloc(C->buf, (size_t)(bytes))
+    ALLOC(rev, sizeof(double) * 3 * setup->nrev);
+    ALLOC(ttime, sizeof(double) * 3 * npair * np);
+    ALLOC(raystat, sizeof(int32_t) * 2 * npair * np);
+    ALLOC(points, sizeof(double) * 3 * S * M);
+    ALLOC(params, sizeof(double) * 3 * S * M);
+    ALLOC(locations, sizeof(double) * 4 * setup->nsrc * M);
+    ALLOC(noise, sizeof(double) * 2 * np * M);
+    ALLOC(out, sizeof(double) * 3 * M);
+    ALLOC(status, sizeof(int32_t) * M);
+    ALLOC(vel, sizeof(float) * gsz * np * M);
+    ALLOC(flag_count, sizeof(int32_t) * M);
+    ALLOC(flag_nodes, sizeof(int32_t) * MCT_MAX_FLAGGED * M);
+    ALLOC(kd_nodes, sizeof(mct_kd_node) * (2 * S + 2) * M);
+    ALLOC(kd_ind, sizeof(int32_t) * (S + 1) * M);
+    ALLOC(kd_rdata, sizeof(double) * 3 * S * M);
+    ALLOC(picks, sizeof(double) * npair * np * M);
+    ALLOC(phase_time, sizeof(double) * npair * np);
+    ALLOC(srdist, sizeof(double) * npair * np);
+    ALLOC(sigma, sizeof(double) * npair * np);
+#undef ALLOC
+    if (!rc) rc = C->eik.alloc(C->G, (long)M * np * C->nsrc_eff);
+    if (!rc && hipHostMalloc((void **)&C->h_out, sizeof(double) * 3, hipHostMallocDefault) != hipSuccess) rc = MCT_ERR_NOMEM;
+    if (!rc && hipHostMalloc((void **)&C->h_status, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) rc = MCT_ERR_NOMEM;
+    if (!rc && hipMemcpy(C->rev.p, setup->rev, sizeof(double) * 3 * setup->nrev, hipMemcpyHostToDevice) != hipSuccess) rc = MCT_ERR_HIP;
+    if (!rc && hipMemcpy(C->ttime.p, setup->ttime, sizeof(double) * 3 * npair * np, hipMemcpyHostToDevice) != hipSuccess) rc = MCT_ERR_HIP;
+    if (!rc && hipMemcpy(C->raystat.p, setup->raystat, sizeof(int32_t) * 2 * npair * np, hipMemcpyHostToDevice) != hipSuccess) rc = MCT_ERR_HIP;
+    if (rc) { mct_destroy(C); return rc; }
+    *out = C;
+    return MCT_OK;
+}
+
+int mct_destroy(mct_ctx *C)
+{
+    if (!C) return MCT_OK;
+    if (C->stream) hipStreamSynchronize(C->stream);
+    dbuf *bufs[] = {&C->rev, &C->ttime, &C->raystat, &C->points, &C->params, &C->locations,
+                    &C->noise, &C->out, &C->status, &C->vel, &C->flag_count, &C->flag_nodes,
+                    &C->kd_nodes, &C->kd_ind, &C->kd_rdata, &C->picks, &C->phase_time,
+                    &C->srdist, &C->sigma};
+    for (dbuf *b : bufs) dfree(*b);
+    C->eik.release();
+    if (C->h_out) hipHostFree(C->h_out);
+    if (C->h_status) hipHostFree(C->h_status);
+    for (int i = 0; i < 4; i++) if (C->ev[i]) hipEventDestroy(C->ev[i]);
+    if (C->stream) hipStreamDestroy(C->stream);
+    delete C;
+    return MCT_OK;
+}
+
+int mct_body_likelihood(mct_ctx *C, const double *points, const double *params, int ncells,
+                        const double *locations, const double *bnoise0, const double *bnoise1,
+                        double *out, double *phase_time, double *srdist, double *sigma)
+{
+    if (!C || !points || !params || !locations || !out) return MCT_ERR_ARG;
+    if (ncells < 1 || ncells > C->S.max_cells) return MCT_ERR_ARG;
+    if (C->S.sigdep && (!bnoise0 || !bnoise1)) return MCT_ERR_ARG;
+    hipStream_t s = C->stream;
+    const int np = C->nwave;
+    const long npair = (long)C->S.nsrc * C->S.nrev;
+    MCT_CHECK(hipMemcpyAsync(C->points.p, points, sizeof(double) * 3 * ncells, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemcpyAsync(C->params.p, params, sizeof(double) * 3 * ncells, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemcpyAsync(C->locations.p, locations, sizeof(double) * 4 * C->S.nsrc, hipMemcpyHostToDevice, s));
+    double noise[4] = {0, 0, 0, 0};
+    if (bnoise0) for (int i = 0; i < np; i++) { noise[i] = bnoise0[i]; noise[np + i] = bnoise1[i]; }
+    MCT_CHECK(hipMemcpyAsync(C->noise.p, noise, sizeof(double) * 2 * np, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemsetAsync(C->status.p, 0, sizeof(int32_t), s));
+    bool arrays = phase_time || srdist || sigma;
+    int rc = body_pipeline(C, 1, P<double>(C->points), P<double>(C->params), nullptr, ncells,
+                           P<double>(C->locations), P<double>(C->noise), P<double>(C->out),
+                           P<int32_t>(C->status), arrays);
+    if (rc) return rc;
+    MCT_CHECK(hipMemcpyAsync(C->h_out, C->out.p, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+    MCT_CHECK(hipMemcpyAsync(C->h_status, C->status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (phase_time) MCT_CHECK(hipMemcpyAsync(phase_time, C->phase_time.p, sizeof(double) * npair * np, hipMemcpyDeviceToHost, s));
+    if (srdist) MCT_CHECK(hipMemcpyAsync(srdist, C->srdist.p, sizeof(double) * npair * np, hipMemcpyDeviceToHost, s));
+    if (sigma) MCT_CHECK(hipMemcpyAsync(sigma, C->sigma.p, sizeof(double) * npair * np, hipMemcpyDeviceToHost, s));
+    MCT_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < 3; i++) out[i] = C->h_out[i];
+    float f;
+    for (int i = 0; i < 3; i++) {
+        MCT_CHECK(hipEventElapsedTime(&f, C->ev[i], C->ev[i + 1]));
+        C->ms[i] = f;
+    }
+    return *C->h_status;
+}
+
+int mct_body_likelihood_batch_dev(mct_ctx *C, int nchain, const double *points_dev,
+                                  const double *params_dev, const int32_t *ncells_dev,
+                                  const double *locations_dev, const double *noise_dev,
+                                  double *out_dev, int32_t *status_dev)
+{
+    if (!C || nchain < 1 || nchain > C->S.max_chains || !points_dev || !params_dev || !ncells_dev ||
+        !locations_dev || !out_dev)
+        return MCT_ERR_ARG;
+    if (status_dev) MCT_CHECK(hipMemsetAsync(status_dev, 0, sizeof(int32_t) * nchain, C->stream));
+    return body_pipeline(C, nchain, points_dev, params_dev, ncells_dev, 0, locations_dev,
+                         noise_dev, out_dev, status_dev, false);
+}
+
+int mct_sync(mct_ctx *C)
+{
+    if (!C) return MCT_ERR_ARG;
+    MCT_CHECK(hipStreamSynchronize(C->stream));
+    float f;
+    for (int i = 0; i < 3; i++)
+        if (hipEventElapsedTime(&f, C->ev[i], C->ev[i + 1]) == hipSuccess) C->ms[i] = f;
+    return MCT_OK;
+}
+
+int mct_last_timings(mct_ctx *C, double *ms4)
+{
+    if (!C || !ms4) return MCT_ERR_ARG;
+    for (int i = 0; i < 4; i++) ms4[i] = C->ms[i];
+    return MCT_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,129 @@
+// mct_internal.h -- shared declarations of the HIP implementation of the
+// MCTomo likelihood forward model (gfx950).  Not part of the public C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+
+#include "../../include/mctomo_hip.h"
+
+#define MCT_TILE 4                 // eikonal tiles are 4x4x4 nodes = one wave64
+#define MCT_TILE_N 64
+#define MCT_KD_BUCKET 12           // kdtree2 bucket_size (src/kdtree2.f90:516)
+#define MCT_MAX_FLAGGED 4096       // near-tie nodes replayed per chain before a full replay
+
+#define MCT_CHECK(expr)                                                       \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) return MCT_ERR_HIP;                             \
+    } while (0)
+
+// Geometry of one rasterised grid and its eikonal tiling.
+struct mct_geom {
+    int nx, ny, nz;
+    double xmin, ymin, zmin;       // z scaled (Voronoi distances)
+    double dx, dy, dz;             // dz scaled
+    double scaling;
+    int ntx, nty, ntz, ntiles;     // tiles of 4x4x4
+};
+
+static inline mct_geom mct_make_geom(const mct_grid *g)
+{
+    mct_geom G;
+    G.nx = g->nx; G.ny = g->ny; G.nz = g->nz;
+    G.xmin = g->xmin; G.ymin = g->ymin; G.zmin = g->zmin;
+    G.dx = g->dx; G.dy = g->dy; G.dz = g->dz;
+    G.scaling = g->scaling > 0 ? g->scaling : 1.0;
+    G.ntx = (G.nx + MCT_TILE - 1) / MCT_TILE;
+    G.nty = (G.ny + MCT_TILE - 1) / MCT_TILE;
+    G.ntz = (G.nz + MCT_TILE - 1) / MCT_TILE;
+    G.ntiles = G.ntx * G.nty * G.ntz;
+    return G;
+}
+
+// Per-chain scratch of the exact kd-tree replay (rare near-tie nodes).
+struct mct_kd_node {
+    double cut_val, cut_val_left, cut_val_right;
+    double lo[3], hi[3];
+    int cut_dim, l, u, left, right, pad;
+};
+
+// ---- nearest-site stage ----------------------------------------------------
+struct mct_nearest_args {
+    mct_geom G;
+    int nchain;
+    int max_cells;
+    const double *points;          // (nchain, max_cells, 3)
+    const double *params;          // (nchain, max_cells, 3)
+    const int32_t *ncells;         // (nchain) or NULL -> ncells_const
+    int ncells_const;
+    int ix0, ix1, iy0, iy1, iz0, iz1;   // 1-based inclusive box
+    const double *pm;              // {vp,vs,rho} or NULL
+    // outputs (any may be NULL)
+    float *vel_tiles;              // (nchain, nwave, ntiles*64) eikonal velocity
+    int nwave;                     // 1: vp only, 2: vp and vs
+    int32_t *sites_id;             // (nchain, nz*ny*nx) Fortran order
+    double *vp, *vs, *rho;         // (nchain, nz*ny*nx) Fortran order
+    // near-tie bookkeeping (per chain)
+    int32_t *flag_count;           // (nchain)
+    int32_t *flag_nodes;           // (nchain, MCT_MAX_FLAGGED) Fortran node index
+    mct_kd_node *kd_nodes;         // (nchain, 2*max_cells+2)
+    int32_t *kd_ind;               // (nchain, max_cells+1)
+    double *kd_rdata;              // (nchain, 3*max_cells)
+};
+
+int mct_launch_nearest(const mct_nearest_args &a, hipStream_t s);
+
+// ---- 3-D eikonal stage -------------------------------------------------------
+struct mct_eik_problem_set {
+    mct_geom G;
+    int nprob;                     // problems = chains * waves * sources
+    int nsrc_eff;                  // sources traced per (chain, wave)
+    int nwave;
+    int order;
+    float *T;                      // (nprob, ntiles*64) arrival times
+    const float *vel;              // (nchain*nwave, ntiles*64)
+    uint32_t *list[2];             // active tile lists (prob*ntiles + tile)
+    uint8_t *flags[2];             // membership flags for list[0]/list[1]
+    uint32_t *counts;              // [3] rotating counters
+    int32_t *status;               // (nprob) or NULL
+};
+
+// Initialise arrival times from source points (Solver::setInitialPoints,
+// fastmarching/fmm/solver.hpp:72-92) and seed the first active list.
+// Source s of chain c is at src_pts + c*src_chain_stride + s*src_stride
+// (doubles); xmin/ymin/zmin are subtracted inside, leaf sizes lx, ly, lz.
+int mct_launch_eik_init(const mct_eik_problem_set &P, const double *src_pts, int src_stride,
+                        long src_chain_stride, double xmin, double ymin, double zmin,
+                        double lx, double ly, double lz, hipStream_t s);
+int mct_launch_eik_round(const mct_eik_problem_set &P, int round, float hx, float hy, float hz,
+                         int grid_blocks, hipStream_t s);
+
+// ---- picks + misfit -------------------------------------------------------------
+struct mct_pick_args {
+    mct_geom G;
+    const float *T;                // (nprob, ntiles*64)
+    int nchain, nwave, nsrc_eff, npick;
+    const double *pick_pts;        // points to pick at, (npick,3) shared or per chain
+    int pick_stride_chain;         // 0 if shared across chains, else doubles per chain
+    int pick_point_stride;         // doubles between points (3 or 4)
+    double xmin, ymin, zmin, lx, ly, lz;
+    double *out;                   // (nchain, nwave, nsrc_eff, npick)
+};
+int mct_launch_picks(const mct_pick_args &a, hipStream_t s);
+
+struct mct_misfit_args {
+    int nchain, nsrc, nrev, np;
+    int reciprocal;                // picks are (rev-major) when traced from receivers
+    const double *picks;           // (nchain, np, nsrc_eff, npick)
+    const double *ttime;           // (np, 3, nsrc*nrev)  observed
+    const int32_t *raystat;        // (np, 2, nsrc*nrev)
+    const double *locations;       // (nchain, nsrc, 4)
+    const double *noise;           // (nchain, 2, np) or NULL
+    int sigdep, nrays;
+    double *out;                   // (nchain, 3)
+    double *phase_time;            // (nchain, np, nsrc, nrev) or NULL
+    double *srdist, *sigma;        // (nchain, np, nsrc*nrev) or NULL
+    int32_t *status;               // (nchain) or NULL
+};
+int mct_launch_misfit(const mct_misfit_args &a, hipStream_t s);

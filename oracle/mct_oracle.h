@@ -1,0 +1,81 @@
+/*
+ * mct_oracle.h -- CPU oracle for the MCTomo likelihood forward model.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is a plain-C restatement of the
+ * reference algorithms (each function cites the reference file:line it
+ * follows).  It is the checker: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product (libmctomo_hip.so)
+ * never links, calls or falls back to it.
+ *
+ * Array conventions follow the reference's Fortran storage:
+ *   grid arrays (nz,ny,nx) column-major  -> flat index k + nz*(j + ny*i)
+ *   points / params (3,ncells)           -> p[3*c + d]
+ *   indices written to sites_id are 1-based site numbers, as in Fortran.
+ */
+#ifndef MCT_ORACLE_H
+#define MCT_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- (a) Voronoi -> grid nearest site ---------------------------------- */
+/* kdtree_to_grid (src/mcmc_loc2.f90:2002-2082) over the whole grid or a
+ * box.  box = {x0,y0,z0,x1,y1,z1} (scaled z, like the reference's d3 box);
+ * NULL means the whole grid.  pm = {vp,vs,rho} or NULL.  Returns 0. */
+int orc_kdtree_to_grid(int nx, int ny, int nz,
+                       double xmin, double ymin, double zmin,
+                       double dx, double dy, double dz,
+                       const double *points, const double *params, int ncells,
+                       const double *box, const double *pm,
+                       int32_t *sites_id, double *vp, double *vs, double *rho);
+
+/* One kdtree2 1-NN query (src/kdtree2.f90:1028-1069) on a tree built from
+ * `points`; returns the 1-based index.  Used by tests on single queries. */
+int orc_kdtree_nearest(const double *points, int ncells, const double *q);
+
+/* Brute-force nearest site with the kdtree2 distance summation order; returns
+ * the smallest squared distance and the number of sites that attain it. */
+int orc_brute_nearest(const double *points, int ncells, const double *q,
+                      double *best_sd, int *n_at_best);
+
+/* ---- (c) 3-D fast marching (SFMM) --------------------------------------- */
+/* Same signature and semantics as the reference C ABI fastMarching3D
+ * (fastmarching/fastMarching.cpp:26-129). vel/tfield are Fortran (nz,ny,nx). */
+int orc_fastMarching3D(int nx, int ny, int nz, double dx, double dy, double dz,
+                       double xmin, double ymin, double zmin,
+                       const double *vel, const double *src, const double *rev,
+                       int nrev, int order, double *ttime, double *tfield);
+
+/* Body-wave forward model for all sources (src/likelihood_body.F90:204-224,
+ * nsrc <= nrev branch), OpenMP over sources like the reference.
+ * locations(4,nsrc), rev(3,nrev); phase_time(nrev,nsrc).  nthreads<=0: all. */
+int orc_body_traveltimes(int nx, int ny, int nz, double dx, double dy, double dz,
+                         double xmin, double ymin, double zmin, double scaling,
+                         const double *vel, int nsrc, const double *locations,
+                         int nrev, const double *rev, int order,
+                         double *phase_time, int nthreads);
+
+/* Test hook: replay push / key-change / pop on the SFMM queue (fmm3d.c) */
+int orc_heap_replay(int ncell, double *keys, int nops, const int *ops, const double *vals,
+                    int *popped);
+
+/* ---- (d) misfit ----------------------------------------------------------- */
+/* Body-wave Gaussian misfit (src/likelihood_body.F90:286-333) for np data
+ * sets (np=1 P, np=2 P&S).  phase_time(nrev,nsrc,np); ttime(nsrc*nrev,3,np);
+ * raystat(nsrc*nrev,2,np); locations(4,nsrc); bnoise0/1(np).
+ * srdist and sigma (nsrc*nrev,np) are written like the reference's
+ * like%srdist / like%sigma.  out = {like, misfit, unweighted_misfit}.
+ * Returns -1 if a used sigma is below EPS (the reference raises a fatal error),
+ * else 0. */
+int orc_body_misfit(int nsrc, int nrev, int np, const double *phase_time,
+                    const double *ttime, const int32_t *raystat,
+                    const double *locations, int sigdep,
+                    const double *bnoise0, const double *bnoise1, int nrays,
+                    double *srdist, double *sigma, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,154 @@
+"""ctypes bindings to oracle/liboracle.so (C restatement of the reference) and,
+when built, oracle/_ref/libmctref.so (the reference's own Fortran, compiled in
+place).  Test infrastructure only."""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+_ref = None
+
+_D = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_I = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """Load liboracle.so (build it with `make -C oracle`)."""
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built; run make -C oracle")
+        _lib = ctypes.CDLL(path)
+    return _lib
+
+
+def ref():
+    """Load oracle/_ref/libmctref.so, or return None where it was not built."""
+    global _ref
+    if _ref is None:
+        path = os.path.join(HERE, "_ref", "libmctref.so")
+        if not os.path.exists(path):
+            return None
+        _ref = ctypes.CDLL(path)
+    return _ref
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def c64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---------------------------------------------------------------- nearest site
+def kdtree_to_grid(grid, points, params, box=None, pm=None, vp=None, vs=None, rho=None):
+    """grid: dict nx,ny,nz,xmin,ymin,zmin,dx,dy,dz (z already scaled).
+    points, params: (ncells,3) arrays (Fortran (3,ncells) storage).
+    Returns sites_id, vp, vs, rho as (nx,ny,nz) C arrays == Fortran (nz,ny,nx)."""
+    nx, ny, nz = grid["nx"], grid["ny"], grid["nz"]
+    points = c64(points)
+    params = c64(params)
+    sid = np.zeros((nx, ny, nz), np.int32)
+    vp = np.ones((nx, ny, nz)) if vp is None else c64(vp).copy()
+    vs = np.ones((nx, ny, nz)) if vs is None else c64(vs).copy()
+    rho = np.ones((nx, ny, nz)) if rho is None else c64(rho).copy()
+    L = lib()
+    L.orc_kdtree_to_grid.restype = ctypes.c_int
+    rc = L.orc_kdtree_to_grid(
+        ctypes.c_int(nx), ctypes.c_int(ny), ctypes.c_int(nz),
+        ctypes.c_double(grid["xmin"]), ctypes.c_double(grid["ymin"]), ctypes.c_double(grid["zmin"]),
+        ctypes.c_double(grid["dx"]), ctypes.c_double(grid["dy"]), ctypes.c_double(grid["dz"]),
+        _p(points), _p(params), ctypes.c_int(points.shape[0]),
+        _p(None if box is None else c64(box)), _p(None if pm is None else c64(pm)),
+        _p(sid), _p(vp), _p(vs), _p(rho))
+    assert rc == 0
+    return sid, vp, vs, rho
+
+
+def kdtree_nearest(points, q):
+    points = c64(points)
+    q = c64(q)
+    return lib().orc_kdtree_nearest(_p(points), ctypes.c_int(points.shape[0]), _p(q))
+
+
+def brute_nearest(points, q):
+    points = c64(points)
+    q = c64(q)
+    sd = ctypes.c_double()
+    n = ctypes.c_int()
+    i = lib().orc_brute_nearest(_p(points), ctypes.c_int(points.shape[0]), _p(q),
+                                ctypes.byref(sd), ctypes.byref(n))
+    return i, sd.value, n.value
+
+
+def ref_kdtree_nearest(points, queries):
+    """Reference kdtree2 (compiled Fortran) 1-NN for each query row."""
+    R = ref()
+    if R is None:
+        return None
+    points = c64(points)
+    queries = c64(queries)
+    out = np.zeros(queries.shape[0], np.int32)
+    R.ref_kdtree_nearest(ctypes.c_int(points.shape[0]), _p(points),
+                         ctypes.c_int(queries.shape[0]), _p(queries), _p(out))
+    return out
+
+
+# ---------------------------------------------------------------- 3-D eikonal
+def fast_marching_3d(vel, dx, dy, dz, xmin, ymin, zmin, src, rev, order, field=False):
+    """vel: (nx,ny,nz) C array (== Fortran (nz,ny,nx)). rev: (nrev,3).
+    Returns ttime (nrev,) and, if field, the (nx,ny,nz) traveltime field."""
+    vel = c64(vel)
+    nx, ny, nz = vel.shape
+    rev = c64(rev).reshape(-1, 3)
+    tt = np.zeros(rev.shape[0])
+    tf = np.zeros_like(vel) if field else None
+    L = lib()
+    L.orc_fastMarching3D(ctypes.c_int(nx), ctypes.c_int(ny), ctypes.c_int(nz),
+                         ctypes.c_double(dx), ctypes.c_double(dy), ctypes.c_double(dz),
+                         ctypes.c_double(xmin), ctypes.c_double(ymin), ctypes.c_double(zmin),
+                         _p(vel), _p(c64(src)), _p(rev), ctypes.c_int(rev.shape[0]),
+                         ctypes.c_int(order), _p(tt), _p(tf))
+    return (tt, tf) if field else tt
+
+
+def body_traveltimes(vel, grid, locations, rev, order, nthreads=0):
+    """All sources (locations (nsrc,4)) -> phase_time (nsrc,nrev) [Fortran (nrev,nsrc)]."""
+    vel = c64(vel)
+    nx, ny, nz = vel.shape
+    loc = c64(locations)
+    rev = c64(rev)
+    out = np.zeros((loc.shape[0], rev.shape[0]))
+    lib().orc_body_traveltimes(
+        ctypes.c_int(nx), ctypes.c_int(ny), ctypes.c_int(nz),
+        ctypes.c_double(grid["dx"]), ctypes.c_double(grid["dy"]), ctypes.c_double(grid["dz"]),
+        ctypes.c_double(grid["xmin"]), ctypes.c_double(grid["ymin"]), ctypes.c_double(grid["zmin"]),
+        ctypes.c_double(grid.get("scaling", 1.0)), _p(vel), ctypes.c_int(loc.shape[0]), _p(loc),
+        ctypes.c_int(rev.shape[0]), _p(rev), ctypes.c_int(order), _p(out), ctypes.c_int(nthreads))
+    return out
+
+
+def body_misfit(phase_time, ttime, raystat, locations, sigdep=0, bnoise0=None, bnoise1=None,
+                nrays=None):
+    """phase_time (np,nsrc,nrev); ttime (np,3,npair); raystat (np,2,npair) int32;
+    locations (nsrc,4).  Returns (like, misfit, unweighted), srdist, sigma, rc."""
+    pt = c64(phase_time)
+    npp, nsrc, nrev = pt.shape
+    tt = c64(ttime)
+    rs = np.ascontiguousarray(raystat, dtype=np.int32)
+    loc = c64(locations)
+    b0 = c64(np.zeros(npp) if bnoise0 is None else bnoise0)
+    b1 = c64(np.zeros(npp) if bnoise1 is None else bnoise1)
+    if nrays is None:
+        nrays = int((rs[:, 0, :] == 1).sum())
+    srd = np.zeros((npp, nsrc * nrev))
+    sig = np.zeros((npp, nsrc * nrev))
+    out = np.zeros(3)
+    rc = lib().orc_body_misfit(ctypes.c_int(nsrc), ctypes.c_int(nrev), ctypes.c_int(npp), _p(pt),
+                               _p(tt), _p(rs), _p(loc), ctypes.c_int(sigdep), _p(b0), _p(b1),
+                               ctypes.c_int(nrays), _p(srd), _p(sig), _p(out))
+    return out, srd, sig, rc

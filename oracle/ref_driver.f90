@@ -1,0 +1,34 @@
+! ref_driver.f90 -- C-callable harness around the reference's own Fortran
+! routines, compiled in place from /root/reference into oracle/_ref/.
+! TEST INFRASTRUCTURE ONLY: it produces golden vectors that pin the C oracle.
+! It calls the reference code exactly as the reference's callers do; it
+! re-implements nothing.
+module mct_ref_driver
+    use iso_c_binding
+    implicit none
+contains
+
+    ! kdtree_to_grid's use of kdtree2 (src/mcmc_loc2.f90:2030-2078) for a list
+    ! of query points: q(3,nq) -> idx(nq), 1-based site numbers.
+    subroutine ref_kdtree_nearest(ncells, points, nq, q, idx) bind(C, name="ref_kdtree_nearest")
+        use kdtree2_precision_module, only : kdkind
+        use kdtree2_module, only : kdtree2, kdtree2_result, kdtree2_create, &
+            kdtree2_destroy, kdtree2_n_nearest
+        integer(c_int), value :: ncells, nq
+        real(c_double), intent(in), target :: points(3, ncells)
+        real(c_double), intent(in) :: q(3, nq)
+        integer(c_int), intent(out) :: idx(nq)
+        type(kdtree2), pointer :: tree
+        type(kdtree2_result), dimension(1) :: results
+        real(kdkind), dimension(3) :: qv
+        integer :: i
+        tree => kdtree2_create(points(:, 1:ncells), sort=.false., rearrange=.true.)
+        do i = 1, nq
+            qv = q(:, i)
+            call kdtree2_n_nearest(tp=tree, qv=qv, nn=1, results=results)
+            idx(i) = results(1)%idx
+        enddo
+        call kdtree2_destroy(tree)
+    end subroutine
+
+end module mct_ref_driver
