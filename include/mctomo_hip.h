@@ -112,13 +112,29 @@ int mct_body_likelihood_batch_dev(mct_ctx *ctx, int nchain, const double *points
                                   const double *locations_dev, const double *noise_dev,
                                   double *out_dev, int32_t *status_dev);
 
+/* Device memory for callers that hold no device allocator of their own (a
+ * Fortran host, or Python without a GPU framework): plain hipMalloc / hipFree
+ * and synchronous copies.  No counterpart in the reference, which has no
+ * device memory; they exist so the _dev entry points are callable from the
+ * same C ABI. */
+int mct_dev_alloc(void **ptr, int64_t bytes);
+int mct_dev_free(void *ptr);
+int mct_copy_h2d(void *dst_dev, const void *src_host, int64_t bytes);
+int mct_copy_d2h(void *dst_host, const void *src_dev, int64_t bytes);
+
 /* Wait for the context's stream. */
 int mct_sync(mct_ctx *ctx);
 
-/* Per-kernel timing of the last synchronised call, in milliseconds, measured
- * with HIP events on the context's stream: [0] nearest-site, [1] eikonal
- * (all rounds), [2] picks+misfit, [3] number of eikonal rounds. */
-int mct_last_timings(mct_ctx *ctx, double *ms4);
+/* Stage timing of the last synchronised call, measured with HIP events on
+ * the context's stream: [0] nearest-site ms, [1] eikonal ms (all rounds with
+ * the convergence checks between them), [2] picks+misfit ms, [3] eikonal
+ * rounds; with kernel timing on, [4] summed duration of the eikonal round
+ * kernels alone (ms) and [5] the number of round launches timed. */
+int mct_last_timings(mct_ctx *ctx, double *ms6);
+
+/* Turn per-launch events around every eikonal round kernel on or off (off by
+ * default: they cost a little host time per round). */
+int mct_set_kernel_timing(mct_ctx *ctx, int on);
 
 /* Library version string and the device name the library runs on. */
 const char *mct_version(void);

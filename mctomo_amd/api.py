@@ -59,7 +59,8 @@ def lib():
         L.mct_version.restype = ctypes.c_char_p
         for name in ("fastMarching3D", "mct_voronoi_to_grid", "mct_body_create", "mct_destroy",
                      "mct_body_likelihood", "mct_body_likelihood_batch_dev", "mct_sync",
-                     "mct_last_timings"):
+                     "mct_last_timings", "mct_set_kernel_timing", "mct_dev_alloc", "mct_dev_free", "mct_copy_h2d",
+                     "mct_copy_d2h"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -81,6 +82,50 @@ def _check(rc, where):
     if rc < 0:
         raise MCTError(rc, where)
     return rc
+
+
+class DeviceArray:
+    """A device allocation made through the library (hipMalloc), with the
+    numpy shape/dtype it mirrors.  Used to hand device pointers to the _dev
+    entry points without any GPU framework in the process."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(lib().mct_dev_alloc(ctypes.byref(p), ctypes.c_int64(self.nbytes)), "mct_dev_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        _check(lib().mct_copy_h2d(ctypes.c_void_p(self.ptr), ctypes.c_void_p(a.ctypes.data),
+                                  ctypes.c_int64(self.nbytes)), "mct_copy_h2d")
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        _check(lib().mct_copy_d2h(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(self.ptr),
+                                  ctypes.c_int64(self.nbytes)), "mct_copy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().mct_dev_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 @dataclass
@@ -222,7 +267,11 @@ class BodyLikelihood:
     def sync(self):
         _check(lib().mct_sync(self._h), "mct_sync")
 
+    def set_kernel_timing(self, on):
+        _check(lib().mct_set_kernel_timing(self._h, int(bool(on))), "mct_set_kernel_timing")
+
     def timings(self):
-        t = np.zeros(4)
+        t = np.zeros(6)
         lib().mct_last_timings(self._h, _p(t))
-        return {"nearest_ms": t[0], "eikonal_ms": t[1], "misfit_ms": t[2], "rounds": int(t[3])}
+        return {"nearest_ms": t[0], "eikonal_ms": t[1], "misfit_ms": t[2], "rounds": int(t[3]),
+                "round_kernels_ms": t[4], "round_launches": int(t[5])}

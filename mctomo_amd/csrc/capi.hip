@@ -25,7 +25,7 @@ int dalloc(dbuf &b, size_t bytes)
 {
     if (bytes == 0) bytes = 16;
     if (b.n >= bytes) return MCT_OK;
-    if (b.p) hipFree(b.p);
+    if (b.p) (void)hipFree(b.p);
     b.p = nullptr; b.n = 0;
     if (hipMalloc(&b.p, bytes) != hipSuccess) { b.p = nullptr; return MCT_ERR_NOMEM; }
     b.n = bytes;
@@ -34,7 +34,7 @@ int dalloc(dbuf &b, size_t bytes)
 
 void dfree(dbuf &b)
 {
-    if (b.p) hipFree(b.p);
+    if (b.p) (void)hipFree(b.p);
     b.p = nullptr; b.n = 0;
 }
 
@@ -61,7 +61,7 @@ struct eik_ws {
     void release()
     {
         dfree(T); dfree(list0); dfree(list1); dfree(flags0); dfree(flags1); dfree(counts);
-        if (h_count) hipHostFree(h_count);
+        if (h_count) (void)hipHostFree(h_count);
         h_count = nullptr;
     }
 };
@@ -80,9 +80,40 @@ int round_grid_blocks()
     return blocks;
 }
 
+#define MCT_MAX_TIMED_ROUNDS 4096
+
+// Optional per-launch timing of the round kernel: events around every launch.
+struct round_timer {
+    bool on = false;
+    hipEvent_t a[MCT_MAX_TIMED_ROUNDS], b[MCT_MAX_TIMED_ROUNDS];
+    int created = 0, used = 0;
+    double kernel_ms = 0.0;
+    int ensure()
+    {
+        for (; created < MCT_MAX_TIMED_ROUNDS; created++) {
+            if (hipEventCreate(&a[created]) != hipSuccess) return MCT_ERR_HIP;
+            if (hipEventCreate(&b[created]) != hipSuccess) { (void)hipEventDestroy(a[created]); return MCT_ERR_HIP; }
+        }
+        return MCT_OK;
+    }
+    void release()
+    {
+        for (int i = 0; i < created; i++) { (void)hipEventDestroy(a[i]); (void)hipEventDestroy(b[i]); }
+        created = 0;
+    }
+    void collect()
+    {
+        kernel_ms = 0.0;
+        for (int i = 0; i < used; i++) {
+            float f = 0.f;
+            if (hipEventElapsedTime(&f, a[i], b[i]) == hipSuccess) kernel_ms += f;
+        }
+    }
+};
+
 // Run eikonal rounds until no tile is active.  Returns rounds used or <0.
 int eik_solve(const mct_eik_problem_set &Pset, eik_ws &ws, float hx, float hy, float hz,
-              hipStream_t s, int *rounds_out)
+              hipStream_t s, int *rounds_out, round_timer *tm)
 {
     const mct_geom &G = Pset.G;
     const int max_rounds = 8 * (G.ntx + G.nty + G.ntz) + 256;
@@ -91,8 +122,11 @@ int eik_solve(const mct_eik_problem_set &Pset, eik_ws &ws, float hx, float hy, f
     int r = 0;
     for (;;) {
         for (int k = 0; k < K; k++, r++) {
+            bool timed = tm && tm->on && r < MCT_MAX_TIMED_ROUNDS;
+            if (timed) MCT_CHECK(hipEventRecord(tm->a[r], s));
             int rc = mct_launch_eik_round(Pset, r, hx, hy, hz, blocks, s);
             if (rc) return rc;
+            if (timed) { MCT_CHECK(hipEventRecord(tm->b[r], s)); tm->used = r + 1; }
         }
         // counts[r % 3] holds the size of the list the next round would read
         MCT_CHECK(hipMemcpyAsync(ws.h_count, Pset.counts + (r % 3), sizeof(uint32_t),
@@ -122,8 +156,10 @@ struct mct_ctx {
     dbuf vel, flag_count, flag_nodes, kd_nodes, kd_ind, kd_rdata, picks;
     dbuf phase_time, srdist, sigma;
     eik_ws eik;
+    round_timer *timer = nullptr;
     double *h_out = nullptr;
     int32_t *h_status = nullptr;
+    double *h_in = nullptr;          // pinned staging of one chain's inputs
 };
 
 static int body_pipeline(mct_ctx *C, int nchain, const double *d_points, const double *d_params,
@@ -169,7 +205,8 @@ static int body_pipeline(mct_ctx *C, int nchain, const double *d_points, const d
     if ((rc = mct_launch_eik_init(E, src_pts, sstride, schain, G.xmin, G.ymin, zmin, G.dx, G.dy, lz, s)))
         return rc;
     int rounds = 0;
-    if ((rc = eik_solve(E, C->eik, (float)G.dx, (float)G.dy, (float)lz, s, &rounds))) return rc;
+    if (C->timer) C->timer->used = 0;
+    if ((rc = eik_solve(E, C->eik, (float)G.dx, (float)G.dy, (float)lz, s, &rounds, C->timer))) return rc;
     MCT_CHECK(hipEventRecord(C->ev[2], s));
 
     // (d) picks + misfit
@@ -248,6 +285,8 @@ int mct_body_create(const mct_grid *grid, const mct_body_setup *setup, mct_ctx *
     if (!rc) rc = C->eik.alloc(C->G, (long)M * np * C->nsrc_eff);
     if (!rc && hipHostMalloc((void **)&C->h_out, sizeof(double) * 3, hipHostMallocDefault) != hipSuccess) rc = MCT_ERR_NOMEM;
     if (!rc && hipHostMalloc((void **)&C->h_status, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) rc = MCT_ERR_NOMEM;
+    if (!rc && hipHostMalloc((void **)&C->h_in, sizeof(double) * (6 * (size_t)S + 4 * (size_t)setup->nsrc + 4),
+                             hipHostMallocDefault) != hipSuccess) rc = MCT_ERR_NOMEM;
     if (!rc && hipMemcpy(C->rev.p, setup->rev, sizeof(double) * 3 * setup->nrev, hipMemcpyHostToDevice) != hipSuccess) rc = MCT_ERR_HIP;
     if (!rc && hipMemcpy(C->ttime.p, setup->ttime, sizeof(double) * 3 * npair * np, hipMemcpyHostToDevice) != hipSuccess) rc = MCT_ERR_HIP;
     if (!rc && hipMemcpy(C->raystat.p, setup->raystat, sizeof(int32_t) * 2 * npair * np, hipMemcpyHostToDevice) != hipSuccess) rc = MCT_ERR_HIP;
@@ -259,17 +298,19 @@ int mct_body_create(const mct_grid *grid, const mct_body_setup *setup, mct_ctx *
 int mct_destroy(mct_ctx *C)
 {
     if (!C) return MCT_OK;
-    if (C->stream) hipStreamSynchronize(C->stream);
+    if (C->stream) (void)hipStreamSynchronize(C->stream);
     dbuf *bufs[] = {&C->rev, &C->ttime, &C->raystat, &C->points, &C->params, &C->locations,
                     &C->noise, &C->out, &C->status, &C->vel, &C->flag_count, &C->flag_nodes,
                     &C->kd_nodes, &C->kd_ind, &C->kd_rdata, &C->picks, &C->phase_time,
                     &C->srdist, &C->sigma};
     for (dbuf *b : bufs) dfree(*b);
     C->eik.release();
-    if (C->h_out) hipHostFree(C->h_out);
-    if (C->h_status) hipHostFree(C->h_status);
-    for (int i = 0; i < 4; i++) if (C->ev[i]) hipEventDestroy(C->ev[i]);
-    if (C->stream) hipStreamDestroy(C->stream);
+    if (C->timer) { C->timer->release(); delete C->timer; }
+    if (C->h_out) (void)hipHostFree(C->h_out);
+    if (C->h_status) (void)hipHostFree(C->h_status);
+    if (C->h_in) (void)hipHostFree(C->h_in);
+    for (int i = 0; i < 4; i++) if (C->ev[i]) (void)hipEventDestroy(C->ev[i]);
+    if (C->stream) (void)hipStreamDestroy(C->stream);
     delete C;
     return MCT_OK;
 }
@@ -284,12 +325,19 @@ int mct_body_likelihood(mct_ctx *C, const double *points, const double *params, 
     hipStream_t s = C->stream;
     const int np = C->nwave;
     const long npair = (long)C->S.nsrc * C->S.nrev;
-    MCT_CHECK(hipMemcpyAsync(C->points.p, points, sizeof(double) * 3 * ncells, hipMemcpyHostToDevice, s));
-    MCT_CHECK(hipMemcpyAsync(C->params.p, params, sizeof(double) * 3 * ncells, hipMemcpyHostToDevice, s));
-    MCT_CHECK(hipMemcpyAsync(C->locations.p, locations, sizeof(double) * 4 * C->S.nsrc, hipMemcpyHostToDevice, s));
-    double noise[4] = {0, 0, 0, 0};
-    if (bnoise0) for (int i = 0; i < np; i++) { noise[i] = bnoise0[i]; noise[np + i] = bnoise1[i]; }
-    MCT_CHECK(hipMemcpyAsync(C->noise.p, noise, sizeof(double) * 2 * np, hipMemcpyHostToDevice, s));
+    // stage through pinned memory: the caller's arrays may be reused on return
+    MCT_CHECK(hipStreamSynchronize(s));
+    double *hp = C->h_in, *hq = hp + 3 * (size_t)ncells, *hl = hq + 3 * (size_t)ncells,
+           *hn = hl + 4 * (size_t)C->S.nsrc;
+    memcpy(hp, points, sizeof(double) * 3 * ncells);
+    memcpy(hq, params, sizeof(double) * 3 * ncells);
+    memcpy(hl, locations, sizeof(double) * 4 * C->S.nsrc);
+    for (int i = 0; i < 2 * np; i++) hn[i] = 0.0;
+    if (bnoise0) for (int i = 0; i < np; i++) { hn[i] = bnoise0[i]; hn[np + i] = bnoise1[i]; }
+    MCT_CHECK(hipMemcpyAsync(C->points.p, hp, sizeof(double) * 3 * ncells, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemcpyAsync(C->params.p, hq, sizeof(double) * 3 * ncells, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemcpyAsync(C->locations.p, hl, sizeof(double) * 4 * C->S.nsrc, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemcpyAsync(C->noise.p, hn, sizeof(double) * 2 * np, hipMemcpyHostToDevice, s));
     MCT_CHECK(hipMemsetAsync(C->status.p, 0, sizeof(int32_t), s));
     bool arrays = phase_time || srdist || sigma;
     int rc = body_pipeline(C, 1, P<double>(C->points), P<double>(C->params), nullptr, ncells,
@@ -303,11 +351,8 @@ int mct_body_likelihood(mct_ctx *C, const double *points, const double *params, 
     if (sigma) MCT_CHECK(hipMemcpyAsync(sigma, C->sigma.p, sizeof(double) * npair * np, hipMemcpyDeviceToHost, s));
     MCT_CHECK(hipStreamSynchronize(s));
     for (int i = 0; i < 3; i++) out[i] = C->h_out[i];
-    float f;
-    for (int i = 0; i < 3; i++) {
-        MCT_CHECK(hipEventElapsedTime(&f, C->ev[i], C->ev[i + 1]));
-        C->ms[i] = f;
-    }
+    int src = mct_sync(C);
+    if (src) return src;
     return *C->h_status;
 }
 
@@ -324,6 +369,34 @@ int mct_body_likelihood_batch_dev(mct_ctx *C, int nchain, const double *points_d
                          noise_dev, out_dev, status_dev, false);
 }
 
+int mct_dev_alloc(void **ptr, int64_t bytes)
+{
+    if (!ptr || bytes < 0) return MCT_ERR_ARG;
+    *ptr = nullptr;
+    if (hipMalloc(ptr, bytes > 0 ? (size_t)bytes : 16) != hipSuccess) { *ptr = nullptr; return MCT_ERR_NOMEM; }
+    return MCT_OK;
+}
+
+int mct_dev_free(void *ptr)
+{
+    if (ptr) MCT_CHECK(hipFree(ptr));
+    return MCT_OK;
+}
+
+int mct_copy_h2d(void *dst_dev, const void *src_host, int64_t bytes)
+{
+    if (bytes < 0 || (bytes > 0 && (!dst_dev || !src_host))) return MCT_ERR_ARG;
+    if (bytes) MCT_CHECK(hipMemcpy(dst_dev, src_host, (size_t)bytes, hipMemcpyHostToDevice));
+    return MCT_OK;
+}
+
+int mct_copy_d2h(void *dst_host, const void *src_dev, int64_t bytes)
+{
+    if (bytes < 0 || (bytes > 0 && (!dst_host || !src_dev))) return MCT_ERR_ARG;
+    if (bytes) MCT_CHECK(hipMemcpy(dst_host, src_dev, (size_t)bytes, hipMemcpyDeviceToHost));
+    return MCT_OK;
+}
+
 int mct_sync(mct_ctx *C)
 {
     if (!C) return MCT_ERR_ARG;
@@ -331,13 +404,28 @@ int mct_sync(mct_ctx *C)
     float f;
     for (int i = 0; i < 3; i++)
         if (hipEventElapsedTime(&f, C->ev[i], C->ev[i + 1]) == hipSuccess) C->ms[i] = f;
+    if (C->timer && C->timer->on) C->timer->collect();
     return MCT_OK;
 }
 
-int mct_last_timings(mct_ctx *C, double *ms4)
+int mct_set_kernel_timing(mct_ctx *C, int on)
 {
-    if (!C || !ms4) return MCT_ERR_ARG;
-    for (int i = 0; i < 4; i++) ms4[i] = C->ms[i];
+    if (!C) return MCT_ERR_ARG;
+    if (on && !C->timer) {
+        C->timer = new (std::nothrow) round_timer();
+        if (!C->timer) return MCT_ERR_NOMEM;
+        if (C->timer->ensure()) { C->timer->release(); delete C->timer; C->timer = nullptr; return MCT_ERR_HIP; }
+    }
+    if (C->timer) C->timer->on = on != 0;
+    return MCT_OK;
+}
+
+int mct_last_timings(mct_ctx *C, double *ms6)
+{
+    if (!C || !ms6) return MCT_ERR_ARG;
+    for (int i = 0; i < 4; i++) ms6[i] = C->ms[i];
+    ms6[4] = (C->timer && C->timer->on) ? C->timer->kernel_ms : 0.0;
+    ms6[5] = (C->timer && C->timer->on) ? (double)C->timer->used : 0.0;
     return MCT_OK;
 }
 

@@ -18,7 +18,7 @@ struct ws_buf {
     {
         if (bytes == 0) bytes = 16;
         if (n >= bytes) return MCT_OK;
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr; n = 0;
         if (hipMalloc(&p, bytes) != hipSuccess) return MCT_ERR_NOMEM;
         n = bytes;
