@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--order", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-evals", type=int, default=20)
+    ap.add_argument("--band-plain", type=float, default=0.5)
+    ap.add_argument("--band-batched", type=float, default=0.25)
     args = ap.parse_args()
     if args.gpus != 1:
         log("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU "
@@ -93,6 +95,7 @@ def main():
         like.sync()
 
     # ---- plain single-request drop-in: latency per evaluation ----------------
+    like.set_tuning(args.band_plain)
     for _ in range(args.warmup):
         run(1)
     lat = []
@@ -106,6 +109,7 @@ def main():
                                                    np.quantile(lat, 0.99) * 1e3), like.timings())
 
     # ---- batched: the timed region ------------------------------------------
+    like.set_tuning(args.band_batched)
     for _ in range(args.warmup):
         run(B)
     t0 = time.perf_counter()
@@ -134,7 +138,7 @@ def main():
     achieved = alg_per_launch / (avg_launch_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-            "kernel": "k_eik_round<0>", "avg_launch_ms": round(avg_launch_ms, 5),
+            "kernel": "k_eik_round<0,double>", "avg_launch_ms": round(avg_launch_ms, 5),
             "launches": launches}
 
     # ---- parity of this run vs the oracle, and the CPU baseline --------------
@@ -155,14 +159,15 @@ def main():
     res = {
         "metric": METRIC, "value": round(batched_eps, 3), "unit": "evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": "3-D body-wave eikonal drop-in: 100 Voronoi sites, Vp U[4,7] km/s, "
                                "101x101x101 grid at 0.2 km, 50 sources x 400 receivers, P, "
                                "order %d; batch of %d chains per step" % (args.order, B),
                    "batch": B, "grid": [w.grid.nx, w.grid.ny, w.grid.nz], "nsrc": w.nsrc,
                    "nrev": int(w.rev.shape[0]), "ncells": ncells, "order": args.order,
-                   "precision": "eikonal f32, nearest-site/picks/misfit f64"},
+                   "precision": "eikonal local solve f32, arrival times f64; nearest-site, picks, "
+                                "misfit f64", "band_tiles": [args.band_plain, args.band_batched]},
         "plain_evals_per_s": round(plain_eps, 3),
         "p50_ms": round(float(np.median(lat)) * 1e3, 3),
         "p99_ms": round(float(np.quantile(lat, 0.99)) * 1e3, 3),

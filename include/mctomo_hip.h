@@ -122,6 +122,12 @@ int mct_dev_free(void *ptr);
 int mct_copy_h2d(void *dst_dev, const void *src_host, int64_t bytes);
 int mct_copy_d2h(void *dst_host, const void *src_dev, int64_t bytes);
 
+/* Eikonal schedule and precision of a context: band_tiles is how many 4^3
+ * tiles the relaxation band advances per round (smaller = fewer re-visits,
+ * more rounds); t64 = 1 stores arrival times in FP64 (the parity setting),
+ * 0 in FP32.  Values <= 0 / < 0 leave the setting unchanged. */
+int mct_set_tuning(mct_ctx *ctx, float band_tiles, int t64);
+
 /* Wait for the context's stream. */
 int mct_sync(mct_ctx *ctx);
 

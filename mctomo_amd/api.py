@@ -59,7 +59,7 @@ def lib():
         L.mct_version.restype = ctypes.c_char_p
         for name in ("fastMarching3D", "mct_voronoi_to_grid", "mct_body_create", "mct_destroy",
                      "mct_body_likelihood", "mct_body_likelihood_batch_dev", "mct_sync",
-                     "mct_last_timings", "mct_set_kernel_timing", "mct_dev_alloc", "mct_dev_free", "mct_copy_h2d",
+                     "mct_last_timings", "mct_set_kernel_timing", "mct_set_tuning", "mct_dev_alloc", "mct_dev_free", "mct_copy_h2d",
                      "mct_copy_d2h"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
@@ -266,6 +266,10 @@ class BodyLikelihood:
 
     def sync(self):
         _check(lib().mct_sync(self._h), "mct_sync")
+
+    def set_tuning(self, band_tiles=0.0, t64=-1):
+        """Eikonal band advance per round (tiles) and arrival-time precision."""
+        _check(lib().mct_set_tuning(self._h, ctypes.c_float(band_tiles), int(t64)), "mct_set_tuning")
 
     def set_kernel_timing(self, on):
         _check(lib().mct_set_kernel_timing(self._h, int(bool(on))), "mct_set_kernel_timing")

@@ -42,25 +42,26 @@ template <class T> T *P(dbuf &b) { return (T *)b.p; }
 
 // Eikonal workspace for a set of problems on one geometry.
 struct eik_ws {
-    dbuf T, list0, list1, flags0, flags1, counts;
+    dbuf T, flags0, flags1, key0, key1, counts, band;
     uint32_t *h_count = nullptr;     // pinned
     int alloc(const mct_geom &G, long nprob)
     {
         long nt = nprob * G.ntiles;
         int rc;
-        if ((rc = dalloc(T, sizeof(float) * nt * MCT_TILE_N))) return rc;
-        if ((rc = dalloc(list0, sizeof(uint32_t) * nt))) return rc;
-        if ((rc = dalloc(list1, sizeof(uint32_t) * nt))) return rc;
+        if ((rc = dalloc(T, sizeof(double) * nt * MCT_TILE_N))) return rc;
         if ((rc = dalloc(flags0, (size_t)((nt + 3) & ~3L)))) return rc;
         if ((rc = dalloc(flags1, (size_t)((nt + 3) & ~3L)))) return rc;
         if ((rc = dalloc(counts, 4 * sizeof(uint32_t)))) return rc;
+        if ((rc = dalloc(key0, sizeof(uint32_t) * nt))) return rc;
+        if ((rc = dalloc(key1, sizeof(uint32_t) * nt))) return rc;
+        if ((rc = dalloc(band, sizeof(float) * (nprob + 1)))) return rc;
         if (!h_count && hipHostMalloc((void **)&h_count, 4 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
             return MCT_ERR_NOMEM;
         return MCT_OK;
     }
     void release()
     {
-        dfree(T); dfree(list0); dfree(list1); dfree(flags0); dfree(flags1); dfree(counts);
+        dfree(T); dfree(flags0); dfree(flags1); dfree(key0); dfree(key1); dfree(counts); dfree(band);
         if (h_count) (void)hipHostFree(h_count);
         h_count = nullptr;
     }
@@ -116,7 +117,7 @@ int eik_solve(const mct_eik_problem_set &Pset, eik_ws &ws, float hx, float hy, f
               hipStream_t s, int *rounds_out, round_timer *tm)
 {
     const mct_geom &G = Pset.G;
-    const int max_rounds = 8 * (G.ntx + G.nty + G.ntz) + 256;
+    const int max_rounds = Pset.band_rounds + 8 * (G.ntx + G.nty + G.ntz) + 256;
     const int K = 8;
     int blocks = round_grid_blocks();
     int r = 0;
@@ -128,8 +129,8 @@ int eik_solve(const mct_eik_problem_set &Pset, eik_ws &ws, float hx, float hy, f
             if (rc) return rc;
             if (timed) { MCT_CHECK(hipEventRecord(tm->b[r], s)); tm->used = r + 1; }
         }
-        // counts[r % 3] holds the size of the list the next round would read
-        MCT_CHECK(hipMemcpyAsync(ws.h_count, Pset.counts + (r % 3), sizeof(uint32_t),
+        // any[r % 3] is set when some tile is active for round r
+        MCT_CHECK(hipMemcpyAsync(ws.h_count, Pset.any + (r % 3), sizeof(uint32_t),
                                  hipMemcpyDeviceToHost, s));
         MCT_CHECK(hipStreamSynchronize(s));
         if (*ws.h_count == 0) break;
@@ -145,6 +146,8 @@ struct mct_ctx {
     mct_geom G;
     mct_body_setup S;
     int nwave, nsrc_eff, npick, reciprocal;
+    float band_tiles;                // eikonal schedule: tiles the band advances per round
+    int t64;                         // eikonal arrival times stored as double
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double ms[4] = {0, 0, 0, 0};
@@ -189,10 +192,16 @@ static int body_pipeline(mct_ctx *C, int nchain, const double *d_points, const d
     memset(&E, 0, sizeof E);
     E.G = G; E.nsrc_eff = C->nsrc_eff; E.nwave = C->nwave; E.order = C->S.order;
     E.nprob = nchain * C->nwave * C->nsrc_eff;
-    E.T = P<float>(C->eik.T); E.vel = P<float>(C->vel);
-    E.list[0] = P<uint32_t>(C->eik.list0); E.list[1] = P<uint32_t>(C->eik.list1);
-    E.flags[0] = P<uint8_t>(C->eik.flags0); E.flags[1] = P<uint8_t>(C->eik.flags1);
-    E.counts = P<uint32_t>(C->eik.counts);
+    E.T = C->eik.T.p; E.t64 = C->t64; E.vel = P<float>(C->vel);
+        E.flags[0] = P<uint8_t>(C->eik.flags0); E.flags[1] = P<uint8_t>(C->eik.flags1);
+    E.any = P<uint32_t>(C->eik.counts);
+    E.key[0] = P<uint32_t>(C->eik.key0); E.key[1] = P<uint32_t>(C->eik.key1);
+    E.band_dt = P<float>(C->eik.band);
+    E.band_rounds = mct_band_rounds(G, C->band_tiles);
+    if ((rc = mct_launch_band_dt(E.vel, (long)G.ntiles * MCT_TILE_N, nchain * C->nwave,
+                                 (float)fmin(fmin(G.dx, G.dy), lz), C->band_tiles,
+                                 P<float>(C->eik.band), s)))
+        return rc;
     const double *src_pts; int sstride; long schain;
     const double *pick_pts; int pstride; int pchain;
     if (!C->reciprocal) {
@@ -211,7 +220,7 @@ static int body_pipeline(mct_ctx *C, int nchain, const double *d_points, const d
 
     // (d) picks + misfit
     mct_pick_args pa;
-    pa.G = G; pa.T = E.T; pa.nchain = nchain; pa.nwave = C->nwave; pa.nsrc_eff = C->nsrc_eff;
+    pa.G = G; pa.T = E.T; pa.t64 = E.t64; pa.nchain = nchain; pa.nwave = C->nwave; pa.nsrc_eff = C->nsrc_eff;
     pa.npick = C->npick; pa.pick_pts = pick_pts; pa.pick_stride_chain = pchain;
     pa.pick_point_stride = pstride; pa.xmin = G.xmin; pa.ymin = G.ymin; pa.zmin = zmin;
     pa.lx = G.dx; pa.ly = G.dy; pa.lz = lz; pa.out = P<double>(C->picks);
@@ -254,6 +263,8 @@ int mct_body_create(const mct_grid *grid, const mct_body_setup *setup, mct_ctx *
     C->reciprocal = setup->nsrc > setup->nrev;
     C->nsrc_eff = C->reciprocal ? setup->nrev : setup->nsrc;
     C->npick = C->reciprocal ? setup->nsrc : setup->nrev;
+    C->band_tiles = mct_default_band_tiles();
+    C->t64 = mct_default_t64();
     int rc = MCT_OK;
     const long npair = (long)setup->nsrc * setup->nrev;
     const int np = C->nwave, M = setup->max_chains, S = setup->max_cells;
@@ -405,6 +416,14 @@ int mct_sync(mct_ctx *C)
     for (int i = 0; i < 3; i++)
         if (hipEventElapsedTime(&f, C->ev[i], C->ev[i + 1]) == hipSuccess) C->ms[i] = f;
     if (C->timer && C->timer->on) C->timer->collect();
+    return MCT_OK;
+}
+
+int mct_set_tuning(mct_ctx *C, float band_tiles, int t64)
+{
+    if (!C) return MCT_ERR_ARG;
+    if (band_tiles > 0.f) C->band_tiles = band_tiles;
+    if (t64 >= 0) C->t64 = t64 ? 1 : 0;
     return MCT_OK;
 }
 

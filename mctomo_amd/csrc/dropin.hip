@@ -29,7 +29,7 @@ struct ws_buf {
 struct dropin_ws {
     std::mutex mu;
     hipStream_t s = nullptr;
-    ws_buf vel_d, vel_t, T, list0, list1, fl0, fl1, counts, src, rev, tt, field, pts, prm,
+    ws_buf vel_d, vel_t, T, fl0, fl1, k0, k1, band, counts, src, rev, tt, field, pts, prm,
         sid, vp, vs, rho, pm, fcount, fnodes, kdn, kdi, kdr;
     uint32_t *h_count = nullptr;
     int init()
@@ -50,7 +50,7 @@ dropin_ws &WS()
     return w;
 }
 
-// Fortran (nz,ny,nx) double velocity -> FP32 eikonal tiles (padding lanes 1.0)
+// Fortran (nz,ny,nx) double velocity -> FP32 eikonal tiles (padding lanes 0)
 __global__ void k_fortran_to_tiles(const double *v, float *vt, mct_geom G)
 {
     long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -59,14 +59,14 @@ __global__ void k_fortran_to_tiles(const double *v, float *vt, mct_geom G)
     int tile = (int)(e >> 6), lane = (int)(e & 63);
     int tx = tile % G.ntx, ty = (tile / G.ntx) % G.nty, tz = tile / (G.ntx * G.nty);
     int x = tx * 4 + (lane & 3), y = ty * 4 + ((lane >> 2) & 3), z = tz * 4 + (lane >> 4);
-    float out = 1.0f;
+    float out = 0.0f;
     if (x < G.nx && y < G.ny && z < G.nz)
         out = (float)v[(long)z + (long)G.nz * ((long)y + (long)G.ny * x)];
     vt[e] = out;
 }
 
-// FP32 tiles -> Fortran (nz,ny,nx) double field
-__global__ void k_tiles_to_fortran(const float *t, double *f, mct_geom G)
+// tiles -> Fortran (nz,ny,nx) double field
+__global__ void k_tiles_to_fortran(const void *tv, int t64, double *f, mct_geom G)
 {
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long n = (long)G.nx * G.ny * G.nz;
@@ -74,14 +74,15 @@ __global__ void k_tiles_to_fortran(const float *t, double *f, mct_geom G)
     int z = (int)(g % G.nz), y = (int)((g / G.nz) % G.ny), x = (int)(g / ((long)G.nz * G.ny));
     int tile = ((z >> 2) * G.nty + (y >> 2)) * G.ntx + (x >> 2);
     int lane = ((z & 3) << 4) | ((y & 3) << 2) | (x & 3);
-    f[g] = (double)t[((long)tile << 6) | lane];
+    long i = ((long)tile << 6) | lane;
+    f[g] = t64 ? ((const double *)tv)[i] : (double)((const float *)tv)[i];
 }
 
 int eik_rounds(const mct_eik_problem_set &E, uint32_t *h_count, float hx, float hy, float hz,
                hipStream_t s)
 {
     const mct_geom &G = E.G;
-    const int max_rounds = 8 * (G.ntx + G.nty + G.ntz) + 256;
+    const int max_rounds = E.band_rounds + 8 * (G.ntx + G.nty + G.ntz) + 256;
     int dev = 0, cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
@@ -92,7 +93,7 @@ int eik_rounds(const mct_eik_problem_set &E, uint32_t *h_count, float hx, float 
             int rc = mct_launch_eik_round(E, r, hx, hy, hz, cus * 32, s);
             if (rc) return rc;
         }
-        MCT_CHECK(hipMemcpyAsync(h_count, E.counts + (r % 3), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        MCT_CHECK(hipMemcpyAsync(h_count, E.any + (r % 3), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         MCT_CHECK(hipStreamSynchronize(s));
         if (*h_count == 0) return MCT_OK;
         if (r >= max_rounds) return MCT_ERR_NOCONV;
@@ -117,9 +118,10 @@ extern "C" int fastMarching3D(int nx, int ny, int nz, double dx, double dy, doub
     mct_geom G = mct_make_geom(&g);
     const long gn = (long)nx * ny * nz, tn = (long)G.ntiles * MCT_TILE_N;
     if ((rc = W.vel_d.need(sizeof(double) * gn)) || (rc = W.vel_t.need(sizeof(float) * tn)) ||
-        (rc = W.T.need(sizeof(float) * tn)) || (rc = W.list0.need(sizeof(uint32_t) * G.ntiles)) ||
-        (rc = W.list1.need(sizeof(uint32_t) * G.ntiles)) || (rc = W.fl0.need((G.ntiles + 3) & ~3)) ||
+        (rc = W.T.need(sizeof(double) * tn)) || (rc = W.fl0.need((G.ntiles + 3) & ~3)) ||
         (rc = W.fl1.need((G.ntiles + 3) & ~3)) || (rc = W.counts.need(4 * sizeof(uint32_t))) ||
+        (rc = W.k0.need(sizeof(uint32_t) * G.ntiles)) || (rc = W.k1.need(sizeof(uint32_t) * G.ntiles)) ||
+        (rc = W.band.need(sizeof(float) * 4)) ||
         (rc = W.src.need(4 * sizeof(double))) || (rc = W.rev.need(sizeof(double) * 3 * (nrev + 1))) ||
         (rc = W.tt.need(sizeof(double) * (nrev + 1))) ||
         (tfield && (rc = W.field.need(sizeof(double) * gn))))
@@ -134,15 +136,20 @@ extern "C" int fastMarching3D(int nx, int ny, int nz, double dx, double dy, doub
     mct_eik_problem_set E;
     memset(&E, 0, sizeof E);
     E.G = G; E.nprob = 1; E.nsrc_eff = 1; E.nwave = 1; E.order = order == 1 ? 1 : 0;
-    E.T = (float *)W.T.p; E.vel = (const float *)W.vel_t.p;
-    E.list[0] = (uint32_t *)W.list0.p; E.list[1] = (uint32_t *)W.list1.p;
-    E.flags[0] = (uint8_t *)W.fl0.p; E.flags[1] = (uint8_t *)W.fl1.p;
-    E.counts = (uint32_t *)W.counts.p;
+    E.T = W.T.p; E.t64 = mct_default_t64(); E.vel = (const float *)W.vel_t.p;
+        E.flags[0] = (uint8_t *)W.fl0.p; E.flags[1] = (uint8_t *)W.fl1.p;
+    E.any = (uint32_t *)W.counts.p;
+    E.key[0] = (uint32_t *)W.k0.p; E.key[1] = (uint32_t *)W.k1.p;
+    E.band_dt = (const float *)W.band.p;
+    E.band_rounds = mct_band_rounds(G, mct_default_band_tiles());
+    if ((rc = mct_launch_band_dt(E.vel, tn, 1, (float)fmin(fmin(dx, dy), dz), mct_default_band_tiles(),
+                                 (float *)W.band.p, s)))
+        return rc;
     if ((rc = mct_launch_eik_init(E, (const double *)W.src.p, 3, 0, xmin, ymin, zmin, dx, dy, dz, s))) return rc;
     if ((rc = eik_rounds(E, W.h_count, (float)dx, (float)dy, (float)dz, s))) return rc;
     if (nrev > 0) {
         mct_pick_args pa;
-        pa.G = G; pa.T = E.T; pa.nchain = 1; pa.nwave = 1; pa.nsrc_eff = 1; pa.npick = nrev;
+        pa.G = G; pa.T = E.T; pa.t64 = E.t64; pa.nchain = 1; pa.nwave = 1; pa.nsrc_eff = 1; pa.npick = nrev;
         pa.pick_pts = (const double *)W.rev.p; pa.pick_stride_chain = 0; pa.pick_point_stride = 3;
         pa.xmin = xmin; pa.ymin = ymin; pa.zmin = zmin; pa.lx = dx; pa.ly = dy; pa.lz = dz;
         pa.out = (double *)W.tt.p;
@@ -151,7 +158,7 @@ extern "C" int fastMarching3D(int nx, int ny, int nz, double dx, double dy, doub
     }
     if (tfield) {
         hipLaunchKernelGGL(k_tiles_to_fortran, dim3((unsigned)((gn + 255) / 256)), dim3(256), 0, s,
-                           (const float *)W.T.p, (double *)W.field.p, G);
+                           (const void *)W.T.p, E.t64, (double *)W.field.p, G);
         MCT_CHECK(hipGetLastError());
         MCT_CHECK(hipMemcpyAsync(tfield, W.field.p, sizeof(double) * gn, hipMemcpyDeviceToHost, s));
     }

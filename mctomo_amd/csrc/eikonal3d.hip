@@ -18,89 +18,110 @@
 // cancellation of the reference's b^2-4ac form.
 #include "mct_internal.h"
 #include <float.h>
+#include <stdlib.h>
 
 #pragma clang fp contract(fast)
 
 #define EIK_INF __builtin_huge_valf()
+template <class TT> __device__ __forceinline__ TT tinf();
+template <> __device__ __forceinline__ float tinf<float>() { return __builtin_huge_valf(); }
+template <> __device__ __forceinline__ double tinf<double>() { return __builtin_huge_val(); }
 // LDS box: the 4^3 tile at offset 2 inside an 8^3 box (2-deep face halo).
 #define BX(x, y, z) ((((z) + 2) << 6) | (((y) + 2) << 3) | ((x) + 2))
 
-struct eik_term { float t1, t2; int dim; };
-
-// Solve sum_i alpha_i (U - tt_i)^2 = s2 for U, shifted by t0 (<= every tt_i).
-__device__ __forceinline__ float quad_root(const float *alpha, const float *dlt, int n, float s2)
+// Compare-and-swap of (time, 1/h^2, h) triples by time; the swap keeps the
+// earlier dimension first on equal times (std::sort on <= 16 elements is an
+// insertion sort, which is stable).
+__device__ __forceinline__ void cas3(float &t0, float &a0, float &h0, float &t1, float &a1, float &h1)
 {
-    float A = 0.f, B = 0.f, C = -s2;
-    for (int i = 0; i < n; i++) {
-        A += alpha[i];
-        B += alpha[i] * dlt[i];
-        C += alpha[i] * dlt[i] * dlt[i];
-    }
-    float disc = B * B - A * C;
-    if (disc < 0.f) return EIK_INF;
-    return (B + sqrtf(disc)) / A;
+    bool sw = t1 < t0;
+    float tt = sw ? t1 : t0, aa = sw ? a1 : a0, hh = sw ? h1 : h0;
+    t1 = sw ? t0 : t1; a1 = sw ? a0 : a1; h1 = sw ? h0 : h1;
+    t0 = tt; a0 = aa; h0 = hh;
 }
 
-// solveEikonal: first order, grow the dimension count while it improves.
+__device__ __forceinline__ void cas4(float &t0, float &u0, float &a0, float &h0,
+                                     float &t1, float &u1, float &a1, float &h1)
+{
+    bool sw = t1 < t0;
+    float tt = sw ? t1 : t0, uu = sw ? u1 : u0, aa = sw ? a1 : a0, hh = sw ? h1 : h0;
+    t1 = sw ? t0 : t1; u1 = sw ? u0 : u1; a1 = sw ? a0 : a1; h1 = sw ? h0 : h1;
+    t0 = tt; u0 = uu; a0 = aa; h0 = hh;
+}
+
+// root of sum_i al_i (tau - dl_i)^2 = s2 (larger root), INF if none
+__device__ __forceinline__ float root2(float a0, float d0, float a1, float d1, float s2)
+{
+    float A = a0 + a1, B = a0 * d0 + a1 * d1, C = a0 * d0 * d0 + a1 * d1 * d1 - s2;
+    float disc = B * B - A * C;
+    return disc < 0.f ? EIK_INF : (B + sqrtf(disc)) / A;
+}
+
+__device__ __forceinline__ float root3(float a0, float d0, float a1, float d1, float a2, float d2,
+                                       float s2)
+{
+    float A = a0 + a1 + a2, B = a0 * d0 + a1 * d1 + a2 * d2;
+    float C = a0 * d0 * d0 + a1 * d1 * d1 + a2 * d2 * d2 - s2;
+    float disc = B * B - A * C;
+    return disc < 0.f ? EIK_INF : (B + sqrtf(disc)) / A;
+}
+
+// solveEikonal (first order): dimensions whose smaller neighbour beats the
+// cell's own value, sorted, grown while the next one is below the solution.
+// Written relative to the smallest neighbour time t0.
 __device__ __forceinline__ float update_order0(const float m[3], float tself, const float h[3],
                                                const float ih2[3], float inv_v, float s2)
 {
-    eik_term T[3];
-    int a = 0;
-    for (int d = 0; d < 3; d++)
-        if (m[d] < tself) { T[a].t1 = m[d]; T[a].dim = d; a++; }   // m is INF when absent
-    if (a == 0) return EIK_INF;
-    // stable insertion sort by t1 (std::sort on <= 16 elements)
-    for (int i = 1; i < a; i++) {
-        eik_term x = T[i];
-        int j = i - 1;
-        while (j >= 0 && x.t1 < T[j].t1) { T[j + 1] = T[j]; j--; }
-        T[j + 1] = x;
-    }
-    float t0 = T[0].t1;
-    float up = t0 + h[T[0].dim] * inv_v;
-    if (a == 1 || up <= T[1].t1) return up;
-    float al[3], dl[3];
-    for (int i = 0; i < a; i++) { al[i] = ih2[T[i].dim]; dl[i] = T[i].t1 - t0; }
-    float tau = quad_root(al, dl, 2, s2);
-    if (a == 2 || tau <= dl[2]) return t0 + tau;
-    tau = quad_root(al, dl, 3, s2);
-    return t0 + tau;
+    float t0 = m[0] < tself ? m[0] : EIK_INF, t1 = m[1] < tself ? m[1] : EIK_INF,
+          t2 = m[2] < tself ? m[2] : EIK_INF;
+    float a0 = ih2[0], a1 = ih2[1], a2 = ih2[2], h0 = h[0], h1 = h[1], h2 = h[2];
+    cas3(t0, a0, h0, t1, a1, h1);
+    cas3(t1, a1, h1, t2, a2, h2);
+    cas3(t0, a0, h0, t1, a1, h1);
+    if (t0 == EIK_INF) return EIK_INF;
+    float u1 = t0 + h0 * inv_v;
+    if (!(u1 > t1)) return u1;                     // one dimension, or the next is not below
+    float d1 = t1 - t0;
+    float tau2 = root2(a0, 0.f, a1, d1, s2);
+    if (!(tau2 + t0 > t2)) return t0 + tau2;
+    float tau3 = root3(a0, 0.f, a1, d1, a2, t2 - t0, s2);
+    return t0 + tau3;
 }
 
-// highAccuracySolveEikonal: second order where the two-away value is upwind.
+// highAccuracySolveEikonal: per dimension the smaller one-away time t and, when
+// the two-away time on that side is upwind (<= t), the second-order value
+// (4t - t2)/3 with weight 9/4; try all included dimensions first and drop the
+// largest while the root does not exceed that dimension's time.
 __device__ __forceinline__ float update_order1(const float m1[3], const float m2[3], float tself,
                                                const float h[3], const float ih2[3], float inv_v,
                                                float s2)
 {
-    eik_term T[3];
-    int a = 0;
+    float t[3], u[3], al[3], hh[3];
+    for (int d = 0; d < 3; d++) {
+        bool inc = m1[d] < tself;
+        bool two = m2[d] < EIK_INF;
+        t[d] = inc ? m1[d] : EIK_INF;
+        u[d] = two ? m2[d] : EIK_INF;
+        al[d] = two ? 2.25f * ih2[d] : ih2[d];
+        hh[d] = h[d];
+    }
+    cas4(t[0], u[0], al[0], hh[0], t[1], u[1], al[1], hh[1]);
+    cas4(t[1], u[1], al[1], hh[1], t[2], u[2], al[2], hh[2]);
+    cas4(t[0], u[0], al[0], hh[0], t[1], u[1], al[1], hh[1]);
+    if (t[0] == EIK_INF) return tself;
+    const float t0 = t[0];
+    float dl[3];
     for (int d = 0; d < 3; d++)
-        if (m1[d] < tself) { T[a].t1 = m1[d]; T[a].t2 = m2[d]; T[a].dim = d; a++; }
-    if (a == 0) return tself;
-    for (int i = 1; i < a; i++) {
-        eik_term x = T[i];
-        int j = i - 1;
-        while (j >= 0 && x.t1 < T[j].t1) { T[j + 1] = T[j]; j--; }
-        T[j + 1] = x;
+        dl[d] = u[d] < EIK_INF ? (4.f * (t[d] - t0) - (u[d] - t0)) * (1.f / 3.f) : t[d] - t0;
+    if (t[2] < EIK_INF) {
+        float tau = root3(al[0], dl[0], al[1], dl[1], al[2], dl[2], s2);
+        if (tau < EIK_INF && tau > t[2] - t0) return t0 + tau;
     }
-    float t0 = T[0].t1;
-    float al[3], dl[3];
-    for (int i = 0; i < a; i++) {
-        float hh = ih2[T[i].dim];
-        if (T[i].t2 < EIK_INF) {
-            al[i] = 2.25f * hh;
-            dl[i] = (4.f * (T[i].t1 - t0) - (T[i].t2 - t0)) * (1.f / 3.f);
-        } else {
-            al[i] = hh;
-            dl[i] = T[i].t1 - t0;
-        }
+    if (t[1] < EIK_INF) {
+        float tau = root2(al[0], dl[0], al[1], dl[1], s2);
+        if (tau < EIK_INF && tau > t[1] - t0) return t0 + tau;
     }
-    for (int i = a; i >= 2; --i) {
-        float tau = quad_root(al, dl, i, s2);
-        if (tau < EIK_INF && tau > T[i - 1].t1 - t0) return t0 + tau;
-    }
-    return t0 + h[T[0].dim] * inv_v;
+    return t0 + hh[0] * inv_v;
 }
 
 // ---- initial points --------------------------------------------------------------
@@ -126,20 +147,14 @@ __device__ __forceinline__ long tl_index(const mct_geom &G, int x, int y, int z)
     return ((long)tile << 6) | lane;
 }
 
-__device__ __forceinline__ bool flag_set(uint8_t *flags, uint32_t key)
-{
-    unsigned int *wordp = (unsigned int *)((uintptr_t)(flags + key) & ~(uintptr_t)3);
-    unsigned int sh = (unsigned int)(((uintptr_t)(flags + key) & 3u) * 8u);
-    unsigned int old = atomicOr(wordp, 1u << sh);
-    return !((old >> sh) & 1u);
-}
-
-__global__ void k_eik_clear(float *T, long n)
+template <class TT>
+__global__ void k_eik_clear(TT *T, long n)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        T[i] = EIK_INF;
+        T[i] = tinf<TT>();
 }
 
+template <class TT>
 __global__ void k_eik_init(mct_eik_problem_set P, const double *src_pts, int src_stride,
                            long src_chain_stride, double xmin, double ymin, double zmin, double lx,
                            double ly, double lz)
@@ -156,7 +171,7 @@ __global__ void k_eik_init(mct_eik_problem_set P, const double *src_pts, int src
     int pc[3];
     pt_coord(G, p, leaf, pc);
     const float *V = P.vel + (long)cw * G.ntiles * MCT_TILE_N;
-    float *T = P.T + (long)prob * G.ntiles * MCT_TILE_N;
+    TT *T = (TT *)P.T + (long)prob * G.ntiles * MCT_TILE_N;
     int cx[8], cy[8], cz[8];
     double vn[8];
     for (int v = 0; v < 8; v++) {
@@ -185,9 +200,10 @@ __global__ void k_eik_init(mct_eik_problem_set P, const double *src_pts, int src
         double dist = 0.0;
         for (int d = 0; d < 3; d++) dist = dist + (q[d] - p[d]) * (q[d] - p[d]);
         dist = sqrt(dist);
-        T[tl_index(G, cx[v], cy[v], cz[v])] = (float)(dist * 2 / (vs + vn[v]));
+        T[tl_index(G, cx[v], cy[v], cz[v])] = (TT)(dist * 2 / (vs + vn[v]));
     }
     // seed: the 3x3x3 block of tiles around each corner's tile
+    P.any[0] = 1u;
     for (int v = 0; v < 8; v++) {
         int tx = cx[v] >> 2, ty = cy[v] >> 2, tz = cz[v] >> 2;
         for (int dz = -1; dz <= 1; dz++)
@@ -195,134 +211,194 @@ __global__ void k_eik_init(mct_eik_problem_set P, const double *src_pts, int src
                 for (int dx = -1; dx <= 1; dx++) {
                     int ax = tx + dx, ay = ty + dy, az = tz + dz;
                     if (ax < 0 || ay < 0 || az < 0 || ax >= G.ntx || ay >= G.nty || az >= G.ntz) continue;
-                    uint32_t key = (uint32_t)prob * (uint32_t)G.ntiles + (uint32_t)((az * G.nty + ay) * G.ntx + ax);
-                    if (flag_set(P.flags[0], key)) {
-                        uint32_t pos = atomicAdd(&P.counts[0], 1u);
-                        P.list[0][pos] = key;
-                    }
+                    long key = (long)prob * G.ntiles + (long)((az * G.nty + ay) * G.ntx + ax);
+                    P.flags[0][key] = 1;
+                    P.key[0][key] = 0u;
                 }
     }
 }
 
 // ---- one relaxation round ----------------------------------------------------------
-template <int ORDER>
+__device__ __forceinline__ float wave_min(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+
+
+// One relaxation round.  Active tiles are byte flags (no lists, no counters);
+// each carries a key, the smallest arrival time that activated it.  A tile is
+// relaxed only once the round's band time tau has passed its key, so the
+// front is swept roughly in arrival order (a tile sees its upwind halo nearly
+// final); tiles not yet due are carried to the next round unchanged.
+//
+// TT is the storage type of arrival times.  The local quadratic is always
+// solved in FP32 on times relative to the smallest upwind neighbour t0 (an
+// increment of about h/v), then added to t0 in TT: with TT = double the
+// rounding of each step is that of the small increment, so errors do not
+// accumulate along a ray at the FP32 level.
+template <int ORDER, class TT>
 __global__ __launch_bounds__(64) void k_eik_round(mct_eik_problem_set P, int round, float hx, float hy, float hz)
 {
-    __shared__ float box[512];
+    __shared__ TT box[512];
     const mct_geom &G = P.G;
+    const TT TINF = tinf<TT>();
     const int cur = round & 1, nxt = cur ^ 1;
-    const uint32_t count = P.counts[round % 3];
     const int lane = threadIdx.x;
     const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
     const float h[3] = {hx, hy, hz};
     const float ih2[3] = {1.f / (hx * hx), 1.f / (hy * hy), 1.f / (hz * hz)};
-    if (blockIdx.x == 0 && lane == 0) P.counts[(round + 2) % 3] = 0u;   // next round's append target
-    const long tsz = (long)G.ntiles * MCT_TILE_N;
+    if (blockIdx.x == 0 && lane == 0) P.any[(round + 2) % 3] = 0u;   // flag of round r+2
+    const uint32_t ntiles = (uint32_t)G.ntiles;
+    const uint32_t nkeys = (uint32_t)P.nprob * ntiles;
+    const uint32_t stride = gridDim.x;
+    uint8_t *fcur = P.flags[cur], *fnxt = P.flags[nxt];
+    uint32_t *kcur = P.key[cur], *knxt = P.key[nxt];
+    unsigned int *any_next = &P.any[(round + 1) % 3];
+    const bool banded = round < P.band_rounds;
+    const TT thr = sizeof(TT) == 8 ? (TT)1e-9 : (TT)1e-7;   // relative change that counts
+    TT *Tall = (TT *)P.T;
+    // face lane masks (bit l = lane l): the layers a neighbour reads as halo
+    const int deep = ORDER == 0 ? 0 : 1;
+    uint64_t fm_mine = 0;
+    for (int l = 0; l < 64; l++) {
+        int ax = l & 3, ay = (l >> 2) & 3, az = l >> 4, f = lane;
+        bool in = (f == 0 && ax <= deep) || (f == 1 && ax >= 3 - deep) || (f == 2 && ay <= deep) ||
+                  (f == 3 && ay >= 3 - deep) || (f == 4 && az <= deep) || (f == 5 && az >= 3 - deep);
+        if (in) fm_mine |= 1ull << l;
+    }
 
-    for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
-        const uint32_t key = P.list[cur][e];
-        const int prob = (int)(key / (uint32_t)G.ntiles);
-        const int tile = (int)(key % (uint32_t)G.ntiles);
-        // clear this round's membership so a neighbour can queue the tile again
-        if (lane == 0) P.flags[cur][key] = 0;
-        const int tx = tile % G.ntx, ty = (tile / G.ntx) % G.nty, tz = tile / (G.ntx * G.nty);
-        const int gx = tx * 4 + lx, gy = ty * 4 + ly, gz = tz * 4 + lz;
-        const bool inside = gx < G.nx && gy < G.ny && gz < G.nz;
-        float *T = P.T + (long)prob * tsz;
-        const int cw = prob / P.nsrc_eff;
-        const float v = P.vel[(long)cw * tsz + ((long)tile << 6) + lane];
-        float t = inside ? T[((long)tile << 6) + lane] : EIK_INF;
-        const float t_in = t;
-        box[BX(lx, ly, lz)] = t;
-        // face halo: 6 faces x 2 layers x 16 nodes = 192 values, 3 per lane
-        for (int r = 0; r < 3; r++) {
-            int q = lane + 64 * r;
-            int face = q >> 5, layer = (q >> 4) & 1, a = q & 3, b = (q >> 2) & 3;
-            int ox, oy, oz;
-            int dep = -1 - layer;
-            switch (face) {
-                case 0: ox = dep; oy = a; oz = b; break;          // -x
-                case 1: ox = 3 - dep; oy = a; oz = b; break;      // +x  (4,5)
-                case 2: ox = a; oy = dep; oz = b; break;          // -y
-                case 3: ox = a; oy = 3 - dep; oz = b; break;      // +y
-                case 4: ox = a; oy = b; oz = dep; break;          // -z
-                default: ox = a; oy = b; oz = 3 - dep; break;     // +z
+    // Keys are visited lane-strided: block b, pass i covers keys
+    // b + stride*(64*i + lane), so every key has exactly one owner.
+    for (uint32_t base = blockIdx.x; base < nkeys; base += stride * 64u) {
+        const uint32_t mykey = base + stride * (uint32_t)lane;
+        uint8_t f = mykey < nkeys ? fcur[mykey] : 0;
+        bool due = false;
+        if (f) {
+            fcur[mykey] = 0;                       // consumed
+            uint32_t kc = kcur[mykey];
+            kcur[mykey] = 0xffffffffu;
+            const uint32_t prob = mykey / ntiles;
+            const float tau = banded ? (float)(round + 1) * P.band_dt[prob / (uint32_t)P.nsrc_eff]
+                                     : __builtin_huge_valf();
+            if (__uint_as_float(kc) <= tau || kc == 0xffffffffu) {
+                due = true;
+            } else {                               // not yet: carry to the next round
+                fnxt[mykey] = 1;
+                atomicMin(&knxt[mykey], kc);
+                *any_next = 1u;
             }
-            int nxg = tx * 4 + ox, nyg = ty * 4 + oy, nzg = tz * 4 + oz;
-            float hv = EIK_INF;
-            if (nxg >= 0 && nyg >= 0 && nzg >= 0 && nxg < G.nx && nyg < G.ny && nzg < G.nz)
-                hv = T[tl_index(G, nxg, nyg, nzg)];
-            box[BX(ox, oy, oz)] = hv;
         }
-        __syncthreads();
-        const float inv_v = 1.f / v;
-        const float s2 = inv_v * inv_v;
-        const bool solve = inside && v >= 2.220446e-11f;   // isOccupied(): velocity < COMP_MARGIN
-        const int me = BX(lx, ly, lz);
-        const int st[3] = {1, 8, 64};
-        bool capped = true;
-        for (int it = 0; it < 48; it++) {
-            float m1[3], m2[3];
-            for (int d = 0; d < 3; d++) {
-                float a1 = box[me - st[d]], b1 = box[me + st[d]];
-                if (ORDER == 0) {
-                    m1[d] = fminf(a1, b1);
-                } else {
-                    float a2 = box[me - 2 * st[d]], b2 = box[me + 2 * st[d]];
-                    // minT over the two sides, minT2 from the winning side if upwind
-                    float mt = EIK_INF, mt2 = EIK_INF;
-                    if (a1 < mt) { mt = a1; mt2 = (a2 <= a1) ? a2 : EIK_INF; }
-                    if (b1 < mt) { mt = b1; mt2 = (b2 <= b1) ? b2 : EIK_INF; }
-                    m1[d] = mt; m2[d] = mt2;
+        uint64_t todo = __ballot(due);
+        while (todo) {
+            const int src_lane = __ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1;
+            const uint32_t key = base + stride * (uint32_t)src_lane;
+            const uint32_t prob = key / ntiles;
+            const uint32_t tile = key - prob * ntiles;
+            const int tx = (int)(tile % (uint32_t)G.ntx);
+            const int tyz = (int)(tile / (uint32_t)G.ntx);
+            const int ty = tyz % G.nty, tz = tyz / G.nty;
+            const int gx = tx * 4 + lx, gy = ty * 4 + ly, gz = tz * 4 + lz;
+            const bool inside = gx < G.nx && gy < G.ny && gz < G.nz;
+            TT *T = Tall + (size_t)prob * ((size_t)ntiles << 6);
+            const uint32_t cw = prob / (uint32_t)P.nsrc_eff;
+            const float v = P.vel[(size_t)cw * ((size_t)ntiles << 6) + ((size_t)tile << 6) + lane];
+            TT t = inside ? T[((size_t)tile << 6) + lane] : TINF;
+            const TT t_in = t;
+            box[BX(lx, ly, lz)] = t;
+            // face halo: 6 faces x 2 layers x 16 nodes = 192 values, 3 per lane
+            for (int r = 0; r < 3; r++) {
+                int q = lane + 64 * r;
+                int face = q >> 5, layer = (q >> 4) & 1, a = q & 3, b = (q >> 2) & 3;
+                int ox, oy, oz;
+                int dep = -1 - layer;
+                switch (face) {
+                    case 0: ox = dep; oy = a; oz = b; break;          // -x
+                    case 1: ox = 3 - dep; oy = a; oz = b; break;      // +x
+                    case 2: ox = a; oy = dep; oz = b; break;          // -y
+                    case 3: ox = a; oy = 3 - dep; oz = b; break;      // +y
+                    case 4: ox = a; oy = b; oz = dep; break;          // -z
+                    default: ox = a; oy = b; oz = 3 - dep; break;     // +z
                 }
+                int nxg = tx * 4 + ox, nyg = ty * 4 + oy, nzg = tz * 4 + oz;
+                TT hv = TINF;
+                if ((ORDER != 0 || layer == 0) && nxg >= 0 && nyg >= 0 && nzg >= 0 && nxg < G.nx &&
+                    nyg < G.ny && nzg < G.nz)
+                    hv = T[tl_index(G, nxg, nyg, nzg)];
+                box[BX(ox, oy, oz)] = hv;
             }
-            float u = EIK_INF;
-            if (solve) u = ORDER == 0 ? update_order0(m1, t, h, ih2, inv_v, s2)
-                                      : update_order1(m1, m2, t, h, ih2, inv_v, s2);
-            bool better = u < t;
-            bool sig = better && (u < t - 1e-7f * t || t == EIK_INF);
-            if (better) t = u;
             __syncthreads();
-            box[me] = t;
-            __syncthreads();
-            if (__ballot(sig) == 0ull) { capped = false; break; }
-        }
-        const bool changed = t < t_in && (t < t_in - 1e-7f * t_in || t_in == EIK_INF);
-        if (inside && t < t_in) T[((long)tile << 6) + lane] = t;
-        // neighbours read a 1-deep (order 0) or 2-deep (order 1) halo
-        const int deep = ORDER == 0 ? 0 : 1;
-        unsigned long long bal = __ballot(changed);
-        if (capped && lane == 0 && flag_set(P.flags[nxt], key)) {
-            // local relaxation hit its cap: queue the tile again
-            uint32_t pos = atomicAdd(&P.counts[(round + 1) % 3], 1u);
-            P.list[nxt][pos] = key;
-        }
-        if (bal) {
-            uint64_t fm[6] = {0, 0, 0, 0, 0, 0};
-            // lane masks of the faces (bit l = lane l)
-            for (int l = 0; l < 64; l++) {
-                int ax = l & 3, ay = (l >> 2) & 3, az = l >> 4;
-                if (ax <= deep) fm[0] |= 1ull << l;
-                if (ax >= 3 - deep) fm[1] |= 1ull << l;
-                if (ay <= deep) fm[2] |= 1ull << l;
-                if (ay >= 3 - deep) fm[3] |= 1ull << l;
-                if (az <= deep) fm[4] |= 1ull << l;
-                if (az >= 3 - deep) fm[5] |= 1ull << l;
+            const float inv_v = 1.f / v;
+            const float s2 = inv_v * inv_v;
+            const bool solve = inside && v >= 2.220446e-11f;   // isOccupied(): velocity < COMP_MARGIN
+            const int me = BX(lx, ly, lz);
+            const int st[3] = {1, 8, 64};
+            bool capped = true;
+            for (int it = 0; it < 48; it++) {
+                TT n1[3], n2[3];
+                TT t0 = TINF;
+                for (int d = 0; d < 3; d++) {
+                    TT a1 = box[me - st[d]], b1 = box[me + st[d]];
+                    bool useb = b1 < a1;
+                    n1[d] = useb ? b1 : a1;
+                    if (ORDER != 0) {
+                        TT two = useb ? box[me + 2 * st[d]] : box[me - 2 * st[d]];
+                        n2[d] = (n1[d] < TINF && two <= n1[d]) ? two : TINF;
+                    }
+                    t0 = n1[d] < t0 ? n1[d] : t0;
+                }
+                TT u = TINF;
+                if (solve && t0 < t) {
+                    // times relative to t0 in FP32; excluded/absent neighbours stay INF
+                    float m1[3], m2[3];
+                    const float fself = t < TINF ? (float)(t - t0) : __builtin_huge_valf();
+                    for (int d = 0; d < 3; d++) {
+                        m1[d] = n1[d] < TINF ? (float)(n1[d] - t0) : __builtin_huge_valf();
+                        if (ORDER != 0) m2[d] = n2[d] < TINF ? (float)(n2[d] - t0) : __builtin_huge_valf();
+                    }
+                    float du = ORDER == 0 ? update_order0(m1, fself, h, ih2, inv_v, s2)
+                                          : update_order1(m1, m2, fself, h, ih2, inv_v, s2);
+                    if (du < __builtin_huge_valf()) u = t0 + (TT)du;
+                }
+                bool better = u < t;
+                bool sig = better && (u < t - thr * t || t == TINF);
+                if (better) t = u;
+                __syncthreads();
+                box[me] = t;
+                __syncthreads();
+                if (__ballot(sig) == 0ull) { capped = false; break; }
             }
-            if (lane < 6 && (bal & fm[lane])) {
-                int ax = tx + (lane == 0 ? -1 : lane == 1 ? 1 : 0);
-                int ay = ty + (lane == 2 ? -1 : lane == 3 ? 1 : 0);
-                int az = tz + (lane == 4 ? -1 : lane == 5 ? 1 : 0);
-                if (ax >= 0 && ay >= 0 && az >= 0 && ax < G.ntx && ay < G.nty && az < G.ntz) {
-                    uint32_t nkey = (uint32_t)prob * (uint32_t)G.ntiles + (uint32_t)((az * G.nty + ay) * G.ntx + ax);
-                    if (flag_set(P.flags[nxt], nkey)) {
-                        uint32_t pos = atomicAdd(&P.counts[(round + 1) % 3], 1u);
-                        P.list[nxt][pos] = nkey;
+            const bool changed = t < t_in && (t < t_in - thr * t_in || t_in == TINF);
+            if (inside && t < t_in) T[((size_t)tile << 6) + lane] = t;
+            const uint64_t bal = __ballot(changed);
+            if (bal || capped) {
+                // key of the tiles this visit activates: the smallest changed time
+                const float kmin = wave_min(changed ? (float)t : __builtin_huge_valf());
+                const float kown = wave_min(inside ? (float)t : __builtin_huge_valf());
+                bool act = false;
+                uint32_t nkey = 0;
+                uint32_t kbits = fbits(kmin);
+                if (lane < 6 && (bal & fm_mine)) {
+                    int ax = tx + (lane == 0 ? -1 : lane == 1 ? 1 : 0);
+                    int ay = ty + (lane == 2 ? -1 : lane == 3 ? 1 : 0);
+                    int az = tz + (lane == 4 ? -1 : lane == 5 ? 1 : 0);
+                    if (ax >= 0 && ay >= 0 && az >= 0 && ax < G.ntx && ay < G.nty && az < G.ntz) {
+                        nkey = prob * ntiles + (uint32_t)((az * G.nty + ay) * G.ntx + ax);
+                        act = true;
                     }
                 }
+                if (lane == 6 && capped) { nkey = key; act = true; kbits = fbits(kown); }
+                if (act) {
+                    fnxt[nkey] = 1;
+                    atomicMin(&knxt[nkey], kbits);
+                }
+                if (__ballot(act) != 0ull && lane == 0) *any_next = 1u;
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -332,13 +408,20 @@ int mct_launch_eik_init(const mct_eik_problem_set &P, const double *src_pts, int
 {
     long n = (long)P.nprob * P.G.ntiles * MCT_TILE_N;
     long nf = (long)P.nprob * P.G.ntiles;
-    hipLaunchKernelGGL(k_eik_clear, dim3(4096), dim3(256), 0, s, P.T, n);
+    if (P.t64) hipLaunchKernelGGL(k_eik_clear<double>, dim3(4096), dim3(256), 0, s, (double *)P.T, n);
+    else hipLaunchKernelGGL(k_eik_clear<float>, dim3(4096), dim3(256), 0, s, (float *)P.T, n);
     MCT_CHECK(hipGetLastError());
     MCT_CHECK(hipMemsetAsync(P.flags[0], 0, (size_t)((nf + 3) & ~3L), s));
     MCT_CHECK(hipMemsetAsync(P.flags[1], 0, (size_t)((nf + 3) & ~3L), s));
-    MCT_CHECK(hipMemsetAsync(P.counts, 0, 3 * sizeof(uint32_t), s));
-    hipLaunchKernelGGL(k_eik_init, dim3((P.nprob + 63) / 64), dim3(64), 0, s, P, src_pts, src_stride,
-                       src_chain_stride, xmin, ymin, zmin, lx, ly, lz);
+    MCT_CHECK(hipMemsetAsync(P.any, 0, 3 * sizeof(uint32_t), s));
+    MCT_CHECK(hipMemsetAsync(P.key[0], 0xff, sizeof(uint32_t) * nf, s));
+    MCT_CHECK(hipMemsetAsync(P.key[1], 0xff, sizeof(uint32_t) * nf, s));
+    if (P.t64)
+        hipLaunchKernelGGL(k_eik_init<double>, dim3((P.nprob + 63) / 64), dim3(64), 0, s, P, src_pts,
+                           src_stride, src_chain_stride, xmin, ymin, zmin, lx, ly, lz);
+    else
+        hipLaunchKernelGGL(k_eik_init<float>, dim3((P.nprob + 63) / 64), dim3(64), 0, s, P, src_pts,
+                           src_stride, src_chain_stride, xmin, ymin, zmin, lx, ly, lz);
     MCT_CHECK(hipGetLastError());
     return MCT_OK;
 }
@@ -346,10 +429,54 @@ int mct_launch_eik_init(const mct_eik_problem_set &P, const double *src_pts, int
 int mct_launch_eik_round(const mct_eik_problem_set &P, int round, float hx, float hy, float hz,
                          int grid_blocks, hipStream_t s)
 {
-    if (P.order == 1)
-        hipLaunchKernelGGL(k_eik_round<1>, dim3(grid_blocks), dim3(64), 0, s, P, round, hx, hy, hz);
-    else
-        hipLaunchKernelGGL(k_eik_round<0>, dim3(grid_blocks), dim3(64), 0, s, P, round, hx, hy, hz);
+    dim3 g(grid_blocks), b(64);
+    if (P.t64) {
+        if (P.order == 1) hipLaunchKernelGGL((k_eik_round<1, double>), g, b, 0, s, P, round, hx, hy, hz);
+        else hipLaunchKernelGGL((k_eik_round<0, double>), g, b, 0, s, P, round, hx, hy, hz);
+    } else {
+        if (P.order == 1) hipLaunchKernelGGL((k_eik_round<1, float>), g, b, 0, s, P, round, hx, hy, hz);
+        else hipLaunchKernelGGL((k_eik_round<0, float>), g, b, 0, s, P, round, hx, hy, hz);
+    }
     MCT_CHECK(hipGetLastError());
     return MCT_OK;
+}
+
+// Band time step per (chain, wave): the time the front needs to cross
+// `tiles_per_round` tiles at the fastest velocity of that model.
+__global__ __launch_bounds__(256) void k_band_dt(const float *vel, long tsz, float hmin, float tpr,
+                                                 float *band_dt)
+{
+    __shared__ float sm[4];
+    const int cw = blockIdx.x;
+    const float *V = vel + (long)cw * tsz;
+    float m = 0.f;
+    for (long i = threadIdx.x; i < tsz; i += blockDim.x) m = fmaxf(m, V[i]);
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float mm = fmaxf(fmaxf(sm[0], sm[1]), fmaxf(sm[2], sm[3]));
+        band_dt[cw] = mm > 0.f ? tpr * 4.f * hmin / mm : EIK_INF;
+    }
+}
+
+int mct_launch_band_dt(const float *vel, long tsz, int ncw, float hmin, float tiles_per_round,
+                       float *band_dt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_band_dt, dim3(ncw), dim3(256), 0, s, vel, tsz, hmin, tiles_per_round, band_dt);
+    MCT_CHECK(hipGetLastError());
+    return MCT_OK;
+}
+
+float mct_default_band_tiles()
+{
+    const char *e = getenv("MCT_BAND_TILES");
+    float v = e ? (float)atof(e) : 0.5f;
+    return v > 0.f ? v : 0.5f;
+}
+
+int mct_default_t64()
+{
+    const char *e = getenv("MCT_EIK_F32");
+    return (e && atoi(e) == 1) ? 0 : 1;
 }

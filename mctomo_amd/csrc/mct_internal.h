@@ -81,11 +81,14 @@ struct mct_eik_problem_set {
     int nsrc_eff;                  // sources traced per (chain, wave)
     int nwave;
     int order;
-    float *T;                      // (nprob, ntiles*64) arrival times
+    void *T;                       // (nprob, ntiles*64) arrival times, float or double
+    int t64;                       // 1: arrival times stored as double
     const float *vel;              // (nchain*nwave, ntiles*64)
-    uint32_t *list[2];             // active tile lists (prob*ntiles + tile)
-    uint8_t *flags[2];             // membership flags for list[0]/list[1]
-    uint32_t *counts;              // [3] rotating counters
+    uint8_t *flags[2];             // active-tile flags of the current / next round
+    uint32_t *key[2];              // per tile: smallest activating time (float bits)
+    uint32_t *any;                 // [3] rotating "some tile is active" flags
+    const float *band_dt;          // (nchain*nwave) band time step per round
+    int band_rounds;               // rounds with the band schedule, then plain FIM
     int32_t *status;               // (nprob) or NULL
 };
 
@@ -98,11 +101,27 @@ int mct_launch_eik_init(const mct_eik_problem_set &P, const double *src_pts, int
                         double lx, double ly, double lz, hipStream_t s);
 int mct_launch_eik_round(const mct_eik_problem_set &P, int round, float hx, float hy, float hz,
                          int grid_blocks, hipStream_t s);
+// band_dt[cw] = tiles_per_round * 4 * min(h) / max velocity of (chain, wave) cw
+int mct_launch_band_dt(const float *vel, long tsz, int ncw, float hmin, float tiles_per_round,
+                       float *band_dt, hipStream_t s);
+
+// Band schedule: the first band rounds advance tau by band_dt per round; then
+// tau is lifted (plain fast-iterative rounds) until no tile is active.
+static inline int mct_band_rounds(const mct_geom &G, float band_tiles)
+{
+    float bt = band_tiles > 0.05f ? band_tiles : 0.05f;
+    return (int)(6.0f * (G.ntx + G.nty + G.ntz) / bt) + 8;
+}
+// Tiles per round of the band schedule (environment MCT_BAND_TILES, else 0.5).
+float mct_default_band_tiles();
+// Arrival-time storage: double unless the environment sets MCT_EIK_F32=1.
+int mct_default_t64();
 
 // ---- picks + misfit -------------------------------------------------------------
 struct mct_pick_args {
     mct_geom G;
-    const float *T;                // (nprob, ntiles*64)
+    const void *T;                 // (nprob, ntiles*64)
+    int t64;                       // T holds doubles
     int nchain, nwave, nsrc_eff, npick;
     const double *pick_pts;        // points to pick at, (npick,3) shared or per chain
     int pick_stride_chain;         // 0 if shared across chains, else doubles per chain

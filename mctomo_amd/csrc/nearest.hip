@@ -90,9 +90,9 @@ __global__ __launch_bounds__(NEAR_BLOCK) void k_nearest_brute(mct_nearest_args a
         if (x < G.nx && y < G.ny && z < G.nz) {
             live = in_box(a, x, y, z) && pm_selects(a, c, x, y, z);
         } else if (a.vel_tiles) {
-            // padding lanes of edge tiles: keep them finite, they are never solved
+            // padding lanes of edge tiles: never solved; zero keeps them out of max(v)
             long wsz = (long)G.ntiles * MCT_TILE_N;
-            for (int w = 0; w < a.nwave; w++) a.vel_tiles[((long)c * a.nwave + w) * wsz + e] = 1.0f;
+            for (int w = 0; w < a.nwave; w++) a.vel_tiles[((long)c * a.nwave + w) * wsz + e] = 0.0f;
         }
     }
     const double qx = G.xmin + (double)x * G.dx;

@@ -42,7 +42,9 @@ __global__ void k_picks(mct_pick_args a)
         int ci = !(qd >= 1.0) ? 0 : (qd >= (double)dims[d] ? dims[d] - 1 : (int)qd);
         pc[d] = ci;
     }
-    const float *T = a.T + prob * (long)G.ntiles * MCT_TILE_N;
+    const long toff = prob * (long)G.ntiles * MCT_TILE_N;
+    const float *Tf = (const float *)a.T + toff;
+    const double *Td = (const double *)a.T + toff;
     double acc = 0.0;
     for (int v = 0; v < 8; v++) {
         int vc[3];
@@ -56,7 +58,8 @@ __global__ void k_picks(mct_pick_args a)
             double wj = fabs((double)(vc[d] - pc[d]) * leaf[d] - leaf[d] + s);
             w = w * wj / leaf[d];
         }
-        acc = acc + w * (double)T[tl_of(G, vc[0], vc[1], vc[2])];
+        long ti = tl_of(G, vc[0], vc[1], vc[2]);
+        acc = acc + w * (a.t64 ? Td[ti] : (double)Tf[ti]);
     }
     a.out[i] = acc;
 }
