@@ -40,6 +40,20 @@ int orc_kdtree_nearest(const double *points, int ncells, const double *q);
 int orc_brute_nearest(const double *points, int ncells, const double *q,
                       double *best_sd, int *n_at_best);
 
+/* ---- (b) dispersion --------------------------------------------------------- */
+/* surfdisp96 (surfmodes/surfdisp96.f:52-382), flat earth, fundamental mode.
+ * iwave 1 Love, 2 Rayleigh; igr 0 phase, 1 group; t(kmax) periods in s.
+ * cp/cg (kmax) as the Fortran writes them; returns ierr.  n_dltar (may be
+ * NULL) receives the number of period-equation evaluations. */
+int orc_surfdisp96(const float *thkm, const float *vpm, const float *vsm, const float *rhom,
+                   int nlayer, int iwave, int igr, int kmax, const double *t, double dphase,
+                   double *cp, double *cg, long *n_dltar);
+/* surfmodes' surfdisp96 branch (surfmodes/surfmodes.f90:79-95): FP64 layers
+ * cast to REAL*4, periods 1/freqs.  raylov 1 Rayleigh, 0 Love. */
+int orc_surfmodes(const double *thick, const double *vp, const double *vs, const double *rho,
+                  int nlayer, const double *freqs, int np, int raylov, int phase_group,
+                  double dc, double *phase, double *group, long *n_dltar);
+
 /* ---- (c) 3-D fast marching (SFMM) --------------------------------------- */
 /* Same signature and semantics as the reference C ABI fastMarching3D
  * (fastmarching/fastMarching.cpp:26-129). vel/tfield are Fortran (nz,ny,nx). */

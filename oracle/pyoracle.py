@@ -152,3 +152,32 @@ def body_misfit(phase_time, ttime, raystat, locations, sigdep=0, bnoise0=None, b
                                _p(tt), _p(rs), _p(loc), ctypes.c_int(sigdep), _p(b0), _p(b1),
                                ctypes.c_int(nrays), _p(srd), _p(sig), _p(out))
     return out, srd, sig, rc
+
+
+# ---------------------------------------------------------------- dispersion
+_F = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+
+
+def surfdisp96(thk, vp, vs, rho, periods, iwave, igr, dphase=1e-3, ref=False):
+    """surfdisp96 on a REAL*4 layer model.  Returns cp, cg, ierr, n_dltar
+    (n_dltar is None for the reference build)."""
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    thk, vp, vs, rho = f32(thk), f32(vp), f32(vs), f32(rho)
+    t = c64(periods)
+    n, k = thk.shape[0], t.shape[0]
+    cp = np.zeros(k)
+    cg = np.zeros(k)
+    if ref:
+        R = globals()["ref"]()
+        if R is None:
+            return None
+        ierr = ctypes.c_int()
+        R.ref_surfdisp96(ctypes.c_int(n), _p(thk), _p(vp), _p(vs), _p(rho), ctypes.c_int(iwave),
+                         ctypes.c_int(igr), ctypes.c_int(k), _p(t), ctypes.c_double(dphase),
+                         _p(cp), _p(cg), ctypes.byref(ierr))
+        return cp, cg, ierr.value, None
+    nd = ctypes.c_long()
+    ierr = lib().orc_surfdisp96(_p(thk), _p(vp), _p(vs), _p(rho), ctypes.c_int(n), ctypes.c_int(iwave),
+                                ctypes.c_int(igr), ctypes.c_int(k), _p(t), ctypes.c_double(dphase),
+                                _p(cp), _p(cg), ctypes.byref(nd))
+    return cp, cg, ierr, nd.value

@@ -31,4 +31,24 @@ contains
         call kdtree2_destroy(tree)
     end subroutine
 
+    ! surfdisp96 exactly as surfmodes calls it (surfmodes/surfmodes.f90:79-95):
+    ! REAL*4 layer model, flat earth, fundamental mode.
+    subroutine ref_surfdisp96(nlayer, thkm, vpm, vsm, rhom, iwave, igr, kmax, t, dphase, &
+                              cp, cg, ierr) bind(C, name="ref_surfdisp96")
+        integer(c_int), value :: nlayer, iwave, igr, kmax
+        real(c_float), intent(in) :: thkm(nlayer), vpm(nlayer), vsm(nlayer), rhom(nlayer)
+        real(c_double), intent(in) :: t(kmax)
+        real(c_double), value :: dphase
+        real(c_double), intent(out) :: cp(kmax), cg(kmax)
+        integer(c_int), intent(out) :: ierr
+        real(4) :: d4(200), a4(200), b4(200), r4(200)
+        real(8) :: tt(60)
+        integer :: ie
+        d4 = 0; a4 = 0; b4 = 0; r4 = 0; tt = 0
+        d4(1:nlayer) = thkm; a4(1:nlayer) = vpm; b4(1:nlayer) = vsm; r4(1:nlayer) = rhom
+        tt(1:kmax) = t
+        call surfdisp96(d4, a4, b4, r4, nlayer, 0, iwave, 1, igr, kmax, tt, dphase, cp, cg, ie)
+        ierr = ie
+    end subroutine
+
 end module mct_ref_driver

@@ -39,8 +39,38 @@ def kdtree_golden():
     np.savez_compressed(os.path.join(OUT, "kdtree2_nearest.npz"), n=len(sets), **arrays)
 
 
+def surfdisp_golden():
+    """surfdisp96 (surfmodes/surfdisp96.f) phase/group, Rayleigh/Love, on
+    seeded layer models: normal dispersion, low-velocity layers, failures."""
+    rng = np.random.default_rng(20261204)
+    arrays = {}
+    n = 0
+    for trial in range(40):
+        L = int(rng.integers(2, 22))
+        thk = rng.uniform(0.3, 3.0, L).astype(np.float32)
+        thk[-1] = 0
+        vs = rng.uniform(2.0, 4.6, L)
+        if trial % 2 == 0:
+            vs = np.sort(vs)
+        vs = vs.astype(np.float32)
+        vp = (vs * 1.73).astype(np.float32)
+        rho = (1.74 * vp.astype(np.float64) ** 0.25).astype(np.float32)
+        per = np.geomspace(1, 5, 10) if trial % 4 < 2 else np.geomspace(0.5, 10, 20)
+        for iw in (1, 2):
+            for igr in (0, 1):
+                cp, cg, ierr, _ = po.surfdisp96(thk, vp, vs, rho, per, iw, igr, ref=True)
+                arrays["m_%d" % n] = np.stack([thk, vp, vs, rho])
+                arrays["t_%d" % n] = per
+                arrays["opt_%d" % n] = np.array([iw, igr, ierr])
+                arrays["cp_%d" % n] = cp
+                arrays["cg_%d" % n] = cg
+                n += 1
+    np.savez_compressed(os.path.join(OUT, "surfdisp96.npz"), n=n, **arrays)
+
+
 if __name__ == "__main__":
     if po.ref() is None:
         sys.exit("oracle/_ref/libmctref.so is not built (make -C oracle ref)")
     kdtree_golden()
+    surfdisp_golden()
     print("wrote golden vectors to", OUT)
