@@ -54,6 +54,18 @@ int mct_voronoi_to_grid(const mct_grid *grid, const double *points, const double
                         int ncells, const double *box, const double *pm,
                         int32_t *sites_id, double *vp, double *vs, double *rho);
 
+/* ---- (b) dispersion ---------------------------------------------------------- */
+/* surfdisp96 (surfmodes/surfdisp96.f:52-382) as surfmodes calls it
+ * (surfmodes/surfmodes.f90:79-95: flat earth, fundamental mode), for nmod
+ * independent REAL*4 layer models at once.  layers (nmod,4,nl_max) holds
+ * thick, vp, vs, rho per model (last layer = half-space, thick 0); nlayer
+ * (nmod); periods (np) in s; iwave 1 Love, 2 Rayleigh; igr 0 phase, 1 group;
+ * dphase the search step (LIKE_SET%dPhaseVel).  phase/group (nmod,np) and
+ * ierr (nmod) receive cp, cg and ierr exactly as surfdisp96 returns them. */
+int mct_surfdisp96_batch(int nmod, int nl_max, const int32_t *nlayer, const float *layers,
+                         int np, const double *periods, int iwave, int igr, double dphase,
+                         double *phase, double *group, int32_t *ierr);
+
 /* ---- (c) 3-D eikonal: link-level drop-in ---------------------------------- */
 /* Same symbol, signature and semantics as the reference C ABI
  * (fastmarching/fastMarching.cpp:26-30; Fortran interface

@@ -60,7 +60,7 @@ def lib():
         for name in ("fastMarching3D", "mct_voronoi_to_grid", "mct_body_create", "mct_destroy",
                      "mct_body_likelihood", "mct_body_likelihood_batch_dev", "mct_sync",
                      "mct_last_timings", "mct_set_kernel_timing", "mct_set_tuning", "mct_dev_alloc", "mct_dev_free", "mct_copy_h2d",
-                     "mct_copy_d2h"):
+                     "mct_copy_d2h", "mct_surfdisp96_batch"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -169,6 +169,28 @@ def fast_marching_3d(vel, dx, dy, dz, xmin, ymin, zmin, src, rev, order, field=F
                               rev.shape[0], int(order), _p(tt), _p(tf))
     _check(rc, "fastMarching3D")
     return (tt, tf) if field else tt
+
+
+def surfdisp96_batch(models, periods, iwave, igr, dphase=1e-3):
+    """surfdisp96 as surfmodes calls it, for many layer models at once.
+    models: list of (thick, vp, vs, rho) arrays (cast to REAL*4).
+    iwave 1 Love, 2 Rayleigh; igr 0 phase, 1 group.
+    Returns phase (nmod,np), group (nmod,np), ierr (nmod,)."""
+    nmod = len(models)
+    nl = np.array([len(m[0]) for m in models], np.int32)
+    nlm = int(nl.max())
+    lay = np.zeros((nmod, 4, nlm), np.float32)
+    for i, m in enumerate(models):
+        for j in range(4):
+            lay[i, j, :nl[i]] = np.asarray(m[j], dtype=np.float32)
+    per = _f64(periods)
+    ph = np.zeros((nmod, per.shape[0]))
+    gr = np.zeros((nmod, per.shape[0]))
+    ie = np.zeros(nmod, np.int32)
+    rc = lib().mct_surfdisp96_batch(nmod, nlm, _p(nl), _p(lay), per.shape[0], _p(per), int(iwave),
+                                    int(igr), ctypes.c_double(dphase), _p(ph), _p(gr), _p(ie))
+    _check(rc, "mct_surfdisp96_batch")
+    return ph, gr, ie
 
 
 def kdtree_to_grid(grid, points, params, box=None, pm=None, vp=None, vs=None, rho=None):

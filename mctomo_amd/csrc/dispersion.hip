@@ -1,0 +1,686 @@
+// dispersion.hip -- per-column fundamental-mode Rayleigh/Love dispersion on gfx950.
+//
+// Replaces, for every grid column, convert_to_layer (src/likelihood_surf.F90
+// :523-629), the surfdisp96 branch of surfmodes (surfmodes/surfmodes.f90
+// :79-95) and surfdisp96 itself (surfmodes/surfdisp96.f:52-382).
+//
+// One wave64 per column.  The layer model lives in LDS.  The expensive part of
+// surfdisp96 is getsol's bracketing walk (:759-815): one period-equation
+// evaluation per dc step, ~10^3 steps for the first period.  Here the 64 lanes
+// evaluate 64 consecutive steps at once; every lane replays the walk's own
+// sequence of FP64 additions (c2 = c1 + dc, the clow floor and direction
+// rules) so its trial velocity is bit-identical to the reference's, and a
+// wave ballot finds the first sign change or exit condition.  The Neville /
+// bisection refinement (:903-1020) is sequential and runs in all lanes in
+// lockstep.  Arithmetic types follow the Fortran (REAL*4 where the reference
+// is implicitly typed real, FP64 in the period equations); FP contraction is
+// off.  The period equations call the device libm (sin, cos, exp, sqrt).
+#include "mct_internal.h"
+#include <float.h>
+
+#pragma clang fp contract(off)
+
+#define DSP_NMAX 100           // NMAX layers per column (src/likelihood_surf.F90:30)
+#define DSP_MAXP 60            // NP periods (surfmodes/surfdisp96.f:59)
+
+struct dsp_model {
+    int mmax, llw;
+    const float *d, *a, *b, *rho;
+};
+
+__device__ __forceinline__ double dsgn(double x) { return copysign(1.0, x); }
+
+__device__ static float d_gtsolh(float a, float b)
+{
+    float c = 0.95f * b;
+    for (int i = 0; i < 5; i++) {
+        float gamma = b / a;
+        float kappa = c / b;
+        float k2 = kappa * kappa;
+        float gk2 = (gamma * kappa) * (gamma * kappa);
+        float fac1 = sqrtf(1.0f - gk2);
+        float fac2 = sqrtf(1.0f - k2);
+        float fr = (2.0f - k2) * (2.0f - k2) - 4.0f * fac1 * fac2;
+        float frp = -(4.0f * (2.0f - k2) * kappa) + 4.0f * fac2 * gamma * gamma * kappa / fac1
+                    + 4.0f * fac1 * kappa / fac2;
+        frp = frp / b;
+        c = c - fr / frp;
+    }
+    return c;
+}
+
+// SH period equation, dltar1 (surfdisp96.f:1056-1115)
+__device__ static double d_dltar1(double wvno, double omega, const dsp_model &M)
+{
+    double beta1 = (double)M.b[M.mmax - 1];
+    double rho1 = (double)M.rho[M.mmax - 1];
+    double xkb = omega / beta1;
+    double wvnop = wvno + xkb;
+    double wvnom = fabs(wvno - xkb);
+    double rb = sqrt(wvnop * wvnom);
+    double e1 = rho1 * rb;
+    double e2 = 1.0 / (beta1 * beta1);
+    for (int m = M.mmax - 1; m >= M.llw; m--) {
+        beta1 = (double)M.b[m - 1];
+        rho1 = (double)M.rho[m - 1];
+        double xmu = rho1 * beta1 * beta1;
+        xkb = omega / beta1;
+        wvnop = wvno + xkb;
+        wvnom = fabs(wvno - xkb);
+        rb = sqrt(wvnop * wvnom);
+        double q = (double)M.d[m - 1] * rb;
+        double sinq, y, z, cosq;
+        if (wvno < xkb) {
+            sinq = sin(q);
+            y = sinq / rb;
+            z = -(rb * sinq);
+            cosq = cos(q);
+        } else if (wvno == xkb) {
+            cosq = 1.0;
+            y = (double)M.d[m - 1];
+            z = 0.0;
+        } else {
+            double fac = 0.0;
+            if (q < 16) fac = exp(-2.0 * q);
+            cosq = (1.0 + fac) * 0.5;
+            sinq = (1.0 - fac) * 0.5;
+            y = sinq / rb;
+            z = rb * sinq;
+        }
+        double e10 = e1 * cosq + e2 * xmu * z;
+        double e20 = e1 * y / xmu + e2 * cosq;
+        double xnor = fabs(e10), ynor = fabs(e20);
+        if (ynor > xnor) xnor = ynor;
+        if (xnor < 1.0e-40) xnor = 1.0;
+        e1 = e10 / xnor;
+        e2 = e20 / xnor;
+    }
+    return e1;
+}
+
+// P-SV period equation, dltar4 with var and dnka inlined (surfdisp96.f:1119-1414)
+__device__ static double d_dltar4(double wvno, double omga, const dsp_model &M)
+{
+    double omega = omga;
+    if (omega < 1.0e-4) omega = 1.0e-4;
+    const double one = 1.0, two = 2.0;
+    double wvno2 = wvno * wvno;
+    int mm = M.mmax - 1;
+    double xka = omega / (double)M.a[mm];
+    double xkb = omega / (double)M.b[mm];
+    double wvnop = wvno + xka, wvnom = fabs(wvno - xka);
+    double ra = sqrt(wvnop * wvnom);
+    wvnop = wvno + xkb;
+    wvnom = fabs(wvno - xkb);
+    double rb = sqrt(wvnop * wvnom);
+    double t = (double)M.b[mm] / omega;
+    double gammk = 2.0 * t * t;
+    double gam = gammk * wvno2;
+    double gamm1 = gam - 1.0;
+    double rho1 = (double)M.rho[mm];
+    double e0 = rho1 * rho1 * (gamm1 * gamm1 - gam * gammk * ra * rb);
+    double e1 = -(rho1 * ra);
+    double e2 = rho1 * (gamm1 - gammk * ra * rb);
+    double e3 = rho1 * rb;
+    double e4 = wvno2 - ra * rb;
+    for (int m = M.mmax - 1; m >= M.llw; m--) {
+        int i0 = m - 1;
+        xka = omega / (double)M.a[i0];
+        xkb = omega / (double)M.b[i0];
+        t = (double)M.b[i0] / omega;
+        gammk = 2.0 * t * t;
+        gam = gammk * wvno2;
+        wvnop = wvno + xka;
+        wvnom = fabs(wvno - xka);
+        ra = sqrt(wvnop * wvnom);
+        wvnop = wvno + xkb;
+        wvnom = fabs(wvno - xkb);
+        rb = sqrt(wvnop * wvnom);
+        double dpth = (double)M.d[i0];
+        rho1 = (double)M.rho[i0];
+        double p = ra * dpth, q = rb * dpth;
+        // ---- var
+        double pex = 0.0, sex = 0.0, sinp, w = 0, x = 0, cosp = 0, sinq, y = 0, z = 0, cosq = 0;
+        if (wvno < xka) {
+            sinp = sin(p);
+            w = sinp / ra;
+            x = -(ra * sinp);
+            cosp = cos(p);
+        } else if (wvno == xka) {
+            cosp = 1.0;
+            w = dpth;
+            x = 0.0;
+        } else if (wvno > xka) {
+            pex = p;
+            double fac = 0.0;
+            if (p < 16) fac = exp(-2.0 * p);
+            cosp = (1.0 + fac) * 0.5;
+            sinp = (1.0 - fac) * 0.5;
+            w = sinp / ra;
+            x = ra * sinp;
+        }
+        if (wvno < xkb) {
+            sinq = sin(q);
+            y = sinq / rb;
+            z = -(rb * sinq);
+            cosq = cos(q);
+        } else if (wvno == xkb) {
+            cosq = 1.0;
+            y = dpth;
+            z = 0.0;
+        } else if (wvno > xkb) {
+            sex = q;
+            double fac = 0.0;
+            if (q < 16) fac = exp(-2.0 * q);
+            cosq = (1.0 + fac) * 0.5;
+            sinq = (1.0 - fac) * 0.5;
+            y = sinq / rb;
+            z = rb * sinq;
+        }
+        double exa = pex + sex;
+        double a0 = 0.0;
+        if (exa < 60.0) a0 = exp(-exa);
+        double cpcq = cosp * cosq, cpy = cosp * y, cpz = cosp * z, cqw = cosq * w, cqx = cosq * x;
+        double xy = x * y, xz = x * z, wy = w * y, wz = w * z;
+        // ---- dnka
+        double g1 = gam - one;
+        double twgm1 = gam + g1;
+        double gmgmk = gam * gammk;
+        double gmgm1 = gam * g1;
+        double gm1sq = g1 * g1;
+        double rho2 = rho1 * rho1;
+        double a0pq = a0 - cpcq;
+        double c11 = cpcq - two * gmgm1 * a0pq - gmgmk * xz - wvno2 * gm1sq * wy;
+        double c12 = (wvno2 * cpy - cqx) / rho1;
+        double c13 = -((twgm1 * a0pq + gammk * xz + wvno2 * g1 * wy) / rho1);
+        double c14 = (cpz - wvno2 * cqw) / rho1;
+        double c15 = -((two * wvno2 * a0pq + xz + wvno2 * wvno2 * wy) / rho2);
+        double c21 = (gmgmk * cpz - gm1sq * cqw) * rho1;
+        double c22 = cpcq;
+        double c23 = gammk * cpz - g1 * cqw;
+        double c24 = -wz;
+        double c25 = c14;
+        double c41 = (gm1sq * cpy - gmgmk * cqx) * rho1;
+        double c42 = -xy;
+        double c43 = g1 * cpy - gammk * cqx;
+        double c44 = c22;
+        double c45 = c12;
+        double c51 = -((two * gmgmk * gm1sq * a0pq + gmgmk * gmgmk * xz + gm1sq * gm1sq * wy) * rho2);
+        double c52 = c41;
+        double c53 = -((gammk * g1 * twgm1 * a0pq + gam * gammk * gammk * xz + g1 * gm1sq * wy) * rho1);
+        double c54 = c21;
+        double c55 = c11;
+        double tt = -(two * wvno2);
+        double c31 = tt * c53;
+        double c32 = tt * c43;
+        double c33 = a0 + two * (cpcq - c11);
+        double c34 = tt * c23;
+        double c35 = tt * c13;
+        // ee(i) = sum_j e(j) ca(j,i), summed in j order from 0
+        double ee0 = 0.0, ee1 = 0.0, ee2 = 0.0, ee3 = 0.0, ee4 = 0.0;
+        ee0 = ee0 + e0 * c11; ee0 = ee0 + e1 * c21; ee0 = ee0 + e2 * c31; ee0 = ee0 + e3 * c41; ee0 = ee0 + e4 * c51;
+        ee1 = ee1 + e0 * c12; ee1 = ee1 + e1 * c22; ee1 = ee1 + e2 * c32; ee1 = ee1 + e3 * c42; ee1 = ee1 + e4 * c52;
+        ee2 = ee2 + e0 * c13; ee2 = ee2 + e1 * c23; ee2 = ee2 + e2 * c33; ee2 = ee2 + e3 * c43; ee2 = ee2 + e4 * c53;
+        ee3 = ee3 + e0 * c14; ee3 = ee3 + e1 * c24; ee3 = ee3 + e2 * c34; ee3 = ee3 + e3 * c44; ee3 = ee3 + e4 * c54;
+        ee4 = ee4 + e0 * c15; ee4 = ee4 + e1 * c25; ee4 = ee4 + e2 * c35; ee4 = ee4 + e3 * c45; ee4 = ee4 + e4 * c55;
+        // ---- normc
+        double t1 = 0.0;
+        if (fabs(ee0) > t1) t1 = fabs(ee0);
+        if (fabs(ee1) > t1) t1 = fabs(ee1);
+        if (fabs(ee2) > t1) t1 = fabs(ee2);
+        if (fabs(ee3) > t1) t1 = fabs(ee3);
+        if (fabs(ee4) > t1) t1 = fabs(ee4);
+        if (t1 < 1.0e-40) t1 = 1.0;
+        e0 = ee0 / t1; e1 = ee1 / t1; e2 = ee2 / t1; e3 = ee3 / t1; e4 = ee4 / t1;
+    }
+    if (M.llw != 1) {   // water layer on top
+        xka = omega / (double)M.a[0];
+        wvnop = wvno + xka;
+        wvnom = fabs(wvno - xka);
+        ra = sqrt(wvnop * wvnom);
+        double dpth = (double)M.d[0];
+        rho1 = (double)M.rho[0];
+        double p = ra * dpth;
+        double w = 0, cosp = 0;
+        if (wvno < xka) { double sp = sin(p); w = sp / ra; cosp = cos(p); }
+        else if (wvno == xka) { cosp = 1.0; w = dpth; }
+        else {
+            double fac = 0.0;
+            if (p < 16) fac = exp(-2.0 * p);
+            cosp = (1.0 + fac) * 0.5;
+            w = ((1.0 - fac) * 0.5) / ra;
+        }
+        double w0 = -(rho1 * w);
+        return cosp * e0 + w0 * e1;
+    }
+    return e0;
+}
+
+__device__ __forceinline__ double d_dltar(double wvno, double omega, int ifunc, const dsp_model &M)
+{
+    return ifunc == 1 ? d_dltar1(wvno, omega, M) : d_dltar4(wvno, omega, M);
+}
+
+// nevill (surfdisp96.f:903-1020), run identically by every lane of the wave
+__device__ static double d_nevill(double t, double c1, double c2, double del1, double del2, int ifunc,
+                                  const dsp_model &M, double twopi, int &ndl)
+{
+    double x[12], y[12];
+    double omega = twopi / t;
+    double c3 = 0.5 * (c1 + c2);
+    double del3 = d_dltar(omega / c3, omega, ifunc, M);
+    ndl++;
+    int nev = 1, nctrl = 1, m = 1;
+    for (;;) {
+        nctrl = nctrl + 1;
+        if (nctrl >= 100) break;
+        if (c3 < fmin(c1, c2) || c3 > fmax(c1, c2)) {
+            nev = 0;
+            c3 = 0.5 * (c1 + c2);
+            del3 = d_dltar(omega / c3, omega, ifunc, M);
+            ndl++;
+        }
+        double s13 = del1 - del3;
+        double s32 = del3 - del2;
+        if (dsgn(del3) * dsgn(del1) < 0.0) { c2 = c3; del2 = del3; }
+        else { c1 = c3; del1 = del3; }
+        if (fabs(c1 - c2) <= 1.0e-6 * c1) break;
+        if (dsgn(s13) != dsgn(s32)) nev = 0;
+        double ss1 = fabs(del1);
+        double s1 = (double)0.01f * ss1;
+        double ss2 = fabs(del2);
+        double s2 = (double)0.01f * ss2;
+        bool halve = s1 > ss2 || s2 > ss1 || nev == 0;
+        if (!halve) {
+            if (nev == 2) {
+                x[m + 1] = c3;
+                y[m + 1] = del3;
+            } else {
+                x[1] = c1; y[1] = del1;
+                x[2] = c2; y[2] = del2;
+                m = 1;
+            }
+            bool bad = false;
+            for (int kk = 1; kk <= m; kk++) {
+                int j = m - kk + 1;
+                double denom = y[m + 1] - y[j];
+                if (fabs(denom) < 1.0e-10 * fabs(y[m + 1])) { bad = true; break; }
+                x[j] = (-(y[j] * x[j + 1]) + y[m + 1] * x[j]) / denom;
+            }
+            if (!bad) {
+                c3 = x[1];
+                del3 = d_dltar(omega / c3, omega, ifunc, M);
+                ndl++;
+                nev = 2;
+                m = m + 1;
+                if (m > 10) m = 10;
+                continue;
+            }
+        }
+        c3 = 0.5 * (c1 + c2);
+        del3 = d_dltar(omega / c3, omega, ifunc, M);
+        ndl++;
+        nev = 1;
+        m = 1;
+    }
+    return c3;
+}
+
+// getsol (surfdisp96.f:734-827) with the bracketing walk spread over the wave.
+__device__ static int d_getsol(double t1, double &c1io, double clow, double dc, double cm, float betmx,
+                               int ifunc, int ifirst, double &del1st, const dsp_model &M, int &ndl)
+{
+    const int lane = threadIdx.x & 63;
+    const double twopi = 2.0 * 3.141592653589793;
+    double c1 = c1io;
+    double omega = twopi / t1;
+    double del1 = d_dltar(omega / c1, omega, ifunc, M);
+    ndl++;
+    if (ifirst == 1) del1st = del1;
+    double plmn = dsgn(del1st) * dsgn(del1);
+    int idir = (ifirst == 1) ? 1 : (plmn >= 0.0 ? 1 : -1);
+    const double cmax = (double)betmx + dc;
+    for (int pass = 0; pass < 4096; pass++) {
+        // replay the walk up to this lane's step: c2 sequence and c1 before it
+        double cc = c1, cprev = c1, cand = c1;
+        int dd = idir;
+        for (int s = 0; s <= lane; s++) {
+            double c2;
+            for (;;) {
+                c2 = dd > 0 ? cc + dc : cc - dc;
+                if (c2 <= clow) { dd = 1; cc = clow; continue; }
+                break;
+            }
+            cprev = cc;
+            cand = c2;
+            cc = c2;
+        }
+        double omg = twopi / t1;
+        double del2 = d_dltar(omg / cand, omg, ifunc, M);
+        double delprev = __shfl_up(del2, 1, 64);
+        if (lane == 0) delprev = del1;
+        bool change = dsgn(delprev) != dsgn(del2);
+        bool fail = !change && (cand < cm || cand >= cmax);
+        unsigned long long ev = __ballot(change || fail);
+        ndl += 64;
+        if (ev) {
+            int e = __ffsll(ev) - 1;
+            bool ch = __shfl(change ? 1 : 0, e, 64) != 0;
+            double ce = __shfl(cand, e, 64);
+            if (!ch) { c1io = ce; return -1; }
+            double cb = __shfl(cprev, e, 64);
+            double d2 = __shfl(del2, e, 64);
+            double d1 = __shfl(delprev, e, 64);
+            double cn = d_nevill(t1, cb, ce, d1, d2, ifunc, M, twopi, ndl);
+            c1io = cn;
+            if (cn > (double)betmx) return -1;
+            return 1;
+        }
+        // no event in these 64 steps: continue from the last one
+        c1 = __shfl(cand, 63, 64);
+        del1 = __shfl(del2, 63, 64);
+        idir = __shfl(dd, 63, 64);
+    }
+    c1io = c1;
+    return -1;
+}
+
+struct mct_dsp_args {
+    int ncol;                     // columns per chain (nx*ny, Fortran order j + ny*i)
+    int nz;
+    int nchain;
+    const double *vs;             // (nchain, nx, ny, nz) Fortran (nz,ny,nx) per chain
+    double dz, scaling;           // grid%dz (scaled) and grid%scaling
+    const double *periods;        // (np) periods in s (1/freqs)
+    int np;
+    int iwave;                    // 1 Love, 2 Rayleigh
+    int igr;                      // 0 phase, 1 group
+    double dphase;
+    double *phase, *group;        // (nchain, ncol, np)
+    int32_t *ierr;                // (nchain, ncol)
+    int32_t *bad_model;           // (nchain): check_model failed
+    int32_t *ndltar;              // (nchain, ncol) or NULL: period-equation evaluations
+};
+
+// One wave per column: check_model, convert_to_layer, surfdisp96.
+__global__ __launch_bounds__(64) void k_dispersion(mct_dsp_args A)
+{
+    __shared__ float sd[DSP_NMAX], sa[DSP_NMAX], sb[DSP_NMAX], sr[DSP_NMAX];
+    __shared__ int s_nl;
+    const int col = blockIdx.x, c = blockIdx.y, lane = threadIdx.x;
+    const double *vs = A.vs + ((long)c * A.ncol + col) * A.nz;
+    const double POISSON = (double)1.730f;     // src/utils.f90:97-112 (vs2vp_3d)
+    const double EPS = (double)1.0e-10f;       // src/likelihood_surf.F90:37
+    // check_model (src/likelihood_surf.F90:631-646): a deeper vs below the top one
+    bool below = false;
+    for (int k = 1 + lane; k < A.nz; k += 64) below |= vs[k] < vs[0];
+    if (__ballot(below) != 0ull && lane == 0) A.bad_model[c] = 1;
+    // convert_to_layer (src/likelihood_surf.F90:523-629), no water layer; vp and
+    // rho from vs2vp_3d / vp2rho_3d (src/likelihood.f90:75-76)
+    if (lane == 0) {
+        int nl = 0, last_k = 1;
+        double last_vs = vs[0];
+        for (int k = 2; k <= A.nz; k++) {
+            if (fabs(vs[k - 1] - last_vs) > EPS) {
+                double vpk = last_vs * POISSON;
+                sa[nl] = (float)vpk;
+                sb[nl] = (float)last_vs;
+                sr[nl] = (float)((double)1.74f * pow(vpk, 0.25));
+                sd[nl] = (float)(((double)(k - last_k) * A.dz) / A.scaling);
+                nl++;
+                last_vs = vs[k - 1];
+                last_k = k;
+            }
+        }
+        if (last_k != A.nz) {
+            double vpk = vs[A.nz - 1] * POISSON;
+            sa[nl] = (float)vpk;
+            sb[nl] = (float)vs[A.nz - 1];
+            sr[nl] = (float)((double)1.74f * pow(vpk, 0.25));
+            sd[nl] = 0.0f;
+        } else {
+            double vpk = vs[A.nz - 1] * POISSON;
+            sa[nl] = (float)vpk;
+            sb[nl] = (float)vs[A.nz - 1];
+            sr[nl] = (float)((double)1.74f * pow(vpk, 0.25));
+            sd[nl] = 0.0f;
+        }
+        s_nl = nl + 1;
+    }
+    __syncthreads();
+    dsp_model M;
+    M.mmax = s_nl;
+    M.d = sd; M.a = sa; M.b = sb; M.rho = sr;
+    M.llw = (sb[0] <= 0.0f) ? 2 : 1;
+    // surfdisp96 (flat earth, fundamental mode)
+    double *cp = A.phase + ((long)c * A.ncol + col) * A.np;
+    double *cg = A.group + ((long)c * A.ncol + col) * A.np;
+    int ierr = 0, ndl = 0;
+    const float sone = 1.500f, h = 0.005f;
+    const float ddc = (float)A.dphase;
+    const double one = 1.0e-2;
+    int jmn = 1, jsol = 1;
+    float betmx = -1.e20f, betmn = 1.e20f;
+    for (int i = 1; i <= M.mmax; i++) {
+        float bi = sb[i - 1], ai = sa[i - 1];
+        if (bi > 0.01f && bi < betmn) { betmn = bi; jmn = i; jsol = 1; }
+        else if (bi <= 0.01f && ai < betmn) { betmn = ai; jmn = i; jsol = 0; }
+        if (bi > betmx) betmx = bi;
+    }
+    const int ifunc = A.iwave;
+    const double onea = (double)sone;
+    float cc1 = jsol == 0 ? betmn : d_gtsolh(sa[jmn - 1], sb[jmn - 1]);
+    cc1 = 0.95f * cc1;
+    cc1 = 0.90f * cc1;
+    const double cc = (double)cc1;
+    const double dc = fabs((double)ddc);
+    const double cm = cc;
+    double cvec[DSP_MAXP], cbv[DSP_MAXP], del1st = 0.0;
+    for (int i = 0; i < A.np; i++) { cvec[i] = 0.0; cbv[i] = 0.0; }
+    int k;
+    for (k = 1; k <= A.np; k++) {
+        double t1 = A.periods[k - 1];
+        float t1a, t1b = 0.0f;
+        if (A.igr > 0) {
+            t1a = (float)(t1 / (double)(1.0f + h));
+            t1b = (float)(t1 / (double)(1.0f - h));
+            t1 = (double)t1a;
+        } else {
+            t1a = (float)t1;
+        }
+        int ifirst;
+        double c1, clow;
+        if (k == 1) {
+            c1 = cc; clow = cc; ifirst = 1;
+        } else {
+            ifirst = 0;
+            c1 = cc;
+            for (int previd = k - 1; previd >= 1; previd--)
+                if (cvec[previd - 1] > 0) { c1 = cvec[previd - 1] - onea * dc; break; }
+            clow = cm;
+        }
+        int iret = d_getsol(t1, c1, clow, dc, cm, betmx, ifunc, ifirst, del1st, M, ndl);
+        if (iret == -1) break;
+        cvec[k - 1] = c1;
+        if (A.igr > 0) {
+            t1 = (double)t1b;
+            ifirst = 0;
+            clow = cbv[k - 1] + one * dc;
+            c1 = c1 - onea * dc;
+            iret = d_getsol(t1, c1, clow, dc, cm, betmx, ifunc, ifirst, del1st, M, ndl);
+            if (iret == -1) { c1 = cvec[k - 1]; ierr = 1; }
+            cbv[k - 1] = c1;
+        } else {
+            c1 = 0.0;
+        }
+        float cc0 = (float)cvec[k - 1];
+        float cc1k = (float)c1;
+        if (lane == 0) {
+            if (A.igr == 0) {
+                cp[k - 1] = (double)cc0;
+                cg[k - 1] = 100.0;
+            } else {
+                float gvel = (1.0f / t1a - 1.0f / t1b) / (1.0f / (t1a * cc0) - 1.0f / (t1b * cc1k));
+                cg[k - 1] = (double)gvel;
+                cp[k - 1] = (double)cc0;
+                if (gvel < 0 || cvec[k - 1] == 0) ierr = 1;
+            }
+        }
+    }
+    if (k <= A.np) {   // no root at period k: the rest keep cp=100, cg=0 (labels 1700/1750)
+        ierr = 1;
+        if (lane == 0)
+            for (int i = k; i <= A.np; i++) { cp[i - 1] = 100.0; cg[i - 1] = 0.0; }
+    }
+    if (lane == 0) {
+        A.ierr[(long)c * A.ncol + col] = ierr;
+        if (A.ndltar) A.ndltar[(long)c * A.ncol + col] = ndl;
+    }
+}
+
+int mct_launch_dispersion(const mct_dsp_args &a, hipStream_t s)
+{
+    if (a.np < 1 || a.np > DSP_MAXP || a.nz < 1 || a.nz > DSP_NMAX) return MCT_ERR_ARG;
+    MCT_CHECK(hipMemsetAsync(a.bad_model, 0, sizeof(int32_t) * a.nchain, s));
+    hipLaunchKernelGGL(k_dispersion, dim3(a.ncol, a.nchain), dim3(64), 0, s, a);
+    MCT_CHECK(hipGetLastError());
+    return MCT_OK;
+}
+
+// ---- C ABI: dispersion of explicit layer models (surfmodes' surfdisp96 call) ----
+struct mct_layer_args {
+    int nmod, nl_max, np;
+    const int32_t *nlayer;        // (nmod)
+    const float *layers;          // (nmod, 4, nl_max): thick, vp, vs, rho (REAL*4)
+    const double *periods;        // (np)
+    int iwave, igr;
+    double dphase;
+    double *phase, *group;        // (nmod, np)
+    int32_t *ierr;                // (nmod)
+};
+
+__global__ __launch_bounds__(64) void k_dispersion_layers(mct_layer_args A)
+{
+    __shared__ float sd[DSP_NMAX * 2], sa[DSP_NMAX * 2], sb[DSP_NMAX * 2], sr[DSP_NMAX * 2];
+    const int mod = blockIdx.x, lane = threadIdx.x;
+    const int nl = A.nlayer[mod];
+    const float *L = A.layers + (long)mod * 4 * A.nl_max;
+    for (int i = lane; i < nl; i += 64) {
+        sd[i] = L[i]; sa[i] = L[A.nl_max + i]; sb[i] = L[2 * A.nl_max + i]; sr[i] = L[3 * A.nl_max + i];
+    }
+    __syncthreads();
+    dsp_model M;
+    M.mmax = nl;
+    M.d = sd; M.a = sa; M.b = sb; M.rho = sr;
+    M.llw = (sb[0] <= 0.0f) ? 2 : 1;
+    double *cp = A.phase + (long)mod * A.np, *cg = A.group + (long)mod * A.np;
+    int ierr = 0, ndl = 0;
+    const float sone = 1.500f, h = 0.005f, ddc = (float)A.dphase;
+    const double one = 1.0e-2, onea = (double)sone;
+    int jmn = 1, jsol = 1;
+    float betmx = -1.e20f, betmn = 1.e20f;
+    for (int i = 1; i <= nl; i++) {
+        float bi = sb[i - 1], ai = sa[i - 1];
+        if (bi > 0.01f && bi < betmn) { betmn = bi; jmn = i; jsol = 1; }
+        else if (bi <= 0.01f && ai < betmn) { betmn = ai; jmn = i; jsol = 0; }
+        if (bi > betmx) betmx = bi;
+    }
+    float cc1 = jsol == 0 ? betmn : d_gtsolh(sa[jmn - 1], sb[jmn - 1]);
+    cc1 = 0.95f * cc1;
+    cc1 = 0.90f * cc1;
+    const double cc = (double)cc1, dc = fabs((double)ddc), cm = cc;
+    double cvec[DSP_MAXP], cbv[DSP_MAXP], del1st = 0.0;
+    for (int i = 0; i < A.np; i++) { cvec[i] = 0.0; cbv[i] = 0.0; }
+    if (lane == 0) for (int i = 0; i < A.np; i++) { cp[i] = 100.0; cg[i] = 100.0; }
+    int k;
+    for (k = 1; k <= A.np; k++) {
+        double t1 = A.periods[k - 1];
+        float t1a, t1b = 0.0f;
+        if (A.igr > 0) {
+            t1a = (float)(t1 / (double)(1.0f + h));
+            t1b = (float)(t1 / (double)(1.0f - h));
+            t1 = (double)t1a;
+        } else {
+            t1a = (float)t1;
+        }
+        int ifirst;
+        double c1, clow;
+        if (k == 1) { c1 = cc; clow = cc; ifirst = 1; }
+        else {
+            ifirst = 0;
+            c1 = cc;
+            for (int previd = k - 1; previd >= 1; previd--)
+                if (cvec[previd - 1] > 0) { c1 = cvec[previd - 1] - onea * dc; break; }
+            clow = cm;
+        }
+        int iret = d_getsol(t1, c1, clow, dc, cm, betmx, A.iwave, ifirst, del1st, M, ndl);
+        if (iret == -1) break;
+        cvec[k - 1] = c1;
+        if (A.igr > 0) {
+            t1 = (double)t1b;
+            clow = cbv[k - 1] + one * dc;
+            c1 = c1 - onea * dc;
+            iret = d_getsol(t1, c1, clow, dc, cm, betmx, A.iwave, 0, del1st, M, ndl);
+            if (iret == -1) { c1 = cvec[k - 1]; ierr = 1; }
+            cbv[k - 1] = c1;
+        } else {
+            c1 = 0.0;
+        }
+        float cc0 = (float)cvec[k - 1], cc1k = (float)c1;
+        if (lane == 0) {
+            if (A.igr == 0) cp[k - 1] = (double)cc0;
+            else {
+                float gvel = (1.0f / t1a - 1.0f / t1b) / (1.0f / (t1a * cc0) - 1.0f / (t1b * cc1k));
+                cg[k - 1] = (double)gvel;
+                cp[k - 1] = (double)cc0;
+                if (gvel < 0 || cvec[k - 1] == 0) ierr = 1;
+            }
+        }
+    }
+    if (k <= A.np) {
+        ierr = 1;
+        if (lane == 0) for (int i = k; i <= A.np; i++) cg[i - 1] = 0.0;
+    }
+    if (lane == 0) A.ierr[mod] = ierr;
+}
+
+extern "C" int mct_surfdisp96_batch(int nmod, int nl_max, const int32_t *nlayer, const float *layers,
+                                    int np, const double *periods, int iwave, int igr, double dphase,
+                                    double *phase, double *group, int32_t *ierr)
+{
+    if (nmod < 1 || nl_max < 1 || nl_max > 2 * DSP_NMAX || np < 1 || np > DSP_MAXP || !nlayer ||
+        !layers || !periods || !phase || !group || !ierr || (iwave != 1 && iwave != 2))
+        return MCT_ERR_ARG;
+    for (int i = 0; i < nmod; i++)
+        if (nlayer[i] < 1 || nlayer[i] > nl_max) return MCT_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MCT_ERR_HIP;
+    void *dn = nullptr, *dl = nullptr, *dp = nullptr, *dph = nullptr, *dg = nullptr, *de = nullptr;
+    int rc = MCT_OK;
+    size_t sl = sizeof(float) * 4 * (size_t)nl_max * nmod;
+    if (hipMalloc(&dn, sizeof(int32_t) * nmod) != hipSuccess || hipMalloc(&dl, sl) != hipSuccess ||
+        hipMalloc(&dp, sizeof(double) * np) != hipSuccess ||
+        hipMalloc(&dph, sizeof(double) * np * (size_t)nmod) != hipSuccess ||
+        hipMalloc(&dg, sizeof(double) * np * (size_t)nmod) != hipSuccess ||
+        hipMalloc(&de, sizeof(int32_t) * nmod) != hipSuccess)
+        rc = MCT_ERR_NOMEM;
+    if (!rc && (hipMemcpy(dn, nlayer, sizeof(int32_t) * nmod, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(dl, layers, sl, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(dp, periods, sizeof(double) * np, hipMemcpyHostToDevice) != hipSuccess))
+        rc = MCT_ERR_HIP;
+    if (!rc) {
+        mct_layer_args a;
+        a.nmod = nmod; a.nl_max = nl_max; a.np = np; a.nlayer = (const int32_t *)dn;
+        a.layers = (const float *)dl; a.periods = (const double *)dp; a.iwave = iwave; a.igr = igr;
+        a.dphase = dphase; a.phase = (double *)dph; a.group = (double *)dg; a.ierr = (int32_t *)de;
+        hipLaunchKernelGGL(k_dispersion_layers, dim3(nmod), dim3(64), 0, 0, a);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = MCT_ERR_HIP;
+    }
+    if (!rc && (hipMemcpy(phase, dph, sizeof(double) * np * (size_t)nmod, hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(group, dg, sizeof(double) * np * (size_t)nmod, hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(ierr, de, sizeof(int32_t) * nmod, hipMemcpyDeviceToHost) != hipSuccess))
+        rc = MCT_ERR_HIP;
+    void *bufs[] = {dn, dl, dp, dph, dg, de};
+    for (void *b : bufs) if (b) (void)hipFree(b);
+    return rc;
+}

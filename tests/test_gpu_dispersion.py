@@ -1,0 +1,67 @@
+"""GPU dispersion (surfdisp96 on gfx950) against the oracle and the reference's
+golden vectors.  Gate: phase and group velocities within 1e-6 km/s, same ierr.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import mctomo_amd as mct
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "surfdisp96.npz")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_golden_vectors():
+    g = np.load(GOLD)
+    n = int(g["n"])
+    groups = {}
+    for i in range(n):
+        iw, igr, _ = (int(v) for v in g["opt_%d" % i])
+        key = (iw, igr, len(g["t_%d" % i]))
+        groups.setdefault(key, []).append(i)
+    worst = 0.0
+    for (iw, igr, npp), idx in groups.items():
+        say("golden", iw, igr, npp, len(idx))
+        models = [tuple(g["m_%d" % i]) for i in idx]
+        ph, gr, ie = mct.surfdisp96_batch(models, g["t_%d" % idx[0]], iw, igr)
+        for r, i in enumerate(idx):
+            assert ie[r] == int(g["opt_%d" % i][2])
+            worst = max(worst, np.max(np.abs(ph[r] - g["cp_%d" % i])),
+                        np.max(np.abs(gr[r] - g["cg_%d" % i])))
+    say("golden worst abs diff km/s", worst)
+    assert worst <= 1e-6
+
+
+@pytest.mark.parametrize("iw,igr", [(2, 0), (1, 0), (2, 1), (1, 1)])
+def test_random_models_vs_oracle(iw, igr):
+    rng = np.random.default_rng(10 + iw * 2 + igr)
+    models = []
+    for t in range(300):
+        L = int(rng.integers(2, 25))
+        thk = rng.uniform(0.2, 3.0, L); thk[-1] = 0
+        vs = rng.uniform(1.5, 4.8, L)
+        if t % 2 == 0:
+            vs = np.sort(vs)
+        vp = 1.73 * vs
+        models.append((thk, vp, vs, 1.74 * vp ** 0.25))
+    per = np.geomspace(0.5, 10, 20)
+    say("random", iw, igr)
+    ph, gr, ie = mct.surfdisp96_batch(models, per, iw, igr)
+    nexact = 0
+    worst = 0.0
+    for r, m in enumerate(models):
+        cp, cg, e, _ = po.surfdisp96(*m, per, iw, igr)
+        assert ie[r] == e, (r, ie[r], e)
+        d = max(np.max(np.abs(ph[r] - cp)), np.max(np.abs(gr[r] - cg)))
+        worst = max(worst, d)
+        nexact += d == 0
+    say("random", iw, igr, "exact models", nexact, "/", len(models), "worst", worst)
+    assert worst <= 1e-6
