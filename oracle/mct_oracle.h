@@ -54,6 +54,20 @@ int orc_surfmodes(const double *thick, const double *vp, const double *vs, const
                   int nlayer, const double *freqs, int np, int raylov, int phase_group,
                   double dc, double *phase, double *group, long *n_dltar);
 
+/* ---- (c) 2-D fast marching (fm2d, phase times) ----------------------------- */
+/* modrays' phase-time path for one period (fm2d/fm2dray_cartesian.f90:67-480):
+ * velv(nvz+2,nvx+2) padded phase-velocity grid (Fortran order), gdx/gdz
+ * B-spline dicing, asgr/sgdl/sgs source refinement, fom order, snb band.
+ * srs (nsrc*nrc) 1 = pair used (receiver fastest); ttime(nrc,nsrc) is written
+ * for used pairs only.  tfield (nsrc, nnx, nnz) may be NULL.
+ * Returns 0, -1 source outside the grid, -2 receiver outside (the reference
+ * STOPs). */
+int orc_fm2d(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvzd,
+             const double *velv, int gdx, int gdz, int asgr, int sgdl, int sgs, int fom,
+             double snb, int nsrc, const double *scx, const double *scz, int nrc,
+             const double *rcx, const double *rcz, const int32_t *srs, double *ttime,
+             double *tfield);
+
 /* ---- (c) 3-D fast marching (SFMM) --------------------------------------- */
 /* Same signature and semantics as the reference C ABI fastMarching3D
  * (fastmarching/fastMarching.cpp:26-129). vel/tfield are Fortran (nz,ny,nx). */

@@ -181,3 +181,38 @@ def surfdisp96(thk, vp, vs, rho, periods, iwave, igr, dphase=1e-3, ref=False):
                                 ctypes.c_int(igr), ctypes.c_int(k), _p(t), ctypes.c_double(dphase),
                                 _p(cp), _p(cg), ctypes.byref(nd))
     return cp, cg, ierr, nd.value
+
+
+# ---------------------------------------------------------------- 2-D eikonal
+def fm2d(velv, goxd, gozd, dvxd, dvzd, scx, scz, rcx, rcz, srs=None, gdx=1, gdz=1, asgr=1,
+         sgdl=4, sgs=8, fom=1, snb=0.5, ttime=None, field=False, ref=False):
+    """modrays phase times for one period.  velv: (nvx+2, nvy+2) C array
+    (= Fortran velv(nvz+2,nvx+2)).  Returns ttime (nsrc, nrc) [, field]."""
+    velv = c64(velv)
+    nvx, nvz = velv.shape[0] - 2, velv.shape[1] - 2
+    scx, scz, rcx, rcz = c64(scx), c64(scz), c64(rcx), c64(rcz)
+    nsrc, nrc = scx.shape[0], rcx.shape[0]
+    if srs is None:
+        srs = np.ones(nsrc * nrc, np.int32)
+    srs = np.ascontiguousarray(srs, dtype=np.int32).reshape(-1)
+    tt = np.zeros((nsrc, nrc)) if ttime is None else c64(ttime).copy()
+    if ref:
+        R = globals()["ref"]()
+        if R is None:
+            return None
+        srs2 = np.ascontiguousarray(np.concatenate([srs, srs]), dtype=np.int32)
+        cr = ctypes.c_int()
+        R.ref_modrays(nsrc, _p(scx), _p(scz), nrc, _p(rcx), _p(rcz), _p(srs2), nvx, nvz,
+                      ctypes.c_double(goxd), ctypes.c_double(gozd), ctypes.c_double(dvxd),
+                      ctypes.c_double(dvzd), _p(velv), gdx, gdz, asgr, sgdl, sgs, fom,
+                      ctypes.c_double(snb), _p(tt), ctypes.byref(cr))
+        return tt
+    nnx, nnz = (nvx - 1) * gdx + 1, (nvz - 1) * gdz + 1
+    tf = np.zeros((nsrc, nnx, nnz)) if field else None
+    rc = lib().orc_fm2d(nvx, nvz, ctypes.c_double(goxd), ctypes.c_double(gozd),
+                        ctypes.c_double(dvxd), ctypes.c_double(dvzd), _p(velv), gdx, gdz, asgr,
+                        sgdl, sgs, fom, ctypes.c_double(snb), nsrc, _p(scx), _p(scz), nrc,
+                        _p(rcx), _p(rcz), _p(srs), _p(tt), _p(tf))
+    if rc:
+        raise RuntimeError("orc_fm2d status %d" % rc)
+    return (tt, tf) if field else tt

@@ -51,4 +51,32 @@ contains
         ierr = ie
     end subroutine
 
+    ! modrays (fm2d/fm2dray_cartesian.f90:67-480) for one period, phase times
+    ! only (uar = 1, no ray tracing): velv(nvz+2,nvx+2) is like%vel(i,:,:).
+    subroutine ref_modrays(nsrc, scx, scz, nrc, rcx, rcz, srs, nvx, nvz, goxd, gozd, dvxd, dvzd, &
+                           velv, gdx, gdz, asgr, sgdl, sgs, fom, snb, ttime, crazy) &
+                           bind(C, name="ref_modrays")
+        use m_fm2d, only : modrays, T_RAY
+        integer(c_int), value :: nsrc, nrc, nvx, nvz, gdx, gdz, asgr, sgdl, sgs, fom
+        real(c_double), intent(in) :: scx(nsrc), scz(nsrc), rcx(nrc), rcz(nrc)
+        integer(c_int), intent(in) :: srs(nrc*nsrc, 2)
+        real(c_double), value :: goxd, gozd, dvxd, dvzd, snb
+        real(c_double), intent(in) :: velv(nvz+2, nvx+2)
+        real(c_double), intent(inout) :: ttime(nrc, nsrc)
+        integer(c_int), intent(out) :: crazy
+        type(T_RAY), allocatable :: rays(:)
+        integer :: cr, srsl(nrc*nsrc, 2)
+        real(8) :: vv(nvz+2, nvx+2)
+        allocate(rays(nrc*nsrc))
+        rays%srcid = 0
+        rays%revid = 0
+        rays%npoints = 0
+        cr = 0
+        srsl = srs
+        vv = velv
+        call modrays(nsrc, scx, scz, nrc, rcx, rcz, srsl, 0, nvx, nvz, goxd, gozd, dvxd, dvzd, vv, &
+                     gdx, gdz, asgr, sgdl, sgs, 6371.0d0, fom, snb, ttime, rays, cr, 1)
+        crazy = cr
+    end subroutine
+
 end module mct_ref_driver

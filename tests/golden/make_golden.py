@@ -68,9 +68,44 @@ def surfdisp_golden():
     np.savez_compressed(os.path.join(OUT, "surfdisp96.npz"), n=n, **arrays)
 
 
+def fm2d_golden():
+    """modrays phase times (fm2d/fm2dray_cartesian.f90) for one period on
+    seeded smooth grids: first and mixed order, with and without the refined
+    source grid.  Sources lie at least one cell inside the model (the
+    reference's refined-grid exit test misbehaves at the model edge)."""
+    rng = np.random.default_rng(20261205)
+    arrays = {}
+    n = 0
+    for trial in range(10):
+        nx, ny, h = int(rng.integers(10, 31)), int(rng.integers(10, 31)), 0.5
+        v = rng.uniform(2.5, 4.0, (nx, ny))
+        velv = np.zeros((nx + 2, ny + 2))
+        velv[1:-1, 1:-1] = v
+        velv[0], velv[-1] = velv[1], velv[-2]
+        velv[:, 0], velv[:, -1] = velv[:, 1], velv[:, -2]
+        L, W = (nx - 1) * h, (ny - 1) * h
+        ns, nr = int(rng.integers(1, 5)), int(rng.integers(2, 9))
+        src = np.c_[rng.uniform(h, L - h, ns), rng.uniform(h, W - h, ns)]
+        rec = np.c_[rng.uniform(0, L, nr), rng.uniform(0, W, nr)]
+        srs = (rng.random(ns * nr) < 0.85).astype(np.int32)
+        for fom in (0, 1):
+            for asgr in (0, 1):
+                tt = po.fm2d(velv, 0, 0, h, h, src[:, 0], src[:, 1], rec[:, 0], rec[:, 1],
+                             srs=srs, fom=fom, asgr=asgr, ref=True)
+                arrays["velv_%d" % n] = velv
+                arrays["src_%d" % n] = src
+                arrays["rec_%d" % n] = rec
+                arrays["srs_%d" % n] = srs
+                arrays["opt_%d" % n] = np.array([fom, asgr])
+                arrays["tt_%d" % n] = tt
+                n += 1
+    np.savez_compressed(os.path.join(OUT, "fm2d_modrays.npz"), n=n, **arrays)
+
+
 if __name__ == "__main__":
     if po.ref() is None:
         sys.exit("oracle/_ref/libmctref.so is not built (make -C oracle ref)")
     kdtree_golden()
     surfdisp_golden()
+    fm2d_golden()
     print("wrote golden vectors to", OUT)
