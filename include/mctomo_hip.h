@@ -66,6 +66,24 @@ int mct_surfdisp96_batch(int nmod, int nl_max, const int32_t *nlayer, const floa
                          int np, const double *periods, int iwave, int igr, double dphase,
                          double *phase, double *group, int32_t *ierr);
 
+/* ---- (c) 2-D phase traveltimes ---------------------------------------------- */
+/* modrays' phase-time path (fm2d/fm2dray_cartesian.f90:67-480, with
+ * crazyray = 0, uar = 1, earth = 6371) for np periods at once.  velv holds np
+ * padded control grids, each Fortran velv(0:nvz+1, 0:nvx+1) as
+ * fm2d_globalp's velv; goxd/gozd the origin and dvxd/dvzd the control
+ * spacing (km); gdx, gdz, asgr, sgdl, sgs, fom as modrays' arguments
+ * (LIKE_SET%gridx/gridy, sgref, sgdic, sgext, order).  scx/scz (nsrc),
+ * rcx/rcz (nrc); srs (np, nsrc, nrc) marks the pairs whose time is computed
+ * (receiver fastest); ttime (np, nsrc, nrc) receives their times, other
+ * entries are left as passed in.  One lane solves one (period, source) with
+ * the reference's sequential algorithm, so the times are the reference's
+ * bit for bit.  Returns MCT_ERR_OUTSIDE where modrays would stop on a source
+ * or receiver outside the grid. */
+int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvzd,
+                int np, const double *velv, int gdx, int gdz, int asgr, int sgdl, int sgs,
+                int fom, int nsrc, const double *scx, const double *scz, int nrc,
+                const double *rcx, const double *rcz, const int32_t *srs, double *ttime);
+
 /* ---- (c) 3-D eikonal: link-level drop-in ---------------------------------- */
 /* Same symbol, signature and semantics as the reference C ABI
  * (fastmarching/fastMarching.cpp:26-30; Fortran interface
@@ -153,6 +171,69 @@ int mct_last_timings(mct_ctx *ctx, double *ms6);
 /* Turn per-launch events around every eikonal round kernel on or off (off by
  * default: they cost a little host time per round). */
 int mct_set_kernel_timing(mct_ctx *ctx, int on);
+
+/* ---- surface-wave likelihood context (datatype 2) ------------------------- */
+/* Replaces likelihood() for datatype 2 (src/likelihood.f90:72-80 ->
+ * surf_likelihood, src/likelihood_surf.F90:110-405) with phase velocities
+ * (LIKE_SET%phaseGroup = 0), no water layer, whole-grid box, as at chain
+ * initialisation.  The setup mirrors T_DATA / T_LIKE_SET
+ * (src/likelihood_settings.f90:19-79); arrays are copied at create time. */
+typedef struct {
+    int raylov;               /* 1 Rayleigh, 0 Love (LIKE_SET%raylov) */
+    int phase_group;          /* must be 0 (phase); group times need ray tracing */
+    double dphase;            /* LIKE_SET%dPhaseVel */
+    int np;                   /* periods, <= 60 */
+    const double *freqs;      /* (np) Hz, dat%freqs */
+    int nsrc, nrev;
+    const double *src;        /* (nsrc,2) x, y: dat%src(1:2,:) */
+    const double *rev;        /* (nrev,2) x, y: dat%rev(1:2,:) */
+    const double *ttime;      /* (np,3,nsrc*nrev) = Fortran dat%ttime(nsrc*nrev,3,np) */
+    const int32_t *raystat;   /* (np,2,nsrc*nrev) = Fortran dat%raystat(nsrc*nrev,2,np) */
+    int sigdep;               /* LIKE_SET%sigdep */
+    int nrays;                /* dat%nrays */
+    int gridx, gridy;         /* LIKE_SET%gridx / gridy: B-spline dicing */
+    int sgref, sgdic, sgext;  /* refined source grid switch, dicing, extent */
+    int order;                /* LIKE_SET%order: 0 first order, 1 mixed order */
+    double band;              /* LIKE_SET%band (narrow-band size fraction) */
+    int max_cells;            /* largest Voronoi site count a call may pass */
+    int max_chains;           /* largest batch for the _batch_dev entry */
+} mct_surf_setup;
+
+typedef struct mct_surf_ctx mct_surf_ctx;
+
+/* like_setup for a surface-wave data set.  grid as in mct_voronoi_to_grid
+ * (dz scaled); grid->nz <= 100 (NMAX, src/likelihood_surf.F90:30). */
+int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_ctx **ctx);
+int mct_surf_destroy(mct_surf_ctx *ctx);
+
+/* One likelihood() call for one chain (host arrays).  points/params
+ * (ncells,3) as RTI%points / RTI%parameters (vp, vs, rho: only vs is used;
+ * vp and rho follow vs2vp_3d / vp2rho_3d).  snoise0/snoise1 (np) (may be NULL
+ * when sigdep = 0).  out = {like, misfit, unweighted_misfit}.  phase_time
+ * (np,nsrc,nrev), srdist and sigma (np,nsrc*nrev), phase_vel (nx,ny,np)
+ * (Fortran pvel(np,ny,nx)) are filled when not NULL.  Returns MCT_OK,
+ * MCT_REJECT_MODEL (check_model or a dispersion failure: the shim sets
+ * like = huge and counts RTI%num_bad_model) or a negative error. */
+int mct_surf_likelihood(mct_surf_ctx *ctx, const double *points, const double *params, int ncells,
+                        const double *snoise0, const double *snoise1, double *out,
+                        double *phase_time, double *srdist, double *sigma, double *phase_vel);
+
+/* nchain likelihood() calls at once on device arrays: points/params
+ * (nchain,max_cells,3), ncells (nchain), noise (nchain,2,np) = snoise0,
+ * snoise1; out (nchain,3), status (nchain).  Asynchronous on the context's
+ * stream; mct_surf_sync waits. */
+int mct_surf_likelihood_batch_dev(mct_surf_ctx *ctx, int nchain, const double *points_dev,
+                                  const double *params_dev, const int32_t *ncells_dev,
+                                  const double *noise_dev, double *out_dev, int32_t *status_dev);
+int mct_surf_sync(mct_surf_ctx *ctx);
+
+/* Stage times of the last synchronised call (ms): nearest site, dispersion,
+ * 2-D traveltimes, misfit. */
+int mct_surf_last_timings(mct_surf_ctx *ctx, double *ms4);
+
+/* Device pointers of the context's phase_time (nchain,np,nsrc,nrev) and
+ * phase velocity (nchain,nx,ny,np) buffers (valid until destroy). */
+int mct_surf_dev_arrays(mct_surf_ctx *ctx, double **phase_time_dev, double **phase_vel_dev);
 
 /* Library version string and the device name the library runs on. */
 const char *mct_version(void);

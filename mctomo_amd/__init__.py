@@ -18,6 +18,6 @@ The library must be built (``python -m mctomo_amd._build``).  Nothing here
 falls back to CPU code: a missing library raises.
 """
 from .api import (  # noqa: F401
-    MCTError, Grid, DeviceArray, lib, lib_path, version, fast_marching_3d, kdtree_to_grid, surfdisp96_batch,
-    BodyLikelihood, REJECT_MODEL, REJECT_RAY,
+    MCTError, Grid, DeviceArray, lib, lib_path, version, fast_marching_3d, kdtree_to_grid, surfdisp96_batch, modrays,
+    BodyLikelihood, SurfLikelihood, REJECT_MODEL, REJECT_RAY,
 )

@@ -60,7 +60,9 @@ def lib():
         for name in ("fastMarching3D", "mct_voronoi_to_grid", "mct_body_create", "mct_destroy",
                      "mct_body_likelihood", "mct_body_likelihood_batch_dev", "mct_sync",
                      "mct_last_timings", "mct_set_kernel_timing", "mct_set_tuning", "mct_dev_alloc", "mct_dev_free", "mct_copy_h2d",
-                     "mct_copy_d2h", "mct_surfdisp96_batch"):
+                     "mct_copy_d2h", "mct_surfdisp96_batch", "mct_modrays", "mct_surf_create",
+                     "mct_surf_destroy", "mct_surf_likelihood", "mct_surf_likelihood_batch_dev",
+                     "mct_surf_sync", "mct_surf_last_timings", "mct_surf_dev_arrays"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -193,6 +195,35 @@ def surfdisp96_batch(models, periods, iwave, igr, dphase=1e-3):
     return ph, gr, ie
 
 
+def modrays(velv, goxd, gozd, dvxd, dvzd, scx, scz, rcx, rcz, srs=None, gdx=1, gdz=1, asgr=1,
+            sgdl=4, sgs=8, fom=1, ttime=None):
+    """modrays phase times (fm2d/fm2dray_cartesian.f90:67-480) on the GPU.
+    velv: (np, nvx+2, nvz+2) or (nvx+2, nvz+2) C array, i.e. Fortran
+    velv(0:nvz+1, 0:nvx+1) per period; srs (np, nsrc, nrc) or (nsrc, nrc)
+    marks the pairs to compute.  Returns ttime (np, nsrc, nrc) (or (nsrc, nrc)
+    for a single grid); unmarked entries keep the values of ``ttime``."""
+    velv = _f64(velv)
+    single = velv.ndim == 2
+    if single:
+        velv = velv[None]
+    npd, nvx, nvz = velv.shape[0], velv.shape[1] - 2, velv.shape[2] - 2
+    scx, scz, rcx, rcz = _f64(scx), _f64(scz), _f64(rcx), _f64(rcz)
+    nsrc, nrc = scx.shape[0], rcx.shape[0]
+    if srs is None:
+        srs = np.ones((npd, nsrc, nrc), np.int32)
+    srs = np.ascontiguousarray(np.broadcast_to(np.asarray(srs, np.int32).reshape(-1, nsrc, nrc),
+                                               (npd, nsrc, nrc)))
+    tt = np.zeros((npd, nsrc, nrc)) if ttime is None else \
+        np.ascontiguousarray(np.broadcast_to(np.asarray(ttime, np.float64).reshape(-1, nsrc, nrc),
+                                             (npd, nsrc, nrc)))
+    rc = lib().mct_modrays(nvx, nvz, ctypes.c_double(goxd), ctypes.c_double(gozd),
+                           ctypes.c_double(dvxd), ctypes.c_double(dvzd), npd, _p(velv), gdx, gdz,
+                           asgr, sgdl, sgs, fom, nsrc, _p(scx), _p(scz), nrc, _p(rcx), _p(rcz),
+                           _p(srs), _p(tt))
+    _check(rc, "mct_modrays")
+    return tt[0] if single else tt
+
+
 def kdtree_to_grid(grid, points, params, box=None, pm=None, vp=None, vs=None, rho=None):
     """kdtree_to_grid: nearest Voronoi site of each node (1-based ids) and the
     site's (vp, vs, rho).  Arrays are (nx,ny,nz) C (= Fortran (nz,ny,nx))."""
@@ -301,3 +332,109 @@ class BodyLikelihood:
         lib().mct_last_timings(self._h, _p(t))
         return {"nearest_ms": t[0], "eikonal_ms": t[1], "misfit_ms": t[2], "rounds": int(t[3]),
                 "round_kernels_ms": t[4], "round_launches": int(t[5])}
+
+
+class _SurfSetup(ctypes.Structure):
+    _fields_ = [("raylov", ctypes.c_int), ("phase_group", ctypes.c_int), ("dphase", ctypes.c_double),
+                ("np", ctypes.c_int), ("freqs", ctypes.c_void_p), ("nsrc", ctypes.c_int),
+                ("nrev", ctypes.c_int), ("src", ctypes.c_void_p), ("rev", ctypes.c_void_p),
+                ("ttime", ctypes.c_void_p), ("raystat", ctypes.c_void_p), ("sigdep", ctypes.c_int),
+                ("nrays", ctypes.c_int), ("gridx", ctypes.c_int), ("gridy", ctypes.c_int),
+                ("sgref", ctypes.c_int), ("sgdic", ctypes.c_int), ("sgext", ctypes.c_int),
+                ("order", ctypes.c_int), ("band", ctypes.c_double), ("max_cells", ctypes.c_int),
+                ("max_chains", ctypes.c_int)]
+
+
+class SurfLikelihood:
+    """Surface-wave likelihood context: like_setup + likelihood for datatype 2
+    with phase velocities (src/likelihood_surf.F90:110-405).
+
+    freqs (np) Hz; src (nsrc,2) and rev (nrev,2) as (x, y); ttime
+    (np,3,nsrc*nrev) = (time, sigma, -) with the receiver index fastest;
+    raystat (np,2,nsrc*nrev).  Settings as LIKE_SET: raylov (1 Rayleigh,
+    0 Love), dphase (dPhaseVel), gridx/gridy, sgref/sgdic/sgext, order, band.
+    """
+
+    def __init__(self, grid, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3, sigdep=0,
+                 nrays=None, gridx=1, gridy=1, sgref=1, sgdic=4, sgext=8, order=1, band=0.5,
+                 max_cells=1024, max_chains=1):
+        self.grid = grid
+        self.freqs = _f64(freqs).reshape(-1)
+        self.np = self.freqs.shape[0]
+        self.src = _f64(src).reshape(-1, 2)
+        self.rev = _f64(rev).reshape(-1, 2)
+        self.nsrc, self.nrev = self.src.shape[0], self.rev.shape[0]
+        npair = self.nsrc * self.nrev
+        self.ttime = _f64(ttime).reshape(self.np, 3, npair)
+        self.raystat = np.ascontiguousarray(raystat, dtype=np.int32).reshape(self.np, 2, npair)
+        if nrays is None:
+            nrays = int((self.raystat[:, 0, :] == 1).sum())
+        self.nrays, self.sigdep = nrays, sigdep
+        self.max_cells, self.max_chains = max_cells, max_chains
+        st = _SurfSetup(int(raylov), 0, float(dphase), self.np, self.freqs.ctypes.data, self.nsrc,
+                        self.nrev, self.src.ctypes.data, self.rev.ctypes.data, self.ttime.ctypes.data,
+                        self.raystat.ctypes.data, int(sigdep), int(nrays), int(gridx), int(gridy),
+                        int(sgref), int(sgdic), int(sgext), int(order), float(band), int(max_cells),
+                        int(max_chains))
+        self._g = grid.c()
+        h = ctypes.c_void_p()
+        _check(lib().mct_surf_create(ctypes.byref(self._g), ctypes.byref(st), ctypes.byref(h)),
+               "mct_surf_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mct_surf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def likelihood(self, points, params, snoise0=None, snoise1=None, arrays=False):
+        """One chain.  Returns (status, (like, misfit, unweighted)[, phase_time
+        (np,nsrc,nrev), srdist (np,npair), sigma (np,npair), phase_vel (nx,ny,np)])."""
+        pts = _f64(points)
+        prm = _f64(params)
+        s0 = None if snoise0 is None else _f64(np.broadcast_to(np.atleast_1d(snoise0), (self.np,)))
+        s1 = None if snoise1 is None else _f64(np.broadcast_to(np.atleast_1d(snoise1), (self.np,)))
+        out = np.zeros(3)
+        pt = sd = sg = pv = None
+        if arrays:
+            npair = self.nsrc * self.nrev
+            pt = np.zeros((self.np, self.nsrc, self.nrev))
+            sd = np.zeros((self.np, npair))
+            sg = np.zeros((self.np, npair))
+            pv = np.zeros((self.grid.nx, self.grid.ny, self.np))
+        rc = lib().mct_surf_likelihood(self._h, _p(pts), _p(prm), pts.shape[0], _p(s0), _p(s1),
+                                       _p(out), _p(pt), _p(sd), _p(sg), _p(pv))
+        _check(rc, "mct_surf_likelihood")
+        if arrays:
+            return rc, out, pt, sd, sg, pv
+        return rc, out
+
+    def likelihood_batch_dev(self, nchain, points_ptr, params_ptr, ncells_ptr, noise_ptr, out_ptr,
+                             status_ptr):
+        """Batched call on device pointers (see DeviceArray)."""
+        rc = lib().mct_surf_likelihood_batch_dev(
+            self._h, int(nchain), ctypes.c_void_p(points_ptr), ctypes.c_void_p(params_ptr),
+            ctypes.c_void_p(ncells_ptr), ctypes.c_void_p(noise_ptr), ctypes.c_void_p(out_ptr),
+            ctypes.c_void_p(status_ptr))
+        return _check(rc, "mct_surf_likelihood_batch_dev")
+
+    def sync(self):
+        _check(lib().mct_surf_sync(self._h), "mct_surf_sync")
+
+    def timings(self):
+        """Stage times (ms) of the last synchronised call: nearest, dispersion, 2-D, misfit."""
+        t = np.zeros(4)
+        lib().mct_surf_last_timings(self._h, _p(t))
+        return t
+
+    def dev_arrays(self):
+        """Device addresses of the phase_time and phase-velocity buffers."""
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        lib().mct_surf_dev_arrays(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value

@@ -7,6 +7,7 @@
 // are only read or written during a call, as with c_loc in
 // src/fastMarching_wrapper.f90:51-74.
 #include "mct_internal.h"
+#include <float.h>
 #include <stdlib.h>
 #include <string.h>
 #include <mutex>
@@ -230,6 +231,7 @@ static int body_pipeline(mct_ctx *C, int nchain, const double *d_points, const d
     ma.reciprocal = C->reciprocal; ma.picks = P<double>(C->picks);
     ma.ttime = P<double>(C->ttime); ma.raystat = P<int32_t>(C->raystat);
     ma.locations = d_locations; ma.noise = d_noise; ma.sigdep = C->S.sigdep; ma.nrays = C->S.nrays;
+    ma.eps = DBL_MIN * 10.0;   // EPS = tiny(0d0)*10 (src/likelihood_body.F90:33)
     ma.out = d_out;
     ma.phase_time = want_arrays ? P<double>(C->phase_time) : nullptr;
     ma.srdist = want_arrays ? P<double>(C->srdist) : nullptr;

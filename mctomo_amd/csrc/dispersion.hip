@@ -385,22 +385,6 @@ __device__ static int d_getsol(double t1, double &c1io, double clow, double dc, 
     return -1;
 }
 
-struct mct_dsp_args {
-    int ncol;                     // columns per chain (nx*ny, Fortran order j + ny*i)
-    int nz;
-    int nchain;
-    const double *vs;             // (nchain, nx, ny, nz) Fortran (nz,ny,nx) per chain
-    double dz, scaling;           // grid%dz (scaled) and grid%scaling
-    const double *periods;        // (np) periods in s (1/freqs)
-    int np;
-    int iwave;                    // 1 Love, 2 Rayleigh
-    int igr;                      // 0 phase, 1 group
-    double dphase;
-    double *phase, *group;        // (nchain, ncol, np)
-    int32_t *ierr;                // (nchain, ncol)
-    int32_t *bad_model;           // (nchain): check_model failed
-    int32_t *ndltar;              // (nchain, ncol) or NULL: period-equation evaluations
-};
 
 // One wave per column: check_model, convert_to_layer, surfdisp96.
 __global__ __launch_bounds__(64) void k_dispersion(mct_dsp_args A)

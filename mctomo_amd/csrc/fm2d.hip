@@ -1,0 +1,658 @@
+// fm2d.hip -- 2-D phase traveltimes of the surface-wave likelihood on gfx950.
+//
+// Replaces modrays' phase-time path (fm2d/fm2dray_cartesian.f90:67-480):
+// gridder (:491-600), bsplrefine (:608-674), travel (fm2d/fm2d_ttime.f90:41-238)
+// with fouds1 / fouds2 (:248-518) and its binary-tree narrow band (:527-680),
+// and srtimes (fm2dray_cartesian.f90:682-797).
+//
+// The reference's answer depends on the order in which its heap freezes nodes:
+// the refined source grid stops at the first popped node on an inner edge
+// (fm2d_ttime.f90:143-163), the coarse solve restarts from the remapped band,
+// and the mixed-order stencil only uses nodes already frozen.  A parallel
+// relaxation cannot reproduce that order, so each (chain, period, source)
+// problem runs the reference's own sequential algorithm in one lane: 64
+// independent problems per wave, each with its grid, state and heap in a
+// private slice of HBM.  Results are bit-identical to the sequential code.
+// The coarse B-spline velocity of a period is computed once by a separate
+// kernel (one thread per node) and shared by all sources of that period.
+#include "mct_internal.h"
+#include <string.h>
+
+#pragma clang fp contract(off)
+
+
+// ---- coarse B-spline velocity ---------------------------------------------------
+__device__ __forceinline__ void bspl_w(double u, double w[4])
+{
+    w[0] = (1.0 - u) * (1.0 - u) * (1.0 - u) / 6.0;
+    w[1] = (4.0 - 6.0 * (u * u) + 3.0 * (u * u * u)) / 6.0;
+    w[2] = (1.0 + 3.0 * u + 3.0 * (u * u) - 3.0 * (u * u * u)) / 6.0;
+    w[3] = (u * u * u) / 6.0;
+}
+
+// gridder: veln(stz,stx) = sum_i1 vi(l,i1) * sum_j1 ui(m,j1) velv(i-2+i1, j-2+j1)
+__global__ void k_fm2d_gridder(mct_fm2d_args A)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nn = (long)A.nnx * A.nnz;
+    if (t >= nn * A.nchain * A.np) return;
+    const int cp = (int)(t / nn);
+    const int node = (int)(t - (long)cp * nn);
+    const int stx = node / A.nnz + 1, stz = node % A.nnz + 1;
+    // the control cell (i,j) and sub-node (l,m) that produce (stz,stx); the last
+    // cell of each axis also owns its far edge node
+    int i = (stz - 1) / A.gdz + 1, l = (stz - 1) % A.gdz + 1;
+    if (i > A.nvz - 1) { i = A.nvz - 1; l = A.gdz + 1; }
+    int j = (stx - 1) / A.gdx + 1, m = (stx - 1) % A.gdx + 1;
+    if (j > A.nvx - 1) { j = A.nvx - 1; m = A.gdx + 1; }
+    double u = A.gdx; u = (m - 1) / u;
+    double v = A.gdz; v = (l - 1) / v;
+    double ui[4], vi[4];
+    bspl_w(u, ui);
+    bspl_w(v, vi);
+    const int ldv = A.nvz + 2;
+    const double *vv = A.velv + (long)cp * ldv * (A.nvx + 2);
+    double sumi = 0.0;
+    for (int i1 = 1; i1 <= 4; i1++) {
+        double sumj = 0.0;
+        for (int j1 = 1; j1 <= 4; j1++) sumj = sumj + ui[j1 - 1] * vv[(j - 2 + j1) * ldv + (i - 2 + i1)];
+        sumi = sumi + vi[i1 - 1] * sumj;
+    }
+    A.veln[(long)cp * nn + (long)(stx - 1) * A.nnz + (stz - 1)] = sumi;
+}
+
+// ---- per-lane sequential solver ----------------------------------------------------
+struct lane_grid {
+    int nnx, nnz, ld;
+    double gox, goz, dnx, dnz;
+    double *t;              // times
+    int32_t *s;             // state: -1 far, 0 alive, >0 heap position
+    const double *v;        // velocity
+    int vld;                // leading dimension of v
+    int32_t *btg;           // heap, 1-based
+    int ntr, fom;
+    int vnl, vnr, vnt, vnb;
+};
+
+#define LT(G, iz, ix) (G).t[((ix) - 1) * (G).ld + ((iz) - 1)]
+#define LS(G, iz, ix) (G).s[((ix) - 1) * (G).ld + ((iz) - 1)]
+#define LV(G, iz, ix) (G).v[((ix) - 1) * (G).vld + ((iz) - 1)]
+#define BPX(e) ((e) >> 16)
+#define BPZ(e) ((e) & 0xffff)
+
+__device__ static void l_addtree(lane_grid &G, int iz, int ix)
+{
+    G.ntr = G.ntr + 1;
+    LS(G, iz, ix) = G.ntr;
+    G.btg[G.ntr] = (ix << 16) | iz;
+    int tpc = G.ntr, tpp = tpc / 2;
+    const double tv = LT(G, iz, ix);
+    while (tpp > 0) {
+        int ep = G.btg[tpp];
+        if (tv < LT(G, BPZ(ep), BPX(ep))) {
+            LS(G, iz, ix) = tpp;
+            LS(G, BPZ(ep), BPX(ep)) = tpc;
+            G.btg[tpp] = G.btg[tpc];
+            G.btg[tpc] = ep;
+            tpc = tpp;
+            tpp = tpc / 2;
+        } else {
+            tpp = 0;
+        }
+    }
+}
+
+__device__ static void l_downtree(lane_grid &G)
+{
+    if (G.ntr == 1) { G.ntr = G.ntr - 1; return; }
+    int el = G.btg[G.ntr];
+    LS(G, BPZ(el), BPX(el)) = 1;
+    G.btg[1] = el;
+    G.ntr = G.ntr - 1;
+    int tpp = 1, tpc = 2 * tpp;
+    while (tpc < G.ntr) {
+        int ec = G.btg[tpc], ec1 = G.btg[tpc + 1];
+        double rd1 = LT(G, BPZ(ec), BPX(ec));
+        double rd2 = LT(G, BPZ(ec1), BPX(ec1));
+        if (rd1 > rd2) tpc = tpc + 1;
+        ec = G.btg[tpc];
+        int ep = G.btg[tpp];
+        rd1 = LT(G, BPZ(ec), BPX(ec));
+        rd2 = LT(G, BPZ(ep), BPX(ep));
+        if (rd1 < rd2) {
+            LS(G, BPZ(ep), BPX(ep)) = tpc;
+            LS(G, BPZ(ec), BPX(ec)) = tpp;
+            G.btg[tpc] = ep;
+            G.btg[tpp] = ec;
+            tpp = tpc;
+            tpc = 2 * tpp;
+        } else {
+            tpc = G.ntr + 1;
+        }
+    }
+    if (tpc == G.ntr) {
+        int ec = G.btg[tpc], ep = G.btg[tpp];
+        double rd1 = LT(G, BPZ(ec), BPX(ec));
+        double rd2 = LT(G, BPZ(ep), BPX(ep));
+        if (rd1 < rd2) {
+            LS(G, BPZ(ep), BPX(ep)) = tpc;
+            LS(G, BPZ(ec), BPX(ec)) = tpp;
+            G.btg[tpc] = ep;
+            G.btg[tpp] = ec;
+        }
+    }
+}
+
+__device__ static void l_updtree(lane_grid &G, int iz, int ix)
+{
+    int tpc = LS(G, iz, ix), tpp = tpc / 2;
+    const double tv = LT(G, iz, ix);
+    while (tpp > 0) {
+        int ep = G.btg[tpp];
+        if (tv < LT(G, BPZ(ep), BPX(ep))) {
+            LS(G, iz, ix) = tpp;
+            LS(G, BPZ(ep), BPX(ep)) = tpc;
+            G.btg[tpp] = G.btg[tpc];
+            G.btg[tpc] = ep;
+            tpc = tpp;
+            tpp = tpc / 2;
+        } else {
+            tpp = 0;
+        }
+    }
+}
+
+__device__ static void l_fouds1(lane_grid &G, int iz, int ix)
+{
+    int tsw1 = 0;
+    double travm = 0.0;
+    const double slown = 1.0 / LV(G, iz, ix);
+    for (int j = ix - 1; j <= ix + 1; j += 2)
+        for (int k = iz - 1; k <= iz + 1; k += 2) {
+            if (!(j >= 1 && j <= G.nnx && k >= 1 && k <= G.nnz)) continue;
+            int swsol = 0;
+            double a = 0, b = 0, c = 0, tref = 0;
+            if (LS(G, iz, j) == 0) {
+                swsol = 1;
+                if (LS(G, k, ix) == 0) {
+                    double u = G.dnx, v = G.dnz;
+                    double em = LT(G, k, ix) - LT(G, iz, j);
+                    a = u * u + v * v;
+                    b = -(2.0 * (u * u) * em);
+                    c = (u * u) * (em * em - (v * v) * (slown * slown));
+                    tref = LT(G, iz, j);
+                } else {
+                    a = 1.0; b = 0.0;
+                    c = -((slown * slown) * (G.dnx * G.dnx));
+                    tref = LT(G, iz, j);
+                }
+            } else if (LS(G, k, ix) == 0) {
+                swsol = 1;
+                a = 1.0; b = 0.0;
+                c = -((slown * G.dnz) * (slown * G.dnz));
+                tref = LT(G, k, ix);
+            }
+            if (swsol == 1) {
+                double rd1 = b * b - 4.0 * a * c;
+                if (rd1 < 0.0) rd1 = 0.0;
+                double tdsh = (-b + sqrt(rd1)) / (2.0 * a);
+                double trav = tref + tdsh;
+                if (tsw1 == 1) travm = fmin(trav, travm);
+                else { travm = trav; tsw1 = 1; }
+            }
+        }
+    LT(G, iz, ix) = travm;
+}
+
+__device__ static void l_fouds2(lane_grid &G, int iz, int ix)
+{
+    int tsw1 = 0;
+    double travm = 0.0;
+    const double slown = 1.0 / LV(G, iz, ix);
+    for (int j = ix - 1; j <= ix + 1; j += 2) {
+        if (!(j >= 1 && j <= G.nnx)) continue;
+        int swj = -1, j2;
+        if (j == ix - 1) {
+            j2 = j - 1;
+            if (j2 >= 1 && LS(G, iz, j2) == 0) swj = 0;
+        } else {
+            j2 = j + 1;
+            if (j2 <= G.nnx && LS(G, iz, j2) == 0) swj = 0;
+        }
+        if (LS(G, iz, j) == 0 && swj == 0) {
+            swj = -1;
+            if (LT(G, iz, j) > LT(G, iz, j2)) swj = 0;
+        } else {
+            swj = -1;
+        }
+        for (int k = iz - 1; k <= iz + 1; k += 2) {
+            if (!(k >= 1 && k <= G.nnz)) continue;
+            int swk = -1, k2;
+            if (k == iz - 1) {
+                k2 = k - 1;
+                if (k2 >= 1 && LS(G, k2, ix) == 0) swk = 0;
+            } else {
+                k2 = k + 1;
+                if (k2 <= G.nnz && LS(G, k2, ix) == 0) swk = 0;
+            }
+            if (LS(G, k, ix) == 0 && swk == 0) {
+                swk = -1;
+                if (LT(G, k, ix) > LT(G, k2, ix)) swk = 0;
+            } else {
+                swk = -1;
+            }
+            int swsol = 0;
+            double a = 0, b = 0, c = 0, tref = 0, tdiv = 1.0, u, v, em;
+            if (swj == 0) {
+                swsol = 1;
+                if (swk == 0) {
+                    u = 2.0 * G.dnx;
+                    v = 2.0 * G.dnz;
+                    em = 4.0 * LT(G, iz, j) - LT(G, iz, j2) - 4.0 * LT(G, k, ix);
+                    em = em + LT(G, k2, ix);
+                    a = v * v + u * u;
+                    b = 2.0 * em * (u * u);
+                    c = (u * u) * (em * em - (slown * slown) * (v * v));
+                    tref = 4.0 * LT(G, iz, j) - LT(G, iz, j2);
+                    tdiv = 3.0;
+                } else if (LS(G, k, ix) == 0) {
+                    u = G.dnz;
+                    v = 2.0 * G.dnx;
+                    em = 3.0 * LT(G, k, ix) - 4.0 * LT(G, iz, j) + LT(G, iz, j2);
+                    a = v * v + 9.0 * (u * u);
+                    b = 6.0 * em * (u * u);
+                    c = (u * u) * (em * em - (slown * slown) * (v * v));
+                    tref = LT(G, k, ix);
+                    tdiv = 1.0;
+                } else {
+                    u = 2.0 * G.dnx;
+                    a = 1.0; b = 0.0;
+                    c = -((u * u) * (slown * slown));
+                    tref = 4.0 * LT(G, iz, j) - LT(G, iz, j2);
+                    tdiv = 3.0;
+                }
+            } else if (LS(G, iz, j) == 0) {
+                swsol = 1;
+                if (swk == 0) {
+                    u = G.dnx;
+                    v = 2.0 * G.dnz;
+                    em = 3.0 * LT(G, iz, j) - 4.0 * LT(G, k, ix) + LT(G, k2, ix);
+                    a = v * v + 9.0 * (u * u);
+                    b = 6.0 * em * (u * u);
+                    c = (u * u) * (em * em - (v * v) * (slown * slown));
+                    tref = LT(G, iz, j);
+                    tdiv = 1.0;
+                } else if (LS(G, k, ix) == 0) {
+                    u = G.dnx;
+                    v = G.dnz;
+                    em = LT(G, k, ix) - LT(G, iz, j);
+                    a = u * u + v * v;
+                    b = -(2.0 * (u * u) * em);
+                    c = (u * u) * (em * em - (v * v) * (slown * slown));
+                    tref = LT(G, iz, j);
+                    tdiv = 1.0;
+                } else {
+                    a = 1.0; b = 0.0;
+                    c = -((slown * slown) * (G.dnx * G.dnx));
+                    tref = LT(G, iz, j);
+                    tdiv = 1.0;
+                }
+            } else {
+                if (swk == 0) {
+                    swsol = 1;
+                    u = 2.0 * G.dnz;
+                    a = 1.0; b = 0.0;
+                    c = -((u * u) * (slown * slown));
+                    tref = 4.0 * LT(G, k, ix) - LT(G, k2, ix);
+                    tdiv = 3.0;
+                } else if (LS(G, k, ix) == 0) {
+                    swsol = 1;
+                    a = 1.0; b = 0.0;
+                    c = -((slown * slown) * (G.dnz * G.dnz));
+                    tref = LT(G, k, ix);
+                    tdiv = 1.0;
+                }
+            }
+            if (swsol == 1) {
+                double rd1 = b * b - 4.0 * a * c;
+                if (rd1 < 0.0) rd1 = 0.0;
+                double tdsh = (-b + sqrt(rd1)) / (2.0 * a);
+                double trav = (tref + tdsh) / tdiv;
+                if (tsw1 == 1) travm = fmin(trav, travm);
+                else { travm = trav; tsw1 = 1; }
+            }
+        }
+    }
+    LT(G, iz, ix) = travm;
+}
+
+__device__ __forceinline__ double l_bilinear(const double nv[2][2], double dsx, double dsz, double dnx, double dnz)
+{
+    double biv = 0.0;
+    for (int i = 1; i <= 2; i++)
+        for (int j = 1; j <= 2; j++) {
+            double produ = (1.0 - fabs(((i - 1) * dnx - dsx) / dnx)) * (1.0 - fabs(((j - 1) * dnz - dsz) / dnz));
+            biv = biv + nv[i - 1][j - 1] * produ;
+        }
+    return biv;
+}
+
+// travel (fm2d_ttime.f90:41-238); returns -1 if the source is outside the grid
+__device__ static int l_travel(lane_grid &G, double scx, double scz, int urg)
+{
+    int isx = (int)((scx - G.gox) / G.dnx) + 1;
+    int isz = (int)((scz - G.goz) / G.dnz) + 1;
+    if (isx < 1 || isx > G.nnx || isz < 1 || isz > G.nnz) return -1;
+    if (isx == G.nnx) isx = isx - 1;
+    if (isz == G.nnz) isz = isz - 1;
+    G.ntr = 0;
+    if (urg == 2) {
+        for (int i = 1; i <= G.nnx; i++)
+            for (int j = 1; j <= G.nnz; j++)
+                if (LS(G, j, i) > 0) l_addtree(G, j, i);
+    } else {
+        for (int i = 1; i <= G.nnx; i++)
+            for (int j = 1; j <= G.nnz; j++) LS(G, j, i) = -1;
+        double vss[2][2];
+        for (int i = 1; i <= 2; i++)
+            for (int j = 1; j <= 2; j++) vss[i - 1][j - 1] = LV(G, isz - 1 + j, isx - 1 + i);
+        double dsx = (scx - G.gox) - (isx - 1) * G.dnx;
+        double dsz = (scz - G.goz) - (isz - 1) * G.dnz;
+        double vsrc = l_bilinear(vss, dsx, dsz, G.dnx, G.dnz);
+        for (int i = 1; i <= 2; i++)
+            for (int j = 1; j <= 2; j++) {
+                double ex = dsx - (i - 1) * G.dnx, ez = dsz - (j - 1) * G.dnz;
+                double ds = sqrt(ex * ex + ez * ez);
+                LT(G, isz - 1 + j, isx - 1 + i) = 2.0 * ds / (vss[i - 1][j - 1] + vsrc);
+                l_addtree(G, isz - 1 + j, isx - 1 + i);
+            }
+    }
+    while (G.ntr > 0) {
+        int e = G.btg[1];
+        int ix = BPX(e), iz = BPZ(e);
+        if (urg == 1) {   // stop when the band reaches an inner edge of the refined grid
+            int swrg = 0;
+            if (ix == 1 && G.vnl != 1) swrg = 1;
+            if (ix == G.nnx && G.vnr != G.nnx) swrg = 1;
+            if (iz == 1 && G.vnt != 1) swrg = 1;
+            if (iz == G.nnz && G.vnb != G.nnz) swrg = 1;
+            if (swrg == 1) { LS(G, iz, ix) = 0; break; }
+        }
+        LS(G, iz, ix) = 0;
+        l_downtree(G);
+        for (int i = ix - 1; i <= ix + 1; i += 2) {
+            if (i >= 1 && i <= G.nnx) {
+                int st = LS(G, iz, i);
+                if (st == -1) {
+                    if (G.fom == 0) l_fouds1(G, iz, i); else l_fouds2(G, iz, i);
+                    l_addtree(G, iz, i);
+                } else if (st > 0) {
+                    if (G.fom == 0) l_fouds1(G, iz, i); else l_fouds2(G, iz, i);
+                    l_updtree(G, iz, i);
+                }
+            }
+        }
+        for (int i = iz - 1; i <= iz + 1; i += 2) {
+            if (i >= 1 && i <= G.nnz) {
+                int st = LS(G, i, ix);
+                if (st == -1) {
+                    if (G.fom == 0) l_fouds1(G, i, ix); else l_fouds2(G, i, ix);
+                    l_addtree(G, i, ix);
+                } else if (st > 0) {
+                    if (G.fom == 0) l_fouds1(G, i, ix); else l_fouds2(G, i, ix);
+                    l_updtree(G, i, ix);
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+// One lane per (chain, period, source): modrays' loop body for that source.
+__global__ __launch_bounds__(64) void k_fm2d_solve(mct_fm2d_args A)
+{
+    const long slot = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= A.nrun) return;
+    const long prob = A.prob0 + slot;
+    const int s = (int)(prob % A.nsrc);
+    const long cp = prob / A.nsrc;          // chain*np + period
+    const int p = (int)(cp % A.np);
+    const int c = (int)(cp / A.np);
+    if (A.status[c] > 0) return;            // model already rejected
+    const int32_t *srs = A.srs + (long)p * A.srs_pstride + (long)s * A.nrc;
+    int any = 0;
+    for (int r = 0; r < A.nrc; r++) any += srs[r];
+    if (any == 0 && s != 0) return;        // modrays skips sources without pairs (:211)
+    const long wsz = (long)A.ld * A.nc;
+    const double *velnc = A.veln + cp * (long)A.nnx * A.nnz;
+    const double *velv = A.velv + cp * (long)(A.nvz + 2) * (A.nvx + 2);
+    const int ldv = A.nvz + 2;
+    const double dnx = A.dvx / A.gdx, dnz = A.dvz / A.gdz;
+    lane_grid G;
+    G.ld = A.ld;
+    G.t = A.ttn + slot * wsz;
+    G.s = A.nsts + slot * wsz;
+    G.btg = A.heap + slot * (wsz + 8);
+    G.fom = A.fom;
+    const double x = A.scx[s], z = A.scz[s];
+    int isx = (int)((x - A.gox) / dnx) + 1, isz = (int)((z - A.goz) / dnz) + 1;
+    if (isx < 1 || isx > A.nnx || isz < 1 || isz > A.nnz) { atomicMin(&A.status[c], (int)MCT_ERR_OUTSIDE); return; }
+    if (isx == A.nnx) isx = isx - 1;
+    if (isz == A.nnz) isz = isz - 1;
+    int vnl = isx - A.sgs, vnr = isx + A.sgs, vnt = isz - A.sgs, vnb = isz + A.sgs;
+    if (vnl < 1) vnl = 1;
+    if (vnr > A.nnx) vnr = A.nnx;
+    if (vnt < 1) vnt = 1;
+    if (vnb > A.nnz) vnb = A.nnz;
+    if (A.asgr == 1) {
+        // refined source grid with B-spline velocities (bsplrefine)
+        double *vr = A.vr + slot * wsz;
+        G.nnx = (vnr - vnl) * A.sgdl + 1;
+        G.nnz = (vnb - vnt) * A.sgdl + 1;
+        G.dnx = A.dvx / (double)(float)(A.gdx * A.sgdl);
+        G.dnz = A.dvz / (double)(float)(A.gdz * A.sgdl);
+        G.gox = A.gox + dnx * (vnl - 1);
+        G.goz = A.goz + dnz * (vnt - 1);
+        G.vnl = vnl; G.vnr = vnr; G.vnt = vnt; G.vnb = vnb;
+        G.v = vr; G.vld = A.ld;
+        const int nrxr = A.gdx * A.sgdl, nrzr = A.gdz * A.sgdl;
+        const int origx = (vnl - 1) * A.sgdl + 1, origz = (vnt - 1) * A.sgdl + 1;
+        for (int ii = 1; ii <= A.nvz - 1; ii++) {
+            int conz = (ii == A.nvz - 1) ? nrzr + 1 : nrzr;
+            for (int jj = 1; jj <= A.nvx - 1; jj++) {
+                int conx = (jj == A.nvx - 1) ? nrxr + 1 : nrxr;
+                for (int k = 1; k <= conz; k++) {
+                    int st1 = A.gdz * (ii - 1) + (k - 1) / A.sgdl + 1;
+                    if (st1 < vnt || st1 > vnb) continue;
+                    st1 = nrzr * (ii - 1) + k;
+                    double v = nrzr; v = (k - 1) / v;
+                    double vi[4];
+                    bspl_w(v, vi);
+                    for (int l = 1; l <= conx; l++) {
+                        int st2 = A.gdx * (jj - 1) + (l - 1) / A.sgdl + 1;
+                        if (st2 < vnl || st2 > vnr) continue;
+                        st2 = nrxr * (jj - 1) + l;
+                        int idm1 = st1 - origz + 1, idm2 = st2 - origx + 1;
+                        if (idm1 < 1 || idm1 > G.nnz || idm2 < 1 || idm2 > G.nnx) continue;
+                        double u = nrxr; u = (l - 1) / u;
+                        double ui[4];
+                        bspl_w(u, ui);
+                        double sum[4];
+                        for (int i1 = 1; i1 <= 4; i1++) {
+                            sum[i1 - 1] = 0.0;
+                            for (int j1 = 1; j1 <= 4; j1++)
+                                sum[i1 - 1] = sum[i1 - 1] + ui[j1 - 1] * velv[(jj - 2 + j1) * ldv + (ii - 2 + i1)];
+                            sum[i1 - 1] = vi[i1 - 1] * sum[i1 - 1];
+                        }
+                        vr[(idm2 - 1) * A.ld + (idm1 - 1)] = sum[0] + sum[1] + sum[2] + sum[3];
+                    }
+                }
+            }
+        }
+        if (l_travel(G, x, z, 1)) { atomicMin(&A.status[c], (int)MCT_ERR_OUTSIDE); return; }
+        // remap the refined grid onto the coarse grid
+        double *tr = A.ttr + slot * wsz;
+        int32_t *nr = A.nstr + slot * wsz;
+        const int rnx = G.nnx, rnz = G.nnz;
+        for (int a = 1; a <= rnx; a++)
+            for (int b = 1; b <= rnz; b++) {
+                long o = (long)(a - 1) * A.ld + (b - 1);
+                tr[o] = G.t[o];
+                nr[o] = G.s[o];
+            }
+        for (int a = 1; a <= A.nnx; a++)
+            for (int b = 1; b <= A.nnz; b++) G.s[(long)(a - 1) * A.ld + (b - 1)] = -1;
+        for (int k = 1; k <= rnz; k += A.sgdl) {
+            int idm1 = vnt + (k - 1) / A.sgdl;
+            for (int l = 1; l <= rnx; l += A.sgdl) {
+                int idm2 = vnl + (l - 1) / A.sgdl;
+                int st = nr[(long)(l - 1) * A.ld + (k - 1)];
+                G.s[(long)(idm2 - 1) * A.ld + (idm1 - 1)] = st;
+                if (st >= 0) G.t[(long)(idm2 - 1) * A.ld + (idm1 - 1)] = tr[(long)(l - 1) * A.ld + (k - 1)];
+            }
+        }
+        G.nnx = A.nnx; G.nnz = A.nnz; G.dnx = dnx; G.dnz = dnz; G.gox = A.gox; G.goz = A.goz;
+        G.v = velnc; G.vld = A.nnz;
+        // complete the narrow band: alive nodes next to far nodes become close
+        for (int k = 1; k <= A.nnx; k++)
+            for (int l = 1; l <= A.nnz; l++) {
+                if (LS(G, l, k) != 0) continue;
+                if ((l - 1 >= 1 && LS(G, l - 1, k) == -1) || (l + 1 <= A.nnz && LS(G, l + 1, k) == -1) ||
+                    (k - 1 >= 1 && LS(G, l, k - 1) == -1) || (k + 1 <= A.nnx && LS(G, l, k + 1) == -1))
+                    LS(G, l, k) = 1;
+            }
+        if (l_travel(G, x, z, 2)) { atomicMin(&A.status[c], (int)MCT_ERR_OUTSIDE); return; }
+    } else {
+        G.nnx = A.nnx; G.nnz = A.nnz; G.dnx = dnx; G.dnz = dnz; G.gox = A.gox; G.goz = A.goz;
+        G.v = velnc; G.vld = A.nnz;
+        if (l_travel(G, x, z, 0)) { atomicMin(&A.status[c], (int)MCT_ERR_OUTSIDE); return; }
+    }
+    // srtimes: receiver picks
+    double *out = A.ttime + prob * A.nrc;
+    for (int r = 1; r <= A.nrc; r++) {
+        if (srs[r - 1] == 0) continue;
+        const double rx = A.rcx[r - 1], rz = A.rcz[r - 1];
+        int irx = (int)floor((rx - A.gox) / dnx) + 1;
+        int irz = (int)floor((rz - A.goz) / dnz) + 1;
+        if (irx < 1 || irx > A.nnx || irz < 1 || irz > A.nnz) { atomicMin(&A.status[c], (int)MCT_ERR_OUTSIDE); return; }
+        if (irx == A.nnx) irx = irx - 1;
+        if (irz == A.nnz) irz = irz - 1;
+        int jsx = (int)floor((x - A.gox) / dnx) + 1, jsz = (int)floor((z - A.goz) / dnz) + 1;
+        double dpl = dnx, rd1 = dnz;
+        if (rd1 < dpl) dpl = rd1;
+        double sred = (x - rx) * (x - rx);
+        sred = sred + (z - rz) * (z - rz);
+        sred = sqrt(sred);
+        int sw = 0;
+        if (sred < dpl) sw = 1;
+        if (jsx == irx && jsz == irz) sw = 1;
+        double trr;
+        if (sw == 1) {
+            double vss[2][2];
+            // the source cell, kept inside the grid (the reference reads past the
+            // grid for a source on its last line or just below its origin)
+            int csx = jsx < A.nnx ? jsx : A.nnx - 1, csz = jsz < A.nnz ? jsz : A.nnz - 1;
+            csx = csx < 1 ? 1 : csx;
+            csz = csz < 1 ? 1 : csz;
+            for (int k = 1; k <= 2; k++)
+                for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = velnc[(long)(csx - 1 + k - 1) * A.nnz + (csz - 1 + l - 1)];
+            double drx = (x - A.gox) - (jsx - 1) * dnx, drz = (z - A.goz) - (jsz - 1) * dnz;
+            double vels = l_bilinear(vss, drx, drz, dnx, dnz);
+            for (int k = 1; k <= 2; k++)
+                for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = velnc[(long)(irx - 1 + k - 1) * A.nnz + (irz - 1 + l - 1)];
+            drx = (rx - A.gox) - (irx - 1) * dnx;
+            drz = (rz - A.goz) - (irz - 1) * dnz;
+            double velr = l_bilinear(vss, drx, drz, dnx, dnz);
+            trr = 2.0 * sred / (vels + velr);
+        } else {
+            double drx = (rx - A.gox) - (irx - 1) * dnx;
+            double drz = (rz - A.goz) - (irz - 1) * dnz;
+            trr = 0.0;
+            for (int k = 1; k <= 2; k++)
+                for (int l = 1; l <= 2; l++) {
+                    double produ = (1.0 - fabs(((l - 1) * dnz - drz) / dnz)) * (1.0 - fabs(((k - 1) * dnx - drx) / dnx));
+                    trr = trr + LT(G, irz - 1 + l, irx - 1 + k) * produ;
+                }
+        }
+        out[r - 1] = trr;
+    }
+}
+
+int mct_launch_fm2d(const mct_fm2d_args &a, long chunk, hipStream_t s)
+{
+    const long nn = (long)a.nnx * a.nnz * a.nchain * a.np;
+    hipLaunchKernelGGL(k_fm2d_gridder, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, a);
+    MCT_CHECK(hipGetLastError());
+    const long nprob = (long)a.nchain * a.np * a.nsrc;
+    if (chunk < 1) return MCT_ERR_ARG;
+    for (long p0 = 0; p0 < nprob; p0 += chunk) {
+        mct_fm2d_args b = a;
+        b.prob0 = p0;
+        b.nrun = nprob - p0 < chunk ? nprob - p0 : chunk;
+        hipLaunchKernelGGL(k_fm2d_solve, dim3((unsigned)((b.nrun + 63) / 64)), dim3(64), 0, s, b);
+        MCT_CHECK(hipGetLastError());
+    }
+    return MCT_OK;
+}
+
+// ---- C ABI: modrays' phase times for one or more periods --------------------------
+extern "C" int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvzd,
+                           int np, const double *velv, int gdx, int gdz, int asgr, int sgdl, int sgs,
+                           int fom, int nsrc, const double *scx, const double *scz, int nrc,
+                           const double *rcx, const double *rcz, const int32_t *srs, double *ttime)
+{
+    if ((long)(nvx - 1) * gdx + 1 > 32767 || (long)(nvz - 1) * gdz + 1 > 32767 || 2L * sgs * sgdl + 1 > 32767)
+        return MCT_ERR_ARG;   // heap entries pack (ix, iz) into 16 bits each
+    if (nvx < 2 || nvz < 2 || np < 1 || gdx < 1 || gdz < 1 || sgdl < 1 || sgs < 1 || nsrc < 1 || nrc < 1 ||
+        !velv || !scx || !scz || !rcx || !rcz || !srs || !ttime || !(dvxd > 0) || !(dvzd > 0))
+        return MCT_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MCT_ERR_HIP;
+    mct_fm2d_args a;
+    memset(&a, 0, sizeof a);
+    a.nvx = nvx; a.nvz = nvz; a.gox = goxd; a.goz = gozd; a.dvx = dvxd; a.dvz = dvzd;
+    a.gdx = gdx; a.gdz = gdz; a.asgr = asgr; a.sgdl = sgdl; a.sgs = sgs; a.fom = fom;
+    a.nnx = (nvx - 1) * gdx + 1; a.nnz = (nvz - 1) * gdz + 1;
+    int rg = 2 * sgs * sgdl + 1;
+    a.ld = a.nnz > rg ? a.nnz : rg;
+    a.nc = a.nnx > rg ? a.nnx : rg;
+    a.nchain = 1; a.np = np; a.nsrc = nsrc; a.nrc = nrc;
+    const long nprob = (long)np * nsrc, wsz = (long)a.ld * a.nc;
+    const size_t nv = (size_t)(nvx + 2) * (nvz + 2) * np, nn = (size_t)a.nnx * a.nnz * np;
+    void *bufs[16] = {nullptr};
+    size_t sizes[16] = {nv * 8, nn * 8, 8 * (size_t)nsrc, 8 * (size_t)nsrc, 8 * (size_t)nrc, 8 * (size_t)nrc,
+                        4 * (size_t)np * nsrc * nrc, 8 * (size_t)(nprob * wsz), 8 * (size_t)(nprob * wsz),
+                        8 * (size_t)(nprob * wsz), 4 * (size_t)(nprob * wsz), 4 * (size_t)(nprob * wsz),
+                        4 * (size_t)(nprob * (wsz + 8)), 8 * (size_t)nprob * nrc, 4, 0};
+    int rc = MCT_OK;
+    for (int i = 0; i < 15 && !rc; i++)
+        if (hipMalloc(&bufs[i], sizes[i]) != hipSuccess) rc = MCT_ERR_NOMEM;
+    if (!rc && (hipMemcpy(bufs[0], velv, nv * 8, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(bufs[2], scx, 8 * (size_t)nsrc, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(bufs[3], scz, 8 * (size_t)nsrc, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(bufs[4], rcx, 8 * (size_t)nrc, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(bufs[5], rcz, 8 * (size_t)nrc, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(bufs[6], srs, 4 * (size_t)np * nsrc * nrc, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(bufs[13], ttime, 8 * (size_t)nprob * nrc, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemset(bufs[14], 0, 4) != hipSuccess))
+        rc = MCT_ERR_HIP;
+    if (!rc) {
+        a.velv = (const double *)bufs[0]; a.veln = (double *)bufs[1];
+        a.scx = (const double *)bufs[2]; a.scz = (const double *)bufs[3];
+        a.rcx = (const double *)bufs[4]; a.rcz = (const double *)bufs[5];
+        a.srs = (const int32_t *)bufs[6];
+        a.ttn = (double *)bufs[7]; a.ttr = (double *)bufs[8]; a.vr = (double *)bufs[9];
+        a.nsts = (int32_t *)bufs[10]; a.nstr = (int32_t *)bufs[11]; a.heap = (int32_t *)bufs[12];
+        a.ttime = (double *)bufs[13]; a.status = (int32_t *)bufs[14];
+        a.srs_pstride = (long)nsrc * nrc;
+        rc = mct_launch_fm2d(a, nprob, 0);
+        if (!rc && hipDeviceSynchronize() != hipSuccess) rc = MCT_ERR_HIP;
+    }
+    int32_t st = 0;
+    if (!rc && (hipMemcpy(ttime, bufs[13], 8 * (size_t)nprob * nrc, hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(&st, bufs[14], 4, hipMemcpyDeviceToHost) != hipSuccess))
+        rc = MCT_ERR_HIP;
+    for (int i = 0; i < 15; i++) if (bufs[i]) (void)hipFree(bufs[i]);
+    if (!rc && st < 0) rc = st;
+    return rc;
+}

@@ -137,12 +137,67 @@ struct mct_misfit_args {
     const double *picks;           // (nchain, np, nsrc_eff, npick)
     const double *ttime;           // (np, 3, nsrc*nrev)  observed
     const int32_t *raystat;        // (np, 2, nsrc*nrev)
-    const double *locations;       // (nchain, nsrc, 4)
+    const double *locations;       // (nchain, nsrc, 4): origin time locations[..][3]; NULL = none
     const double *noise;           // (nchain, 2, np) or NULL
     int sigdep, nrays;
-    double *out;                   // (nchain, 3)
+    double eps;                    // smallest admissible sigma
+    double *out;                   // (nchain, 3); like = DBL_MAX for a chain whose status is > 0
     double *phase_time;            // (nchain, np, nsrc, nrev) or NULL
     double *srdist, *sigma;        // (nchain, np, nsrc*nrev) or NULL
     int32_t *status;               // (nchain) or NULL
 };
 int mct_launch_misfit(const mct_misfit_args &a, hipStream_t s);
+
+// ---- 2-D phase traveltimes (fm2d) -------------------------------------------------
+struct mct_fm2d_args {
+    int nvx, nvz;               // control grid nodes (grid nx, ny)
+    double gox, goz, dvx, dvz;  // origin and control spacing
+    int gdx, gdz, asgr, sgdl, sgs, fom;
+    int nnx, nnz;               // coarse (B-spline) grid
+    int ld, nc;                 // working arrays: ld rows (z) x nc columns (x)
+    int nchain, np, nsrc, nrc;
+    const double *velv;         // (nchain*np, (nvx+2)*(nvz+2)) padded control grids, Fortran (nvz+2,nvx+2)
+    double *veln;               // (nchain*np, nnx*nnz) coarse velocity, Fortran (nnz,nnx)
+    const double *scx, *scz, *rcx, *rcz;
+    const int32_t *srs;         // per period srs_pstride entries; the first nsrc*nrc mark the pairs used (receiver fastest)
+    long srs_pstride;
+    double *ttn, *ttr;          // (nprob, ld*nc) coarse/refined times (ttr = refined copy)
+    double *vr;                 // (nprob, ld*nc) refined velocity
+    int32_t *nsts, *nstr;       // (nprob, ld*nc)
+    int32_t *heap;              // (nprob, ld*nc+8) packed (px<<16)|pz
+    double *ttime;              // (nchain, np, nsrc, nrc): phase times, receiver fastest
+    int32_t *status;            // (nchain): MCT_ERR_OUTSIDE for a source or receiver outside the
+                                // grid; chains whose status is > 0 (rejected) are skipped
+    long prob0, nrun;           // problems [prob0, prob0+nrun) use workspace slots 0..nrun-1
+};
+// Coarse B-spline velocity per (chain, period), then one lane per
+// (chain, period, source) running the sequential modrays phase solve.
+// Workspace slots (ttn..heap) are sized for `chunk` problems; problems run in
+// chunks of that many.
+int mct_launch_fm2d(const mct_fm2d_args &a, long chunk, hipStream_t s);
+// Bytes of fm2d workspace per problem slot.
+static inline long mct_fm2d_slot_bytes(long ld, long nc)
+{
+    return ld * nc * (8 + 8 + 8 + 4 + 4) + 4 * (ld * nc + 8);
+}
+
+// ---- dispersion (surfdisp96 per column) ------------------------------------------
+struct mct_dsp_args {
+    int ncol;                     // columns per chain (nx*ny, Fortran order j + ny*i)
+    int nz;
+    int nchain;
+    const double *vs;             // (nchain, nx, ny, nz) Fortran (nz,ny,nx) per chain
+    double dz, scaling;           // grid%dz (scaled) and grid%scaling
+    const double *periods;        // (np) periods in s (1/freqs)
+    int np;
+    int iwave;                    // 1 Love, 2 Rayleigh
+    int igr;                      // 0 phase, 1 group
+    double dphase;
+    double *phase, *group;        // (nchain, ncol, np)
+    int32_t *ierr;                // (nchain, ncol)
+    int32_t *bad_model;           // (nchain): check_model failed
+    int32_t *ndltar;              // (nchain, ncol) or NULL: period-equation evaluations
+};
+// One wave per (column, chain): check_model, convert_to_layer, surfdisp96.
+// Requires np <= 60 and nz <= 100.
+int mct_launch_dispersion(const mct_dsp_args &a, hipStream_t s);

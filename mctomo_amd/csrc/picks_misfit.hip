@@ -95,7 +95,11 @@ __global__ __launch_bounds__(MIS_BLOCK) void k_misfit(mct_misfit_args a)
     const int c = blockIdx.x;
     const long npair = (long)a.nsrc * a.nrev;
     const double PI2 = (double)6.283185f;   // default-real literal (src/likelihood_body.F90:32)
-    const double EPS = DBL_MIN * 10.0;      // tiny(0d0)*10 (:33)
+    const double EPS = a.eps;
+    if (a.status && a.status[c] > 0) {      // rejected model: like = huge(1d0)
+        if (threadIdx.x == 0) { a.out[c * 3 + 0] = DBL_MAX; a.out[c * 3 + 1] = 0.0; a.out[c * 3 + 2] = 0.0; }
+        return;
+    }
     if (threadIdx.x == 0) fatal = 0;
     __syncthreads();
     double like = 0.0, misfit = 0.0, unw = 0.0, slog = 0.0;
@@ -112,7 +116,7 @@ __global__ __launch_bounds__(MIS_BLOCK) void k_misfit(mct_misfit_args a)
             double sg = a.sigdep ? b0 * t + b1 : a.ttime[((long)i * 3 + 1) * npair + nrr];
             if (a.raystat[((long)i * 2 + 0) * npair + nrr] == 1) {
                 if (sg < EPS) fatal = 1;
-                double r = t + a.locations[((long)c * a.nsrc + j) * 4 + 3] - a.ttime[((long)i * 3 + 0) * npair + nrr];
+                double r = t + (a.locations ? a.locations[((long)c * a.nsrc + j) * 4 + 3] : 0.0) - a.ttime[((long)i * 3 + 0) * npair + nrr];
                 like = like + (r * r) / (2 * (sg * sg));
                 misfit = misfit + (r * r) / (sg * sg);
                 unw = unw + r * r;

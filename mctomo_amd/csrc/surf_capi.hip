@@ -1,0 +1,369 @@
+// surf_capi.hip -- surface-wave likelihood context (datatype 2, phase
+// velocities) behind the C ABI declared in include/mctomo_hip.h.
+//
+// One call is the reference's likelihood() for datatype 2 (src/likelihood.f90
+// :72-80 -> src/likelihood_surf.F90:110-405) over the whole grid, for one
+// chain (host arrays) or many chains (device arrays), all on the context's
+// stream:
+//   nearest site -> vs grid (Fortran (nz,ny,nx) per chain)
+//   k_dispersion: check_model, convert_to_layer, surfdisp96 per column
+//   k_surf_status: reject chains with check_model / dispersion failures
+//   k_surf_pad: like%vel(np, ny+2, nx+2) with replicated edges
+//   fm2d: B-spline gridder + one lane per (chain, period, source) modrays solve
+//   misfit: srdist = phaseTime, sigma, Gaussian misfit
+#include "mct_internal.h"
+#include <float.h>
+#include <stdlib.h>
+#include <string.h>
+#include <new>
+
+namespace {
+
+struct sbuf {
+    void *p = nullptr;
+    size_t n = 0;
+    int alloc(size_t bytes)
+    {
+        if (bytes == 0) bytes = 16;
+        if (n >= bytes) return MCT_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr; n = 0;
+        if (hipMalloc(&p, bytes) != hipSuccess) { p = nullptr; return MCT_ERR_NOMEM; }
+        n = bytes;
+        return MCT_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr; n = 0;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+// status[c] = MCT_REJECT_MODEL when check_model failed or any column's
+// surfdisp96 returned ierr (src/likelihood_surf.F90:161-164,212-224)
+__global__ void k_surf_status(const int32_t *bad_model, const int32_t *ierr, int ncol, int32_t *status)
+{
+    const int c = blockIdx.x;
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = bad_model[c];
+    __syncthreads();
+    int mine = 0;
+    for (int i = threadIdx.x; i < ncol; i += blockDim.x) mine |= ierr[(long)c * ncol + i];
+    if (mine) atomicOr(&bad, 1);
+    __syncthreads();
+    if (threadIdx.x == 0 && bad && status[c] == 0) status[c] = MCT_REJECT_MODEL;
+}
+
+// like%vel(:, iy0+1:iy1+1, ix0+1:ix1+1) = pvel with the edge copies of
+// src/likelihood_surf.F90:259-264 for the whole grid: node (ix, iy) of the
+// padded grid takes column (clamp(ix,1,nx), clamp(iy,1,ny)).
+__global__ void k_surf_pad(const double *phase, int nx, int ny, int np, int nchain, double *velv)
+{
+    const long per = (long)(nx + 2) * (ny + 2);
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per * np * nchain) return;
+    const long cp = t / per;
+    const int node = (int)(t - cp * per);
+    const int c = (int)(cp / np), p = (int)(cp % np);
+    const int ix = node / (ny + 2), iy = node % (ny + 2);
+    const int cx = ix < 1 ? 1 : (ix > nx ? nx : ix), cy = iy < 1 ? 1 : (iy > ny ? ny : iy);
+    const long col = (long)(cx - 1) * ny + (cy - 1);
+    velv[t] = phase[((long)c * nx * ny + col) * np + p];
+}
+
+} // namespace
+
+struct mct_surf_ctx {
+    mct_geom G;
+    mct_surf_setup S;
+    mct_fm2d_args F;                 // geometry part of the 2-D solve
+    long fm2d_chunk;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    double ms[4] = {0, 0, 0, 0};
+    // immutable data
+    sbuf periods, scx, scz, rcx, rcz, ttime, raystat;
+    // per-chain inputs staged by the host entry point
+    sbuf points, params, noise, out, status;
+    // stage buffers
+    sbuf vs, flag_count, flag_nodes, kd_nodes, kd_ind, kd_rdata;
+    sbuf phase, group, ierr, bad_model, ndltar, velv, veln, phase_time, srdist, sigma;
+    sbuf ttn, ttr, vr, nsts, nstr, heap;
+    double *h_in = nullptr, *h_out = nullptr;
+    int32_t *h_status = nullptr;
+};
+
+static int surf_pipeline(mct_surf_ctx *C, int nchain, const double *d_points, const double *d_params,
+                         const int32_t *d_ncells, int ncells_const, const double *d_noise,
+                         double *d_out, int32_t *d_status, bool want_arrays)
+{
+    const mct_geom &G = C->G;
+    const mct_surf_setup &S = C->S;
+    hipStream_t s = C->stream;
+    const int ncol = G.nx * G.ny;
+    int rc;
+    MCT_CHECK(hipEventRecord(C->ev[0], s));
+    // (a) nearest site -> vs
+    mct_nearest_args na;
+    memset(&na, 0, sizeof na);
+    na.G = G; na.nchain = nchain; na.max_cells = S.max_cells;
+    na.points = d_points; na.params = d_params; na.ncells = d_ncells; na.ncells_const = ncells_const;
+    na.ix0 = 1; na.ix1 = G.nx; na.iy0 = 1; na.iy1 = G.ny; na.iz0 = 1; na.iz1 = G.nz;
+    na.vs = C->vs.as<double>(); na.nwave = 1;
+    na.flag_count = C->flag_count.as<int32_t>(); na.flag_nodes = C->flag_nodes.as<int32_t>();
+    na.kd_nodes = C->kd_nodes.as<mct_kd_node>(); na.kd_ind = C->kd_ind.as<int32_t>();
+    na.kd_rdata = C->kd_rdata.as<double>();
+    if ((rc = mct_launch_nearest(na, s))) return rc;
+    MCT_CHECK(hipEventRecord(C->ev[1], s));
+    // (b) dispersion per column
+    mct_dsp_args da;
+    memset(&da, 0, sizeof da);
+    da.ncol = ncol; da.nz = G.nz; da.nchain = nchain; da.vs = C->vs.as<double>();
+    da.dz = G.dz; da.scaling = G.scaling; da.periods = C->periods.as<double>(); da.np = S.np;
+    da.iwave = S.raylov == 1 ? 2 : 1; da.igr = 0; da.dphase = S.dphase;
+    da.phase = C->phase.as<double>(); da.group = C->group.as<double>(); da.ierr = C->ierr.as<int32_t>();
+    da.bad_model = C->bad_model.as<int32_t>(); da.ndltar = C->ndltar.as<int32_t>();
+    if ((rc = mct_launch_dispersion(da, s))) return rc;
+    hipLaunchKernelGGL(k_surf_status, dim3(nchain), dim3(256), 0, s, (const int32_t *)da.bad_model,
+                       (const int32_t *)da.ierr, ncol, d_status);
+    MCT_CHECK(hipGetLastError());
+    MCT_CHECK(hipEventRecord(C->ev[2], s));
+    // (c) padded phase-velocity grids, then modrays per (chain, period, source)
+    {
+        const long n = (long)(G.nx + 2) * (G.ny + 2) * S.np * nchain;
+        hipLaunchKernelGGL(k_surf_pad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           (const double *)da.phase, G.nx, G.ny, S.np, nchain, C->velv.as<double>());
+        MCT_CHECK(hipGetLastError());
+    }
+    mct_fm2d_args fa = C->F;
+    fa.nchain = nchain;
+    fa.velv = C->velv.as<double>(); fa.veln = C->veln.as<double>();
+    fa.scx = C->scx.as<double>(); fa.scz = C->scz.as<double>();
+    fa.rcx = C->rcx.as<double>(); fa.rcz = C->rcz.as<double>();
+    fa.srs = C->raystat.as<int32_t>(); fa.srs_pstride = 2L * S.nsrc * S.nrev;
+    fa.ttn = C->ttn.as<double>(); fa.ttr = C->ttr.as<double>(); fa.vr = C->vr.as<double>();
+    fa.nsts = C->nsts.as<int32_t>(); fa.nstr = C->nstr.as<int32_t>(); fa.heap = C->heap.as<int32_t>();
+    fa.ttime = C->phase_time.as<double>(); fa.status = d_status;
+    if ((rc = mct_launch_fm2d(fa, C->fm2d_chunk, s))) return rc;
+    MCT_CHECK(hipEventRecord(C->ev[3], s));
+    // (d) misfit: srdist = phaseTime (src/likelihood_surf.F90:329-330)
+    mct_misfit_args ma;
+    memset(&ma, 0, sizeof ma);
+    ma.nchain = nchain; ma.nsrc = S.nsrc; ma.nrev = S.nrev; ma.np = S.np; ma.reciprocal = 0;
+    ma.picks = C->phase_time.as<double>(); ma.ttime = C->ttime.as<double>();
+    ma.raystat = C->raystat.as<int32_t>(); ma.locations = nullptr; ma.noise = d_noise;
+    ma.sigdep = S.sigdep; ma.nrays = S.nrays;
+    ma.eps = (double)1.0e-10f;   // EPS, src/likelihood_surf.F90:37
+    ma.out = d_out; ma.status = d_status;
+    ma.srdist = want_arrays ? C->srdist.as<double>() : nullptr;   // chain 0 only
+    ma.sigma = want_arrays ? C->sigma.as<double>() : nullptr;
+    if ((rc = mct_launch_misfit(ma, s))) return rc;
+    MCT_CHECK(hipEventRecord(C->ev[4], s));
+    return MCT_OK;
+}
+
+extern "C" {
+
+int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_ctx **out)
+{
+    if (!grid || !setup || !out) return MCT_ERR_ARG;
+    *out = nullptr;
+    const mct_surf_setup &S = *setup;
+    if (grid->nx < 2 || grid->ny < 2 || grid->nz < 2 || S.np < 1 || S.np > 60 || !S.freqs ||
+        S.nsrc < 1 || S.nrev < 1 || !S.src || !S.rev || !S.ttime || !S.raystat || S.max_cells < 1 ||
+        S.max_chains < 1 || S.gridx < 1 || S.gridy < 1 || S.sgdic < 1 || S.sgext < 1 || grid->nz > 100)
+        return MCT_ERR_ARG;
+    if (S.raylov != 0 && S.raylov != 1) return MCT_ERR_ARG;
+    if (S.phase_group != 0) return MCT_ERR_ARG;   // group traveltimes need ray tracing
+    const long nnx = (long)(grid->nx - 1) * S.gridx + 1, nnz = (long)(grid->ny - 1) * S.gridy + 1;
+    const long rg = 2L * S.sgext * S.sgdic + 1;
+    if (nnx > 32767 || nnz > 32767 || rg > 32767) return MCT_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MCT_ERR_HIP;
+    mct_surf_ctx *C = new (std::nothrow) mct_surf_ctx();
+    if (!C) return MCT_ERR_NOMEM;
+    C->G = mct_make_geom(grid);
+    C->S = S;
+    C->S.freqs = nullptr; C->S.src = nullptr; C->S.rev = nullptr; C->S.ttime = nullptr; C->S.raystat = nullptr;
+    mct_fm2d_args &F = C->F;
+    memset(&F, 0, sizeof F);
+    F.nvx = grid->nx; F.nvz = grid->ny; F.gox = grid->xmin; F.goz = grid->ymin;
+    F.dvx = grid->dx; F.dvz = grid->dy;
+    F.gdx = S.gridx; F.gdz = S.gridy; F.asgr = S.sgref; F.sgdl = S.sgdic; F.sgs = S.sgext; F.fom = S.order;
+    F.nnx = (int)nnx; F.nnz = (int)nnz;
+    F.ld = (int)(nnz > rg ? nnz : rg); F.nc = (int)(nnx > rg ? nnx : rg);
+    F.np = S.np; F.nsrc = S.nsrc; F.nrc = S.nrev;
+    const int M = S.max_chains, NC = S.max_cells, np = S.np;
+    const long npair = (long)S.nsrc * S.nrev, gsz = (long)grid->nx * grid->ny * grid->nz;
+    const long ncol = (long)grid->nx * grid->ny;
+    const long nprob = (long)M * np * S.nsrc;
+    // fm2d workspace: at most 4 GiB of problem slots, whole waves of 64
+    const long slot = mct_fm2d_slot_bytes(F.ld, F.nc);
+    long chunk = (4L << 30) / slot;
+    chunk = chunk < 64 ? 64 : (chunk / 64) * 64;
+    C->fm2d_chunk = chunk < nprob ? chunk : nprob;
+    int rc = MCT_OK;
+    if (hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking) != hipSuccess) { delete C; return MCT_ERR_HIP; }
+    for (int i = 0; i < 5; i++)
+        if (hipEventCreate(&C->ev[i]) != hipSuccess) { mct_surf_destroy(C); return MCT_ERR_HIP; }
+    const long wsz = (long)F.ld * F.nc, ch = C->fm2d_chunk;
+#define ALLOC(buf, bytes) if (!rc) rc = C->buf.alloc((size_t)(bytes))
+    ALLOC(periods, sizeof(double) * np);
+    ALLOC(scx, sizeof(double) * S.nsrc); ALLOC(scz, sizeof(double) * S.nsrc);
+    ALLOC(rcx, sizeof(double) * S.nrev); ALLOC(rcz, sizeof(double) * S.nrev);
+    ALLOC(ttime, sizeof(double) * 3 * npair * np);
+    ALLOC(raystat, sizeof(int32_t) * 2 * npair * np);
+    ALLOC(points, sizeof(double) * 3 * NC * M);
+    ALLOC(params, sizeof(double) * 3 * NC * M);
+    ALLOC(noise, sizeof(double) * 2 * np * M);
+    ALLOC(out, sizeof(double) * 3 * M);
+    ALLOC(status, sizeof(int32_t) * M);
+    ALLOC(vs, sizeof(double) * gsz * M);
+    ALLOC(flag_count, sizeof(int32_t) * M);
+    ALLOC(flag_nodes, sizeof(int32_t) * MCT_MAX_FLAGGED * M);
+    ALLOC(kd_nodes, sizeof(mct_kd_node) * (2 * NC + 2) * M);
+    ALLOC(kd_ind, sizeof(int32_t) * (NC + 1) * M);
+    ALLOC(kd_rdata, sizeof(double) * 3 * NC * M);
+    ALLOC(phase, sizeof(double) * ncol * np * M);
+    ALLOC(group, sizeof(double) * ncol * np * M);
+    ALLOC(ierr, sizeof(int32_t) * ncol * M);
+    ALLOC(bad_model, sizeof(int32_t) * M);
+    ALLOC(ndltar, sizeof(int32_t) * ncol * M);
+    ALLOC(velv, sizeof(double) * (grid->nx + 2) * (grid->ny + 2) * np * M);
+    ALLOC(veln, sizeof(double) * nnx * nnz * np * M);
+    ALLOC(phase_time, sizeof(double) * npair * np * M);
+    ALLOC(srdist, sizeof(double) * npair * np);
+    ALLOC(sigma, sizeof(double) * npair * np);
+    ALLOC(ttn, sizeof(double) * wsz * ch);
+    ALLOC(ttr, sizeof(double) * wsz * ch);
+    ALLOC(vr, sizeof(double) * wsz * ch);
+    ALLOC(nsts, sizeof(int32_t) * wsz * ch);
+    ALLOC(nstr, sizeof(int32_t) * wsz * ch);
+    ALLOC(heap, sizeof(int32_t) * (wsz + 8) * ch);
+#undef ALLOC
+    if (!rc && hipHostMalloc((void **)&C->h_in, sizeof(double) * (6 * (size_t)NC + 2 * np + 8), hipHostMallocDefault) != hipSuccess)
+        rc = MCT_ERR_NOMEM;
+    if (!rc && hipHostMalloc((void **)&C->h_out, sizeof(double) * 3, hipHostMallocDefault) != hipSuccess) rc = MCT_ERR_NOMEM;
+    if (!rc && hipHostMalloc((void **)&C->h_status, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) rc = MCT_ERR_NOMEM;
+    if (!rc) {
+        // periods = dble(1/freqs) (surfmodes/surfmodes.f90:81-83); source/receiver (x, y)
+        double *per = (double *)malloc(sizeof(double) * np);
+        double *xy = (double *)malloc(sizeof(double) * 2 * (S.nsrc > S.nrev ? S.nsrc : S.nrev));
+        if (!per || !xy) rc = MCT_ERR_NOMEM;
+        if (!rc) {
+            for (int i = 0; i < np; i++) per[i] = 1 / S.freqs[i];
+            if (hipMemcpy(C->periods.p, per, sizeof(double) * np, hipMemcpyHostToDevice) != hipSuccess) rc = MCT_ERR_HIP;
+            for (int j = 0; j < S.nsrc; j++) { xy[j] = S.src[2 * j]; xy[S.nsrc + j] = S.src[2 * j + 1]; }
+            if (!rc && (hipMemcpy(C->scx.p, xy, sizeof(double) * S.nsrc, hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemcpy(C->scz.p, xy + S.nsrc, sizeof(double) * S.nsrc, hipMemcpyHostToDevice) != hipSuccess))
+                rc = MCT_ERR_HIP;
+            for (int k = 0; k < S.nrev; k++) { xy[k] = S.rev[2 * k]; xy[S.nrev + k] = S.rev[2 * k + 1]; }
+            if (!rc && (hipMemcpy(C->rcx.p, xy, sizeof(double) * S.nrev, hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemcpy(C->rcz.p, xy + S.nrev, sizeof(double) * S.nrev, hipMemcpyHostToDevice) != hipSuccess))
+                rc = MCT_ERR_HIP;
+        }
+        free(per); free(xy);
+    }
+    if (!rc && hipMemcpy(C->ttime.p, S.ttime, sizeof(double) * 3 * npair * np, hipMemcpyHostToDevice) != hipSuccess) rc = MCT_ERR_HIP;
+    if (!rc && hipMemcpy(C->raystat.p, S.raystat, sizeof(int32_t) * 2 * npair * np, hipMemcpyHostToDevice) != hipSuccess) rc = MCT_ERR_HIP;
+    if (rc) { mct_surf_destroy(C); return rc; }
+    *out = C;
+    return MCT_OK;
+}
+
+int mct_surf_destroy(mct_surf_ctx *C)
+{
+    if (!C) return MCT_OK;
+    if (C->stream) (void)hipStreamSynchronize(C->stream);
+    sbuf *bufs[] = {&C->periods, &C->scx, &C->scz, &C->rcx, &C->rcz, &C->ttime, &C->raystat,
+                    &C->points, &C->params, &C->noise, &C->out, &C->status, &C->vs, &C->flag_count,
+                    &C->flag_nodes, &C->kd_nodes, &C->kd_ind, &C->kd_rdata, &C->phase, &C->group,
+                    &C->ierr, &C->bad_model, &C->ndltar, &C->velv, &C->veln, &C->phase_time, &C->srdist, &C->sigma,
+                    &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap};
+    for (sbuf *b : bufs) b->release();
+    if (C->h_in) (void)hipHostFree(C->h_in);
+    if (C->h_out) (void)hipHostFree(C->h_out);
+    if (C->h_status) (void)hipHostFree(C->h_status);
+    for (int i = 0; i < 5; i++) if (C->ev[i]) (void)hipEventDestroy(C->ev[i]);
+    if (C->stream) (void)hipStreamDestroy(C->stream);
+    delete C;
+    return MCT_OK;
+}
+
+int mct_surf_sync(mct_surf_ctx *C)
+{
+    if (!C) return MCT_ERR_ARG;
+    MCT_CHECK(hipStreamSynchronize(C->stream));
+    float f;
+    for (int i = 0; i < 4; i++)
+        if (hipEventElapsedTime(&f, C->ev[i], C->ev[i + 1]) == hipSuccess) C->ms[i] = f;
+    return MCT_OK;
+}
+
+int mct_surf_last_timings(mct_surf_ctx *C, double *ms4)
+{
+    if (!C || !ms4) return MCT_ERR_ARG;
+    for (int i = 0; i < 4; i++) ms4[i] = C->ms[i];
+    return MCT_OK;
+}
+
+int mct_surf_likelihood(mct_surf_ctx *C, const double *points, const double *params, int ncells,
+                        const double *snoise0, const double *snoise1, double *out,
+                        double *phase_time, double *srdist, double *sigma, double *phase_vel)
+{
+    if (!C || !points || !params || !out) return MCT_ERR_ARG;
+    if (ncells < 1 || ncells > C->S.max_cells) return MCT_ERR_ARG;
+    if (C->S.sigdep && (!snoise0 || !snoise1)) return MCT_ERR_ARG;
+    hipStream_t s = C->stream;
+    const int np = C->S.np;
+    const long npair = (long)C->S.nsrc * C->S.nrev, ncol = (long)C->G.nx * C->G.ny;
+    MCT_CHECK(hipStreamSynchronize(s));
+    double *hp = C->h_in, *hq = hp + 3 * (size_t)ncells, *hn = hq + 3 * (size_t)ncells;
+    memcpy(hp, points, sizeof(double) * 3 * ncells);
+    memcpy(hq, params, sizeof(double) * 3 * ncells);
+    for (int i = 0; i < 2 * np; i++) hn[i] = 0.0;
+    if (snoise0) for (int i = 0; i < np; i++) { hn[i] = snoise0[i]; hn[np + i] = snoise1[i]; }
+    MCT_CHECK(hipMemcpyAsync(C->points.p, hp, sizeof(double) * 3 * ncells, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemcpyAsync(C->params.p, hq, sizeof(double) * 3 * ncells, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemcpyAsync(C->noise.p, hn, sizeof(double) * 2 * np, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemsetAsync(C->status.p, 0, sizeof(int32_t), s));
+    int rc = surf_pipeline(C, 1, C->points.as<double>(), C->params.as<double>(), nullptr, ncells,
+                           C->noise.as<double>(), C->out.as<double>(), C->status.as<int32_t>(),
+                           srdist || sigma);
+    if (rc) return rc;
+    MCT_CHECK(hipMemcpyAsync(C->h_out, C->out.p, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+    MCT_CHECK(hipMemcpyAsync(C->h_status, C->status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (phase_time) MCT_CHECK(hipMemcpyAsync(phase_time, C->phase_time.p, sizeof(double) * npair * np, hipMemcpyDeviceToHost, s));
+    if (phase_vel) MCT_CHECK(hipMemcpyAsync(phase_vel, C->phase.p, sizeof(double) * ncol * np, hipMemcpyDeviceToHost, s));
+    if (srdist) MCT_CHECK(hipMemcpyAsync(srdist, C->srdist.p, sizeof(double) * npair * np, hipMemcpyDeviceToHost, s));
+    if (sigma) MCT_CHECK(hipMemcpyAsync(sigma, C->sigma.p, sizeof(double) * npair * np, hipMemcpyDeviceToHost, s));
+    MCT_CHECK(hipStreamSynchronize(s));
+    const int st = *C->h_status;
+    for (int i = 0; i < 3; i++) out[i] = C->h_out[i];
+    int src = mct_surf_sync(C);
+    if (src) return src;
+    return st;
+}
+
+int mct_surf_likelihood_batch_dev(mct_surf_ctx *C, int nchain, const double *points_dev,
+                                  const double *params_dev, const int32_t *ncells_dev,
+                                  const double *noise_dev, double *out_dev, int32_t *status_dev)
+{
+    if (!C || nchain < 1 || nchain > C->S.max_chains || !points_dev || !params_dev || !ncells_dev ||
+        !noise_dev || !out_dev || !status_dev)
+        return MCT_ERR_ARG;
+    MCT_CHECK(hipMemsetAsync(status_dev, 0, sizeof(int32_t) * nchain, C->stream));
+    return surf_pipeline(C, nchain, points_dev, params_dev, ncells_dev, 0, noise_dev, out_dev, status_dev,
+                         false);
+}
+
+int mct_surf_dev_arrays(mct_surf_ctx *C, double **phase_time_dev, double **phase_vel_dev)
+{
+    if (!C) return MCT_ERR_ARG;
+    if (phase_time_dev) *phase_time_dev = C->phase_time.as<double>();
+    if (phase_vel_dev) *phase_vel_dev = C->phase.as<double>();
+    return MCT_OK;
+}
+
+} // extern "C"

@@ -67,3 +67,104 @@ def c2_body(seed=20261203, order=0):
     """The 3-D body-wave config: 100 sites, Vp U[4,7] km/s, 101^3 grid at
     0.2 km, 50 sources at depth 2-18 km, 400 surface receivers, P only."""
     return body_workload(101, 0.2, 100, 50, 400, 0, order, seed=seed)
+
+
+@dataclass
+class SurfWorkload:
+    grid: Grid
+    points: np.ndarray      # (ncells, 3)   RTI%points
+    params: np.ndarray      # (ncells, 3)   (vp, vs, rho); only vs is used (datatype 2)
+    freqs: np.ndarray       # (np,) Hz, ascending period
+    src: np.ndarray         # (nsrc, 2)     dat%src(1:2,:)
+    rev: np.ndarray         # (nrev, 2)     dat%rev(1:2,:)
+    ttime: np.ndarray       # (np, 3, nsrc*nrev)  observed (t, sigma, -)
+    raystat: np.ndarray     # (np, 2, nsrc*nrev)  int32
+    raylov: int = 1
+    order: int = 1
+    rejected_draws: int = 0  # site sets redrawn because they failed check_model
+
+    @property
+    def nsrc(self):
+        return self.src.shape[0]
+
+
+def _surface_sites(rng, ncells, L, Z, vs_range):
+    """Sites uniform in the box, Vs uniform in vs_range assigned in order of
+    site depth, so that slower sites lie shallower."""
+    pts = np.column_stack([rng.uniform(0.0, L, ncells), rng.uniform(0.0, L, ncells),
+                           rng.uniform(0.0, Z, ncells)])
+    vs = np.sort(rng.uniform(*vs_range, ncells))[np.argsort(np.argsort(pts[:, 2]))]
+    vp = 1.73 * vs
+    return pts, np.column_stack([vp, vs, 0.32 * vp + 0.77])
+
+
+def top_is_slowest(grid, points, params):
+    """check_model (src/likelihood_surf.F90:631-646) on the nearest-site grid,
+    from a brute-force argmin of squared distances (no ties in random sites)."""
+    xs = grid.xmin + grid.dx * np.arange(grid.nx)
+    ys = grid.ymin + grid.dy * np.arange(grid.ny)
+    zs = grid.zmin + grid.dz * np.arange(grid.nz)
+    vs = params[:, 1]
+    top = np.full((grid.nx, grid.ny), np.inf)
+    ok = True
+    for k, z in enumerate(zs):
+        d = ((xs[:, None, None] - points[None, None, :, 0]) ** 2 +
+             (ys[None, :, None] - points[None, None, :, 1]) ** 2) + (z - points[None, None, :, 2]) ** 2
+        v = vs[np.argmin(d, axis=2)]
+        if k == 0:
+            top = v
+        elif np.any(v < top):
+            ok = False
+            break
+    return ok
+
+
+def surface_workload(n=41, nz=21, L=20.0, Z=10.0, ncells=30, np_=10, periods=(1.0, 5.0),
+                     nsrc=16, nrev=16, vs_range=(2.5, 4.5), sigma=0.05, raylov=1, order=1,
+                     seed=20261202, ttime=None):
+    """Surface-wave likelihood inputs: ncells Voronoi sites in an L x L x Z km
+    box rasterised to n x n x nz nodes, np_ periods log-spaced in `periods`
+    (freqs = 1/T, ascending T), nsrc sources and nrev receivers uniform at the
+    surface at least one cell inside the model, every pair used.  Site sets
+    that fail check_model are redrawn (counted in rejected_draws).  Observed
+    times default to straight-ray times at 2.9 km/s plus Gaussian noise."""
+    rng = np.random.default_rng(seed)
+    h, hz = L / (n - 1), Z / (nz - 1)
+    grid = Grid(n, n, nz, 0.0, 0.0, 0.0, h, h, hz, 1.0)
+    rejected = 0
+    while True:
+        points, params = _surface_sites(rng, ncells, L, Z, vs_range)
+        if top_is_slowest(grid, points, params):
+            break
+        rejected += 1
+        if rejected > 100:
+            raise RuntimeError("no site set passed check_model")
+    T = np.geomspace(periods[0], periods[1], np_)
+    freqs = 1.0 / T
+    src = rng.uniform(h, L - h, (nsrc, 2))
+    rev = rng.uniform(h, L - h, (nrev, 2))
+    npair = nsrc * nrev
+    tt = np.zeros((np_, 3, npair))
+    if ttime is None:
+        d = np.linalg.norm(src[:, None, :] - rev[None, :, :], axis=2).reshape(-1)
+        for i in range(np_):
+            tt[i, 0] = d / 2.9 + rng.normal(0.0, sigma, npair)
+    else:
+        tt[:, 0] = ttime
+    tt[:, 1] = sigma
+    rs = np.ones((np_, 2, npair), np.int32)
+    return SurfWorkload(grid, points, params, freqs, src, rev, tt, rs, raylov, order, rejected)
+
+
+def c1_surface(seed=20261202, order=1):
+    """The plain surface-wave config: 30 sites in 20x20x10 km, Vs U[2.5,4.5],
+    41x41x21 grid, Rayleigh phase velocity at 10 periods 1-5 s, 16 sources
+    and 16 receivers, sigma 0.05 s."""
+    return surface_workload(41, 21, 20.0, 10.0, 30, 10, (1.0, 5.0), 16, 16, seed=seed, order=order)
+
+
+def surface_201(seed=20261206, order=1, ncells=200, np_=20, nsrc=16, nrev=16):
+    """The 201x201x61 surface-wave grid of the served mix: 100 x 100 x 30 km
+    at 0.5 km, Rayleigh phase velocity at np_ periods 1-10 s."""
+    return surface_workload(201, 61, 100.0, 30.0, ncells, np_, (1.0, 10.0), nsrc, nrev, seed=seed,
+                            order=order)

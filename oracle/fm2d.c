@@ -528,9 +528,12 @@ int orc_fm2d(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvz
             if (sw == 1) {
                 double vss[2][2];
                 /* the reference reads veln(isz-1+l, isx-1+k) with floor() indices and
-                 * no clamp; a source on the last grid line would read past the grid,
-                 * so the cell index is clamped here (no effect anywhere else) */
+                 * no clamp; a source on the last grid line, or just below the origin
+                 * (accepted by travel's truncation), would read outside the grid, so
+                 * the cell index is clamped here (no effect anywhere else) */
                 int csx = jsx < nnx ? jsx : nnx - 1, csz = jsz < nnz ? jsz : nnz - 1;
+                csx = csx < 1 ? 1 : csx;
+                csz = csz < 1 ? 1 : csz;
                 for (int k = 1; k <= 2; k++)
                     for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = VN(&S, csz - 1 + l, csx - 1 + k);
                 double drx = (x - gox) - (jsx - 1) * dnx, drz = (z - goz) - (jsz - 1) * dnz;

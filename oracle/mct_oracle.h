@@ -103,6 +103,27 @@ int orc_body_misfit(int nsrc, int nrev, int np, const double *phase_time,
                     const double *bnoise0, const double *bnoise1, int nrays,
                     double *srdist, double *sigma, double *out);
 
+
+/* Surface-wave likelihood (datatype 2, phase velocities), surf.c.  Grid as
+ * T_GRID (dz scaled, scaling = grid%scaling); points/params (ncells,3);
+ * freqs (np) Hz; raylov 1 Rayleigh, 0 Love; src (nsrc,2), rev (nrev,2) as
+ * (x, y); ttime (np,3,nsrc*nrev) and raystat (np,2,nsrc*nrev) as T_DATA;
+ * snoise0/1 (np).  Outputs: out {like, misfit, unweighted}, phase_time
+ * (np,nsrc,nrev), srdist and sigma (np,nsrc*nrev), phase_vel (nx,ny,np) (may
+ * be NULL), sites_id (nx,ny,nz) (may be NULL).  Returns 0, 1 (model
+ * rejected: check_model or a dispersion failure), or < 0 (fatal). */
+int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double zmin,
+                        double dx, double dy, double dz, double scaling,
+                        const double *points, const double *params, int ncells,
+                        int np, const double *freqs, int raylov, double dphase,
+                        int nsrc, const double *src, int nrev, const double *rev,
+                        const double *ttime, const int32_t *raystat, int sigdep,
+                        const double *snoise0, const double *snoise1, int nrays,
+                        int gridx, int gridy, int sgref, int sgdic, int sgext, int order,
+                        double band, int nthreads,
+                        double *out, double *phase_time, double *srdist, double *sigma,
+                        double *phase_vel, int32_t *sites_id);
+
 #ifdef __cplusplus
 }
 #endif

@@ -216,3 +216,43 @@ def fm2d(velv, goxd, gozd, dvxd, dvzd, scx, scz, rcx, rcz, srs=None, gdx=1, gdz=
     if rc:
         raise RuntimeError("orc_fm2d status %d" % rc)
     return (tt, tf) if field else tt
+
+
+# ---------------------------------------------------------------- surface likelihood
+def surf_likelihood(grid, points, params, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3,
+                    sigdep=0, snoise0=None, snoise1=None, nrays=None, gridx=1, gridy=1, sgref=1,
+                    sgdic=4, sgext=8, order=1, band=0.5, nthreads=0):
+    """orc_surf_likelihood.  grid: dict nx..dz (+ scaling).  Returns (rc, out(3),
+    phase_time (np,nsrc,nrev), srdist, sigma (np,npair), phase_vel (nx,ny,np),
+    sites_id (nx,ny,nz))."""
+    nx, ny, nz = grid["nx"], grid["ny"], grid["nz"]
+    pts, prm = c64(points), c64(params)
+    fr = c64(freqs).reshape(-1)
+    npp = fr.shape[0]
+    s, r = c64(src).reshape(-1, 2), c64(rev).reshape(-1, 2)
+    nsrc, nrev = s.shape[0], r.shape[0]
+    npair = nsrc * nrev
+    tt = c64(ttime).reshape(npp, 3, npair)
+    rs = np.ascontiguousarray(raystat, dtype=np.int32).reshape(npp, 2, npair)
+    if nrays is None:
+        nrays = int((rs[:, 0, :] == 1).sum())
+    s0 = c64(np.zeros(npp) if snoise0 is None else np.broadcast_to(snoise0, (npp,)))
+    s1 = c64(np.zeros(npp) if snoise1 is None else np.broadcast_to(snoise1, (npp,)))
+    out = np.zeros(3)
+    pt = np.zeros((npp, nsrc, nrev))
+    sd = np.zeros((npp, npair))
+    sg = np.zeros((npp, npair))
+    pv = np.zeros((nx, ny, npp))
+    sid = np.zeros((nx, ny, nz), np.int32)
+    L = lib()
+    rc = L.orc_surf_likelihood(
+        ctypes.c_int(nx), ctypes.c_int(ny), ctypes.c_int(nz),
+        ctypes.c_double(grid["xmin"]), ctypes.c_double(grid["ymin"]), ctypes.c_double(grid["zmin"]),
+        ctypes.c_double(grid["dx"]), ctypes.c_double(grid["dy"]), ctypes.c_double(grid["dz"]),
+        ctypes.c_double(grid.get("scaling", 1.0)), _p(pts), _p(prm), ctypes.c_int(pts.shape[0]),
+        ctypes.c_int(npp), _p(fr), ctypes.c_int(raylov), ctypes.c_double(dphase),
+        ctypes.c_int(nsrc), _p(s), ctypes.c_int(nrev), _p(r), _p(tt), _p(rs), ctypes.c_int(sigdep),
+        _p(s0), _p(s1), ctypes.c_int(nrays), ctypes.c_int(gridx), ctypes.c_int(gridy),
+        ctypes.c_int(sgref), ctypes.c_int(sgdic), ctypes.c_int(sgext), ctypes.c_int(order),
+        ctypes.c_double(band), ctypes.c_int(nthreads), _p(out), _p(pt), _p(sd), _p(sg), _p(pv), _p(sid))
+    return rc, out, pt, sd, sg, pv, sid

@@ -1,0 +1,158 @@
+/*
+ * surf.c -- oracle for the surface-wave likelihood (datatype 2, phase
+ * velocities).  TEST INFRASTRUCTURE ONLY (see mct_oracle.h).
+ *
+ * Composes the restated stages in the order of likelihood (src/likelihood.f90
+ * :72-80) and surf_likelihood (src/likelihood_surf.F90:110-405):
+ *   kdtree_to_grid (nearest.c) -> vs2vp_3d / vp2rho_3d (src/utils.f90:102-134)
+ *   -> check_model (:631-646) -> convert_to_layer (:523-629) -> surfmodes per
+ *   column (surfdisp.c) -> ierr reject (:212-224) -> like%vel padding
+ *   (:259-264) -> modrays per period (fm2d.c) -> srdist = phaseTime (:329-330)
+ *   -> sigma (:356-372) -> misfit (:374-404).
+ * OpenMP runs over columns and over periods, as the reference does.
+ */
+#include "mct_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define SURF_EPS ((double)1.0e-10f)      /* src/likelihood_surf.F90:37, default-real literal */
+#define SURF_PI2 ((double)6.283185f)     /* :36 */
+#define SURF_NMAX 100                    /* :30 */
+
+/* convert_to_layer for one column (no water layer, no vbnd): vs(nz) -> layers.
+ * vp and rho follow vs2vp_3d / vp2rho_3d.  Returns the layer count. */
+static int column_layers(const double *vs, int nz, double dz, double scaling,
+                         double *thick, double *vp, double *vsl, double *rho)
+{
+    const double POISSON = (double)1.730f;
+    int nl = 0, last_k = 1;
+    double last_vs = vs[0];
+    for (int k = 2; k <= nz; k++) {
+        if (fabs(vs[k - 1] - last_vs) > SURF_EPS) {
+            vsl[nl] = last_vs;
+            vp[nl] = last_vs * POISSON;
+            rho[nl] = (double)1.74f * pow(vp[nl], 0.25);
+            thick[nl] = ((double)(k - last_k) * dz) / scaling;
+            nl++;
+            last_vs = vs[k - 1];
+            last_k = k;
+        }
+    }
+    /* the last layer is the half-space (thickness 0) with the bottom node's values */
+    vsl[nl] = vs[nz - 1];
+    vp[nl] = vs[nz - 1] * POISSON;
+    rho[nl] = (double)1.74f * pow(vp[nl], 0.25);
+    thick[nl] = 0.0;
+    return nl + 1;
+}
+
+int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double zmin,
+                        double dx, double dy, double dz, double scaling,
+                        const double *points, const double *params, int ncells,
+                        int np, const double *freqs, int raylov, double dphase,
+                        int nsrc, const double *src, int nrev, const double *rev,
+                        const double *ttime, const int32_t *raystat, int sigdep,
+                        const double *snoise0, const double *snoise1, int nrays,
+                        int gridx, int gridy, int sgref, int sgdic, int sgext, int order,
+                        double band, int nthreads,
+                        double *out, double *phase_time, double *srdist, double *sigma,
+                        double *phase_vel, int32_t *sites_id)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const long G = (long)nx * ny * nz, C = (long)nx * ny, npair = (long)nsrc * nrev;
+    double *vp = malloc(sizeof(double) * G), *vs = malloc(sizeof(double) * G), *rho = malloc(sizeof(double) * G);
+    int32_t *sid = sites_id ? sites_id : malloc(sizeof(int32_t) * G);
+    for (long i = 0; i < G; i++) { vp[i] = 0; vs[i] = 0; rho[i] = 0; }
+    orc_kdtree_to_grid(nx, ny, nz, xmin, ymin, zmin, dx, dy, dz, points, params, ncells,
+                       NULL, NULL, sid, vp, vs, rho);
+    int rc = 0;
+    /* check_model: any column whose deeper vs is below the top vs */
+    for (long c = 0; c < C && !rc; c++)
+        for (int k = 1; k < nz; k++)
+            if (vs[c * nz + k] < vs[c * nz]) { rc = 1; break; }
+    double *pvel = phase_vel ? phase_vel : malloc(sizeof(double) * C * np);
+    for (long i = 0; i < C * np; i++) pvel[i] = 100.0;
+    if (!rc) {
+        int bad = 0;
+        /* columns in Fortran order: c = iy + ny*ix; pvel (np, ny, nx) */
+        #pragma omp parallel for schedule(dynamic, 16) reduction(|:bad)
+        for (long c = 0; c < C; c++) {
+            double th[SURF_NMAX + 1], a[SURF_NMAX + 1], b[SURF_NMAX + 1], r[SURF_NMAX + 1];
+            double grp[64];
+            int nl = column_layers(vs + c * nz, nz, dz, scaling, th, a, b, r);
+            int ierr = orc_surfmodes(th, a, b, r, nl, freqs, np, raylov, 0, dphase,
+                                     pvel + c * np, grp, NULL);
+            if (ierr) bad = 1;
+        }
+        if (bad) rc = 1;
+    }
+    if (!rc) {
+        /* like%vel(np, ny+2, nx+2) with replicated edges; per period a Fortran
+         * velv(0:ny+1, 0:nx+1) grid */
+        const int ldv = ny + 2;
+        double *velv = malloc(sizeof(double) * (size_t)np * ldv * (nx + 2));
+        for (int p = 0; p < np; p++)
+            for (int ix = 0; ix < nx + 2; ix++)
+                for (int iy = 0; iy < ny + 2; iy++) {
+                    int cx = ix < 1 ? 1 : (ix > nx ? nx : ix), cy = iy < 1 ? 1 : (iy > ny ? ny : iy);
+                    velv[((size_t)p * (nx + 2) + ix) * ldv + iy] = pvel[((long)(cx - 1) * ny + (cy - 1)) * np + p];
+                }
+        double *scx = malloc(sizeof(double) * nsrc), *scz = malloc(sizeof(double) * nsrc);
+        double *rcx = malloc(sizeof(double) * nrev), *rcz = malloc(sizeof(double) * nrev);
+        for (int j = 0; j < nsrc; j++) { scx[j] = src[2 * j]; scz[j] = src[2 * j + 1]; }
+        for (int k = 0; k < nrev; k++) { rcx[k] = rev[2 * k]; rcz[k] = rev[2 * k + 1]; }
+        int fatal = 0;
+        #pragma omp parallel for schedule(dynamic, 1) reduction(|:fatal)
+        for (int p = 0; p < np; p++) {
+            int e = orc_fm2d(nx, ny, xmin, ymin, dx, dy, velv + (size_t)p * (nx + 2) * ldv, gridx, gridy,
+                             sgref, sgdic, sgext, order, band, nsrc, scx, scz, nrev, rcx, rcz,
+                             raystat + (long)p * 2 * npair, phase_time + (long)p * npair, NULL);
+            if (e) fatal = 1;
+        }
+        free(velv); free(scx); free(scz); free(rcx); free(rcz);
+        if (fatal) rc = -6;
+    }
+    if (!rc) {
+        /* srdist = phaseTime (phase velocities); sigma; misfit in the reference's loop order */
+        for (long i = 0; i < (long)np * npair; i++) srdist[i] = phase_time[i];
+        for (int i = 0; i < np; i++)
+            for (long r = 0; r < npair; r++) {
+                long o = (long)i * npair + r;
+                if (sigdep)
+                    sigma[o] = raystat[(long)i * 2 * npair + r] == 1 ? snoise0[i] * srdist[o] + snoise1[i] : 1.0;
+                else
+                    sigma[o] = ttime[(long)i * 3 * npair + npair + r];
+            }
+        double like = 0, misfit = 0, unw = 0;
+        for (int i = 0; i < np && !rc; i++)
+            for (long r = 0; r < npair; r++) {
+                long o = (long)i * npair + r;
+                if (raystat[(long)i * 2 * npair + r] == 1) {
+                    if (sigma[o] < SURF_EPS) { rc = -4; break; }
+                    double d = phase_time[o] - ttime[(long)i * 3 * npair + r];
+                    like = like + (d * d) / (2 * (sigma[o] * sigma[o]));
+                    misfit = misfit + (d * d) / (sigma[o] * sigma[o]);
+                    unw = unw + d * d;
+                } else {
+                    sigma[o] = 1.0;
+                }
+            }
+        double slog = 0;
+        for (long i = 0; i < (long)np * npair; i++) slog += log(sigma[i]);
+        out[0] = like + slog + (double)((float)nrays / 2.0f) * log(SURF_PI2);
+        out[1] = misfit;
+        out[2] = unw;
+    }
+    free(vp); free(vs); free(rho);
+    if (!sites_id) free(sid);
+    if (!phase_vel) free(pvel);
+    return rc;
+}

@@ -1,0 +1,133 @@
+"""GPU surface-wave likelihood (datatype 2, phase velocities) against the oracle.
+
+Gates: phase velocities within 1e-6 km/s, traveltimes within 1e-4 relative,
+log-likelihood within 1e-6 relative.  Given the GPU's own phase velocities,
+the 2-D traveltimes must equal the oracle's modrays bit for bit (one lane
+runs the reference's sequential solve).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import mctomo_amd as mct
+from mctomo_amd import workloads as W
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gdict(g):
+    return dict(nx=g.nx, ny=g.ny, nz=g.nz, xmin=g.xmin, ymin=g.ymin, zmin=g.zmin, dx=g.dx,
+                dy=g.dy, dz=g.dz, scaling=g.scaling)
+
+
+def ctx_for(w, **kw):
+    return mct.SurfLikelihood(w.grid, w.freqs, w.src, w.rev, w.ttime, w.raystat, raylov=w.raylov,
+                              order=w.order, **kw)
+
+
+def check_against_oracle(w, got, tag, sigdep=0, s0=None, s1=None):
+    rc, out, pt, sd, sg, pv = got
+    orc = po.surf_likelihood(gdict(w.grid), w.points, w.params, w.freqs, w.src, w.rev, w.ttime,
+                             w.raystat, raylov=w.raylov, order=w.order, sigdep=sigdep,
+                             snoise0=s0, snoise1=s1)
+    assert rc == orc[0] == 0, (rc, orc[0])
+    dv = np.max(np.abs(pv - orc[5]))
+    dt = np.max(np.abs(pt - orc[2]) / orc[2])
+    dl = abs(out[0] - orc[1][0]) / abs(orc[1][0])
+    say(tag, "phase vel max |d| %.3g km/s" % dv, "exact cols %.4f" % np.mean(np.all(pv == orc[5], axis=2)),
+        "time max rel %.3g" % dt, "like rel %.3g" % dl)
+    assert dv <= 1e-6
+    assert dt <= 1e-4
+    assert dl <= 1e-6
+    # given the GPU's phase velocities the traveltimes are the oracle's, bit for bit
+    g = w.grid
+    for p in range(w.freqs.shape[0]):
+        velv = np.pad(pv[:, :, p], 1, mode="edge")
+        tt = po.fm2d(velv, g.xmin, g.ymin, g.dx, g.dy, w.src[:, 0], w.src[:, 1], w.rev[:, 0],
+                     w.rev[:, 1], srs=w.raystat[p, 0], fom=w.order)
+        np.testing.assert_array_equal(pt[p], tt, err_msg="%s period %d" % (tag, p))
+
+
+@pytest.mark.parametrize("order,raylov", [(1, 1), (0, 1), (1, 0)])
+def test_plain_call_matches_oracle(order, raylov):
+    w = W.surface_workload(n=41, nz=21, ncells=30, np_=10, nsrc=16, nrev=16, seed=20261202, order=order)
+    w.raylov = raylov
+    say("surf plain", "order", order, "raylov", raylov)
+    L = ctx_for(w)
+    got = L.likelihood(w.points, w.params, arrays=True)
+    check_against_oracle(w, got, "plain o%d r%d" % (order, raylov))
+    L.close()
+
+
+def test_sigdep_noise_and_unused_pairs():
+    w = W.surface_workload(n=33, nz=17, L=16.0, Z=8.0, ncells=25, np_=7, nsrc=9, nrev=11, seed=5)
+    w.raystat[:, 0, ::4] = 0
+    s0, s1 = np.linspace(0.01, 0.02, 7), np.full(7, 0.03)
+    say("surf sigdep")
+    L = ctx_for(w, sigdep=1)
+    got = L.likelihood(w.points, w.params, snoise0=s0, snoise1=s1, arrays=True)
+    check_against_oracle(w, got, "sigdep", sigdep=1, s0=s0, s1=s1)
+    orc = po.surf_likelihood(gdict(w.grid), w.points, w.params, w.freqs, w.src, w.rev, w.ttime,
+                             w.raystat, order=w.order, sigdep=1, snoise0=s0, snoise1=s1)
+    np.testing.assert_allclose(got[4], orc[4], rtol=1e-12)
+    L.close()
+
+
+def test_rejected_model_reports_status_and_huge_like():
+    w = W.surface_workload(n=21, nz=11, L=10.0, Z=5.0, ncells=15, np_=5, nsrc=4, nrev=4, seed=9)
+    prm = w.params.copy()
+    prm[np.argmax(w.points[:, 2]), 1] = prm[:, 1].min() - 0.3
+    say("surf reject")
+    L = ctx_for(w)
+    rc, out = L.likelihood(w.points, prm)
+    assert rc == mct.REJECT_MODEL
+    assert out[0] == np.finfo(np.float64).max
+    # the context stays usable
+    rc, out = L.likelihood(w.points, w.params)
+    assert rc == 0 and np.isfinite(out[0])
+    L.close()
+
+
+def test_batched_chains_match_plain_calls_and_oracle():
+    base = W.surface_workload(n=33, nz=17, L=16.0, Z=8.0, ncells=40, np_=6, nsrc=8, nrev=8, seed=21)
+    nchain, maxc = 5, 40
+    rng = np.random.default_rng(4)
+    pts = np.zeros((nchain, maxc, 3))
+    prm = np.zeros((nchain, maxc, 3))
+    ncl = np.zeros(nchain, np.int32)
+    for c in range(nchain):
+        k = int(rng.integers(10, maxc + 1))
+        w = W.surface_workload(n=33, nz=17, L=16.0, Z=8.0, ncells=k, np_=6, nsrc=8, nrev=8, seed=100 + c)
+        pts[c, :k], prm[c, :k], ncl[c] = w.points, w.params, k
+    if True:   # one chain with a rejected model in the middle of the batch
+        k = ncl[2]
+        prm[2, int(np.argmax(pts[2, :k, 2])), 1] = prm[2, :k, 1].min() - 0.3
+    L = ctx_for(base, max_cells=maxc, max_chains=nchain)
+    say("surf batch", nchain)
+    dP, dQ = mct.DeviceArray.from_numpy(pts), mct.DeviceArray.from_numpy(prm)
+    dN = mct.DeviceArray.from_numpy(ncl)
+    dS = mct.DeviceArray.from_numpy(np.zeros((nchain, 2, 6)))
+    dO = mct.DeviceArray((nchain, 3), np.float64)
+    dT = mct.DeviceArray((nchain,), np.int32)
+    L.likelihood_batch_dev(nchain, dP.ptr, dQ.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
+    L.sync()
+    out, st = dO.numpy(), dT.numpy()
+    for c in range(nchain):
+        k = ncl[c]
+        rc, o1 = L.likelihood(pts[c, :k], prm[c, :k])
+        assert st[c] == rc, (c, st[c], rc)
+        if rc == 0:
+            np.testing.assert_array_equal(out[c], o1)
+            orc = po.surf_likelihood(gdict(base.grid), pts[c, :k], prm[c, :k], base.freqs, base.src,
+                                     base.rev, base.ttime, base.raystat, order=base.order)
+            assert abs(out[c, 0] - orc[1][0]) <= 1e-6 * abs(orc[1][0])
+        else:
+            assert c == 2 and rc == mct.REJECT_MODEL
+    L.close()

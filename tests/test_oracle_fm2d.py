@@ -37,6 +37,29 @@ def test_matches_compiled_reference_live():
             np.testing.assert_array_equal(a, b)
 
 
+def test_matches_compiled_reference_grid_options():
+    """B-spline dicing (gridx/gridy > 1), refined-grid dicing/extent and the
+    refined grid switched off, against the reference's own Fortran."""
+    if po.ref() is None:
+        pytest.skip("oracle/_ref not built here")
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        gdx, gdz = int(rng.integers(1, 4)), int(rng.integers(1, 4))
+        sgdl, sgs = int(rng.integers(2, 6)), int(rng.integers(2, 9))
+        nx, ny, h = int(rng.integers(8, 25)), int(rng.integers(8, 25)), 0.6
+        velv = rng.uniform(2.0, 4.5, (nx + 2, ny + 2))
+        L, W = (nx - 1) * h, (ny - 1) * h
+        ns, nr = int(rng.integers(1, 5)), int(rng.integers(1, 8))
+        sx, sz = rng.uniform(h, L - h, ns) - 3.0, rng.uniform(h, W - h, ns) + 1.5
+        rx, rz = rng.uniform(0, L, nr) - 3.0, rng.uniform(0, W, nr) + 1.5
+        for fom in (0, 1):
+            for asgr in (0, 1):
+                kw = dict(gdx=gdx, gdz=gdz, sgdl=sgdl, sgs=sgs, fom=fom, asgr=asgr)
+                a = po.fm2d(velv, -3.0, 1.5, h, h, sx, sz, rx, rz, **kw)
+                b = po.fm2d(velv, -3.0, 1.5, h, h, sx, sz, rx, rz, ref=True, **kw)
+                np.testing.assert_array_equal(a, b, err_msg=str(kw))
+
+
 @pytest.mark.parametrize("fom", [0, 1])
 def test_homogeneous_medium_times_converge_to_distance_over_velocity(fom):
     v = 3.0
