@@ -75,14 +75,18 @@ int mct_surfdisp96_batch(int nmod, int nl_max, const int32_t *nlayer, const floa
  * (LIKE_SET%gridx/gridy, sgref, sgdic, sgext, order).  scx/scz (nsrc),
  * rcx/rcz (nrc); srs (np, nsrc, nrc) marks the pairs whose time is computed
  * (receiver fastest); ttime (np, nsrc, nrc) receives their times, other
- * entries are left as passed in.  One lane solves one (period, source) with
- * the reference's sequential algorithm, so the times are the reference's
- * bit for bit.  Returns MCT_ERR_OUTSIDE where modrays would stop on a source
- * or receiver outside the grid. */
+ * entries are left as passed in.  method 1: one lane solves one (period,
+ * source) with the reference's sequential heap loop.  method 0: the heap loop
+ * only where its history shows (the source seed, the refined grid's stop and
+ * the coarse restart around the source), the rest of the grid by parallel
+ * sweeps to the same fixed point; problems that do not fit fall back to
+ * method 1.  Returns MCT_ERR_OUTSIDE where modrays would stop on a source or
+ * receiver outside the grid. */
 int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvzd,
                 int np, const double *velv, int gdx, int gdz, int asgr, int sgdl, int sgs,
                 int fom, int nsrc, const double *scx, const double *scz, int nrc,
-                const double *rcx, const double *rcz, const int32_t *srs, double *ttime);
+                const double *rcx, const double *rcz, const int32_t *srs, double *ttime,
+                int method);
 
 /* ---- (c) 3-D eikonal: link-level drop-in ---------------------------------- */
 /* Same symbol, signature and semantics as the reference C ABI
@@ -230,6 +234,10 @@ int mct_surf_sync(mct_surf_ctx *ctx);
 /* Stage times of the last synchronised call (ms): nearest site, dispersion,
  * 2-D traveltimes, misfit. */
 int mct_surf_last_timings(mct_surf_ctx *ctx, double *ms4);
+
+/* 2-D traveltime method: 0 parallel sweeps with the heap loop near the
+ * source (default), 1 the sequential heap loop per (period, source). */
+int mct_surf_set_method(mct_surf_ctx *ctx, int fm2d_method);
 
 /* Device pointers of the context's phase_time (nchain,np,nsrc,nrev) and
  * phase velocity (nchain,nx,ny,np) buffers (valid until destroy). */

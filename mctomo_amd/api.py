@@ -62,7 +62,8 @@ def lib():
                      "mct_last_timings", "mct_set_kernel_timing", "mct_set_tuning", "mct_dev_alloc", "mct_dev_free", "mct_copy_h2d",
                      "mct_copy_d2h", "mct_surfdisp96_batch", "mct_modrays", "mct_surf_create",
                      "mct_surf_destroy", "mct_surf_likelihood", "mct_surf_likelihood_batch_dev",
-                     "mct_surf_sync", "mct_surf_last_timings", "mct_surf_dev_arrays"):
+                     "mct_surf_sync", "mct_surf_last_timings", "mct_surf_dev_arrays",
+                     "mct_surf_set_method"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -196,12 +197,14 @@ def surfdisp96_batch(models, periods, iwave, igr, dphase=1e-3):
 
 
 def modrays(velv, goxd, gozd, dvxd, dvzd, scx, scz, rcx, rcz, srs=None, gdx=1, gdz=1, asgr=1,
-            sgdl=4, sgs=8, fom=1, ttime=None):
+            sgdl=4, sgs=8, fom=1, ttime=None, method=0):
     """modrays phase times (fm2d/fm2dray_cartesian.f90:67-480) on the GPU.
     velv: (np, nvx+2, nvz+2) or (nvx+2, nvz+2) C array, i.e. Fortran
     velv(0:nvz+1, 0:nvx+1) per period; srs (np, nsrc, nrc) or (nsrc, nrc)
     marks the pairs to compute.  Returns ttime (np, nsrc, nrc) (or (nsrc, nrc)
-    for a single grid); unmarked entries keep the values of ``ttime``."""
+    for a single grid); unmarked entries keep the values of ``ttime``.
+    method 0: parallel sweeps with the heap loop near the source; 1: the
+    sequential heap loop per (period, source)."""
     velv = _f64(velv)
     single = velv.ndim == 2
     if single:
@@ -219,7 +222,7 @@ def modrays(velv, goxd, gozd, dvxd, dvzd, scx, scz, rcx, rcz, srs=None, gdx=1, g
     rc = lib().mct_modrays(nvx, nvz, ctypes.c_double(goxd), ctypes.c_double(gozd),
                            ctypes.c_double(dvxd), ctypes.c_double(dvzd), npd, _p(velv), gdx, gdz,
                            asgr, sgdl, sgs, fom, nsrc, _p(scx), _p(scz), nrc, _p(rcx), _p(rcz),
-                           _p(srs), _p(tt))
+                           _p(srs), _p(tt), int(method))
     _check(rc, "mct_modrays")
     return tt[0] if single else tt
 
@@ -432,6 +435,10 @@ class SurfLikelihood:
         t = np.zeros(4)
         lib().mct_surf_last_timings(self._h, _p(t))
         return t
+
+    def set_method(self, fm2d_method):
+        """2-D traveltimes: 0 parallel sweeps (default), 1 sequential heap loop."""
+        _check(lib().mct_surf_set_method(self._h, int(fm2d_method)), "mct_surf_set_method")
 
     def dev_arrays(self):
         """Device addresses of the phase_time and phase-velocity buffers."""

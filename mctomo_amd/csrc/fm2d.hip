@@ -413,6 +413,7 @@ __global__ __launch_bounds__(64) void k_fm2d_solve(mct_fm2d_args A)
 {
     const long slot = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= A.nrun) return;
+    if (A.only_flag && A.only_flag[slot] != 1) return;   // only problems handed over by the parallel path
     const long prob = A.prob0 + slot;
     const int s = (int)(prob % A.nsrc);
     const long cp = prob / A.nsrc;          // chain*np + period
@@ -578,6 +579,21 @@ __global__ __launch_bounds__(64) void k_fm2d_solve(mct_fm2d_args A)
     }
 }
 
+int mct_launch_fm2d_gridder(const mct_fm2d_args &a, hipStream_t s)
+{
+    const long nn = (long)a.nnx * a.nnz * a.nchain * a.np;
+    hipLaunchKernelGGL(k_fm2d_gridder, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, a);
+    MCT_CHECK(hipGetLastError());
+    return MCT_OK;
+}
+
+int mct_launch_fm2d_solve(const mct_fm2d_args &b, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fm2d_solve, dim3((unsigned)((b.nrun + 63) / 64)), dim3(64), 0, s, b);
+    MCT_CHECK(hipGetLastError());
+    return MCT_OK;
+}
+
 int mct_launch_fm2d(const mct_fm2d_args &a, long chunk, hipStream_t s)
 {
     const long nn = (long)a.nnx * a.nnz * a.nchain * a.np;
@@ -599,7 +615,8 @@ int mct_launch_fm2d(const mct_fm2d_args &a, long chunk, hipStream_t s)
 extern "C" int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvzd,
                            int np, const double *velv, int gdx, int gdz, int asgr, int sgdl, int sgs,
                            int fom, int nsrc, const double *scx, const double *scz, int nrc,
-                           const double *rcx, const double *rcz, const int32_t *srs, double *ttime)
+                           const double *rcx, const double *rcz, const int32_t *srs, double *ttime,
+                           int method)
 {
     if ((long)(nvx - 1) * gdx + 1 > 32767 || (long)(nvz - 1) * gdz + 1 > 32767 || 2L * sgs * sgdl + 1 > 32767)
         return MCT_ERR_ARG;   // heap entries pack (ix, iz) into 16 bits each
@@ -619,13 +636,15 @@ extern "C" int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dv
     a.nchain = 1; a.np = np; a.nsrc = nsrc; a.nrc = nrc;
     const long nprob = (long)np * nsrc, wsz = (long)a.ld * a.nc;
     const size_t nv = (size_t)(nvx + 2) * (nvz + 2) * np, nn = (size_t)a.nnx * a.nnz * np;
+    a.nnx = (nvx - 1) * gdx + 1; a.nnz = (nvz - 1) * gdz + 1;   // for the LDS check below
+    const bool fast = method == 0 && mct_fm2d_fast_ok(a);
     void *bufs[16] = {nullptr};
     size_t sizes[16] = {nv * 8, nn * 8, 8 * (size_t)nsrc, 8 * (size_t)nsrc, 8 * (size_t)nrc, 8 * (size_t)nrc,
                         4 * (size_t)np * nsrc * nrc, 8 * (size_t)(nprob * wsz), 8 * (size_t)(nprob * wsz),
                         8 * (size_t)(nprob * wsz), 4 * (size_t)(nprob * wsz), 4 * (size_t)(nprob * wsz),
-                        4 * (size_t)(nprob * (wsz + 8)), 8 * (size_t)nprob * nrc, 4, 0};
+                        4 * (size_t)(nprob * (wsz + 8)), 8 * (size_t)nprob * nrc, 4, 4 * (size_t)nprob};
     int rc = MCT_OK;
-    for (int i = 0; i < 15 && !rc; i++)
+    for (int i = 0; i < 16 && !rc; i++)
         if (hipMalloc(&bufs[i], sizes[i]) != hipSuccess) rc = MCT_ERR_NOMEM;
     if (!rc && (hipMemcpy(bufs[0], velv, nv * 8, hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(bufs[2], scx, 8 * (size_t)nsrc, hipMemcpyHostToDevice) != hipSuccess ||
@@ -645,14 +664,14 @@ extern "C" int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dv
         a.nsts = (int32_t *)bufs[10]; a.nstr = (int32_t *)bufs[11]; a.heap = (int32_t *)bufs[12];
         a.ttime = (double *)bufs[13]; a.status = (int32_t *)bufs[14];
         a.srs_pstride = (long)nsrc * nrc;
-        rc = mct_launch_fm2d(a, nprob, 0);
+        rc = fast ? mct_launch_fm2d_fast(a, nprob, (int32_t *)bufs[15], 0) : mct_launch_fm2d(a, nprob, 0);
         if (!rc && hipDeviceSynchronize() != hipSuccess) rc = MCT_ERR_HIP;
     }
     int32_t st = 0;
     if (!rc && (hipMemcpy(ttime, bufs[13], 8 * (size_t)nprob * nrc, hipMemcpyDeviceToHost) != hipSuccess ||
                 hipMemcpy(&st, bufs[14], 4, hipMemcpyDeviceToHost) != hipSuccess))
         rc = MCT_ERR_HIP;
-    for (int i = 0; i < 15; i++) if (bufs[i]) (void)hipFree(bufs[i]);
+    for (int i = 0; i < 16; i++) if (bufs[i]) (void)hipFree(bufs[i]);
     if (!rc && st < 0) rc = st;
     return rc;
 }

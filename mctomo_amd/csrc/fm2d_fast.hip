@@ -1,0 +1,942 @@
+// fm2d_fast.hip -- parallel 2-D phase traveltimes with the reference's answer.
+//
+// modrays' narrow-band fast marching (fm2d/fm2d_ttime.f90:41-238) freezes
+// nodes in time order; a node's final time is the local upwind update
+// (fouds1/fouds2, :248-518) over the neighbours frozen before it, i.e. over
+// the neighbours with smaller times.  That makes the final field the fixed
+// point of
+//     T(x) = fouds(x; alive = {neighbours y : T(y) < T(x)})
+// which an order-free iteration reaches as well, with the same floating-point
+// operations per node.  Three places depend on the heap's history and are
+// reproduced as the reference does them:
+//   * the source: the four corner times of the source cell are heap keys that
+//     later pops overwrite, so the first pops run the reference's own heap
+//     loop until the four corners are frozen (the seed);
+//   * the refined source grid stops when the first node on an inner edge
+//     reaches the top of the heap (:143-163).  From the converged refined
+//     field the stop is rebuilt: frozen = times below that node's time, close
+//     = unfrozen nodes with a frozen neighbour, whose tentative time is the
+//     update over the neighbours frozen when the last of them was;
+//   * the coarse restart from the remapped band (:86-98, re-banded frozen
+//     nodes) runs the reference's heap loop until the window of the refined
+//     grid and one ring around it are frozen.
+// Everything else is a Gauss-Seidel sweep over anti-diagonals (no two nodes of
+// one anti-diagonal share a stencil, so the nodes of a diagonal update in
+// parallel) until a whole round of four sweep directions changes nothing.
+//
+// k_fm2d_near: one wave per (chain, period, source), LDS-resident refined
+//   grid and coarse neighbourhood of the source; writes the frozen coarse
+//   nodes and a fixed mask.
+// k_fm2d_sweep: one workgroup per problem, sweeps the coarse grid in global
+//   memory, then picks receivers (srtimes, fm2dray_cartesian.f90:682-797).
+// A problem whose heap loop would leave the LDS neighbourhood is flagged and
+// solved by the sequential kernel instead.
+#include "mct_internal.h"
+#include <string.h>
+
+#pragma clang fp contract(off)
+
+#define FF_BIG 1e38
+#define FF_THR0 1e37
+#define FF_FIN 1e30
+#define FF_MARGIN 16          // coarse neighbourhood: refined window +- this many nodes
+#define FF_MAX_ROUNDS 512
+
+namespace {
+
+// A view of (part of) a grid: 1-based (iz, ix) in [1,nnz] x [1,nnx]; storage
+// covers [oz+1, oz+lnz] x [ox+1, ox+lnx] with leading dimension ld (z fastest).
+struct fview {
+    int nnx, nnz;
+    double dnx, dnz;
+    double *T;
+    int16_t *ns;          // LDS states (-1 far, 0 frozen, >0 heap position) or null
+    int ld, ox, oz, lnx, lnz;
+    __device__ __forceinline__ long at(int iz, int ix) const { return (long)(ix - ox - 1) * ld + (iz - oz - 1); }
+    __device__ __forceinline__ double &t(int iz, int ix) const { return T[at(iz, ix)]; }
+    __device__ __forceinline__ int16_t &n(int iz, int ix) const { return ns[at(iz, ix)]; }
+    __device__ __forceinline__ bool local(int iz, int ix) const
+    {
+        return ix > ox && ix <= ox + lnx && iz > oz && iz <= oz + lnz;
+    }
+};
+
+// ---- the reference's local updates over an "alive" predicate ---------------------
+template <class AL>
+__device__ double ff_fouds1(const fview &V, const AL &al, int iz, int ix, double slown)
+{
+    int tsw1 = 0;
+    double travm = 0.0;
+    for (int j = ix - 1; j <= ix + 1; j += 2)
+        for (int k = iz - 1; k <= iz + 1; k += 2) {
+            if (!(j >= 1 && j <= V.nnx && k >= 1 && k <= V.nnz)) continue;
+            int swsol = 0;
+            double a = 0, b = 0, c = 0, tref = 0;
+            if (al(iz, j)) {
+                swsol = 1;
+                if (al(k, ix)) {
+                    double u = V.dnx, v = V.dnz;
+                    double em = V.t(k, ix) - V.t(iz, j);
+                    a = u * u + v * v;
+                    b = -(2.0 * (u * u) * em);
+                    c = (u * u) * (em * em - (v * v) * (slown * slown));
+                    tref = V.t(iz, j);
+                } else {
+                    a = 1.0; b = 0.0;
+                    c = -((slown * slown) * (V.dnx * V.dnx));
+                    tref = V.t(iz, j);
+                }
+            } else if (al(k, ix)) {
+                swsol = 1;
+                a = 1.0; b = 0.0;
+                c = -((slown * V.dnz) * (slown * V.dnz));
+                tref = V.t(k, ix);
+            }
+            if (swsol == 1) {
+                double rd1 = b * b - 4.0 * a * c;
+                if (rd1 < 0.0) rd1 = 0.0;
+                double tdsh = (-b + sqrt(rd1)) / (2.0 * a);
+                double trav = tref + tdsh;
+                if (tsw1 == 1) travm = fmin(trav, travm);
+                else { travm = trav; tsw1 = 1; }
+            }
+        }
+    return travm;
+}
+
+template <class AL>
+__device__ double ff_fouds2(const fview &V, const AL &al, int iz, int ix, double slown)
+{
+    int tsw1 = 0;
+    double travm = 0.0;
+    for (int j = ix - 1; j <= ix + 1; j += 2) {
+        if (!(j >= 1 && j <= V.nnx)) continue;
+        int swj = -1, j2;
+        if (j == ix - 1) {
+            j2 = j - 1;
+            if (j2 >= 1 && al(iz, j2)) swj = 0;
+        } else {
+            j2 = j + 1;
+            if (j2 <= V.nnx && al(iz, j2)) swj = 0;
+        }
+        if (al(iz, j) && swj == 0) {
+            swj = -1;
+            if (V.t(iz, j) > V.t(iz, j2)) swj = 0;
+        } else {
+            swj = -1;
+        }
+        for (int k = iz - 1; k <= iz + 1; k += 2) {
+            if (!(k >= 1 && k <= V.nnz)) continue;
+            int swk = -1, k2;
+            if (k == iz - 1) {
+                k2 = k - 1;
+                if (k2 >= 1 && al(k2, ix)) swk = 0;
+            } else {
+                k2 = k + 1;
+                if (k2 <= V.nnz && al(k2, ix)) swk = 0;
+            }
+            if (al(k, ix) && swk == 0) {
+                swk = -1;
+                if (V.t(k, ix) > V.t(k2, ix)) swk = 0;
+            } else {
+                swk = -1;
+            }
+            int swsol = 0;
+            double a = 0, b = 0, c = 0, tref = 0, tdiv = 1.0, u, v, em;
+            if (swj == 0) {
+                swsol = 1;
+                if (swk == 0) {
+                    u = 2.0 * V.dnx;
+                    v = 2.0 * V.dnz;
+                    em = 4.0 * V.t(iz, j) - V.t(iz, j2) - 4.0 * V.t(k, ix);
+                    em = em + V.t(k2, ix);
+                    a = v * v + u * u;
+                    b = 2.0 * em * (u * u);
+                    c = (u * u) * (em * em - (slown * slown) * (v * v));
+                    tref = 4.0 * V.t(iz, j) - V.t(iz, j2);
+                    tdiv = 3.0;
+                } else if (al(k, ix)) {
+                    u = V.dnz;
+                    v = 2.0 * V.dnx;
+                    em = 3.0 * V.t(k, ix) - 4.0 * V.t(iz, j) + V.t(iz, j2);
+                    a = v * v + 9.0 * (u * u);
+                    b = 6.0 * em * (u * u);
+                    c = (u * u) * (em * em - (slown * slown) * (v * v));
+                    tref = V.t(k, ix);
+                    tdiv = 1.0;
+                } else {
+                    u = 2.0 * V.dnx;
+                    a = 1.0; b = 0.0;
+                    c = -((u * u) * (slown * slown));
+                    tref = 4.0 * V.t(iz, j) - V.t(iz, j2);
+                    tdiv = 3.0;
+                }
+            } else if (al(iz, j)) {
+                swsol = 1;
+                if (swk == 0) {
+                    u = V.dnx;
+                    v = 2.0 * V.dnz;
+                    em = 3.0 * V.t(iz, j) - 4.0 * V.t(k, ix) + V.t(k2, ix);
+                    a = v * v + 9.0 * (u * u);
+                    b = 6.0 * em * (u * u);
+                    c = (u * u) * (em * em - (v * v) * (slown * slown));
+                    tref = V.t(iz, j);
+                    tdiv = 1.0;
+                } else if (al(k, ix)) {
+                    u = V.dnx;
+                    v = V.dnz;
+                    em = V.t(k, ix) - V.t(iz, j);
+                    a = u * u + v * v;
+                    b = -(2.0 * (u * u) * em);
+                    c = (u * u) * (em * em - (v * v) * (slown * slown));
+                    tref = V.t(iz, j);
+                    tdiv = 1.0;
+                } else {
+                    a = 1.0; b = 0.0;
+                    c = -((slown * slown) * (V.dnx * V.dnx));
+                    tref = V.t(iz, j);
+                    tdiv = 1.0;
+                }
+            } else {
+                if (swk == 0) {
+                    swsol = 1;
+                    u = 2.0 * V.dnz;
+                    a = 1.0; b = 0.0;
+                    c = -((u * u) * (slown * slown));
+                    tref = 4.0 * V.t(k, ix) - V.t(k2, ix);
+                    tdiv = 3.0;
+                } else if (al(k, ix)) {
+                    swsol = 1;
+                    a = 1.0; b = 0.0;
+                    c = -((slown * slown) * (V.dnz * V.dnz));
+                    tref = V.t(k, ix);
+                    tdiv = 1.0;
+                }
+            }
+            if (swsol == 1) {
+                double rd1 = b * b - 4.0 * a * c;
+                if (rd1 < 0.0) rd1 = 0.0;
+                double tdsh = (-b + sqrt(rd1)) / (2.0 * a);
+                double trav = (tref + tdsh) / tdiv;
+                if (tsw1 == 1) travm = fmin(trav, travm);
+                else { travm = trav; tsw1 = 1; }
+            }
+        }
+    }
+    return travm;
+}
+
+template <class AL>
+__device__ __forceinline__ double ff_fouds(const fview &V, const AL &al, int iz, int ix, double slown, int fom)
+{
+    return fom == 0 ? ff_fouds1(V, al, iz, ix, slown) : ff_fouds2(V, al, iz, ix, slown);
+}
+
+// The update over the neighbours whose times are below thr.
+__device__ double ff_cand1(const fview &V, int iz, int ix, double thr, double slown, int fom)
+{
+    auto al = [&](int z, int x) -> bool {
+        double t = V.t(z, x);
+        return t < thr && t < FF_FIN;
+    };
+    bool any = (iz > 1 && al(iz - 1, ix)) || (iz < V.nnz && al(iz + 1, ix)) ||
+               (ix > 1 && al(iz, ix - 1)) || (ix < V.nnx && al(iz, ix + 1));
+    if (!any) return FF_BIG;
+    return ff_fouds(V, al, iz, ix, slown, fom);
+}
+
+// Self-consistent update: the neighbours counted are those below the result.
+__device__ double ff_cand(const fview &V, int iz, int ix, double slown, int fom)
+{
+    double c = ff_cand1(V, iz, ix, FF_THR0, slown, fom);
+    for (int k = 0; k < 8; k++) {
+        double c2 = ff_cand1(V, iz, ix, c, slown, fom);
+        if (c2 == c || c2 >= FF_THR0) break;
+        c = c2;
+    }
+    return c;
+}
+
+// Gauss-Seidel rounds over anti-diagonals in four directions until a round
+// changes no time by more than 1e-14 relative.  fixed(z,x) nodes are kept.
+// Returns the rounds used, or -1 without convergence.
+template <class FX, class VL>
+__device__ int ff_sweep(const fview &V, const FX &fixed, const VL &vel, int fom, int *sh_flag)
+{
+    const int nt = blockDim.x, tid = threadIdx.x;
+    for (int round = 0; round < FF_MAX_ROUNDS; round++) {
+        int changed = 0;
+        for (int dir = 0; dir < 4; dir++) {
+            for (int d = 0; d < V.nnx + V.nnz - 1; d++) {
+                int a_lo = d - (V.nnz - 1) > 0 ? d - (V.nnz - 1) : 0;
+                int a_hi = d < V.nnx - 1 ? d : V.nnx - 1;
+                for (int a = a_lo + tid; a <= a_hi; a += nt) {
+                    int b = d - a;
+                    int x = (dir & 1) ? V.nnx - a : 1 + a;
+                    int z = (dir & 2) ? V.nnz - b : 1 + b;
+                    if (fixed(z, x)) continue;
+                    double slown = 1.0 / vel(z, x);
+                    double c = ff_cand(V, z, x, slown, fom);
+                    double t = V.t(z, x);
+                    if (c < FF_THR0 && c != t) {
+                        if (fabs(c - t) > 1e-14 * c) changed = 1;
+                        V.t(z, x) = c;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (tid == 0) *sh_flag = 0;
+        __syncthreads();
+        if (changed) *sh_flag = 1;
+        __syncthreads();
+        int any = *sh_flag;
+        __syncthreads();
+        if (!any) return round + 1;
+    }
+    return -1;
+}
+
+// ---- heap on an LDS view (fm2d_ttime.f90:527-680) ---------------------------------
+#define HPX(e) ((e) >> 16)
+#define HPZ(e) ((e) & 0xffff)
+
+struct ff_heap {
+    int32_t *btg;    // 1-based
+    int ntr;
+};
+
+__device__ void ff_addtree(const fview &V, ff_heap &H, int iz, int ix)
+{
+    H.ntr = H.ntr + 1;
+    V.n(iz, ix) = (int16_t)H.ntr;
+    H.btg[H.ntr] = (ix << 16) | iz;
+    int tpc = H.ntr, tpp = tpc / 2;
+    const double tv = V.t(iz, ix);
+    while (tpp > 0) {
+        int ep = H.btg[tpp];
+        if (tv < V.t(HPZ(ep), HPX(ep))) {
+            V.n(iz, ix) = (int16_t)tpp;
+            V.n(HPZ(ep), HPX(ep)) = (int16_t)tpc;
+            H.btg[tpp] = H.btg[tpc];
+            H.btg[tpc] = ep;
+            tpc = tpp;
+            tpp = tpc / 2;
+        } else {
+            tpp = 0;
+        }
+    }
+}
+
+__device__ void ff_downtree(const fview &V, ff_heap &H)
+{
+    if (H.ntr == 1) { H.ntr = H.ntr - 1; return; }
+    int el = H.btg[H.ntr];
+    V.n(HPZ(el), HPX(el)) = 1;
+    H.btg[1] = el;
+    H.ntr = H.ntr - 1;
+    int tpp = 1, tpc = 2 * tpp;
+    while (tpc < H.ntr) {
+        int ec = H.btg[tpc], ec1 = H.btg[tpc + 1];
+        double rd1 = V.t(HPZ(ec), HPX(ec));
+        double rd2 = V.t(HPZ(ec1), HPX(ec1));
+        if (rd1 > rd2) tpc = tpc + 1;
+        ec = H.btg[tpc];
+        int ep = H.btg[tpp];
+        rd1 = V.t(HPZ(ec), HPX(ec));
+        rd2 = V.t(HPZ(ep), HPX(ep));
+        if (rd1 < rd2) {
+            V.n(HPZ(ep), HPX(ep)) = (int16_t)tpc;
+            V.n(HPZ(ec), HPX(ec)) = (int16_t)tpp;
+            H.btg[tpc] = ep;
+            H.btg[tpp] = ec;
+            tpp = tpc;
+            tpc = 2 * tpp;
+        } else {
+            tpc = H.ntr + 1;
+        }
+    }
+    if (tpc == H.ntr) {
+        int ec = H.btg[tpc], ep = H.btg[tpp];
+        double rd1 = V.t(HPZ(ec), HPX(ec));
+        double rd2 = V.t(HPZ(ep), HPX(ep));
+        if (rd1 < rd2) {
+            V.n(HPZ(ep), HPX(ep)) = (int16_t)tpc;
+            V.n(HPZ(ec), HPX(ec)) = (int16_t)tpp;
+            H.btg[tpc] = ep;
+            H.btg[tpp] = ec;
+        }
+    }
+}
+
+__device__ void ff_updtree(const fview &V, ff_heap &H, int iz, int ix)
+{
+    int tpc = V.n(iz, ix), tpp = tpc / 2;
+    const double tv = V.t(iz, ix);
+    while (tpp > 0) {
+        int ep = H.btg[tpp];
+        if (tv < V.t(HPZ(ep), HPX(ep))) {
+            V.n(iz, ix) = (int16_t)tpp;
+            V.n(HPZ(ep), HPX(ep)) = (int16_t)tpc;
+            H.btg[tpp] = H.btg[tpc];
+            H.btg[tpc] = ep;
+            tpc = tpp;
+            tpp = tpc / 2;
+        } else {
+            tpp = 0;
+        }
+    }
+}
+
+__device__ __forceinline__ double ff_bilinear(const double nv[2][2], double dsx, double dsz, double dnx, double dnz)
+{
+    double biv = 0.0;
+    for (int i = 1; i <= 2; i++)
+        for (int j = 1; j <= 2; j++) {
+            double produ = (1.0 - fabs(((i - 1) * dnx - dsx) / dnx)) * (1.0 - fabs(((j - 1) * dnz - dsz) / dnz));
+            biv = biv + nv[i - 1][j - 1] * produ;
+        }
+    return biv;
+}
+
+// Heap-loop stop rules.
+struct ff_stop {
+    int mode;                 // 0 none, 1 refined inner-edge exit, 2 block frozen
+    int vnl, vnr, vnt, vnb;   // refined-grid bookkeeping (mode 1)
+    int x0, x1, z0, z1;       // block (mode 2)
+    int need, have;
+    int exited;               // mode 1: exit node reached
+    int overflow;             // a node outside the LDS storage would be touched
+};
+
+// The reference's heap loop over the view (travel's :137-237) with fouds and
+// velocities vel(z,x); stops by rule S.  Runs in one lane.
+template <class VL>
+__device__ void ff_heap_loop(const fview &V, ff_heap &H, const VL &vel, int fom, ff_stop &S)
+{
+    auto al = [&](int z, int x) -> bool { return V.n(z, x) == 0; };
+    while (H.ntr > 0) {
+        int e = H.btg[1];
+        int ix = HPX(e), iz = HPZ(e);
+        if (S.mode == 1) {
+            int swrg = 0;
+            if (ix == 1 && S.vnl != 1) swrg = 1;
+            if (ix == V.nnx && S.vnr != V.nnx) swrg = 1;
+            if (iz == 1 && S.vnt != 1) swrg = 1;
+            if (iz == V.nnz && S.vnb != V.nnz) swrg = 1;
+            if (swrg == 1) { V.n(iz, ix) = 0; S.exited = 1; return; }
+        }
+        if (S.mode == 2 && S.have >= S.need) return;
+        // the updates around a popped node read up to three nodes away
+        {
+            int xa = ix - 3 < 1 ? 1 : ix - 3, xb = ix + 3 > V.nnx ? V.nnx : ix + 3;
+            int za = iz - 3 < 1 ? 1 : iz - 3, zb = iz + 3 > V.nnz ? V.nnz : iz + 3;
+            if (!V.local(za, xa) || !V.local(zb, xb)) { S.overflow = 1; return; }
+        }
+        V.n(iz, ix) = 0;
+        if (S.mode == 2 && ix >= S.x0 && ix <= S.x1 && iz >= S.z0 && iz <= S.z1) S.have++;
+        ff_downtree(V, H);
+        for (int i = ix - 1; i <= ix + 1; i += 2) {
+            if (i >= 1 && i <= V.nnx) {
+                int st = V.n(iz, i);
+                if (st == -1) {
+                    V.t(iz, i) = ff_fouds(V, al, iz, i, 1.0 / vel(iz, i), fom);
+                    ff_addtree(V, H, iz, i);
+                } else if (st > 0) {
+                    V.t(iz, i) = ff_fouds(V, al, iz, i, 1.0 / vel(iz, i), fom);
+                    ff_updtree(V, H, iz, i);
+                }
+            }
+        }
+        for (int i = iz - 1; i <= iz + 1; i += 2) {
+            if (i >= 1 && i <= V.nnz) {
+                int st = V.n(i, ix);
+                if (st == -1) {
+                    V.t(i, ix) = ff_fouds(V, al, i, ix, 1.0 / vel(i, ix), fom);
+                    ff_addtree(V, H, i, ix);
+                } else if (st > 0) {
+                    V.t(i, ix) = ff_fouds(V, al, i, ix, 1.0 / vel(i, ix), fom);
+                    ff_updtree(V, H, i, ix);
+                }
+            }
+        }
+    }
+}
+
+// travel's corner initialisation (fm2d_ttime.f90:105-126) and heap start.
+template <class VL>
+__device__ void ff_corners(const fview &V, ff_heap &H, const VL &vel, double scx, double scz,
+                           double gox, double goz, int isx, int isz)
+{
+    double vss[2][2];
+    for (int i = 1; i <= 2; i++)
+        for (int j = 1; j <= 2; j++) vss[i - 1][j - 1] = vel(isz - 1 + j, isx - 1 + i);
+    double dsx = (scx - gox) - (isx - 1) * V.dnx;
+    double dsz = (scz - goz) - (isz - 1) * V.dnz;
+    double vsrc = ff_bilinear(vss, dsx, dsz, V.dnx, V.dnz);
+    H.ntr = 0;
+    for (int i = 1; i <= 2; i++)
+        for (int j = 1; j <= 2; j++) {
+            double ex = dsx - (i - 1) * V.dnx, ez = dsz - (j - 1) * V.dnz;
+            double ds = sqrt(ex * ex + ez * ez);
+            V.t(isz - 1 + j, isx - 1 + i) = 2.0 * ds / (vss[i - 1][j - 1] + vsrc);
+            ff_addtree(V, H, isz - 1 + j, isx - 1 + i);
+        }
+}
+
+__device__ __forceinline__ void bspl_w4(double u, double w[4])
+{
+    w[0] = (1.0 - u) * (1.0 - u) * (1.0 - u) / 6.0;
+    w[1] = (4.0 - 6.0 * (u * u) + 3.0 * (u * u * u)) / 6.0;
+    w[2] = (1.0 + 3.0 * u + 3.0 * (u * u) - 3.0 * (u * u * u)) / 6.0;
+    w[3] = (u * u * u) / 6.0;
+}
+
+struct near_geom {
+    int s, p, c;
+    long prob, slot;
+    int isx, isz, vnl, vnr, vnt, vnb;
+};
+
+// Problem bookkeeping shared by both kernels; returns false to skip.
+__device__ bool ff_problem(const mct_fm2d_args &A, long slot, near_geom &g, bool report)
+{
+    g.slot = slot;
+    g.prob = A.prob0 + slot;
+    g.s = (int)(g.prob % A.nsrc);
+    const long cp = g.prob / A.nsrc;
+    g.p = (int)(cp % A.np);
+    g.c = (int)(cp / A.np);
+    if (A.status[g.c] > 0) return false;
+    const int32_t *srs = A.srs + (long)g.p * A.srs_pstride + (long)g.s * A.nrc;
+    int any = 0;
+    for (int r = 0; r < A.nrc; r++) any += srs[r];
+    if (any == 0 && g.s != 0) return false;   // modrays skips sources without pairs
+    const double dnx = A.dvx / A.gdx, dnz = A.dvz / A.gdz;
+    const double x = A.scx[g.s], z = A.scz[g.s];
+    int isx = (int)((x - A.gox) / dnx) + 1, isz = (int)((z - A.goz) / dnz) + 1;
+    if (isx < 1 || isx > A.nnx || isz < 1 || isz > A.nnz) {
+        if (report) atomicMin(&A.status[g.c], (int)MCT_ERR_OUTSIDE);
+        return false;
+    }
+    if (isx == A.nnx) isx = isx - 1;
+    if (isz == A.nnz) isz = isz - 1;
+    g.isx = isx; g.isz = isz;
+    g.vnl = isx - A.sgs > 1 ? isx - A.sgs : 1;
+    g.vnr = isx + A.sgs < A.nnx ? isx + A.sgs : A.nnx;
+    g.vnt = isz - A.sgs > 1 ? isz - A.sgs : 1;
+    g.vnb = isz + A.sgs < A.nnz ? isz + A.sgs : A.nnz;
+    return true;
+}
+
+// wave-wide lexicographic minimum of (t, idx)
+__device__ void ff_wave_min(double &t, int &idx)
+{
+    for (int o = 32; o >= 1; o >>= 1) {
+        double t2 = __shfl_xor(t, o, 64);
+        int i2 = __shfl_xor(idx, o, 64);
+        if (t2 < t || (t2 == t && i2 < idx)) { t = t2; idx = i2; }
+    }
+}
+
+} // namespace
+
+// One wave per problem.  LDS: region (doubles + int16 states) sized for the
+// larger of the refined grid and the coarse neighbourhood, the heap, and the
+// remap staging of the refined window's coarse nodes.
+__global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_nodes, int heap_cap,
+                                                  int win_nodes, int32_t *pflag)
+{
+    extern __shared__ double ff_lds[];
+    double *RT = ff_lds;
+    int16_t *RN = (int16_t *)(RT + region_nodes);
+    int32_t *HB = (int32_t *)(RN + ((region_nodes + 3) & ~3));
+    double *WT = (double *)(HB + ((heap_cap + 2 + 1) & ~1));
+    int16_t *WN = (int16_t *)(WT + win_nodes);
+    __shared__ int sh_flag, sh_exit, sh_ovf;
+    __shared__ double sh_texit;
+    __shared__ int sh_eidx;
+
+    const int lane = threadIdx.x;
+    near_geom g;
+    const long slot = blockIdx.x;
+    if (!ff_problem(A, slot, g, true)) {
+        if (lane == 0) pflag[slot] = 2;   // nothing to do
+        return;
+    }
+    const long wsz = (long)A.ld * A.nc;
+    const double dnx = A.dvx / A.gdx, dnz = A.dvz / A.gdz;
+    const double x = A.scx[g.s], z = A.scz[g.s];
+    const long cp = g.prob / A.nsrc;
+    const double *velnc = A.veln + cp * (long)A.nnx * A.nnz;
+    const double *velv = A.velv + cp * (long)(A.nvz + 2) * (A.nvx + 2);
+    const int ldv = A.nvz + 2;
+    auto cvel = [&](int iz, int ix) -> double { return velnc[(long)(ix - 1) * A.nnz + (iz - 1)]; };
+    if (lane == 0) { sh_exit = 0; sh_ovf = 0; }
+    __syncthreads();
+
+    // coarse neighbourhood of the source
+    int lx0, lx1, lz0, lz1;
+    if (A.asgr == 1) {
+        lx0 = g.vnl - FF_MARGIN; lx1 = g.vnr + FF_MARGIN; lz0 = g.vnt - FF_MARGIN; lz1 = g.vnb + FF_MARGIN;
+    } else {
+        lx0 = g.isx - FF_MARGIN; lx1 = g.isx + 1 + FF_MARGIN; lz0 = g.isz - FF_MARGIN; lz1 = g.isz + 1 + FF_MARGIN;
+    }
+    lx0 = lx0 < 1 ? 1 : lx0; lz0 = lz0 < 1 ? 1 : lz0;
+    lx1 = lx1 > A.nnx ? A.nnx : lx1; lz1 = lz1 > A.nnz ? A.nnz : lz1;
+    fview C;
+    C.nnx = A.nnx; C.nnz = A.nnz; C.dnx = dnx; C.dnz = dnz;
+    C.T = RT; C.ns = RN; C.ox = lx0 - 1; C.oz = lz0 - 1; C.lnx = lx1 - lx0 + 1; C.lnz = lz1 - lz0 + 1; C.ld = C.lnz;
+    ff_heap H;
+    H.btg = HB;
+    H.ntr = 0;
+
+    if (A.asgr == 1) {
+        // ---- refined source grid: bsplrefine, seed, sweep, rebuilt stop ----
+        fview R;
+        R.nnx = (g.vnr - g.vnl) * A.sgdl + 1;
+        R.nnz = (g.vnb - g.vnt) * A.sgdl + 1;
+        R.dnx = A.dvx / (double)(float)(A.gdx * A.sgdl);
+        R.dnz = A.dvz / (double)(float)(A.gdz * A.sgdl);
+        R.T = RT; R.ns = RN; R.ox = 0; R.oz = 0; R.lnx = R.nnx; R.lnz = R.nnz; R.ld = R.nnz;
+        const double gorx = A.gox + dnx * (g.vnl - 1), gorz = A.goz + dnz * (g.vnt - 1);
+        double *vr = A.vr + slot * wsz;
+        const int nrxr = A.gdx * A.sgdl, nrzr = A.gdz * A.sgdl;
+        const int origx = (g.vnl - 1) * A.sgdl + 1, origz = (g.vnt - 1) * A.sgdl + 1;
+        const int rn = R.nnx * R.nnz;
+        for (int q = lane; q < rn; q += 64) {
+            int idm2 = q / R.nnz + 1, idm1 = q % R.nnz + 1;
+            int st1 = idm1 + origz - 1, st2 = idm2 + origx - 1;   // fine indices
+            int ii = (st1 - 1) / nrzr + 1, k = (st1 - 1) % nrzr + 1;
+            if (ii > A.nvz - 1) { ii = A.nvz - 1; k = nrzr + 1; }
+            int jj = (st2 - 1) / nrxr + 1, l = (st2 - 1) % nrxr + 1;
+            if (jj > A.nvx - 1) { jj = A.nvx - 1; l = nrxr + 1; }
+            double u = nrxr; u = (l - 1) / u;
+            double v = nrzr; v = (k - 1) / v;
+            double ui[4], vi[4];
+            bspl_w4(u, ui);
+            bspl_w4(v, vi);
+            double sum[4];
+            for (int i1 = 1; i1 <= 4; i1++) {
+                sum[i1 - 1] = 0.0;
+                for (int j1 = 1; j1 <= 4; j1++)
+                    sum[i1 - 1] = sum[i1 - 1] + ui[j1 - 1] * velv[(jj - 2 + j1) * ldv + (ii - 2 + i1)];
+                sum[i1 - 1] = vi[i1 - 1] * sum[i1 - 1];
+            }
+            vr[(long)(idm2 - 1) * A.ld + (idm1 - 1)] = sum[0] + sum[1] + sum[2] + sum[3];
+            RN[q] = -1;
+            RT[q] = FF_BIG;
+        }
+        __syncthreads();
+        auto rvel = [&](int iz, int ix) -> double { return vr[(long)(ix - 1) * A.ld + (iz - 1)]; };
+        // seed: the reference's heap loop until the source cell's corners are frozen
+        if (lane == 0) {
+            int risx = (int)((x - gorx) / R.dnx) + 1, risz = (int)((z - gorz) / R.dnz) + 1;
+            bool ok = !(risx < 1 || risx > R.nnx || risz < 1 || risz > R.nnz);
+            if (ok) {
+                if (risx == R.nnx) risx = risx - 1;
+                if (risz == R.nnz) risz = risz - 1;
+                ff_corners(R, H, rvel, x, z, gorx, gorz, risx, risz);
+                ff_stop S;
+                memset(&S, 0, sizeof S);
+                S.mode = 1;
+                S.vnl = g.vnl; S.vnr = g.vnr; S.vnt = g.vnt; S.vnb = g.vnb;
+                // the block rule is checked with the exit rule: run mode 1 while counting
+                S.x0 = risx; S.x1 = risx + 1; S.z0 = risz; S.z1 = risz + 1; S.need = 4;
+                auto al = [&](int zz, int xx) -> bool { return R.n(zz, xx) == 0; };
+                (void)al;
+                // heap loop with both rules (exit first, as the reference checks it first)
+                while (H.ntr > 0) {
+                    int e = H.btg[1];
+                    int ix = HPX(e), iz = HPZ(e);
+                    int swrg = 0;
+                    if (ix == 1 && S.vnl != 1) swrg = 1;
+                    if (ix == R.nnx && S.vnr != R.nnx) swrg = 1;
+                    if (iz == 1 && S.vnt != 1) swrg = 1;
+                    if (iz == R.nnz && S.vnb != R.nnz) swrg = 1;
+                    if (swrg == 1) { R.n(iz, ix) = 0; S.exited = 1; break; }
+                    if (S.have >= S.need) break;
+                    R.n(iz, ix) = 0;
+                    if (ix >= S.x0 && ix <= S.x1 && iz >= S.z0 && iz <= S.z1) S.have++;
+                    ff_downtree(R, H);
+                    for (int i = ix - 1; i <= ix + 1; i += 2)
+                        if (i >= 1 && i <= R.nnx) {
+                            int st = R.n(iz, i);
+                            if (st == -1) { R.t(iz, i) = ff_fouds(R, al, iz, i, 1.0 / rvel(iz, i), A.fom); ff_addtree(R, H, iz, i); }
+                            else if (st > 0) { R.t(iz, i) = ff_fouds(R, al, iz, i, 1.0 / rvel(iz, i), A.fom); ff_updtree(R, H, iz, i); }
+                        }
+                    for (int i = iz - 1; i <= iz + 1; i += 2)
+                        if (i >= 1 && i <= R.nnz) {
+                            int st = R.n(i, ix);
+                            if (st == -1) { R.t(i, ix) = ff_fouds(R, al, i, ix, 1.0 / rvel(i, ix), A.fom); ff_addtree(R, H, i, ix); }
+                            else if (st > 0) { R.t(i, ix) = ff_fouds(R, al, i, ix, 1.0 / rvel(i, ix), A.fom); ff_updtree(R, H, i, ix); }
+                        }
+                }
+                sh_exit = S.exited ? 1 : 0;
+            } else {
+                sh_ovf = 3;   // source outside the refined grid: use the sequential solver
+            }
+        }
+        __syncthreads();
+        if (sh_ovf) {
+            if (lane == 0) pflag[slot] = 1;
+            return;
+        }
+        if (!sh_exit) {
+            // frozen seed nodes are fixed; everything else is solved by sweeping
+            for (int q = lane; q < rn; q += 64) {
+                if (RN[q] != 0) RT[q] = FF_BIG;
+            }
+            __syncthreads();
+            auto rfixed = [&](int iz, int ix) -> bool { return R.n(iz, ix) == 0; };
+            int rounds = ff_sweep(R, rfixed, rvel, A.fom, &sh_flag);
+            if (rounds < 0) {
+                if (lane == 0) pflag[slot] = 1;
+                return;
+            }
+            // the stop: the smallest time on an inner edge (first in the reference's scan order)
+            double tm = 1e300;
+            int im = 0x7fffffff;
+            for (int q = lane; q < rn; q += 64) {
+                int i = q / R.nnz + 1, j = q % R.nnz + 1;
+                // the reference compares the coarse window bounds with the refined
+                // grid's size here (fm2d_ttime.f90:143-163); kept as it is
+                bool inner = (i == 1 && g.vnl != 1) || (i == R.nnx && g.vnr != R.nnx) ||
+                             (j == 1 && g.vnt != 1) || (j == R.nnz && g.vnb != R.nnz);
+                if (inner && (RT[q] < tm || (RT[q] == tm && q < im))) { tm = RT[q]; im = q; }
+            }
+            ff_wave_min(tm, im);
+            if (lane == 0) { sh_texit = tm; sh_eidx = tm < 1e300 ? im : -1; }
+            __syncthreads();
+            const double texit = sh_texit;
+            const int eidx = sh_eidx;
+            for (int q = lane; q < rn; q += 64) RN[q] = (RT[q] < texit || q == eidx) ? 0 : -1;
+            __syncthreads();
+            // close nodes and their tentative times
+            const int ex_x = eidx >= 0 ? eidx / R.nnz + 1 : -1, ex_z = eidx >= 0 ? eidx % R.nnz + 1 : -1;
+            for (int q = lane; q < rn; q += 64) {
+                if (RN[q] == 0) continue;
+                int i = q / R.nnz + 1, j = q % R.nnz + 1;
+                double tlast = -1;
+                const int nb[4][2] = {{j, i - 1}, {j, i + 1}, {j - 1, i}, {j + 1, i}};
+                for (int w = 0; w < 4; w++) {
+                    int zz = nb[w][0], xx = nb[w][1];
+                    if (zz < 1 || zz > R.nnz || xx < 1 || xx > R.nnx) continue;
+                    if (R.n(zz, xx) == 0 && !(zz == ex_z && xx == ex_x) && R.t(zz, xx) > tlast) tlast = R.t(zz, xx);
+                }
+                if (tlast >= 0) {
+                    auto alt = [&](int zz, int xx) -> bool {
+                        return R.n(zz, xx) == 0 && !(zz == ex_z && xx == ex_x) && R.t(zz, xx) <= tlast;
+                    };
+                    double tv = ff_fouds(R, alt, j, i, 1.0 / rvel(j, i), A.fom);
+                    RT[q] = tv;
+                    RN[q] = 1;
+                }
+            }
+            __syncthreads();
+        } else {
+            for (int q = lane; q < rn; q += 64)
+                if (RN[q] > 0) RN[q] = 1;
+            __syncthreads();
+        }
+        // stage the refined values at the window's coarse nodes (remap, :354-416)
+        const int wnx = g.vnr - g.vnl + 1, wnz = g.vnb - g.vnt + 1;
+        for (int q = lane; q < wnx * wnz; q += 64) {
+            int a = q / wnz, b = q % wnz;
+            int l = a * A.sgdl + 1, k = b * A.sgdl + 1;
+            WN[q] = R.n(k, l);
+            WT[q] = R.t(k, l);
+        }
+        __syncthreads();
+        // coarse neighbourhood: far everywhere, then the remapped window
+        for (int q = lane; q < C.lnx * C.lnz; q += 64) { RN[q] = -1; RT[q] = FF_BIG; }
+        __syncthreads();
+        for (int q = lane; q < wnx * wnz; q += 64) {
+            int a = q / wnz, b = q % wnz;
+            int idm2 = g.vnl + a, idm1 = g.vnt + b;
+            int st = WN[q];
+            C.n(idm1, idm2) = (int16_t)st;
+            if (st >= 0) C.t(idm1, idm2) = WT[q];
+        }
+        __syncthreads();
+        // frozen nodes next to far nodes re-enter the band
+        for (int q = lane; q < C.lnx * C.lnz; q += 64) {
+            int k = C.ox + q / C.lnz + 1, l = C.oz + q % C.lnz + 1;
+            if (C.n(l, k) != 0) continue;
+            bool nb = (l - 1 >= 1 && C.n(l - 1, k) == -1) || (l + 1 <= A.nnz && C.n(l + 1, k) == -1) ||
+                      (k - 1 >= 1 && C.n(l, k - 1) == -1) || (k + 1 <= A.nnx && C.n(l, k + 1) == -1);
+            if (nb) C.n(l, k) = 1;
+        }
+        __syncthreads();
+        // the reference's coarse heap loop until the window and one ring are frozen
+        if (lane == 0) {
+            H.ntr = 0;
+            for (int i = lx0; i <= lx1; i++)
+                for (int j = lz0; j <= lz1; j++)
+                    if (C.n(j, i) > 0) ff_addtree(C, H, j, i);
+            ff_stop S;
+            memset(&S, 0, sizeof S);
+            S.mode = 2;
+            S.x0 = g.vnl - 1 < 1 ? 1 : g.vnl - 1; S.x1 = g.vnr + 1 > A.nnx ? A.nnx : g.vnr + 1;
+            S.z0 = g.vnt - 1 < 1 ? 1 : g.vnt - 1; S.z1 = g.vnb + 1 > A.nnz ? A.nnz : g.vnb + 1;
+            S.need = (S.x1 - S.x0 + 1) * (S.z1 - S.z0 + 1);
+            S.have = 0;
+            for (int k = S.x0; k <= S.x1; k++)
+                for (int l = S.z0; l <= S.z1; l++) S.have += C.n(l, k) == 0;
+            ff_heap_loop(C, H, cvel, A.fom, S);
+            sh_ovf = S.overflow;
+        }
+        __syncthreads();
+    } else {
+        // ---- no refined grid: seed on the coarse grid ----
+        for (int q = lane; q < C.lnx * C.lnz; q += 64) { RN[q] = -1; RT[q] = FF_BIG; }
+        __syncthreads();
+        if (lane == 0) {
+            ff_corners(C, H, cvel, x, z, A.gox, A.goz, g.isx, g.isz);
+            ff_stop S;
+            memset(&S, 0, sizeof S);
+            S.mode = 2;
+            S.x0 = g.isx; S.x1 = g.isx + 1; S.z0 = g.isz; S.z1 = g.isz + 1; S.need = 4; S.have = 0;
+            ff_heap_loop(C, H, cvel, A.fom, S);
+            sh_ovf = S.overflow;
+        }
+        __syncthreads();
+    }
+    if (sh_ovf) {
+        if (lane == 0) pflag[slot] = 1;
+        return;
+    }
+    // write the coarse grid: frozen nodes fixed with their times, the rest open
+    double *T = A.ttn + slot * wsz;
+    int32_t *F = A.nsts + slot * wsz;
+    for (long q = lane; q < (long)A.nnx * A.nnz; q += 64) {
+        int k = (int)(q / A.nnz) + 1, l = (int)(q % A.nnz) + 1;
+        long o = (long)(k - 1) * A.ld + (l - 1);
+        if (C.local(l, k) && C.n(l, k) == 0) { T[o] = C.t(l, k); F[o] = 1; }
+        else { T[o] = FF_BIG; F[o] = 0; }
+    }
+    if (lane == 0) pflag[slot] = 0;
+}
+
+// One workgroup per problem: sweep the coarse grid, then pick the receivers.
+__global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, const int32_t *pflag)
+{
+    __shared__ int sh_flag;
+    const long slot = blockIdx.x;
+    if (pflag[slot] != 0) return;
+    near_geom g;
+    if (!ff_problem(A, slot, g, false)) return;
+    const long wsz = (long)A.ld * A.nc;
+    const double dnx = A.dvx / A.gdx, dnz = A.dvz / A.gdz;
+    const long cp = g.prob / A.nsrc;
+    const double *velnc = A.veln + cp * (long)A.nnx * A.nnz;
+    fview V;
+    V.nnx = A.nnx; V.nnz = A.nnz; V.dnx = dnx; V.dnz = dnz;
+    V.T = A.ttn + slot * wsz; V.ns = nullptr; V.ox = 0; V.oz = 0; V.lnx = A.nnx; V.lnz = A.nnz; V.ld = A.ld;
+    const int32_t *F = A.nsts + slot * wsz;
+    auto fixed = [&](int iz, int ix) -> bool { return F[(long)(ix - 1) * A.ld + (iz - 1)] != 0; };
+    auto vel = [&](int iz, int ix) -> double { return velnc[(long)(ix - 1) * A.nnz + (iz - 1)]; };
+    int rounds = ff_sweep(V, fixed, vel, A.fom, &sh_flag);
+    if (rounds < 0) {
+        if (threadIdx.x == 0) atomicMin(&A.status[g.c], (int)MCT_ERR_NOCONV);
+        return;
+    }
+    // srtimes
+    const double x = A.scx[g.s], z = A.scz[g.s];
+    const int32_t *srs = A.srs + (long)g.p * A.srs_pstride + (long)g.s * A.nrc;
+    double *out = A.ttime + g.prob * A.nrc;
+    for (int r = threadIdx.x + 1; r <= A.nrc; r += blockDim.x) {
+        if (srs[r - 1] == 0) continue;
+        const double rx = A.rcx[r - 1], rz = A.rcz[r - 1];
+        int irx = (int)floor((rx - A.gox) / dnx) + 1;
+        int irz = (int)floor((rz - A.goz) / dnz) + 1;
+        if (irx < 1 || irx > A.nnx || irz < 1 || irz > A.nnz) { atomicMin(&A.status[g.c], (int)MCT_ERR_OUTSIDE); continue; }
+        if (irx == A.nnx) irx = irx - 1;
+        if (irz == A.nnz) irz = irz - 1;
+        int jsx = (int)floor((x - A.gox) / dnx) + 1, jsz = (int)floor((z - A.goz) / dnz) + 1;
+        double dpl = dnx, rd1 = dnz;
+        if (rd1 < dpl) dpl = rd1;
+        double sred = (x - rx) * (x - rx);
+        sred = sred + (z - rz) * (z - rz);
+        sred = sqrt(sred);
+        int sw = 0;
+        if (sred < dpl) sw = 1;
+        if (jsx == irx && jsz == irz) sw = 1;
+        double trr;
+        if (sw == 1) {
+            double vss[2][2];
+            int csx = jsx < A.nnx ? jsx : A.nnx - 1, csz = jsz < A.nnz ? jsz : A.nnz - 1;
+            csx = csx < 1 ? 1 : csx;
+            csz = csz < 1 ? 1 : csz;
+            for (int k = 1; k <= 2; k++)
+                for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = vel(csz - 1 + l, csx - 1 + k);
+            double drx = (x - A.gox) - (jsx - 1) * dnx, drz = (z - A.goz) - (jsz - 1) * dnz;
+            double vels = ff_bilinear(vss, drx, drz, dnx, dnz);
+            for (int k = 1; k <= 2; k++)
+                for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = vel(irz - 1 + l, irx - 1 + k);
+            drx = (rx - A.gox) - (irx - 1) * dnx;
+            drz = (rz - A.goz) - (irz - 1) * dnz;
+            double velr = ff_bilinear(vss, drx, drz, dnx, dnz);
+            trr = 2.0 * sred / (vels + velr);
+        } else {
+            double drx = (rx - A.gox) - (irx - 1) * dnx;
+            double drz = (rz - A.goz) - (irz - 1) * dnz;
+            trr = 0.0;
+            for (int k = 1; k <= 2; k++)
+                for (int l = 1; l <= 2; l++) {
+                    double produ = (1.0 - fabs(((l - 1) * dnz - drz) / dnz)) * (1.0 - fabs(((k - 1) * dnx - drx) / dnx));
+                    trr = trr + V.t(irz - 1 + l, irx - 1 + k) * produ;
+                }
+        }
+        out[r - 1] = trr;
+    }
+}
+
+// LDS bytes of k_fm2d_near for the geometry in a.
+static size_t ff_near_lds(const mct_fm2d_args &a, int &region, int &heap_cap, int &win)
+{
+    const int rg = 2 * a.sgs * a.sgdl + 1;
+    const int wn = 2 * a.sgs + 1;
+    const int ln = (a.asgr == 1 ? wn : 2) + 2 * FF_MARGIN;
+    const int lnx = ln < a.nnx ? ln : a.nnx, lnz = ln < a.nnz ? ln : a.nnz;
+    const int rgx = rg < (a.nnx - 1) * a.sgdl + 1 ? rg : (a.nnx - 1) * a.sgdl + 1;
+    const int rgz = rg < (a.nnz - 1) * a.sgdl + 1 ? rg : (a.nnz - 1) * a.sgdl + 1;
+    const int rnodes = a.asgr == 1 ? rgx * rgz : 0;
+    region = rnodes > lnx * lnz ? rnodes : lnx * lnz;
+    heap_cap = region;
+    win = a.asgr == 1 ? wn * wn : 1;
+    size_t b = 8 * (size_t)region + 2 * (size_t)((region + 3) & ~3) + 4 * (size_t)((heap_cap + 2 + 1) & ~1) +
+               8 * (size_t)win + 2 * (size_t)win + 64;
+    return b;
+}
+
+int mct_fm2d_fast_ok(const mct_fm2d_args &a)
+{
+    int region, heap_cap, win;
+    size_t b = ff_near_lds(a, region, heap_cap, win);
+    // heap positions and coordinates are 16-bit
+    return b <= 64 * 1024 && region < 32767 && a.nnx < 32767 && a.nnz < 32767;
+}
+
+int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hipStream_t s)
+{
+    int region, heap_cap, win;
+    size_t lds = ff_near_lds(a, region, heap_cap, win);
+    int rc = mct_launch_fm2d_gridder(a, s);
+    if (rc) return rc;
+    const long nprob = (long)a.nchain * a.np * a.nsrc;
+    if (chunk < 1) return MCT_ERR_ARG;
+    for (long p0 = 0; p0 < nprob; p0 += chunk) {
+        mct_fm2d_args b = a;
+        b.prob0 = p0;
+        b.nrun = nprob - p0 < chunk ? nprob - p0 : chunk;
+        hipLaunchKernelGGL(k_fm2d_near, dim3((unsigned)b.nrun), dim3(64), lds, s, b, region, heap_cap, win, pflag);
+        MCT_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_fm2d_sweep, dim3((unsigned)b.nrun), dim3(256), 0, s, b, (const int32_t *)pflag);
+        MCT_CHECK(hipGetLastError());
+        // problems whose heap loop left the LDS neighbourhood: the sequential solver
+        b.only_flag = pflag;
+        if ((rc = mct_launch_fm2d_solve(b, s))) return rc;
+    }
+    return MCT_OK;
+}

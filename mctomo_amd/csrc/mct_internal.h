@@ -169,12 +169,21 @@ struct mct_fm2d_args {
     int32_t *status;            // (nchain): MCT_ERR_OUTSIDE for a source or receiver outside the
                                 // grid; chains whose status is > 0 (rejected) are skipped
     long prob0, nrun;           // problems [prob0, prob0+nrun) use workspace slots 0..nrun-1
+    const int32_t *only_flag;   // sequential solver: when set, only slots whose flag is 1
 };
 // Coarse B-spline velocity per (chain, period), then one lane per
 // (chain, period, source) running the sequential modrays phase solve.
 // Workspace slots (ttn..heap) are sized for `chunk` problems; problems run in
 // chunks of that many.
 int mct_launch_fm2d(const mct_fm2d_args &a, long chunk, hipStream_t s);
+int mct_launch_fm2d_gridder(const mct_fm2d_args &a, hipStream_t s);
+int mct_launch_fm2d_solve(const mct_fm2d_args &b, hipStream_t s);
+// Parallel path: seed and source neighbourhood in LDS per problem, coarse
+// sweeps per problem, the sequential solver for flagged problems.  pflag
+// holds one int per workspace slot.  mct_fm2d_fast_ok tells whether the
+// geometry fits the LDS budget of the parallel path.
+int mct_fm2d_fast_ok(const mct_fm2d_args &a);
+int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hipStream_t s);
 // Bytes of fm2d workspace per problem slot.
 static inline long mct_fm2d_slot_bytes(long ld, long nc)
 {

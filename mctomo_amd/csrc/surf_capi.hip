@@ -79,6 +79,7 @@ struct mct_surf_ctx {
     mct_surf_setup S;
     mct_fm2d_args F;                 // geometry part of the 2-D solve
     long fm2d_chunk;
+    int fm2d_method;                 // 0 parallel sweeps, 1 sequential heap loop
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double ms[4] = {0, 0, 0, 0};
@@ -89,7 +90,7 @@ struct mct_surf_ctx {
     // stage buffers
     sbuf vs, flag_count, flag_nodes, kd_nodes, kd_ind, kd_rdata;
     sbuf phase, group, ierr, bad_model, ndltar, velv, veln, phase_time, srdist, sigma;
-    sbuf ttn, ttr, vr, nsts, nstr, heap;
+    sbuf ttn, ttr, vr, nsts, nstr, heap, pflag;
     double *h_in = nullptr, *h_out = nullptr;
     int32_t *h_status = nullptr;
 };
@@ -145,7 +146,11 @@ static int surf_pipeline(mct_surf_ctx *C, int nchain, const double *d_points, co
     fa.ttn = C->ttn.as<double>(); fa.ttr = C->ttr.as<double>(); fa.vr = C->vr.as<double>();
     fa.nsts = C->nsts.as<int32_t>(); fa.nstr = C->nstr.as<int32_t>(); fa.heap = C->heap.as<int32_t>();
     fa.ttime = C->phase_time.as<double>(); fa.status = d_status;
-    if ((rc = mct_launch_fm2d(fa, C->fm2d_chunk, s))) return rc;
+    if (C->fm2d_method == 0 && mct_fm2d_fast_ok(fa))
+        rc = mct_launch_fm2d_fast(fa, C->fm2d_chunk, C->pflag.as<int32_t>(), s);
+    else
+        rc = mct_launch_fm2d(fa, C->fm2d_chunk, s);
+    if (rc) return rc;
     MCT_CHECK(hipEventRecord(C->ev[3], s));
     // (d) misfit: srdist = phaseTime (src/likelihood_surf.F90:329-330)
     mct_misfit_args ma;
@@ -203,6 +208,10 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     long chunk = (4L << 30) / slot;
     chunk = chunk < 64 ? 64 : (chunk / 64) * 64;
     C->fm2d_chunk = chunk < nprob ? chunk : nprob;
+    {
+        const char *e = getenv("MCT_FM2D_METHOD");
+        C->fm2d_method = e && atoi(e) == 1 ? 1 : 0;
+    }
     int rc = MCT_OK;
     if (hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking) != hipSuccess) { delete C; return MCT_ERR_HIP; }
     for (int i = 0; i < 5; i++)
@@ -241,6 +250,7 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     ALLOC(nsts, sizeof(int32_t) * wsz * ch);
     ALLOC(nstr, sizeof(int32_t) * wsz * ch);
     ALLOC(heap, sizeof(int32_t) * (wsz + 8) * ch);
+    ALLOC(pflag, sizeof(int32_t) * ch);
 #undef ALLOC
     if (!rc && hipHostMalloc((void **)&C->h_in, sizeof(double) * (6 * (size_t)NC + 2 * np + 8), hipHostMallocDefault) != hipSuccess)
         rc = MCT_ERR_NOMEM;
@@ -280,7 +290,7 @@ int mct_surf_destroy(mct_surf_ctx *C)
                     &C->points, &C->params, &C->noise, &C->out, &C->status, &C->vs, &C->flag_count,
                     &C->flag_nodes, &C->kd_nodes, &C->kd_ind, &C->kd_rdata, &C->phase, &C->group,
                     &C->ierr, &C->bad_model, &C->ndltar, &C->velv, &C->veln, &C->phase_time, &C->srdist, &C->sigma,
-                    &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap};
+                    &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap, &C->pflag};
     for (sbuf *b : bufs) b->release();
     if (C->h_in) (void)hipHostFree(C->h_in);
     if (C->h_out) (void)hipHostFree(C->h_out);
@@ -356,6 +366,13 @@ int mct_surf_likelihood_batch_dev(mct_surf_ctx *C, int nchain, const double *poi
     MCT_CHECK(hipMemsetAsync(status_dev, 0, sizeof(int32_t) * nchain, C->stream));
     return surf_pipeline(C, nchain, points_dev, params_dev, ncells_dev, 0, noise_dev, out_dev, status_dev,
                          false);
+}
+
+int mct_surf_set_method(mct_surf_ctx *C, int fm2d_method)
+{
+    if (!C || (fm2d_method != 0 && fm2d_method != 1)) return MCT_ERR_ARG;
+    C->fm2d_method = fm2d_method;
+    return MCT_OK;
 }
 
 int mct_surf_dev_arrays(mct_surf_ctx *C, double **phase_time_dev, double **phase_vel_dev)

@@ -1,9 +1,12 @@
 """GPU 2-D phase traveltimes (modrays on gfx950) against the reference's golden
 vectors and the oracle.
 
-Each (period, source) runs the reference's sequential fast marching in one
-lane, so every time must be identical to the oracle and to the reference's
-compiled Fortran, bit for bit.
+method 1 runs the reference's sequential fast marching in one lane per
+(period, source): every time must equal the oracle's and the reference's
+compiled Fortran's, bit for bit.  method 0 runs the heap loop only near the
+source and sweeps the rest to the same fixed point with the same arithmetic:
+times must agree to 1e-6 relative (the traveltime gate is 1e-4), and in
+practice they are identical.
 """
 import os
 import sys
@@ -23,20 +26,35 @@ def say(*a):
     sys.stdout.flush()
 
 
-def test_golden_vectors():
+def agree(got, want, method, msg=""):
+    if method == 1:
+        np.testing.assert_array_equal(got, want, err_msg=msg)
+        return 1.0
+    m = np.isfinite(want) & (want > 0)
+    rel = np.abs(got[m] - want[m]) / want[m]
+    assert rel.max() <= 1e-6, (msg, rel.max())
+    assert np.array_equal(got[~m], want[~m]), msg
+    return float(np.mean(got == want))
+
+
+@pytest.mark.parametrize("method", [1, 0])
+def test_golden_vectors(method):
     g = np.load(GOLD)
     n = int(g["n"])
-    say("fm2d golden", n)
+    say("fm2d golden", n, "method", method)
+    ex = []
     for i in range(n):
         fom, asgr = (int(v) for v in g["opt_%d" % i])
         s, r = g["src_%d" % i], g["rec_%d" % i]
         tt = mct.modrays(g["velv_%d" % i], 0, 0, 0.5, 0.5, s[:, 0], s[:, 1], r[:, 0], r[:, 1],
-                         srs=g["srs_%d" % i], fom=fom, asgr=asgr)
-        np.testing.assert_array_equal(tt, g["tt_%d" % i], err_msg="golden case %d" % i)
+                         srs=g["srs_%d" % i], fom=fom, asgr=asgr, method=method)
+        ex.append(agree(tt, g["tt_%d" % i], method, "golden case %d" % i))
+    say("  identical fraction", np.mean(ex))
 
 
+@pytest.mark.parametrize("method", [1, 0])
 @pytest.mark.parametrize("gd,sgdl,sgs", [((1, 1), 4, 8), ((2, 2), 4, 8), ((2, 1), 2, 3), ((1, 3), 5, 2)])
-def test_multi_period_batch_matches_oracle(gd, sgdl, sgs):
+def test_multi_period_batch_matches_oracle(gd, sgdl, sgs, method):
     rng = np.random.default_rng(100 + 7 * gd[0] + gd[1] + sgdl + sgs)
     nvx, nvz, h = int(rng.integers(10, 30)), int(rng.integers(10, 30)), 0.6
     npd, ns, nr = 5, 6, 9
@@ -49,17 +67,19 @@ def test_multi_period_batch_matches_oracle(gd, sgdl, sgs):
     srs[1, 2, :] = 0                                   # a source with no pairs at one period
     for fom in (0, 1):
         for asgr in (0, 1):
-            say("fm2d batch", gd, sgdl, sgs, "fom", fom, "asgr", asgr)
+            say("fm2d batch", gd, sgdl, sgs, "fom", fom, "asgr", asgr, "method", method)
             got = mct.modrays(velv, ox, oz, h, h, sx, sz, rx, rz, srs=srs, gdx=gd[0], gdz=gd[1],
-                              asgr=asgr, sgdl=sgdl, sgs=sgs, fom=fom, ttime=np.full((npd, ns, nr), -1.0))
+                              asgr=asgr, sgdl=sgdl, sgs=sgs, fom=fom, ttime=np.full((npd, ns, nr), -1.0),
+                              method=method)
             for p in range(npd):
                 want = po.fm2d(velv[p], ox, oz, h, h, sx, sz, rx, rz, srs=srs[p], gdx=gd[0], gdz=gd[1],
                                asgr=asgr, sgdl=sgdl, sgs=sgs, fom=fom, ttime=np.full((ns, nr), -1.0))
-                np.testing.assert_array_equal(got[p], want, err_msg="period %d" % p)
+                agree(got[p], want, method, "period %d" % p)
             assert (got[srs == 0] == -1.0).all()
 
 
-def test_sources_on_grid_nodes_and_receivers_on_edges():
+@pytest.mark.parametrize("method", [1, 0])
+def test_sources_on_grid_nodes_and_receivers_on_edges(method):
     n, h = 25, 0.5
     rng = np.random.default_rng(3)
     velv = rng.uniform(2.5, 3.5, (n + 2, n + 2))
@@ -68,14 +88,41 @@ def test_sources_on_grid_nodes_and_receivers_on_edges():
     rx = np.array([0.0, L, 0.0, L, 5.25, 2.0])
     rz = np.array([0.0, 0.0, L, L, 0.0, L])
     for fom in (0, 1):
-        got = mct.modrays(velv, 0, 0, h, h, sx, sz, rx, rz, fom=fom)
+        got = mct.modrays(velv, 0, 0, h, h, sx, sz, rx, rz, fom=fom, method=method)
         want = po.fm2d(velv, 0, 0, h, h, sx, sz, rx, rz, fom=fom)
-        np.testing.assert_array_equal(got, want)
+        agree(got, want, method)
 
 
-def test_source_outside_the_grid_is_an_error():
+@pytest.mark.parametrize("method", [1, 0])
+def test_source_outside_the_grid_is_an_error(method):
     velv = np.full((12, 12), 3.0)
     with pytest.raises(mct.MCTError):
-        mct.modrays(velv, 0, 0, 1.0, 1.0, np.array([50.0]), np.array([2.0]), np.array([1.0]), np.array([1.0]))
+        mct.modrays(velv, 0, 0, 1.0, 1.0, np.array([50.0]), np.array([2.0]), np.array([1.0]), np.array([1.0]),
+                    method=method)
     with pytest.raises(mct.MCTError):
-        mct.modrays(velv, 0, 0, 1.0, 1.0, np.array([2.0]), np.array([2.0]), np.array([1.0]), np.array([-40.0]))
+        mct.modrays(velv, 0, 0, 1.0, 1.0, np.array([2.0]), np.array([2.0]), np.array([1.0]), np.array([-40.0]),
+                    method=method)
+
+
+@pytest.mark.parametrize("n,blk", [(101, 3), (101, 12), (201, 0)])
+def test_parallel_path_on_large_and_contrasted_grids(n, blk):
+    """Large grids and a 2.0/4.5 km/s checkerboard (long heap detours near the
+    source; problems that leave the LDS neighbourhood take the sequential path)."""
+    rng = np.random.default_rng(n + blk)
+    h = 0.5
+    L = (n - 1) * h
+    if blk:
+        ii = np.arange(n + 2)
+        velv = np.where(((ii[:, None] // blk) + (ii[None, :] // blk)) % 2 == 0, 2.0, 4.5)
+    else:
+        from scipy.ndimage import zoom
+        base = rng.uniform(2.6, 3.4, (n // 10 + 2, n // 10 + 2))
+        velv = zoom(base, (n + 2) / base.shape[0], order=1)[:n + 2, :n + 2]
+    ns, nr = 6, 12
+    sx, sz = rng.uniform(h, L - h, ns), rng.uniform(h, L - h, ns)
+    rx, rz = rng.uniform(0, L, nr), rng.uniform(0, L, nr)
+    for fom in (0, 1):
+        say("fm2d large", n, blk, "fom", fom)
+        got = mct.modrays(velv, 0, 0, h, h, sx, sz, rx, rz, fom=fom, method=0)
+        want = po.fm2d(velv, 0, 0, h, h, sx, sz, rx, rz, fom=fom)
+        say("  identical fraction", agree(got, want, 0))
