@@ -239,6 +239,12 @@ int mct_surf_last_timings(mct_surf_ctx *ctx, double *ms4);
  * source (default), 1 the sequential heap loop per (period, source). */
 int mct_surf_set_method(mct_surf_ctx *ctx, int fm2d_method);
 
+/* Work of the last call's dispersion stage over its first nchain chains:
+ * evals = period-equation evaluations of surfdisp96's sequential root walk
+ * (dltar calls), layer_evals = the same weighted by (layers - 1), the
+ * per-evaluation layer recursion count. */
+int mct_surf_dispersion_work(mct_surf_ctx *ctx, int nchain, int64_t *evals, int64_t *layer_evals);
+
 /* Device pointers of the context's phase_time (nchain,np,nsrc,nrev) and
  * phase velocity (nchain,nx,ny,np) buffers (valid until destroy). */
 int mct_surf_dev_arrays(mct_surf_ctx *ctx, double **phase_time_dev, double **phase_vel_dev);

@@ -63,7 +63,7 @@ def lib():
                      "mct_copy_d2h", "mct_surfdisp96_batch", "mct_modrays", "mct_surf_create",
                      "mct_surf_destroy", "mct_surf_likelihood", "mct_surf_likelihood_batch_dev",
                      "mct_surf_sync", "mct_surf_last_timings", "mct_surf_dev_arrays",
-                     "mct_surf_set_method"):
+                     "mct_surf_set_method", "mct_surf_dispersion_work"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -439,6 +439,13 @@ class SurfLikelihood:
     def set_method(self, fm2d_method):
         """2-D traveltimes: 0 parallel sweeps (default), 1 sequential heap loop."""
         _check(lib().mct_surf_set_method(self._h, int(fm2d_method)), "mct_surf_set_method")
+
+    def dispersion_work(self, nchain=1):
+        """(dltar evaluations, evaluations x (layers-1)) of the last call's first nchain chains."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().mct_surf_dispersion_work(self._h, int(nchain), ctypes.byref(a), ctypes.byref(b)),
+               "mct_surf_dispersion_work")
+        return a.value, b.value
 
     def dev_arrays(self):
         """Device addresses of the phase_time and phase-velocity buffers."""

@@ -362,9 +362,10 @@ __device__ static int d_getsol(double t1, double &c1io, double clow, double dc, 
         bool change = dsgn(delprev) != dsgn(del2);
         bool fail = !change && (cand < cm || cand >= cmax);
         unsigned long long ev = __ballot(change || fail);
-        ndl += 64;
         if (ev) {
             int e = __ffsll(ev) - 1;
+            ndl += e + 1;             // the sequential walk stops at step e
+
             bool ch = __shfl(change ? 1 : 0, e, 64) != 0;
             double ce = __shfl(cand, e, 64);
             if (!ch) { c1io = ce; return -1; }
@@ -376,6 +377,7 @@ __device__ static int d_getsol(double t1, double &c1io, double clow, double dc, 
             if (cn > (double)betmx) return -1;
             return 1;
         }
+        ndl += 64;
         // no event in these 64 steps: continue from the last one
         c1 = __shfl(cand, 63, 64);
         del1 = __shfl(del2, 63, 64);
@@ -519,6 +521,7 @@ __global__ __launch_bounds__(64) void k_dispersion(mct_dsp_args A)
     if (lane == 0) {
         A.ierr[(long)c * A.ncol + col] = ierr;
         if (A.ndltar) A.ndltar[(long)c * A.ncol + col] = ndl;
+        if (A.nlayer) A.nlayer[(long)c * A.ncol + col] = M.mmax;
     }
 }
 

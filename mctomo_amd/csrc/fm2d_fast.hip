@@ -11,7 +11,8 @@
 // reproduced as the reference does them:
 //   * the source: the four corner times of the source cell are heap keys that
 //     later pops overwrite, so the first pops run the reference's own heap
-//     loop until the four corners are frozen (the seed);
+//     loop until the source cell and two rings around it are frozen (the
+//     seed);
 //   * the refined source grid stops when the first node on an inner edge
 //     reaches the top of the heap (:143-163).  From the converged refined
 //     field the stop is rebuilt: frozen = times below that node's time, close
@@ -41,6 +42,7 @@
 #define FF_FIN 1e30
 #define FF_MARGIN 16          // coarse neighbourhood: refined window +- this many nodes
 #define FF_MAX_ROUNDS 512
+#define FF_SEED_RINGS 2       // the seed freezes the source cell and this many rings around it
 
 namespace {
 
@@ -642,7 +644,11 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
                 S.mode = 1;
                 S.vnl = g.vnl; S.vnr = g.vnr; S.vnt = g.vnt; S.vnb = g.vnb;
                 // the block rule is checked with the exit rule: run mode 1 while counting
-                S.x0 = risx; S.x1 = risx + 1; S.z0 = risz; S.z1 = risz + 1; S.need = 4;
+                S.x0 = risx - FF_SEED_RINGS < 1 ? 1 : risx - FF_SEED_RINGS;
+                S.x1 = risx + 1 + FF_SEED_RINGS > R.nnx ? R.nnx : risx + 1 + FF_SEED_RINGS;
+                S.z0 = risz - FF_SEED_RINGS < 1 ? 1 : risz - FF_SEED_RINGS;
+                S.z1 = risz + 1 + FF_SEED_RINGS > R.nnz ? R.nnz : risz + 1 + FF_SEED_RINGS;
+                S.need = (S.x1 - S.x0 + 1) * (S.z1 - S.z0 + 1);
                 auto al = [&](int zz, int xx) -> bool { return R.n(zz, xx) == 0; };
                 (void)al;
                 // heap loop with both rules (exit first, as the reference checks it first)
@@ -796,7 +802,12 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
             ff_stop S;
             memset(&S, 0, sizeof S);
             S.mode = 2;
-            S.x0 = g.isx; S.x1 = g.isx + 1; S.z0 = g.isz; S.z1 = g.isz + 1; S.need = 4; S.have = 0;
+            S.x0 = g.isx - FF_SEED_RINGS < 1 ? 1 : g.isx - FF_SEED_RINGS;
+            S.x1 = g.isx + 1 + FF_SEED_RINGS > A.nnx ? A.nnx : g.isx + 1 + FF_SEED_RINGS;
+            S.z0 = g.isz - FF_SEED_RINGS < 1 ? 1 : g.isz - FF_SEED_RINGS;
+            S.z1 = g.isz + 1 + FF_SEED_RINGS > A.nnz ? A.nnz : g.isz + 1 + FF_SEED_RINGS;
+            S.need = (S.x1 - S.x0 + 1) * (S.z1 - S.z0 + 1);
+            S.have = 0;
             ff_heap_loop(C, H, cvel, A.fom, S);
             sh_ovf = S.overflow;
         }

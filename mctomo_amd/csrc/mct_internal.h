@@ -205,7 +205,9 @@ struct mct_dsp_args {
     double *phase, *group;        // (nchain, ncol, np)
     int32_t *ierr;                // (nchain, ncol)
     int32_t *bad_model;           // (nchain): check_model failed
-    int32_t *ndltar;              // (nchain, ncol) or NULL: period-equation evaluations
+    int32_t *ndltar;              // (nchain, ncol) or NULL: period-equation evaluations of the
+                                  // sequential walk (surfdisp96's own count)
+    int32_t *nlayer;              // (nchain, ncol) or NULL: layers of the column
 };
 // One wave per (column, chain): check_model, convert_to_layer, surfdisp96.
 // Requires np <= 60 and nz <= 100.

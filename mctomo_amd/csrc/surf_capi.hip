@@ -89,7 +89,7 @@ struct mct_surf_ctx {
     sbuf points, params, noise, out, status;
     // stage buffers
     sbuf vs, flag_count, flag_nodes, kd_nodes, kd_ind, kd_rdata;
-    sbuf phase, group, ierr, bad_model, ndltar, velv, veln, phase_time, srdist, sigma;
+    sbuf phase, group, ierr, bad_model, ndltar, nlayer, velv, veln, phase_time, srdist, sigma;
     sbuf ttn, ttr, vr, nsts, nstr, heap, pflag;
     double *h_in = nullptr, *h_out = nullptr;
     int32_t *h_status = nullptr;
@@ -125,6 +125,7 @@ static int surf_pipeline(mct_surf_ctx *C, int nchain, const double *d_points, co
     da.iwave = S.raylov == 1 ? 2 : 1; da.igr = 0; da.dphase = S.dphase;
     da.phase = C->phase.as<double>(); da.group = C->group.as<double>(); da.ierr = C->ierr.as<int32_t>();
     da.bad_model = C->bad_model.as<int32_t>(); da.ndltar = C->ndltar.as<int32_t>();
+    da.nlayer = C->nlayer.as<int32_t>();
     if ((rc = mct_launch_dispersion(da, s))) return rc;
     hipLaunchKernelGGL(k_surf_status, dim3(nchain), dim3(256), 0, s, (const int32_t *)da.bad_model,
                        (const int32_t *)da.ierr, ncol, d_status);
@@ -239,6 +240,7 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     ALLOC(ierr, sizeof(int32_t) * ncol * M);
     ALLOC(bad_model, sizeof(int32_t) * M);
     ALLOC(ndltar, sizeof(int32_t) * ncol * M);
+    ALLOC(nlayer, sizeof(int32_t) * ncol * M);
     ALLOC(velv, sizeof(double) * (grid->nx + 2) * (grid->ny + 2) * np * M);
     ALLOC(veln, sizeof(double) * nnx * nnz * np * M);
     ALLOC(phase_time, sizeof(double) * npair * np * M);
@@ -275,6 +277,8 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
         }
         free(per); free(xy);
     }
+    // like%phaseTime starts at zero; pairs that are never traced keep it
+    if (!rc && hipMemset(C->phase_time.p, 0, sizeof(double) * npair * np * M) != hipSuccess) rc = MCT_ERR_HIP;
     if (!rc && hipMemcpy(C->ttime.p, S.ttime, sizeof(double) * 3 * npair * np, hipMemcpyHostToDevice) != hipSuccess) rc = MCT_ERR_HIP;
     if (!rc && hipMemcpy(C->raystat.p, S.raystat, sizeof(int32_t) * 2 * npair * np, hipMemcpyHostToDevice) != hipSuccess) rc = MCT_ERR_HIP;
     if (rc) { mct_surf_destroy(C); return rc; }
@@ -289,7 +293,7 @@ int mct_surf_destroy(mct_surf_ctx *C)
     sbuf *bufs[] = {&C->periods, &C->scx, &C->scz, &C->rcx, &C->rcz, &C->ttime, &C->raystat,
                     &C->points, &C->params, &C->noise, &C->out, &C->status, &C->vs, &C->flag_count,
                     &C->flag_nodes, &C->kd_nodes, &C->kd_ind, &C->kd_rdata, &C->phase, &C->group,
-                    &C->ierr, &C->bad_model, &C->ndltar, &C->velv, &C->veln, &C->phase_time, &C->srdist, &C->sigma,
+                    &C->ierr, &C->bad_model, &C->ndltar, &C->nlayer, &C->velv, &C->veln, &C->phase_time, &C->srdist, &C->sigma,
                     &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap, &C->pflag};
     for (sbuf *b : bufs) b->release();
     if (C->h_in) (void)hipHostFree(C->h_in);
@@ -372,6 +376,26 @@ int mct_surf_set_method(mct_surf_ctx *C, int fm2d_method)
 {
     if (!C || (fm2d_method != 0 && fm2d_method != 1)) return MCT_ERR_ARG;
     C->fm2d_method = fm2d_method;
+    return MCT_OK;
+}
+
+int mct_surf_dispersion_work(mct_surf_ctx *C, int nchain, int64_t *evals, int64_t *layer_evals)
+{
+    if (!C || nchain < 1 || nchain > C->S.max_chains) return MCT_ERR_ARG;
+    const long n = (long)nchain * C->G.nx * C->G.ny;
+    int32_t *a = (int32_t *)malloc(sizeof(int32_t) * n), *b = (int32_t *)malloc(sizeof(int32_t) * n);
+    if (!a || !b) { free(a); free(b); return MCT_ERR_NOMEM; }
+    int rc = MCT_OK;
+    if (hipStreamSynchronize(C->stream) != hipSuccess ||
+        hipMemcpy(a, C->ndltar.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(b, C->nlayer.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = MCT_ERR_HIP;
+    int64_t e = 0, le = 0;
+    for (long i = 0; !rc && i < n; i++) { e += a[i]; le += (int64_t)a[i] * (b[i] > 1 ? b[i] - 1 : 0); }
+    free(a); free(b);
+    if (rc) return rc;
+    if (evals) *evals = e;
+    if (layer_evals) *layer_evals = le;
     return MCT_OK;
 }
 
