@@ -245,6 +245,10 @@ int mct_surf_set_method(mct_surf_ctx *ctx, int fm2d_method);
  * per-evaluation layer recursion count. */
 int mct_surf_dispersion_work(mct_surf_ctx *ctx, int nchain, int64_t *evals, int64_t *layer_evals);
 
+/* Sweep rounds of the parallel 2-D path in the last launch chunk: slots
+ * swept, largest and mean rounds (one round = four sweep directions). */
+int mct_surf_fm2d_stats(mct_surf_ctx *ctx, int64_t *nslots, int32_t *rounds_max, double *rounds_mean);
+
 /* Device pointers of the context's phase_time (nchain,np,nsrc,nrev) and
  * phase velocity (nchain,nx,ny,np) buffers (valid until destroy). */
 int mct_surf_dev_arrays(mct_surf_ctx *ctx, double **phase_time_dev, double **phase_vel_dev);

@@ -63,7 +63,7 @@ def lib():
                      "mct_copy_d2h", "mct_surfdisp96_batch", "mct_modrays", "mct_surf_create",
                      "mct_surf_destroy", "mct_surf_likelihood", "mct_surf_likelihood_batch_dev",
                      "mct_surf_sync", "mct_surf_last_timings", "mct_surf_dev_arrays",
-                     "mct_surf_set_method", "mct_surf_dispersion_work"):
+                     "mct_surf_set_method", "mct_surf_dispersion_work", "mct_surf_fm2d_stats"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -446,6 +446,13 @@ class SurfLikelihood:
         _check(lib().mct_surf_dispersion_work(self._h, int(nchain), ctypes.byref(a), ctypes.byref(b)),
                "mct_surf_dispersion_work")
         return a.value, b.value
+
+    def fm2d_stats(self):
+        """(slots swept, max rounds, mean rounds) of the last 2-D launch chunk."""
+        n, mx, mean = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_double()
+        _check(lib().mct_surf_fm2d_stats(self._h, ctypes.byref(n), ctypes.byref(mx), ctypes.byref(mean)),
+               "mct_surf_fm2d_stats")
+        return n.value, mx.value, mean.value
 
     def dev_arrays(self):
         """Device addresses of the phase_time and phase-velocity buffers."""

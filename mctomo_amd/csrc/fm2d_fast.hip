@@ -259,33 +259,266 @@ __device__ double ff_cand(const fview &V, int iz, int ix, double slown, int fom)
     return c;
 }
 
+// The same updates on a register copy of the node's axis stencil: [0] the
+// x-1 / z-1 side, [1] the x+1 / z+1 side; t1 one node away, t2 two away;
+// in* false outside the grid.  Arithmetic identical to ff_fouds1/ff_fouds2.
+struct ff_sten {
+    double tx1[2], tx2[2], tz1[2], tz2[2];
+    bool ix1[2], ix2[2], iz1[2], iz2[2];
+};
+
+__device__ __forceinline__ bool ff_al(double t, bool in, double thr) { return in && t < thr && t < FF_FIN; }
+
+__device__ double ff_fouds1s(const ff_sten &S, double thr, double dnx, double dnz, double slown)
+{
+    int tsw1 = 0;
+    double travm = 0.0;
+    for (int sj = 0; sj < 2; sj++)
+        for (int sk = 0; sk < 2; sk++) {
+            if (!(S.ix1[sj] && S.iz1[sk])) continue;
+            int swsol = 0;
+            double a = 0, b = 0, c = 0, tref = 0;
+            const double tj = S.tx1[sj], tk = S.tz1[sk];
+            if (ff_al(tj, true, thr)) {
+                swsol = 1;
+                if (ff_al(tk, true, thr)) {
+                    double u = dnx, v = dnz;
+                    double em = tk - tj;
+                    a = u * u + v * v;
+                    b = -(2.0 * (u * u) * em);
+                    c = (u * u) * (em * em - (v * v) * (slown * slown));
+                    tref = tj;
+                } else {
+                    a = 1.0; b = 0.0;
+                    c = -((slown * slown) * (dnx * dnx));
+                    tref = tj;
+                }
+            } else if (ff_al(tk, true, thr)) {
+                swsol = 1;
+                a = 1.0; b = 0.0;
+                c = -((slown * dnz) * (slown * dnz));
+                tref = tk;
+            }
+            if (swsol == 1) {
+                double rd1 = b * b - 4.0 * a * c;
+                if (rd1 < 0.0) rd1 = 0.0;
+                double tdsh = (-b + sqrt(rd1)) / (2.0 * a);
+                double trav = tref + tdsh;
+                if (tsw1 == 1) travm = fmin(trav, travm);
+                else { travm = trav; tsw1 = 1; }
+            }
+        }
+    return travm;
+}
+
+__device__ double ff_fouds2s(const ff_sten &S, double thr, double dnx, double dnz, double slown)
+{
+    int tsw1 = 0;
+    double travm = 0.0;
+    for (int sj = 0; sj < 2; sj++) {
+        if (!S.ix1[sj]) continue;
+        const double tj = S.tx1[sj], tj2 = S.tx2[sj];
+        int swj = -1;
+        if (ff_al(tj2, S.ix2[sj], thr)) swj = 0;
+        if (ff_al(tj, true, thr) && swj == 0) {
+            swj = -1;
+            if (tj > tj2) swj = 0;
+        } else {
+            swj = -1;
+        }
+        for (int sk = 0; sk < 2; sk++) {
+            if (!S.iz1[sk]) continue;
+            const double tk = S.tz1[sk], tk2 = S.tz2[sk];
+            int swk = -1;
+            if (ff_al(tk2, S.iz2[sk], thr)) swk = 0;
+            if (ff_al(tk, true, thr) && swk == 0) {
+                swk = -1;
+                if (tk > tk2) swk = 0;
+            } else {
+                swk = -1;
+            }
+            int swsol = 0;
+            double a = 0, b = 0, c = 0, tref = 0, tdiv = 1.0, u, v, em;
+            if (swj == 0) {
+                swsol = 1;
+                if (swk == 0) {
+                    u = 2.0 * dnx;
+                    v = 2.0 * dnz;
+                    em = 4.0 * tj - tj2 - 4.0 * tk;
+                    em = em + tk2;
+                    a = v * v + u * u;
+                    b = 2.0 * em * (u * u);
+                    c = (u * u) * (em * em - (slown * slown) * (v * v));
+                    tref = 4.0 * tj - tj2;
+                    tdiv = 3.0;
+                } else if (ff_al(tk, true, thr)) {
+                    u = dnz;
+                    v = 2.0 * dnx;
+                    em = 3.0 * tk - 4.0 * tj + tj2;
+                    a = v * v + 9.0 * (u * u);
+                    b = 6.0 * em * (u * u);
+                    c = (u * u) * (em * em - (slown * slown) * (v * v));
+                    tref = tk;
+                    tdiv = 1.0;
+                } else {
+                    u = 2.0 * dnx;
+                    a = 1.0; b = 0.0;
+                    c = -((u * u) * (slown * slown));
+                    tref = 4.0 * tj - tj2;
+                    tdiv = 3.0;
+                }
+            } else if (ff_al(tj, true, thr)) {
+                swsol = 1;
+                if (swk == 0) {
+                    u = dnx;
+                    v = 2.0 * dnz;
+                    em = 3.0 * tj - 4.0 * tk + tk2;
+                    a = v * v + 9.0 * (u * u);
+                    b = 6.0 * em * (u * u);
+                    c = (u * u) * (em * em - (v * v) * (slown * slown));
+                    tref = tj;
+                    tdiv = 1.0;
+                } else if (ff_al(tk, true, thr)) {
+                    u = dnx;
+                    v = dnz;
+                    em = tk - tj;
+                    a = u * u + v * v;
+                    b = -(2.0 * (u * u) * em);
+                    c = (u * u) * (em * em - (v * v) * (slown * slown));
+                    tref = tj;
+                    tdiv = 1.0;
+                } else {
+                    a = 1.0; b = 0.0;
+                    c = -((slown * slown) * (dnx * dnx));
+                    tref = tj;
+                    tdiv = 1.0;
+                }
+            } else {
+                if (swk == 0) {
+                    swsol = 1;
+                    u = 2.0 * dnz;
+                    a = 1.0; b = 0.0;
+                    c = -((u * u) * (slown * slown));
+                    tref = 4.0 * tk - tk2;
+                    tdiv = 3.0;
+                } else if (ff_al(tk, true, thr)) {
+                    swsol = 1;
+                    a = 1.0; b = 0.0;
+                    c = -((slown * slown) * (dnz * dnz));
+                    tref = tk;
+                    tdiv = 1.0;
+                }
+            }
+            if (swsol == 1) {
+                double rd1 = b * b - 4.0 * a * c;
+                if (rd1 < 0.0) rd1 = 0.0;
+                double tdsh = (-b + sqrt(rd1)) / (2.0 * a);
+                double trav = (tref + tdsh) / tdiv;
+                if (tsw1 == 1) travm = fmin(trav, travm);
+                else { travm = trav; tsw1 = 1; }
+            }
+        }
+    }
+    return travm;
+}
+
+__device__ __forceinline__ double ff_cand1s(const ff_sten &S, double thr, double dnx, double dnz,
+                                            double slown, int fom)
+{
+    bool any = ff_al(S.tz1[0], S.iz1[0], thr) || ff_al(S.tz1[1], S.iz1[1], thr) ||
+               ff_al(S.tx1[0], S.ix1[0], thr) || ff_al(S.tx1[1], S.ix1[1], thr);
+    if (!any) return FF_BIG;
+    return fom == 0 ? ff_fouds1s(S, thr, dnx, dnz, slown) : ff_fouds2s(S, thr, dnx, dnz, slown);
+}
+
+__device__ double ff_cands(const ff_sten &S, double dnx, double dnz, double slown, int fom)
+{
+    double c = ff_cand1s(S, FF_THR0, dnx, dnz, slown, fom);
+    for (int k = 0; k < 8; k++) {
+        double c2 = ff_cand1s(S, c, dnx, dnz, slown, fom);
+        if (c2 == c || c2 >= FF_THR0) break;
+        c = c2;
+    }
+    return c;
+}
+
+__device__ __forceinline__ void ff_load_sten(const fview &V, int iz, int ix, ff_sten &S)
+{
+    S.ix1[0] = ix - 1 >= 1; S.ix1[1] = ix + 1 <= V.nnx;
+    S.ix2[0] = ix - 2 >= 1; S.ix2[1] = ix + 2 <= V.nnx;
+    S.iz1[0] = iz - 1 >= 1; S.iz1[1] = iz + 1 <= V.nnz;
+    S.iz2[0] = iz - 2 >= 1; S.iz2[1] = iz + 2 <= V.nnz;
+    S.tx1[0] = S.ix1[0] ? V.t(iz, ix - 1) : FF_BIG;
+    S.tx1[1] = S.ix1[1] ? V.t(iz, ix + 1) : FF_BIG;
+    S.tx2[0] = S.ix2[0] ? V.t(iz, ix - 2) : FF_BIG;
+    S.tx2[1] = S.ix2[1] ? V.t(iz, ix + 2) : FF_BIG;
+    S.tz1[0] = S.iz1[0] ? V.t(iz - 1, ix) : FF_BIG;
+    S.tz1[1] = S.iz1[1] ? V.t(iz + 1, ix) : FF_BIG;
+    S.tz2[0] = S.iz2[0] ? V.t(iz - 2, ix) : FF_BIG;
+    S.tz2[1] = S.iz2[1] ? V.t(iz + 2, ix) : FF_BIG;
+}
+
 // Gauss-Seidel rounds over anti-diagonals in four directions until a round
 // changes no time by more than 1e-14 relative.  fixed(z,x) nodes are kept.
-// Returns the rounds used, or -1 without convergence.
+// dirty (LDS bitmap over the grid, z fastest, or null) marks the nodes whose
+// stencil changed since their last update; clean nodes are skipped, and a
+// node that changes marks the nodes whose stencil holds it (two along each
+// axis).  Returns the rounds used, or -1 without convergence.
 template <class FX, class VL>
-__device__ int ff_sweep(const fview &V, const FX &fixed, const VL &vel, int fom, int *sh_flag)
+__device__ int ff_sweep(const fview &V, const FX &fixed, const VL &vel, int fom, int *sh_flag,
+                        uint32_t *dirty)
 {
     const int nt = blockDim.x, tid = threadIdx.x;
+    const int reach = fom == 0 ? 1 : 2;
+    const long nn = (long)V.nnx * V.nnz;
+    auto bit = [&](int iz, int ix) -> long { return (long)(ix - 1) * V.nnz + (iz - 1); };
+    if (dirty) {
+        for (long w = tid; w < (nn + 31) / 32; w += nt) dirty[w] = 0xffffffffu;
+        __syncthreads();
+    }
     for (int round = 0; round < FF_MAX_ROUNDS; round++) {
         int changed = 0;
         for (int dir = 0; dir < 4; dir++) {
             for (int d = 0; d < V.nnx + V.nnz - 1; d++) {
                 int a_lo = d - (V.nnz - 1) > 0 ? d - (V.nnz - 1) : 0;
                 int a_hi = d < V.nnx - 1 ? d : V.nnx - 1;
+                int wrote = 0;
                 for (int a = a_lo + tid; a <= a_hi; a += nt) {
                     int b = d - a;
                     int x = (dir & 1) ? V.nnx - a : 1 + a;
                     int z = (dir & 2) ? V.nnz - b : 1 + b;
+                    if (dirty) {
+                        long q = bit(z, x);
+                        uint32_t m = 1u << (q & 31);
+                        if (!(dirty[q >> 5] & m)) continue;
+                        atomicAnd(&dirty[q >> 5], ~m);
+                    }
                     if (fixed(z, x)) continue;
+                    ff_sten S;
+                    ff_load_sten(V, z, x, S);
+                    const double t = V.t(z, x);
                     double slown = 1.0 / vel(z, x);
-                    double c = ff_cand(V, z, x, slown, fom);
-                    double t = V.t(z, x);
+                    double c = ff_cands(S, V.dnx, V.dnz, slown, fom);
                     if (c < FF_THR0 && c != t) {
-                        if (fabs(c - t) > 1e-14 * c) changed = 1;
+                        bool big = fabs(c - t) > 1e-14 * c;
+                        if (big) changed = 1;
                         V.t(z, x) = c;
+                        wrote = 1;
+                        if (dirty && big) {
+                            for (int r = 1; r <= reach; r++) {
+                                if (x - r >= 1) { long q = bit(z, x - r); atomicOr(&dirty[q >> 5], 1u << (q & 31)); }
+                                if (x + r <= V.nnx) { long q = bit(z, x + r); atomicOr(&dirty[q >> 5], 1u << (q & 31)); }
+                                if (z - r >= 1) { long q = bit(z - r, x); atomicOr(&dirty[q >> 5], 1u << (q & 31)); }
+                                if (z + r <= V.nnz) { long q = bit(z + r, x); atomicOr(&dirty[q >> 5], 1u << (q & 31)); }
+                            }
+                        }
                     }
                 }
-                __syncthreads();
+                if (dirty) {
+                    if (__syncthreads_or(wrote)) continue;
+                } else {
+                    __syncthreads();
+                }
             }
         }
         if (tid == 0) *sh_flag = 0;
@@ -555,6 +788,7 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
     int32_t *HB = (int32_t *)(RN + ((region_nodes + 3) & ~3));
     double *WT = (double *)(HB + ((heap_cap + 2 + 1) & ~1));
     int16_t *WN = (int16_t *)(WT + win_nodes);
+    uint32_t *DB = (uint32_t *)(WN + ((win_nodes + 1) & ~1));
     __shared__ int sh_flag, sh_exit, sh_ovf;
     __shared__ double sh_texit;
     __shared__ int sh_eidx;
@@ -695,7 +929,7 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
             }
             __syncthreads();
             auto rfixed = [&](int iz, int ix) -> bool { return R.n(iz, ix) == 0; };
-            int rounds = ff_sweep(R, rfixed, rvel, A.fom, &sh_flag);
+            int rounds = ff_sweep(R, rfixed, rvel, A.fom, &sh_flag, DB);
             if (rounds < 0) {
                 if (lane == 0) pflag[slot] = 1;
                 return;
@@ -830,8 +1064,9 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
 }
 
 // One workgroup per problem: sweep the coarse grid, then pick the receivers.
-__global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, const int32_t *pflag)
+__global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, const int32_t *pflag, int use_dirty)
 {
+    extern __shared__ uint32_t ff_dirty[];
     __shared__ int sh_flag;
     const long slot = blockIdx.x;
     if (pflag[slot] != 0) return;
@@ -847,11 +1082,12 @@ __global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, const int32
     const int32_t *F = A.nsts + slot * wsz;
     auto fixed = [&](int iz, int ix) -> bool { return F[(long)(ix - 1) * A.ld + (iz - 1)] != 0; };
     auto vel = [&](int iz, int ix) -> double { return velnc[(long)(ix - 1) * A.nnz + (iz - 1)]; };
-    int rounds = ff_sweep(V, fixed, vel, A.fom, &sh_flag);
+    int rounds = ff_sweep(V, fixed, vel, A.fom, &sh_flag, use_dirty ? ff_dirty : nullptr);
     if (rounds < 0) {
         if (threadIdx.x == 0) atomicMin(&A.status[g.c], (int)MCT_ERR_NOCONV);
         return;
     }
+    if (threadIdx.x == 0 && A.rounds_out) A.rounds_out[slot] = rounds;
     // srtimes
     const double x = A.scx[g.s], z = A.scz[g.s];
     const int32_t *srs = A.srs + (long)g.p * A.srs_pstride + (long)g.s * A.nrc;
@@ -917,7 +1153,7 @@ static size_t ff_near_lds(const mct_fm2d_args &a, int &region, int &heap_cap, in
     heap_cap = region;
     win = a.asgr == 1 ? wn * wn : 1;
     size_t b = 8 * (size_t)region + 2 * (size_t)((region + 3) & ~3) + 4 * (size_t)((heap_cap + 2 + 1) & ~1) +
-               8 * (size_t)win + 2 * (size_t)win + 64;
+               8 * (size_t)win + 2 * (size_t)((win + 1) & ~1) + 4 * (size_t)((region + 31) / 32) + 64;
     return b;
 }
 
@@ -935,6 +1171,10 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
     size_t lds = ff_near_lds(a, region, heap_cap, win);
     int rc = mct_launch_fm2d_gridder(a, s);
     if (rc) return rc;
+    // the sweep's dirty bitmap in LDS when it fits
+    const size_t dirty_bytes = 4 * (((size_t)a.nnx * a.nnz + 31) / 32);
+    const int use_dirty = dirty_bytes <= 48 * 1024;
+    const size_t dirty_lds = use_dirty ? dirty_bytes : 0;
     const long nprob = (long)a.nchain * a.np * a.nsrc;
     if (chunk < 1) return MCT_ERR_ARG;
     for (long p0 = 0; p0 < nprob; p0 += chunk) {
@@ -943,7 +1183,8 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
         b.nrun = nprob - p0 < chunk ? nprob - p0 : chunk;
         hipLaunchKernelGGL(k_fm2d_near, dim3((unsigned)b.nrun), dim3(64), lds, s, b, region, heap_cap, win, pflag);
         MCT_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_fm2d_sweep, dim3((unsigned)b.nrun), dim3(256), 0, s, b, (const int32_t *)pflag);
+        hipLaunchKernelGGL(k_fm2d_sweep, dim3((unsigned)b.nrun), dim3(256), dirty_lds, s, b,
+                           (const int32_t *)pflag, use_dirty);
         MCT_CHECK(hipGetLastError());
         // problems whose heap loop left the LDS neighbourhood: the sequential solver
         b.only_flag = pflag;

@@ -90,7 +90,7 @@ struct mct_surf_ctx {
     // stage buffers
     sbuf vs, flag_count, flag_nodes, kd_nodes, kd_ind, kd_rdata;
     sbuf phase, group, ierr, bad_model, ndltar, nlayer, velv, veln, phase_time, srdist, sigma;
-    sbuf ttn, ttr, vr, nsts, nstr, heap, pflag;
+    sbuf ttn, ttr, vr, nsts, nstr, heap, pflag, rounds;
     double *h_in = nullptr, *h_out = nullptr;
     int32_t *h_status = nullptr;
 };
@@ -147,6 +147,7 @@ static int surf_pipeline(mct_surf_ctx *C, int nchain, const double *d_points, co
     fa.ttn = C->ttn.as<double>(); fa.ttr = C->ttr.as<double>(); fa.vr = C->vr.as<double>();
     fa.nsts = C->nsts.as<int32_t>(); fa.nstr = C->nstr.as<int32_t>(); fa.heap = C->heap.as<int32_t>();
     fa.ttime = C->phase_time.as<double>(); fa.status = d_status;
+    fa.rounds_out = C->rounds.as<int32_t>();
     if (C->fm2d_method == 0 && mct_fm2d_fast_ok(fa))
         rc = mct_launch_fm2d_fast(fa, C->fm2d_chunk, C->pflag.as<int32_t>(), s);
     else
@@ -253,6 +254,8 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     ALLOC(nstr, sizeof(int32_t) * wsz * ch);
     ALLOC(heap, sizeof(int32_t) * (wsz + 8) * ch);
     ALLOC(pflag, sizeof(int32_t) * ch);
+    ALLOC(rounds, sizeof(int32_t) * ch);
+    if (!rc && hipMemset(C->rounds.p, 0, sizeof(int32_t) * ch) != hipSuccess) rc = MCT_ERR_HIP;
 #undef ALLOC
     if (!rc && hipHostMalloc((void **)&C->h_in, sizeof(double) * (6 * (size_t)NC + 2 * np + 8), hipHostMallocDefault) != hipSuccess)
         rc = MCT_ERR_NOMEM;
@@ -294,7 +297,7 @@ int mct_surf_destroy(mct_surf_ctx *C)
                     &C->points, &C->params, &C->noise, &C->out, &C->status, &C->vs, &C->flag_count,
                     &C->flag_nodes, &C->kd_nodes, &C->kd_ind, &C->kd_rdata, &C->phase, &C->group,
                     &C->ierr, &C->bad_model, &C->ndltar, &C->nlayer, &C->velv, &C->veln, &C->phase_time, &C->srdist, &C->sigma,
-                    &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap, &C->pflag};
+                    &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap, &C->pflag, &C->rounds};
     for (sbuf *b : bufs) b->release();
     if (C->h_in) (void)hipHostFree(C->h_in);
     if (C->h_out) (void)hipHostFree(C->h_out);
@@ -370,6 +373,28 @@ int mct_surf_likelihood_batch_dev(mct_surf_ctx *C, int nchain, const double *poi
     MCT_CHECK(hipMemsetAsync(status_dev, 0, sizeof(int32_t) * nchain, C->stream));
     return surf_pipeline(C, nchain, points_dev, params_dev, ncells_dev, 0, noise_dev, out_dev, status_dev,
                          false);
+}
+
+int mct_surf_fm2d_stats(mct_surf_ctx *C, int64_t *nslots, int32_t *rounds_max, double *rounds_mean)
+{
+    if (!C) return MCT_ERR_ARG;
+    const long n = C->fm2d_chunk;
+    int32_t *r = (int32_t *)malloc(sizeof(int32_t) * n);
+    if (!r) return MCT_ERR_NOMEM;
+    if (hipStreamSynchronize(C->stream) != hipSuccess ||
+        hipMemcpy(r, C->rounds.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+        free(r);
+        return MCT_ERR_HIP;
+    }
+    long cnt = 0, sum = 0;
+    int mx = 0;
+    for (long i = 0; i < n; i++)
+        if (r[i] > 0 && r[i] < 100000) { cnt++; sum += r[i]; if (r[i] > mx) mx = r[i]; }
+    free(r);
+    if (nslots) *nslots = cnt;
+    if (rounds_max) *rounds_max = mx;
+    if (rounds_mean) *rounds_mean = cnt ? (double)sum / cnt : 0.0;
+    return MCT_OK;
 }
 
 int mct_surf_set_method(mct_surf_ctx *C, int fm2d_method)
