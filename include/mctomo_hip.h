@@ -239,6 +239,10 @@ int mct_surf_last_timings(mct_surf_ctx *ctx, double *ms4);
  * source (default), 1 the sequential heap loop per (period, source). */
 int mct_surf_set_method(mct_surf_ctx *ctx, int fm2d_method);
 
+/* Dispersion launch shape: 0 by size (default), 1 one lane per column
+ * (throughput), 2 one wave per column (latency).  Results are the same. */
+int mct_surf_set_dispersion_mode(mct_surf_ctx *ctx, int mode);
+
 /* Work of the last call's dispersion stage over its first nchain chains:
  * evals = period-equation evaluations of surfdisp96's sequential root walk
  * (dltar calls), layer_evals = the same weighted by (layers - 1), the

@@ -63,7 +63,8 @@ def lib():
                      "mct_copy_d2h", "mct_surfdisp96_batch", "mct_modrays", "mct_surf_create",
                      "mct_surf_destroy", "mct_surf_likelihood", "mct_surf_likelihood_batch_dev",
                      "mct_surf_sync", "mct_surf_last_timings", "mct_surf_dev_arrays",
-                     "mct_surf_set_method", "mct_surf_dispersion_work", "mct_surf_fm2d_stats"):
+                     "mct_surf_set_method", "mct_surf_dispersion_work", "mct_surf_fm2d_stats",
+                     "mct_surf_set_dispersion_mode"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -446,6 +447,10 @@ class SurfLikelihood:
         _check(lib().mct_surf_dispersion_work(self._h, int(nchain), ctypes.byref(a), ctypes.byref(b)),
                "mct_surf_dispersion_work")
         return a.value, b.value
+
+    def set_dispersion_mode(self, mode):
+        """Dispersion launch shape: 0 by size, 1 lane per column, 2 wave per column."""
+        _check(lib().mct_surf_set_dispersion_mode(self._h, int(mode)), "mct_surf_set_dispersion_mode")
 
     def fm2d_stats(self):
         """(slots swept, max rounds, mean rounds) of the last 2-D launch chunk."""

@@ -4,7 +4,9 @@
 // :523-629), the surfdisp96 branch of surfmodes (surfmodes/surfmodes.f90
 // :79-95) and surfdisp96 itself (surfmodes/surfdisp96.f:52-382).
 //
-// One wave64 per column.  The layer model lives in LDS.  The expensive part of
+// Two launch shapes.  k_dispersion_lane: one lane per column, each lane runs
+// surfdisp96 as written (64 columns per wave; throughput).  k_dispersion: one
+// wave64 per column with the layer model in LDS (latency).  The expensive part of
 // surfdisp96 is getsol's bracketing walk (:759-815): one period-equation
 // evaluation per dc step, ~10^3 steps for the first period.  Here the 64 lanes
 // evaluate 64 consecutive steps at once; every lane replays the walk's own
@@ -25,6 +27,7 @@
 
 struct dsp_model {
     int mmax, llw;
+    int st;                        // element stride of the layer arrays
     const float *d, *a, *b, *rho;
 };
 
@@ -52,8 +55,8 @@ __device__ static float d_gtsolh(float a, float b)
 // SH period equation, dltar1 (surfdisp96.f:1056-1115)
 __device__ static double d_dltar1(double wvno, double omega, const dsp_model &M)
 {
-    double beta1 = (double)M.b[M.mmax - 1];
-    double rho1 = (double)M.rho[M.mmax - 1];
+    double beta1 = (double)M.b[M.st * (M.mmax - 1)];
+    double rho1 = (double)M.rho[M.st * (M.mmax - 1)];
     double xkb = omega / beta1;
     double wvnop = wvno + xkb;
     double wvnom = fabs(wvno - xkb);
@@ -61,14 +64,14 @@ __device__ static double d_dltar1(double wvno, double omega, const dsp_model &M)
     double e1 = rho1 * rb;
     double e2 = 1.0 / (beta1 * beta1);
     for (int m = M.mmax - 1; m >= M.llw; m--) {
-        beta1 = (double)M.b[m - 1];
-        rho1 = (double)M.rho[m - 1];
+        beta1 = (double)M.b[M.st * (m - 1)];
+        rho1 = (double)M.rho[M.st * (m - 1)];
         double xmu = rho1 * beta1 * beta1;
         xkb = omega / beta1;
         wvnop = wvno + xkb;
         wvnom = fabs(wvno - xkb);
         rb = sqrt(wvnop * wvnom);
-        double q = (double)M.d[m - 1] * rb;
+        double q = (double)M.d[M.st * (m - 1)] * rb;
         double sinq, y, z, cosq;
         if (wvno < xkb) {
             sinq = sin(q);
@@ -77,7 +80,7 @@ __device__ static double d_dltar1(double wvno, double omega, const dsp_model &M)
             cosq = cos(q);
         } else if (wvno == xkb) {
             cosq = 1.0;
-            y = (double)M.d[m - 1];
+            y = (double)M.d[M.st * (m - 1)];
             z = 0.0;
         } else {
             double fac = 0.0;
@@ -106,18 +109,18 @@ __device__ static double d_dltar4(double wvno, double omga, const dsp_model &M)
     const double one = 1.0, two = 2.0;
     double wvno2 = wvno * wvno;
     int mm = M.mmax - 1;
-    double xka = omega / (double)M.a[mm];
-    double xkb = omega / (double)M.b[mm];
+    double xka = omega / (double)M.a[M.st * (mm)];
+    double xkb = omega / (double)M.b[M.st * (mm)];
     double wvnop = wvno + xka, wvnom = fabs(wvno - xka);
     double ra = sqrt(wvnop * wvnom);
     wvnop = wvno + xkb;
     wvnom = fabs(wvno - xkb);
     double rb = sqrt(wvnop * wvnom);
-    double t = (double)M.b[mm] / omega;
+    double t = (double)M.b[M.st * (mm)] / omega;
     double gammk = 2.0 * t * t;
     double gam = gammk * wvno2;
     double gamm1 = gam - 1.0;
-    double rho1 = (double)M.rho[mm];
+    double rho1 = (double)M.rho[M.st * (mm)];
     double e0 = rho1 * rho1 * (gamm1 * gamm1 - gam * gammk * ra * rb);
     double e1 = -(rho1 * ra);
     double e2 = rho1 * (gamm1 - gammk * ra * rb);
@@ -125,9 +128,9 @@ __device__ static double d_dltar4(double wvno, double omga, const dsp_model &M)
     double e4 = wvno2 - ra * rb;
     for (int m = M.mmax - 1; m >= M.llw; m--) {
         int i0 = m - 1;
-        xka = omega / (double)M.a[i0];
-        xkb = omega / (double)M.b[i0];
-        t = (double)M.b[i0] / omega;
+        xka = omega / (double)M.a[M.st * (i0)];
+        xkb = omega / (double)M.b[M.st * (i0)];
+        t = (double)M.b[M.st * (i0)] / omega;
         gammk = 2.0 * t * t;
         gam = gammk * wvno2;
         wvnop = wvno + xka;
@@ -136,8 +139,8 @@ __device__ static double d_dltar4(double wvno, double omga, const dsp_model &M)
         wvnop = wvno + xkb;
         wvnom = fabs(wvno - xkb);
         rb = sqrt(wvnop * wvnom);
-        double dpth = (double)M.d[i0];
-        rho1 = (double)M.rho[i0];
+        double dpth = (double)M.d[M.st * (i0)];
+        rho1 = (double)M.rho[M.st * (i0)];
         double p = ra * dpth, q = rb * dpth;
         // ---- var
         double pex = 0.0, sex = 0.0, sinp, w = 0, x = 0, cosp = 0, sinq, y = 0, z = 0, cosq = 0;
@@ -234,12 +237,12 @@ __device__ static double d_dltar4(double wvno, double omga, const dsp_model &M)
         e0 = ee0 / t1; e1 = ee1 / t1; e2 = ee2 / t1; e3 = ee3 / t1; e4 = ee4 / t1;
     }
     if (M.llw != 1) {   // water layer on top
-        xka = omega / (double)M.a[0];
+        xka = omega / (double)M.a[M.st * (0)];
         wvnop = wvno + xka;
         wvnom = fabs(wvno - xka);
         ra = sqrt(wvnop * wvnom);
-        double dpth = (double)M.d[0];
-        rho1 = (double)M.rho[0];
+        double dpth = (double)M.d[M.st * (0)];
+        rho1 = (double)M.rho[M.st * (0)];
         double p = ra * dpth;
         double w = 0, cosp = 0;
         if (wvno < xka) { double sp = sin(p); w = sp / ra; cosp = cos(p); }
@@ -436,6 +439,7 @@ __global__ __launch_bounds__(64) void k_dispersion(mct_dsp_args A)
     __syncthreads();
     dsp_model M;
     M.mmax = s_nl;
+    M.st = 1;
     M.d = sd; M.a = sa; M.b = sb; M.rho = sr;
     M.llw = (sb[0] <= 0.0f) ? 2 : 1;
     // surfdisp96 (flat earth, fundamental mode)
@@ -525,13 +529,189 @@ __global__ __launch_bounds__(64) void k_dispersion(mct_dsp_args A)
     }
 }
 
+// getsol (surfdisp96.f:734-827) as the reference runs it: one lane walks the
+// bracket step by step.
+__device__ static int d_getsol_seq(double t1, double &c1io, double clow, double dc, double cm, float betmx,
+                                   int ifunc, int ifirst, double &del1st, const dsp_model &M, int &ndl)
+{
+    const double twopi = 2.0 * 3.141592653589793;
+    double c1 = c1io;
+    double omega = twopi / t1;
+    double del1 = d_dltar(omega / c1, omega, ifunc, M);
+    ndl++;
+    if (ifirst == 1) del1st = del1;
+    double plmn = dsgn(del1st) * dsgn(del1);
+    int idir = (ifirst == 1) ? 1 : (plmn >= 0.0 ? 1 : -1);
+    const double cmax = (double)betmx + dc;
+    for (int it = 0; it < (1 << 24); it++) {
+        double c2;
+        for (;;) {
+            c2 = idir > 0 ? c1 + dc : c1 - dc;
+            if (c2 <= clow) { idir = 1; c1 = clow; continue; }
+            break;
+        }
+        omega = twopi / t1;
+        double del2 = d_dltar(omega / c2, omega, ifunc, M);
+        ndl++;
+        if (dsgn(del1) != dsgn(del2)) {
+            double cn = d_nevill(t1, c1, c2, del1, del2, ifunc, M, twopi, ndl);
+            c1io = cn;
+            if (cn > (double)betmx) return -1;
+            return 1;
+        }
+        c1 = c2;
+        del1 = del2;
+        if (c1 < cm || c1 >= cmax) { c1io = c1; return -1; }
+    }
+    c1io = c1;
+    return -1;
+}
+
+// One lane per column: check_model, convert_to_layer and surfdisp96 for 64
+// columns per wave.  The layer model of lane i is stored at lay + i with a
+// stride of 64 floats (planes d, a, b, rho of nlmax x 64 floats per wave).
+__global__ __launch_bounds__(64) void k_dispersion_lane(mct_dsp_args A, float *lay, int nlmax)
+{
+    const int lane = threadIdx.x;
+    const long col = (long)blockIdx.x * 64 + lane;
+    const int c = blockIdx.y;
+    if (col >= A.ncol) return;
+    const double *vs = A.vs + ((long)c * A.ncol + col) * A.nz;
+    const long plane = (long)nlmax * 64;
+    float *base = lay + ((long)c * gridDim.x + blockIdx.x) * 4 * plane + lane;
+    float *sd = base, *sa = base + plane, *sb = base + 2 * plane, *sr = base + 3 * plane;
+    const double POISSON = (double)1.730f;     // src/utils.f90:97-112 (vs2vp_3d)
+    const double EPS = (double)1.0e-10f;       // src/likelihood_surf.F90:37
+    // check_model (src/likelihood_surf.F90:631-646)
+    bool below = false;
+    const double top = vs[0];
+    for (int k = 1; k < A.nz; k++) below |= vs[k] < top;
+    if (below) A.bad_model[c] = 1;
+    // convert_to_layer (src/likelihood_surf.F90:523-629), vp and rho as vs2vp_3d / vp2rho_3d
+    int nl = 0, last_k = 1;
+    double last_vs = vs[0];
+    for (int k = 2; k <= A.nz; k++) {
+        double vk = vs[k - 1];
+        if (fabs(vk - last_vs) > EPS) {
+            double vpk = last_vs * POISSON;
+            sa[64 * nl] = (float)vpk;
+            sb[64 * nl] = (float)last_vs;
+            sr[64 * nl] = (float)((double)1.74f * pow(vpk, 0.25));
+            sd[64 * nl] = (float)(((double)(k - last_k) * A.dz) / A.scaling);
+            nl++;
+            last_vs = vk;
+            last_k = k;
+        }
+    }
+    {
+        double vpk = vs[A.nz - 1] * POISSON;
+        sa[64 * nl] = (float)vpk;
+        sb[64 * nl] = (float)vs[A.nz - 1];
+        sr[64 * nl] = (float)((double)1.74f * pow(vpk, 0.25));
+        sd[64 * nl] = 0.0f;
+    }
+    dsp_model M;
+    M.mmax = nl + 1;
+    M.st = 64;
+    M.d = sd; M.a = sa; M.b = sb; M.rho = sr;
+    M.llw = (sb[0] <= 0.0f) ? 2 : 1;
+    double *cp = A.phase + ((long)c * A.ncol + col) * A.np;
+    double *cg = A.group + ((long)c * A.ncol + col) * A.np;
+    int ierr = 0, ndl = 0;
+    const float sone = 1.500f, h = 0.005f;
+    const float ddc = (float)A.dphase;
+    const double one = 1.0e-2;
+    int jmn = 1, jsol = 1;
+    float betmx = -1.e20f, betmn = 1.e20f;
+    for (int i = 1; i <= M.mmax; i++) {
+        float bi = sb[64 * (i - 1)], ai = sa[64 * (i - 1)];
+        if (bi > 0.01f && bi < betmn) { betmn = bi; jmn = i; jsol = 1; }
+        else if (bi <= 0.01f && ai < betmn) { betmn = ai; jmn = i; jsol = 0; }
+        if (bi > betmx) betmx = bi;
+    }
+    const int ifunc = A.iwave;
+    const double onea = (double)sone;
+    float cc1 = jsol == 0 ? betmn : d_gtsolh(sa[64 * (jmn - 1)], sb[64 * (jmn - 1)]);
+    cc1 = 0.95f * cc1;
+    cc1 = 0.90f * cc1;
+    const double cc = (double)cc1;
+    const double dc = fabs((double)ddc);
+    const double cm = cc;
+    // the fundamental mode only: the previous period's root seeds the next
+    // (cvec of surfdisp96), the group pass starts from cb(k) = 0
+    double cprev = 0.0, del1st = 0.0;
+    int k;
+    for (k = 1; k <= A.np; k++) {
+        double t1 = A.periods[k - 1];
+        float t1a, t1b = 0.0f;
+        if (A.igr > 0) {
+            t1a = (float)(t1 / (double)(1.0f + h));
+            t1b = (float)(t1 / (double)(1.0f - h));
+            t1 = (double)t1a;
+        } else {
+            t1a = (float)t1;
+        }
+        int ifirst;
+        double c1, clow;
+        if (k == 1) {
+            c1 = cc; clow = cc; ifirst = 1;
+        } else {
+            ifirst = 0;
+            c1 = cprev > 0 ? cprev - onea * dc : cc;
+            clow = cm;
+        }
+        int iret = d_getsol_seq(t1, c1, clow, dc, cm, betmx, ifunc, ifirst, del1st, M, ndl);
+        if (iret == -1) break;
+        const double croot = c1;
+        cprev = croot;
+        if (A.igr > 0) {
+            t1 = (double)t1b;
+            ifirst = 0;
+            clow = 0.0 + one * dc;
+            c1 = c1 - onea * dc;
+            iret = d_getsol_seq(t1, c1, clow, dc, cm, betmx, ifunc, ifirst, del1st, M, ndl);
+            if (iret == -1) { c1 = croot; ierr = 1; }
+        } else {
+            c1 = 0.0;
+        }
+        float cc0 = (float)croot;
+        float cc1k = (float)c1;
+        if (A.igr == 0) {
+            cp[k - 1] = (double)cc0;
+            cg[k - 1] = 100.0;
+        } else {
+            float gvel = (1.0f / t1a - 1.0f / t1b) / (1.0f / (t1a * cc0) - 1.0f / (t1b * cc1k));
+            cg[k - 1] = (double)gvel;
+            cp[k - 1] = (double)cc0;
+            if (gvel < 0 || croot == 0) ierr = 1;
+        }
+    }
+    if (k <= A.np) {   // no root at period k: the rest keep cp=100, cg=0 (labels 1700/1750)
+        ierr = 1;
+        for (int i = k; i <= A.np; i++) { cp[i - 1] = 100.0; cg[i - 1] = 0.0; }
+    }
+    A.ierr[(long)c * A.ncol + col] = ierr;
+    if (A.ndltar) A.ndltar[(long)c * A.ncol + col] = ndl;
+    if (A.nlayer) A.nlayer[(long)c * A.ncol + col] = M.mmax;
+}
+
 int mct_launch_dispersion(const mct_dsp_args &a, hipStream_t s)
 {
     if (a.np < 1 || a.np > DSP_MAXP || a.nz < 1 || a.nz > DSP_NMAX) return MCT_ERR_ARG;
     MCT_CHECK(hipMemsetAsync(a.bad_model, 0, sizeof(int32_t) * a.nchain, s));
-    hipLaunchKernelGGL(k_dispersion, dim3(a.ncol, a.nchain), dim3(64), 0, s, a);
+    if (a.lay) {
+        const unsigned nb = (unsigned)((a.ncol + 63) / 64);
+        hipLaunchKernelGGL(k_dispersion_lane, dim3(nb, a.nchain), dim3(64), 0, s, a, a.lay, a.nz);
+    } else {
+        hipLaunchKernelGGL(k_dispersion, dim3(a.ncol, a.nchain), dim3(64), 0, s, a);
+    }
     MCT_CHECK(hipGetLastError());
     return MCT_OK;
+}
+
+long mct_dispersion_lay_floats(int ncol, int nz, int nchain)
+{
+    return (long)nchain * ((ncol + 63) / 64) * 4L * nz * 64;
 }
 
 // ---- C ABI: dispersion of explicit layer models (surfmodes' surfdisp96 call) ----
@@ -558,6 +738,7 @@ __global__ __launch_bounds__(64) void k_dispersion_layers(mct_layer_args A)
     __syncthreads();
     dsp_model M;
     M.mmax = nl;
+    M.st = 1;
     M.d = sd; M.a = sa; M.b = sb; M.rho = sr;
     M.llw = (sb[0] <= 0.0f) ? 2 : 1;
     double *cp = A.phase + (long)mod * A.np, *cg = A.group + (long)mod * A.np;

@@ -209,7 +209,11 @@ struct mct_dsp_args {
     int32_t *ndltar;              // (nchain, ncol) or NULL: period-equation evaluations of the
                                   // sequential walk (surfdisp96's own count)
     int32_t *nlayer;              // (nchain, ncol) or NULL: layers of the column
+    float *lay;                   // lane-per-column scratch (mct_dispersion_lay_floats) or NULL
+                                  // for one wave per column
 };
 // One wave per (column, chain): check_model, convert_to_layer, surfdisp96.
 // Requires np <= 60 and nz <= 100.
 int mct_launch_dispersion(const mct_dsp_args &a, hipStream_t s);
+// Floats of layer scratch the lane-per-column kernel needs.
+long mct_dispersion_lay_floats(int ncol, int nz, int nchain);

@@ -80,6 +80,7 @@ struct mct_surf_ctx {
     mct_fm2d_args F;                 // geometry part of the 2-D solve
     long fm2d_chunk;
     int fm2d_method;                 // 0 parallel sweeps, 1 sequential heap loop
+    int dsp_mode;                    // 0 by size, 1 lane per column, 2 wave per column
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double ms[4] = {0, 0, 0, 0};
@@ -89,7 +90,7 @@ struct mct_surf_ctx {
     sbuf points, params, noise, out, status;
     // stage buffers
     sbuf vs, flag_count, flag_nodes, kd_nodes, kd_ind, kd_rdata;
-    sbuf phase, group, ierr, bad_model, ndltar, nlayer, velv, veln, phase_time, srdist, sigma;
+    sbuf phase, group, ierr, bad_model, ndltar, nlayer, lay, velv, veln, phase_time, srdist, sigma;
     sbuf ttn, ttr, vr, nsts, nstr, heap, pflag, rounds;
     double *h_in = nullptr, *h_out = nullptr;
     int32_t *h_status = nullptr;
@@ -126,6 +127,8 @@ static int surf_pipeline(mct_surf_ctx *C, int nchain, const double *d_points, co
     da.phase = C->phase.as<double>(); da.group = C->group.as<double>(); da.ierr = C->ierr.as<int32_t>();
     da.bad_model = C->bad_model.as<int32_t>(); da.ndltar = C->ndltar.as<int32_t>();
     da.nlayer = C->nlayer.as<int32_t>();
+    // many columns: one lane per column (throughput); few: one wave per column (latency)
+    da.lay = (C->dsp_mode == 1 || (C->dsp_mode == 0 && (long)ncol * nchain >= 100000)) ? C->lay.as<float>() : nullptr;
     if ((rc = mct_launch_dispersion(da, s))) return rc;
     hipLaunchKernelGGL(k_surf_status, dim3(nchain), dim3(256), 0, s, (const int32_t *)da.bad_model,
                        (const int32_t *)da.ierr, ncol, d_status);
@@ -213,6 +216,9 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     {
         const char *e = getenv("MCT_FM2D_METHOD");
         C->fm2d_method = e && atoi(e) == 1 ? 1 : 0;
+        const char *d = getenv("MCT_DSP_MODE");
+        C->dsp_mode = d ? atoi(d) : 0;
+        if (C->dsp_mode < 0 || C->dsp_mode > 2) C->dsp_mode = 0;
     }
     int rc = MCT_OK;
     if (hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking) != hipSuccess) { delete C; return MCT_ERR_HIP; }
@@ -242,6 +248,7 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     ALLOC(bad_model, sizeof(int32_t) * M);
     ALLOC(ndltar, sizeof(int32_t) * ncol * M);
     ALLOC(nlayer, sizeof(int32_t) * ncol * M);
+    ALLOC(lay, sizeof(float) * mct_dispersion_lay_floats((int)ncol, grid->nz, M));
     ALLOC(velv, sizeof(double) * (grid->nx + 2) * (grid->ny + 2) * np * M);
     ALLOC(veln, sizeof(double) * nnx * nnz * np * M);
     ALLOC(phase_time, sizeof(double) * npair * np * M);
@@ -296,7 +303,7 @@ int mct_surf_destroy(mct_surf_ctx *C)
     sbuf *bufs[] = {&C->periods, &C->scx, &C->scz, &C->rcx, &C->rcz, &C->ttime, &C->raystat,
                     &C->points, &C->params, &C->noise, &C->out, &C->status, &C->vs, &C->flag_count,
                     &C->flag_nodes, &C->kd_nodes, &C->kd_ind, &C->kd_rdata, &C->phase, &C->group,
-                    &C->ierr, &C->bad_model, &C->ndltar, &C->nlayer, &C->velv, &C->veln, &C->phase_time, &C->srdist, &C->sigma,
+                    &C->ierr, &C->bad_model, &C->ndltar, &C->nlayer, &C->lay, &C->velv, &C->veln, &C->phase_time, &C->srdist, &C->sigma,
                     &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap, &C->pflag, &C->rounds};
     for (sbuf *b : bufs) b->release();
     if (C->h_in) (void)hipHostFree(C->h_in);
@@ -401,6 +408,13 @@ int mct_surf_set_method(mct_surf_ctx *C, int fm2d_method)
 {
     if (!C || (fm2d_method != 0 && fm2d_method != 1)) return MCT_ERR_ARG;
     C->fm2d_method = fm2d_method;
+    return MCT_OK;
+}
+
+int mct_surf_set_dispersion_mode(mct_surf_ctx *C, int mode)
+{
+    if (!C || mode < 0 || mode > 2) return MCT_ERR_ARG;
+    C->dsp_mode = mode;
     return MCT_OK;
 }
 

@@ -58,17 +58,19 @@ def check_against_oracle(w, got, tag, sigdep=0, s0=None, s1=None):
         assert rel.max() <= 1e-5, (tag, p, rel.max())
 
 
-@pytest.mark.parametrize("order,raylov,method", [(1, 1, 0), (0, 1, 0), (1, 0, 0), (1, 1, 1)])
-def test_plain_call_matches_oracle(order, raylov, method):
+@pytest.mark.parametrize("order,raylov,method,dsp", [(1, 1, 0, 0), (0, 1, 0, 0), (1, 0, 0, 0), (1, 1, 1, 0),
+                                                     (1, 1, 0, 1), (1, 0, 0, 1)])
+def test_plain_call_matches_oracle(order, raylov, method, dsp):
     # Love roots fail (ierr) for many of these models; seed 20261207 is one that works
     seed = 20261202 if raylov == 1 else 20261207
     w = W.surface_workload(n=41, nz=21, ncells=30, np_=10, nsrc=16, nrev=16, seed=seed, order=order)
     w.raylov = raylov
-    say("surf plain", "order", order, "raylov", raylov, "method", method)
+    say("surf plain", "order", order, "raylov", raylov, "method", method, "dispersion mode", dsp)
     L = ctx_for(w)
     L.set_method(method)
+    L.set_dispersion_mode(dsp)
     got = L.likelihood(w.points, w.params, arrays=True)
-    check_against_oracle(w, got, "plain o%d r%d m%d" % (order, raylov, method))
+    check_against_oracle(w, got, "plain o%d r%d m%d d%d" % (order, raylov, method, dsp))
     L.close()
 
 
