@@ -128,7 +128,8 @@ def run_surface(args):
         "2-D %.2f misfit %.2f" % (lat.mean() * 1e3, np.median(lat) * 1e3,
                                   np.quantile(lat, 0.99) * 1e3, *st_plain))
     ev1, lev1 = like.dispersion_work(1)
-    log("2-D sweep rounds (plain): slots %d max %d mean %.2f" % like.fm2d_stats())
+    log("2-D sweep (plain): slots %d, per problem: rounds %.2f, diagonal steps %.0f, node updates %.0f"
+        % like.fm2d_stats())
 
     # ---- batched: the timed region -----------------------------------------------
     for _ in range(args.warmup):

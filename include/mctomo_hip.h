@@ -249,9 +249,10 @@ int mct_surf_set_dispersion_mode(mct_surf_ctx *ctx, int mode);
  * per-evaluation layer recursion count. */
 int mct_surf_dispersion_work(mct_surf_ctx *ctx, int nchain, int64_t *evals, int64_t *layer_evals);
 
-/* Sweep rounds of the parallel 2-D path in the last launch chunk: slots
- * swept, largest and mean rounds (one round = four sweep directions). */
-int mct_surf_fm2d_stats(mct_surf_ctx *ctx, int64_t *nslots, int32_t *rounds_max, double *rounds_mean);
+/* Work of the parallel 2-D path in the last launch chunk: {slots swept, mean
+ * sweep rounds (four directions each), mean diagonal steps executed, mean
+ * node updates} per (period, source) problem. */
+int mct_surf_fm2d_stats(mct_surf_ctx *ctx, double *stats4);
 
 /* Device pointers of the context's phase_time (nchain,np,nsrc,nrev) and
  * phase velocity (nchain,nx,ny,np) buffers (valid until destroy). */

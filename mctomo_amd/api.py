@@ -453,11 +453,10 @@ class SurfLikelihood:
         _check(lib().mct_surf_set_dispersion_mode(self._h, int(mode)), "mct_surf_set_dispersion_mode")
 
     def fm2d_stats(self):
-        """(slots swept, max rounds, mean rounds) of the last 2-D launch chunk."""
-        n, mx, mean = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_double()
-        _check(lib().mct_surf_fm2d_stats(self._h, ctypes.byref(n), ctypes.byref(mx), ctypes.byref(mean)),
-               "mct_surf_fm2d_stats")
-        return n.value, mx.value, mean.value
+        """(slots, mean rounds, mean diagonal steps, mean node updates) of the last 2-D chunk."""
+        t = np.zeros(4)
+        _check(lib().mct_surf_fm2d_stats(self._h, _p(t)), "mct_surf_fm2d_stats")
+        return tuple(t)
 
     def dev_arrays(self):
         """Device addresses of the phase_time and phase-velocity buffers."""

@@ -25,9 +25,11 @@
 #define DSP_NMAX 100           // NMAX layers per column (src/likelihood_surf.F90:30)
 #define DSP_MAXP 60            // NP periods (surfmodes/surfdisp96.f:59)
 
-struct dsp_model {
+// A column's layer model; ST is the element stride of the layer arrays (1 in
+// LDS, 64 when lanes interleave their models).
+template <int ST> struct dsp_model_t {
+    static constexpr int st = ST;
     int mmax, llw;
-    int st;                        // element stride of the layer arrays
     const float *d, *a, *b, *rho;
 };
 
@@ -53,7 +55,8 @@ __device__ static float d_gtsolh(float a, float b)
 }
 
 // SH period equation, dltar1 (surfdisp96.f:1056-1115)
-__device__ static double d_dltar1(double wvno, double omega, const dsp_model &M)
+template <class MD>
+__device__ static double d_dltar1(double wvno, double omega, const MD &M)
 {
     double beta1 = (double)M.b[M.st * (M.mmax - 1)];
     double rho1 = (double)M.rho[M.st * (M.mmax - 1)];
@@ -102,7 +105,8 @@ __device__ static double d_dltar1(double wvno, double omega, const dsp_model &M)
 }
 
 // P-SV period equation, dltar4 with var and dnka inlined (surfdisp96.f:1119-1414)
-__device__ static double d_dltar4(double wvno, double omga, const dsp_model &M)
+template <class MD>
+__device__ static double d_dltar4(double wvno, double omga, const MD &M)
 {
     double omega = omga;
     if (omega < 1.0e-4) omega = 1.0e-4;
@@ -259,14 +263,16 @@ __device__ static double d_dltar4(double wvno, double omga, const dsp_model &M)
     return e0;
 }
 
-__device__ __forceinline__ double d_dltar(double wvno, double omega, int ifunc, const dsp_model &M)
+template <class MD>
+__device__ __forceinline__ double d_dltar(double wvno, double omega, int ifunc, const MD &M)
 {
     return ifunc == 1 ? d_dltar1(wvno, omega, M) : d_dltar4(wvno, omega, M);
 }
 
 // nevill (surfdisp96.f:903-1020), run identically by every lane of the wave
+template <class MD>
 __device__ static double d_nevill(double t, double c1, double c2, double del1, double del2, int ifunc,
-                                  const dsp_model &M, double twopi, int &ndl)
+                                  const MD &M, double twopi, int &ndl)
 {
     double x[12], y[12];
     double omega = twopi / t;
@@ -330,8 +336,9 @@ __device__ static double d_nevill(double t, double c1, double c2, double del1, d
 }
 
 // getsol (surfdisp96.f:734-827) with the bracketing walk spread over the wave.
+template <class MD>
 __device__ static int d_getsol(double t1, double &c1io, double clow, double dc, double cm, float betmx,
-                               int ifunc, int ifirst, double &del1st, const dsp_model &M, int &ndl)
+                               int ifunc, int ifirst, double &del1st, const MD &M, int &ndl)
 {
     const int lane = threadIdx.x & 63;
     const double twopi = 2.0 * 3.141592653589793;
@@ -437,9 +444,8 @@ __global__ __launch_bounds__(64) void k_dispersion(mct_dsp_args A)
         s_nl = nl + 1;
     }
     __syncthreads();
-    dsp_model M;
+    dsp_model_t<1> M;
     M.mmax = s_nl;
-    M.st = 1;
     M.d = sd; M.a = sa; M.b = sb; M.rho = sr;
     M.llw = (sb[0] <= 0.0f) ? 2 : 1;
     // surfdisp96 (flat earth, fundamental mode)
@@ -531,8 +537,9 @@ __global__ __launch_bounds__(64) void k_dispersion(mct_dsp_args A)
 
 // getsol (surfdisp96.f:734-827) as the reference runs it: one lane walks the
 // bracket step by step.
+template <class MD>
 __device__ static int d_getsol_seq(double t1, double &c1io, double clow, double dc, double cm, float betmx,
-                                   int ifunc, int ifirst, double &del1st, const dsp_model &M, int &ndl)
+                                   int ifunc, int ifirst, double &del1st, const MD &M, int &ndl)
 {
     const double twopi = 2.0 * 3.141592653589793;
     double c1 = c1io;
@@ -610,9 +617,8 @@ __global__ __launch_bounds__(64) void k_dispersion_lane(mct_dsp_args A, float *l
         sr[64 * nl] = (float)((double)1.74f * pow(vpk, 0.25));
         sd[64 * nl] = 0.0f;
     }
-    dsp_model M;
+    dsp_model_t<64> M;
     M.mmax = nl + 1;
-    M.st = 64;
     M.d = sd; M.a = sa; M.b = sb; M.rho = sr;
     M.llw = (sb[0] <= 0.0f) ? 2 : 1;
     double *cp = A.phase + ((long)c * A.ncol + col) * A.np;
@@ -736,9 +742,8 @@ __global__ __launch_bounds__(64) void k_dispersion_layers(mct_layer_args A)
         sd[i] = L[i]; sa[i] = L[A.nl_max + i]; sb[i] = L[2 * A.nl_max + i]; sr[i] = L[3 * A.nl_max + i];
     }
     __syncthreads();
-    dsp_model M;
+    dsp_model_t<1> M;
     M.mmax = nl;
-    M.st = 1;
     M.d = sd; M.a = sa; M.b = sb; M.rho = sr;
     M.llw = (sb[0] <= 0.0f) ? 2 : 1;
     double *cp = A.phase + (long)mod * A.np, *cg = A.group + (long)mod * A.np;

@@ -34,6 +34,7 @@
 // solved by the sequential kernel instead.
 #include "mct_internal.h"
 #include <string.h>
+#include <stdlib.h>
 
 #pragma clang fp contract(off)
 
@@ -269,7 +270,25 @@ struct ff_sten {
 
 __device__ __forceinline__ bool ff_al(double t, bool in, double thr) { return in && t < thr && t < FF_FIN; }
 
-__device__ double ff_fouds1s(const ff_sten &S, double thr, double dnx, double dnz, double slown)
+// Alive mask of a stencil for threshold thr: bit s x-1 side / x+1 side one
+// away, 2+s two away, 4+s z one away, 6+s z two away.
+__device__ __forceinline__ unsigned ff_mask(const ff_sten &S, double thr)
+{
+    unsigned m = 0;
+    for (int s = 0; s < 2; s++) {
+        m |= (unsigned)ff_al(S.tx1[s], S.ix1[s], thr) << s;
+        m |= (unsigned)ff_al(S.tx2[s], S.ix2[s], thr) << (2 + s);
+        m |= (unsigned)ff_al(S.tz1[s], S.iz1[s], thr) << (4 + s);
+        m |= (unsigned)ff_al(S.tz2[s], S.iz2[s], thr) << (6 + s);
+    }
+    return m;
+}
+#define FF_AX1(s) ((m >> (s)) & 1u)
+#define FF_AX2(s) ((m >> (2 + (s))) & 1u)
+#define FF_AZ1(s) ((m >> (4 + (s))) & 1u)
+#define FF_AZ2(s) ((m >> (6 + (s))) & 1u)
+
+__device__ double ff_fouds1s(const ff_sten &S, unsigned m, double dnx, double dnz, double slown)
 {
     int tsw1 = 0;
     double travm = 0.0;
@@ -279,9 +298,9 @@ __device__ double ff_fouds1s(const ff_sten &S, double thr, double dnx, double dn
             int swsol = 0;
             double a = 0, b = 0, c = 0, tref = 0;
             const double tj = S.tx1[sj], tk = S.tz1[sk];
-            if (ff_al(tj, true, thr)) {
+            if (FF_AX1(sj)) {
                 swsol = 1;
-                if (ff_al(tk, true, thr)) {
+                if (FF_AZ1(sk)) {
                     double u = dnx, v = dnz;
                     double em = tk - tj;
                     a = u * u + v * v;
@@ -293,7 +312,7 @@ __device__ double ff_fouds1s(const ff_sten &S, double thr, double dnx, double dn
                     c = -((slown * slown) * (dnx * dnx));
                     tref = tj;
                 }
-            } else if (ff_al(tk, true, thr)) {
+            } else if (FF_AZ1(sk)) {
                 swsol = 1;
                 a = 1.0; b = 0.0;
                 c = -((slown * dnz) * (slown * dnz));
@@ -311,7 +330,7 @@ __device__ double ff_fouds1s(const ff_sten &S, double thr, double dnx, double dn
     return travm;
 }
 
-__device__ double ff_fouds2s(const ff_sten &S, double thr, double dnx, double dnz, double slown)
+__device__ double ff_fouds2s(const ff_sten &S, unsigned m, double dnx, double dnz, double slown)
 {
     int tsw1 = 0;
     double travm = 0.0;
@@ -319,8 +338,8 @@ __device__ double ff_fouds2s(const ff_sten &S, double thr, double dnx, double dn
         if (!S.ix1[sj]) continue;
         const double tj = S.tx1[sj], tj2 = S.tx2[sj];
         int swj = -1;
-        if (ff_al(tj2, S.ix2[sj], thr)) swj = 0;
-        if (ff_al(tj, true, thr) && swj == 0) {
+        if (FF_AX2(sj)) swj = 0;
+        if (FF_AX1(sj) && swj == 0) {
             swj = -1;
             if (tj > tj2) swj = 0;
         } else {
@@ -330,8 +349,8 @@ __device__ double ff_fouds2s(const ff_sten &S, double thr, double dnx, double dn
             if (!S.iz1[sk]) continue;
             const double tk = S.tz1[sk], tk2 = S.tz2[sk];
             int swk = -1;
-            if (ff_al(tk2, S.iz2[sk], thr)) swk = 0;
-            if (ff_al(tk, true, thr) && swk == 0) {
+            if (FF_AZ2(sk)) swk = 0;
+            if (FF_AZ1(sk) && swk == 0) {
                 swk = -1;
                 if (tk > tk2) swk = 0;
             } else {
@@ -351,7 +370,7 @@ __device__ double ff_fouds2s(const ff_sten &S, double thr, double dnx, double dn
                     c = (u * u) * (em * em - (slown * slown) * (v * v));
                     tref = 4.0 * tj - tj2;
                     tdiv = 3.0;
-                } else if (ff_al(tk, true, thr)) {
+                } else if (FF_AZ1(sk)) {
                     u = dnz;
                     v = 2.0 * dnx;
                     em = 3.0 * tk - 4.0 * tj + tj2;
@@ -367,7 +386,7 @@ __device__ double ff_fouds2s(const ff_sten &S, double thr, double dnx, double dn
                     tref = 4.0 * tj - tj2;
                     tdiv = 3.0;
                 }
-            } else if (ff_al(tj, true, thr)) {
+            } else if (FF_AX1(sj)) {
                 swsol = 1;
                 if (swk == 0) {
                     u = dnx;
@@ -378,7 +397,7 @@ __device__ double ff_fouds2s(const ff_sten &S, double thr, double dnx, double dn
                     c = (u * u) * (em * em - (v * v) * (slown * slown));
                     tref = tj;
                     tdiv = 1.0;
-                } else if (ff_al(tk, true, thr)) {
+                } else if (FF_AZ1(sk)) {
                     u = dnx;
                     v = dnz;
                     em = tk - tj;
@@ -401,7 +420,7 @@ __device__ double ff_fouds2s(const ff_sten &S, double thr, double dnx, double dn
                     c = -((u * u) * (slown * slown));
                     tref = 4.0 * tk - tk2;
                     tdiv = 3.0;
-                } else if (ff_al(tk, true, thr)) {
+                } else if (FF_AZ1(sk)) {
                     swsol = 1;
                     a = 1.0; b = 0.0;
                     c = -((slown * slown) * (dnz * dnz));
@@ -422,22 +441,28 @@ __device__ double ff_fouds2s(const ff_sten &S, double thr, double dnx, double dn
     return travm;
 }
 
-__device__ __forceinline__ double ff_cand1s(const ff_sten &S, double thr, double dnx, double dnz,
+__device__ __forceinline__ double ff_cand1s(const ff_sten &S, unsigned m, double dnx, double dnz,
                                             double slown, int fom)
 {
-    bool any = ff_al(S.tz1[0], S.iz1[0], thr) || ff_al(S.tz1[1], S.iz1[1], thr) ||
-               ff_al(S.tx1[0], S.ix1[0], thr) || ff_al(S.tx1[1], S.ix1[1], thr);
-    if (!any) return FF_BIG;
-    return fom == 0 ? ff_fouds1s(S, thr, dnx, dnz, slown) : ff_fouds2s(S, thr, dnx, dnz, slown);
+    if (!(m & 0x33u)) return FF_BIG;   // no 4-neighbour alive
+    return fom == 0 ? ff_fouds1s(S, m, dnx, dnz, slown) : ff_fouds2s(S, m, dnx, dnz, slown);
 }
 
+// The self-consistent update: the neighbours counted are those below the
+// result.  An update over the same alive set is the same number, so the loop
+// stops as soon as the set repeats.
 __device__ double ff_cands(const ff_sten &S, double dnx, double dnz, double slown, int fom)
 {
-    double c = ff_cand1s(S, FF_THR0, dnx, dnz, slown, fom);
+    unsigned mp = ff_mask(S, FF_THR0);
+    double c = ff_cand1s(S, mp, dnx, dnz, slown, fom);
     for (int k = 0; k < 8; k++) {
-        double c2 = ff_cand1s(S, c, dnx, dnz, slown, fom);
+        if (c >= FF_THR0) break;
+        unsigned m = ff_mask(S, c);
+        if (m == mp) break;                 // the update would return c again
+        double c2 = ff_cand1s(S, m, dnx, dnz, slown, fom);
         if (c2 == c || c2 >= FF_THR0) break;
         c = c2;
+        mp = m;
     }
     return c;
 }
@@ -460,65 +485,99 @@ __device__ __forceinline__ void ff_load_sten(const fview &V, int iz, int ix, ff_
 
 // Gauss-Seidel rounds over anti-diagonals in four directions until a round
 // changes no time by more than 1e-14 relative.  fixed(z,x) nodes are kept.
-// dirty (LDS bitmap over the grid, z fastest, or null) marks the nodes whose
-// stencil changed since their last update; clean nodes are skipped, and a
-// node that changes marks the nodes whose stencil holds it (two along each
-// axis).  Returns the rounds used, or -1 without convergence.
+// With the LDS work lists (dirty bitmap over the grid, z fastest, and dirty
+// counts per anti-diagonal of both families) only nodes whose stencil changed
+// since their last update are updated, and diagonals without such nodes are
+// skipped whole.  A node that changes marks the nodes whose stencil holds it
+// (up to two along each axis); at the start the open neighbours of fixed nodes
+// are marked.  This updates exactly the nodes a full sweep would change, in
+// the same order.  Returns the rounds used, or -1 without convergence.
+struct ff_work {
+    uint32_t *bits;   // nnx*nnz bits
+    int *cs, *cd;     // (nnx+nnz-1) counts: sum diagonals (x-1)+(z-1), difference diagonals (nnx-x)+(z-1)
+};
+
 template <class FX, class VL>
 __device__ int ff_sweep(const fview &V, const FX &fixed, const VL &vel, int fom, int *sh_flag,
-                        uint32_t *dirty)
+                        const ff_work *W, int *stat = nullptr)
 {
+    int n_steps = 0, n_evals = 0;
     const int nt = blockDim.x, tid = threadIdx.x;
     const int reach = fom == 0 ? 1 : 2;
     const long nn = (long)V.nnx * V.nnz;
+    const int ND = V.nnx + V.nnz - 1;
     auto bit = [&](int iz, int ix) -> long { return (long)(ix - 1) * V.nnz + (iz - 1); };
-    if (dirty) {
-        for (long w = tid; w < (nn + 31) / 32; w += nt) dirty[w] = 0xffffffffu;
+    auto mark = [&](int iz, int ix) {
+        long q = bit(iz, ix);
+        uint32_t m = 1u << (q & 31);
+        uint32_t old = atomicOr(&W->bits[q >> 5], m);
+        if (!(old & m)) {
+            atomicAdd(&W->cs[(ix - 1) + (iz - 1)], 1);
+            atomicAdd(&W->cd[(V.nnx - ix) + (iz - 1)], 1);
+        }
+    };
+    if (W) {
+        for (long w = tid; w < (nn + 31) / 32; w += nt) W->bits[w] = 0u;
+        for (int d = tid; d < ND; d += nt) { W->cs[d] = 0; W->cd[d] = 0; }
+        __syncthreads();
+        for (long q = tid; q < nn; q += nt) {
+            int x = (int)(q / V.nnz) + 1, z = (int)(q % V.nnz) + 1;
+            if (fixed(z, x)) continue;
+            bool nb = (x > 1 && fixed(z, x - 1)) || (x < V.nnx && fixed(z, x + 1)) ||
+                      (z > 1 && fixed(z - 1, x)) || (z < V.nnz && fixed(z + 1, x));
+            if (nb) mark(z, x);
+        }
         __syncthreads();
     }
     for (int round = 0; round < FF_MAX_ROUNDS; round++) {
         int changed = 0;
         for (int dir = 0; dir < 4; dir++) {
-            for (int d = 0; d < V.nnx + V.nnz - 1; d++) {
+            for (int d = 0; d < ND; d++) {
+                if (W) {
+                    int cnt = dir == 0 ? W->cs[d] : dir == 3 ? W->cs[ND - 1 - d]
+                            : dir == 1 ? W->cd[d] : W->cd[ND - 1 - d];
+                    if (cnt == 0) continue;     // nothing to do on this diagonal
+                    // every wave reads the count before any wave of this step changes
+                    // counts (a step marks nodes on earlier diagonals too)
+                    if (nt > 64) __syncthreads();
+                }
+                n_steps++;
                 int a_lo = d - (V.nnz - 1) > 0 ? d - (V.nnz - 1) : 0;
                 int a_hi = d < V.nnx - 1 ? d : V.nnx - 1;
-                int wrote = 0;
                 for (int a = a_lo + tid; a <= a_hi; a += nt) {
                     int b = d - a;
                     int x = (dir & 1) ? V.nnx - a : 1 + a;
                     int z = (dir & 2) ? V.nnz - b : 1 + b;
-                    if (dirty) {
+                    if (W) {
                         long q = bit(z, x);
                         uint32_t m = 1u << (q & 31);
-                        if (!(dirty[q >> 5] & m)) continue;
-                        atomicAnd(&dirty[q >> 5], ~m);
+                        if (!(W->bits[q >> 5] & m)) continue;
+                        uint32_t old = atomicAnd(&W->bits[q >> 5], ~m);
+                        if (!(old & m)) continue;
+                        atomicSub(&W->cs[(x - 1) + (z - 1)], 1);
+                        atomicSub(&W->cd[(V.nnx - x) + (z - 1)], 1);
                     }
                     if (fixed(z, x)) continue;
+                    n_evals++;
                     ff_sten S;
                     ff_load_sten(V, z, x, S);
                     const double t = V.t(z, x);
                     double slown = 1.0 / vel(z, x);
                     double c = ff_cands(S, V.dnx, V.dnz, slown, fom);
                     if (c < FF_THR0 && c != t) {
-                        bool big = fabs(c - t) > 1e-14 * c;
-                        if (big) changed = 1;
+                        if (fabs(c - t) > 1e-14 * c) changed = 1;
                         V.t(z, x) = c;
-                        wrote = 1;
-                        if (dirty && big) {
+                        if (W) {
                             for (int r = 1; r <= reach; r++) {
-                                if (x - r >= 1) { long q = bit(z, x - r); atomicOr(&dirty[q >> 5], 1u << (q & 31)); }
-                                if (x + r <= V.nnx) { long q = bit(z, x + r); atomicOr(&dirty[q >> 5], 1u << (q & 31)); }
-                                if (z - r >= 1) { long q = bit(z - r, x); atomicOr(&dirty[q >> 5], 1u << (q & 31)); }
-                                if (z + r <= V.nnz) { long q = bit(z + r, x); atomicOr(&dirty[q >> 5], 1u << (q & 31)); }
+                                if (x - r >= 1 && !fixed(z, x - r)) mark(z, x - r);
+                                if (x + r <= V.nnx && !fixed(z, x + r)) mark(z, x + r);
+                                if (z - r >= 1 && !fixed(z - r, x)) mark(z - r, x);
+                                if (z + r <= V.nnz && !fixed(z + r, x)) mark(z + r, x);
                             }
                         }
                     }
                 }
-                if (dirty) {
-                    if (__syncthreads_or(wrote)) continue;
-                } else {
-                    __syncthreads();
-                }
+                __syncthreads();
             }
         }
         if (tid == 0) *sh_flag = 0;
@@ -527,7 +586,10 @@ __device__ int ff_sweep(const fview &V, const FX &fixed, const VL &vel, int fom,
         __syncthreads();
         int any = *sh_flag;
         __syncthreads();
-        if (!any) return round + 1;
+        if (!any) {
+            if (stat) { atomicAdd(&stat[1], tid == 0 ? n_steps : 0); atomicAdd(&stat[2], n_evals); }
+            return round + 1;
+        }
     }
     return -1;
 }
@@ -789,6 +851,9 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
     double *WT = (double *)(HB + ((heap_cap + 2 + 1) & ~1));
     int16_t *WN = (int16_t *)(WT + win_nodes);
     uint32_t *DB = (uint32_t *)(WN + ((win_nodes + 1) & ~1));
+    int *DC = (int *)(DB + (region_nodes + 31) / 32);   // 2 x diag_cap counts
+    ff_work RW;
+    RW.bits = DB;
     __shared__ int sh_flag, sh_exit, sh_ovf;
     __shared__ double sh_texit;
     __shared__ int sh_eidx;
@@ -929,7 +994,8 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
             }
             __syncthreads();
             auto rfixed = [&](int iz, int ix) -> bool { return R.n(iz, ix) == 0; };
-            int rounds = ff_sweep(R, rfixed, rvel, A.fom, &sh_flag, DB);
+            RW.cs = DC; RW.cd = DC + (R.nnx + R.nnz - 1);
+            int rounds = ff_sweep(R, rfixed, rvel, A.fom, &sh_flag, &RW);
             if (rounds < 0) {
                 if (lane == 0) pflag[slot] = 1;
                 return;
@@ -1082,12 +1148,19 @@ __global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, const int32
     const int32_t *F = A.nsts + slot * wsz;
     auto fixed = [&](int iz, int ix) -> bool { return F[(long)(ix - 1) * A.ld + (iz - 1)] != 0; };
     auto vel = [&](int iz, int ix) -> double { return velnc[(long)(ix - 1) * A.nnz + (iz - 1)]; };
-    int rounds = ff_sweep(V, fixed, vel, A.fom, &sh_flag, use_dirty ? ff_dirty : nullptr);
+    ff_work W;
+    W.bits = ff_dirty;
+    W.cs = (int *)(ff_dirty + ((long)A.nnx * A.nnz + 31) / 32);
+    W.cd = W.cs + (A.nnx + A.nnz - 1);
+    int *stat = A.rounds_out ? A.rounds_out + 4 * slot : nullptr;
+    if (stat && threadIdx.x == 0) { stat[1] = 0; stat[2] = 0; }
+    __syncthreads();
+    int rounds = ff_sweep(V, fixed, vel, A.fom, &sh_flag, use_dirty ? &W : nullptr, stat);
     if (rounds < 0) {
         if (threadIdx.x == 0) atomicMin(&A.status[g.c], (int)MCT_ERR_NOCONV);
         return;
     }
-    if (threadIdx.x == 0 && A.rounds_out) A.rounds_out[slot] = rounds;
+    if (threadIdx.x == 0 && stat) stat[0] = rounds;
     // srtimes
     const double x = A.scx[g.s], z = A.scz[g.s];
     const int32_t *srs = A.srs + (long)g.p * A.srs_pstride + (long)g.s * A.nrc;
@@ -1153,7 +1226,8 @@ static size_t ff_near_lds(const mct_fm2d_args &a, int &region, int &heap_cap, in
     heap_cap = region;
     win = a.asgr == 1 ? wn * wn : 1;
     size_t b = 8 * (size_t)region + 2 * (size_t)((region + 3) & ~3) + 4 * (size_t)((heap_cap + 2 + 1) & ~1) +
-               8 * (size_t)win + 2 * (size_t)((win + 1) & ~1) + 4 * (size_t)((region + 31) / 32) + 64;
+               8 * (size_t)win + 2 * (size_t)((win + 1) & ~1) + 4 * (size_t)((region + 31) / 32) +
+               4 * 2 * (size_t)(rgx + rgz) + 64;
     return b;
 }
 
@@ -1172,9 +1246,17 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
     int rc = mct_launch_fm2d_gridder(a, s);
     if (rc) return rc;
     // the sweep's dirty bitmap in LDS when it fits
-    const size_t dirty_bytes = 4 * (((size_t)a.nnx * a.nnz + 31) / 32);
+    const size_t dirty_bytes = 4 * (((size_t)a.nnx * a.nnz + 31) / 32) + 4 * 2 * (size_t)(a.nnx + a.nnz);
     const int use_dirty = dirty_bytes <= 48 * 1024;
     const size_t dirty_lds = use_dirty ? dirty_bytes : 0;
+    // threads per 2-D sweep (one problem per workgroup; one wave measured fastest):
+    // environment MCT_SWEEP_THREADS
+    static int sweep_threads = 0;
+    if (!sweep_threads) {
+        const char *e = getenv("MCT_SWEEP_THREADS");
+        int v = e ? atoi(e) : 64;
+        sweep_threads = (v == 64 || v == 128 || v == 256) ? v : 64;
+    }
     const long nprob = (long)a.nchain * a.np * a.nsrc;
     if (chunk < 1) return MCT_ERR_ARG;
     for (long p0 = 0; p0 < nprob; p0 += chunk) {
@@ -1183,7 +1265,7 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
         b.nrun = nprob - p0 < chunk ? nprob - p0 : chunk;
         hipLaunchKernelGGL(k_fm2d_near, dim3((unsigned)b.nrun), dim3(64), lds, s, b, region, heap_cap, win, pflag);
         MCT_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_fm2d_sweep, dim3((unsigned)b.nrun), dim3(256), dirty_lds, s, b,
+        hipLaunchKernelGGL(k_fm2d_sweep, dim3((unsigned)b.nrun), dim3(sweep_threads), dirty_lds, s, b,
                            (const int32_t *)pflag, use_dirty);
         MCT_CHECK(hipGetLastError());
         // problems whose heap loop left the LDS neighbourhood: the sequential solver

@@ -261,8 +261,8 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     ALLOC(nstr, sizeof(int32_t) * wsz * ch);
     ALLOC(heap, sizeof(int32_t) * (wsz + 8) * ch);
     ALLOC(pflag, sizeof(int32_t) * ch);
-    ALLOC(rounds, sizeof(int32_t) * ch);
-    if (!rc && hipMemset(C->rounds.p, 0, sizeof(int32_t) * ch) != hipSuccess) rc = MCT_ERR_HIP;
+    ALLOC(rounds, sizeof(int32_t) * 4 * ch);
+    if (!rc && hipMemset(C->rounds.p, 0, sizeof(int32_t) * 4 * ch) != hipSuccess) rc = MCT_ERR_HIP;
 #undef ALLOC
     if (!rc && hipHostMalloc((void **)&C->h_in, sizeof(double) * (6 * (size_t)NC + 2 * np + 8), hipHostMallocDefault) != hipSuccess)
         rc = MCT_ERR_NOMEM;
@@ -382,25 +382,31 @@ int mct_surf_likelihood_batch_dev(mct_surf_ctx *C, int nchain, const double *poi
                          false);
 }
 
-int mct_surf_fm2d_stats(mct_surf_ctx *C, int64_t *nslots, int32_t *rounds_max, double *rounds_mean)
+int mct_surf_fm2d_stats(mct_surf_ctx *C, double *stats4)
 {
-    if (!C) return MCT_ERR_ARG;
+    if (!C || !stats4) return MCT_ERR_ARG;
     const long n = C->fm2d_chunk;
-    int32_t *r = (int32_t *)malloc(sizeof(int32_t) * n);
+    int32_t *r = (int32_t *)malloc(sizeof(int32_t) * 4 * n);
     if (!r) return MCT_ERR_NOMEM;
     if (hipStreamSynchronize(C->stream) != hipSuccess ||
-        hipMemcpy(r, C->rounds.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+        hipMemcpy(r, C->rounds.p, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost) != hipSuccess) {
         free(r);
         return MCT_ERR_HIP;
     }
-    long cnt = 0, sum = 0;
+    long cnt = 0;
+    double sr = 0, ss = 0, se = 0;
     int mx = 0;
     for (long i = 0; i < n; i++)
-        if (r[i] > 0 && r[i] < 100000) { cnt++; sum += r[i]; if (r[i] > mx) mx = r[i]; }
+        if (r[4 * i] > 0) {
+            cnt++; sr += r[4 * i]; ss += r[4 * i + 1]; se += r[4 * i + 2];
+            if (r[4 * i] > mx) mx = r[4 * i];
+        }
     free(r);
-    if (nslots) *nslots = cnt;
-    if (rounds_max) *rounds_max = mx;
-    if (rounds_mean) *rounds_mean = cnt ? (double)sum / cnt : 0.0;
+    stats4[0] = (double)cnt;
+    stats4[1] = cnt ? sr / cnt : 0.0;
+    stats4[2] = cnt ? ss / cnt : 0.0;
+    stats4[3] = cnt ? se / cnt : 0.0;
+    (void)mx;
     return MCT_OK;
 }
 
