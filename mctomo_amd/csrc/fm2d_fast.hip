@@ -601,10 +601,13 @@ __device__ int ff_sweep(const fview &V, const FX &fixed, const VL &vel, int fom,
 struct ff_heap {
     int32_t *btg;    // 1-based
     int ntr;
+    int cap;         // entries the LDS heap holds
+    int full;        // an insertion did not fit
 };
 
 __device__ void ff_addtree(const fview &V, ff_heap &H, int iz, int ix)
 {
+    if (H.ntr + 1 > H.cap) { H.full = 1; return; }
     H.ntr = H.ntr + 1;
     V.n(iz, ix) = (int16_t)H.ntr;
     H.btg[H.ntr] = (ix << 16) | iz;
@@ -698,13 +701,26 @@ __device__ __forceinline__ double ff_bilinear(const double nv[2][2], double dsx,
 
 // Heap-loop stop rules.
 struct ff_stop {
-    int mode;                 // 0 none, 1 refined inner-edge exit, 2 block frozen
+    int mode;                 // 0 none, 1 refined inner-edge exit, 2 block frozen, 3 two generations
     int vnl, vnr, vnt, vnb;   // refined-grid bookkeeping (mode 1)
     int x0, x1, z0, z1;       // block (mode 2)
     int need, have;
     int exited;               // mode 1: exit node reached
     int overflow;             // a node outside the LDS storage would be touched
+    uint32_t *gen_bits;       // mode 3: nodes of the two generations (bit per stored node)
+    int gen;                  // mode 3: 0 first generation, 1 second
 };
+
+__device__ __forceinline__ void ff_gen_set(const fview &V, uint32_t *b, int iz, int ix)
+{
+    long q = V.at(iz, ix);
+    b[q >> 5] |= 1u << (q & 31);
+}
+__device__ __forceinline__ bool ff_gen_get(const fview &V, const uint32_t *b, int iz, int ix)
+{
+    long q = V.at(iz, ix);
+    return (b[q >> 5] >> (q & 31)) & 1u;
+}
 
 // The reference's heap loop over the view (travel's :137-237) with fouds and
 // velocities vel(z,x); stops by rule S.  Runs in one lane.
@@ -713,6 +729,7 @@ __device__ void ff_heap_loop(const fview &V, ff_heap &H, const VL &vel, int fom,
 {
     auto al = [&](int z, int x) -> bool { return V.n(z, x) == 0; };
     while (H.ntr > 0) {
+        if (H.full) { S.overflow = 1; return; }
         int e = H.btg[1];
         int ix = HPX(e), iz = HPZ(e);
         if (S.mode == 1) {
@@ -724,6 +741,16 @@ __device__ void ff_heap_loop(const fview &V, ff_heap &H, const VL &vel, int fom,
             if (swrg == 1) { V.n(iz, ix) = 0; S.exited = 1; return; }
         }
         if (S.mode == 2 && S.have >= S.need) return;
+        if (S.mode == 3 && S.have >= S.need) {
+            // every node of the generation is frozen: the second generation is
+            // the band at this moment; after it the heap loop is a plain one
+            if (S.gen == 1 || H.ntr == 0) return;
+            for (int q = 1; q <= H.ntr; q++) ff_gen_set(V, S.gen_bits, HPZ(H.btg[q]), HPX(H.btg[q]));
+            S.need += H.ntr;
+            S.gen = 1;
+            e = H.btg[1];
+            ix = HPX(e); iz = HPZ(e);
+        }
         // the updates around a popped node read up to three nodes away
         {
             int xa = ix - 3 < 1 ? 1 : ix - 3, xb = ix + 3 > V.nnx ? V.nnx : ix + 3;
@@ -732,6 +759,7 @@ __device__ void ff_heap_loop(const fview &V, ff_heap &H, const VL &vel, int fom,
         }
         V.n(iz, ix) = 0;
         if (S.mode == 2 && ix >= S.x0 && ix <= S.x1 && iz >= S.z0 && iz <= S.z1) S.have++;
+        if (S.mode == 3 && ff_gen_get(V, S.gen_bits, iz, ix)) S.have++;
         ff_downtree(V, H);
         for (int i = ix - 1; i <= ix + 1; i += 2) {
             if (i >= 1 && i <= V.nnx) {
@@ -891,6 +919,8 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
     ff_heap H;
     H.btg = HB;
     H.ntr = 0;
+    H.cap = heap_cap;
+    H.full = 0;
 
     if (A.asgr == 1) {
         // ---- refined source grid: bsplrefine, seed, sweep, rebuilt stop ----
@@ -952,6 +982,7 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
                 (void)al;
                 // heap loop with both rules (exit first, as the reference checks it first)
                 while (H.ntr > 0) {
+                    if (H.full) break;
                     int e = H.btg[1];
                     int ix = HPX(e), iz = HPZ(e);
                     int swrg = 0;
@@ -978,6 +1009,7 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
                         }
                 }
                 sh_exit = S.exited ? 1 : 0;
+                if (H.full) sh_ovf = 2;
             } else {
                 sh_ovf = 3;   // source outside the refined grid: use the sequential solver
             }
@@ -1074,21 +1106,24 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
             if (nb) C.n(l, k) = 1;
         }
         __syncthreads();
-        // the reference's coarse heap loop until the window and one ring are frozen
+        // the reference's coarse heap loop over two generations: the remapped
+        // band (with the re-banded frozen nodes), then the band at the moment
+        // the last of those froze; after that every update is a plain one
+        for (int w = lane; w < (C.lnx * C.lnz + 31) / 32; w += 64) DB[w] = 0u;
+        __syncthreads();
         if (lane == 0) {
+            ff_stop S;
+            memset(&S, 0, sizeof S);
+            S.mode = 3;
+            S.gen_bits = DB;
             H.ntr = 0;
             for (int i = lx0; i <= lx1; i++)
                 for (int j = lz0; j <= lz1; j++)
-                    if (C.n(j, i) > 0) ff_addtree(C, H, j, i);
-            ff_stop S;
-            memset(&S, 0, sizeof S);
-            S.mode = 2;
-            S.x0 = g.vnl - 1 < 1 ? 1 : g.vnl - 1; S.x1 = g.vnr + 1 > A.nnx ? A.nnx : g.vnr + 1;
-            S.z0 = g.vnt - 1 < 1 ? 1 : g.vnt - 1; S.z1 = g.vnb + 1 > A.nnz ? A.nnz : g.vnb + 1;
-            S.need = (S.x1 - S.x0 + 1) * (S.z1 - S.z0 + 1);
-            S.have = 0;
-            for (int k = S.x0; k <= S.x1; k++)
-                for (int l = S.z0; l <= S.z1; l++) S.have += C.n(l, k) == 0;
+                    if (C.n(j, i) > 0) {
+                        ff_addtree(C, H, j, i);
+                        ff_gen_set(C, DB, j, i);
+                        S.need++;
+                    }
             ff_heap_loop(C, H, cvel, A.fom, S);
             sh_ovf = S.overflow;
         }
@@ -1223,7 +1258,9 @@ static size_t ff_near_lds(const mct_fm2d_args &a, int &region, int &heap_cap, in
     const int rgz = rg < (a.nnz - 1) * a.sgdl + 1 ? rg : (a.nnz - 1) * a.sgdl + 1;
     const int rnodes = a.asgr == 1 ? rgx * rgz : 0;
     region = rnodes > lnx * lnz ? rnodes : lnx * lnz;
-    heap_cap = region;
+    // the heap holds a narrow band only; a problem whose band outgrows it goes
+    // to the sequential solver
+    heap_cap = region < 1024 ? region : 1024;
     win = a.asgr == 1 ? wn * wn : 1;
     size_t b = 8 * (size_t)region + 2 * (size_t)((region + 3) & ~3) + 4 * (size_t)((heap_cap + 2 + 1) & ~1) +
                8 * (size_t)win + 2 * (size_t)((win + 1) & ~1) + 4 * (size_t)((region + 31) / 32) +
