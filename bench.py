@@ -37,6 +37,16 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
+def ctypes_vp(p):
+    import ctypes
+    return ctypes.c_void_p(p)
+
+
+def ctypes_i64(n):
+    import ctypes
+    return ctypes.c_int64(n)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -147,6 +157,38 @@ def run_surface(args):
     log("batched: B=%d %.1f ms/step %.2f evals/s  stages(ms) nearest %.2f dispersion %.2f "
         "2-D %.2f misfit %.2f" % (B, ms_step, batched_eps, *st_b))
 
+    # ---- served: Poisson arrivals into the continuous-batching server -----------
+    served = None
+    if args.served > 0:
+        srv = mct.SurfServer(w.grid, w.freqs, w.src, w.rev, w.ttime, w.raystat, max_batch=B, nlanes=2,
+                             raylov=1, order=args.order, max_cells=cmax)
+        reqs = [(P[c % B, :N[c % B]], Q[c % B, :N[c % B]]) for c in range(args.served)]
+        for t in [srv.submit(*reqs[i]) for i in range(min(B, len(reqs)))]:
+            srv.wait(t)                                   # warm-up
+        rate = 0.8 * batched_eps
+        gaps = np.random.default_rng(7).exponential(1.0 / rate, len(reqs))
+        tickets = []
+        t0 = time.perf_counter()
+        for i, (p_, q_) in enumerate(reqs):
+            t_next = t0 + gaps[:i + 1].sum()
+            now = time.perf_counter()
+            if t_next > now:
+                time.sleep(t_next - now)
+            tickets.append(srv.submit(p_, q_))
+        res_s = [srv.wait(t) for t in tickets]
+        t_all = time.perf_counter() - t0
+        lat_s = np.array([r[2] for r in res_s])
+        bad = [r[0] for r in res_s if r[0] != 0]
+        nb, ne, rc = srv.stats()
+        srv.close()
+        if bad or rc:
+            raise RuntimeError("served statuses %s rc %d" % (bad[:5], rc))
+        served = {"requests": len(reqs), "arrival_rate_per_s": round(rate, 3),
+                  "evals_per_s": round(len(reqs) / t_all, 3),
+                  "p50_ms": round(float(np.median(lat_s)), 3), "p99_ms": round(float(np.quantile(lat_s, 0.99)), 3),
+                  "batches": int(nb - 1), "lanes": 2, "max_batch": B}
+        log("served:", served)
+
     # ---- roofline: the dispersion stage (FP64 VALU), live HIP events -----------
     # algorithmic flops = 170 per (dltar evaluation x layer step), Rayleigh
     flops = 170.0 * levB
@@ -204,6 +246,7 @@ def run_surface(args):
         "p99_ms": round(float(np.quantile(lat, 0.99)) * 1e3, 3),
         "batched_vs_plain": round(batched_eps / plain_eps, 3),
         "plain_stage_ms": [round(v, 3) for v in st_plain],
+        "served": served,
         "roofline": roof,
         "cpu_baseline": cpu,
         "max_error_vs_oracle": err,
@@ -306,11 +349,22 @@ def main():
     ap.add_argument("--fm2d-method", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-evals", type=int, default=10)
+    ap.add_argument("--served", type=int, default=48, help="requests for the served measurement (0: skip)")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         log("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU "
             "path (chains would shard across GPUs with no device traffic); use --gpus 1.")
         return 2
+    if args.calib_copy_mb > 0:
+        # a device-to-device copy of known size: the reference reading for the
+        # HBM traffic counters (FETCH_SIZE / WRITE_SIZE) of a profiled run
+        import mctomo_amd as mct
+        n = args.calib_copy_mb * (1 << 20) // 8
+        a, b = mct.DeviceArray((n,), np.float64), mct.DeviceArray((n,), np.float64)
+        mct.lib().mct_copy_d2d(ctypes_vp(b.ptr), ctypes_vp(a.ptr), ctypes_i64(8 * n))
+        a.free(); b.free()
     res = run_surface(args) if args.workload == "surf201" else run_body(args)
     print(json.dumps(res), flush=True)
     return 0

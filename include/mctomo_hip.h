@@ -155,6 +155,9 @@ int mct_dev_alloc(void **ptr, int64_t bytes);
 int mct_dev_free(void *ptr);
 int mct_copy_h2d(void *dst_dev, const void *src_host, int64_t bytes);
 int mct_copy_d2h(void *dst_host, const void *src_dev, int64_t bytes);
+/* Device-to-device copy (also the known-size copy that calibrates HBM
+ * traffic counters). */
+int mct_copy_d2d(void *dst_dev, const void *src_dev, int64_t bytes);
 
 /* Eikonal schedule and precision of a context: band_tiles is how many 4^3
  * tiles the relaxation band advances per round (smaller = fewer re-visits,
@@ -253,6 +256,33 @@ int mct_surf_dispersion_work(mct_surf_ctx *ctx, int nchain, int64_t *evals, int6
  * sweep rounds (four directions each), mean diagonal steps executed, mean
  * node updates} per (period, source) problem. */
 int mct_surf_fm2d_stats(mct_surf_ctx *ctx, double *stats4);
+
+/* The context's HIP stream (hipStream_t) for callers that queue their own
+ * copies in order with the context's work. */
+int mct_surf_stream(mct_surf_ctx *ctx, void **stream);
+
+/* ---- served form: continuous batching -------------------------------------- */
+/* No counterpart in the reference (its sampler calls likelihood() one chain
+ * at a time).  A server owns nlanes surface contexts, each with room for
+ * max_batch chains; a dispatcher thread packs whatever requests are pending
+ * into a batch for the first free lane, so batches on different lanes
+ * overlap on the device and requests that arrive meanwhile are admitted as
+ * soon as a lane frees.  Device memory is fixed at create time (admission is
+ * bounded by nlanes x max_batch slots).  Each result equals the plain call's. */
+typedef struct mct_server mct_server;
+int mct_serve_create(const mct_grid *grid, const mct_surf_setup *setup, int max_batch, int nlanes,
+                     mct_server **srv);
+/* Copies the inputs (points/params (ncells,3), snoise0/1 (np) or NULL) and
+ * returns at once with a ticket. */
+int mct_serve_submit(mct_server *srv, const double *points, const double *params, int ncells,
+                     const double *snoise0, const double *snoise1, int64_t *ticket);
+/* Blocks until the ticket's evaluation is done: out {like, misfit,
+ * unweighted}, status as mct_surf_likelihood's, latency submit -> done (ms). */
+int mct_serve_wait(mct_server *srv, int64_t ticket, double *out, int32_t *status, double *latency_ms);
+/* Batches launched and evaluations finished so far; returns the first error. */
+int mct_serve_stats(mct_server *srv, int64_t *batches, int64_t *evals);
+/* Finishes the pending requests and frees everything. */
+int mct_serve_destroy(mct_server *srv);
 
 /* Device pointers of the context's phase_time (nchain,np,nsrc,nrev) and
  * phase velocity (nchain,nx,ny,np) buffers (valid until destroy). */

@@ -19,5 +19,5 @@ falls back to CPU code: a missing library raises.
 """
 from .api import (  # noqa: F401
     MCTError, Grid, DeviceArray, lib, lib_path, version, fast_marching_3d, kdtree_to_grid, surfdisp96_batch, modrays,
-    BodyLikelihood, SurfLikelihood, REJECT_MODEL, REJECT_RAY,
+    BodyLikelihood, SurfLikelihood, SurfServer, REJECT_MODEL, REJECT_RAY,
 )

@@ -60,11 +60,12 @@ def lib():
         for name in ("fastMarching3D", "mct_voronoi_to_grid", "mct_body_create", "mct_destroy",
                      "mct_body_likelihood", "mct_body_likelihood_batch_dev", "mct_sync",
                      "mct_last_timings", "mct_set_kernel_timing", "mct_set_tuning", "mct_dev_alloc", "mct_dev_free", "mct_copy_h2d",
-                     "mct_copy_d2h", "mct_surfdisp96_batch", "mct_modrays", "mct_surf_create",
+                     "mct_copy_d2h", "mct_copy_d2d", "mct_surfdisp96_batch", "mct_modrays", "mct_surf_create",
                      "mct_surf_destroy", "mct_surf_likelihood", "mct_surf_likelihood_batch_dev",
                      "mct_surf_sync", "mct_surf_last_timings", "mct_surf_dev_arrays",
                      "mct_surf_set_method", "mct_surf_dispersion_work", "mct_surf_fm2d_stats",
-                     "mct_surf_set_dispersion_mode"):
+                     "mct_surf_set_dispersion_mode", "mct_surf_stream", "mct_serve_create",
+                     "mct_serve_submit", "mct_serve_wait", "mct_serve_stats", "mct_serve_destroy"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -349,6 +350,25 @@ class _SurfSetup(ctypes.Structure):
                 ("max_chains", ctypes.c_int)]
 
 
+def _surf_setup(grid, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3, sigdep=0, nrays=None,
+                gridx=1, gridy=1, sgref=1, sgdic=4, sgext=8, order=1, band=0.5, max_cells=1024,
+                max_chains=1):
+    """A filled mct_surf_setup and the arrays it points to (keep them alive)."""
+    fr = _f64(freqs).reshape(-1)
+    npd = fr.shape[0]
+    sr, rv = _f64(src).reshape(-1, 2), _f64(rev).reshape(-1, 2)
+    npair = sr.shape[0] * rv.shape[0]
+    tt = _f64(ttime).reshape(npd, 3, npair)
+    rs = np.ascontiguousarray(raystat, dtype=np.int32).reshape(npd, 2, npair)
+    if nrays is None:
+        nrays = int((rs[:, 0, :] == 1).sum())
+    st = _SurfSetup(int(raylov), 0, float(dphase), npd, fr.ctypes.data, sr.shape[0], rv.shape[0],
+                    sr.ctypes.data, rv.ctypes.data, tt.ctypes.data, rs.ctypes.data, int(sigdep), int(nrays),
+                    int(gridx), int(gridy), int(sgref), int(sgdic), int(sgext), int(order), float(band),
+                    int(max_cells), int(max_chains))
+    return st, (fr, sr, rv, tt, rs)
+
+
 class SurfLikelihood:
     """Surface-wave likelihood context: like_setup + likelihood for datatype 2
     with phase velocities (src/likelihood_surf.F90:110-405).
@@ -463,3 +483,52 @@ class SurfLikelihood:
         a, b = ctypes.c_void_p(), ctypes.c_void_p()
         lib().mct_surf_dev_arrays(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+
+class SurfServer:
+    """Continuous-batching server for surface-wave likelihoods (one grid and
+    data set): nlanes contexts of max_batch chains each, fed by a dispatcher
+    thread in the library.  submit() returns a ticket at once; wait() blocks
+    until that evaluation is done and returns (status, out[3], latency_ms)."""
+
+    def __init__(self, grid, freqs, src, rev, ttime, raystat, max_batch=16, nlanes=2, **kw):
+        self.np = len(freqs)
+        st, keep = _surf_setup(grid, freqs, src, rev, ttime, raystat, **kw)
+        self._keep = keep
+        self._g = grid.c()
+        h = ctypes.c_void_p()
+        _check(lib().mct_serve_create(ctypes.byref(self._g), ctypes.byref(st), int(max_batch), int(nlanes),
+                                      ctypes.byref(h)), "mct_serve_create")
+        self._h = h
+
+    def submit(self, points, params, snoise0=None, snoise1=None):
+        pts, prm = _f64(points), _f64(params)
+        s0 = None if snoise0 is None else _f64(np.broadcast_to(np.atleast_1d(snoise0), (self.np,)))
+        s1 = None if snoise1 is None else _f64(np.broadcast_to(np.atleast_1d(snoise1), (self.np,)))
+        t = ctypes.c_int64()
+        _check(lib().mct_serve_submit(self._h, _p(pts), _p(prm), pts.shape[0], _p(s0), _p(s1), ctypes.byref(t)),
+               "mct_serve_submit")
+        return t.value
+
+    def wait(self, ticket):
+        out = np.zeros(3)
+        st, lat = ctypes.c_int32(), ctypes.c_double()
+        _check(lib().mct_serve_wait(self._h, ctypes.c_int64(ticket), _p(out), ctypes.byref(st), ctypes.byref(lat)),
+               "mct_serve_wait")
+        return st.value, out, lat.value
+
+    def stats(self):
+        b, e = ctypes.c_int64(), ctypes.c_int64()
+        rc = lib().mct_serve_stats(self._h, ctypes.byref(b), ctypes.byref(e))
+        return b.value, e.value, rc
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mct_serve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -403,6 +403,13 @@ int mct_copy_h2d(void *dst_dev, const void *src_host, int64_t bytes)
     return MCT_OK;
 }
 
+int mct_copy_d2d(void *dst_dev, const void *src_dev, int64_t bytes)
+{
+    if (bytes < 0 || (bytes > 0 && (!dst_dev || !src_dev))) return MCT_ERR_ARG;
+    if (bytes) MCT_CHECK(hipMemcpy(dst_dev, src_dev, (size_t)bytes, hipMemcpyDeviceToDevice));
+    return MCT_OK;
+}
+
 int mct_copy_d2h(void *dst_host, const void *src_dev, int64_t bytes)
 {
     if (bytes < 0 || (bytes > 0 && (!dst_host || !src_dev))) return MCT_ERR_ARG;

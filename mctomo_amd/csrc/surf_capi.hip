@@ -444,6 +444,13 @@ int mct_surf_dispersion_work(mct_surf_ctx *C, int nchain, int64_t *evals, int64_
     return MCT_OK;
 }
 
+int mct_surf_stream(mct_surf_ctx *C, void **stream)
+{
+    if (!C || !stream) return MCT_ERR_ARG;
+    *stream = (void *)C->stream;
+    return MCT_OK;
+}
+
 int mct_surf_dev_arrays(mct_surf_ctx *C, double **phase_time_dev, double **phase_vel_dev)
 {
     if (!C) return MCT_ERR_ARG;
