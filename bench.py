@@ -344,7 +344,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--order", type=int, default=1)
+    ap.add_argument("--order", type=int, default=0)
     ap.add_argument("--workload", default="surf201", choices=["surf201", "body101"])
     ap.add_argument("--fm2d-method", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")

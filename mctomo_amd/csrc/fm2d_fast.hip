@@ -1181,7 +1181,30 @@ __global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, const int32
     V.nnx = A.nnx; V.nnz = A.nnz; V.dnx = dnx; V.dnz = dnz;
     V.T = A.ttn + slot * wsz; V.ns = nullptr; V.ox = 0; V.oz = 0; V.lnx = A.nnx; V.lnz = A.nnz; V.ld = A.ld;
     const int32_t *F = A.nsts + slot * wsz;
-    auto fixed = [&](int iz, int ix) -> bool { return F[(long)(ix - 1) * A.ld + (iz - 1)] != 0; };
+    // the fixed mask as an LDS bitmap (z fastest) when the work lists are in LDS
+    const long nn = (long)A.nnx * A.nnz;
+    uint32_t *FB = use_dirty ? ff_dirty + (nn + 31) / 32 + 2 * (A.nnx + A.nnz) : nullptr;
+    if (FB) {
+        for (long w = threadIdx.x; w < (nn + 31) / 32; w += blockDim.x) {
+            uint32_t m = 0;
+            for (int b = 0; b < 32; b++) {
+                long q = w * 32 + b;
+                if (q < nn) {
+                    int x = (int)(q / A.nnz) + 1, z = (int)(q % A.nnz) + 1;
+                    if (F[(long)(x - 1) * A.ld + (z - 1)] != 0) m |= 1u << b;
+                }
+            }
+            FB[w] = m;
+        }
+        __syncthreads();
+    }
+    auto fixed = [&](int iz, int ix) -> bool {
+        if (FB) {
+            long q = (long)(ix - 1) * A.nnz + (iz - 1);
+            return (FB[q >> 5] >> (q & 31)) & 1u;
+        }
+        return F[(long)(ix - 1) * A.ld + (iz - 1)] != 0;
+    };
     auto vel = [&](int iz, int ix) -> double { return velnc[(long)(ix - 1) * A.nnz + (iz - 1)]; };
     ff_work W;
     W.bits = ff_dirty;
@@ -1283,7 +1306,7 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
     int rc = mct_launch_fm2d_gridder(a, s);
     if (rc) return rc;
     // the sweep's dirty bitmap in LDS when it fits
-    const size_t dirty_bytes = 4 * (((size_t)a.nnx * a.nnz + 31) / 32) + 4 * 2 * (size_t)(a.nnx + a.nnz);
+    const size_t dirty_bytes = 2 * 4 * (((size_t)a.nnx * a.nnz + 31) / 32) + 4 * 2 * (size_t)(a.nnx + a.nnz);
     const int use_dirty = dirty_bytes <= 48 * 1024;
     const size_t dirty_lds = use_dirty ? dirty_bytes : 0;
     // threads per 2-D sweep (one problem per workgroup; one wave measured fastest):

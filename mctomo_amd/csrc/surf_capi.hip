@@ -208,9 +208,20 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     const long npair = (long)S.nsrc * S.nrev, gsz = (long)grid->nx * grid->ny * grid->nz;
     const long ncol = (long)grid->nx * grid->ny;
     const long nprob = (long)M * np * S.nsrc;
-    // fm2d workspace: at most 4 GiB of problem slots, whole waves of 64
+    // fm2d workspace: problem slots for at most MCT_FM2D_WS_GB GiB (default 16),
+    // whole waves of 64; more problems run in chunks
     const long slot = mct_fm2d_slot_bytes(F.ld, F.nc);
-    long chunk = (4L << 30) / slot;
+    long ws_gb = 16;
+    {
+        const char *e = getenv("MCT_FM2D_WS_GB");
+        if (e && atol(e) > 0) ws_gb = atol(e);
+    }
+    long chunk = (ws_gb << 30) / slot;
+    {
+        const char *e = getenv("MCT_FM2D_CHUNK");     // problems per 2-D launch (tuning)
+        long v = e ? atol(e) : 0;
+        if (v > 0 && v < chunk) chunk = v;
+    }
     chunk = chunk < 64 ? 64 : (chunk / 64) * 64;
     C->fm2d_chunk = chunk < nprob ? chunk : nprob;
     {
