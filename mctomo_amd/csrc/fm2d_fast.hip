@@ -817,6 +817,15 @@ __device__ __forceinline__ void bspl_w4(double u, double w[4])
     w[3] = (u * u * u) / 6.0;
 }
 
+// Work-group -> problem slot map that keeps neighbouring slots (the sources
+// of one period, which read the same velocity grid) on one XCD: work-groups
+// are dealt to the 8 XCDs round-robin, so XCD x gets a contiguous slot range.
+__device__ __forceinline__ long ff_xcd_slot(long b, long nb)
+{
+    const long q = nb / 8, r = nb % 8, x = b % 8;
+    return x * q + (x < r ? x : r) + b / 8;
+}
+
 struct near_geom {
     int s, p, c;
     long prob, slot;
@@ -888,7 +897,7 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
 
     const int lane = threadIdx.x;
     near_geom g;
-    const long slot = blockIdx.x;
+    const long slot = ff_xcd_slot(blockIdx.x, gridDim.x);
     if (!ff_problem(A, slot, g, true)) {
         if (lane == 0) pflag[slot] = 2;   // nothing to do
         return;
@@ -1169,7 +1178,7 @@ __global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, const int32
 {
     extern __shared__ uint32_t ff_dirty[];
     __shared__ int sh_flag;
-    const long slot = blockIdx.x;
+    const long slot = ff_xcd_slot(blockIdx.x, gridDim.x);
     if (pflag[slot] != 0) return;
     near_geom g;
     if (!ff_problem(A, slot, g, false)) return;
