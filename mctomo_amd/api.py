@@ -469,7 +469,8 @@ class SurfLikelihood:
         return a.value, b.value
 
     def set_dispersion_mode(self, mode):
-        """Dispersion launch shape: 0 by size, 1 lane per column, 2 wave per column."""
+        """Dispersion launch shape: 0 by size, 1 lane per column, 2 wave per column,
+        3 lane per column with columns taken from a queue."""
         _check(lib().mct_surf_set_dispersion_mode(self._h, int(mode)), "mct_surf_set_dispersion_mode")
 
     def fm2d_stats(self):

@@ -4,8 +4,12 @@
 // :523-629), the surfdisp96 branch of surfmodes (surfmodes/surfmodes.f90
 // :79-95) and surfdisp96 itself (surfmodes/surfdisp96.f:52-382).
 //
-// Two launch shapes.  k_dispersion_lane: one lane per column, each lane runs
-// surfdisp96 as written (64 columns per wave; throughput).  k_dispersion: one
+// Three launch shapes.  k_dispersion_sm (throughput): one lane per column,
+// columns taken from a queue, surfdisp96's control flow as a per-lane state
+// machine with one period-equation call site, so the 64 lanes of a wave share
+// every evaluation whatever stage of the root search each is in.
+// k_dispersion_lane: one lane per column running surfdisp96 as written (lanes
+// wait for the wave's longest walk at every branch).  k_dispersion: one
 // wave64 per column with the layer model in LDS (latency).  The expensive part of
 // surfdisp96 is getsol's bracketing walk (:759-815): one period-equation
 // evaluation per dc step, ~10^3 steps for the first period.  Here the 64 lanes
@@ -701,11 +705,321 @@ __global__ __launch_bounds__(64) void k_dispersion_lane(mct_dsp_args A, float *l
     if (A.nlayer) A.nlayer[(long)c * A.ncol + col] = M.mmax;
 }
 
+// One lane per column, columns handed out from a queue, and surfdisp96's
+// control flow as a per-lane state machine with a single period-equation
+// call site: every iteration each lane advances its own column (walk step,
+// Neville step, next period, next column) until it needs dltar, then the
+// whole wave evaluates dltar together.  Lanes never wait for one another's
+// walks, and a lane that finishes a column takes the next one.  The sequence
+// of operations per column is surfdisp96's (the same as k_dispersion_lane).
+enum {
+    SM_NEXT, SM_PERIOD, SM_G0, SM_G1, SM_N0, SM_NLOOP, SM_N1, SM_NBODY, SM_N2, SM_N3,
+    SM_NDONE, SM_GDONE, SM_FINISH, SM_DONE
+};
+
+__global__ void k_dsp_queue_reset(unsigned long long *q)
+{
+    if (threadIdx.x == 0) q[0] = 0ull;
+}
+
+__global__ __launch_bounds__(64) void k_dispersion_sm(mct_dsp_args A, float *lay, int nlmax, long total)
+{
+    const int lane = threadIdx.x;
+    const long plane = (long)nlmax * 64;
+    float *base = lay + (long)blockIdx.x * 4 * plane + lane;
+    float *sd = base, *sa = base + plane, *sb = base + 2 * plane, *sr = base + 3 * plane;
+    const double POISSON = (double)1.730f;     // src/utils.f90:97-112 (vs2vp_3d)
+    const double EPS = (double)1.0e-10f;       // src/likelihood_surf.F90:37
+    const double twopi = 2.0 * 3.141592653589793;
+    const float sone = 1.500f, h = 0.005f;
+    const float ddc = (float)A.dphase;
+    const double one = 1.0e-2, onea = (double)sone;
+    const double dc = fabs((double)ddc);
+    const int ifunc = A.iwave;
+
+    dsp_model_t<64> M;
+    M.d = sd; M.a = sa; M.b = sb; M.rho = sr;
+    M.mmax = 1; M.llw = 1;
+    // column state
+    long job = -1;
+    int c = 0;
+    long col = 0;
+    double *cp = nullptr, *cg = nullptr;
+    int ierr = 0, ndl = 0, k = 1, pass = 0, ifirst = 0;
+    float betmx = 0.0f;
+    double cc = 0.0, cm = 0.0, cmax = 0.0, cprev = 0.0, del1st = 0.0, croot = 0.0;
+    float t1a = 0.0f, t1b = 0.0f;
+    double t1 = 0.0, omega = 0.0;
+    // getsol / nevill state
+    double c1 = 0.0, c2 = 0.0, del1 = 0.0, del2 = 0.0, clow = 0.0;
+    int idir = 1, gret = 0, walk = 0;
+    double gc = 0.0;
+    double nc1 = 0.0, nc2 = 0.0, nd1 = 0.0, nd2 = 0.0, c3 = 0.0, del3 = 0.0;
+    int nev = 0, nctrl = 0, m = 1;
+    // Neville tables in LDS, lane-interleaved
+    __shared__ double xs[12 * 64], ys[12 * 64];
+#define X(j) xs[(j) * 64 + lane]
+#define Y(j) ys[(j) * 64 + lane]
+    double res = 0.0, ceval = 0.0;
+    int pc = SM_NEXT;
+
+    for (;;) {
+        bool need = false;
+        while (!need && pc != SM_DONE) {
+            switch (pc) {
+            case SM_NEXT: {
+                job = (long)atomicAdd((unsigned long long *)A.queue, 1ull);
+                if (job >= total) { pc = SM_DONE; break; }
+                c = (int)(job / A.ncol);
+                col = job - (long)c * A.ncol;
+                const double *vs = A.vs + ((long)c * A.ncol + col) * A.nz;
+                // check_model (src/likelihood_surf.F90:631-646)
+                bool below = false;
+                const double top = vs[0];
+                for (int kk = 1; kk < A.nz; kk++) below |= vs[kk] < top;
+                if (below) A.bad_model[c] = 1;
+                // convert_to_layer (src/likelihood_surf.F90:523-629)
+                int nl = 0, last_k = 1;
+                double last_vs = vs[0];
+                for (int kk = 2; kk <= A.nz; kk++) {
+                    double vk = vs[kk - 1];
+                    if (fabs(vk - last_vs) > EPS) {
+                        double vpk = last_vs * POISSON;
+                        sa[64 * nl] = (float)vpk;
+                        sb[64 * nl] = (float)last_vs;
+                        sr[64 * nl] = (float)((double)1.74f * pow(vpk, 0.25));
+                        sd[64 * nl] = (float)(((double)(kk - last_k) * A.dz) / A.scaling);
+                        nl++;
+                        last_vs = vk;
+                        last_k = kk;
+                    }
+                }
+                {
+                    double vpk = vs[A.nz - 1] * POISSON;
+                    sa[64 * nl] = (float)vpk;
+                    sb[64 * nl] = (float)vs[A.nz - 1];
+                    sr[64 * nl] = (float)((double)1.74f * pow(vpk, 0.25));
+                    sd[64 * nl] = 0.0f;
+                }
+                M.mmax = nl + 1;
+                M.llw = (sb[0] <= 0.0f) ? 2 : 1;
+                cp = A.phase + ((long)c * A.ncol + col) * A.np;
+                cg = A.group + ((long)c * A.ncol + col) * A.np;
+                ierr = 0; ndl = 0;
+                int jmn = 1, jsol = 1;
+                betmx = -1.e20f;
+                float betmn = 1.e20f;
+                for (int ii = 1; ii <= M.mmax; ii++) {
+                    float bi = sb[64 * (ii - 1)], ai = sa[64 * (ii - 1)];
+                    if (bi > 0.01f && bi < betmn) { betmn = bi; jmn = ii; jsol = 1; }
+                    else if (bi <= 0.01f && ai < betmn) { betmn = ai; jmn = ii; jsol = 0; }
+                    if (bi > betmx) betmx = bi;
+                }
+                float cc1 = jsol == 0 ? betmn : d_gtsolh(sa[64 * (jmn - 1)], sb[64 * (jmn - 1)]);
+                cc1 = 0.95f * cc1;
+                cc1 = 0.90f * cc1;
+                cc = (double)cc1;
+                cm = cc;
+                cmax = (double)betmx + dc;
+                cprev = 0.0; del1st = 0.0;
+                k = 1; pass = 0;
+                pc = SM_PERIOD;
+                break;
+            }
+            case SM_PERIOD: {
+                t1 = A.periods[k - 1];
+                if (A.igr > 0) {
+                    t1a = (float)(t1 / (double)(1.0f + h));
+                    t1b = (float)(t1 / (double)(1.0f - h));
+                    t1 = (double)t1a;
+                } else {
+                    t1a = (float)t1;
+                }
+                if (k == 1) { c1 = cc; clow = cc; ifirst = 1; }
+                else { ifirst = 0; c1 = cprev > 0 ? cprev - onea * dc : cc; clow = cm; }
+                pass = 0;
+                omega = twopi / t1;          // getsol: wvno = omega / c1
+                ceval = c1; need = true; pc = SM_G0;
+                break;
+            }
+            case SM_G0: {
+                del1 = res; ndl++;
+                if (ifirst == 1) del1st = del1;
+                double plmn = dsgn(del1st) * dsgn(del1);
+                idir = (ifirst == 1) ? 1 : (plmn >= 0.0 ? 1 : -1);
+                for (;;) {
+                    c2 = idir > 0 ? c1 + dc : c1 - dc;
+                    if (c2 <= clow) { idir = 1; c1 = clow; continue; }
+                    break;
+                }
+                walk = 0;
+                omega = twopi / t1;
+                ceval = c2; need = true; pc = SM_G1;
+                break;
+            }
+            case SM_G1: {
+                del2 = res; ndl++;
+                if (dsgn(del1) != dsgn(del2)) {
+                    nc1 = c1; nc2 = c2; nd1 = del1; nd2 = del2;
+                    omega = twopi / t1;      // nevill (surfdisp96.f:903-1020)
+                    c3 = 0.5 * (nc1 + nc2);
+                    ceval = c3; need = true; pc = SM_N0;
+                    break;
+                }
+                c1 = c2;
+                del1 = del2;
+                if (c1 < cm || c1 >= cmax || ++walk >= (1 << 24)) { gret = -1; gc = c1; pc = SM_GDONE; break; }
+                for (;;) {
+                    c2 = idir > 0 ? c1 + dc : c1 - dc;
+                    if (c2 <= clow) { idir = 1; c1 = clow; continue; }
+                    break;
+                }
+                omega = twopi / t1;
+                ceval = c2; need = true; pc = SM_G1;
+                break;
+            }
+            case SM_N0:
+                del3 = res; ndl++;
+                nev = 1; nctrl = 1; m = 1;
+                pc = SM_NLOOP;
+                break;
+            case SM_NLOOP:
+                nctrl = nctrl + 1;
+                if (nctrl >= 100) { pc = SM_NDONE; break; }
+                if (c3 < fmin(nc1, nc2) || c3 > fmax(nc1, nc2)) {
+                    nev = 0;
+                    c3 = 0.5 * (nc1 + nc2);
+                    ceval = c3; need = true; pc = SM_N1;
+                    break;
+                }
+                pc = SM_NBODY;
+                break;
+            case SM_N1:
+                del3 = res; ndl++;
+                pc = SM_NBODY;
+                break;
+            case SM_NBODY: {
+                double s13 = nd1 - del3;
+                double s32 = del3 - nd2;
+                if (dsgn(del3) * dsgn(nd1) < 0.0) { nc2 = c3; nd2 = del3; }
+                else { nc1 = c3; nd1 = del3; }
+                if (fabs(nc1 - nc2) <= 1.0e-6 * nc1) { pc = SM_NDONE; break; }
+                if (dsgn(s13) != dsgn(s32)) nev = 0;
+                double ss1 = fabs(nd1);
+                double s1 = (double)0.01f * ss1;
+                double ss2 = fabs(nd2);
+                double s2 = (double)0.01f * ss2;
+                bool halve = s1 > ss2 || s2 > ss1 || nev == 0;
+                if (!halve) {
+                    if (nev == 2) {
+                        X(m + 1) = c3;
+                        Y(m + 1) = del3;
+                    } else {
+                        X(1) = nc1; Y(1) = nd1;
+                        X(2) = nc2; Y(2) = nd2;
+                        m = 1;
+                    }
+                    bool bad = false;
+                    for (int kk = 1; kk <= m; kk++) {
+                        int j = m - kk + 1;
+                        double denom = Y(m + 1) - Y(j);
+                        if (fabs(denom) < 1.0e-10 * fabs(Y(m + 1))) { bad = true; break; }
+                        X(j) = (-(Y(j) * X(j + 1)) + Y(m + 1) * X(j)) / denom;
+                    }
+                    if (!bad) {
+                        c3 = X(1);
+                        ceval = c3; need = true; pc = SM_N2;
+                        break;
+                    }
+                }
+                c3 = 0.5 * (nc1 + nc2);
+                ceval = c3; need = true; pc = SM_N3;
+                break;
+            }
+            case SM_N2:
+                del3 = res; ndl++;
+                nev = 2;
+                m = m + 1;
+                if (m > 10) m = 10;
+                pc = SM_NLOOP;
+                break;
+            case SM_N3:
+                del3 = res; ndl++;
+                nev = 1;
+                m = 1;
+                pc = SM_NLOOP;
+                break;
+            case SM_NDONE:
+                gc = c3;
+                gret = (c3 > (double)betmx) ? -1 : 1;
+                pc = SM_GDONE;
+                break;
+            case SM_GDONE:
+                if (pass == 0) {
+                    if (gret == -1) {
+                        // no root at period k: the rest keep cp=100, cg=0 (labels 1700/1750)
+                        ierr = 1;
+                        for (int ii = k; ii <= A.np; ii++) { cp[ii - 1] = 100.0; cg[ii - 1] = 0.0; }
+                        pc = SM_FINISH;
+                        break;
+                    }
+                    croot = gc;
+                    cprev = croot;
+                    if (A.igr > 0) {
+                        pass = 1;
+                        t1 = (double)t1b;
+                        ifirst = 0;
+                        clow = 0.0 + one * dc;
+                        c1 = croot - onea * dc;
+                        omega = twopi / t1;
+                        ceval = c1; need = true; pc = SM_G0;
+                        break;
+                    }
+                    cp[k - 1] = (double)(float)croot;
+                    cg[k - 1] = 100.0;
+                } else {
+                    double cgr = gret == -1 ? croot : gc;
+                    if (gret == -1) ierr = 1;
+                    float cc0 = (float)croot, cc1k = (float)cgr;
+                    float gvel = (1.0f / t1a - 1.0f / t1b) / (1.0f / (t1a * cc0) - 1.0f / (t1b * cc1k));
+                    cg[k - 1] = (double)gvel;
+                    cp[k - 1] = (double)cc0;
+                    if (gvel < 0 || croot == 0) ierr = 1;
+                    pass = 0;
+                }
+                k++;
+                pc = k > A.np ? SM_FINISH : SM_PERIOD;
+                break;
+            case SM_FINISH:
+                A.ierr[(long)c * A.ncol + col] = ierr;
+                if (A.ndltar) A.ndltar[(long)c * A.ncol + col] = ndl;
+                if (A.nlayer) A.nlayer[(long)c * A.ncol + col] = M.mmax;
+                pc = SM_NEXT;
+                break;
+            default:
+                pc = SM_DONE;
+                break;
+            }
+        }
+        if (__ballot(need) == 0ull) break;     // every lane has run out of columns
+        if (need) res = d_dltar(omega / ceval, omega, ifunc, M);
+    }
+#undef X
+#undef Y
+}
+
 int mct_launch_dispersion(const mct_dsp_args &a, hipStream_t s)
 {
     if (a.np < 1 || a.np > DSP_MAXP || a.nz < 1 || a.nz > DSP_NMAX) return MCT_ERR_ARG;
     MCT_CHECK(hipMemsetAsync(a.bad_model, 0, sizeof(int32_t) * a.nchain, s));
-    if (a.lay) {
+    if (a.lay && a.queue) {
+        // persistent waves, columns from a queue
+        hipLaunchKernelGGL(k_dsp_queue_reset, dim3(1), dim3(64), 0, s, a.queue);
+        const long total = (long)a.ncol * a.nchain;
+        long nw = (total + 63) / 64;
+        if (nw > a.lay_waves) nw = a.lay_waves;
+        hipLaunchKernelGGL(k_dispersion_sm, dim3((unsigned)nw), dim3(64), 0, s, a, a.lay, a.nz, total);
+    } else if (a.lay) {
         const unsigned nb = (unsigned)((a.ncol + 63) / 64);
         hipLaunchKernelGGL(k_dispersion_lane, dim3(nb, a.nchain), dim3(64), 0, s, a, a.lay, a.nz);
     } else {

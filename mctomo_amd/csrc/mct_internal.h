@@ -211,6 +211,8 @@ struct mct_dsp_args {
     int32_t *nlayer;              // (nchain, ncol) or NULL: layers of the column
     float *lay;                   // lane-per-column scratch (mct_dispersion_lay_floats) or NULL
                                   // for one wave per column
+    unsigned long long *queue;    // column queue counter: with lay, persistent waves
+    long lay_waves;               // waves the scratch holds (queue mode)
 };
 // One wave per (column, chain): check_model, convert_to_layer, surfdisp96.
 // Requires np <= 60 and nz <= 100.

@@ -80,7 +80,9 @@ struct mct_surf_ctx {
     mct_fm2d_args F;                 // geometry part of the 2-D solve
     long fm2d_chunk;
     int fm2d_method;                 // 0 parallel sweeps, 1 sequential heap loop
-    int dsp_mode;                    // 0 by size, 1 lane per column, 2 wave per column
+    long dsp_waves;                  // queue mode: at most this many waves (0: one per 64 columns)
+    int dsp_mode;                    // 0 by size, 1 lane per column, 2 wave per column,
+                                     // 3 lane per column from a column queue
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double ms[4] = {0, 0, 0, 0};
@@ -90,7 +92,7 @@ struct mct_surf_ctx {
     sbuf points, params, noise, out, status;
     // stage buffers
     sbuf vs, flag_count, flag_nodes, kd_nodes, kd_ind, kd_rdata;
-    sbuf phase, group, ierr, bad_model, ndltar, nlayer, lay, velv, veln, phase_time, srdist, sigma;
+    sbuf phase, group, ierr, bad_model, ndltar, nlayer, lay, dsp_queue, velv, veln, phase_time, srdist, sigma;
     sbuf ttn, ttr, vr, nsts, nstr, heap, pflag, rounds;
     double *h_in = nullptr, *h_out = nullptr;
     int32_t *h_status = nullptr;
@@ -127,8 +129,16 @@ static int surf_pipeline(mct_surf_ctx *C, int nchain, const double *d_points, co
     da.phase = C->phase.as<double>(); da.group = C->group.as<double>(); da.ierr = C->ierr.as<int32_t>();
     da.bad_model = C->bad_model.as<int32_t>(); da.ndltar = C->ndltar.as<int32_t>();
     da.nlayer = C->nlayer.as<int32_t>();
-    // many columns: one lane per column (throughput); few: one wave per column (latency)
-    da.lay = (C->dsp_mode == 1 || (C->dsp_mode == 0 && (long)ncol * nchain >= 100000)) ? C->lay.as<float>() : nullptr;
+    // many columns: one lane per column from a column queue (throughput); few:
+    // one wave per column (latency)
+    int dmode = C->dsp_mode;
+    if (dmode == 0) dmode = (long)ncol * nchain >= 16384 ? 3 : 2;
+    da.lay = (dmode == 1 || dmode == 3) ? C->lay.as<float>() : nullptr;
+    if (da.lay && dmode == 3) {
+        da.queue = C->dsp_queue.as<unsigned long long>();
+        da.lay_waves = (long)nchain * ((ncol + 63) / 64);
+        if (C->dsp_waves > 0 && da.lay_waves > C->dsp_waves) da.lay_waves = C->dsp_waves;
+    }
     if ((rc = mct_launch_dispersion(da, s))) return rc;
     hipLaunchKernelGGL(k_surf_status, dim3(nchain), dim3(256), 0, s, (const int32_t *)da.bad_model,
                        (const int32_t *)da.ierr, ncol, d_status);
@@ -229,7 +239,9 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
         C->fm2d_method = e && atoi(e) == 1 ? 1 : 0;
         const char *d = getenv("MCT_DSP_MODE");
         C->dsp_mode = d ? atoi(d) : 0;
-        if (C->dsp_mode < 0 || C->dsp_mode > 2) C->dsp_mode = 0;
+        if (C->dsp_mode < 0 || C->dsp_mode > 3) C->dsp_mode = 0;
+        const char *w = getenv("MCT_DSP_WAVES");
+        C->dsp_waves = w ? atol(w) : 0;
     }
     int rc = MCT_OK;
     if (hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking) != hipSuccess) { delete C; return MCT_ERR_HIP; }
@@ -260,6 +272,7 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     ALLOC(ndltar, sizeof(int32_t) * ncol * M);
     ALLOC(nlayer, sizeof(int32_t) * ncol * M);
     ALLOC(lay, sizeof(float) * mct_dispersion_lay_floats((int)ncol, grid->nz, M));
+    ALLOC(dsp_queue, 256);
     ALLOC(velv, sizeof(double) * (grid->nx + 2) * (grid->ny + 2) * np * M);
     ALLOC(veln, sizeof(double) * nnx * nnz * np * M);
     ALLOC(phase_time, sizeof(double) * npair * np * M);
@@ -430,7 +443,7 @@ int mct_surf_set_method(mct_surf_ctx *C, int fm2d_method)
 
 int mct_surf_set_dispersion_mode(mct_surf_ctx *C, int mode)
 {
-    if (!C || mode < 0 || mode > 2) return MCT_ERR_ARG;
+    if (!C || mode < 0 || mode > 3) return MCT_ERR_ARG;
     C->dsp_mode = mode;
     return MCT_OK;
 }

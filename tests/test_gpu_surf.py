@@ -5,6 +5,7 @@ log-likelihood within 1e-6 relative.  Given the GPU's own phase velocities,
 the 2-D traveltimes must match the oracle's modrays to 1e-6 relative with the
 parallel 2-D method and bit for bit with the sequential one.
 """
+import ctypes
 import sys
 
 import numpy as np
@@ -59,7 +60,7 @@ def check_against_oracle(w, got, tag, sigdep=0, s0=None, s1=None):
 
 
 @pytest.mark.parametrize("order,raylov,method,dsp", [(1, 1, 0, 0), (0, 1, 0, 0), (1, 0, 0, 0), (1, 1, 1, 0),
-                                                     (1, 1, 0, 1), (1, 0, 0, 1)])
+                                                     (1, 1, 0, 1), (1, 0, 0, 1), (1, 1, 0, 3), (1, 0, 0, 3)])
 def test_plain_call_matches_oracle(order, raylov, method, dsp):
     # Love roots fail (ierr) for many of these models; seed 20261207 is one that works
     seed = 20261202 if raylov == 1 else 20261207
@@ -149,4 +150,48 @@ def test_batched_chains_match_plain_calls_and_oracle():
             assert abs(out[c, 0] - orc[1][0]) <= 1e-6 * abs(orc[1][0])
         else:
             assert c == 2 and rc == mct.REJECT_MODEL
+    L.close()
+
+
+@pytest.mark.parametrize("waves", [0, 3])
+def test_column_queue_dispersion_matches_lane_per_column(waves, monkeypatch):
+    """Dispersion mode 3 (columns from a queue, per-lane state machine) gives the
+    same phase velocities, statuses, likelihoods and period-equation counts as
+    one lane per column; with 3 waves every lane runs many columns in turn."""
+    if waves:
+        monkeypatch.setenv("MCT_DSP_WAVES", str(waves))
+    nchain, maxc = 4, 30
+    rng = np.random.default_rng(17)
+    pts = np.zeros((nchain, maxc, 3))
+    prm = np.zeros((nchain, maxc, 3))
+    ncl = np.zeros(nchain, np.int32)
+    for c in range(nchain):
+        k = int(rng.integers(8, maxc + 1))
+        w = W.surface_workload(n=29, nz=15, L=14.0, Z=7.0, ncells=k, np_=8, nsrc=6, nrev=6, seed=300 + c)
+        pts[c, :k], prm[c, :k], ncl[c] = w.points, w.params, k
+    k = ncl[1]
+    prm[1, int(np.argmax(pts[1, :k, 2])), 1] = prm[1, :k, 1].min() - 0.3
+    L = ctx_for(w, max_cells=maxc, max_chains=nchain)
+    dP, dQ = mct.DeviceArray.from_numpy(pts), mct.DeviceArray.from_numpy(prm)
+    dN = mct.DeviceArray.from_numpy(ncl)
+    dS = mct.DeviceArray.from_numpy(np.zeros((nchain, 2, 8)))
+    res = {}
+    for mode in (1, 3):
+        say("surf dispersion mode", mode, "waves", waves)
+        L.set_dispersion_mode(mode)
+        dO = mct.DeviceArray((nchain, 3), np.float64)
+        dT = mct.DeviceArray((nchain,), np.int32)
+        L.likelihood_batch_dev(nchain, dP.ptr, dQ.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
+        L.sync()
+        ph = np.empty((nchain, 29 * 29, 8))
+        assert mct.lib().mct_copy_d2h(ctypes.c_void_p(ph.ctypes.data), ctypes.c_void_p(L.dev_arrays()[1]),
+                                      ctypes.c_int64(ph.nbytes)) == 0
+        res[mode] = (dO.numpy(), dT.numpy(), ph, L.dispersion_work(nchain))
+    o1, t1, p1, w1 = res[1]
+    o3, t3, p3, w3 = res[3]
+    assert t1[1] == mct.REJECT_MODEL
+    np.testing.assert_array_equal(t1, t3)
+    np.testing.assert_array_equal(o1[t1 == 0], o3[t3 == 0])
+    np.testing.assert_array_equal(p1, p3)
+    assert w1 == w3
     L.close()
