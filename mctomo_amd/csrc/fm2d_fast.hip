@@ -235,31 +235,6 @@ __device__ __forceinline__ double ff_fouds(const fview &V, const AL &al, int iz,
     return fom == 0 ? ff_fouds1(V, al, iz, ix, slown) : ff_fouds2(V, al, iz, ix, slown);
 }
 
-// The update over the neighbours whose times are below thr.
-__device__ double ff_cand1(const fview &V, int iz, int ix, double thr, double slown, int fom)
-{
-    auto al = [&](int z, int x) -> bool {
-        double t = V.t(z, x);
-        return t < thr && t < FF_FIN;
-    };
-    bool any = (iz > 1 && al(iz - 1, ix)) || (iz < V.nnz && al(iz + 1, ix)) ||
-               (ix > 1 && al(iz, ix - 1)) || (ix < V.nnx && al(iz, ix + 1));
-    if (!any) return FF_BIG;
-    return ff_fouds(V, al, iz, ix, slown, fom);
-}
-
-// Self-consistent update: the neighbours counted are those below the result.
-__device__ double ff_cand(const fview &V, int iz, int ix, double slown, int fom)
-{
-    double c = ff_cand1(V, iz, ix, FF_THR0, slown, fom);
-    for (int k = 0; k < 8; k++) {
-        double c2 = ff_cand1(V, iz, ix, c, slown, fom);
-        if (c2 == c || c2 >= FF_THR0) break;
-        c = c2;
-    }
-    return c;
-}
-
 // The same updates on a register copy of the node's axis stencil: [0] the
 // x-1 / z-1 side, [1] the x+1 / z+1 side; t1 one node away, t2 two away;
 // in* false outside the grid.  Arithmetic identical to ff_fouds1/ff_fouds2.
@@ -268,11 +243,11 @@ struct ff_sten {
     bool ix1[2], ix2[2], iz1[2], iz2[2];
 };
 
-__device__ __forceinline__ bool ff_al(double t, bool in, double thr) { return in && t < thr && t < FF_FIN; }
+__host__ __device__ __forceinline__ bool ff_al(double t, bool in, double thr) { return in && t < thr && t < FF_FIN; }
 
 // Alive mask of a stencil for threshold thr: bit s x-1 side / x+1 side one
 // away, 2+s two away, 4+s z one away, 6+s z two away.
-__device__ __forceinline__ unsigned ff_mask(const ff_sten &S, double thr)
+__host__ __device__ __forceinline__ unsigned ff_mask(const ff_sten &S, double thr)
 {
     unsigned m = 0;
     for (int s = 0; s < 2; s++) {
@@ -287,8 +262,10 @@ __device__ __forceinline__ unsigned ff_mask(const ff_sten &S, double thr)
 #define FF_AX2(s) ((m >> (2 + (s))) & 1u)
 #define FF_AZ1(s) ((m >> (4 + (s))) & 1u)
 #define FF_AZ2(s) ((m >> (6 + (s))) & 1u)
+#define FF_AX1_(m, s) (((m) >> (s)) & 1u)
+#define FF_AZ1_(m, s) (((m) >> (4 + (s))) & 1u)
 
-__device__ double ff_fouds1s(const ff_sten &S, unsigned m, double dnx, double dnz, double slown)
+__host__ __device__ double ff_fouds1s(const ff_sten &S, unsigned m, double dnx, double dnz, double slown)
 {
     int tsw1 = 0;
     double travm = 0.0;
@@ -330,7 +307,7 @@ __device__ double ff_fouds1s(const ff_sten &S, unsigned m, double dnx, double dn
     return travm;
 }
 
-__device__ double ff_fouds2s(const ff_sten &S, unsigned m, double dnx, double dnz, double slown)
+__host__ __device__ double ff_fouds2s(const ff_sten &S, unsigned m, double dnx, double dnz, double slown)
 {
     int tsw1 = 0;
     double travm = 0.0;
@@ -441,25 +418,27 @@ __device__ double ff_fouds2s(const ff_sten &S, unsigned m, double dnx, double dn
     return travm;
 }
 
-__device__ __forceinline__ double ff_cand1s(const ff_sten &S, unsigned m, double dnx, double dnz,
-                                            double slown, int fom)
+template <int FOM>
+__host__ __device__ __forceinline__ double ff_cand1s(const ff_sten &S, unsigned m, double dnx, double dnz, double slown)
 {
     if (!(m & 0x33u)) return FF_BIG;   // no 4-neighbour alive
-    return fom == 0 ? ff_fouds1s(S, m, dnx, dnz, slown) : ff_fouds2s(S, m, dnx, dnz, slown);
+    return FOM == 0 ? ff_fouds1s(S, m, dnx, dnz, slown) : ff_fouds2s(S, m, dnx, dnz, slown);
 }
 
 // The self-consistent update: the neighbours counted are those below the
 // result.  An update over the same alive set is the same number, so the loop
-// stops as soon as the set repeats.
-__device__ double ff_cands(const ff_sten &S, double dnx, double dnz, double slown, int fom)
+// stops as soon as the set repeats.  The first-order update (FOM 0) reads the
+// four nearest neighbours only; its stencil leaves the two-away entries out.
+template <int FOM>
+__host__ __device__ double ff_cands(const ff_sten &S, double dnx, double dnz, double slown)
 {
     unsigned mp = ff_mask(S, FF_THR0);
-    double c = ff_cand1s(S, mp, dnx, dnz, slown, fom);
+    double c = ff_cand1s<FOM>(S, mp, dnx, dnz, slown);
     for (int k = 0; k < 8; k++) {
         if (c >= FF_THR0) break;
         unsigned m = ff_mask(S, c);
         if (m == mp) break;                 // the update would return c again
-        double c2 = ff_cand1s(S, m, dnx, dnz, slown, fom);
+        double c2 = ff_cand1s<FOM>(S, m, dnx, dnz, slown);
         if (c2 == c || c2 >= FF_THR0) break;
         c = c2;
         mp = m;
@@ -467,12 +446,89 @@ __device__ double ff_cands(const ff_sten &S, double dnx, double dnz, double slow
     return c;
 }
 
+// The first-order self-consistent update (ff_cands<0>) with each square root
+// and quadratic solve done once per node.  fouds1's candidates for a mask are
+// the two-neighbour solution of a quadrant whose x and z neighbours are both
+// alive, else the one-neighbour time of its alive neighbour; the masks of the
+// later iterations are subsets of the first one (thresholds below FF_THR0), so
+// the two-neighbour solutions of the first mask cover them all.  Every value is
+// computed by the same operations as in ff_fouds1s, so the result is the same
+// number.
+struct ff_k0 {
+    double uu, vv, a4, a2;    // dnx^2, dnz^2, 4a, 2a with a = dnx^2 + dnz^2
+    double dnz;
+};
+__host__ __device__ __forceinline__ ff_k0 ff_k0_make(double dnx, double dnz)
+{
+    ff_k0 K;
+    K.uu = dnx * dnx;
+    K.vv = dnz * dnz;
+    const double a = K.uu + K.vv;
+    K.a4 = 4.0 * a;
+    K.a2 = 2.0 * a;
+    K.dnz = dnz;
+    return K;
+}
+
+__host__ __device__ __forceinline__ double ff_q2(double tj, double tk, const ff_k0 &K, double ss)
+{
+    const double em = tk - tj;
+    const double b = -(2.0 * K.uu * em);
+    const double c = K.uu * (em * em - K.vv * ss);
+    double rd1 = b * b - K.a4 * c;
+    if (rd1 < 0.0) rd1 = 0.0;
+    return tj + (-b + sqrt(rd1)) / K.a2;
+}
+
+__host__ __device__ double ff_cands0f(const ff_sten &S, const ff_k0 &K, double slown)
+{
+    unsigned mp = ff_mask(S, FF_THR0);
+    if (!(mp & 0x33u)) return FF_BIG;
+    const double ss = slown * slown;
+    // one alive neighbour: a = 1, b = 0, so tdsh = sqrt(-4c) / 2
+    const double hx = sqrt(4.0 * (ss * K.uu)) * 0.5;
+    const double sdz = slown * K.dnz;
+    const double hz = sqrt(4.0 * (sdz * sdz)) * 0.5;
+    double q[2][2];
+    for (int sj = 0; sj < 2; sj++)
+        for (int sk = 0; sk < 2; sk++)
+            q[sj][sk] = (S.ix1[sj] && S.iz1[sk] && FF_AX1_(mp, sj) && FF_AZ1_(mp, sk))
+                            ? ff_q2(S.tx1[sj], S.tz1[sk], K, ss) : 0.0;
+    auto sel = [&](unsigned m) -> double {
+        int any = 0;
+        double travm = 0.0;
+        for (int sj = 0; sj < 2; sj++)
+            for (int sk = 0; sk < 2; sk++) {
+                if (!(S.ix1[sj] && S.iz1[sk])) continue;
+                double trav;
+                if (FF_AX1_(m, sj)) trav = FF_AZ1_(m, sk) ? q[sj][sk] : S.tx1[sj] + hx;
+                else if (FF_AZ1_(m, sk)) trav = S.tz1[sk] + hz;
+                else continue;
+                travm = any ? fmin(trav, travm) : trav;
+                any = 1;
+            }
+        return travm;
+    };
+    double c = sel(mp);
+    for (int k = 0; k < 8; k++) {
+        if (c >= FF_THR0) break;
+        unsigned m = ff_mask(S, c);
+        if (m == mp) break;
+        double c2 = (m & 0x33u) ? sel(m) : FF_BIG;
+        if (c2 == c || c2 >= FF_THR0) break;
+        c = c2;
+        mp = m;
+    }
+    return c;
+}
+
+template <int FOM>
 __device__ __forceinline__ void ff_load_sten(const fview &V, int iz, int ix, ff_sten &S)
 {
     S.ix1[0] = ix - 1 >= 1; S.ix1[1] = ix + 1 <= V.nnx;
-    S.ix2[0] = ix - 2 >= 1; S.ix2[1] = ix + 2 <= V.nnx;
+    S.ix2[0] = FOM != 0 && ix - 2 >= 1; S.ix2[1] = FOM != 0 && ix + 2 <= V.nnx;
     S.iz1[0] = iz - 1 >= 1; S.iz1[1] = iz + 1 <= V.nnz;
-    S.iz2[0] = iz - 2 >= 1; S.iz2[1] = iz + 2 <= V.nnz;
+    S.iz2[0] = FOM != 0 && iz - 2 >= 1; S.iz2[1] = FOM != 0 && iz + 2 <= V.nnz;
     S.tx1[0] = S.ix1[0] ? V.t(iz, ix - 1) : FF_BIG;
     S.tx1[1] = S.ix1[1] ? V.t(iz, ix + 1) : FF_BIG;
     S.tx2[0] = S.ix2[0] ? V.t(iz, ix - 2) : FF_BIG;
@@ -485,36 +541,36 @@ __device__ __forceinline__ void ff_load_sten(const fview &V, int iz, int ix, ff_
 
 // Gauss-Seidel rounds over anti-diagonals in four directions until a round
 // changes no time by more than 1e-14 relative.  fixed(z,x) nodes are kept.
-// With the LDS work lists (dirty bitmap over the grid, z fastest, and dirty
-// counts per anti-diagonal of both families) only nodes whose stencil changed
-// since their last update are updated, and diagonals without such nodes are
-// skipped whole.  A node that changes marks the nodes whose stencil holds it
-// (up to two along each axis); at the start the open neighbours of fixed nodes
-// are marked.  This updates exactly the nodes a full sweep would change, in
-// the same order.  Returns the rounds used, or -1 without convergence.
+// With the LDS work lists (dirty bitmap over the grid, z fastest, and a flag
+// per anti-diagonal of both families, set when one of its nodes is marked)
+// only nodes whose stencil changed since their last update are updated, and
+// diagonals without marks since their last visit are skipped whole.  A node
+// that changes marks the nodes whose stencil holds it (up to two along each
+// axis); at the start the open neighbours of fixed nodes are marked.  An update
+// of a node whose stencil has not changed returns its time again, so skipping
+// it changes nothing: the times are those of full sweeps in the same order.
+// Returns the rounds used, or -1 without convergence.
 struct ff_work {
     uint32_t *bits;   // nnx*nnz bits
-    int *cs, *cd;     // (nnx+nnz-1) counts: sum diagonals (x-1)+(z-1), difference diagonals (nnx-x)+(z-1)
+    int *cs, *cd;     // (nnx+nnz-1) flags: sum diagonals (x-1)+(z-1), difference diagonals (nnx-x)+(z-1)
 };
 
-template <class FX, class VL>
-__device__ int ff_sweep(const fview &V, const FX &fixed, const VL &vel, int fom, int *sh_flag,
+template <int FOM, bool FAST, class FX, class VL>
+__device__ int ff_sweep(const fview &V, const FX &fixed, const VL &vel, int *sh_flag,
                         const ff_work *W, int *stat = nullptr)
 {
+    const ff_k0 K0 = ff_k0_make(V.dnx, V.dnz);
     int n_steps = 0, n_evals = 0;
     const int nt = blockDim.x, tid = threadIdx.x;
-    const int reach = fom == 0 ? 1 : 2;
+    const int reach = FOM == 0 ? 1 : 2;
     const long nn = (long)V.nnx * V.nnz;
     const int ND = V.nnx + V.nnz - 1;
     auto bit = [&](int iz, int ix) -> long { return (long)(ix - 1) * V.nnz + (iz - 1); };
     auto mark = [&](int iz, int ix) {
         long q = bit(iz, ix);
-        uint32_t m = 1u << (q & 31);
-        uint32_t old = atomicOr(&W->bits[q >> 5], m);
-        if (!(old & m)) {
-            atomicAdd(&W->cs[(ix - 1) + (iz - 1)], 1);
-            atomicAdd(&W->cd[(V.nnx - ix) + (iz - 1)], 1);
-        }
+        atomicOr(&W->bits[q >> 5], 1u << (q & 31));
+        W->cs[(ix - 1) + (iz - 1)] = 1;
+        W->cd[(V.nnx - ix) + (iz - 1)] = 1;
     };
     if (W) {
         for (long w = tid; w < (nn + 31) / 32; w += nt) W->bits[w] = 0u;
@@ -534,12 +590,14 @@ __device__ int ff_sweep(const fview &V, const FX &fixed, const VL &vel, int fom,
         for (int dir = 0; dir < 4; dir++) {
             for (int d = 0; d < ND; d++) {
                 if (W) {
-                    int cnt = dir == 0 ? W->cs[d] : dir == 3 ? W->cs[ND - 1 - d]
-                            : dir == 1 ? W->cd[d] : W->cd[ND - 1 - d];
-                    if (cnt == 0) continue;     // nothing to do on this diagonal
-                    // every wave reads the count before any wave of this step changes
-                    // counts (a step marks nodes on earlier diagonals too)
+                    int *fl = dir == 0 ? &W->cs[d] : dir == 3 ? &W->cs[ND - 1 - d]
+                            : dir == 1 ? &W->cd[d] : &W->cd[ND - 1 - d];
+                    const int f = *fl;
+                    // every wave reads the flag before it is cleared (marks of this
+                    // step fall on the neighbouring diagonals, never on this one)
                     if (nt > 64) __syncthreads();
+                    if (f == 0) continue;       // nothing to do on this diagonal
+                    if (tid == 0) *fl = 0;
                 }
                 n_steps++;
                 int a_lo = d - (V.nnz - 1) > 0 ? d - (V.nnz - 1) : 0;
@@ -552,18 +610,15 @@ __device__ int ff_sweep(const fview &V, const FX &fixed, const VL &vel, int fom,
                         long q = bit(z, x);
                         uint32_t m = 1u << (q & 31);
                         if (!(W->bits[q >> 5] & m)) continue;
-                        uint32_t old = atomicAnd(&W->bits[q >> 5], ~m);
-                        if (!(old & m)) continue;
-                        atomicSub(&W->cs[(x - 1) + (z - 1)], 1);
-                        atomicSub(&W->cd[(V.nnx - x) + (z - 1)], 1);
+                        atomicAnd(&W->bits[q >> 5], ~m);
                     }
                     if (fixed(z, x)) continue;
                     n_evals++;
                     ff_sten S;
-                    ff_load_sten(V, z, x, S);
+                    ff_load_sten<FOM>(V, z, x, S);
                     const double t = V.t(z, x);
                     double slown = 1.0 / vel(z, x);
-                    double c = ff_cands(S, V.dnx, V.dnz, slown, fom);
+                    double c = (FAST && FOM == 0) ? ff_cands0f(S, K0, slown) : ff_cands<FOM>(S, V.dnx, V.dnz, slown);
                     if (c < FF_THR0 && c != t) {
                         if (fabs(c - t) > 1e-14 * c) changed = 1;
                         V.t(z, x) = c;
@@ -875,6 +930,62 @@ __device__ void ff_wave_min(double &t, int &idx)
 
 } // namespace
 
+// srtimes (fm2dray_cartesian.f90:682-797): receiver times from the converged
+// coarse field of one problem, one thread per receiver.
+template <class VL>
+__device__ void ff_srtimes(const mct_fm2d_args &A, const near_geom &g, const fview &V, const VL &vel)
+{
+    const double dnx = V.dnx, dnz = V.dnz;
+    const double x = A.scx[g.s], z = A.scz[g.s];
+    const int32_t *srs = A.srs + (long)g.p * A.srs_pstride + (long)g.s * A.nrc;
+    double *out = A.ttime + g.prob * A.nrc;
+    for (int r = threadIdx.x + 1; r <= A.nrc; r += blockDim.x) {
+        if (srs[r - 1] == 0) continue;
+        const double rx = A.rcx[r - 1], rz = A.rcz[r - 1];
+        int irx = (int)floor((rx - A.gox) / dnx) + 1;
+        int irz = (int)floor((rz - A.goz) / dnz) + 1;
+        if (irx < 1 || irx > A.nnx || irz < 1 || irz > A.nnz) { atomicMin(&A.status[g.c], (int)MCT_ERR_OUTSIDE); continue; }
+        if (irx == A.nnx) irx = irx - 1;
+        if (irz == A.nnz) irz = irz - 1;
+        int jsx = (int)floor((x - A.gox) / dnx) + 1, jsz = (int)floor((z - A.goz) / dnz) + 1;
+        double dpl = dnx, rd1 = dnz;
+        if (rd1 < dpl) dpl = rd1;
+        double sred = (x - rx) * (x - rx);
+        sred = sred + (z - rz) * (z - rz);
+        sred = sqrt(sred);
+        int sw = 0;
+        if (sred < dpl) sw = 1;
+        if (jsx == irx && jsz == irz) sw = 1;
+        double trr;
+        if (sw == 1) {
+            double vss[2][2];
+            int csx = jsx < A.nnx ? jsx : A.nnx - 1, csz = jsz < A.nnz ? jsz : A.nnz - 1;
+            csx = csx < 1 ? 1 : csx;
+            csz = csz < 1 ? 1 : csz;
+            for (int k = 1; k <= 2; k++)
+                for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = vel(csz - 1 + l, csx - 1 + k);
+            double drx = (x - A.gox) - (jsx - 1) * dnx, drz = (z - A.goz) - (jsz - 1) * dnz;
+            double vels = ff_bilinear(vss, drx, drz, dnx, dnz);
+            for (int k = 1; k <= 2; k++)
+                for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = vel(irz - 1 + l, irx - 1 + k);
+            drx = (rx - A.gox) - (irx - 1) * dnx;
+            drz = (rz - A.goz) - (irz - 1) * dnz;
+            double velr = ff_bilinear(vss, drx, drz, dnx, dnz);
+            trr = 2.0 * sred / (vels + velr);
+        } else {
+            double drx = (rx - A.gox) - (irx - 1) * dnx;
+            double drz = (rz - A.goz) - (irz - 1) * dnz;
+            trr = 0.0;
+            for (int k = 1; k <= 2; k++)
+                for (int l = 1; l <= 2; l++) {
+                    double produ = (1.0 - fabs(((l - 1) * dnz - drz) / dnz)) * (1.0 - fabs(((k - 1) * dnx - drx) / dnx));
+                    trr = trr + V.t(irz - 1 + l, irx - 1 + k) * produ;
+                }
+        }
+        out[r - 1] = trr;
+    }
+}
+
 // One wave per problem.  LDS: region (doubles + int16 states) sized for the
 // larger of the refined grid and the coarse neighbourhood, the heap, and the
 // remap staging of the refined window's coarse nodes.
@@ -1036,7 +1147,8 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
             __syncthreads();
             auto rfixed = [&](int iz, int ix) -> bool { return R.n(iz, ix) == 0; };
             RW.cs = DC; RW.cd = DC + (R.nnx + R.nnz - 1);
-            int rounds = ff_sweep(R, rfixed, rvel, A.fom, &sh_flag, &RW);
+            int rounds = A.fom == 0 ? ff_sweep<0, true>(R, rfixed, rvel, &sh_flag, &RW)
+                                    : ff_sweep<1, true>(R, rfixed, rvel, &sh_flag, &RW);
             if (rounds < 0) {
                 if (lane == 0) pflag[slot] = 1;
                 return;
@@ -1174,6 +1286,7 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
 }
 
 // One workgroup per problem: sweep the coarse grid, then pick the receivers.
+template <int FOM>
 __global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, const int32_t *pflag, int use_dirty)
 {
     extern __shared__ uint32_t ff_dirty[];
@@ -1222,61 +1335,257 @@ __global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, const int32
     int *stat = A.rounds_out ? A.rounds_out + 4 * slot : nullptr;
     if (stat && threadIdx.x == 0) { stat[1] = 0; stat[2] = 0; }
     __syncthreads();
-    int rounds = ff_sweep(V, fixed, vel, A.fom, &sh_flag, use_dirty ? &W : nullptr, stat);
+    int rounds = ff_sweep<FOM, false>(V, fixed, vel, &sh_flag, use_dirty ? &W : nullptr, stat);
     if (rounds < 0) {
         if (threadIdx.x == 0) atomicMin(&A.status[g.c], (int)MCT_ERR_NOCONV);
         return;
     }
     if (threadIdx.x == 0 && stat) stat[0] = rounds;
-    // srtimes
-    const double x = A.scx[g.s], z = A.scz[g.s];
-    const int32_t *srs = A.srs + (long)g.p * A.srs_pstride + (long)g.s * A.nrc;
-    double *out = A.ttime + g.prob * A.nrc;
-    for (int r = threadIdx.x + 1; r <= A.nrc; r += blockDim.x) {
-        if (srs[r - 1] == 0) continue;
-        const double rx = A.rcx[r - 1], rz = A.rcz[r - 1];
-        int irx = (int)floor((rx - A.gox) / dnx) + 1;
-        int irz = (int)floor((rz - A.goz) / dnz) + 1;
-        if (irx < 1 || irx > A.nnx || irz < 1 || irz > A.nnz) { atomicMin(&A.status[g.c], (int)MCT_ERR_OUTSIDE); continue; }
-        if (irx == A.nnx) irx = irx - 1;
-        if (irz == A.nnz) irz = irz - 1;
-        int jsx = (int)floor((x - A.gox) / dnx) + 1, jsz = (int)floor((z - A.goz) / dnz) + 1;
-        double dpl = dnx, rd1 = dnz;
-        if (rd1 < dpl) dpl = rd1;
-        double sred = (x - rx) * (x - rx);
-        sred = sred + (z - rz) * (z - rz);
-        sred = sqrt(sred);
-        int sw = 0;
-        if (sred < dpl) sw = 1;
-        if (jsx == irx && jsz == irz) sw = 1;
-        double trr;
-        if (sw == 1) {
-            double vss[2][2];
-            int csx = jsx < A.nnx ? jsx : A.nnx - 1, csz = jsz < A.nnz ? jsz : A.nnz - 1;
-            csx = csx < 1 ? 1 : csx;
-            csz = csz < 1 ? 1 : csz;
-            for (int k = 1; k <= 2; k++)
-                for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = vel(csz - 1 + l, csx - 1 + k);
-            double drx = (x - A.gox) - (jsx - 1) * dnx, drz = (z - A.goz) - (jsz - 1) * dnz;
-            double vels = ff_bilinear(vss, drx, drz, dnx, dnz);
-            for (int k = 1; k <= 2; k++)
-                for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = vel(irz - 1 + l, irx - 1 + k);
-            drx = (rx - A.gox) - (irx - 1) * dnx;
-            drz = (rz - A.goz) - (irz - 1) * dnz;
-            double velr = ff_bilinear(vss, drx, drz, dnx, dnz);
-            trr = 2.0 * sred / (vels + velr);
-        } else {
-            double drx = (rx - A.gox) - (irx - 1) * dnx;
-            double drz = (rz - A.goz) - (irz - 1) * dnz;
-            trr = 0.0;
-            for (int k = 1; k <= 2; k++)
-                for (int l = 1; l <= 2; l++) {
-                    double produ = (1.0 - fabs(((l - 1) * dnz - drz) / dnz)) * (1.0 - fabs(((k - 1) * dnx - drx) / dnx));
-                    trr = trr + V.t(irz - 1 + l, irx - 1 + k) * produ;
-                }
+    ff_srtimes(A, g, V, vel);
+}
+
+// ---- strip-pipelined sweep -------------------------------------------------------
+// The Gauss-Seidel sweeps of ff_sweep in another visiting order that reads the
+// same values.  A sweep crosses the grid in strips of 64 columns along its x
+// direction; in a strip lane i owns column x_i and at step t updates the node
+// b = t - i along the sweep's z direction.  Node (x_i, b) so comes after
+// (x_i, b-1) and (x_i, b-2) (same lane, earlier steps) and after (x_{i-1}, b)
+// and (x_{i-2}, b) (lanes i-1, i-2 one and two steps earlier, or an earlier
+// strip), and before (x_i, b+1), (x_i, b+2), (x_{i+1}, b), (x_{i+2}, b): the
+// same relations to its stencil as in the anti-diagonal order, so every update
+// reads the values it reads there, and the dirty marks fall as they do there.
+// Each lane keeps a window of its own column in registers (new times behind b,
+// old times ahead), takes its x neighbours' times from the neighbouring lanes,
+// and streams its column's old times and velocities from memory FF_PF to
+// 2*FF_PF steps ahead; the strip's outer neighbour columns are staged in LDS.
+// A strip without dirty nodes is skipped.
+#define FF_PF 8
+
+template <int FOM>
+__global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, const int32_t *pflag)
+{
+    extern __shared__ double ff_sld[];
+    const long slot = ff_xcd_slot(blockIdx.x, gridDim.x);
+    if (pflag[slot] != 0) return;
+    near_geom g;
+    if (!ff_problem(A, slot, g, false)) return;
+    const int lane = threadIdx.x;
+    const int nnx = A.nnx, nnz = A.nnz, ld = A.ld;
+    const long wsz = (long)ld * A.nc;
+    const double dnx = A.dvx / A.gdx, dnz = A.dvz / A.gdz;
+    const long cp = g.prob / A.nsrc;
+    const double *velnc = A.veln + cp * (long)nnx * nnz;
+    double *T = A.ttn + slot * wsz;
+    const int32_t *F = A.nsts + slot * wsz;
+    const long nn = (long)nnx * nnz;
+    const int nw = (int)((nn + 31) / 32);
+    const int ns = (nnx + 63) / 64;
+    constexpr int NSIDE = FOM == 0 ? 2 : 4;
+    constexpr int reach = FOM == 0 ? 1 : 2;
+    const ff_k0 K0 = ff_k0_make(dnx, dnz);
+    uint32_t *DB = (uint32_t *)ff_sld;        // dirty bits, z fastest
+    uint32_t *FB = DB + nw;                   // fixed bits
+    int *cnt = (int *)(FB + nw);              // per strip of 64 columns: marked since its last visit
+    double *side = (double *)(cnt + ((ns + 1) & ~1));   // staged columns L1, R1, L2, R2 (nnz each)
+
+    for (int w = lane; w < nw; w += 64) {
+        uint32_t m = 0;
+        for (int b = 0; b < 32; b++) {
+            long q = (long)w * 32 + b;
+            if (q < nn) {
+                int x = (int)(q / nnz), z = (int)(q % nnz);
+                if (F[(long)x * ld + z] != 0) m |= 1u << b;
+            }
         }
-        out[r - 1] = trr;
+        FB[w] = m;
+        DB[w] = 0u;
     }
+    for (int k = lane; k < ns; k += 64) cnt[k] = 0;
+    __syncthreads();
+    auto fixed = [&](int z, int x) -> bool {       // 1-based
+        long q = (long)(x - 1) * nnz + (z - 1);
+        return (FB[q >> 5] >> (q & 31)) & 1u;
+    };
+    auto mark = [&](int z, int x) {
+        long q = (long)(x - 1) * nnz + (z - 1);
+        atomicOr(&DB[q >> 5], 1u << (q & 31));
+        cnt[(x - 1) >> 6] = 1;
+    };
+    for (long q = lane; q < nn; q += 64) {
+        int x = (int)(q / nnz) + 1, z = (int)(q % nnz) + 1;
+        if (fixed(z, x)) continue;
+        bool nb = (x > 1 && fixed(z, x - 1)) || (x < nnx && fixed(z, x + 1)) ||
+                  (z > 1 && fixed(z - 1, x)) || (z < nnz && fixed(z + 1, x));
+        if (nb) mark(z, x);
+    }
+    __syncthreads();
+
+    int n_steps = 0, n_evals = 0, rounds = -1;
+    for (int round = 0; round < FF_MAX_ROUNDS && rounds < 0; round++) {
+        int changed = 0;
+        for (int dir = 0; dir < 4; dir++) {
+            const int xs = (dir & 1) ? -1 : 1, zs = (dir & 2) ? -1 : 1;
+            for (int si = 0; si < ns; si++) {
+                const int s = xs > 0 ? si : ns - 1 - si;
+                __syncthreads();
+                if (cnt[s] == 0) continue;
+                cnt[s] = 0;     // marks of this strip's steps set it again
+                const int xl0 = xs > 0 ? 64 * s + 1 : 64 * s + 64;      // lane 0's column
+                const int xl63 = xs > 0 ? xl0 + 63 : xl0 - 63;
+                const int x = xs > 0 ? xl0 + lane : xl0 - lane;
+                const bool xin = x >= 1 && x <= nnx;
+                {
+                    const int cx[4] = {xl0 - xs, xl63 + xs, xl0 - 2 * xs, xl63 + 2 * xs};
+                    for (int c = 0; c < NSIDE; c++) {
+                        const bool ok = cx[c] >= 1 && cx[c] <= nnx;
+                        for (int k = lane; k < nnz; k += 64)
+                            side[c * nnz + k] = ok ? T[(long)(cx[c] - 1) * ld + k] : FF_BIG;
+                    }
+                }
+                __syncthreads();
+                const int first = xs > 0 ? 0 : (xl0 > nnx ? xl0 - nnx : 0);
+                const int last = xs > 0 ? (nnx - xl0 < 63 ? nnx - xl0 : 63) : 63;
+                const int tend = nnz - 1 + last;
+                const long colT = (long)(x - 1) * ld, colV = (long)(x - 1) * nnz;
+                auto told = [&](int bb) -> double {
+                    if (!xin || bb < 0 || bb >= nnz) return FF_BIG;
+                    return T[colT + (zs > 0 ? bb : nnz - 1 - bb)];
+                };
+                auto vload = [&](int bb) -> double {
+                    if (!xin || bb < 0 || bb >= nnz) return 1.0;
+                    return velnc[colV + (zs > 0 ? bb : nnz - 1 - bb)];
+                };
+                constexpr int D = 2 + FOM;      // the stream delivers the old time D nodes ahead
+                const int b0 = first - lane;
+                double pn1 = FF_BIG, pn2 = FF_BIG;
+                double c0 = told(b0), o1 = told(b0 + 1), o2 = FOM ? told(b0 + 2) : 0.0;
+                double sa[FF_PF], sb[FF_PF], va[FF_PF], vb[FF_PF];
+#pragma unroll
+                for (int j = 0; j < FF_PF; j++) { sa[j] = told(b0 + j + D); va[j] = vload(b0 + j); }
+                // the window and the first chunk in registers before the step loop, so the
+                // loop's waits are for the chunk in flight only
+                __builtin_amdgcn_s_waitcnt(0);
+                auto step = [&](int t, double snext, double vcur) {
+                    const int b = t - lane;
+                    const bool act = xin && b >= 0 && b < nnz;
+                    const int z0 = zs > 0 ? b : nnz - 1 - b;              // 0-based z
+                    double xm1 = __shfl_up(pn1, 1, 64), xp1 = __shfl_down(o1, 1, 64);
+                    double xm2 = 0.0, xp2 = 0.0;
+                    if (FOM) { xm2 = __shfl_up(pn2, 2, 64); xp2 = __shfl_down(o2, 2, 64); }
+                    double newv = c0;
+                    if (act) {
+                        const long q = colV + z0;
+                        const uint32_t m = 1u << (q & 31);
+                        if (!(FB[q >> 5] & m) && (DB[q >> 5] & m)) {
+                            atomicAnd(&DB[q >> 5], ~m);
+                            n_evals++;
+                            if (lane == 0) xm1 = side[z0];
+                            if (lane == 63) xp1 = side[nnz + z0];
+                            if (FOM) {
+                                if (lane == 0) xm2 = side[2 * nnz + z0];
+                                if (lane == 1) xm2 = side[z0];
+                                if (lane == 63) xp2 = side[3 * nnz + z0];
+                                if (lane == 62) xp2 = side[nnz + z0];
+                            }
+                            // the stencil in grid orientation: [0] the x-1 / z-1 side
+                            ff_sten S;
+                            S.ix1[0] = x - 1 >= 1; S.ix1[1] = x + 1 <= nnx;
+                            S.iz1[0] = z0 >= 1; S.iz1[1] = z0 + 1 < nnz;
+                            S.ix2[0] = FOM != 0 && x - 2 >= 1; S.ix2[1] = FOM != 0 && x + 2 <= nnx;
+                            S.iz2[0] = FOM != 0 && z0 >= 2; S.iz2[1] = FOM != 0 && z0 + 2 < nnz;
+                            const double lx1 = xs > 0 ? xm1 : xp1, hx1 = xs > 0 ? xp1 : xm1;
+                            const double lz1 = zs > 0 ? pn1 : o1, hz1 = zs > 0 ? o1 : pn1;
+                            S.tx1[0] = S.ix1[0] ? lx1 : FF_BIG; S.tx1[1] = S.ix1[1] ? hx1 : FF_BIG;
+                            S.tz1[0] = S.iz1[0] ? lz1 : FF_BIG; S.tz1[1] = S.iz1[1] ? hz1 : FF_BIG;
+                            if (FOM) {
+                                const double lx2 = xs > 0 ? xm2 : xp2, hx2 = xs > 0 ? xp2 : xm2;
+                                const double lz2 = zs > 0 ? pn2 : o2, hz2 = zs > 0 ? o2 : pn2;
+                                S.tx2[0] = S.ix2[0] ? lx2 : FF_BIG; S.tx2[1] = S.ix2[1] ? hx2 : FF_BIG;
+                                S.tz2[0] = S.iz2[0] ? lz2 : FF_BIG; S.tz2[1] = S.iz2[1] ? hz2 : FF_BIG;
+                            } else {
+                                S.tx2[0] = S.tx2[1] = S.tz2[0] = S.tz2[1] = FF_BIG;
+                            }
+                            const double c = FOM == 0 ? ff_cands0f(S, K0, 1.0 / vcur)
+                                                      : ff_cands<FOM>(S, dnx, dnz, 1.0 / vcur);
+                            if (c < FF_THR0 && c != c0) {
+                                if (fabs(c - c0) > 1e-14 * c) changed = 1;
+                                newv = c;
+                                T[colT + z0] = c;
+                                const int z = z0 + 1;
+                                for (int r = 1; r <= reach; r++) {
+                                    if (x - r >= 1 && !fixed(z, x - r)) mark(z, x - r);
+                                    if (x + r <= nnx && !fixed(z, x + r)) mark(z, x + r);
+                                    if (z - r >= 1 && !fixed(z - r, x)) mark(z - r, x);
+                                    if (z + r <= nnz && !fixed(z + r, x)) mark(z + r, x);
+                                }
+                            }
+                        }
+                    }
+                    pn2 = pn1; pn1 = newv; c0 = o1;
+                    if (FOM) { o1 = o2; o2 = snext; } else { o1 = snext; }
+                };
+                for (int t0 = first; t0 <= tend; t0 += 2 * FF_PF) {
+#pragma unroll
+                    for (int j = 0; j < FF_PF; j++) {
+                        sb[j] = told(t0 + FF_PF + j - lane + D);
+                        vb[j] = vload(t0 + FF_PF + j - lane);
+                    }
+#pragma unroll 1
+                    for (int j = 0; j < FF_PF && t0 + j <= tend; j++) step(t0 + j, sa[j], va[j]);
+#pragma unroll
+                    for (int j = 0; j < FF_PF; j++) {
+                        sa[j] = told(t0 + 2 * FF_PF + j - lane + D);
+                        va[j] = vload(t0 + 2 * FF_PF + j - lane);
+                    }
+#pragma unroll 1
+                    for (int j = 0; j < FF_PF && t0 + FF_PF + j <= tend; j++)
+                        step(t0 + FF_PF + j, sb[j], vb[j]);
+                }
+                if (lane == 0) n_steps += tend - first + 1;
+                // the flag stays set only if the strip still holds marked nodes (marks
+                // of its own steps ahead of the pipeline were taken in this visit)
+                bool left = false;
+                if (xin) {
+                    const long q0 = colV, q1 = colV + nnz;
+                    for (long w = q0 >> 5; w <= (q1 - 1) >> 5; w++) {
+                        uint32_t v = DB[w];
+                        if (w == (q0 >> 5)) v &= ~0u << (q0 & 31);
+                        if (w == ((q1 - 1) >> 5) && (q1 & 31)) v &= ~(~0u << (q1 & 31));
+                        left |= v != 0u;
+                    }
+                }
+                if (__ballot(left) == 0ull) cnt[s] = 0;
+            }
+        }
+        if (__ballot(changed) == 0ull) rounds = round + 1;
+    }
+    __syncthreads();
+    int *stat = A.rounds_out ? A.rounds_out + 4 * slot : nullptr;
+    if (stat) {
+        if (lane == 0) { stat[0] = rounds; stat[1] = n_steps; }
+        int tot = n_evals;
+        for (int o = 32; o >= 1; o >>= 1) tot += __shfl_xor(tot, o, 64);
+        if (lane == 0) stat[2] = tot;
+    }
+    if (rounds < 0) {
+        if (lane == 0) atomicMin(&A.status[g.c], (int)MCT_ERR_NOCONV);
+        return;
+    }
+    fview V;
+    V.nnx = nnx; V.nnz = nnz; V.dnx = dnx; V.dnz = dnz;
+    V.T = T; V.ns = nullptr; V.ox = 0; V.oz = 0; V.lnx = nnx; V.lnz = nnz; V.ld = ld;
+    auto vel = [&](int iz, int ix) -> double { return velnc[(long)(ix - 1) * nnz + (iz - 1)]; };
+    ff_srtimes(A, g, V, vel);
+}
+
+// LDS bytes of k_fm2d_strip, or 0 when the grid needs more than the limit.
+static size_t ff_strip_lds(const mct_fm2d_args &a)
+{
+    const long nn = (long)a.nnx * a.nnz;
+    const long nw = (nn + 31) / 32;
+    const long ns = (a.nnx + 63) / 64;
+    const size_t b = 4 * (size_t)(2 * nw + ((ns + 1) & ~1L)) + 8 * (size_t)(a.fom == 0 ? 2 : 4) * a.nnz;
+    return b <= 48 * 1024 ? b : 0;
 }
 
 // LDS bytes of k_fm2d_near for the geometry in a.
@@ -1292,8 +1601,9 @@ static size_t ff_near_lds(const mct_fm2d_args &a, int &region, int &heap_cap, in
     region = rnodes > lnx * lnz ? rnodes : lnx * lnz;
     // the heap holds a narrow band only; a problem whose band outgrows it goes
     // to the sequential solver
-    heap_cap = region < 1024 ? region : 1024;
+    heap_cap = region < 512 ? region : 512;
     win = a.asgr == 1 ? wn * wn : 1;
+    // times and states of the region, heap, remap staging, bit set, diagonal flags
     size_t b = 8 * (size_t)region + 2 * (size_t)((region + 3) & ~3) + 4 * (size_t)((heap_cap + 2 + 1) & ~1) +
                8 * (size_t)win + 2 * (size_t)((win + 1) & ~1) + 4 * (size_t)((region + 31) / 32) +
                4 * 2 * (size_t)(rgx + rgz) + 64;
@@ -1304,8 +1614,16 @@ int mct_fm2d_fast_ok(const mct_fm2d_args &a)
 {
     int region, heap_cap, win;
     size_t b = ff_near_lds(a, region, heap_cap, win);
-    // heap positions and coordinates are 16-bit
-    return b <= 64 * 1024 && region < 32767 && a.nnx < 32767 && a.nnz < 32767;
+    // heap positions and coordinates are 16-bit; LDS within the device's limit per workgroup
+    static int lds_max = 0;
+    if (!lds_max) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 64 * 1024;
+        lds_max = v;
+    }
+    return b <= (size_t)lds_max && region < 32767 && a.nnx < 32767 && a.nnz < 32767;
 }
 
 int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hipStream_t s)
@@ -1314,6 +1632,8 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
     size_t lds = ff_near_lds(a, region, heap_cap, win);
     int rc = mct_launch_fm2d_gridder(a, s);
     if (rc) return rc;
+    if (lds > 64 * 1024)
+        MCT_CHECK(hipFuncSetAttribute((const void *)k_fm2d_near, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     // the sweep's dirty bitmap in LDS when it fits
     const size_t dirty_bytes = 2 * 4 * (((size_t)a.nnx * a.nnz + 31) / 32) + 4 * 2 * (size_t)(a.nnx + a.nnz);
     const int use_dirty = dirty_bytes <= 48 * 1024;
@@ -1326,6 +1646,10 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
         int v = e ? atoi(e) : 64;
         sweep_threads = (v == 64 || v == 128 || v == 256) ? v : 64;
     }
+    // the strip-pipelined sweep unless disabled (MCT_SWEEP_KIND=diag) or too large for LDS
+    const char *sk = getenv("MCT_SWEEP_KIND");
+    const int sweep_kind = (sk && strcmp(sk, "diag") == 0) ? 0 : 1;
+    const size_t strip_lds = sweep_kind ? ff_strip_lds(a) : 0;
     const long nprob = (long)a.nchain * a.np * a.nsrc;
     if (chunk < 1) return MCT_ERR_ARG;
     for (long p0 = 0; p0 < nprob; p0 += chunk) {
@@ -1334,8 +1658,16 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
         b.nrun = nprob - p0 < chunk ? nprob - p0 : chunk;
         hipLaunchKernelGGL(k_fm2d_near, dim3((unsigned)b.nrun), dim3(64), lds, s, b, region, heap_cap, win, pflag);
         MCT_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_fm2d_sweep, dim3((unsigned)b.nrun), dim3(sweep_threads), dirty_lds, s, b,
-                           (const int32_t *)pflag, use_dirty);
+        if (strip_lds && a.fom == 0)
+            hipLaunchKernelGGL(k_fm2d_strip<0>, dim3((unsigned)b.nrun), dim3(64), strip_lds, s, b, (const int32_t *)pflag);
+        else if (strip_lds)
+            hipLaunchKernelGGL(k_fm2d_strip<1>, dim3((unsigned)b.nrun), dim3(64), strip_lds, s, b, (const int32_t *)pflag);
+        else if (a.fom == 0)
+            hipLaunchKernelGGL(k_fm2d_sweep<0>, dim3((unsigned)b.nrun), dim3(sweep_threads), dirty_lds, s, b,
+                               (const int32_t *)pflag, use_dirty);
+        else
+            hipLaunchKernelGGL(k_fm2d_sweep<1>, dim3((unsigned)b.nrun), dim3(sweep_threads), dirty_lds, s, b,
+                               (const int32_t *)pflag, use_dirty);
         MCT_CHECK(hipGetLastError());
         // problems whose heap loop left the LDS neighbourhood: the sequential solver
         b.only_flag = pflag;
