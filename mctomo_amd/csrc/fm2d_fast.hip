@@ -66,7 +66,7 @@ struct fview {
 
 // ---- the reference's local updates over an "alive" predicate ---------------------
 template <class AL>
-__device__ double ff_fouds1(const fview &V, const AL &al, int iz, int ix, double slown)
+__device__ __forceinline__ double ff_fouds1(const fview &V, const AL &al, int iz, int ix, double slown)
 {
     int tsw1 = 0;
     double travm = 0.0;
@@ -108,7 +108,7 @@ __device__ double ff_fouds1(const fview &V, const AL &al, int iz, int ix, double
 }
 
 template <class AL>
-__device__ double ff_fouds2(const fview &V, const AL &al, int iz, int ix, double slown)
+__device__ __forceinline__ double ff_fouds2(const fview &V, const AL &al, int iz, int ix, double slown)
 {
     int tsw1 = 0;
     double travm = 0.0;
@@ -556,7 +556,7 @@ struct ff_work {
 };
 
 template <int FOM, bool FAST, class FX, class VL>
-__device__ int ff_sweep(const fview &V, const FX &fixed, const VL &vel, int *sh_flag,
+__device__ __forceinline__ int ff_sweep(const fview &V, const FX &fixed, const VL &vel, int *sh_flag,
                         const ff_work *W, int *stat = nullptr)
 {
     const ff_k0 K0 = ff_k0_make(V.dnx, V.dnz);
@@ -660,7 +660,7 @@ struct ff_heap {
     int full;        // an insertion did not fit
 };
 
-__device__ void ff_addtree(const fview &V, ff_heap &H, int iz, int ix)
+__device__ __forceinline__ void ff_addtree(const fview &V, ff_heap &H, int iz, int ix)
 {
     if (H.ntr + 1 > H.cap) { H.full = 1; return; }
     H.ntr = H.ntr + 1;
@@ -683,7 +683,7 @@ __device__ void ff_addtree(const fview &V, ff_heap &H, int iz, int ix)
     }
 }
 
-__device__ void ff_downtree(const fview &V, ff_heap &H)
+__device__ __forceinline__ void ff_downtree(const fview &V, ff_heap &H)
 {
     if (H.ntr == 1) { H.ntr = H.ntr - 1; return; }
     int el = H.btg[H.ntr];
@@ -724,7 +724,7 @@ __device__ void ff_downtree(const fview &V, ff_heap &H)
     }
 }
 
-__device__ void ff_updtree(const fview &V, ff_heap &H, int iz, int ix)
+__device__ __forceinline__ void ff_updtree(const fview &V, ff_heap &H, int iz, int ix)
 {
     int tpc = V.n(iz, ix), tpp = tpc / 2;
     const double tv = V.t(iz, ix);
@@ -780,7 +780,7 @@ __device__ __forceinline__ bool ff_gen_get(const fview &V, const uint32_t *b, in
 // The reference's heap loop over the view (travel's :137-237) with fouds and
 // velocities vel(z,x); stops by rule S.  Runs in one lane.
 template <class VL>
-__device__ void ff_heap_loop(const fview &V, ff_heap &H, const VL &vel, int fom, ff_stop &S)
+__device__ __forceinline__ void ff_heap_loop(const fview &V, ff_heap &H, const VL &vel, int fom, ff_stop &S)
 {
     auto al = [&](int z, int x) -> bool { return V.n(z, x) == 0; };
     while (H.ntr > 0) {
@@ -845,7 +845,7 @@ __device__ void ff_heap_loop(const fview &V, ff_heap &H, const VL &vel, int fom,
 
 // travel's corner initialisation (fm2d_ttime.f90:105-126) and heap start.
 template <class VL>
-__device__ void ff_corners(const fview &V, ff_heap &H, const VL &vel, double scx, double scz,
+__device__ __forceinline__ void ff_corners(const fview &V, ff_heap &H, const VL &vel, double scx, double scz,
                            double gox, double goz, int isx, int isz)
 {
     double vss[2][2];
@@ -888,7 +888,7 @@ struct near_geom {
 };
 
 // Problem bookkeeping shared by both kernels; returns false to skip.
-__device__ bool ff_problem(const mct_fm2d_args &A, long slot, near_geom &g, bool report)
+__device__ __forceinline__ bool ff_problem(const mct_fm2d_args &A, long slot, near_geom &g, bool report)
 {
     g.slot = slot;
     g.prob = A.prob0 + slot;
@@ -919,7 +919,7 @@ __device__ bool ff_problem(const mct_fm2d_args &A, long slot, near_geom &g, bool
 }
 
 // wave-wide lexicographic minimum of (t, idx)
-__device__ void ff_wave_min(double &t, int &idx)
+__device__ __forceinline__ void ff_wave_min(double &t, int &idx)
 {
     for (int o = 32; o >= 1; o >>= 1) {
         double t2 = __shfl_xor(t, o, 64);
@@ -933,7 +933,7 @@ __device__ void ff_wave_min(double &t, int &idx)
 // srtimes (fm2dray_cartesian.f90:682-797): receiver times from the converged
 // coarse field of one problem, one thread per receiver.
 template <class VL>
-__device__ void ff_srtimes(const mct_fm2d_args &A, const near_geom &g, const fview &V, const VL &vel)
+__device__ __forceinline__ void ff_srtimes(const mct_fm2d_args &A, const near_geom &g, const fview &V, const VL &vel)
 {
     const double dnx = V.dnx, dnz = V.dnz;
     const double x = A.scx[g.s], z = A.scz[g.s];
