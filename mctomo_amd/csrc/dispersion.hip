@@ -717,6 +717,82 @@ enum {
     SM_NDONE, SM_GDONE, SM_FINISH, SM_DONE
 };
 
+// Column order for the queue: by layer count, most layers first, so the lanes
+// of a wave hold columns whose period equations loop over as many layers
+// (a counting sort; columns of one count keep no particular order, which
+// changes nothing since every column is computed on its own).
+#define DSP_SORT_IPT 4
+__device__ __forceinline__ int dsp_layer_bin(const mct_dsp_args &A, long idx)
+{
+    const double EPS = (double)1.0e-10f;       // src/likelihood_surf.F90:37
+    const double *vs = A.vs + idx * A.nz;
+    int nl = 0;
+    double last = vs[0];
+    for (int k = 1; k < A.nz; k++) {
+        const double v = vs[k];
+        if (fabs(v - last) > EPS) { nl++; last = v; }
+    }
+    return A.nz - 1 - nl;     // mmax = nl + 1 in [1, nz]: bin 0 holds the most layers
+}
+
+__global__ __launch_bounds__(256) void k_dsp_sort_count(mct_dsp_args A, long total)
+{
+    __shared__ int h[128];
+    const int nb = A.nz;
+    for (int b = threadIdx.x; b < nb; b += 256) h[b] = 0;
+    __syncthreads();
+    const long base = (long)blockIdx.x * 256 * DSP_SORT_IPT;
+#pragma unroll
+    for (int i = 0; i < DSP_SORT_IPT; i++) {
+        const long idx = base + i * 256 + threadIdx.x;
+        if (idx < total) {
+            const int b = dsp_layer_bin(A, idx);
+            A.keys[idx] = b;
+            atomicAdd(&h[b], 1);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += 256)
+        if (h[b]) atomicAdd(&A.hist[b], h[b]);
+}
+
+// exclusive prefix sum of the bin counts: the bins' first positions
+__global__ void k_dsp_sort_scan(int32_t *hist, int nb, unsigned long long *queue)
+{
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int b = 0; b < nb; b++) { const int c = hist[b]; hist[b] = acc; acc += c; }
+        queue[0] = 0ull;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dsp_sort_place(mct_dsp_args A, long total)
+{
+    __shared__ int h[128], base[128];
+    const int nb = A.nz;
+    for (int b = threadIdx.x; b < nb; b += 256) h[b] = 0;
+    __syncthreads();
+    const long b0 = (long)blockIdx.x * 256 * DSP_SORT_IPT;
+    int bin[DSP_SORT_IPT], rank[DSP_SORT_IPT];
+#pragma unroll
+    for (int i = 0; i < DSP_SORT_IPT; i++) {
+        const long idx = b0 + i * 256 + threadIdx.x;
+        bin[i] = idx < total ? A.keys[idx] : -1;
+        rank[i] = bin[i] >= 0 ? atomicAdd(&h[bin[i]], 1) : 0;
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += 256) base[b] = h[b] ? atomicAdd(&A.hist[b], h[b]) : 0;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < DSP_SORT_IPT; i++)
+        if (bin[i] >= 0) A.jobs[base[bin[i]] + rank[i]] = (int32_t)(b0 + i * 256 + threadIdx.x);
+}
+
+__global__ void k_dsp_sort_reset(int32_t *hist, int nb)
+{
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) hist[b] = 0;
+}
+
 __global__ void k_dsp_queue_reset(unsigned long long *q)
 {
     if (threadIdx.x == 0) q[0] = 0ull;
@@ -770,6 +846,7 @@ __global__ __launch_bounds__(64) void k_dispersion_sm(mct_dsp_args A, float *lay
             case SM_NEXT: {
                 job = (long)atomicAdd((unsigned long long *)A.queue, 1ull);
                 if (job >= total) { pc = SM_DONE; break; }
+                if (A.jobs) job = A.jobs[job];
                 c = (int)(job / A.ncol);
                 col = job - (long)c * A.ncol;
                 const double *vs = A.vs + ((long)c * A.ncol + col) * A.nz;
@@ -1013,12 +1090,24 @@ int mct_launch_dispersion(const mct_dsp_args &a, hipStream_t s)
     if (a.np < 1 || a.np > DSP_MAXP || a.nz < 1 || a.nz > DSP_NMAX) return MCT_ERR_ARG;
     MCT_CHECK(hipMemsetAsync(a.bad_model, 0, sizeof(int32_t) * a.nchain, s));
     if (a.lay && a.queue) {
-        // persistent waves, columns from a queue
-        hipLaunchKernelGGL(k_dsp_queue_reset, dim3(1), dim3(64), 0, s, a.queue);
+        // persistent waves, columns from a queue (ordered by layer count when
+        // the order's buffers are given)
+        mct_dsp_args b = a;
         const long total = (long)a.ncol * a.nchain;
+        if (b.jobs && b.keys && b.hist && b.nz <= 128 && total < 0x7fffffffL) {
+            const unsigned nblk = (unsigned)((total + 256 * DSP_SORT_IPT - 1) / (256 * DSP_SORT_IPT));
+            hipLaunchKernelGGL(k_dsp_sort_reset, dim3(1), dim3(128), 0, s, b.hist, b.nz);
+            hipLaunchKernelGGL(k_dsp_sort_count, dim3(nblk), dim3(256), 0, s, b, total);
+            hipLaunchKernelGGL(k_dsp_sort_scan, dim3(1), dim3(64), 0, s, b.hist, b.nz, b.queue);
+            hipLaunchKernelGGL(k_dsp_sort_place, dim3(nblk), dim3(256), 0, s, b, total);
+            MCT_CHECK(hipGetLastError());
+        } else {
+            b.jobs = nullptr;
+            hipLaunchKernelGGL(k_dsp_queue_reset, dim3(1), dim3(64), 0, s, b.queue);
+        }
         long nw = (total + 63) / 64;
-        if (nw > a.lay_waves) nw = a.lay_waves;
-        hipLaunchKernelGGL(k_dispersion_sm, dim3((unsigned)nw), dim3(64), 0, s, a, a.lay, a.nz, total);
+        if (nw > b.lay_waves) nw = b.lay_waves;
+        hipLaunchKernelGGL(k_dispersion_sm, dim3((unsigned)nw), dim3(64), 0, s, b, b.lay, b.nz, total);
     } else if (a.lay) {
         const unsigned nb = (unsigned)((a.ncol + 63) / 64);
         hipLaunchKernelGGL(k_dispersion_lane, dim3(nb, a.nchain), dim3(64), 0, s, a, a.lay, a.nz);

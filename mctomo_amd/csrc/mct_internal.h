@@ -213,6 +213,8 @@ struct mct_dsp_args {
                                   // for one wave per column
     unsigned long long *queue;    // column queue counter: with lay, persistent waves
     long lay_waves;               // waves the scratch holds (queue mode)
+    int32_t *jobs, *keys;         // queue mode: (nchain*ncol) columns ordered by layer count, or NULL
+    int32_t *hist;                // (nz + 1) bins for that order
 };
 // One wave per (column, chain): check_model, convert_to_layer, surfdisp96.
 // Requires np <= 60 and nz <= 100.

@@ -81,6 +81,7 @@ struct mct_surf_ctx {
     long fm2d_chunk;
     int fm2d_method;                 // 0 parallel sweeps, 1 sequential heap loop
     long dsp_waves;                  // queue mode: at most this many waves (0: one per 64 columns)
+    int dsp_unsorted;                // queue mode: columns in grid order instead of by layer count
     int dsp_mode;                    // 0 by size, 1 lane per column, 2 wave per column,
                                      // 3 lane per column from a column queue
     hipStream_t stream = nullptr;
@@ -92,7 +93,7 @@ struct mct_surf_ctx {
     sbuf points, params, noise, out, status;
     // stage buffers
     sbuf vs, flag_count, flag_nodes, kd_nodes, kd_ind, kd_rdata;
-    sbuf phase, group, ierr, bad_model, ndltar, nlayer, lay, dsp_queue, velv, veln, phase_time, srdist, sigma;
+    sbuf phase, group, ierr, bad_model, ndltar, nlayer, lay, dsp_queue, dsp_jobs, dsp_keys, dsp_hist, velv, veln, phase_time, srdist, sigma;
     sbuf ttn, ttr, vr, nsts, nstr, heap, pflag, rounds;
     double *h_in = nullptr, *h_out = nullptr;
     int32_t *h_status = nullptr;
@@ -138,6 +139,11 @@ static int surf_pipeline(mct_surf_ctx *C, int nchain, const double *d_points, co
         da.queue = C->dsp_queue.as<unsigned long long>();
         da.lay_waves = (long)nchain * ((ncol + 63) / 64);
         if (C->dsp_waves > 0 && da.lay_waves > C->dsp_waves) da.lay_waves = C->dsp_waves;
+        if (!C->dsp_unsorted) {
+            da.jobs = C->dsp_jobs.as<int32_t>();
+            da.keys = C->dsp_keys.as<int32_t>();
+            da.hist = C->dsp_hist.as<int32_t>();
+        }
     }
     if ((rc = mct_launch_dispersion(da, s))) return rc;
     hipLaunchKernelGGL(k_surf_status, dim3(nchain), dim3(256), 0, s, (const int32_t *)da.bad_model,
@@ -242,6 +248,8 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
         if (C->dsp_mode < 0 || C->dsp_mode > 3) C->dsp_mode = 0;
         const char *w = getenv("MCT_DSP_WAVES");
         C->dsp_waves = w ? atol(w) : 0;
+        const char *u = getenv("MCT_DSP_UNSORTED");     // columns in grid order (tuning)
+        C->dsp_unsorted = u && atoi(u) != 0;
     }
     int rc = MCT_OK;
     if (hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking) != hipSuccess) { delete C; return MCT_ERR_HIP; }
@@ -273,6 +281,9 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     ALLOC(nlayer, sizeof(int32_t) * ncol * M);
     ALLOC(lay, sizeof(float) * mct_dispersion_lay_floats((int)ncol, grid->nz, M));
     ALLOC(dsp_queue, 256);
+    ALLOC(dsp_jobs, sizeof(int32_t) * ncol * M);
+    ALLOC(dsp_keys, sizeof(int32_t) * ncol * M);
+    ALLOC(dsp_hist, sizeof(int32_t) * 256);
     ALLOC(velv, sizeof(double) * (grid->nx + 2) * (grid->ny + 2) * np * M);
     ALLOC(veln, sizeof(double) * nnx * nnz * np * M);
     ALLOC(phase_time, sizeof(double) * npair * np * M);

@@ -153,13 +153,16 @@ def test_batched_chains_match_plain_calls_and_oracle():
     L.close()
 
 
-@pytest.mark.parametrize("waves", [0, 3])
-def test_column_queue_dispersion_matches_lane_per_column(waves, monkeypatch):
+@pytest.mark.parametrize("waves,unsorted", [(0, 0), (3, 0), (0, 1)])
+def test_column_queue_dispersion_matches_lane_per_column(waves, unsorted, monkeypatch):
     """Dispersion mode 3 (columns from a queue, per-lane state machine) gives the
     same phase velocities, statuses, likelihoods and period-equation counts as
-    one lane per column; with 3 waves every lane runs many columns in turn."""
+    one lane per column, with the queue in layer-count order or in grid order;
+    with 3 waves every lane runs many columns in turn."""
     if waves:
         monkeypatch.setenv("MCT_DSP_WAVES", str(waves))
+    if unsorted:
+        monkeypatch.setenv("MCT_DSP_UNSORTED", "1")
     nchain, maxc = 4, 30
     rng = np.random.default_rng(17)
     pts = np.zeros((nchain, maxc, 3))
