@@ -1434,6 +1434,36 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, const int32_
                 const int xl63 = xs > 0 ? xl0 + 63 : xl0 - 63;
                 const int x = xs > 0 ? xl0 + lane : xl0 - lane;
                 const bool xin = x >= 1 && x <= nnx;
+                const long colT = (long)(x - 1) * ld, colV = (long)(x - 1) * nnz;
+                // the strip's marked nodes as pipeline steps t = b + lane: the visit runs
+                // from the first to the last of them, and on while its changes mark
+                // nodes further on
+                int tl = 0x7fffffff, th = -1;
+                if (xin) {
+                    int zlo = -1, zhi = -1;
+                    const long q0 = colV, q1 = colV + nnz;
+                    for (long w = q0 >> 5; w <= (q1 - 1) >> 5; w++) {
+                        uint32_t v = DB[w];
+                        if (w == (q0 >> 5)) v &= ~0u << (q0 & 31);
+                        if (w == ((q1 - 1) >> 5) && (q1 & 31)) v &= ~(~0u << (q1 & 31));
+                        if (v) {
+                            const int lo = (int)(w * 32 - q0) + __ffs((int)v) - 1;
+                            const int hi = (int)(w * 32 - q0) + 31 - __clz((int)v);
+                            if (zlo < 0) zlo = lo;
+                            zhi = hi;
+                        }
+                    }
+                    if (zlo >= 0) {
+                        tl = (zs > 0 ? zlo : nnz - 1 - zhi) + lane;
+                        th = (zs > 0 ? zhi : nnz - 1 - zlo) + lane;
+                    }
+                }
+                for (int o = 32; o >= 1; o >>= 1) {
+                    const int tl2 = __shfl_xor(tl, o, 64), th2 = __shfl_xor(th, o, 64);
+                    tl = tl2 < tl ? tl2 : tl;
+                    th = th2 > th ? th2 : th;
+                }
+                if (th < 0) continue;       // no marked node left in the strip
                 {
                     const int cx[4] = {xl0 - xs, xl63 + xs, xl0 - 2 * xs, xl63 + 2 * xs};
                     for (int c = 0; c < NSIDE; c++) {
@@ -1445,8 +1475,9 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, const int32_
                 __syncthreads();
                 const int first = xs > 0 ? 0 : (xl0 > nnx ? xl0 - nnx : 0);
                 const int last = xs > 0 ? (nnx - xl0 < 63 ? nnx - xl0 : 63) : 63;
-                const int tend = nnz - 1 + last;
-                const long colT = (long)(x - 1) * ld, colV = (long)(x - 1) * nnz;
+                const int tmax = nnz - 1 + last;
+                const int tstart = tl > first ? tl : first;
+                int tend = th < tmax ? th : tmax;
                 auto told = [&](int bb) -> double {
                     if (!xin || bb < 0 || bb >= nnz) return FF_BIG;
                     return T[colT + (zs > 0 ? bb : nnz - 1 - bb)];
@@ -1456,8 +1487,9 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, const int32_
                     return velnc[colV + (zs > 0 ? bb : nnz - 1 - bb)];
                 };
                 constexpr int D = 2 + FOM;      // the stream delivers the old time D nodes ahead
-                const int b0 = first - lane;
-                double pn1 = FF_BIG, pn2 = FF_BIG;
+                const int b0 = tstart - lane;
+                // behind the start nothing has changed in this visit: the stored times
+                double pn1 = told(b0 - 1), pn2 = FOM ? told(b0 - 2) : FF_BIG;
                 double c0 = told(b0), o1 = told(b0 + 1), o2 = FOM ? told(b0 + 2) : 0.0;
                 double sa[FF_PF], sb[FF_PF], va[FF_PF], vb[FF_PF];
 #pragma unroll
@@ -1521,10 +1553,14 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, const int32_
                             }
                         }
                     }
+                    if (__ballot(newv != c0) != 0ull) {
+                        const int e = t + reach < tmax ? t + reach : tmax;
+                        tend = e > tend ? e : tend;
+                    }
                     pn2 = pn1; pn1 = newv; c0 = o1;
                     if (FOM) { o1 = o2; o2 = snext; } else { o1 = snext; }
                 };
-                for (int t0 = first; t0 <= tend; t0 += 2 * FF_PF) {
+                for (int t0 = tstart; t0 <= tend; t0 += 2 * FF_PF) {
 #pragma unroll
                     for (int j = 0; j < FF_PF; j++) {
                         sb[j] = told(t0 + FF_PF + j - lane + D);
@@ -1541,7 +1577,7 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, const int32_
                     for (int j = 0; j < FF_PF && t0 + FF_PF + j <= tend; j++)
                         step(t0 + FF_PF + j, sb[j], vb[j]);
                 }
-                if (lane == 0) n_steps += tend - first + 1;
+                if (lane == 0) n_steps += tend - tstart + 1;
                 // the flag stays set only if the strip still holds marked nodes (marks
                 // of its own steps ahead of the pipeline were taken in this visit)
                 bool left = false;
