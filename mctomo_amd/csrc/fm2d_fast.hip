@@ -756,7 +756,8 @@ __device__ __forceinline__ double ff_bilinear(const double nv[2][2], double dsx,
 
 // Heap-loop stop rules.
 struct ff_stop {
-    int mode;                 // 0 none, 1 refined inner-edge exit, 2 block frozen, 3 two generations
+    int mode;                 // 0 none, 1 refined inner-edge exit, 2 block frozen, 3 two generations,
+                              // 4 the exit rule, then the block rule
     int vnl, vnr, vnt, vnb;   // refined-grid bookkeeping (mode 1)
     int x0, x1, z0, z1;       // block (mode 2)
     int need, have;
@@ -778,68 +779,83 @@ __device__ __forceinline__ bool ff_gen_get(const fview &V, const uint32_t *b, in
 }
 
 // The reference's heap loop over the view (travel's :137-237) with fouds and
-// velocities vel(z,x); stops by rule S.  Runs in one lane.
+// velocities vel(z,x); stops by rule S.  Called by the whole wave: lane 0
+// keeps the heap and the stop rule (H and S are lane 0's); the updates of the
+// popped node's four neighbours read frozen nodes only, which none of the four
+// is, so lanes 0-3 compute them at once and lane 0 then enters them into the
+// heap in the reference's order (x-1, x+1, z-1, z+1).
 template <class VL>
 __device__ __forceinline__ void ff_heap_loop(const fview &V, ff_heap &H, const VL &vel, int fom, ff_stop &S)
 {
+    const int lane = threadIdx.x;
     auto al = [&](int z, int x) -> bool { return V.n(z, x) == 0; };
-    while (H.ntr > 0) {
-        if (H.full) { S.overflow = 1; return; }
-        int e = H.btg[1];
-        int ix = HPX(e), iz = HPZ(e);
-        if (S.mode == 1) {
-            int swrg = 0;
-            if (ix == 1 && S.vnl != 1) swrg = 1;
-            if (ix == V.nnx && S.vnr != V.nnx) swrg = 1;
-            if (iz == 1 && S.vnt != 1) swrg = 1;
-            if (iz == V.nnz && S.vnb != V.nnz) swrg = 1;
-            if (swrg == 1) { V.n(iz, ix) = 0; S.exited = 1; return; }
-        }
-        if (S.mode == 2 && S.have >= S.need) return;
-        if (S.mode == 3 && S.have >= S.need) {
-            // every node of the generation is frozen: the second generation is
-            // the band at this moment; after it the heap loop is a plain one
-            if (S.gen == 1 || H.ntr == 0) return;
-            for (int q = 1; q <= H.ntr; q++) ff_gen_set(V, S.gen_bits, HPZ(H.btg[q]), HPX(H.btg[q]));
-            S.need += H.ntr;
-            S.gen = 1;
-            e = H.btg[1];
-            ix = HPX(e); iz = HPZ(e);
-        }
-        // the updates around a popped node read up to three nodes away
-        {
-            int xa = ix - 3 < 1 ? 1 : ix - 3, xb = ix + 3 > V.nnx ? V.nnx : ix + 3;
-            int za = iz - 3 < 1 ? 1 : iz - 3, zb = iz + 3 > V.nnz ? V.nnz : iz + 3;
-            if (!V.local(za, xa) || !V.local(zb, xb)) { S.overflow = 1; return; }
-        }
-        V.n(iz, ix) = 0;
-        if (S.mode == 2 && ix >= S.x0 && ix <= S.x1 && iz >= S.z0 && iz <= S.z1) S.have++;
-        if (S.mode == 3 && ff_gen_get(V, S.gen_bits, iz, ix)) S.have++;
-        ff_downtree(V, H);
-        for (int i = ix - 1; i <= ix + 1; i += 2) {
-            if (i >= 1 && i <= V.nnx) {
-                int st = V.n(iz, i);
-                if (st == -1) {
-                    V.t(iz, i) = ff_fouds(V, al, iz, i, 1.0 / vel(iz, i), fom);
-                    ff_addtree(V, H, iz, i);
-                } else if (st > 0) {
-                    V.t(iz, i) = ff_fouds(V, al, iz, i, 1.0 / vel(iz, i), fom);
-                    ff_updtree(V, H, iz, i);
+    for (;;) {
+        int go = 0, ix = 0, iz = 0;
+        if (lane == 0) {
+            do {
+                if (H.ntr <= 0) break;
+                if (H.full) { S.overflow = 1; break; }
+                int e = H.btg[1];
+                ix = HPX(e); iz = HPZ(e);
+                if (S.mode == 1 || S.mode == 4) {
+                    int swrg = 0;
+                    if (ix == 1 && S.vnl != 1) swrg = 1;
+                    if (ix == V.nnx && S.vnr != V.nnx) swrg = 1;
+                    if (iz == 1 && S.vnt != 1) swrg = 1;
+                    if (iz == V.nnz && S.vnb != V.nnz) swrg = 1;
+                    if (swrg == 1) { V.n(iz, ix) = 0; S.exited = 1; break; }
                 }
+                if ((S.mode == 2 || S.mode == 4) && S.have >= S.need) break;
+                if (S.mode == 3 && S.have >= S.need) {
+                    // every node of the generation is frozen: the second generation is
+                    // the band at this moment; after it the heap loop is a plain one
+                    if (S.gen == 1 || H.ntr == 0) break;
+                    for (int q = 1; q <= H.ntr; q++) ff_gen_set(V, S.gen_bits, HPZ(H.btg[q]), HPX(H.btg[q]));
+                    S.need += H.ntr;
+                    S.gen = 1;
+                    e = H.btg[1];
+                    ix = HPX(e); iz = HPZ(e);
+                }
+                // the updates around a popped node read up to three nodes away
+                {
+                    int xa = ix - 3 < 1 ? 1 : ix - 3, xb = ix + 3 > V.nnx ? V.nnx : ix + 3;
+                    int za = iz - 3 < 1 ? 1 : iz - 3, zb = iz + 3 > V.nnz ? V.nnz : iz + 3;
+                    if (!V.local(za, xa) || !V.local(zb, xb)) { S.overflow = 1; break; }
+                }
+                V.n(iz, ix) = 0;
+                if ((S.mode == 2 || S.mode == 4) && ix >= S.x0 && ix <= S.x1 && iz >= S.z0 && iz <= S.z1) S.have++;
+                if (S.mode == 3 && ff_gen_get(V, S.gen_bits, iz, ix)) S.have++;
+                ff_downtree(V, H);
+                go = 1;
+            } while (0);
+        }
+        go = __shfl(go, 0, 64);
+        if (!go) break;
+        ix = __shfl(ix, 0, 64);
+        iz = __shfl(iz, 0, 64);
+        __syncthreads();
+        int nx = ix, nz = iz;
+        if (lane == 0) nx = ix - 1;
+        else if (lane == 1) nx = ix + 1;
+        else if (lane == 2) nz = iz - 1;
+        else nz = iz + 1;
+        const bool inb = lane < 4 && nx >= 1 && nx <= V.nnx && nz >= 1 && nz <= V.nnz;
+        const int st = inb ? (int)V.n(nz, nx) : 0;
+        double tv = 0.0;
+        if (st != 0) tv = ff_fouds(V, al, nz, nx, 1.0 / vel(nz, nx), fom);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int stk = __shfl(st, k, 64);
+            const double tvk = __shfl(tv, k, 64);
+            if (lane == 0 && stk != 0) {
+                const int xk = k == 0 ? ix - 1 : (k == 1 ? ix + 1 : ix);
+                const int zk = k == 2 ? iz - 1 : (k == 3 ? iz + 1 : iz);
+                V.t(zk, xk) = tvk;
+                if (stk == -1) ff_addtree(V, H, zk, xk);
+                else ff_updtree(V, H, zk, xk);
             }
         }
-        for (int i = iz - 1; i <= iz + 1; i += 2) {
-            if (i >= 1 && i <= V.nnz) {
-                int st = V.n(i, ix);
-                if (st == -1) {
-                    V.t(i, ix) = ff_fouds(V, al, i, ix, 1.0 / vel(i, ix), fom);
-                    ff_addtree(V, H, i, ix);
-                } else if (st > 0) {
-                    V.t(i, ix) = ff_fouds(V, al, i, ix, 1.0 / vel(i, ix), fom);
-                    ff_updtree(V, H, i, ix);
-                }
-            }
-        }
+        __syncthreads();
     }
 }
 
@@ -1081,6 +1097,10 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
         __syncthreads();
         auto rvel = [&](int iz, int ix) -> double { return vr[(long)(ix - 1) * A.ld + (iz - 1)]; };
         // seed: the reference's heap loop until the source cell's corners are frozen
+        // and two rings around them (the block rule is checked after the exit rule,
+        // as the reference checks its exit first)
+        ff_stop S;
+        memset(&S, 0, sizeof S);
         if (lane == 0) {
             int risx = (int)((x - gorx) / R.dnx) + 1, risz = (int)((z - gorz) / R.dnz) + 1;
             bool ok = !(risx < 1 || risx > R.nnx || risz < 1 || risz > R.nnz);
@@ -1088,50 +1108,23 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
                 if (risx == R.nnx) risx = risx - 1;
                 if (risz == R.nnz) risz = risz - 1;
                 ff_corners(R, H, rvel, x, z, gorx, gorz, risx, risz);
-                ff_stop S;
-                memset(&S, 0, sizeof S);
-                S.mode = 1;
+                S.mode = 4;
                 S.vnl = g.vnl; S.vnr = g.vnr; S.vnt = g.vnt; S.vnb = g.vnb;
-                // the block rule is checked with the exit rule: run mode 1 while counting
                 S.x0 = risx - FF_SEED_RINGS < 1 ? 1 : risx - FF_SEED_RINGS;
                 S.x1 = risx + 1 + FF_SEED_RINGS > R.nnx ? R.nnx : risx + 1 + FF_SEED_RINGS;
                 S.z0 = risz - FF_SEED_RINGS < 1 ? 1 : risz - FF_SEED_RINGS;
                 S.z1 = risz + 1 + FF_SEED_RINGS > R.nnz ? R.nnz : risz + 1 + FF_SEED_RINGS;
                 S.need = (S.x1 - S.x0 + 1) * (S.z1 - S.z0 + 1);
-                auto al = [&](int zz, int xx) -> bool { return R.n(zz, xx) == 0; };
-                (void)al;
-                // heap loop with both rules (exit first, as the reference checks it first)
-                while (H.ntr > 0) {
-                    if (H.full) break;
-                    int e = H.btg[1];
-                    int ix = HPX(e), iz = HPZ(e);
-                    int swrg = 0;
-                    if (ix == 1 && S.vnl != 1) swrg = 1;
-                    if (ix == R.nnx && S.vnr != R.nnx) swrg = 1;
-                    if (iz == 1 && S.vnt != 1) swrg = 1;
-                    if (iz == R.nnz && S.vnb != R.nnz) swrg = 1;
-                    if (swrg == 1) { R.n(iz, ix) = 0; S.exited = 1; break; }
-                    if (S.have >= S.need) break;
-                    R.n(iz, ix) = 0;
-                    if (ix >= S.x0 && ix <= S.x1 && iz >= S.z0 && iz <= S.z1) S.have++;
-                    ff_downtree(R, H);
-                    for (int i = ix - 1; i <= ix + 1; i += 2)
-                        if (i >= 1 && i <= R.nnx) {
-                            int st = R.n(iz, i);
-                            if (st == -1) { R.t(iz, i) = ff_fouds(R, al, iz, i, 1.0 / rvel(iz, i), A.fom); ff_addtree(R, H, iz, i); }
-                            else if (st > 0) { R.t(iz, i) = ff_fouds(R, al, iz, i, 1.0 / rvel(iz, i), A.fom); ff_updtree(R, H, iz, i); }
-                        }
-                    for (int i = iz - 1; i <= iz + 1; i += 2)
-                        if (i >= 1 && i <= R.nnz) {
-                            int st = R.n(i, ix);
-                            if (st == -1) { R.t(i, ix) = ff_fouds(R, al, i, ix, 1.0 / rvel(i, ix), A.fom); ff_addtree(R, H, i, ix); }
-                            else if (st > 0) { R.t(i, ix) = ff_fouds(R, al, i, ix, 1.0 / rvel(i, ix), A.fom); ff_updtree(R, H, i, ix); }
-                        }
-                }
-                sh_exit = S.exited ? 1 : 0;
-                if (H.full) sh_ovf = 2;
             } else {
                 sh_ovf = 3;   // source outside the refined grid: use the sequential solver
+            }
+        }
+        __syncthreads();
+        if (!sh_ovf) {
+            ff_heap_loop(R, H, rvel, A.fom, S);
+            if (lane == 0) {
+                sh_exit = S.exited ? 1 : 0;
+                if (H.full || S.overflow) sh_ovf = 2;
             }
         }
         __syncthreads();
@@ -1232,12 +1225,12 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
         // the last of those froze; after that every update is a plain one
         for (int w = lane; w < (C.lnx * C.lnz + 31) / 32; w += 64) DB[w] = 0u;
         __syncthreads();
+        memset(&S, 0, sizeof S);
         if (lane == 0) {
-            ff_stop S;
-            memset(&S, 0, sizeof S);
             S.mode = 3;
             S.gen_bits = DB;
             H.ntr = 0;
+            H.full = 0;
             for (int i = lx0; i <= lx1; i++)
                 for (int j = lz0; j <= lz1; j++)
                     if (C.n(j, i) > 0) {
@@ -1245,18 +1238,19 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
                         ff_gen_set(C, DB, j, i);
                         S.need++;
                     }
-            ff_heap_loop(C, H, cvel, A.fom, S);
-            sh_ovf = S.overflow;
         }
+        __syncthreads();
+        ff_heap_loop(C, H, cvel, A.fom, S);
+        if (lane == 0) sh_ovf = S.overflow || H.full;
         __syncthreads();
     } else {
         // ---- no refined grid: seed on the coarse grid ----
         for (int q = lane; q < C.lnx * C.lnz; q += 64) { RN[q] = -1; RT[q] = FF_BIG; }
         __syncthreads();
+        ff_stop S;
+        memset(&S, 0, sizeof S);
         if (lane == 0) {
             ff_corners(C, H, cvel, x, z, A.gox, A.goz, g.isx, g.isz);
-            ff_stop S;
-            memset(&S, 0, sizeof S);
             S.mode = 2;
             S.x0 = g.isx - FF_SEED_RINGS < 1 ? 1 : g.isx - FF_SEED_RINGS;
             S.x1 = g.isx + 1 + FF_SEED_RINGS > A.nnx ? A.nnx : g.isx + 1 + FF_SEED_RINGS;
@@ -1264,9 +1258,10 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
             S.z1 = g.isz + 1 + FF_SEED_RINGS > A.nnz ? A.nnz : g.isz + 1 + FF_SEED_RINGS;
             S.need = (S.x1 - S.x0 + 1) * (S.z1 - S.z0 + 1);
             S.have = 0;
-            ff_heap_loop(C, H, cvel, A.fom, S);
-            sh_ovf = S.overflow;
         }
+        __syncthreads();
+        ff_heap_loop(C, H, cvel, A.fom, S);
+        if (lane == 0) sh_ovf = S.overflow || H.full;
         __syncthreads();
     }
     if (sh_ovf) {
@@ -1359,7 +1354,7 @@ __global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, const int32
 // and streams its column's old times and velocities from memory FF_PF to
 // 2*FF_PF steps ahead; the strip's outer neighbour columns are staged in LDS.
 // A strip without dirty nodes is skipped.
-#define FF_PF 8
+#define FF_PF 4
 
 template <int FOM>
 __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, const int32_t *pflag)
