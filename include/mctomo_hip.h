@@ -228,22 +228,31 @@ int mct_surf_likelihood(mct_surf_ctx *ctx, const double *points, const double *p
 /* nchain likelihood() calls at once on device arrays: points/params
  * (nchain,max_cells,3), ncells (nchain), noise (nchain,2,np) = snoise0,
  * snoise1; out (nchain,3), status (nchain).  Asynchronous on the context's
- * stream; mct_surf_sync waits. */
+ * stream; mct_surf_sync waits.  A batch can run in parts (environment
+ * MCT_SURF_SPLIT at create, default 1) on streams of the context, each part
+ * started when the previous part's dispersion is done so that one part's
+ * 2-D solves overlap the next part's dispersion; results do not depend on it. */
 int mct_surf_likelihood_batch_dev(mct_surf_ctx *ctx, int nchain, const double *points_dev,
                                   const double *params_dev, const int32_t *ncells_dev,
                                   const double *noise_dev, double *out_dev, int32_t *status_dev);
 int mct_surf_sync(mct_surf_ctx *ctx);
 
 /* Stage times of the last synchronised call (ms): nearest site, dispersion,
- * 2-D traveltimes, misfit. */
+ * 2-D traveltimes, misfit.  For a batch run in parts the first three are the
+ * first part's stages and the last runs to the end of the whole batch. */
 int mct_surf_last_timings(mct_surf_ctx *ctx, double *ms4);
+
+/* Number of parts a batch of nchain chains runs in (return value, or < 0 on
+ * error) and the chains of the first part. */
+int mct_surf_split_parts(mct_surf_ctx *ctx, int nchain, int *first_part_chains);
 
 /* 2-D traveltime method: 0 parallel sweeps with the heap loop near the
  * source (default), 1 the sequential heap loop per (period, source). */
 int mct_surf_set_method(mct_surf_ctx *ctx, int fm2d_method);
 
 /* Dispersion launch shape: 0 by size (default), 1 one lane per column
- * (throughput), 2 one wave per column (latency).  Results are the same. */
+ * (throughput), 2 one wave per column (latency), 3 one lane per column with
+ * columns taken from a queue in layer-count order.  Results are the same. */
 int mct_surf_set_dispersion_mode(mct_surf_ctx *ctx, int mode);
 
 /* Work of the last call's dispersion stage over its first nchain chains:

@@ -64,7 +64,7 @@ def lib():
                      "mct_surf_destroy", "mct_surf_likelihood", "mct_surf_likelihood_batch_dev",
                      "mct_surf_sync", "mct_surf_last_timings", "mct_surf_dev_arrays",
                      "mct_surf_set_method", "mct_surf_dispersion_work", "mct_surf_fm2d_stats",
-                     "mct_surf_set_dispersion_mode", "mct_surf_stream", "mct_serve_create",
+                     "mct_surf_set_dispersion_mode", "mct_surf_split_parts", "mct_surf_stream", "mct_serve_create",
                      "mct_serve_submit", "mct_serve_wait", "mct_serve_stats", "mct_serve_destroy"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
@@ -467,6 +467,13 @@ class SurfLikelihood:
         _check(lib().mct_surf_dispersion_work(self._h, int(nchain), ctypes.byref(a), ctypes.byref(b)),
                "mct_surf_dispersion_work")
         return a.value, b.value
+
+    def split_parts(self, nchain):
+        """(parts, chains of the first part) a batch of nchain chains runs in."""
+        n = ctypes.c_int()
+        p = lib().mct_surf_split_parts(self._h, int(nchain), ctypes.byref(n))
+        _check(min(p, 0), "mct_surf_split_parts")
+        return p, n.value
 
     def set_dispersion_mode(self, mode):
         """Dispersion launch shape: 0 by size, 1 lane per column, 2 wave per column,

@@ -74,6 +74,8 @@ __global__ void k_surf_pad(const double *phase, int nx, int ny, int np, int ncha
 
 } // namespace
 
+#define MCT_SURF_SPLIT_MAX 4
+
 struct mct_surf_ctx {
     mct_geom G;
     mct_surf_setup S;
@@ -86,6 +88,11 @@ struct mct_surf_ctx {
                                      // 3 lane per column from a column queue
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // a batch in nsplit parts on the sub streams (MCT_SURF_SPLIT, default 1)
+    int nsplit = 1;
+    hipStream_t sub[MCT_SURF_SPLIT_MAX] = {};
+    hipEvent_t sev[MCT_SURF_SPLIT_MAX][2] = {};
+    hipEvent_t fork = nullptr;
     double ms[4] = {0, 0, 0, 0};
     // immutable data
     sbuf periods, scx, scz, rcx, rcz, ttime, raystat;
@@ -99,85 +106,110 @@ struct mct_surf_ctx {
     int32_t *h_status = nullptr;
 };
 
-static int surf_pipeline(mct_surf_ctx *C, int nchain, const double *d_points, const double *d_params,
+// The pipeline for chains [c0, c0 + nchain) of the batch on stream s: the
+// chain-indexed buffers are offset by c0, and the part's own 2-D problem slots
+// (nparts parts share the workspace), queue counter and histogram are used.
+// ev_dsp (may be null) is recorded once the dispersion stage is queued.  The
+// stage events ev[1..3] follow part 0; ev[0] and ev[4] bracket a one-part run
+// here and a split run in the caller.
+static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int nparts, hipStream_t s,
+                         const double *d_points, const double *d_params,
                          const int32_t *d_ncells, int ncells_const, const double *d_noise,
-                         double *d_out, int32_t *d_status, bool want_arrays)
+                         double *d_out, int32_t *d_status, bool want_arrays, hipEvent_t ev_dsp)
 {
     const mct_geom &G = C->G;
     const mct_surf_setup &S = C->S;
-    hipStream_t s = C->stream;
     const int ncol = G.nx * G.ny;
+    const long gsz = (long)ncol * G.nz;
+    const int NC = S.max_cells, np = S.np;
+    const long npair = (long)S.nsrc * S.nrev;
+    const long wpc = (ncol + 63) / 64;          // waves of lane-per-column scratch per chain
+    const bool timing = part == 0;
     int rc;
-    MCT_CHECK(hipEventRecord(C->ev[0], s));
+    if (nparts == 1) MCT_CHECK(hipEventRecord(C->ev[0], s));
     // (a) nearest site -> vs
+    double *vs = C->vs.as<double>() + c0 * gsz;
     mct_nearest_args na;
     memset(&na, 0, sizeof na);
-    na.G = G; na.nchain = nchain; na.max_cells = S.max_cells;
+    na.G = G; na.nchain = nchain; na.max_cells = NC;
     na.points = d_points; na.params = d_params; na.ncells = d_ncells; na.ncells_const = ncells_const;
     na.ix0 = 1; na.ix1 = G.nx; na.iy0 = 1; na.iy1 = G.ny; na.iz0 = 1; na.iz1 = G.nz;
-    na.vs = C->vs.as<double>(); na.nwave = 1;
-    na.flag_count = C->flag_count.as<int32_t>(); na.flag_nodes = C->flag_nodes.as<int32_t>();
-    na.kd_nodes = C->kd_nodes.as<mct_kd_node>(); na.kd_ind = C->kd_ind.as<int32_t>();
-    na.kd_rdata = C->kd_rdata.as<double>();
+    na.vs = vs; na.nwave = 1;
+    na.flag_count = C->flag_count.as<int32_t>() + c0;
+    na.flag_nodes = C->flag_nodes.as<int32_t>() + (long)c0 * MCT_MAX_FLAGGED;
+    na.kd_nodes = C->kd_nodes.as<mct_kd_node>() + (long)c0 * (2 * NC + 2);
+    na.kd_ind = C->kd_ind.as<int32_t>() + (long)c0 * (NC + 1);
+    na.kd_rdata = C->kd_rdata.as<double>() + (long)c0 * 3 * NC;
     if ((rc = mct_launch_nearest(na, s))) return rc;
-    MCT_CHECK(hipEventRecord(C->ev[1], s));
+    if (timing) MCT_CHECK(hipEventRecord(C->ev[1], s));
     // (b) dispersion per column
     mct_dsp_args da;
     memset(&da, 0, sizeof da);
-    da.ncol = ncol; da.nz = G.nz; da.nchain = nchain; da.vs = C->vs.as<double>();
-    da.dz = G.dz; da.scaling = G.scaling; da.periods = C->periods.as<double>(); da.np = S.np;
+    da.ncol = ncol; da.nz = G.nz; da.nchain = nchain; da.vs = vs;
+    da.dz = G.dz; da.scaling = G.scaling; da.periods = C->periods.as<double>(); da.np = np;
     da.iwave = S.raylov == 1 ? 2 : 1; da.igr = 0; da.dphase = S.dphase;
-    da.phase = C->phase.as<double>(); da.group = C->group.as<double>(); da.ierr = C->ierr.as<int32_t>();
-    da.bad_model = C->bad_model.as<int32_t>(); da.ndltar = C->ndltar.as<int32_t>();
-    da.nlayer = C->nlayer.as<int32_t>();
+    da.phase = C->phase.as<double>() + (long)c0 * ncol * np;
+    da.group = C->group.as<double>() + (long)c0 * ncol * np;
+    da.ierr = C->ierr.as<int32_t>() + (long)c0 * ncol;
+    da.bad_model = C->bad_model.as<int32_t>() + c0;
+    da.ndltar = C->ndltar.as<int32_t>() + (long)c0 * ncol;
+    da.nlayer = C->nlayer.as<int32_t>() + (long)c0 * ncol;
     // many columns: one lane per column from a column queue (throughput); few:
     // one wave per column (latency)
     int dmode = C->dsp_mode;
     if (dmode == 0) dmode = (long)ncol * nchain >= 16384 ? 3 : 2;
-    da.lay = (dmode == 1 || dmode == 3) ? C->lay.as<float>() : nullptr;
+    da.lay = (dmode == 1 || dmode == 3) ? C->lay.as<float>() + (long)c0 * wpc * 4 * G.nz * 64 : nullptr;
     if (da.lay && dmode == 3) {
-        da.queue = C->dsp_queue.as<unsigned long long>();
-        da.lay_waves = (long)nchain * ((ncol + 63) / 64);
+        da.queue = C->dsp_queue.as<unsigned long long>() + part;
+        da.lay_waves = (long)nchain * wpc;
         if (C->dsp_waves > 0 && da.lay_waves > C->dsp_waves) da.lay_waves = C->dsp_waves;
         if (!C->dsp_unsorted) {
-            da.jobs = C->dsp_jobs.as<int32_t>();
-            da.keys = C->dsp_keys.as<int32_t>();
-            da.hist = C->dsp_hist.as<int32_t>();
+            da.jobs = C->dsp_jobs.as<int32_t>() + (long)c0 * ncol;
+            da.keys = C->dsp_keys.as<int32_t>() + (long)c0 * ncol;
+            da.hist = C->dsp_hist.as<int32_t>() + 128 * part;
         }
     }
     if ((rc = mct_launch_dispersion(da, s))) return rc;
     hipLaunchKernelGGL(k_surf_status, dim3(nchain), dim3(256), 0, s, (const int32_t *)da.bad_model,
                        (const int32_t *)da.ierr, ncol, d_status);
     MCT_CHECK(hipGetLastError());
-    MCT_CHECK(hipEventRecord(C->ev[2], s));
+    if (timing) MCT_CHECK(hipEventRecord(C->ev[2], s));
+    if (ev_dsp) MCT_CHECK(hipEventRecord(ev_dsp, s));
     // (c) padded phase-velocity grids, then modrays per (chain, period, source)
+    double *velv = C->velv.as<double>() + (long)c0 * (G.nx + 2) * (G.ny + 2) * np;
     {
-        const long n = (long)(G.nx + 2) * (G.ny + 2) * S.np * nchain;
+        const long n = (long)(G.nx + 2) * (G.ny + 2) * np * nchain;
         hipLaunchKernelGGL(k_surf_pad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                           (const double *)da.phase, G.nx, G.ny, S.np, nchain, C->velv.as<double>());
+                           (const double *)da.phase, G.nx, G.ny, np, nchain, velv);
         MCT_CHECK(hipGetLastError());
     }
     mct_fm2d_args fa = C->F;
+    const long wsz = (long)fa.ld * fa.nc;
+    const long chunk = C->fm2d_chunk / nparts, soff = chunk * part;
     fa.nchain = nchain;
-    fa.velv = C->velv.as<double>(); fa.veln = C->veln.as<double>();
+    fa.velv = velv;
+    fa.veln = C->veln.as<double>() + (long)c0 * fa.nnx * fa.nnz * np;
     fa.scx = C->scx.as<double>(); fa.scz = C->scz.as<double>();
     fa.rcx = C->rcx.as<double>(); fa.rcz = C->rcz.as<double>();
     fa.srs = C->raystat.as<int32_t>(); fa.srs_pstride = 2L * S.nsrc * S.nrev;
-    fa.ttn = C->ttn.as<double>(); fa.ttr = C->ttr.as<double>(); fa.vr = C->vr.as<double>();
-    fa.nsts = C->nsts.as<int32_t>(); fa.nstr = C->nstr.as<int32_t>(); fa.heap = C->heap.as<int32_t>();
-    fa.ttime = C->phase_time.as<double>(); fa.status = d_status;
-    fa.rounds_out = C->rounds.as<int32_t>();
+    fa.ttn = C->ttn.as<double>() + soff * wsz; fa.ttr = C->ttr.as<double>() + soff * wsz;
+    fa.vr = C->vr.as<double>() + soff * wsz;
+    fa.nsts = C->nsts.as<int32_t>() + soff * wsz; fa.nstr = C->nstr.as<int32_t>() + soff * wsz;
+    fa.heap = C->heap.as<int32_t>() + soff * (wsz + 8);
+    double *phase_time = C->phase_time.as<double>() + (long)c0 * npair * np;
+    fa.ttime = phase_time; fa.status = d_status;
+    fa.rounds_out = C->rounds.as<int32_t>() + 4 * soff;
     if (C->fm2d_method == 0 && mct_fm2d_fast_ok(fa))
-        rc = mct_launch_fm2d_fast(fa, C->fm2d_chunk, C->pflag.as<int32_t>(), s);
+        rc = mct_launch_fm2d_fast(fa, chunk, C->pflag.as<int32_t>() + soff, s);
     else
-        rc = mct_launch_fm2d(fa, C->fm2d_chunk, s);
+        rc = mct_launch_fm2d(fa, chunk, s);
     if (rc) return rc;
-    MCT_CHECK(hipEventRecord(C->ev[3], s));
+    if (timing) MCT_CHECK(hipEventRecord(C->ev[3], s));
     // (d) misfit: srdist = phaseTime (src/likelihood_surf.F90:329-330)
     mct_misfit_args ma;
     memset(&ma, 0, sizeof ma);
-    ma.nchain = nchain; ma.nsrc = S.nsrc; ma.nrev = S.nrev; ma.np = S.np; ma.reciprocal = 0;
-    ma.picks = C->phase_time.as<double>(); ma.ttime = C->ttime.as<double>();
+    ma.nchain = nchain; ma.nsrc = S.nsrc; ma.nrev = S.nrev; ma.np = np; ma.reciprocal = 0;
+    ma.picks = phase_time; ma.ttime = C->ttime.as<double>();
     ma.raystat = C->raystat.as<int32_t>(); ma.locations = nullptr; ma.noise = d_noise;
     ma.sigdep = S.sigdep; ma.nrays = S.nrays;
     ma.eps = (double)1.0e-10f;   // EPS, src/likelihood_surf.F90:37
@@ -185,8 +217,16 @@ static int surf_pipeline(mct_surf_ctx *C, int nchain, const double *d_points, co
     ma.srdist = want_arrays ? C->srdist.as<double>() : nullptr;   // chain 0 only
     ma.sigma = want_arrays ? C->sigma.as<double>() : nullptr;
     if ((rc = mct_launch_misfit(ma, s))) return rc;
-    MCT_CHECK(hipEventRecord(C->ev[4], s));
+    if (nparts == 1) MCT_CHECK(hipEventRecord(C->ev[4], s));
     return MCT_OK;
+}
+
+// parts a batch of nchain chains runs in
+static int surf_parts(const mct_surf_ctx *C, int nchain)
+{
+    int P = C->nsplit < nchain ? C->nsplit : nchain;
+    if (P < 1 || C->fm2d_chunk / P < 1) P = 1;
+    return P;
 }
 
 extern "C" {
@@ -224,10 +264,10 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     const long npair = (long)S.nsrc * S.nrev, gsz = (long)grid->nx * grid->ny * grid->nz;
     const long ncol = (long)grid->nx * grid->ny;
     const long nprob = (long)M * np * S.nsrc;
-    // fm2d workspace: problem slots for at most MCT_FM2D_WS_GB GiB (default 16),
+    // fm2d workspace: problem slots for at most MCT_FM2D_WS_GB GiB (default 32),
     // whole waves of 64; more problems run in chunks
     const long slot = mct_fm2d_slot_bytes(F.ld, F.nc);
-    long ws_gb = 16;
+    long ws_gb = 32;
     {
         const char *e = getenv("MCT_FM2D_WS_GB");
         if (e && atol(e) > 0) ws_gb = atol(e);
@@ -253,6 +293,20 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     }
     int rc = MCT_OK;
     if (hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking) != hipSuccess) { delete C; return MCT_ERR_HIP; }
+    {
+        const char *e = getenv("MCT_SURF_SPLIT");
+        int v = e ? atoi(e) : 1;
+        C->nsplit = v < 1 ? 1 : (v > MCT_SURF_SPLIT_MAX ? MCT_SURF_SPLIT_MAX : v);
+        if (C->nsplit > 1) {
+            bool ok = hipEventCreateWithFlags(&C->fork, hipEventDisableTiming) == hipSuccess;
+            for (int p = 0; p < C->nsplit && ok; p++) {
+                ok = hipStreamCreateWithFlags(&C->sub[p], hipStreamNonBlocking) == hipSuccess &&
+                     hipEventCreateWithFlags(&C->sev[p][0], hipEventDisableTiming) == hipSuccess &&
+                     hipEventCreateWithFlags(&C->sev[p][1], hipEventDisableTiming) == hipSuccess;
+            }
+            if (!ok) { mct_surf_destroy(C); return MCT_ERR_HIP; }
+        }
+    }
     for (int i = 0; i < 5; i++)
         if (hipEventCreate(&C->ev[i]) != hipSuccess) { mct_surf_destroy(C); return MCT_ERR_HIP; }
     const long wsz = (long)F.ld * F.nc, ch = C->fm2d_chunk;
@@ -283,7 +337,7 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     ALLOC(dsp_queue, 256);
     ALLOC(dsp_jobs, sizeof(int32_t) * ncol * M);
     ALLOC(dsp_keys, sizeof(int32_t) * ncol * M);
-    ALLOC(dsp_hist, sizeof(int32_t) * 256);
+    ALLOC(dsp_hist, sizeof(int32_t) * 128 * MCT_SURF_SPLIT_MAX);
     ALLOC(velv, sizeof(double) * (grid->nx + 2) * (grid->ny + 2) * np * M);
     ALLOC(veln, sizeof(double) * nnx * nnz * np * M);
     ALLOC(phase_time, sizeof(double) * npair * np * M);
@@ -338,13 +392,19 @@ int mct_surf_destroy(mct_surf_ctx *C)
     sbuf *bufs[] = {&C->periods, &C->scx, &C->scz, &C->rcx, &C->rcz, &C->ttime, &C->raystat,
                     &C->points, &C->params, &C->noise, &C->out, &C->status, &C->vs, &C->flag_count,
                     &C->flag_nodes, &C->kd_nodes, &C->kd_ind, &C->kd_rdata, &C->phase, &C->group,
-                    &C->ierr, &C->bad_model, &C->ndltar, &C->nlayer, &C->lay, &C->velv, &C->veln, &C->phase_time, &C->srdist, &C->sigma,
+                    &C->ierr, &C->bad_model, &C->ndltar, &C->nlayer, &C->lay, &C->dsp_queue, &C->dsp_jobs,
+                    &C->dsp_keys, &C->dsp_hist, &C->velv, &C->veln, &C->phase_time, &C->srdist, &C->sigma,
                     &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap, &C->pflag, &C->rounds};
     for (sbuf *b : bufs) b->release();
     if (C->h_in) (void)hipHostFree(C->h_in);
     if (C->h_out) (void)hipHostFree(C->h_out);
     if (C->h_status) (void)hipHostFree(C->h_status);
     for (int i = 0; i < 5; i++) if (C->ev[i]) (void)hipEventDestroy(C->ev[i]);
+    for (int p = 0; p < MCT_SURF_SPLIT_MAX; p++) {
+        if (C->sub[p]) { (void)hipStreamSynchronize(C->sub[p]); (void)hipStreamDestroy(C->sub[p]); }
+        for (int k = 0; k < 2; k++) if (C->sev[p][k]) (void)hipEventDestroy(C->sev[p][k]);
+    }
+    if (C->fork) (void)hipEventDestroy(C->fork);
     if (C->stream) (void)hipStreamDestroy(C->stream);
     delete C;
     return MCT_OK;
@@ -387,9 +447,9 @@ int mct_surf_likelihood(mct_surf_ctx *C, const double *points, const double *par
     MCT_CHECK(hipMemcpyAsync(C->params.p, hq, sizeof(double) * 3 * ncells, hipMemcpyHostToDevice, s));
     MCT_CHECK(hipMemcpyAsync(C->noise.p, hn, sizeof(double) * 2 * np, hipMemcpyHostToDevice, s));
     MCT_CHECK(hipMemsetAsync(C->status.p, 0, sizeof(int32_t), s));
-    int rc = surf_pipeline(C, 1, C->points.as<double>(), C->params.as<double>(), nullptr, ncells,
+    int rc = surf_pipeline(C, 0, 1, 0, 1, s, C->points.as<double>(), C->params.as<double>(), nullptr, ncells,
                            C->noise.as<double>(), C->out.as<double>(), C->status.as<int32_t>(),
-                           srdist || sigma);
+                           srdist || sigma, nullptr);
     if (rc) return rc;
     MCT_CHECK(hipMemcpyAsync(C->h_out, C->out.p, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
     MCT_CHECK(hipMemcpyAsync(C->h_status, C->status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -413,8 +473,38 @@ int mct_surf_likelihood_batch_dev(mct_surf_ctx *C, int nchain, const double *poi
         !noise_dev || !out_dev || !status_dev)
         return MCT_ERR_ARG;
     MCT_CHECK(hipMemsetAsync(status_dev, 0, sizeof(int32_t) * nchain, C->stream));
-    return surf_pipeline(C, nchain, points_dev, params_dev, ncells_dev, 0, noise_dev, out_dev, status_dev,
-                         false);
+    const int P = surf_parts(C, nchain);
+    if (P <= 1)
+        return surf_pipeline(C, 0, nchain, 0, 1, C->stream, points_dev, params_dev, ncells_dev, 0, noise_dev,
+                             out_dev, status_dev, false, nullptr);
+    // parts of the batch on their own streams, each started when the previous
+    // part's dispersion is done, so one part's 2-D solves overlap the next
+    // part's dispersion; the context's stream waits for all of them
+    const long NC = C->S.max_cells, np = C->S.np;
+    MCT_CHECK(hipEventRecord(C->ev[0], C->stream));
+    MCT_CHECK(hipEventRecord(C->fork, C->stream));
+    int rc = MCT_OK;
+    for (int p = 0; p < P && !rc; p++) {
+        const int c0 = (int)((long)nchain * p / P), c1 = (int)((long)nchain * (p + 1) / P);
+        hipStream_t sp = C->sub[p];
+        MCT_CHECK(hipStreamWaitEvent(sp, C->fork, 0));
+        if (p > 0) MCT_CHECK(hipStreamWaitEvent(sp, C->sev[p - 1][0], 0));
+        rc = surf_pipeline(C, c0, c1 - c0, p, P, sp, points_dev + c0 * 3 * NC, params_dev + c0 * 3 * NC,
+                           ncells_dev + c0, 0, noise_dev + c0 * 2 * np, out_dev + 3L * c0, status_dev + c0,
+                           false, C->sev[p][0]);
+        MCT_CHECK(hipEventRecord(C->sev[p][1], sp));
+        MCT_CHECK(hipStreamWaitEvent(C->stream, C->sev[p][1], 0));
+    }
+    if (!rc) MCT_CHECK(hipEventRecord(C->ev[4], C->stream));
+    return rc;
+}
+
+int mct_surf_split_parts(mct_surf_ctx *C, int nchain, int *first_part_chains)
+{
+    if (!C || nchain < 1 || !first_part_chains) return MCT_ERR_ARG;
+    const int P = surf_parts(C, nchain);
+    *first_part_chains = (int)((long)nchain / P);
+    return P;
 }
 
 int mct_surf_fm2d_stats(mct_surf_ctx *C, double *stats4)

@@ -198,3 +198,40 @@ def test_column_queue_dispersion_matches_lane_per_column(waves, unsorted, monkey
     np.testing.assert_array_equal(p1, p3)
     assert w1 == w3
     L.close()
+
+
+@pytest.mark.parametrize("split", [1, 3])
+def test_batch_in_parts_matches_one_part(split, monkeypatch):
+    """A batch run in parts on the context's streams (one part's 2-D solves
+    overlapping the next part's dispersion) gives the one-part results bit for
+    bit, statuses included."""
+    nchain, maxc = 7, 30
+    rng = np.random.default_rng(29)
+    pts = np.zeros((nchain, maxc, 3))
+    prm = np.zeros((nchain, maxc, 3))
+    ncl = np.zeros(nchain, np.int32)
+    for c in range(nchain):
+        k = int(rng.integers(8, maxc + 1))
+        w = W.surface_workload(n=25, nz=13, L=12.0, Z=6.0, ncells=k, np_=6, nsrc=5, nrev=7, seed=500 + c)
+        pts[c, :k], prm[c, :k], ncl[c] = w.points, w.params, k
+    k = ncl[4]
+    prm[4, int(np.argmax(pts[4, :k, 2])), 1] = prm[4, :k, 1].min() - 0.3
+    dP, dQ = mct.DeviceArray.from_numpy(pts), mct.DeviceArray.from_numpy(prm)
+    dN = mct.DeviceArray.from_numpy(ncl)
+    dS = mct.DeviceArray.from_numpy(np.zeros((nchain, 2, 6)))
+    res = {}
+    for parts in (1, split):
+        monkeypatch.setenv("MCT_SURF_SPLIT", str(parts))
+        L = ctx_for(w, max_cells=maxc, max_chains=nchain)
+        assert L.split_parts(nchain)[0] == parts
+        say("surf batch in parts", parts)
+        dO = mct.DeviceArray((nchain, 3), np.float64)
+        dT = mct.DeviceArray((nchain,), np.int32)
+        L.likelihood_batch_dev(nchain, dP.ptr, dQ.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
+        L.sync()
+        res[parts] = (dO.numpy(), dT.numpy())
+        L.close()
+    (o1, t1), (o2, t2) = res[1], res[split]
+    assert t1[4] == mct.REJECT_MODEL
+    np.testing.assert_array_equal(t1, t2)
+    np.testing.assert_array_equal(o1[t1 == 0], o2[t2 == 0])
