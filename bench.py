@@ -156,6 +156,7 @@ def run_surface(args):
     batched_eps = B * args.steps / dt
     log("batched: B=%d %.1f ms/step %.2f evals/s  stages(ms) nearest %.2f dispersion %.2f "
         "2-D %.2f misfit %.2f" % (B, ms_step, batched_eps, *st_b))
+    log("2-D stage: near-source %.2f ms, sweep %.2f ms" % like.timings2d())
 
     # ---- served: Poisson arrivals into the continuous-batching server -----------
     served = None
@@ -189,28 +190,67 @@ def run_surface(args):
                   "batches": int(nb - 1), "lanes": 2, "max_batch": B}
         log("served:", served)
 
-    # ---- roofline: the dispersion launch (FP64 VALU), live HIP events -----------
-    # algorithmic flops = 170 per (dltar evaluation x layer step), Rayleigh.  A
-    # batch runs in parts; the stage times follow the first part, whose
-    # dispersion launch runs alone (the next part waits for it).
+    # ---- roofline of the dominant kernel, live HIP events ------------------------
+    # Two candidates, timed by events on the stream they run on: the dispersion
+    # launch k_dispersion_sm (FP64 VALU; algorithmic flops = 170 per dltar
+    # evaluation x layer step, Rayleigh) and the 2-D sweep launch k_fm2d_strip
+    # (streams the time and velocity grids; algorithmic bytes per solve =
+    # 8 C velocities read + 8 C times written + 8 per receiver pick).  A batch
+    # may run in parts; the stage times follow the first part.
     nparts, n0 = like.split_parts(B)
     ev0, lev0 = like.dispersion_work(n0)
-    flops = 170.0 * lev0
-    t_disp = st_b[1] * 1e-3
-    achieved = flops / t_disp / 1e12
     G = w.grid.nx * w.grid.ny * w.grid.nz
     C = w.grid.nx * w.grid.ny
     npd = w.freqs.shape[0]
-    roof = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 6), "traffic": None,
-            "kernel": "k_dispersion_sm", "launch_chains": int(n0), "batch_parts": int(nparts),
-            "note": "FP64 vector arithmetic, no MFMA: the peak quoted is MI355X's FP64 figure "
-                    "(78.6 TFLOP/s, vector and matrix alike); flops = 170 x sum over columns of "
-                    "dltar evaluations x (layers-1) (the sequential root walk's count)",
-            "launch_ms": round(st_b[1], 3), "dltar_evals": int(ev0), "layer_evals": int(lev0),
-            "hbm_alg_bytes": int(n0 * (4 * G + 8 * C * npd)),
-            "stage_share": {k: round(v / max(sum(st_b), 1e-9), 4) for k, v in
-                            zip(("nearest", "dispersion", "fm2d", "misfit"), st_b)}}
+    nsrc, nrev = w.src.shape[0], w.rev.shape[0]
+    t_near, t_sweep = like.timings2d()
+    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_surf201.json")
+    try:
+        with open(prof) as f:
+            traffic = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        traffic = {}
+
+    def traffic_of(kernel, chains):
+        # HBM bytes per launch from the committed counter profile of this workload
+        # (FETCH_SIZE and WRITE_SIZE, scaled by a copy of known size), per chain
+        kt = traffic.get(kernel)
+        return int(kt["bytes_per_chain"] * chains) if kt else None
+
+    flops = 170.0 * lev0
+    a_disp = flops / (st_b[1] * 1e-3) / 1e12
+    r_disp = {"bound": "mfma", "achieved": round(a_disp, 4), "peak": FP64_PEAK_TFLOPS,
+              "unit": "TFLOP/s", "frac": round(a_disp / FP64_PEAK_TFLOPS, 6),
+              "traffic": traffic_of("k_dispersion_sm", n0),
+              "kernel": "k_dispersion_sm", "launch_chains": int(n0), "launch_ms": round(st_b[1], 3),
+              "note": "FP64 vector arithmetic, no MFMA: the peak quoted is MI355X's FP64 figure "
+                      "(78.6 TFLOP/s, vector and matrix alike); flops = 170 x sum over columns of "
+                      "dltar evaluations x (layers-1) (the sequential root walk's count)",
+              "dltar_evals": int(ev0), "layer_evals": int(lev0),
+              "alg_bytes": int(n0 * (4 * G + 8 * C * npd))}
+    solves = n0 * npd * nsrc
+    alg_sweep = solves * (16.0 * C + 8.0 * nrev)
+    r_sweep = None
+    if t_sweep > 0:
+        a_sweep = alg_sweep / (t_sweep * 1e-3) / 1e9
+        r_sweep = {"bound": "hbm", "achieved": round(a_sweep, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(a_sweep / HBM_PEAK_GBS, 6), "traffic": traffic_of("k_fm2d_strip<0>", n0),
+                   "kernel": "k_fm2d_strip", "launch_solves": int(solves), "launch_ms": round(t_sweep, 3),
+                   "alg_bytes": int(alg_sweep),
+                   "note": "Gauss-Seidel sweeps of the coarse grid, one wave per (chain, period, source); "
+                           "the traffic counts every sweep's re-read of the grid, the algorithmic bytes "
+                           "one read of the velocities and one write of the times; bound in practice by "
+                           "FP64 VALU issue and latency, not HBM"}
+    roof, other = r_disp, r_sweep
+    if r_sweep and t_sweep > st_b[1]:
+        roof, other = r_sweep, r_disp
+    roof = dict(roof)
+    roof["batch_parts"] = int(nparts)
+    roof["stage_share"] = {k: round(v / max(sum(st_b), 1e-9), 4) for k, v in
+                           zip(("nearest", "dispersion", "fm2d", "misfit"), st_b)}
+    roof["fm2d_split_ms"] = {"near_source": round(t_near, 3), "sweep": round(t_sweep, 3)}
+    if other:
+        roof["next_kernel"] = other
 
     # ---- parity of this run vs the oracle, and the CPU baseline ------------------
     cpu = err = None

@@ -242,6 +242,11 @@ int mct_surf_sync(mct_surf_ctx *ctx);
  * first part's stages and the last runs to the end of the whole batch. */
 int mct_surf_last_timings(mct_surf_ctx *ctx, double *ms4);
 
+/* The 2-D stage of the last synchronised call split at the sweep launch (ms):
+ * velocity grids and the near-source kernel, then the sweep with the receiver
+ * picks (first launch chunk; 0 when the stage did not run the parallel path). */
+int mct_surf_last_timings2d(mct_surf_ctx *ctx, double *near_ms, double *sweep_ms);
+
 /* Number of parts a batch of nchain chains runs in (return value, or < 0 on
  * error) and the chains of the first part. */
 int mct_surf_split_parts(mct_surf_ctx *ctx, int nchain, int *first_part_chains);

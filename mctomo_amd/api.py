@@ -64,7 +64,7 @@ def lib():
                      "mct_surf_destroy", "mct_surf_likelihood", "mct_surf_likelihood_batch_dev",
                      "mct_surf_sync", "mct_surf_last_timings", "mct_surf_dev_arrays",
                      "mct_surf_set_method", "mct_surf_dispersion_work", "mct_surf_fm2d_stats",
-                     "mct_surf_set_dispersion_mode", "mct_surf_split_parts", "mct_surf_stream", "mct_serve_create",
+                     "mct_surf_set_dispersion_mode", "mct_surf_split_parts", "mct_surf_last_timings2d", "mct_surf_stream", "mct_serve_create",
                      "mct_serve_submit", "mct_serve_wait", "mct_serve_stats", "mct_serve_destroy"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
@@ -466,6 +466,12 @@ class SurfLikelihood:
         a, b = ctypes.c_int64(), ctypes.c_int64()
         _check(lib().mct_surf_dispersion_work(self._h, int(nchain), ctypes.byref(a), ctypes.byref(b)),
                "mct_surf_dispersion_work")
+        return a.value, b.value
+
+    def timings2d(self):
+        """(near-source ms, sweep ms) of the last synchronised call's 2-D stage."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().mct_surf_last_timings2d(self._h, ctypes.byref(a), ctypes.byref(b)), "mct_surf_last_timings2d")
         return a.value, b.value
 
     def split_parts(self, nchain):

@@ -1689,6 +1689,7 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
         b.nrun = nprob - p0 < chunk ? nprob - p0 : chunk;
         hipLaunchKernelGGL(k_fm2d_near, dim3((unsigned)b.nrun), dim3(64), lds, s, b, region, heap_cap, win, pflag);
         MCT_CHECK(hipGetLastError());
+        if (a.ev_sweep && p0 == 0) MCT_CHECK(hipEventRecord(a.ev_sweep, s));
         if (strip_lds && a.fom == 0)
             hipLaunchKernelGGL(k_fm2d_strip<0>, dim3((unsigned)b.nrun), dim3(64), strip_lds, s, b, (const int32_t *)pflag);
         else if (strip_lds)

@@ -171,6 +171,7 @@ struct mct_fm2d_args {
     long prob0, nrun;           // problems [prob0, prob0+nrun) use workspace slots 0..nrun-1
     const int32_t *only_flag;   // sequential solver: when set, only slots whose flag is 1
     int32_t *rounds_out;        // parallel path: sweep rounds per slot, or NULL
+    hipEvent_t ev_sweep;        // parallel path: recorded before the first chunk's sweep, or NULL
 };
 // Coarse B-spline velocity per (chain, period), then one lane per
 // (chain, period, source) running the sequential modrays phase solve.

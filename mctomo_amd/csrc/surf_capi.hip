@@ -93,7 +93,10 @@ struct mct_surf_ctx {
     hipStream_t sub[MCT_SURF_SPLIT_MAX] = {};
     hipEvent_t sev[MCT_SURF_SPLIT_MAX][2] = {};
     hipEvent_t fork = nullptr;
+    hipEvent_t ev_sweep = nullptr;   // between the 2-D near-source and sweep launches
     double ms[4] = {0, 0, 0, 0};
+    double ms2d[2] = {0, 0};
+    bool sweep_rec = false;          // ev_sweep was recorded by the last call
     // immutable data
     sbuf periods, scx, scz, rcx, rcz, ttime, raystat;
     // per-chain inputs staged by the host entry point
@@ -199,7 +202,10 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     double *phase_time = C->phase_time.as<double>() + (long)c0 * npair * np;
     fa.ttime = phase_time; fa.status = d_status;
     fa.rounds_out = C->rounds.as<int32_t>() + 4 * soff;
-    if (C->fm2d_method == 0 && mct_fm2d_fast_ok(fa))
+    fa.ev_sweep = timing ? C->ev_sweep : nullptr;
+    const bool fast = C->fm2d_method == 0 && mct_fm2d_fast_ok(fa);
+    if (timing) C->sweep_rec = fast;
+    if (fast)
         rc = mct_launch_fm2d_fast(fa, chunk, C->pflag.as<int32_t>() + soff, s);
     else
         rc = mct_launch_fm2d(fa, chunk, s);
@@ -309,6 +315,7 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     }
     for (int i = 0; i < 5; i++)
         if (hipEventCreate(&C->ev[i]) != hipSuccess) { mct_surf_destroy(C); return MCT_ERR_HIP; }
+    if (hipEventCreate(&C->ev_sweep) != hipSuccess) { mct_surf_destroy(C); return MCT_ERR_HIP; }
     const long wsz = (long)F.ld * F.nc, ch = C->fm2d_chunk;
 #define ALLOC(buf, bytes) if (!rc) rc = C->buf.alloc((size_t)(bytes))
     ALLOC(periods, sizeof(double) * np);
@@ -400,6 +407,7 @@ int mct_surf_destroy(mct_surf_ctx *C)
     if (C->h_out) (void)hipHostFree(C->h_out);
     if (C->h_status) (void)hipHostFree(C->h_status);
     for (int i = 0; i < 5; i++) if (C->ev[i]) (void)hipEventDestroy(C->ev[i]);
+    if (C->ev_sweep) (void)hipEventDestroy(C->ev_sweep);
     for (int p = 0; p < MCT_SURF_SPLIT_MAX; p++) {
         if (C->sub[p]) { (void)hipStreamSynchronize(C->sub[p]); (void)hipStreamDestroy(C->sub[p]); }
         for (int k = 0; k < 2; k++) if (C->sev[p][k]) (void)hipEventDestroy(C->sev[p][k]);
@@ -417,6 +425,19 @@ int mct_surf_sync(mct_surf_ctx *C)
     float f;
     for (int i = 0; i < 4; i++)
         if (hipEventElapsedTime(&f, C->ev[i], C->ev[i + 1]) == hipSuccess) C->ms[i] = f;
+    C->ms2d[0] = C->ms2d[1] = 0.0;
+    if (C->sweep_rec && hipEventElapsedTime(&f, C->ev[2], C->ev_sweep) == hipSuccess) {
+        C->ms2d[0] = f;
+        if (hipEventElapsedTime(&f, C->ev_sweep, C->ev[3]) == hipSuccess) C->ms2d[1] = f;
+    }
+    return MCT_OK;
+}
+
+int mct_surf_last_timings2d(mct_surf_ctx *C, double *near_ms, double *sweep_ms)
+{
+    if (!C || !near_ms || !sweep_ms) return MCT_ERR_ARG;
+    *near_ms = C->ms2d[0];
+    *sweep_ms = C->ms2d[1];
     return MCT_OK;
 }
 
