@@ -134,6 +134,21 @@ module mctomo_hip
             integer(c_int) :: mct_surf_likelihood
         end function
 
+        ! noise_likelihood (src/likelihood.f90:131-159)
+        function mct_body_noise_likelihood(ctx, locations, bnoise0, bnoise1, out) &
+                bind(C, name="mct_body_noise_likelihood")
+            import
+            type(c_ptr), value :: ctx, locations, bnoise0, bnoise1, out
+            integer(c_int) :: mct_body_noise_likelihood
+        end function
+
+        function mct_surf_noise_likelihood(ctx, snoise0, snoise1, out) &
+                bind(C, name="mct_surf_noise_likelihood")
+            import
+            type(c_ptr), value :: ctx, snoise0, snoise1, out
+            integer(c_int) :: mct_surf_noise_likelihood
+        end function
+
     end interface
 
 end module mctomo_hip

@@ -146,6 +146,18 @@ int mct_body_likelihood_batch_dev(mct_ctx *ctx, int nchain, const double *points
                                   const double *locations_dev, const double *noise_dev,
                                   double *out_dev, int32_t *status_dev);
 
+/* noise_likelihood for datatype 0, 1 (src/likelihood.f90:131-146 ->
+ * body_noise_likelihood, src/likelihood_body.F90:38-78): the likelihood of the
+ * traveltimes kept from the context's last likelihood call under new noise
+ * parameters, sigma = bnoise0*t + bnoise1 (np each), with the origin times
+ * locations(4,nsrc) (NULL: zero).  No eikonal solve.  Returns the status. */
+int mct_body_noise_likelihood(mct_ctx *ctx, const double *locations, const double *bnoise0,
+                              const double *bnoise1, double *out);
+/* The same for the first nchain chains of the last batch: locations_dev
+ * (nchain, nsrc, 4) or NULL, noise_dev (nchain, 2, np); asynchronous. */
+int mct_body_noise_likelihood_batch_dev(mct_ctx *ctx, int nchain, const double *locations_dev,
+                                        const double *noise_dev, double *out_dev, int32_t *status_dev);
+
 /* Device memory for callers that hold no device allocator of their own (a
  * Fortran host, or Python without a GPU framework): plain hipMalloc / hipFree
  * and synchronous copies.  No counterpart in the reference, which has no
@@ -236,6 +248,17 @@ int mct_surf_likelihood_batch_dev(mct_surf_ctx *ctx, int nchain, const double *p
                                   const double *params_dev, const int32_t *ncells_dev,
                                   const double *noise_dev, double *out_dev, int32_t *status_dev);
 int mct_surf_sync(mct_surf_ctx *ctx);
+
+/* noise_likelihood for datatype 2 (src/likelihood.f90:131-150 ->
+ * surf_noise_likelihood, src/likelihood_surf.F90:56-99): the likelihood of the
+ * phase times kept from the context's last likelihood call under new noise
+ * parameters, sigma = snoise0*t + snoise1 (np each).  No dispersion or 2-D
+ * solve.  MCT_ERR_SIGMA where the reference stops on a zero noise level. */
+int mct_surf_noise_likelihood(mct_surf_ctx *ctx, const double *snoise0, const double *snoise1, double *out);
+/* The same for the first nchain chains of the last batch: noise_dev
+ * (nchain, 2, np); asynchronous on the context's stream. */
+int mct_surf_noise_likelihood_batch_dev(mct_surf_ctx *ctx, int nchain, const double *noise_dev,
+                                        double *out_dev, int32_t *status_dev);
 
 /* Stage times of the last synchronised call (ms): nearest site, dispersion,
  * 2-D traveltimes, misfit.  For a batch run in parts the first three are the

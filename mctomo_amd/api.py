@@ -65,7 +65,9 @@ def lib():
                      "mct_surf_sync", "mct_surf_last_timings", "mct_surf_dev_arrays",
                      "mct_surf_set_method", "mct_surf_dispersion_work", "mct_surf_fm2d_stats",
                      "mct_surf_set_dispersion_mode", "mct_surf_split_parts", "mct_surf_last_timings2d", "mct_surf_stream", "mct_serve_create",
-                     "mct_serve_submit", "mct_serve_wait", "mct_serve_stats", "mct_serve_destroy"):
+                     "mct_serve_submit", "mct_serve_wait", "mct_serve_stats", "mct_serve_destroy",
+                     "mct_body_noise_likelihood", "mct_body_noise_likelihood_batch_dev",
+                     "mct_surf_noise_likelihood", "mct_surf_noise_likelihood_batch_dev"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -322,6 +324,24 @@ class BodyLikelihood:
             ctypes.c_void_p(status_ptr) if status_ptr else None)
         return _check(rc, "mct_body_likelihood_batch_dev")
 
+    def noise_likelihood(self, bnoise0, bnoise1, locations=None):
+        """noise_likelihood: the last call's traveltimes under new noise
+        parameters (and origin times).  Returns (status, out[3])."""
+        b0 = _f64(np.atleast_1d(bnoise0))
+        b1 = _f64(np.atleast_1d(bnoise1))
+        loc = None if locations is None else _f64(locations)
+        out = np.zeros(3)
+        rc = lib().mct_body_noise_likelihood(self._h, _p(loc), _p(b0), _p(b1), _p(out))
+        _check(rc, "mct_body_noise_likelihood")
+        return rc, out
+
+    def noise_likelihood_batch_dev(self, nchain, locations_ptr, noise_ptr, out_ptr, status_ptr):
+        rc = lib().mct_body_noise_likelihood_batch_dev(
+            self._h, int(nchain), ctypes.c_void_p(locations_ptr) if locations_ptr else None,
+            ctypes.c_void_p(noise_ptr), ctypes.c_void_p(out_ptr),
+            ctypes.c_void_p(status_ptr) if status_ptr else None)
+        return _check(rc, "mct_body_noise_likelihood_batch_dev")
+
     def sync(self):
         _check(lib().mct_sync(self._h), "mct_sync")
 
@@ -447,6 +467,22 @@ class SurfLikelihood:
             ctypes.c_void_p(ncells_ptr), ctypes.c_void_p(noise_ptr), ctypes.c_void_p(out_ptr),
             ctypes.c_void_p(status_ptr))
         return _check(rc, "mct_surf_likelihood_batch_dev")
+
+    def noise_likelihood(self, snoise0, snoise1):
+        """noise_likelihood: the last call's phase times under new noise
+        parameters.  Returns (status, out[3])."""
+        s0 = _f64(np.broadcast_to(np.atleast_1d(snoise0), (self.np,)))
+        s1 = _f64(np.broadcast_to(np.atleast_1d(snoise1), (self.np,)))
+        out = np.zeros(3)
+        rc = lib().mct_surf_noise_likelihood(self._h, _p(s0), _p(s1), _p(out))
+        _check(rc, "mct_surf_noise_likelihood")
+        return rc, out
+
+    def noise_likelihood_batch_dev(self, nchain, noise_ptr, out_ptr, status_ptr):
+        rc = lib().mct_surf_noise_likelihood_batch_dev(
+            self._h, int(nchain), ctypes.c_void_p(noise_ptr), ctypes.c_void_p(out_ptr),
+            ctypes.c_void_p(status_ptr))
+        return _check(rc, "mct_surf_noise_likelihood_batch_dev")
 
     def sync(self):
         _check(lib().mct_surf_sync(self._h), "mct_surf_sync")

@@ -243,6 +243,24 @@ static int body_pipeline(mct_ctx *C, int nchain, const double *d_points, const d
     return MCT_OK;
 }
 
+// body_noise_likelihood (src/likelihood_body.F90:38-78): the misfit of the
+// picks kept from the last likelihood call with sigma = bnoise0*t + bnoise1
+// (srdist is the traveltime, :296) and the origin times of d_locations; the
+// reference checks no lower bound on sigma here.
+static int body_noise(mct_ctx *C, int nchain, const double *d_locations, const double *d_noise,
+                      double *d_out, int32_t *d_status)
+{
+    mct_misfit_args ma;
+    memset(&ma, 0, sizeof ma);
+    ma.nchain = nchain; ma.nsrc = C->S.nsrc; ma.nrev = C->S.nrev; ma.np = C->nwave;
+    ma.reciprocal = C->reciprocal; ma.picks = P<double>(C->picks);
+    ma.ttime = P<double>(C->ttime); ma.raystat = P<int32_t>(C->raystat);
+    ma.locations = d_locations; ma.noise = d_noise; ma.sigdep = 1; ma.nrays = C->S.nrays;
+    ma.eps = -DBL_MAX;
+    ma.out = d_out; ma.status = d_status;
+    return mct_launch_misfit(ma, C->stream);
+}
+
 extern "C" {
 
 const char *mct_version(void) { return MCT_VERSION; }
@@ -380,6 +398,36 @@ int mct_body_likelihood_batch_dev(mct_ctx *C, int nchain, const double *points_d
     if (status_dev) MCT_CHECK(hipMemsetAsync(status_dev, 0, sizeof(int32_t) * nchain, C->stream));
     return body_pipeline(C, nchain, points_dev, params_dev, ncells_dev, 0, locations_dev,
                          noise_dev, out_dev, status_dev, false);
+}
+
+int mct_body_noise_likelihood(mct_ctx *C, const double *locations, const double *bnoise0,
+                              const double *bnoise1, double *out)
+{
+    if (!C || !bnoise0 || !bnoise1 || !out) return MCT_ERR_ARG;
+    hipStream_t s = C->stream;
+    const int np = C->nwave;
+    MCT_CHECK(hipStreamSynchronize(s));
+    double *hl = C->h_in, *hn = hl + 4 * (size_t)C->S.nsrc;
+    for (int j = 0; j < 4 * C->S.nsrc; j++) hl[j] = locations ? locations[j] : 0.0;
+    for (int i = 0; i < np; i++) { hn[i] = bnoise0[i]; hn[np + i] = bnoise1[i]; }
+    MCT_CHECK(hipMemcpyAsync(C->locations.p, hl, sizeof(double) * 4 * C->S.nsrc, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemcpyAsync(C->noise.p, hn, sizeof(double) * 2 * np, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemsetAsync(C->status.p, 0, sizeof(int32_t), s));
+    int rc = body_noise(C, 1, P<double>(C->locations), P<double>(C->noise), P<double>(C->out), P<int32_t>(C->status));
+    if (rc) return rc;
+    MCT_CHECK(hipMemcpyAsync(C->h_out, C->out.p, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+    MCT_CHECK(hipMemcpyAsync(C->h_status, C->status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    MCT_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < 3; i++) out[i] = C->h_out[i];
+    return *C->h_status;
+}
+
+int mct_body_noise_likelihood_batch_dev(mct_ctx *C, int nchain, const double *locations_dev,
+                                        const double *noise_dev, double *out_dev, int32_t *status_dev)
+{
+    if (!C || nchain < 1 || nchain > C->S.max_chains || !noise_dev || !out_dev) return MCT_ERR_ARG;
+    if (status_dev) MCT_CHECK(hipMemsetAsync(status_dev, 0, sizeof(int32_t) * nchain, C->stream));
+    return body_noise(C, nchain, locations_dev, noise_dev, out_dev, status_dev);
 }
 
 int mct_dev_alloc(void **ptr, int64_t bytes)

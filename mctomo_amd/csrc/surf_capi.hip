@@ -227,6 +227,24 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     return MCT_OK;
 }
 
+// surf_noise_likelihood (src/likelihood_surf.F90:56-99): the misfit of the
+// phase times kept from the last likelihood call with sigma = snoise0*t +
+// snoise1 (srdist is the phase time, :329-330); sigma below EPS on a used pair
+// is the reference's stop.
+static int surf_noise(mct_surf_ctx *C, int nchain, const double *d_noise, double *d_out, int32_t *d_status)
+{
+    const mct_surf_setup &S = C->S;
+    mct_misfit_args ma;
+    memset(&ma, 0, sizeof ma);
+    ma.nchain = nchain; ma.nsrc = S.nsrc; ma.nrev = S.nrev; ma.np = S.np; ma.reciprocal = 0;
+    ma.picks = C->phase_time.as<double>(); ma.ttime = C->ttime.as<double>();
+    ma.raystat = C->raystat.as<int32_t>(); ma.locations = nullptr; ma.noise = d_noise;
+    ma.sigdep = 1; ma.nrays = S.nrays;
+    ma.eps = (double)1.0e-10f;   // EPS, src/likelihood_surf.F90:37
+    ma.out = d_out; ma.status = d_status;
+    return mct_launch_misfit(ma, C->stream);
+}
+
 // parts a batch of nchain chains runs in
 static int surf_parts(const mct_surf_ctx *C, int nchain)
 {
@@ -518,6 +536,33 @@ int mct_surf_likelihood_batch_dev(mct_surf_ctx *C, int nchain, const double *poi
     }
     if (!rc) MCT_CHECK(hipEventRecord(C->ev[4], C->stream));
     return rc;
+}
+
+int mct_surf_noise_likelihood(mct_surf_ctx *C, const double *snoise0, const double *snoise1, double *out)
+{
+    if (!C || !snoise0 || !snoise1 || !out) return MCT_ERR_ARG;
+    hipStream_t s = C->stream;
+    const int np = C->S.np;
+    MCT_CHECK(hipStreamSynchronize(s));
+    double *hn = C->h_in;
+    for (int i = 0; i < np; i++) { hn[i] = snoise0[i]; hn[np + i] = snoise1[i]; }
+    MCT_CHECK(hipMemcpyAsync(C->noise.p, hn, sizeof(double) * 2 * np, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemsetAsync(C->status.p, 0, sizeof(int32_t), s));
+    int rc = surf_noise(C, 1, C->noise.as<double>(), C->out.as<double>(), C->status.as<int32_t>());
+    if (rc) return rc;
+    MCT_CHECK(hipMemcpyAsync(C->h_out, C->out.p, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+    MCT_CHECK(hipMemcpyAsync(C->h_status, C->status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    MCT_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < 3; i++) out[i] = C->h_out[i];
+    return *C->h_status;
+}
+
+int mct_surf_noise_likelihood_batch_dev(mct_surf_ctx *C, int nchain, const double *noise_dev, double *out_dev,
+                                        int32_t *status_dev)
+{
+    if (!C || nchain < 1 || nchain > C->S.max_chains || !noise_dev || !out_dev || !status_dev) return MCT_ERR_ARG;
+    MCT_CHECK(hipMemsetAsync(status_dev, 0, sizeof(int32_t) * nchain, C->stream));
+    return surf_noise(C, nchain, noise_dev, out_dev, status_dev);
 }
 
 int mct_surf_split_parts(mct_surf_ctx *C, int nchain, int *first_part_chains)

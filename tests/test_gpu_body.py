@@ -198,3 +198,26 @@ def test_batched_equals_single():
     got = dout.numpy()
     assert (dst.numpy() == 0).all()
     np.testing.assert_allclose(got, np.array(single), rtol=1e-6)
+
+
+def test_noise_likelihood_reuses_the_last_traveltimes():
+    """noise_likelihood (src/likelihood_body.F90:38-78): new noise parameters and
+    origin times over the traveltimes of the last call give what a full call with
+    them gives, and the oracle's misfit of the same picks."""
+    say("body noise likelihood")
+    w = body_workload(n=29, h=0.3, ncells=30, nsrc=5, nrev=12, datatype=1, seed=5)
+    like = mct.BodyLikelihood(w.grid, w.rev, w.ttime, w.raystat, 5, datatype=1, sigdep=1,
+                              max_cells=32)
+    b0, b1 = np.array([0.02, 0.03]), np.array([0.01, 0.02])
+    rc, out1, ptg, _, _ = like.likelihood(w.points, w.params, w.locations, b0, b1, arrays=True)
+    assert rc == 0
+    c0, c1 = np.array([0.05, 0.01]), np.array([0.03, 0.04])
+    loc2 = w.locations.copy()
+    loc2[:, 3] += 0.05
+    rc, out2 = like.noise_likelihood(c0, c1, loc2)
+    assert rc == 0
+    rc, full = like.likelihood(w.points, w.params, loc2, c0, c1)
+    np.testing.assert_array_equal(out2, full)
+    ref, _, _, _ = po.body_misfit(ptg, w.ttime, w.raystat, loc2, sigdep=1, bnoise0=c0, bnoise1=c1)
+    assert abs(out2[0] - ref[0]) <= 1e-12 * abs(ref[0]), (out2, ref)
+    np.testing.assert_allclose(out2[1:], ref[1:], rtol=1e-12)

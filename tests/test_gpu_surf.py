@@ -235,3 +235,50 @@ def test_batch_in_parts_matches_one_part(split, monkeypatch):
     assert t1[4] == mct.REJECT_MODEL
     np.testing.assert_array_equal(t1, t2)
     np.testing.assert_array_equal(o1[t1 == 0], o2[t2 == 0])
+
+
+def test_noise_likelihood_reuses_the_last_phase_times():
+    """noise_likelihood (src/likelihood_surf.F90:56-99): new noise parameters over
+    the phase times of the last call give what a full call with them gives; a
+    zero noise level on a used pair is the reference's stop."""
+    w = W.surface_workload(n=33, nz=17, L=16.0, Z=8.0, ncells=25, np_=7, nsrc=9, nrev=11, seed=5)
+    w.raystat[:, 0, ::4] = 0
+    L = ctx_for(w, sigdep=1)
+    s0, s1 = np.linspace(0.01, 0.02, 7), np.full(7, 0.03)
+    say("surf noise likelihood")
+    rc, out1 = L.likelihood(w.points, w.params, snoise0=s0, snoise1=s1)
+    assert rc == 0
+    t0, t1 = np.full(7, 0.04), np.linspace(0.02, 0.05, 7)
+    rc, out2 = L.noise_likelihood(t0, t1)
+    assert rc == 0
+    rc, full = L.likelihood(w.points, w.params, snoise0=t0, snoise1=t1)
+    np.testing.assert_array_equal(out2, full)
+    orc = po.surf_likelihood(gdict(w.grid), w.points, w.params, w.freqs, w.src, w.rev, w.ttime,
+                             w.raystat, order=w.order, sigdep=1, snoise0=t0, snoise1=t1)
+    assert abs(out2[0] - orc[1][0]) <= 1e-6 * abs(orc[1][0])
+    with pytest.raises(mct.MCTError):
+        L.noise_likelihood(np.zeros(7), np.zeros(7))
+    # the batched form over the chains of the last batch
+    nchain, maxc = 3, 30
+    pts = np.zeros((nchain, maxc, 3))
+    prm = np.zeros((nchain, maxc, 3))
+    ncl = np.zeros(nchain, np.int32)
+    for c in range(nchain):
+        wc = W.surface_workload(n=33, nz=17, L=16.0, Z=8.0, ncells=20 + c, np_=7, nsrc=9, nrev=11, seed=40 + c)
+        pts[c, :20 + c], prm[c, :20 + c], ncl[c] = wc.points, wc.params, 20 + c
+    Lb = ctx_for(w, sigdep=1, max_cells=maxc, max_chains=nchain)
+    dP, dQ, dN = (mct.DeviceArray.from_numpy(a) for a in (pts, prm, ncl))
+    n1 = np.stack([np.stack([s0, s1])] * nchain)
+    n2 = np.stack([np.stack([t0 * (1 + c), t1]) for c in range(nchain)])
+    dS1, dS2 = mct.DeviceArray.from_numpy(n1), mct.DeviceArray.from_numpy(n2)
+    dO, dT = mct.DeviceArray((nchain, 3), np.float64), mct.DeviceArray((nchain,), np.int32)
+    Lb.likelihood_batch_dev(nchain, dP.ptr, dQ.ptr, dN.ptr, dS1.ptr, dO.ptr, dT.ptr)
+    Lb.noise_likelihood_batch_dev(nchain, dS2.ptr, dO.ptr, dT.ptr)
+    Lb.sync()
+    got, st = dO.numpy(), dT.numpy()
+    Lb.likelihood_batch_dev(nchain, dP.ptr, dQ.ptr, dN.ptr, dS2.ptr, dO.ptr, dT.ptr)
+    Lb.sync()
+    assert (st == 0).all() and (dT.numpy() == 0).all()
+    np.testing.assert_array_equal(got, dO.numpy())
+    L.close()
+    Lb.close()
