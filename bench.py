@@ -158,6 +158,51 @@ def run_surface(args):
         "2-D %.2f misfit %.2f" % (B, ms_step, batched_eps, *st_b))
     log("2-D stage: near-source %.2f ms, sweep %.2f ms" % like.timings2d())
 
+    # ---- perturbed-box proposals (the secondary unit) ----------------------------
+    # One proposal per chain: the Vs of the chain's deepest cell rises by 0.05
+    # km/s; the perturbed box is that cell's Voronoi region (its nodes in the GPU
+    # nearest-site grid), what the sampler's box covers for a velocity change.
+    # Dispersion runs on the box's columns widened by one, the 2-D solves and the
+    # misfit on the whole grid; the batch above is the accepted state.
+    pbox = None
+    if args.box_steps > 0:
+        like.accept_batch(np.ones(B, np.int32))
+        Q2 = Q.copy()
+        cols = []
+        for c in range(B):
+            k = int(np.argmax(P[c, :N[c], 2]))
+            sid, _, _, _ = mct.kdtree_to_grid(w.grid, P[c, :N[c]], Q[c, :N[c]])
+            ix, iy = np.nonzero(np.any(sid == k + 1, axis=2))
+            e = 1e-9
+            box = [w.grid.xmin + ix.min() * w.grid.dx + e, w.grid.ymin + iy.min() * w.grid.dy + e, 0.0,
+                   w.grid.xmin + ix.max() * w.grid.dx + e, w.grid.ymin + iy.max() * w.grid.dy + e, 0.0]
+            cols.append(like.box_columns(box))
+            Q2[c, k, 1] += 0.05
+            Q2[c, k, 0] = 1.73 * Q2[c, k, 1]
+        cols = np.array(cols, np.int32)
+        dQ2 = mct.DeviceArray.from_numpy(Q2)
+
+        def run_box():
+            like.likelihood_box_batch_dev(B, dP.ptr, dQ2.ptr, dN.ptr, cols, dS0.ptr, dO.ptr, dS.ptr)
+            like.sync()
+
+        run_box()
+        o_box, s_box = dO.numpy().copy(), dS.numpy().copy()
+        st_box = like.timings()
+        t0 = time.perf_counter()
+        for _ in range(args.box_steps):
+            run_box()
+        dtb = time.perf_counter() - t0
+        like.likelihood_batch_dev(B, dP.ptr, dQ2.ptr, dN.ptr, dS0.ptr, dO.ptr, dS.ptr)
+        like.sync()
+        same = bool(np.array_equal(o_box, dO.numpy()) and np.array_equal(s_box, dS.numpy()))
+        frac = float(np.mean([(b[1] - b[0] + 1) * (b[3] - b[2] + 1) for b in cols]) / (w.grid.nx * w.grid.ny))
+        pbox = {"evals_per_s": round(B * args.box_steps / dtb, 3), "steps": args.box_steps,
+                "mean_box_column_fraction": round(frac, 4), "statuses_ok": bool((s_box == 0).all()),
+                "equal_to_whole_grid_call": same, "stage_ms": [round(v, 3) for v in st_box],
+                "proposal": "Vs of each chain's deepest cell + 0.05 km/s; box = that cell's Voronoi region"}
+        log("perturbed box:", pbox)
+
     # ---- served: Poisson arrivals into the continuous-batching server -----------
     served = None
     if args.served > 0:
@@ -291,6 +336,7 @@ def run_surface(args):
         "batched_vs_plain": round(batched_eps / plain_eps, 3),
         "plain_stage_ms": [round(v, 3) for v in st_plain],
         "served": served,
+        "perturbed_box": pbox,
         "roofline": roof,
         "cpu_baseline": cpu,
         "max_error_vs_oracle": err,
@@ -394,6 +440,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-evals", type=int, default=10)
     ap.add_argument("--served", type=int, default=48, help="requests for the served measurement (0: skip)")
+    ap.add_argument("--box-steps", type=int, default=3,
+                    help="timed perturbed-box batches (secondary unit; 0: skip)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()

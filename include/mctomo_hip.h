@@ -249,6 +249,35 @@ int mct_surf_likelihood_batch_dev(mct_surf_ctx *ctx, int nchain, const double *p
                                   const double *noise_dev, double *out_dev, int32_t *status_dev);
 int mct_surf_sync(mct_surf_ctx *ctx);
 
+/* likelihood() for a proposal that changed the model only inside
+ * perturbed_box (src/likelihood_surf.F90:110-270): convert_to_layer and
+ * surfdisp96 run only for the box's columns widened by one (:155-169); every
+ * other column keeps the phase velocities of the chain's accepted state, as
+ * like%vel keeps them there; the 2-D solves and the misfit cover the whole
+ * grid as in the reference.  box = {x0, y0, z0, x1, y1, z1}
+ * (perturbed_box(1:2)).  The chain needs an accepted state (mct_surf_accept
+ * after a call whose proposal the sampler accepted), else MCT_ERR_ARG.  The
+ * result equals the whole-grid call's when the model outside the box is the
+ * accepted one. */
+int mct_surf_likelihood_box(mct_surf_ctx *ctx, const double *points, const double *params, int ncells,
+                            const double *box, const double *snoise0, const double *snoise1, double *out,
+                            double *phase_time, double *srdist, double *sigma, double *phase_vel);
+/* The column range of box as the reference computes it: cols = {ix0, ix1,
+ * iy0, iy1}, 1-based inclusive, widened by one and clamped to the grid. */
+int mct_surf_box_columns(mct_surf_ctx *ctx, const double *box, int32_t *cols);
+/* Batched form: cols (nchain, 4) host int32 column ranges (as
+ * mct_surf_box_columns gives), the rest as mct_surf_likelihood_batch_dev. */
+int mct_surf_likelihood_box_batch_dev(mct_surf_ctx *ctx, int nchain, const double *points_dev,
+                                      const double *params_dev, const int32_t *ncells_dev, const int32_t *cols,
+                                      const double *noise_dev, double *out_dev, int32_t *status_dev);
+/* The sampler accepted the proposal of the context's last likelihood call for
+ * chain (src/mcmc_loc2.f90:549-567 keeps like then; on reject it restores
+ * like_copy): its phase velocities become the chain's accepted state, from
+ * which later box calls start.  Queued on the context's stream. */
+int mct_surf_accept(mct_surf_ctx *ctx, int chain);
+/* The same for chains 0..nchain-1 with accepted[c] != 0. */
+int mct_surf_accept_batch(mct_surf_ctx *ctx, int nchain, const int32_t *accepted);
+
 /* noise_likelihood for datatype 2 (src/likelihood.f90:131-150 ->
  * surf_noise_likelihood, src/likelihood_surf.F90:56-99): the likelihood of the
  * phase times kept from the context's last likelihood call under new noise

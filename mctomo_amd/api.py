@@ -67,7 +67,9 @@ def lib():
                      "mct_surf_set_dispersion_mode", "mct_surf_split_parts", "mct_surf_last_timings2d", "mct_surf_stream", "mct_serve_create",
                      "mct_serve_submit", "mct_serve_wait", "mct_serve_stats", "mct_serve_destroy",
                      "mct_body_noise_likelihood", "mct_body_noise_likelihood_batch_dev",
-                     "mct_surf_noise_likelihood", "mct_surf_noise_likelihood_batch_dev"):
+                     "mct_surf_noise_likelihood", "mct_surf_noise_likelihood_batch_dev",
+                     "mct_surf_likelihood_box", "mct_surf_box_columns", "mct_surf_likelihood_box_batch_dev",
+                     "mct_surf_accept", "mct_surf_accept_batch"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -467,6 +469,55 @@ class SurfLikelihood:
             ctypes.c_void_p(ncells_ptr), ctypes.c_void_p(noise_ptr), ctypes.c_void_p(out_ptr),
             ctypes.c_void_p(status_ptr))
         return _check(rc, "mct_surf_likelihood_batch_dev")
+
+    def likelihood_box(self, points, params, box, snoise0=None, snoise1=None, arrays=False):
+        """likelihood() of a proposal that changed the model only inside box =
+        (x0, y0, z0, x1, y1, z1): dispersion only for the box's columns, the
+        others from the accepted state (see accept).  Same returns as likelihood."""
+        pts, prm = _f64(points), _f64(params)
+        bx = _f64(np.asarray(box, dtype=np.float64).reshape(6))
+        s0 = None if snoise0 is None else _f64(np.broadcast_to(np.atleast_1d(snoise0), (self.np,)))
+        s1 = None if snoise1 is None else _f64(np.broadcast_to(np.atleast_1d(snoise1), (self.np,)))
+        out = np.zeros(3)
+        pt = sd = sg = pv = None
+        if arrays:
+            npair = self.nsrc * self.nrev
+            pt = np.zeros((self.np, self.nsrc, self.nrev))
+            sd = np.zeros((self.np, npair))
+            sg = np.zeros((self.np, npair))
+            pv = np.zeros((self.grid.nx, self.grid.ny, self.np))
+        rc = lib().mct_surf_likelihood_box(self._h, _p(pts), _p(prm), pts.shape[0], _p(bx), _p(s0), _p(s1),
+                                           _p(out), _p(pt), _p(sd), _p(sg), _p(pv))
+        _check(rc, "mct_surf_likelihood_box")
+        if arrays:
+            return rc, out, pt, sd, sg, pv
+        return rc, out
+
+    def box_columns(self, box):
+        """(ix0, ix1, iy0, iy1), 1-based inclusive: the columns a box call recomputes."""
+        bx = _f64(np.asarray(box, dtype=np.float64).reshape(6))
+        cols = np.zeros(4, np.int32)
+        _check(lib().mct_surf_box_columns(self._h, _p(bx), _p(cols)), "mct_surf_box_columns")
+        return tuple(int(v) for v in cols)
+
+    def likelihood_box_batch_dev(self, nchain, points_ptr, params_ptr, ncells_ptr, cols, noise_ptr, out_ptr,
+                                 status_ptr):
+        """Batched box call; cols (nchain, 4) int32 host column ranges."""
+        c = np.ascontiguousarray(cols, dtype=np.int32).reshape(-1, 4)
+        rc = lib().mct_surf_likelihood_box_batch_dev(
+            self._h, int(nchain), ctypes.c_void_p(points_ptr), ctypes.c_void_p(params_ptr),
+            ctypes.c_void_p(ncells_ptr), _p(c), ctypes.c_void_p(noise_ptr), ctypes.c_void_p(out_ptr),
+            ctypes.c_void_p(status_ptr))
+        return _check(rc, "mct_surf_likelihood_box_batch_dev")
+
+    def accept(self, chain=0):
+        """The last call's proposal for chain was accepted: its phase velocities
+        become the state box calls start from."""
+        _check(lib().mct_surf_accept(self._h, int(chain)), "mct_surf_accept")
+
+    def accept_batch(self, accepted):
+        a = np.ascontiguousarray(accepted, dtype=np.int32)
+        _check(lib().mct_surf_accept_batch(self._h, a.shape[0], _p(a)), "mct_surf_accept_batch")
 
     def noise_likelihood(self, snoise0, snoise1):
         """noise_likelihood: the last call's phase times under new noise

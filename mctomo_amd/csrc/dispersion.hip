@@ -722,10 +722,27 @@ enum {
 // (a counting sort; columns of one count keep no particular order, which
 // changes nothing since every column is computed on its own).
 #define DSP_SORT_IPT 4
-__device__ __forceinline__ int dsp_layer_bin(const mct_dsp_args &A, long idx)
+// the global column (chain * ncol + column) of queue position idx: all columns
+// in grid order, or the columns of each chain's box
+__device__ __forceinline__ long dsp_job_column(const mct_dsp_args &A, long idx)
+{
+    if (!A.box_off) return idx;
+    int lo = 0, hi = A.nchain - 1;           // the chain whose box holds idx
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (A.box_off[mid] <= idx) lo = mid; else hi = mid - 1;
+    }
+    const int32_t *b = A.box + 4 * lo;
+    const long local = idx - A.box_off[lo];
+    const int w = b[3] - b[2] + 1;
+    const int ix = b[0] + (int)(local / w), iy = b[2] + (int)(local % w);
+    return (long)lo * A.ncol + (long)(ix - 1) * A.ny + (iy - 1);
+}
+
+__device__ __forceinline__ int dsp_layer_bin(const mct_dsp_args &A, long g)
 {
     const double EPS = (double)1.0e-10f;       // src/likelihood_surf.F90:37
-    const double *vs = A.vs + idx * A.nz;
+    const double *vs = A.vs + g * A.nz;
     int nl = 0;
     double last = vs[0];
     for (int k = 1; k < A.nz; k++) {
@@ -746,7 +763,7 @@ __global__ __launch_bounds__(256) void k_dsp_sort_count(mct_dsp_args A, long tot
     for (int i = 0; i < DSP_SORT_IPT; i++) {
         const long idx = base + i * 256 + threadIdx.x;
         if (idx < total) {
-            const int b = dsp_layer_bin(A, idx);
+            const int b = dsp_layer_bin(A, dsp_job_column(A, idx));
             A.keys[idx] = b;
             atomicAdd(&h[b], 1);
         }
@@ -785,7 +802,7 @@ __global__ __launch_bounds__(256) void k_dsp_sort_place(mct_dsp_args A, long tot
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < DSP_SORT_IPT; i++)
-        if (bin[i] >= 0) A.jobs[base[bin[i]] + rank[i]] = (int32_t)(b0 + i * 256 + threadIdx.x);
+        if (bin[i] >= 0) A.jobs[base[bin[i]] + rank[i]] = (int32_t)dsp_job_column(A, b0 + i * 256 + threadIdx.x);
 }
 
 __global__ void k_dsp_sort_reset(int32_t *hist, int nb)
@@ -1093,8 +1110,10 @@ int mct_launch_dispersion(const mct_dsp_args &a, hipStream_t s)
         // persistent waves, columns from a queue (ordered by layer count when
         // the order's buffers are given)
         mct_dsp_args b = a;
-        const long total = (long)a.ncol * a.nchain;
-        if (b.jobs && b.keys && b.hist && b.nz <= 128 && total < 0x7fffffffL) {
+        const long total = a.box_off ? a.box_total : (long)a.ncol * a.nchain;
+        if (a.box_off && !(b.jobs && b.keys && b.hist)) return MCT_ERR_ARG;
+        if (total < 1) return MCT_OK;
+        if (b.jobs && b.keys && b.hist && b.nz <= 128 && (long)a.ncol * a.nchain < 0x7fffffffL) {
             const unsigned nblk = (unsigned)((total + 256 * DSP_SORT_IPT - 1) / (256 * DSP_SORT_IPT));
             hipLaunchKernelGGL(k_dsp_sort_reset, dim3(1), dim3(128), 0, s, b.hist, b.nz);
             hipLaunchKernelGGL(k_dsp_sort_count, dim3(nblk), dim3(256), 0, s, b, total);

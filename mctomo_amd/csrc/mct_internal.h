@@ -216,6 +216,11 @@ struct mct_dsp_args {
     long lay_waves;               // waves the scratch holds (queue mode)
     int32_t *jobs, *keys;         // queue mode: (nchain*ncol) columns ordered by layer count, or NULL
     int32_t *hist;                // (nz + 1) bins for that order
+    // queue mode: only the columns of a box per chain (a perturbed-box call), or NULL
+    const int32_t *box;           // (nchain, 4): ix0, ix1, iy0, iy1, 1-based inclusive
+    const int32_t *box_off;       // (nchain + 1): first compact index of each chain's box
+    long box_total;               // box columns over all chains
+    int ny;                       // columns are (ix-1)*ny + (iy-1)
 };
 // One wave per (column, chain): check_model, convert_to_layer, surfdisp96.
 // Requires np <= 60 and nz <= 100.

@@ -12,6 +12,7 @@
 //   fm2d: B-spline gridder + one lane per (chain, period, source) modrays solve
 //   misfit: srdist = phaseTime, sigma, Gaussian misfit
 #include "mct_internal.h"
+#include <vector>
 #include <float.h>
 #include <stdlib.h>
 #include <string.h>
@@ -105,6 +106,10 @@ struct mct_surf_ctx {
     sbuf vs, flag_count, flag_nodes, kd_nodes, kd_ind, kd_rdata;
     sbuf phase, group, ierr, bad_model, ndltar, nlayer, lay, dsp_queue, dsp_jobs, dsp_keys, dsp_hist, velv, veln, phase_time, srdist, sigma;
     sbuf ttn, ttr, vr, nsts, nstr, heap, pflag, rounds;
+    // perturbed-box calls: accepted phase velocities per chain, the boxes of a call
+    sbuf phase_keep, box;
+    int32_t *h_box = nullptr;        // pinned: (nchain, 4) boxes then (nchain + 1) offsets
+    std::vector<char> keep_valid;
     double *h_in = nullptr, *h_out = nullptr;
     int32_t *h_status = nullptr;
 };
@@ -118,7 +123,8 @@ struct mct_surf_ctx {
 static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int nparts, hipStream_t s,
                          const double *d_points, const double *d_params,
                          const int32_t *d_ncells, int ncells_const, const double *d_noise,
-                         double *d_out, int32_t *d_status, bool want_arrays, hipEvent_t ev_dsp)
+                         double *d_out, int32_t *d_status, bool want_arrays, hipEvent_t ev_dsp,
+                         const int32_t *d_box = nullptr, const int32_t *d_box_off = nullptr, long box_total = 0)
 {
     const mct_geom &G = C->G;
     const mct_surf_setup &S = C->S;
@@ -161,12 +167,21 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     // one wave per column (latency)
     int dmode = C->dsp_mode;
     if (dmode == 0) dmode = (long)ncol * nchain >= 16384 ? 3 : 2;
+    if (d_box) {
+        // a perturbed-box call (src/likelihood_surf.F90:155-170): the box's
+        // columns are recomputed, the others keep the accepted phase velocities
+        dmode = 3;
+        MCT_CHECK(hipMemcpyAsync(da.phase, C->phase_keep.as<double>() + (long)c0 * ncol * np,
+                                 sizeof(double) * ncol * np * nchain, hipMemcpyDeviceToDevice, s));
+        MCT_CHECK(hipMemsetAsync(da.ierr, 0, sizeof(int32_t) * ncol * nchain, s));
+        da.box = d_box; da.box_off = d_box_off; da.box_total = box_total; da.ny = G.ny;
+    }
     da.lay = (dmode == 1 || dmode == 3) ? C->lay.as<float>() + (long)c0 * wpc * 4 * G.nz * 64 : nullptr;
     if (da.lay && dmode == 3) {
         da.queue = C->dsp_queue.as<unsigned long long>() + part;
         da.lay_waves = (long)nchain * wpc;
         if (C->dsp_waves > 0 && da.lay_waves > C->dsp_waves) da.lay_waves = C->dsp_waves;
-        if (!C->dsp_unsorted) {
+        if (!C->dsp_unsorted || d_box) {
             da.jobs = C->dsp_jobs.as<int32_t>() + (long)c0 * ncol;
             da.keys = C->dsp_keys.as<int32_t>() + (long)c0 * ncol;
             da.hist = C->dsp_hist.as<int32_t>() + 128 * part;
@@ -359,6 +374,8 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     ALLOC(ndltar, sizeof(int32_t) * ncol * M);
     ALLOC(nlayer, sizeof(int32_t) * ncol * M);
     ALLOC(lay, sizeof(float) * mct_dispersion_lay_floats((int)ncol, grid->nz, M));
+    ALLOC(phase_keep, sizeof(double) * ncol * np * M);
+    ALLOC(box, sizeof(int32_t) * (5 * (size_t)M + 1));
     ALLOC(dsp_queue, 256);
     ALLOC(dsp_jobs, sizeof(int32_t) * ncol * M);
     ALLOC(dsp_keys, sizeof(int32_t) * ncol * M);
@@ -382,6 +399,9 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
         rc = MCT_ERR_NOMEM;
     if (!rc && hipHostMalloc((void **)&C->h_out, sizeof(double) * 3, hipHostMallocDefault) != hipSuccess) rc = MCT_ERR_NOMEM;
     if (!rc && hipHostMalloc((void **)&C->h_status, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) rc = MCT_ERR_NOMEM;
+    if (!rc && hipHostMalloc((void **)&C->h_box, sizeof(int32_t) * (5 * (size_t)M + 1), hipHostMallocDefault) != hipSuccess)
+        rc = MCT_ERR_NOMEM;
+    if (!rc) C->keep_valid.assign((size_t)M, 0);
     if (!rc) {
         // periods = dble(1/freqs) (surfmodes/surfmodes.f90:81-83); source/receiver (x, y)
         double *per = (double *)malloc(sizeof(double) * np);
@@ -419,11 +439,13 @@ int mct_surf_destroy(mct_surf_ctx *C)
                     &C->flag_nodes, &C->kd_nodes, &C->kd_ind, &C->kd_rdata, &C->phase, &C->group,
                     &C->ierr, &C->bad_model, &C->ndltar, &C->nlayer, &C->lay, &C->dsp_queue, &C->dsp_jobs,
                     &C->dsp_keys, &C->dsp_hist, &C->velv, &C->veln, &C->phase_time, &C->srdist, &C->sigma,
-                    &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap, &C->pflag, &C->rounds};
+                    &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap, &C->pflag, &C->rounds,
+                    &C->phase_keep, &C->box};
     for (sbuf *b : bufs) b->release();
     if (C->h_in) (void)hipHostFree(C->h_in);
     if (C->h_out) (void)hipHostFree(C->h_out);
     if (C->h_status) (void)hipHostFree(C->h_status);
+    if (C->h_box) (void)hipHostFree(C->h_box);
     for (int i = 0; i < 5; i++) if (C->ev[i]) (void)hipEventDestroy(C->ev[i]);
     if (C->ev_sweep) (void)hipEventDestroy(C->ev_sweep);
     for (int p = 0; p < MCT_SURF_SPLIT_MAX; p++) {
@@ -466,9 +488,10 @@ int mct_surf_last_timings(mct_surf_ctx *C, double *ms4)
     return MCT_OK;
 }
 
-int mct_surf_likelihood(mct_surf_ctx *C, const double *points, const double *params, int ncells,
-                        const double *snoise0, const double *snoise1, double *out,
-                        double *phase_time, double *srdist, double *sigma, double *phase_vel)
+// One chain, host arrays; box (6: x0, y0, z0, x1, y1, z1) or NULL for the whole grid.
+static int surf_plain(mct_surf_ctx *C, const double *points, const double *params, int ncells, const double *box,
+                      const double *snoise0, const double *snoise1, double *out,
+                      double *phase_time, double *srdist, double *sigma, double *phase_vel)
 {
     if (!C || !points || !params || !out) return MCT_ERR_ARG;
     if (ncells < 1 || ncells > C->S.max_cells) return MCT_ERR_ARG;
@@ -476,6 +499,12 @@ int mct_surf_likelihood(mct_surf_ctx *C, const double *points, const double *par
     hipStream_t s = C->stream;
     const int np = C->S.np;
     const long npair = (long)C->S.nsrc * C->S.nrev, ncol = (long)C->G.nx * C->G.ny;
+    int32_t cols[4];
+    if (box) {
+        if (!C->keep_valid[0]) return MCT_ERR_ARG;      // no accepted state to start from
+        int rcb = mct_surf_box_columns(C, box, cols);
+        if (rcb) return rcb;
+    }
     MCT_CHECK(hipStreamSynchronize(s));
     double *hp = C->h_in, *hq = hp + 3 * (size_t)ncells, *hn = hq + 3 * (size_t)ncells;
     memcpy(hp, points, sizeof(double) * 3 * ncells);
@@ -486,9 +515,21 @@ int mct_surf_likelihood(mct_surf_ctx *C, const double *points, const double *par
     MCT_CHECK(hipMemcpyAsync(C->params.p, hq, sizeof(double) * 3 * ncells, hipMemcpyHostToDevice, s));
     MCT_CHECK(hipMemcpyAsync(C->noise.p, hn, sizeof(double) * 2 * np, hipMemcpyHostToDevice, s));
     MCT_CHECK(hipMemsetAsync(C->status.p, 0, sizeof(int32_t), s));
+    const int32_t *d_box = nullptr, *d_off = nullptr;
+    long total = 0;
+    if (box) {
+        int32_t *hb = C->h_box;
+        for (int k = 0; k < 4; k++) hb[k] = cols[k];
+        total = (long)(cols[1] - cols[0] + 1) * (cols[3] - cols[2] + 1);
+        hb[4] = 0;
+        hb[5] = (int32_t)total;
+        MCT_CHECK(hipMemcpyAsync(C->box.p, hb, sizeof(int32_t) * 6, hipMemcpyHostToDevice, s));
+        d_box = C->box.as<int32_t>();
+        d_off = d_box + 4;
+    }
     int rc = surf_pipeline(C, 0, 1, 0, 1, s, C->points.as<double>(), C->params.as<double>(), nullptr, ncells,
                            C->noise.as<double>(), C->out.as<double>(), C->status.as<int32_t>(),
-                           srdist || sigma, nullptr);
+                           srdist || sigma, nullptr, d_box, d_off, total);
     if (rc) return rc;
     MCT_CHECK(hipMemcpyAsync(C->h_out, C->out.p, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
     MCT_CHECK(hipMemcpyAsync(C->h_status, C->status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -502,6 +543,92 @@ int mct_surf_likelihood(mct_surf_ctx *C, const double *points, const double *par
     int src = mct_surf_sync(C);
     if (src) return src;
     return st;
+}
+
+int mct_surf_likelihood(mct_surf_ctx *C, const double *points, const double *params, int ncells,
+                        const double *snoise0, const double *snoise1, double *out,
+                        double *phase_time, double *srdist, double *sigma, double *phase_vel)
+{
+    return surf_plain(C, points, params, ncells, nullptr, snoise0, snoise1, out, phase_time, srdist, sigma,
+                      phase_vel);
+}
+
+int mct_surf_box_columns(mct_surf_ctx *C, const double *box, int32_t *cols)
+{
+    if (!C || !box || !cols) return MCT_ERR_ARG;
+    const mct_geom &G = C->G;
+    const int expand = 1;   // src/likelihood_surf.F90:39
+    // src/likelihood_surf.F90:155-158, then the clamp of :166-169
+    int ix0 = (int)floor((box[0] - G.xmin) / G.dx) + 1 - expand;
+    int ix1 = (int)floor((box[3] - G.xmin) / G.dx) + 1 + expand;
+    int iy0 = (int)floor((box[1] - G.ymin) / G.dy) + 1 - expand;
+    int iy1 = (int)floor((box[4] - G.ymin) / G.dy) + 1 + expand;
+    if (ix0 < 1) ix0 = 1;
+    if (iy0 < 1) iy0 = 1;
+    if (ix1 > G.nx) ix1 = G.nx;
+    if (iy1 > G.ny) iy1 = G.ny;
+    if (ix0 > ix1 || iy0 > iy1) return MCT_ERR_ARG;
+    cols[0] = ix0; cols[1] = ix1; cols[2] = iy0; cols[3] = iy1;
+    return MCT_OK;
+}
+
+int mct_surf_likelihood_box(mct_surf_ctx *C, const double *points, const double *params, int ncells,
+                            const double *box, const double *snoise0, const double *snoise1, double *out,
+                            double *phase_time, double *srdist, double *sigma, double *phase_vel)
+{
+    if (!box) return MCT_ERR_ARG;
+    return surf_plain(C, points, params, ncells, box, snoise0, snoise1, out, phase_time, srdist, sigma, phase_vel);
+}
+
+int mct_surf_likelihood_box_batch_dev(mct_surf_ctx *C, int nchain, const double *points_dev,
+                                      const double *params_dev, const int32_t *ncells_dev, const int32_t *cols,
+                                      const double *noise_dev, double *out_dev, int32_t *status_dev)
+{
+    if (!C || nchain < 1 || nchain > C->S.max_chains || !points_dev || !params_dev || !ncells_dev || !cols ||
+        !noise_dev || !out_dev || !status_dev)
+        return MCT_ERR_ARG;
+    const mct_geom &G = C->G;
+    for (int c = 0; c < nchain; c++) {
+        const int32_t *b = cols + 4 * c;
+        if (!C->keep_valid[c] || b[0] < 1 || b[1] > G.nx || b[0] > b[1] || b[2] < 1 || b[3] > G.ny || b[2] > b[3])
+            return MCT_ERR_ARG;
+    }
+    hipStream_t s = C->stream;
+    MCT_CHECK(hipStreamSynchronize(s));   // the pinned box staging is free again
+    int32_t *hb = C->h_box, *ho = hb + 4 * (size_t)nchain;
+    long total = 0;
+    for (int c = 0; c < nchain; c++) {
+        for (int k = 0; k < 4; k++) hb[4 * c + k] = cols[4 * c + k];
+        ho[c] = (int32_t)total;
+        total += (long)(cols[4 * c + 1] - cols[4 * c] + 1) * (cols[4 * c + 3] - cols[4 * c + 2] + 1);
+    }
+    ho[nchain] = (int32_t)total;
+    MCT_CHECK(hipMemcpyAsync(C->box.p, hb, sizeof(int32_t) * (5 * (size_t)nchain + 1), hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemsetAsync(status_dev, 0, sizeof(int32_t) * nchain, s));
+    const int32_t *d_box = C->box.as<int32_t>();
+    return surf_pipeline(C, 0, nchain, 0, 1, s, points_dev, params_dev, ncells_dev, 0, noise_dev, out_dev,
+                         status_dev, false, nullptr, d_box, d_box + 4 * nchain, total);
+}
+
+int mct_surf_accept(mct_surf_ctx *C, int chain)
+{
+    if (!C || chain < 0 || chain >= C->S.max_chains) return MCT_ERR_ARG;
+    const long n = (long)C->G.nx * C->G.ny * C->S.np;
+    MCT_CHECK(hipMemcpyAsync(C->phase_keep.as<double>() + chain * n, C->phase.as<double>() + chain * n,
+                             sizeof(double) * n, hipMemcpyDeviceToDevice, C->stream));
+    C->keep_valid[chain] = 1;
+    return MCT_OK;
+}
+
+int mct_surf_accept_batch(mct_surf_ctx *C, int nchain, const int32_t *accepted)
+{
+    if (!C || nchain < 1 || nchain > C->S.max_chains || !accepted) return MCT_ERR_ARG;
+    for (int c = 0; c < nchain; c++) {
+        if (!accepted[c]) continue;
+        int rc = mct_surf_accept(C, c);
+        if (rc) return rc;
+    }
+    return MCT_OK;
 }
 
 int mct_surf_likelihood_batch_dev(mct_surf_ctx *C, int nchain, const double *points_dev,

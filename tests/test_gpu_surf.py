@@ -282,3 +282,122 @@ def test_noise_likelihood_reuses_the_last_phase_times():
     np.testing.assert_array_equal(got, dO.numpy())
     L.close()
     Lb.close()
+
+
+def _changed_columns(w, prm_old, prm_new):
+    """(ix, iy) mask of the columns whose nearest-site Vs changed."""
+    g = gdict(w.grid)
+    _, _, vs_a, _ = po.kdtree_to_grid(g, w.points, prm_old)
+    _, _, vs_b, _ = po.kdtree_to_grid(g, w.points, prm_new)
+    return np.any(vs_a != vs_b, axis=2)
+
+
+def _changed_box(w, prm_old, prm_new):
+    """The perturbed box of a velocity change: the bounding box of the columns
+    whose nearest-site Vs changed (what the sampler's CGAL box covers)."""
+    g = gdict(w.grid)
+    ix, iy = np.nonzero(_changed_columns(w, prm_old, prm_new))
+    assert ix.size > 0
+    e = 1e-9
+    return [g["xmin"] + ix.min() * g["dx"] + e, g["ymin"] + iy.min() * g["dy"] + e, 0.0,
+            g["xmin"] + ix.max() * g["dx"] + e, g["ymin"] + iy.max() * g["dy"] + e, 1.0]
+
+
+def _bump(prm, k, dv):
+    p = prm.copy()
+    p[k, 1] += dv
+    p[k, 0] = 1.73 * p[k, 1]
+    return p
+
+
+def test_perturbed_box_call_equals_the_whole_grid_call():
+    """A proposal that changes one deep cell's Vs: dispersion on the box's
+    columns only, the rest from the accepted state, gives the whole-grid call's
+    phase velocities and likelihood bit for bit; after a rejected proposal the
+    next box call still starts from the accepted state; after mct_surf_accept
+    from the new one."""
+    w = W.surface_workload(n=41, nz=21, ncells=30, np_=10, nsrc=16, nrev=16, seed=20261202)
+    L = ctx_for(w)
+    say("surf perturbed box")
+    with pytest.raises(mct.MCTError):     # no accepted state yet
+        L.likelihood_box(w.points, w.params, [1, 1, 0, 2, 2, 1])
+    rc, outA = L.likelihood(w.points, w.params)
+    assert rc == 0
+    L.accept(0)
+    ncol = w.grid.nx * w.grid.ny
+    # the deeper half of the cells, smallest changed region first
+    deep = np.argsort(-w.points[:, 2])[:15]
+    area = [_changed_columns(w, w.params, _bump(w.params, k, 0.1)).sum() for k in deep]
+    cells = [int(deep[i]) for i in np.argsort(area)]
+
+    def check(prm_old, prm_new):
+        box = _changed_box(w, prm_old, prm_new)
+        c = L.box_columns(box)
+        assert (c[1] - c[0] + 1) * (c[3] - c[2] + 1) < ncol
+        rb, ob, _, _, _, pvb = L.likelihood_box(w.points, prm_new, box, arrays=True)
+        rf, of, _, _, _, pvf = L.likelihood(w.points, prm_new, arrays=True)
+        assert rb == rf == 0
+        np.testing.assert_array_equal(pvb, pvf)
+        np.testing.assert_array_equal(ob, of)
+        return ob
+
+    B = _bump(w.params, cells[0], 0.15)
+    # a box that misses the change keeps the accepted phase velocities there
+    # (a negative control: the box call does not recompute the whole grid)
+    chg = _changed_columns(w, w.params, B)
+    far = None
+    for x0 in (0.0, 17.5):
+        for y0 in (0.0, 17.5):
+            bx = [x0 + 1e-9, y0 + 1e-9, 0.0, x0 + 2.0, y0 + 2.0, 1.0]
+            c = L.box_columns(bx)
+            if not chg[c[0] - 1:c[1], c[2] - 1:c[3]].any():
+                far = bx
+    assert far is not None
+    _, _, _, _, _, pvm = L.likelihood_box(w.points, B, far, arrays=True)
+    _, _, _, _, _, pvf = L.likelihood(w.points, B, arrays=True)
+    assert not np.array_equal(pvm, pvf)
+    check(w.params, B)                     # rejected: not accepted
+    C = _bump(w.params, cells[1], 0.1)
+    check(w.params, C)                     # starts from A again
+    L.accept(0)                            # C accepted
+    D = _bump(C, cells[2], 0.12)
+    check(C, D)
+    L.close()
+
+
+def test_perturbed_box_batch_equals_whole_grid_batch():
+    nchain, maxc = 3, 30
+    ws = [W.surface_workload(n=33, nz=17, L=16.0, Z=8.0, ncells=25, np_=6, nsrc=8, nrev=8, seed=700 + c)
+          for c in range(nchain)]
+    w = ws[0]
+    pts = np.zeros((nchain, maxc, 3))
+    prm = np.zeros((nchain, maxc, 3))
+    ncl = np.full(nchain, 25, np.int32)
+    for c, wc in enumerate(ws):
+        pts[c, :25], prm[c, :25] = wc.points, wc.params
+    L = ctx_for(w, max_cells=maxc, max_chains=nchain)
+    dP, dQ, dN = (mct.DeviceArray.from_numpy(a) for a in (pts, prm, ncl))
+    dS = mct.DeviceArray.from_numpy(np.zeros((nchain, 2, 6)))
+    dO, dT = mct.DeviceArray((nchain, 3), np.float64), mct.DeviceArray((nchain,), np.int32)
+    L.likelihood_batch_dev(nchain, dP.ptr, dQ.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
+    L.sync()
+    assert (dT.numpy() == 0).all()
+    L.accept_batch(np.ones(nchain, np.int32))
+    prm2 = prm.copy()
+    cols = []
+    for c, wc in enumerate(ws):
+        k = int(np.argmax(wc.points[:, 2]))
+        new = _bump(wc.params, k, 0.1 + 0.05 * c)
+        prm2[c, :25] = new
+        wc2 = W.surface_workload(n=33, nz=17, L=16.0, Z=8.0, ncells=25, np_=6, nsrc=8, nrev=8, seed=700 + c)
+        cols.append(L.box_columns(_changed_box(wc2, wc.params, new)))
+    say("surf perturbed box batch", cols)
+    dQ2 = mct.DeviceArray.from_numpy(prm2)
+    L.likelihood_box_batch_dev(nchain, dP.ptr, dQ2.ptr, dN.ptr, np.array(cols), dS.ptr, dO.ptr, dT.ptr)
+    L.sync()
+    ob, tb = dO.numpy(), dT.numpy()
+    L.likelihood_batch_dev(nchain, dP.ptr, dQ2.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
+    L.sync()
+    np.testing.assert_array_equal(tb, dT.numpy())
+    np.testing.assert_array_equal(ob, dO.numpy())
+    L.close()
