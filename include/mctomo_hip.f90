@@ -134,6 +134,24 @@ module mctomo_hip
             integer(c_int) :: mct_surf_likelihood
         end function
 
+        ! likelihood with a perturbed box (src/likelihood_surf.F90:155-270)
+        function mct_surf_likelihood_box(ctx, points, params, ncells, box, snoise0, snoise1, out, &
+                phase_time, srdist, sigma, phase_vel) bind(C, name="mct_surf_likelihood_box")
+            import
+            type(c_ptr), value :: ctx, points, params
+            integer(c_int), value :: ncells
+            type(c_ptr), value :: box, snoise0, snoise1, out, phase_time, srdist, sigma, phase_vel
+            integer(c_int) :: mct_surf_likelihood_box
+        end function
+
+        ! the sampler kept the last proposal (src/mcmc_loc2.f90:549-567)
+        function mct_surf_accept(ctx, chain) bind(C, name="mct_surf_accept")
+            import
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: chain
+            integer(c_int) :: mct_surf_accept
+        end function
+
         ! noise_likelihood (src/likelihood.f90:131-159)
         function mct_body_noise_likelihood(ctx, locations, bnoise0, bnoise1, out) &
                 bind(C, name="mct_body_noise_likelihood")
