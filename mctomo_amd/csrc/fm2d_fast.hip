@@ -934,6 +934,35 @@ __device__ __forceinline__ bool ff_problem(const mct_fm2d_args &A, long slot, ne
     return true;
 }
 
+// The coarse neighbourhood of the source held by k_fm2d_near (1-based,
+// inclusive): the refined window (or the source cell) +- FF_MARGIN, clamped.
+// Fixed (frozen) nodes of the 2-D problem lie in it, nowhere else.
+struct ff_win {
+    int x0, x1, z0, z1;
+    __device__ __forceinline__ bool in(int iz, int ix) const { return ix >= x0 && ix <= x1 && iz >= z0 && iz <= z1; }
+    __device__ __forceinline__ int nz() const { return z1 - z0 + 1; }
+    __device__ __forceinline__ int nodes() const { return (x1 - x0 + 1) * (z1 - z0 + 1); }
+};
+// words of a bitmap over the largest such neighbourhood
+__host__ __device__ __forceinline__ int ff_win_words(const mct_fm2d_args &A)
+{
+    const int ln = (A.asgr == 1 ? 2 * A.sgs + 1 : 2) + 2 * FF_MARGIN;
+    const int lnx = ln < A.nnx ? ln : A.nnx, lnz = ln < A.nnz ? ln : A.nnz;
+    return (lnx * lnz + 31) / 32;
+}
+__device__ __forceinline__ ff_win ff_window(const mct_fm2d_args &A, const near_geom &g)
+{
+    ff_win w;
+    if (A.asgr == 1) {
+        w.x0 = g.vnl - FF_MARGIN; w.x1 = g.vnr + FF_MARGIN; w.z0 = g.vnt - FF_MARGIN; w.z1 = g.vnb + FF_MARGIN;
+    } else {
+        w.x0 = g.isx - FF_MARGIN; w.x1 = g.isx + 1 + FF_MARGIN; w.z0 = g.isz - FF_MARGIN; w.z1 = g.isz + 1 + FF_MARGIN;
+    }
+    w.x0 = w.x0 < 1 ? 1 : w.x0; w.z0 = w.z0 < 1 ? 1 : w.z0;
+    w.x1 = w.x1 > A.nnx ? A.nnx : w.x1; w.z1 = w.z1 > A.nnz ? A.nnz : w.z1;
+    return w;
+}
+
 // wave-wide lexicographic minimum of (t, idx)
 __device__ __forceinline__ void ff_wave_min(double &t, int &idx)
 {
@@ -1041,14 +1070,8 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
     __syncthreads();
 
     // coarse neighbourhood of the source
-    int lx0, lx1, lz0, lz1;
-    if (A.asgr == 1) {
-        lx0 = g.vnl - FF_MARGIN; lx1 = g.vnr + FF_MARGIN; lz0 = g.vnt - FF_MARGIN; lz1 = g.vnb + FF_MARGIN;
-    } else {
-        lx0 = g.isx - FF_MARGIN; lx1 = g.isx + 1 + FF_MARGIN; lz0 = g.isz - FF_MARGIN; lz1 = g.isz + 1 + FF_MARGIN;
-    }
-    lx0 = lx0 < 1 ? 1 : lx0; lz0 = lz0 < 1 ? 1 : lz0;
-    lx1 = lx1 > A.nnx ? A.nnx : lx1; lz1 = lz1 > A.nnz ? A.nnz : lz1;
+    const ff_win win = ff_window(A, g);
+    const int lx0 = win.x0, lx1 = win.x1, lz0 = win.z0, lz1 = win.z1;
     fview C;
     C.nnx = A.nnx; C.nnz = A.nnz; C.dnx = dnx; C.dnz = dnz;
     C.T = RT; C.ns = RN; C.ox = lx0 - 1; C.oz = lz0 - 1; C.lnx = lx1 - lx0 + 1; C.lnz = lz1 - lz0 + 1; C.ld = C.lnz;
@@ -1268,14 +1291,18 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
         if (lane == 0) pflag[slot] = 1;
         return;
     }
-    // write the coarse grid: frozen nodes fixed with their times, the rest open
+    // write the coarse grid: frozen nodes with their times, the rest open; the
+    // fixed flags only in the neighbourhood (the sweeps read them nowhere else)
     double *T = A.ttn + slot * wsz;
     int32_t *F = A.nsts + slot * wsz;
     for (long q = lane; q < (long)A.nnx * A.nnz; q += 64) {
         int k = (int)(q / A.nnz) + 1, l = (int)(q % A.nnz) + 1;
         long o = (long)(k - 1) * A.ld + (l - 1);
-        if (C.local(l, k) && C.n(l, k) == 0) { T[o] = C.t(l, k); F[o] = 1; }
-        else { T[o] = FF_BIG; F[o] = 0; }
+        T[o] = (C.local(l, k) && C.n(l, k) == 0) ? C.t(l, k) : FF_BIG;
+    }
+    for (int q = lane; q < C.lnx * C.lnz; q += 64) {
+        int k = C.ox + q / C.lnz + 1, l = C.oz + q % C.lnz + 1;
+        F[(long)(k - 1) * A.ld + (l - 1)] = C.n(l, k) == 0 ? 1 : 0;
     }
     if (lane == 0) pflag[slot] = 0;
 }
@@ -1298,6 +1325,7 @@ __global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, const int32
     V.nnx = A.nnx; V.nnz = A.nnz; V.dnx = dnx; V.dnz = dnz;
     V.T = A.ttn + slot * wsz; V.ns = nullptr; V.ox = 0; V.oz = 0; V.lnx = A.nnx; V.lnz = A.nnz; V.ld = A.ld;
     const int32_t *F = A.nsts + slot * wsz;
+    const ff_win win = ff_window(A, g);      // fixed flags are written there only
     // the fixed mask as an LDS bitmap (z fastest) when the work lists are in LDS
     const long nn = (long)A.nnx * A.nnz;
     uint32_t *FB = use_dirty ? ff_dirty + (nn + 31) / 32 + 2 * (A.nnx + A.nnz) : nullptr;
@@ -1308,7 +1336,7 @@ __global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, const int32
                 long q = w * 32 + b;
                 if (q < nn) {
                     int x = (int)(q / A.nnz) + 1, z = (int)(q % A.nnz) + 1;
-                    if (F[(long)(x - 1) * A.ld + (z - 1)] != 0) m |= 1u << b;
+                    if (win.in(z, x) && F[(long)(x - 1) * A.ld + (z - 1)] != 0) m |= 1u << b;
                 }
             }
             FB[w] = m;
@@ -1320,7 +1348,7 @@ __global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, const int32
             long q = (long)(ix - 1) * A.nnz + (iz - 1);
             return (FB[q >> 5] >> (q & 31)) & 1u;
         }
-        return F[(long)(ix - 1) * A.ld + (iz - 1)] != 0;
+        return win.in(iz, ix) && F[(long)(ix - 1) * A.ld + (iz - 1)] != 0;
     };
     auto vel = [&](int iz, int ix) -> double { return velnc[(long)(ix - 1) * A.nnz + (iz - 1)]; };
     ff_work W;
@@ -1378,27 +1406,32 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, const int32_
     constexpr int NSIDE = FOM == 0 ? 2 : 4;
     constexpr int reach = FOM == 0 ? 1 : 2;
     const ff_k0 K0 = ff_k0_make(dnx, dnz);
+    // fixed nodes lie in the near-source neighbourhood: a bitmap of it only
+    const ff_win win = ff_window(A, g);
+    const int fw = ff_win_words(A);
     uint32_t *DB = (uint32_t *)ff_sld;        // dirty bits, z fastest
-    uint32_t *FB = DB + nw;                   // fixed bits
-    int *cnt = (int *)(FB + nw);              // per strip of 64 columns: marked since its last visit
+    uint32_t *FB = DB + nw;                   // fixed bits of the neighbourhood, z fastest
+    int *cnt = (int *)(FB + fw + ((nw + fw) & 1));   // per strip of 64 columns: marked since its last visit
     double *side = (double *)(cnt + ((ns + 1) & ~1));   // staged columns L1, R1, L2, R2 (nnz each)
 
-    for (int w = lane; w < nw; w += 64) {
+    for (int w = lane; w < nw; w += 64) DB[w] = 0u;
+    const int wn = win.nodes(), wnz = win.nz();
+    for (int w = lane; w < fw; w += 64) {
         uint32_t m = 0;
         for (int b = 0; b < 32; b++) {
-            long q = (long)w * 32 + b;
-            if (q < nn) {
-                int x = (int)(q / nnz), z = (int)(q % nnz);
-                if (F[(long)x * ld + z] != 0) m |= 1u << b;
+            const int q = w * 32 + b;
+            if (q < wn) {
+                const int x = win.x0 + q / wnz, z = win.z0 + q % wnz;
+                if (F[(long)(x - 1) * ld + (z - 1)] != 0) m |= 1u << b;
             }
         }
         FB[w] = m;
-        DB[w] = 0u;
     }
     for (int k = lane; k < ns; k += 64) cnt[k] = 0;
     __syncthreads();
     auto fixed = [&](int z, int x) -> bool {       // 1-based
-        long q = (long)(x - 1) * nnz + (z - 1);
+        if (!win.in(z, x)) return false;
+        const int q = (x - win.x0) * wnz + (z - win.z0);
         return (FB[q >> 5] >> (q & 31)) & 1u;
     };
     auto mark = [&](int z, int x) {
@@ -1406,12 +1439,18 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, const int32_
         atomicOr(&DB[q >> 5], 1u << (q & 31));
         cnt[(x - 1) >> 6] = 1;
     };
-    for (long q = lane; q < nn; q += 64) {
-        int x = (int)(q / nnz) + 1, z = (int)(q % nnz) + 1;
-        if (fixed(z, x)) continue;
-        bool nb = (x > 1 && fixed(z, x - 1)) || (x < nnx && fixed(z, x + 1)) ||
-                  (z > 1 && fixed(z - 1, x)) || (z < nnz && fixed(z + 1, x));
-        if (nb) mark(z, x);
+    {
+        // the open neighbours of fixed nodes: within the neighbourhood and one around it
+        const int ex0 = win.x0 > 1 ? win.x0 - 1 : 1, ex1 = win.x1 < nnx ? win.x1 + 1 : nnx;
+        const int ez0 = win.z0 > 1 ? win.z0 - 1 : 1, ez1 = win.z1 < nnz ? win.z1 + 1 : nnz;
+        const int enz = ez1 - ez0 + 1, en = (ex1 - ex0 + 1) * enz;
+        for (int q = lane; q < en; q += 64) {
+            const int x = ex0 + q / enz, z = ez0 + q % enz;
+            if (fixed(z, x)) continue;
+            bool nb = (x > 1 && fixed(z, x - 1)) || (x < nnx && fixed(z, x + 1)) ||
+                      (z > 1 && fixed(z - 1, x)) || (z < nnz && fixed(z + 1, x));
+            if (nb) mark(z, x);
+        }
     }
     __syncthreads();
 
@@ -1503,7 +1542,7 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, const int32_
                     if (act) {
                         const long q = colV + z0;
                         const uint32_t m = 1u << (q & 31);
-                        if (!(FB[q >> 5] & m) && (DB[q >> 5] & m)) {
+                        if ((DB[q >> 5] & m) && !fixed(z0 + 1, x)) {
                             atomicAnd(&DB[q >> 5], ~m);
                             n_evals++;
                             if (lane == 0) xm1 = side[z0];
@@ -1615,7 +1654,9 @@ static size_t ff_strip_lds(const mct_fm2d_args &a)
     const long nn = (long)a.nnx * a.nnz;
     const long nw = (nn + 31) / 32;
     const long ns = (a.nnx + 63) / 64;
-    const size_t b = 4 * (size_t)(2 * nw + ((ns + 1) & ~1L)) + 8 * (size_t)(a.fom == 0 ? 2 : 4) * a.nnz;
+    const long fw = ff_win_words(a);
+    const size_t b = 4 * (size_t)(nw + fw + ((nw + fw) & 1) + ((ns + 1) & ~1L)) +
+                     8 * (size_t)(a.fom == 0 ? 2 : 4) * a.nnz;
     return b <= 48 * 1024 ? b : 0;
 }
 
