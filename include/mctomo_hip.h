@@ -193,13 +193,18 @@ int mct_set_kernel_timing(mct_ctx *ctx, int on);
 
 /* ---- surface-wave likelihood context (datatype 2) ------------------------- */
 /* Replaces likelihood() for datatype 2 (src/likelihood.f90:72-80 ->
- * surf_likelihood, src/likelihood_surf.F90:110-405) with phase velocities
- * (LIKE_SET%phaseGroup = 0), no water layer, whole-grid box, as at chain
- * initialisation.  The setup mirrors T_DATA / T_LIKE_SET
- * (src/likelihood_settings.f90:19-79); arrays are copied at create time. */
+ * surf_likelihood, src/likelihood_surf.F90:110-405) with phase or group
+ * velocities (LIKE_SET%phaseGroup), no water layer.  The setup mirrors T_DATA /
+ * T_LIKE_SET (src/likelihood_settings.f90:19-79); arrays are copied at create
+ * time.  With group velocities modrays traces a ray per used pair (rpaths,
+ * fm2d/fm2dray_cartesian.f90:801-1420), the group delays along the rays
+ * (CalGroupTime, likelihood_surf.F90:454-521) are the times, srdist holds the
+ * ray lengths, and a ray that hits rpaths' point limit rejects the model
+ * (MCT_REJECT_RAY, :341-345). */
 typedef struct {
     int raylov;               /* 1 Rayleigh, 0 Love (LIKE_SET%raylov) */
-    int phase_group;          /* must be 0 (phase); group times need ray tracing */
+    int phase_group;          /* 0 phase, 1 group (LIKE_SET%phaseGroup); 1 needs sgref = 1 and
+                                 raystat(n,2) = n on used pairs, as the data readers set it */
     double dphase;            /* LIKE_SET%dPhaseVel */
     int np;                   /* periods, <= 60 */
     const double *freqs;      /* (np) Hz, dat%freqs */
@@ -277,6 +282,11 @@ int mct_surf_likelihood_box_batch_dev(mct_surf_ctx *ctx, int nchain, const doubl
 int mct_surf_accept(mct_surf_ctx *ctx, int chain);
 /* The same for chains 0..nchain-1 with accepted[c] != 0. */
 int mct_surf_accept_batch(mct_surf_ctx *ctx, int nchain, const int32_t *accepted);
+
+/* Group velocities of chain's last call, (nx,ny,np) = Fortran gvel(np,ny,nx)
+ * (surfmodes' gvel output, like%gvel; meaningful with phase_group = 1).
+ * Waits for the context's stream. */
+int mct_surf_group_velocities(mct_surf_ctx *ctx, int chain, double *group_vel);
 
 /* noise_likelihood for datatype 2 (src/likelihood.f90:131-150 ->
  * surf_noise_likelihood, src/likelihood_surf.F90:56-99): the likelihood of the

@@ -69,7 +69,7 @@ def lib():
                      "mct_body_noise_likelihood", "mct_body_noise_likelihood_batch_dev",
                      "mct_surf_noise_likelihood", "mct_surf_noise_likelihood_batch_dev",
                      "mct_surf_likelihood_box", "mct_surf_box_columns", "mct_surf_likelihood_box_batch_dev",
-                     "mct_surf_accept", "mct_surf_accept_batch"):
+                     "mct_surf_accept", "mct_surf_accept_batch", "mct_surf_group_velocities"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -393,17 +393,18 @@ def _surf_setup(grid, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3, si
 
 class SurfLikelihood:
     """Surface-wave likelihood context: like_setup + likelihood for datatype 2
-    with phase velocities (src/likelihood_surf.F90:110-405).
+    (src/likelihood_surf.F90:110-405).
 
     freqs (np) Hz; src (nsrc,2) and rev (nrev,2) as (x, y); ttime
     (np,3,nsrc*nrev) = (time, sigma, -) with the receiver index fastest;
     raystat (np,2,nsrc*nrev).  Settings as LIKE_SET: raylov (1 Rayleigh,
-    0 Love), dphase (dPhaseVel), gridx/gridy, sgref/sgdic/sgext, order, band.
+    0 Love), phase_group (0 phase, 1 group velocities: group delays along
+    rays), dphase (dPhaseVel), gridx/gridy, sgref/sgdic/sgext, order, band.
     """
 
     def __init__(self, grid, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3, sigdep=0,
                  nrays=None, gridx=1, gridy=1, sgref=1, sgdic=4, sgext=8, order=1, band=0.5,
-                 max_cells=1024, max_chains=1):
+                 max_cells=1024, max_chains=1, phase_group=0):
         self.grid = grid
         self.freqs = _f64(freqs).reshape(-1)
         self.np = self.freqs.shape[0]
@@ -417,7 +418,8 @@ class SurfLikelihood:
             nrays = int((self.raystat[:, 0, :] == 1).sum())
         self.nrays, self.sigdep = nrays, sigdep
         self.max_cells, self.max_chains = max_cells, max_chains
-        st = _SurfSetup(int(raylov), 0, float(dphase), self.np, self.freqs.ctypes.data, self.nsrc,
+        self.phase_group = int(phase_group)
+        st = _SurfSetup(int(raylov), self.phase_group, float(dphase), self.np, self.freqs.ctypes.data, self.nsrc,
                         self.nrev, self.src.ctypes.data, self.rev.ctypes.data, self.ttime.ctypes.data,
                         self.raystat.ctypes.data, int(sigdep), int(nrays), int(gridx), int(gridy),
                         int(sgref), int(sgdic), int(sgext), int(order), float(band), int(max_cells),
@@ -518,6 +520,12 @@ class SurfLikelihood:
     def accept_batch(self, accepted):
         a = np.ascontiguousarray(accepted, dtype=np.int32)
         _check(lib().mct_surf_accept_batch(self._h, a.shape[0], _p(a)), "mct_surf_accept_batch")
+
+    def group_velocities(self, chain=0):
+        """Group velocities of chain's last call, (nx, ny, np)."""
+        out = np.zeros((self.grid.nx, self.grid.ny, self.np))
+        _check(lib().mct_surf_group_velocities(self._h, int(chain), _p(out)), "mct_surf_group_velocities")
+        return out
 
     def noise_likelihood(self, snoise0, snoise1):
         """noise_likelihood: the last call's phase times under new noise

@@ -227,6 +227,7 @@ static int body_pipeline(mct_ctx *C, int nchain, const double *d_points, const d
     pa.lx = G.dx; pa.ly = G.dy; pa.lz = lz; pa.out = P<double>(C->picks);
     if ((rc = mct_launch_picks(pa, s))) return rc;
     mct_misfit_args ma;
+    memset(&ma, 0, sizeof ma);
     ma.nchain = nchain; ma.nsrc = C->S.nsrc; ma.nrev = C->S.nrev; ma.np = C->nwave;
     ma.reciprocal = C->reciprocal; ma.picks = P<double>(C->picks);
     ma.ttime = P<double>(C->ttime); ma.raystat = P<int32_t>(C->raystat);

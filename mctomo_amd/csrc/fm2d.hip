@@ -607,6 +607,8 @@ int mct_launch_fm2d(const mct_fm2d_args &a, long chunk, hipStream_t s)
         b.nrun = nprob - p0 < chunk ? nprob - p0 : chunk;
         hipLaunchKernelGGL(k_fm2d_solve, dim3((unsigned)((b.nrun + 63) / 64)), dim3(64), 0, s, b);
         MCT_CHECK(hipGetLastError());
+        int rc = mct_launch_fm2d_rays(b, s);
+        if (rc) return rc;
     }
     return MCT_OK;
 }

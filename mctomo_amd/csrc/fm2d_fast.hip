@@ -1214,6 +1214,17 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
                 if (RN[q] > 0) RN[q] = 1;
             __syncthreads();
         }
+        if (A.rays) {
+            // rpaths reads the refined grid after the solve (ttnr / nstsr, :395-400):
+            // keep its times and states (0 frozen) in the problem's slot
+            double *tr = A.ttr + slot * wsz;
+            int32_t *nr = A.nstr + slot * wsz;
+            for (int q = lane; q < rn; q += 64) {
+                const long o = (long)(q / R.nnz) * A.ld + (q % R.nnz);
+                tr[o] = RT[q];
+                nr[o] = RN[q];
+            }
+        }
         // stage the refined values at the window's coarse nodes (remap, :354-416)
         const int wnx = g.vnr - g.vnl + 1, wnz = g.vnb - g.vnt + 1;
         for (int q = lane; q < wnx * wnz; q += 64) {
@@ -1745,6 +1756,7 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
         // problems whose heap loop left the LDS neighbourhood: the sequential solver
         b.only_flag = pflag;
         if ((rc = mct_launch_fm2d_solve(b, s))) return rc;
+        if ((rc = mct_launch_fm2d_rays(b, s))) return rc;
     }
     return MCT_OK;
 }

@@ -145,6 +145,8 @@ struct mct_misfit_args {
     double *phase_time;            // (nchain, np, nsrc, nrev) or NULL
     double *srdist, *sigma;        // (nchain, np, nsrc*nrev) or NULL
     int32_t *status;               // (nchain) or NULL
+    const double *dist;            // (nchain, np, nsrc*nrev) srdist when it is not the pick (ray
+                                   // lengths for group times), or NULL
 };
 int mct_launch_misfit(const mct_misfit_args &a, hipStream_t s);
 
@@ -172,6 +174,13 @@ struct mct_fm2d_args {
     const int32_t *only_flag;   // sequential solver: when set, only slots whose flag is 1
     int32_t *rounds_out;        // parallel path: sweep rounds per slot, or NULL
     hipEvent_t ev_sweep;        // parallel path: recorded before the first chunk's sweep, or NULL
+    // group velocities (modrays with uar = 0): after each chunk's solve, rays
+    // from the receivers of every used pair; the group delays along them
+    // replace ttime and their lengths go to rlen.  The solvers then also keep
+    // each problem's refined source grid (times in ttr, states in nstr).
+    int rays;
+    const double *gvel;         // (nchain, nvx*nvz, np) group velocities, column iy + nvz*ix (0-based)
+    double *rlen;               // (nchain, np, nsrc, nrc) ray lengths, receiver fastest
 };
 // Coarse B-spline velocity per (chain, period), then one lane per
 // (chain, period, source) running the sequential modrays phase solve.
@@ -186,6 +195,9 @@ int mct_launch_fm2d_solve(const mct_fm2d_args &b, hipStream_t s);
 // geometry fits the LDS budget of the parallel path.
 int mct_fm2d_fast_ok(const mct_fm2d_args &a);
 int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hipStream_t s);
+// rpaths + ray length + CalGroupTime for the chunk [b.prob0, b.prob0 + b.nrun):
+// one thread per (problem, receiver)
+int mct_launch_fm2d_rays(const mct_fm2d_args &b, hipStream_t s);
 // Bytes of fm2d workspace per problem slot.
 static inline long mct_fm2d_slot_bytes(long ld, long nc)
 {

@@ -113,7 +113,9 @@ __global__ __launch_bounds__(MIS_BLOCK) void k_misfit(mct_misfit_args a)
             long pidx = a.reciprocal ? (((long)c * a.np + i) * a.nrev + k) * a.nsrc + j
                                      : (((long)c * a.np + i) * a.nsrc + j) * a.nrev + k;
             double t = a.picks[pidx];
-            double sg = a.sigdep ? b0 * t + b1 : a.ttime[((long)i * 3 + 1) * npair + nrr];
+            // srdist: the phase time, or the ray length for group times (likelihood_surf.F90:324-331)
+            double sd = a.dist ? a.dist[((long)c * a.np + i) * npair + nrr] : t;
+            double sg = a.sigdep ? b0 * sd + b1 : a.ttime[((long)i * 3 + 1) * npair + nrr];
             if (a.raystat[((long)i * 2 + 0) * npair + nrr] == 1) {
                 if (sg < EPS) fatal = 1;
                 double r = t + (a.locations ? a.locations[((long)c * a.nsrc + j) * 4 + 3] : 0.0) - a.ttime[((long)i * 3 + 0) * npair + nrr];
@@ -125,7 +127,7 @@ __global__ __launch_bounds__(MIS_BLOCK) void k_misfit(mct_misfit_args a)
             }
             slog = slog + log(sg);
             if (a.phase_time) a.phase_time[(((long)c * a.np + i) * a.nsrc + j) * a.nrev + k] = t;
-            if (a.srdist) a.srdist[((long)c * a.np + i) * npair + nrr] = t;
+            if (a.srdist) a.srdist[((long)c * a.np + i) * npair + nrr] = sd;
             if (a.sigma) a.sigma[((long)c * a.np + i) * npair + nrr] = sg;
         }
     }

@@ -108,6 +108,8 @@ struct mct_surf_ctx {
     sbuf ttn, ttr, vr, nsts, nstr, heap, pflag, rounds;
     // perturbed-box calls: accepted phase velocities per chain, the boxes of a call
     sbuf phase_keep, box;
+    // group velocities: ray lengths (srdist) per chain and the accepted group velocities
+    sbuf rlen, group_keep;
     int32_t *h_box = nullptr;        // pinned: (nchain, 4) boxes then (nchain + 1) offsets
     std::vector<char> keep_valid;
     double *h_in = nullptr, *h_out = nullptr;
@@ -156,7 +158,7 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     memset(&da, 0, sizeof da);
     da.ncol = ncol; da.nz = G.nz; da.nchain = nchain; da.vs = vs;
     da.dz = G.dz; da.scaling = G.scaling; da.periods = C->periods.as<double>(); da.np = np;
-    da.iwave = S.raylov == 1 ? 2 : 1; da.igr = 0; da.dphase = S.dphase;
+    da.iwave = S.raylov == 1 ? 2 : 1; da.igr = S.phase_group; da.dphase = S.dphase;
     da.phase = C->phase.as<double>() + (long)c0 * ncol * np;
     da.group = C->group.as<double>() + (long)c0 * ncol * np;
     da.ierr = C->ierr.as<int32_t>() + (long)c0 * ncol;
@@ -173,6 +175,9 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
         dmode = 3;
         MCT_CHECK(hipMemcpyAsync(da.phase, C->phase_keep.as<double>() + (long)c0 * ncol * np,
                                  sizeof(double) * ncol * np * nchain, hipMemcpyDeviceToDevice, s));
+        if (S.phase_group)   // like%gvel keeps the accepted group velocities too (:227-229)
+            MCT_CHECK(hipMemcpyAsync(da.group, C->group_keep.as<double>() + (long)c0 * ncol * np,
+                                     sizeof(double) * ncol * np * nchain, hipMemcpyDeviceToDevice, s));
         MCT_CHECK(hipMemsetAsync(da.ierr, 0, sizeof(int32_t) * ncol * nchain, s));
         da.box = d_box; da.box_off = d_box_off; da.box_total = box_total; da.ny = G.ny;
     }
@@ -218,6 +223,10 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     fa.ttime = phase_time; fa.status = d_status;
     fa.rounds_out = C->rounds.as<int32_t>() + 4 * soff;
     fa.ev_sweep = timing ? C->ev_sweep : nullptr;
+    // group velocities: rays, group delays (CalGroupTime) and ray lengths
+    // (likelihood_surf.F90:245,311-351)
+    double *rlen = S.phase_group ? C->rlen.as<double>() + (long)c0 * npair * np : nullptr;
+    fa.rays = S.phase_group; fa.gvel = da.group; fa.rlen = rlen;
     const bool fast = C->fm2d_method == 0 && mct_fm2d_fast_ok(fa);
     if (timing) C->sweep_rec = fast;
     if (fast)
@@ -226,7 +235,8 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
         rc = mct_launch_fm2d(fa, chunk, s);
     if (rc) return rc;
     if (timing) MCT_CHECK(hipEventRecord(C->ev[3], s));
-    // (d) misfit: srdist = phaseTime (src/likelihood_surf.F90:329-330)
+    // (d) misfit: srdist = phaseTime, or the ray length for group times
+    // (src/likelihood_surf.F90:324-331)
     mct_misfit_args ma;
     memset(&ma, 0, sizeof ma);
     ma.nchain = nchain; ma.nsrc = S.nsrc; ma.nrev = S.nrev; ma.np = np; ma.reciprocal = 0;
@@ -237,6 +247,7 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     ma.out = d_out; ma.status = d_status;
     ma.srdist = want_arrays ? C->srdist.as<double>() : nullptr;   // chain 0 only
     ma.sigma = want_arrays ? C->sigma.as<double>() : nullptr;
+    ma.dist = rlen;
     if ((rc = mct_launch_misfit(ma, s))) return rc;
     if (nparts == 1) MCT_CHECK(hipEventRecord(C->ev[4], s));
     return MCT_OK;
@@ -244,7 +255,8 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
 
 // surf_noise_likelihood (src/likelihood_surf.F90:56-99): the misfit of the
 // phase times kept from the last likelihood call with sigma = snoise0*t +
-// snoise1 (srdist is the phase time, :329-330); sigma below EPS on a used pair
+// snoise1 (srdist is the phase time, :329-330, or the ray length for group
+// times, :326-327); sigma below EPS on a used pair
 // is the reference's stop.
 static int surf_noise(mct_surf_ctx *C, int nchain, const double *d_noise, double *d_out, int32_t *d_status)
 {
@@ -257,6 +269,7 @@ static int surf_noise(mct_surf_ctx *C, int nchain, const double *d_noise, double
     ma.sigdep = 1; ma.nrays = S.nrays;
     ma.eps = (double)1.0e-10f;   // EPS, src/likelihood_surf.F90:37
     ma.out = d_out; ma.status = d_status;
+    ma.dist = S.phase_group ? C->rlen.as<double>() : nullptr;
     return mct_launch_misfit(ma, C->stream);
 }
 
@@ -280,7 +293,19 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
         S.max_chains < 1 || S.gridx < 1 || S.gridy < 1 || S.sgdic < 1 || S.sgext < 1 || grid->nz > 100)
         return MCT_ERR_ARG;
     if (S.raylov != 0 && S.raylov != 1) return MCT_ERR_ARG;
-    if (S.phase_group != 0) return MCT_ERR_ARG;   // group traveltimes need ray tracing
+    // group times trace rays from the refined source grid (rpaths reads it)
+    if (S.phase_group != 0 && S.phase_group != 1) return MCT_ERR_ARG;
+    if (S.phase_group == 1 && S.sgref != 1) return MCT_ERR_ARG;
+    if (S.phase_group == 1) {
+        // rpaths stores pair n's ray at slot raystat(n,2) and CalGroupTime reads
+        // slot n for pair n (fm2dray_cartesian.f90:1403, likelihood_surf.F90:477):
+        // the data readers set raystat(n,2) = n (likelihood_settings.f90:401)
+        const long npair0 = (long)S.nsrc * S.nrev;
+        for (int i = 0; i < S.np; i++)
+            for (long n = 0; n < npair0; n++)
+                if (S.raystat[(long)i * 2 * npair0 + n] != 0 && S.raystat[((long)i * 2 + 1) * npair0 + n] != n + 1)
+                    return MCT_ERR_ARG;
+    }
     const long nnx = (long)(grid->nx - 1) * S.gridx + 1, nnz = (long)(grid->ny - 1) * S.gridy + 1;
     const long rg = 2L * S.sgext * S.sgdic + 1;
     if (nnx > 32767 || nnz > 32767 || rg > 32767) return MCT_ERR_ARG;
@@ -375,6 +400,10 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     ALLOC(nlayer, sizeof(int32_t) * ncol * M);
     ALLOC(lay, sizeof(float) * mct_dispersion_lay_floats((int)ncol, grid->nz, M));
     ALLOC(phase_keep, sizeof(double) * ncol * np * M);
+    if (S.phase_group) {
+        ALLOC(rlen, sizeof(double) * npair * np * M);
+        ALLOC(group_keep, sizeof(double) * ncol * np * M);
+    }
     ALLOC(box, sizeof(int32_t) * (5 * (size_t)M + 1));
     ALLOC(dsp_queue, 256);
     ALLOC(dsp_jobs, sizeof(int32_t) * ncol * M);
@@ -440,7 +469,7 @@ int mct_surf_destroy(mct_surf_ctx *C)
                     &C->ierr, &C->bad_model, &C->ndltar, &C->nlayer, &C->lay, &C->dsp_queue, &C->dsp_jobs,
                     &C->dsp_keys, &C->dsp_hist, &C->velv, &C->veln, &C->phase_time, &C->srdist, &C->sigma,
                     &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap, &C->pflag, &C->rounds,
-                    &C->phase_keep, &C->box};
+                    &C->phase_keep, &C->box, &C->rlen, &C->group_keep};
     for (sbuf *b : bufs) b->release();
     if (C->h_in) (void)hipHostFree(C->h_in);
     if (C->h_out) (void)hipHostFree(C->h_out);
@@ -616,7 +645,19 @@ int mct_surf_accept(mct_surf_ctx *C, int chain)
     const long n = (long)C->G.nx * C->G.ny * C->S.np;
     MCT_CHECK(hipMemcpyAsync(C->phase_keep.as<double>() + chain * n, C->phase.as<double>() + chain * n,
                              sizeof(double) * n, hipMemcpyDeviceToDevice, C->stream));
+    if (C->S.phase_group)
+        MCT_CHECK(hipMemcpyAsync(C->group_keep.as<double>() + chain * n, C->group.as<double>() + chain * n,
+                                 sizeof(double) * n, hipMemcpyDeviceToDevice, C->stream));
     C->keep_valid[chain] = 1;
+    return MCT_OK;
+}
+
+int mct_surf_group_velocities(mct_surf_ctx *C, int chain, double *group_vel)
+{
+    if (!C || chain < 0 || chain >= C->S.max_chains || !group_vel) return MCT_ERR_ARG;
+    const long n = (long)C->G.nx * C->G.ny * C->S.np;
+    MCT_CHECK(hipStreamSynchronize(C->stream));
+    MCT_CHECK(hipMemcpy(group_vel, C->group.as<double>() + chain * n, sizeof(double) * n, hipMemcpyDeviceToHost));
     return MCT_OK;
 }
 

@@ -152,7 +152,10 @@ def surface_workload(n=41, nz=21, L=20.0, Z=10.0, ncells=30, np_=10, periods=(1.
     else:
         tt[:, 0] = ttime
     tt[:, 1] = sigma
+    # raystat(n,1) = 1 used pair, raystat(n,2) = n as read_times_3 sets it
+    # (src/likelihood_settings.f90:400-401)
     rs = np.ones((np_, 2, npair), np.int32)
+    rs[:, 1, :] = np.arange(1, npair + 1, dtype=np.int32)
     return SurfWorkload(grid, points, params, freqs, src, rev, tt, rs, raylov, order, rejected)
 
 

@@ -8,8 +8,11 @@
  * :41-238: corner initialisation, refined-grid early exit, remap restart),
  * fouds1 (:248-332) / fouds2 (:342-518), the narrow-band binary tree addtree /
  * downtree / updtree (:527-680), bilinear (:690-711) and srtimes
- * (fm2dray_cartesian.f90:682-797).  Ray tracing (rpaths) is not part of the
- * phase-velocity path.
+ * (fm2dray_cartesian.f90:682-797).  For group velocities (phaseGroup = 1,
+ * uar = 0) it adds rpaths (:801-1420: the gradient ray from each receiver back
+ * to the source, no Frechet derivatives), the ray length (T_RAY%length,
+ * :51-65) and CalGroupTime / GetVelocity (src/likelihood_surf.F90:454-521:
+ * the group delay along the ray, bilinear group velocity at its points).
  *
  * Arrays are Fortran column-major: a(iz,ix) with iz fastest, 1-based in the
  * comments; the C indices below are 0-based.
@@ -361,11 +364,196 @@ static void bspl(double u, double w[4])
     w[3] = (u * u * u) / 6.0;
 }
 
+/* GetVelocity (src/likelihood_surf.F90:491-521): bilinear value of the grid
+ * vel(ny,nx) (C: gv[(ix-1)*ny + iy-1]) at a point, the cell clamped inside. */
+static double get_velocity(const double *gv, int nx, int ny, double xmin, double ymin, double dx,
+                           double dy, double px, double py)
+{
+    int ix = (int)floor((px - xmin) / dx) + 1;
+    int iy = (int)floor((py - ymin) / dy) + 1;
+    if (ix < 1) ix = 1;
+    if (iy < 1) iy = 1;
+    if (ix >= nx) ix = nx - 1;
+    if (iy >= ny) iy = ny - 1;
+    double dsx = px - (xmin + (ix - 1) * dx);
+    double dsy = py - (ymin + (iy - 1) * dy);
+    double qv = 0;
+    for (int i = 1; i <= 2; i++)
+        for (int j = 1; j <= 2; j++) {
+            double weight = (1.0 - fabs((i - 1) * dx - dsx) / dx) * (1.0 - fabs((j - 1) * dy - dsy) / dy);
+            qv = qv + weight * gv[(long)(ix + i - 2) * ny + (iy + j - 2)];
+        }
+    return qv;
+}
+
+/* The refined source grid as rpaths sees it (module variables goxr, gozr,
+ * dnxr, dnzr, nnxr, nnzr and the copies ttnr / nstsr made after the refined
+ * solve, fm2dray_cartesian.f90:352-353,395-400). */
+typedef struct {
+    int nnxr, nnzr, ld;
+    double goxr, gozr, dnxr, dnzr;
+    const double *ttnr;
+    const int *nstsr;
+} ray_refined;
+
+/* A ray's running sums: points (optional), length (ray_length, :51-65) and
+ * group delay (CalGroupTime's loop body, likelihood_surf.F90:480-489). */
+typedef struct {
+    int n, maxpts;
+    double *pts;
+    double px, py, len, gt, vhead;
+    const double *gv;
+    int nx, ny;
+    double xmin, ymin, dx, dy;
+} ray_acc;
+
+static void ray_point(ray_acc *A, double x, double y)
+{
+    if (A->pts && A->n < A->maxpts) { A->pts[2 * A->n] = x; A->pts[2 * A->n + 1] = y; }
+    double vt = A->gv ? get_velocity(A->gv, A->nx, A->ny, A->xmin, A->ymin, A->dx, A->dy, x, y) : 0.0;
+    if (A->n > 0) {
+        double dist = (x - A->px) * (x - A->px) + (y - A->py) * (y - A->py);
+        dist = sqrt(dist);
+        A->len = A->len + dist;
+        if (A->gv) A->gt = A->gt + dist * 2 / (A->vhead + vt);
+    }
+    A->vhead = vt;
+    A->px = x; A->py = y;
+    A->n++;
+}
+
+/* rpaths for one receiver (fm2dray_cartesian.f90:937-1397 with cfd = 0).
+ * S is the converged coarse grid.  Returns 0, 1 for a crazy ray (the point
+ * limit nnx*nnz was reached: the ray keeps only the receiver), or -2 for a
+ * receiver outside the model (the reference STOPs). */
+static int rpath_one(const fm2d_state *S, const ray_refined *R, int asgr, double scx, double scz,
+                     double rx, double rz, ray_acc *A)
+{
+    const int nnx = S->nnx, nnz = S->nnz;
+    const double gox = S->gox, goz = S->goz, dnx = S->dnx, dnz = S->dnz;
+    const long maxrp = (long)nnx * nnz;
+    double dpl = dnx, rd1 = dnz;
+    if (rd1 < dpl) dpl = rd1;
+    dpl = 0.5 * dpl;
+    int isx, isz;
+    if (asgr == 1) {
+        isx = (int)floor((scx - R->goxr) / R->dnxr) + 1;
+        isz = (int)floor((scz - R->gozr) / R->dnzr) + 1;
+    } else {
+        isx = (int)floor((scx - gox) / dnx) + 1;
+        isz = (int)floor((scz - goz) / dnz) + 1;
+    }
+#define TR(iz, ix) R->ttnr[(long)((ix) - 1) * R->ld + ((iz) - 1)]
+#define NR(iz, ix) R->nstsr[(long)((ix) - 1) * R->ld + ((iz) - 1)]
+    int ipx = (int)floor((rx - gox) / dnx) + 1;
+    int ipz = (int)floor((rz - goz) / dnz) + 1;
+    if (ipx < 1 || ipx >= nnx || ipz < 1 || ipz >= nnz) return -2;
+    double cx = rx, cz = rz;
+    ray_point(A, cx, cz);
+    int sw = 0;
+    double sred = (scx - cx) * (scx - cx);
+    sred = sred + (scz - cz) * (scz - cz);
+    sred = sqrt(sred);
+    if (sred < 2.0 * dpl) { ray_point(A, scx, scz); sw = 1; }
+    int ipxr = 0, ipzr = 0, igref = 0;
+    if (asgr == 1) {
+        ipxr = (int)floor((rx - R->goxr) / R->dnxr) + 1;
+        ipzr = (int)floor((rz - R->gozr) / R->dnzr) + 1;
+        igref = 1;
+        if (ipxr < 1 || ipxr >= R->nnxr) igref = 0;
+        if (ipzr < 1 || ipzr >= R->nnzr) igref = 0;
+        if (igref == 1) {
+            if (NR(ipzr, ipxr) != 0 || NR(ipzr + 1, ipxr) != 0) igref = 0;
+            if (NR(ipzr, ipxr + 1) != 0 || NR(ipzr + 1, ipxr + 1) != 0) igref = 0;
+        }
+    }
+    if (sw == 0) {
+        if (asgr == 1) {
+            if (igref == 1 && ipxr == isx && ipzr == isz) { ray_point(A, scx, scz); sw = 1; }
+        } else if (ipx == isx && ipz == isz) {
+            ray_point(A, scx, scz);
+            sw = 1;
+        }
+    }
+    for (long j = 1; j <= maxrp; j++) {
+        if (sw == 1) break;
+        double dtx, dtz;
+        if (igref == 1) {
+            if (ipxr == 1) dtx = (TR(ipzr, ipxr + 1) - TR(ipzr, ipxr)) / R->dnxr;
+            else if (ipxr == R->nnxr) dtx = (TR(ipzr, ipxr) - TR(ipzr, ipxr - 1)) / R->dnxr;
+            else dtx = (TR(ipzr, ipxr + 1) - TR(ipzr, ipxr - 1)) / (2.0 * R->dnxr);
+            if (ipzr == 1) dtz = (TR(ipzr + 1, ipxr) - TR(ipzr, ipxr)) / R->dnzr;
+            else if (ipzr == R->nnzr) dtz = (TR(ipzr, ipxr) - TR(ipzr - 1, ipxr)) / R->dnzr;
+            else dtz = (TR(ipzr + 1, ipxr) - TR(ipzr - 1, ipxr)) / (2.0 * R->dnzr);
+        } else {
+            if (ipx == 1) dtx = (TT(S, ipz, ipx + 1) - TT(S, ipz, ipx)) / dnx;
+            else if (ipx == nnx) dtx = (TT(S, ipz, ipx) - TT(S, ipz, ipx - 1)) / dnx;
+            else dtx = (TT(S, ipz, ipx + 1) - TT(S, ipz, ipx - 1)) / (2.0 * dnx);
+            /* the reference tests the refined-grid index here (:1090): a point in
+             * the refined grid's last row takes the one-sided difference */
+            if (ipz == 1) dtz = (TT(S, ipz + 1, ipx) - TT(S, ipz, ipx)) / dnz;
+            else if (asgr == 1 && ipzr == R->nnzr) dtz = (TT(S, ipz, ipx) - TT(S, ipz - 1, ipx)) / dnz;
+            else dtz = (TT(S, ipz + 1, ipx) - TT(S, ipz - 1, ipx)) / (2.0 * dnz);
+        }
+        rd1 = sqrt(dtx * dtx + dtz * dtz);
+        double nx_ = cx - dpl * dtx / rd1;
+        double nz_ = cz - dpl * dtz / rd1;
+        if (asgr == 1) {
+            ipxr = (int)floor((nx_ - R->goxr) / R->dnxr) + 1;
+            ipzr = (int)floor((nz_ - R->gozr) / R->dnzr) + 1;
+            igref = 1;
+            if (ipxr < 1 || ipxr >= R->nnxr) igref = 0;
+            if (ipzr < 1 || ipzr >= R->nnzr) igref = 0;
+            if (igref == 1) {
+                if (NR(ipzr, ipxr) != 0 || NR(ipzr + 1, ipxr) != 0) igref = 0;
+                if (NR(ipzr, ipxr + 1) != 0 || NR(ipzr + 1, ipxr + 1) != 0) igref = 0;
+            }
+        } else {
+            igref = 0;
+        }
+        ipx = (int)floor((nx_ - gox) / dnx) + 1;
+        ipz = (int)floor((nz_ - goz) / dnz) + 1;
+        sred = (scx - nx_) * (scx - nx_);
+        sred = sred + (scz - nz_) * (scz - nz_);
+        sred = sqrt(sred);
+        if (sred < 2.0 * dpl) { ray_point(A, nx_, nz_); ray_point(A, scx, scz); return 0; }
+        if (asgr == 1) {
+            if (igref == 1 && ipxr == isx && ipzr == isz) { ray_point(A, nx_, nz_); ray_point(A, scx, scz); return 0; }
+        } else if (ipx == isx && ipz == isz) {
+            ray_point(A, nx_, nz_);
+            ray_point(A, scx, scz);
+            return 0;
+        }
+        if (ipx < 1) { nx_ = gox; ipx = 1; }
+        if (ipx >= nnx) { nx_ = gox + (nnx - 1) * dnx; ipx = nnx - 1; }
+        if (ipz < 1) { nz_ = goz; ipz = 1; }
+        if (ipz >= nnz) { nz_ = goz + (nnz - 1) * dnz; ipz = nnz - 1; }
+        if (j == maxrp - 1) return 1;
+        ray_point(A, nx_, nz_);
+        cx = nx_; cz = nz_;
+    }
+#undef TR
+#undef NR
+    return 0;
+}
+
 int orc_fm2d(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvzd,
              const double *velv, int gdx, int gdz, int asgr, int sgdl, int sgs, int fom,
              double snb, int nsrc, const double *scx, const double *scz, int nrc,
              const double *rcx, const double *rcz, const int32_t *srs, double *ttime,
              double *tfield)
+{
+    return orc_fm2d_rays(nvx, nvz, goxd, gozd, dvxd, dvzd, velv, gdx, gdz, asgr, sgdl, sgs, fom, snb,
+                         nsrc, scx, scz, nrc, rcx, rcz, srs, ttime, tfield, 0, NULL, NULL, NULL, NULL,
+                         0, NULL, NULL);
+}
+
+int orc_fm2d_rays(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvzd,
+                  const double *velv, int gdx, int gdz, int asgr, int sgdl, int sgs, int fom,
+                  double snb, int nsrc, const double *scx, const double *scz, int nrc,
+                  const double *rcx, const double *rcz, const int32_t *srs, double *ttime,
+                  double *tfield, int rays, const double *gvel, double *gtime, double *rlen,
+                  int32_t *crazy, int maxpts, int32_t *npts, double *pts)
 {
     /* gridder: velv(0:nvz+1, 0:nvx+1) is the padded control grid */
     const int ldv = nvz + 2;
@@ -413,6 +601,18 @@ int orc_fm2d(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvz
     S.btg = (bp *)calloc((size_t)S.maxbt + 1, sizeof(bp));
     S.fom = fom;
     int rc = 0;
+    ray_refined RR;
+    memset(&RR, 0, sizeof RR);
+    RR.ld = ldm; RR.ttnr = ttnr; RR.nstsr = nstsr;
+    if (rays) {
+        /* CalGroupTime starts from zero times; rays not traced keep no points */
+        for (long q = 0; q < (long)nsrc * nrc; q++) {
+            if (gtime) gtime[q] = 0.0;
+            if (rlen) rlen[q] = 0.0;
+            if (npts) npts[q] = 0;
+        }
+        if (crazy) *crazy = 0;
+    }
     for (int i = 1; i <= nsrc; i++) {
         int any = 0;
         for (int j = 0; j < nrc; j++) any += srs[(i - 1) * nrc + j];
@@ -474,6 +674,7 @@ int orc_fm2d(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvz
                 }
             }
             if (travel(&S, x, z, 1)) { rc = -1; break; }
+            RR.nnxr = S.nnx; RR.nnzr = S.nnz; RR.goxr = S.gox; RR.gozr = S.goz; RR.dnxr = S.dnx; RR.dnzr = S.dnz;
             /* remap the refined grid onto the coarse grid */
             memcpy(ttnr, S.ttn, sizeof(double) * (size_t)ldm * ncm);
             memcpy(nstsr, S.nsts, sizeof(int) * (size_t)ldm * ncm);
@@ -557,6 +758,28 @@ int orc_fm2d(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvz
             ttime[(size_t)(i - 1) * nrc + (r - 1)] = trr;
         }
         if (rc) break;
+        if (rays) {
+            /* rpaths from every receiver of a used pair back to source i */
+            for (int r = 1; r <= nrc && !rc; r++) {
+                long q = (long)(i - 1) * nrc + (r - 1);
+                if (srs[q] == 0) continue;
+                ray_acc A;
+                memset(&A, 0, sizeof A);
+                A.maxpts = maxpts;
+                A.pts = pts ? pts + 2 * (size_t)maxpts * q : NULL;
+                A.gv = gvel; A.nx = nvx; A.ny = nvz; A.xmin = goxd; A.ymin = gozd; A.dx = dvxd; A.dy = dvzd;
+                int e = rpath_one(&S, &RR, asgr, x, z, rcx[r - 1], rcz[r - 1], &A);
+                if (e < 0) { rc = e; break; }
+                if (e == 1) {
+                    if (crazy) (*crazy)++;
+                    A.n = 1; A.len = 0.0; A.gt = 0.0;
+                }
+                if (npts) npts[q] = A.n;
+                if (rlen) rlen[q] = A.len;
+                if (gtime) gtime[q] = A.n >= 2 ? A.gt : 0.0;
+            }
+            if (rc) break;
+        }
     }
 #undef VV
     free(velnc); free(S.ttn); free(S.veln); free(S.nsts); free(ttnr); free(nstsr); free(S.btg);

@@ -67,6 +67,18 @@ int orc_fm2d(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvz
              double snb, int nsrc, const double *scx, const double *scz, int nrc,
              const double *rcx, const double *rcz, const int32_t *srs, double *ttime,
              double *tfield);
+/* orc_fm2d plus, when rays != 0, the group-velocity leg (uar = 0): rpaths from
+ * every receiver of a used pair, the ray length rlen (nsrc,nrc) and, when gvel
+ * (the group velocity grid, Fortran gvel(nvz,nvx)) is given, the group delay
+ * gtime (nsrc,nrc) (CalGroupTime); unused pairs get 0.  crazy counts rays that
+ * hit the point limit.  npts/pts (nsrc*nrc, maxpts, 2) receive the ray points
+ * when not NULL (a ray longer than maxpts keeps its first maxpts). */
+int orc_fm2d_rays(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvzd,
+                  const double *velv, int gdx, int gdz, int asgr, int sgdl, int sgs, int fom,
+                  double snb, int nsrc, const double *scx, const double *scz, int nrc,
+                  const double *rcx, const double *rcz, const int32_t *srs, double *ttime,
+                  double *tfield, int rays, const double *gvel, double *gtime, double *rlen,
+                  int32_t *crazy, int maxpts, int32_t *npts, double *pts);
 
 /* ---- (c) 3-D fast marching (SFMM) --------------------------------------- */
 /* Same signature and semantics as the reference C ABI fastMarching3D
@@ -104,14 +116,17 @@ int orc_body_misfit(int nsrc, int nrev, int np, const double *phase_time,
                     double *srdist, double *sigma, double *out);
 
 
-/* Surface-wave likelihood (datatype 2, phase velocities), surf.c.  Grid as
+/* Surface-wave likelihood (datatype 2), surf.c.  Grid as
  * T_GRID (dz scaled, scaling = grid%scaling); points/params (ncells,3);
  * freqs (np) Hz; raylov 1 Rayleigh, 0 Love; src (nsrc,2), rev (nrev,2) as
  * (x, y); ttime (np,3,nsrc*nrev) and raystat (np,2,nsrc*nrev) as T_DATA;
  * snoise0/1 (np).  Outputs: out {like, misfit, unweighted}, phase_time
  * (np,nsrc,nrev), srdist and sigma (np,nsrc*nrev), phase_vel (nx,ny,np) (may
- * be NULL), sites_id (nx,ny,nz) (may be NULL).  Returns 0, 1 (model
- * rejected: check_model or a dispersion failure), or < 0 (fatal). */
+ * be NULL), sites_id (nx,ny,nz) (may be NULL).  phase_group 1 computes group
+ * velocities and times: group delays along the rays (CalGroupTime), srdist =
+ * ray lengths; group_vel (nx,ny,np) may be NULL.  Returns 0, 1 (model
+ * rejected: check_model or a dispersion failure), 2 (a crazy ray), or < 0
+ * (fatal). */
 int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double zmin,
                         double dx, double dy, double dz, double scaling,
                         const double *points, const double *params, int ncells,
@@ -122,7 +137,8 @@ int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double
                         int gridx, int gridy, int sgref, int sgdic, int sgext, int order,
                         double band, int nthreads,
                         double *out, double *phase_time, double *srdist, double *sigma,
-                        double *phase_vel, int32_t *sites_id);
+                        double *phase_vel, int32_t *sites_id, int phase_group,
+                        double *group_vel);
 
 #ifdef __cplusplus
 }

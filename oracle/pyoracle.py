@@ -218,13 +218,56 @@ def fm2d(velv, goxd, gozd, dvxd, dvzd, scx, scz, rcx, rcz, srs=None, gdx=1, gdz=
     return (tt, tf) if field else tt
 
 
+def fm2d_rays(velv, goxd, gozd, dvxd, dvzd, scx, scz, rcx, rcz, srs=None, gvel=None, gdx=1, gdz=1,
+              asgr=1, sgdl=4, sgs=8, fom=1, snb=0.5, maxpts=4096, ref=False):
+    """modrays with ray tracing (uar = 0) for one period.  gvel: (nvx, nvy) C
+    array (= Fortran gvel(nvz,nvx)) or None.  Returns a dict: tt (nsrc, nrc)
+    phase times, npts (nsrc*nrc), pts (nsrc*nrc, maxpts, 2), crazy, and from
+    the oracle also rlen / gtime (nsrc, nrc).  ref=True runs the reference's own
+    modrays/rpaths (oracle/_ref) instead (no rlen / gtime)."""
+    velv = c64(velv)
+    nvx, nvz = velv.shape[0] - 2, velv.shape[1] - 2
+    scx, scz, rcx, rcz = c64(scx), c64(scz), c64(rcx), c64(rcz)
+    nsrc, nrc = scx.shape[0], rcx.shape[0]
+    if srs is None:
+        srs = np.ones(nsrc * nrc, np.int32)
+    srs = np.ascontiguousarray(srs, dtype=np.int32).reshape(-1)
+    tt = np.zeros((nsrc, nrc))
+    npts = np.zeros(nsrc * nrc, np.int32)
+    pts = np.zeros((nsrc * nrc, maxpts, 2))
+    cr = ctypes.c_int()
+    if ref:
+        R = globals()["ref"]()
+        if R is None:
+            return None
+        slot = np.where(srs != 0, np.arange(1, nsrc * nrc + 1), 0).astype(np.int32)
+        srs2 = np.ascontiguousarray(np.concatenate([srs, slot]), dtype=np.int32)
+        R.ref_modrays_rays(nsrc, _p(scx), _p(scz), nrc, _p(rcx), _p(rcz), _p(srs2), nvx, nvz,
+                           ctypes.c_double(goxd), ctypes.c_double(gozd), ctypes.c_double(dvxd),
+                           ctypes.c_double(dvzd), _p(velv), gdx, gdz, asgr, sgdl, sgs, fom,
+                           ctypes.c_double(snb), _p(tt), maxpts, _p(npts), _p(pts), ctypes.byref(cr))
+        return dict(tt=tt, npts=npts, pts=pts, crazy=cr.value)
+    gv = None if gvel is None else c64(gvel)
+    rlen = np.zeros((nsrc, nrc))
+    gt = np.zeros((nsrc, nrc))
+    rc = lib().orc_fm2d_rays(nvx, nvz, ctypes.c_double(goxd), ctypes.c_double(gozd),
+                             ctypes.c_double(dvxd), ctypes.c_double(dvzd), _p(velv), gdx, gdz, asgr,
+                             sgdl, sgs, fom, ctypes.c_double(snb), nsrc, _p(scx), _p(scz), nrc,
+                             _p(rcx), _p(rcz), _p(srs), _p(tt), None, 1, _p(gv), _p(gt), _p(rlen),
+                             ctypes.byref(cr), maxpts, _p(npts), _p(pts))
+    if rc:
+        raise RuntimeError("orc_fm2d_rays status %d" % rc)
+    return dict(tt=tt, npts=npts, pts=pts, crazy=cr.value, rlen=rlen, gtime=gt)
+
+
 # ---------------------------------------------------------------- surface likelihood
 def surf_likelihood(grid, points, params, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3,
                     sigdep=0, snoise0=None, snoise1=None, nrays=None, gridx=1, gridy=1, sgref=1,
-                    sgdic=4, sgext=8, order=1, band=0.5, nthreads=0):
+                    sgdic=4, sgext=8, order=1, band=0.5, nthreads=0, phase_group=0, group_vel=None):
     """orc_surf_likelihood.  grid: dict nx..dz (+ scaling).  Returns (rc, out(3),
     phase_time (np,nsrc,nrev), srdist, sigma (np,npair), phase_vel (nx,ny,np),
-    sites_id (nx,ny,nz))."""
+    sites_id (nx,ny,nz)).  phase_group=1: group times along rays; group_vel, an
+    (nx,ny,np) array, receives the group velocities."""
     nx, ny, nz = grid["nx"], grid["ny"], grid["nz"]
     pts, prm = c64(points), c64(params)
     fr = c64(freqs).reshape(-1)
@@ -254,5 +297,6 @@ def surf_likelihood(grid, points, params, freqs, src, rev, ttime, raystat, raylo
         ctypes.c_int(nsrc), _p(s), ctypes.c_int(nrev), _p(r), _p(tt), _p(rs), ctypes.c_int(sigdep),
         _p(s0), _p(s1), ctypes.c_int(nrays), ctypes.c_int(gridx), ctypes.c_int(gridy),
         ctypes.c_int(sgref), ctypes.c_int(sgdic), ctypes.c_int(sgext), ctypes.c_int(order),
-        ctypes.c_double(band), ctypes.c_int(nthreads), _p(out), _p(pt), _p(sd), _p(sg), _p(pv), _p(sid))
+        ctypes.c_double(band), ctypes.c_int(nthreads), _p(out), _p(pt), _p(sd), _p(sg), _p(pv), _p(sid),
+        ctypes.c_int(phase_group), _p(group_vel))
     return rc, out, pt, sd, sg, pv, sid

@@ -79,4 +79,45 @@ contains
         crazy = cr
     end subroutine
 
+    ! modrays with ray tracing (uar = 0, as surf_likelihood calls it for group
+    ! velocities, src/likelihood_surf.F90:245,311-320): phase times plus the
+    ! rays rpaths stores at slot srs(:,2) (the pair index, as read_times_3
+    ! sets dat%raystat(:,2,:)).  npts(slot) and pts(2, maxpts, slot) return the
+    ! ray points (the first maxpts of a longer ray).
+    subroutine ref_modrays_rays(nsrc, scx, scz, nrc, rcx, rcz, srs, nvx, nvz, goxd, gozd, dvxd, dvzd, &
+                                velv, gdx, gdz, asgr, sgdl, sgs, fom, snb, ttime, maxpts, npts, pts, crazy) &
+                                bind(C, name="ref_modrays_rays")
+        use m_fm2d, only : modrays, T_RAY
+        integer(c_int), value :: nsrc, nrc, nvx, nvz, gdx, gdz, asgr, sgdl, sgs, fom, maxpts
+        real(c_double), intent(in) :: scx(nsrc), scz(nsrc), rcx(nrc), rcz(nrc)
+        integer(c_int), intent(in) :: srs(nrc*nsrc, 2)
+        real(c_double), value :: goxd, gozd, dvxd, dvzd, snb
+        real(c_double), intent(in) :: velv(nvz+2, nvx+2)
+        real(c_double), intent(inout) :: ttime(nrc, nsrc)
+        integer(c_int), intent(out) :: npts(nrc*nsrc)
+        real(c_double), intent(out) :: pts(2, maxpts, nrc*nsrc)
+        integer(c_int), intent(out) :: crazy
+        type(T_RAY), allocatable :: rays(:)
+        integer :: cr, q, n, srsl(nrc*nsrc, 2)
+        real(8) :: vv(nvz+2, nvx+2)
+        allocate(rays(nrc*nsrc))
+        rays%srcid = 0
+        rays%revid = 0
+        rays%npoints = 0
+        cr = 0
+        srsl = srs
+        vv = velv
+        call modrays(nsrc, scx, scz, nrc, rcx, rcz, srsl, 0, nvx, nvz, goxd, gozd, dvxd, dvzd, vv, &
+                     gdx, gdz, asgr, sgdl, sgs, 6371.0d0, fom, snb, ttime, rays, cr, 0)
+        crazy = cr
+        pts = 0
+        do q = 1, nrc*nsrc
+            npts(q) = rays(q)%npoints
+            if (rays(q)%npoints > 0 .and. allocated(rays(q)%points)) then
+                n = min(rays(q)%npoints, maxpts)
+                pts(:, 1:n, q) = rays(q)%points(:, 1:n)
+            endif
+        enddo
+    end subroutine
+
 end module mct_ref_driver

@@ -9,6 +9,10 @@
  *   column (surfdisp.c) -> ierr reject (:212-224) -> like%vel padding
  *   (:259-264) -> modrays per period (fm2d.c) -> srdist = phaseTime (:329-330)
  *   -> sigma (:356-372) -> misfit (:374-404).
+ * With group velocities (phaseGroup = 1): modrays traces rays (uar = 0,
+ * :245), srdist = the ray lengths (:326-327), a crazy ray rejects the model
+ * (:341-345), and the group delays along the rays replace the phase times
+ * (CalGroupTime, :348-351).
  * OpenMP runs over columns and over periods, as the reference does.
  */
 #include "mct_oracle.h"
@@ -60,7 +64,7 @@ int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double
                         int gridx, int gridy, int sgref, int sgdic, int sgext, int order,
                         double band, int nthreads,
                         double *out, double *phase_time, double *srdist, double *sigma,
-                        double *phase_vel, int32_t *sites_id)
+                        double *phase_vel, int32_t *sites_id, int phase_group, double *group_vel)
 {
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -79,17 +83,17 @@ int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double
         for (int k = 1; k < nz; k++)
             if (vs[c * nz + k] < vs[c * nz]) { rc = 1; break; }
     double *pvel = phase_vel ? phase_vel : malloc(sizeof(double) * C * np);
-    for (long i = 0; i < C * np; i++) pvel[i] = 100.0;
+    double *gvel = group_vel ? group_vel : malloc(sizeof(double) * C * np);
+    for (long i = 0; i < C * np; i++) { pvel[i] = 100.0; gvel[i] = 100.0; }
     if (!rc) {
         int bad = 0;
         /* columns in Fortran order: c = iy + ny*ix; pvel (np, ny, nx) */
         #pragma omp parallel for schedule(dynamic, 16) reduction(|:bad)
         for (long c = 0; c < C; c++) {
             double th[SURF_NMAX + 1], a[SURF_NMAX + 1], b[SURF_NMAX + 1], r[SURF_NMAX + 1];
-            double grp[64];
             int nl = column_layers(vs + c * nz, nz, dz, scaling, th, a, b, r);
-            int ierr = orc_surfmodes(th, a, b, r, nl, freqs, np, raylov, 0, dphase,
-                                     pvel + c * np, grp, NULL);
+            int ierr = orc_surfmodes(th, a, b, r, nl, freqs, np, raylov, phase_group, dphase,
+                                     pvel + c * np, gvel + c * np, NULL);
             if (ierr) bad = 1;
         }
         if (bad) rc = 1;
@@ -109,20 +113,43 @@ int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double
         double *rcx = malloc(sizeof(double) * nrev), *rcz = malloc(sizeof(double) * nrev);
         for (int j = 0; j < nsrc; j++) { scx[j] = src[2 * j]; scz[j] = src[2 * j + 1]; }
         for (int k = 0; k < nrev; k++) { rcx[k] = rev[2 * k]; rcz[k] = rev[2 * k + 1]; }
-        int fatal = 0;
-        #pragma omp parallel for schedule(dynamic, 1) reduction(|:fatal)
+        /* group velocities per period as Fortran gvel(ny,nx) grids (like%gvel(i,:,:)) */
+        double *gvp = phase_group ? malloc(sizeof(double) * (size_t)np * C) : NULL;
+        if (gvp)
+            for (int p = 0; p < np; p++)
+                for (long c = 0; c < C; c++) gvp[(size_t)p * C + c] = gvel[c * np + p];
+        double *rlen = phase_group ? malloc(sizeof(double) * (size_t)np * npair) : NULL;
+        int fatal = 0, ncrazy = 0;
+        #pragma omp parallel for schedule(dynamic, 1) reduction(|:fatal) reduction(+:ncrazy)
         for (int p = 0; p < np; p++) {
-            int e = orc_fm2d(nx, ny, xmin, ymin, dx, dy, velv + (size_t)p * (nx + 2) * ldv, gridx, gridy,
+            int e;
+            if (phase_group) {
+                int32_t cr = 0;
+                double *tp = malloc(sizeof(double) * npair);
+                e = orc_fm2d_rays(nx, ny, xmin, ymin, dx, dy, velv + (size_t)p * (nx + 2) * ldv, gridx, gridy,
+                                  sgref, sgdic, sgext, order, band, nsrc, scx, scz, nrev, rcx, rcz,
+                                  raystat + (long)p * 2 * npair, tp, NULL, 1, gvp + (size_t)p * C,
+                                  phase_time + (long)p * npair, rlen + (long)p * npair, &cr, 0, NULL, NULL);
+                free(tp);
+                ncrazy += cr;
+            } else {
+                e = orc_fm2d(nx, ny, xmin, ymin, dx, dy, velv + (size_t)p * (nx + 2) * ldv, gridx, gridy,
                              sgref, sgdic, sgext, order, band, nsrc, scx, scz, nrev, rcx, rcz,
                              raystat + (long)p * 2 * npair, phase_time + (long)p * npair, NULL);
+            }
             if (e) fatal = 1;
         }
-        free(velv); free(scx); free(scz); free(rcx); free(rcz);
+        free(velv); free(scx); free(scz); free(rcx); free(rcz); free(gvp);
         if (fatal) rc = -6;
+        else if (ncrazy > 0) rc = 2;
+        if (!rc) {
+            /* srdist = phaseTime (phase velocities) or the ray length (group) */
+            for (long i = 0; i < (long)np * npair; i++) srdist[i] = phase_group ? rlen[i] : phase_time[i];
+        }
+        free(rlen);
     }
     if (!rc) {
-        /* srdist = phaseTime (phase velocities); sigma; misfit in the reference's loop order */
-        for (long i = 0; i < (long)np * npair; i++) srdist[i] = phase_time[i];
+        /* sigma; misfit in the reference's loop order */
         for (int i = 0; i < np; i++)
             for (long r = 0; r < npair; r++) {
                 long o = (long)i * npair + r;
@@ -154,5 +181,6 @@ int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double
     free(vp); free(vs); free(rho);
     if (!sites_id) free(sid);
     if (!phase_vel) free(pvel);
+    if (!group_vel) free(gvel);
     return rc;
 }

@@ -102,10 +102,45 @@ def fm2d_golden():
     np.savez_compressed(os.path.join(OUT, "fm2d_modrays.npz"), n=n, **arrays)
 
 
+def fm2d_rays_golden():
+    """rpaths ray points (fm2d/fm2dray_cartesian.f90:801-1420, uar = 0) for
+    one period on seeded grids, first and mixed order, refined source grid."""
+    rng = np.random.default_rng(20261206)
+    arrays = {}
+    n = 0
+    for trial in range(8):
+        nx, ny, h = int(rng.integers(12, 33)), int(rng.integers(12, 33)), 0.5
+        v = rng.uniform(2.5, 4.0, (nx, ny))
+        velv = np.zeros((nx + 2, ny + 2))
+        velv[1:-1, 1:-1] = v
+        velv[0], velv[-1] = velv[1], velv[-2]
+        velv[:, 0], velv[:, -1] = velv[:, 1], velv[:, -2]
+        L, W = (nx - 1) * h, (ny - 1) * h
+        ns, nr = int(rng.integers(1, 4)), int(rng.integers(3, 9))
+        src = np.c_[rng.uniform(h, L - h, ns), rng.uniform(h, W - h, ns)]
+        rec = np.c_[rng.uniform(0, L - 1e-6, nr), rng.uniform(0, W - 1e-6, nr)]
+        srs = (rng.random(ns * nr) < 0.85).astype(np.int32)
+        for fom in (0, 1):
+            R = po.fm2d_rays(velv, 0, 0, h, h, src[:, 0], src[:, 1], rec[:, 0], rec[:, 1],
+                             srs=srs, fom=fom, asgr=1, ref=True)
+            m = int(R["npts"].max())
+            arrays["velv_%d" % n] = velv
+            arrays["src_%d" % n] = src
+            arrays["rec_%d" % n] = rec
+            arrays["srs_%d" % n] = srs
+            arrays["fom_%d" % n] = np.array([fom])
+            arrays["npts_%d" % n] = R["npts"]
+            arrays["pts_%d" % n] = R["pts"][:, :m, :]
+            arrays["crazy_%d" % n] = np.array([R["crazy"]])
+            n += 1
+    np.savez_compressed(os.path.join(OUT, "fm2d_rpaths.npz"), n=n, **arrays)
+
+
 if __name__ == "__main__":
     if po.ref() is None:
         sys.exit("oracle/_ref/libmctref.so is not built (make -C oracle ref)")
     kdtree_golden()
     surfdisp_golden()
     fm2d_golden()
+    fm2d_rays_golden()
     print("wrote golden vectors to", OUT)

@@ -12,6 +12,7 @@
 // the launchers fm2d_fast.hip refers to live in fm2d.hip; not used here
 int mct_launch_fm2d_gridder(const mct_fm2d_args &, hipStream_t) { return 0; }
 int mct_launch_fm2d_solve(const mct_fm2d_args &, hipStream_t) { return 0; }
+int mct_launch_fm2d_rays(const mct_fm2d_args &, hipStream_t) { return 0; }
 
 int main(int argc, char **argv)
 {

@@ -1,0 +1,142 @@
+"""GPU surface-wave likelihood with group velocities (LIKE_SET%phaseGroup = 1)
+against the oracle: rays from every receiver of a used pair (rpaths), group
+delays along them (CalGroupTime) as the times, ray lengths as srdist.
+
+Gates: phase and group velocities within 1e-6 km/s, group times and ray
+lengths within 1e-4 relative, log-likelihood within 1e-6 relative.  Given the
+GPU's own velocities, the rays, lengths and group times are the oracle's bit
+for bit with first-order times and with the sequential 2-D method.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import mctomo_amd as mct
+from mctomo_amd import workloads as W
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gdict(g):
+    return dict(nx=g.nx, ny=g.ny, nz=g.nz, xmin=g.xmin, ymin=g.ymin, zmin=g.zmin, dx=g.dx,
+                dy=g.dy, dz=g.dz, scaling=g.scaling)
+
+
+def ctx_for(w, **kw):
+    return mct.SurfLikelihood(w.grid, w.freqs, w.src, w.rev, w.ttime, w.raystat, raylov=w.raylov,
+                              order=w.order, phase_group=1, **kw)
+
+
+def oracle_group(w, points=None, params=None, sigdep=0, s0=None, s1=None):
+    g = w.grid
+    gv = np.zeros((g.nx, g.ny, w.freqs.shape[0]))
+    res = po.surf_likelihood(gdict(g), w.points if points is None else points,
+                             w.params if params is None else params, w.freqs, w.src, w.rev, w.ttime,
+                             w.raystat, raylov=w.raylov, order=w.order, sigdep=sigdep, snoise0=s0,
+                             snoise1=s1, phase_group=1, group_vel=gv)
+    return res, gv
+
+
+def rays_given_velocities(w, pv, gv, p, fom):
+    g = w.grid
+    velv = np.pad(pv[:, :, p], 1, mode="edge")
+    return po.fm2d_rays(velv, g.xmin, g.ymin, g.dx, g.dy, w.src[:, 0], w.src[:, 1], w.rev[:, 0],
+                        w.rev[:, 1], srs=w.raystat[p, 0], fom=fom, gvel=gv[:, :, p], maxpts=1)
+
+
+@pytest.mark.parametrize("order,method", [(0, 0), (1, 0), (1, 1), (0, 1)])
+def test_group_call_matches_oracle(order, method):
+    w = W.surface_workload(n=41, nz=21, ncells=30, np_=10, nsrc=16, nrev=16, seed=20261202, order=order)
+    say("group plain", "order", order, "method", method)
+    L = ctx_for(w)
+    L.set_method(method)
+    rc, out, pt, sd, sg, pv = L.likelihood(w.points, w.params, arrays=True)
+    gvg = L.group_velocities()
+    (orc, oout, opt, osd, osg, opv, _), ogv = oracle_group(w)
+    assert rc == orc == 0, (rc, orc)
+    used = w.raystat[:, 0].reshape(pt.shape) == 1
+    dv = np.max(np.abs(pv - opv))
+    dg = np.max(np.abs(gvg - ogv))
+    dt = np.max(np.abs(pt[used] - opt[used]) / opt[used])
+    dlen = np.max(np.abs(sd - osd)[used.reshape(sd.shape)] / osd[used.reshape(sd.shape)])
+    dl = abs(out[0] - oout[0]) / abs(oout[0])
+    say("  phase vel %.3g km/s, group vel %.3g km/s, group time rel %.3g, ray length rel %.3g, like rel %.3g"
+        % (dv, dg, dt, dlen, dl))
+    assert dv <= 1e-6 and dg <= 1e-6
+    assert dt <= 1e-4 and dlen <= 1e-4
+    assert dl <= 1e-6
+    # group times are not phase times: the ray integral of the group slowness
+    (prc, _, ppt, *_rest) = po.surf_likelihood(gdict(w.grid), w.points, w.params, w.freqs, w.src, w.rev,
+                                               w.ttime, w.raystat, raylov=w.raylov, order=order)
+    assert np.max(np.abs(ppt[used] - pt[used]) / ppt[used]) > 1e-3
+    # given the GPU's velocities, the rays are the oracle's
+    exact = order == 0 or method == 1
+    for p in range(w.freqs.shape[0]):
+        R = rays_given_velocities(w, pv, gvg, p, order)
+        u = used[p]
+        if exact:
+            np.testing.assert_array_equal(pt[p][u], R["gtime"][u], err_msg="period %d" % p)
+            np.testing.assert_array_equal(sd[p].reshape(pt[p].shape)[u], R["rlen"][u])
+        else:
+            assert np.max(np.abs(pt[p][u] - R["gtime"][u]) / R["gtime"][u]) <= 1e-4
+    L.close()
+
+
+def test_group_batch_equals_plain_calls():
+    base = W.surface_workload(n=33, nz=17, L=16.0, Z=8.0, ncells=40, np_=6, nsrc=8, nrev=8, seed=21)
+    nchain, maxc = 4, 40
+    rng = np.random.default_rng(4)
+    pts = np.zeros((nchain, maxc, 3))
+    prm = np.zeros((nchain, maxc, 3))
+    ncl = np.zeros(nchain, np.int32)
+    for c in range(nchain):
+        k = int(rng.integers(10, maxc + 1))
+        w = W.surface_workload(n=33, nz=17, L=16.0, Z=8.0, ncells=k, np_=6, nsrc=8, nrev=8, seed=100 + c)
+        pts[c, :k], prm[c, :k], ncl[c] = w.points, w.params, k
+    L = ctx_for(base, max_cells=maxc, max_chains=nchain)
+    say("group batch", nchain)
+    dP, dQ = mct.DeviceArray.from_numpy(pts), mct.DeviceArray.from_numpy(prm)
+    dN = mct.DeviceArray.from_numpy(ncl)
+    dS = mct.DeviceArray.from_numpy(np.zeros((nchain, 2, 6)))
+    dO = mct.DeviceArray((nchain, 3), np.float64)
+    dT = mct.DeviceArray((nchain,), np.int32)
+    L.likelihood_batch_dev(nchain, dP.ptr, dQ.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
+    L.sync()
+    out, st = dO.numpy(), dT.numpy()
+    for c in range(nchain):
+        k = ncl[c]
+        rc, o1 = L.likelihood(pts[c, :k], prm[c, :k])
+        say("chain", c, "status", int(st[c]), rc, "like", out[c][0], o1[0])
+        assert st[c] == rc, (c, st[c], rc)
+        if rc == 0:
+            np.testing.assert_array_equal(out[c], o1)
+            (orc, oout, *_), _ = oracle_group(base, points=pts[c, :k], params=prm[c, :k])
+            assert orc == 0 and abs(out[c, 0] - oout[0]) <= 1e-6 * abs(oout[0])
+    L.close()
+
+
+def test_group_noise_likelihood_uses_ray_lengths():
+    """noise_likelihood in group mode: sigma = snoise0 * ray length + snoise1
+    (srdist holds the lengths), equal to a full call with the new noise."""
+    w = W.surface_workload(n=41, nz=21, ncells=30, np_=10, nsrc=16, nrev=16, seed=20261202, order=1)
+    npp = w.freqs.shape[0]
+    L = ctx_for(w, sigdep=1)
+    s0, s1 = np.full(npp, 0.01), np.full(npp, 0.05)
+    rc, full = L.likelihood(w.points, w.params, snoise0=s0, snoise1=s1)
+    assert rc == 0
+    t0, t1 = np.full(npp, 0.02), np.full(npp, 0.03)
+    rc2, full2 = L.likelihood(w.points, w.params, snoise0=t0, snoise1=t1)
+    rc3, cheap = L.noise_likelihood(t0, t1)
+    assert rc3 == 0 and np.array_equal(cheap[:2], full2[:2])
+    (orc, oout, *_), _ = oracle_group(w, sigdep=1, s0=t0, s1=t1)
+    assert orc == 0
+    say("group noise like", cheap[0], "oracle", oout[0])
+    assert abs(cheap[0] - oout[0]) <= 1e-6 * abs(oout[0])
+    L.close()
