@@ -203,6 +203,31 @@ def run_surface(args):
                 "proposal": "Vs of each chain's deepest cell + 0.05 km/s; box = that cell's Voronoi region"}
         log("perturbed box:", pbox)
 
+    # ---- group velocities (phaseGroup = 1, a secondary figure) -----------------
+    # The same chains with group times: surfdisp96's group pass, rays from every
+    # receiver (rpaths) and the group delays along them (CalGroupTime).
+    group = None
+    if args.group_steps > 0:
+        lg = mct.SurfLikelihood(w.grid, w.freqs, w.src, w.rev, w.ttime, w.raystat, raylov=1,
+                                order=args.order, max_cells=cmax, max_chains=B, phase_group=1)
+        lg.set_method(args.fm2d_method)
+
+        def run_group():
+            lg.likelihood_batch_dev(B, dP.ptr, dQ.ptr, dN.ptr, dS0.ptr, dO.ptr, dS.ptr)
+            lg.sync()
+
+        run_group()
+        t0 = time.perf_counter()
+        for _ in range(args.group_steps):
+            run_group()
+        dtg = time.perf_counter() - t0
+        sg_ = dS.numpy()
+        group = {"evals_per_s": round(B * args.group_steps / dtg, 3), "steps": args.group_steps,
+                 "statuses": {str(int(v)): int((sg_ == v).sum()) for v in np.unique(sg_)},
+                 "stage_ms": [round(v, 3) for v in lg.timings()]}
+        log("group velocities:", group)
+        lg.close()
+
     # ---- served: Poisson arrivals into the continuous-batching server -----------
     served = None
     if args.served > 0:
@@ -337,6 +362,7 @@ def run_surface(args):
         "plain_stage_ms": [round(v, 3) for v in st_plain],
         "served": served,
         "perturbed_box": pbox,
+        "group_velocity": group,
         "roofline": roof,
         "cpu_baseline": cpu,
         "max_error_vs_oracle": err,
@@ -442,6 +468,8 @@ def main():
     ap.add_argument("--served", type=int, default=48, help="requests for the served measurement (0: skip)")
     ap.add_argument("--box-steps", type=int, default=3,
                     help="timed perturbed-box batches (secondary unit; 0: skip)")
+    ap.add_argument("--group-steps", type=int, default=2,
+                    help="timed group-velocity batches (secondary figure; 0: skip)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()

@@ -167,6 +167,15 @@ module mctomo_hip
             integer(c_int) :: mct_surf_noise_likelihood
         end function
 
+        ! like%gvel after a call with phase_group = 1 (src/likelihood_surf.F90:227-229)
+        function mct_surf_group_velocities(ctx, chain, group_vel) &
+                bind(C, name="mct_surf_group_velocities")
+            import
+            type(c_ptr), value :: ctx, group_vel
+            integer(c_int), value :: chain
+            integer(c_int) :: mct_surf_group_velocities
+        end function
+
     end interface
 
 end module mctomo_hip

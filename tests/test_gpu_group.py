@@ -140,3 +140,64 @@ def test_group_noise_likelihood_uses_ray_lengths():
     say("group noise like", cheap[0], "oracle", oout[0])
     assert abs(cheap[0] - oout[0]) <= 1e-6 * abs(oout[0])
     L.close()
+
+
+def _changed_columns(w, prm_old, prm_new):
+    g = gdict(w.grid)
+    _, _, vs_a, _ = po.kdtree_to_grid(g, w.points, prm_old)
+    _, _, vs_b, _ = po.kdtree_to_grid(g, w.points, prm_new)
+    return np.any(vs_a != vs_b, axis=2)
+
+
+def _bump(prm, k, dv):
+    p = prm.copy()
+    p[k, 1] += dv
+    p[k, 0] = 1.73 * p[k, 1]
+    return p
+
+
+def test_group_perturbed_box_call_equals_the_whole_grid_call():
+    """Perturbed-box calls keep the accepted group velocities outside the box
+    as like%gvel does (likelihood_surf.F90:227-229): the box call equals the
+    whole-grid call bit for bit; a box that misses the change keeps the
+    accepted group velocities there (negative control)."""
+    w = W.surface_workload(n=41, nz=21, ncells=30, np_=10, nsrc=16, nrev=16, seed=20261202)
+    L = ctx_for(w)
+    rc, _ = L.likelihood(w.points, w.params)
+    assert rc == 0
+    L.accept(0)
+    deep = np.argsort(-w.points[:, 2])[:15]
+    area = [_changed_columns(w, w.params, _bump(w.params, k, 0.1)).sum() for k in deep]
+    k = int(deep[int(np.argmin(area))])
+    B = _bump(w.params, k, 0.15)
+    chg = _changed_columns(w, w.params, B)
+    ix, iy = np.nonzero(chg)
+    g = w.grid
+    e = 1e-9
+    box = [g.xmin + ix.min() * g.dx + e, g.ymin + iy.min() * g.dy + e, 0.0,
+           g.xmin + ix.max() * g.dx + e, g.ymin + iy.max() * g.dy + e, 1.0]
+    rb, ob, ptb, sdb, _, pvb = L.likelihood_box(w.points, B, box, arrays=True)
+    gvb = L.group_velocities()
+    rf, of, ptf, sdf, _, pvf = L.likelihood(w.points, B, arrays=True)
+    gvf = L.group_velocities()
+    say("group box", L.box_columns(box), "like", ob[0], of[0])
+    assert rb == rf == 0
+    np.testing.assert_array_equal(gvb, gvf)
+    np.testing.assert_array_equal(pvb, pvf)
+    np.testing.assert_array_equal(ptb, ptf)
+    np.testing.assert_array_equal(sdb, sdf)
+    np.testing.assert_array_equal(ob, of)
+    far = None
+    for x0 in (0.0, 17.5):
+        for y0 in (0.0, 17.5):
+            bx = [x0 + 1e-9, y0 + 1e-9, 0.0, x0 + 2.0, y0 + 2.0, 1.0]
+            c = L.box_columns(bx)
+            if not chg[c[0] - 1:c[1], c[2] - 1:c[3]].any():
+                far = bx
+    assert far is not None
+    L.likelihood_box(w.points, B, far)
+    gvm = L.group_velocities()
+    assert not np.array_equal(gvm, gvf)
+    # outside the changed columns the kept group velocities are the accepted ones
+    np.testing.assert_array_equal(gvm[~chg], gvf[~chg])
+    L.close()
