@@ -10,6 +10,7 @@ module mctomo_hip
     integer(c_int), parameter :: MCT_OK = 0, MCT_REJECT_MODEL = 1, MCT_REJECT_RAY = 2
     integer(c_int), parameter :: MCT_ERR_ARG = -1, MCT_ERR_HIP = -2, MCT_ERR_NOMEM = -3
     integer(c_int), parameter :: MCT_ERR_SIGMA = -4, MCT_ERR_NOCONV = -5, MCT_ERR_OUTSIDE = -6
+    integer(c_int), parameter :: MCT_ERR_IO = -7
 
     ! T_GRID (src/settings.f90:20-28), zmin and dz scaled
     type, bind(C) :: mct_grid
@@ -165,6 +166,45 @@ module mctomo_hip
             import
             type(c_ptr), value :: ctx, snoise0, snoise1, out
             integer(c_int) :: mct_surf_noise_likelihood
+        end function
+
+        ! data files: read_sources / read_times_3 (src/likelihood_settings.f90:198-420),
+        ! write_sources, write_atimes (src/read_write.f90:484-530); paths are
+        ! NUL-terminated character(kind=c_char) strings
+        function mct_read_sources(path, bnd, cap, n, ndim, src) bind(C, name="mct_read_sources")
+            import
+            character(kind=c_char), dimension(*) :: path
+            type(c_ptr), value :: bnd, src
+            integer(c_int), value :: cap
+            integer(c_int) :: n, ndim
+            integer(c_int) :: mct_read_sources
+        end function
+
+        function mct_read_times(path, nsrc, nrev, cap_np, np, freqs, ttime, raystat, nrays) &
+                bind(C, name="mct_read_times")
+            import
+            character(kind=c_char), dimension(*) :: path
+            integer(c_int), value :: nsrc, nrev, cap_np
+            integer(c_int) :: np, nrays
+            type(c_ptr), value :: freqs, ttime, raystat
+            integer(c_int) :: mct_read_times
+        end function
+
+        function mct_write_sources(path, n, ndim, src) bind(C, name="mct_write_sources")
+            import
+            character(kind=c_char), dimension(*) :: path
+            integer(c_int), value :: n, ndim
+            type(c_ptr), value :: src
+            integer(c_int) :: mct_write_sources
+        end function
+
+        function mct_write_times(path, np, freqs, nsrc, nrev, raystat, phase_time, sigma, locations) &
+                bind(C, name="mct_write_times")
+            import
+            character(kind=c_char), dimension(*) :: path
+            integer(c_int), value :: np, nsrc, nrev
+            type(c_ptr), value :: freqs, raystat, phase_time, sigma, locations
+            integer(c_int) :: mct_write_times
         end function
 
         ! like%gvel after a call with phase_group = 1 (src/likelihood_surf.F90:227-229)

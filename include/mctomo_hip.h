@@ -32,7 +32,8 @@ enum {
     MCT_ERR_NOMEM = -3,       /* device allocation failed */
     MCT_ERR_SIGMA = -4,       /* noise level below EPS ('The noise level is 0!') */
     MCT_ERR_NOCONV = -5,      /* eikonal solver hit its iteration cap */
-    MCT_ERR_OUTSIDE = -6      /* source or receiver outside the grid */
+    MCT_ERR_OUTSIDE = -6,     /* source or receiver outside the grid */
+    MCT_ERR_IO = -7           /* a data file could not be opened, read or written */
 };
 
 /* T_GRID, src/settings.f90:20-28.  zmin/dz are the SCALED values held by the
@@ -157,6 +158,33 @@ int mct_body_noise_likelihood(mct_ctx *ctx, const double *locations, const doubl
  * (nchain, nsrc, 4) or NULL, noise_dev (nchain, 2, np); asynchronous. */
 int mct_body_noise_likelihood_batch_dev(mct_ctx *ctx, int nchain, const double *locations_dev,
                                         const double *noise_dev, double *out_dev, int32_t *status_dev);
+
+/* ---- data files (host only) ------------------------------------------------ */
+/* read_sources (src/likelihood_settings.f90:198-228): "nsrc ndim", then one
+ * record of ndim values per source, Fortran list-directed.  With src NULL only
+ * n and ndim are returned; otherwise src (n, ndim) (= Fortran src(ndim,n)) of
+ * capacity cap sources.  bnd (2*ndim: min, max per coordinate) may be NULL;
+ * a point outside it is MCT_ERR_OUTSIDE (out_bnd, src/settings.f90:143-154). */
+int mct_read_sources(const char *path, const double *bnd, int cap, int *n, int *ndim, double *src);
+/* read_times_3 (src/likelihood_settings.f90:332-420) as read_data calls it (no
+ * ds file): the number of periods (or wave types) np and their values, then per
+ * (source, receiver) pair a validity flag and, for a valid pair, np records
+ * "t sigma".  freqs (np), ttime (np,3,nsrc*nrev) = Fortran ttime(npair,3,np),
+ * raystat (np,2,nsrc*nrev) = Fortran raystat(npair,2,np); nrays.  A used pair
+ * (t /= -1) gets raystat = (1, pair index); an invalid pair ttime = (-1, 1, 0).
+ * With freqs, ttime or raystat NULL only np is returned (cap_np is the
+ * arrays' period capacity). */
+int mct_read_times(const char *path, int nsrc, int nrev, int cap_np, int *np, double *freqs, double *ttime,
+                   int32_t *raystat, int *nrays);
+/* write_sources (src/likelihood_settings.f90:230-257). */
+int mct_write_sources(const char *path, int n, int ndim, const double *src);
+/* write_atimes (src/read_write.f90:484-530), the file write_data writes: np,
+ * the freqs, then per pair 1 and np records "time sigma" when any period uses
+ * the pair (time + origin time locations(4,src) on used periods when
+ * locations (nsrc,4) is given), else 0.  phase_time (np,nsrc,nrev), sigma
+ * (np,nsrc*nrev), raystat as in mct_read_times.  Readable by read_times_3. */
+int mct_write_times(const char *path, int np, const double *freqs, int nsrc, int nrev, const int32_t *raystat,
+                    const double *phase_time, const double *sigma, const double *locations);
 
 /* Device memory for callers that hold no device allocator of their own (a
  * Fortran host, or Python without a GPU framework): plain hipMalloc / hipFree

@@ -19,6 +19,7 @@ _ERRORS = {
     -4: "The noise level is 0!",
     -5: "eikonal solver did not converge",
     -6: "source or receiver outside the grid",
+    -7: "data file could not be opened, read or written",
 }
 
 
@@ -69,7 +70,8 @@ def lib():
                      "mct_body_noise_likelihood", "mct_body_noise_likelihood_batch_dev",
                      "mct_surf_noise_likelihood", "mct_surf_noise_likelihood_batch_dev",
                      "mct_surf_likelihood_box", "mct_surf_box_columns", "mct_surf_likelihood_box_batch_dev",
-                     "mct_surf_accept", "mct_surf_accept_batch", "mct_surf_group_velocities"):
+                     "mct_surf_accept", "mct_surf_accept_batch", "mct_surf_group_velocities",
+                     "mct_read_sources", "mct_read_times", "mct_write_sources", "mct_write_times"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -77,6 +79,65 @@ def lib():
 
 def version():
     return lib().mct_version().decode()
+
+
+# ---------------------------------------------------------------- data files
+def _path(p):
+    return os.fsencode(os.fspath(p))
+
+
+def read_sources(path, bnd=None):
+    """read_sources (src/likelihood_settings.f90:198-228): (n, ndim) array.
+    bnd: (min, max) per coordinate, or None."""
+    n, nd = ctypes.c_int(), ctypes.c_int()
+    _check(lib().mct_read_sources(_path(path), None, 0, ctypes.byref(n), ctypes.byref(nd), None),
+           "mct_read_sources")
+    out = np.zeros((n.value, nd.value))
+    b = None if bnd is None else _f64(bnd).reshape(-1)
+    _check(lib().mct_read_sources(_path(path), _p(b), n.value, ctypes.byref(n), ctypes.byref(nd), _p(out)),
+           "mct_read_sources")
+    return out
+
+
+def read_times(path, nsrc, nrev):
+    """read_times_3 (src/likelihood_settings.f90:332-420): returns (freqs (np),
+    ttime (np,3,nsrc*nrev), raystat (np,2,nsrc*nrev), nrays)."""
+    npp = ctypes.c_int()
+    _check(lib().mct_read_times(_path(path), int(nsrc), int(nrev), 0, ctypes.byref(npp), None, None, None, None),
+           "mct_read_times")
+    k = npp.value
+    fr = np.zeros(k)
+    tt = np.zeros((k, 3, nsrc * nrev))
+    rs = np.zeros((k, 2, nsrc * nrev), np.int32)
+    nr = ctypes.c_int()
+    _check(lib().mct_read_times(_path(path), int(nsrc), int(nrev), k, ctypes.byref(npp), _p(fr), _p(tt), _p(rs),
+                                ctypes.byref(nr)), "mct_read_times")
+    return fr, tt, rs, nr.value
+
+
+def read_data(sources_file, receivers_file, data_file):
+    """read_data for one data set (src/likelihood_settings.f90:166-196):
+    (src, rev, freqs, ttime, raystat, nrays)."""
+    src = read_sources(sources_file)
+    rev = read_sources(receivers_file)
+    return (src, rev) + read_times(data_file, src.shape[0], rev.shape[0])
+
+
+def write_sources(path, src):
+    a = _f64(src)
+    _check(lib().mct_write_sources(_path(path), a.shape[0], a.shape[1], _p(a)), "mct_write_sources")
+
+
+def write_times(path, freqs, raystat, phase_time, sigma, locations=None):
+    """write_atimes (src/read_write.f90:484-530).  phase_time (np,nsrc,nrev),
+    sigma (np,nsrc*nrev), raystat (np,2,nsrc*nrev), locations (nsrc,4) or None."""
+    fr = _f64(freqs).reshape(-1)
+    pt = _f64(phase_time)
+    sg = _f64(sigma)
+    rs = np.ascontiguousarray(raystat, dtype=np.int32)
+    loc = None if locations is None else _f64(locations)
+    _check(lib().mct_write_times(_path(path), fr.shape[0], _p(fr), pt.shape[1], pt.shape[2], _p(rs), _p(pt),
+                                 _p(sg), _p(loc)), "mct_write_times")
 
 
 def _p(a):

@@ -260,6 +260,28 @@ def fm2d_rays(velv, goxd, gozd, dvxd, dvzd, scx, scz, rcx, rcz, srs=None, gvel=N
     return dict(tt=tt, npts=npts, pts=pts, crazy=cr.value, rlen=rlen, gtime=gt)
 
 
+# ---------------------------------------------------------------- data files
+def ref_read_data(datatype, sources_file, receivers_file, data_file):
+    """read_data (src/likelihood_settings.f90:166-196) from the reference's own
+    compiled Fortran: dict src (n,ndim), rev, freqs, ttime (np,3,npair), raystat
+    (np,2,npair), nrays, nmodes; None where oracle/_ref is not built."""
+    R = globals()["ref"]()
+    if R is None:
+        return None
+    f = [os.fsencode(os.fspath(x)) + b"\0" for x in (sources_file, receivers_file, data_file)]
+    sz = np.zeros(7, np.int32)
+    z = np.zeros(1)
+    zi = np.zeros(1, np.int32)
+    R.ref_read_data(int(datatype), f[0], f[1], f[2], 0, 0, 0, 0, _p(sz), _p(z), _p(z), _p(z), _p(z), _p(zi))
+    ns, ds, nr, dr, npp, nrays, nmodes = (int(v) for v in sz)
+    npair = ns * nr
+    src, rev, fr = np.zeros((ns, ds)), np.zeros((nr, dr)), np.zeros(npp)
+    tt, rs = np.zeros((npp, 3, npair)), np.zeros((npp, 2, npair), np.int32)
+    R.ref_read_data(int(datatype), f[0], f[1], f[2], ns, nr, npp, npair, _p(sz), _p(src), _p(rev), _p(fr),
+                    _p(tt), _p(rs))
+    return dict(src=src, rev=rev, freqs=fr, ttime=tt, raystat=rs, nrays=nrays, nmodes=nmodes)
+
+
 # ---------------------------------------------------------------- surface likelihood
 def surf_likelihood(grid, points, params, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3,
                     sigdep=0, snoise0=None, snoise1=None, nrays=None, gridx=1, gridy=1, sgref=1,

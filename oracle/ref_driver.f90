@@ -120,4 +120,70 @@ contains
         enddo
     end subroutine
 
+    ! read_data (src/likelihood_settings.f90:166-196) for one data set:
+    ! datatype 2 reads the surface-wave files, 0 the body-wave files.  Paths
+    ! are NUL-terminated.  Sizes come back first (cap_* = 0), then the arrays:
+    ! src (ndims, cap_src), rev (ndimr, cap_rev), freqs (cap_np),
+    ! ttime (cap_pair, 3, cap_np), raystat (cap_pair, 2, cap_np).
+    subroutine ref_read_data(datatype, fsrc, frev, fdat, cap_src, cap_rev, cap_np, cap_pair, &
+                             sizes, src, rev, freqs, ttime, raystat) bind(C, name="ref_read_data")
+        use like_settings, only : T_DATA, T_LIKE_SET, read_data
+        integer(c_int), value :: datatype, cap_src, cap_rev, cap_np, cap_pair
+        character(kind=c_char), intent(in) :: fsrc(*), frev(*), fdat(*)
+        integer(c_int), intent(out) :: sizes(7)
+        real(c_double), intent(out) :: src(*), rev(*), freqs(*), ttime(*)
+        integer(c_int), intent(out) :: raystat(*)
+        type(T_DATA) :: dat(1)
+        type(T_LIKE_SET) :: ls
+        integer :: ns, nr, np, npair, ds, dr, i, j, k, o
+        ls%datatype = datatype
+        if (datatype == 2) then
+            ls%ssources_file = cstr(fsrc); ls%sreceivers_file = cstr(frev); ls%sdata_file = cstr(fdat)
+        else
+            ls%bsources_file = cstr(fsrc); ls%breceivers_file = cstr(frev); ls%bdata_file = cstr(fdat)
+        endif
+        call read_data(dat, ls)
+        ds = size(dat(1)%src, 1); ns = size(dat(1)%src, 2)
+        dr = size(dat(1)%rev, 1); nr = size(dat(1)%rev, 2)
+        np = dat(1)%np; npair = ns * nr
+        sizes = [ns, ds, nr, dr, np, dat(1)%nrays, dat(1)%nmodes]
+        if (cap_src < ns .or. cap_rev < nr .or. cap_np < np .or. cap_pair < npair) return
+        o = 0
+        do i = 1, ns
+            do j = 1, ds
+                o = o + 1; src(o) = dat(1)%src(j, i)
+            enddo
+        enddo
+        o = 0
+        do i = 1, nr
+            do j = 1, dr
+                o = o + 1; rev(o) = dat(1)%rev(j, i)
+            enddo
+        enddo
+        freqs(1:np) = dat(1)%freqs(1:np)
+        do k = 1, np
+            do j = 1, 3
+                do i = 1, npair
+                    ttime(i + (j - 1) * npair + (k - 1) * 3 * npair) = dat(1)%ttime(i, j, k)
+                enddo
+            enddo
+            do j = 1, 2
+                do i = 1, npair
+                    raystat(i + (j - 1) * npair + (k - 1) * 2 * npair) = dat(1)%raystat(i, j, k)
+                enddo
+            enddo
+        enddo
+    end subroutine
+
+    function cstr(c) result(f)
+        character(kind=c_char), intent(in) :: c(*)
+        character(len=100) :: f
+        integer :: i
+        f = ' '
+        do i = 1, 100
+            if (c(i) == c_null_char) exit
+            f(i:i) = c(i)
+        enddo
+    end function
+
 end module mct_ref_driver

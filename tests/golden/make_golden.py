@@ -136,6 +136,28 @@ def fm2d_rays_golden():
     np.savez_compressed(os.path.join(OUT, "fm2d_rpaths.npz"), n=n, **arrays)
 
 
+# the reference's example data files, copied into tests/golden/formats/
+FORMAT_SETS = [
+    ("example2 surface", 2, "ssources.dat", "sreceivers.dat", "stimes.dat"),
+    ("example2 body", 0, "bsources.dat", "breceivers.dat", "btimes.dat"),
+    ("example1", 2, "ex1_sources.dat", "ex1_receivers.dat", "ex1_otimes.dat"),
+    ("2dExample", 2, "ex2d_sources.dat", "ex2d_receivers.dat", "ex2d_otimes_noise.dat"),
+]
+
+
+def formats_golden():
+    """read_data (src/likelihood_settings.f90:166-196, read_sources and
+    read_times_3) on the reference's example data files."""
+    d = os.path.join(OUT, "formats")
+    arrays = {}
+    for i, (_, dt, a, b, c) in enumerate(FORMAT_SETS):
+        R = po.ref_read_data(dt, os.path.join(d, a), os.path.join(d, b), os.path.join(d, c))
+        for k in ("src", "rev", "freqs", "ttime", "raystat"):
+            arrays["%s_%d" % (k, i)] = R[k]
+        arrays["nrays_%d" % i] = np.array([R["nrays"]])
+    np.savez_compressed(os.path.join(OUT, "formats_read_data.npz"), n=len(FORMAT_SETS), **arrays)
+
+
 if __name__ == "__main__":
     if po.ref() is None:
         sys.exit("oracle/_ref/libmctref.so is not built (make -C oracle ref)")
@@ -143,4 +165,5 @@ if __name__ == "__main__":
     surfdisp_golden()
     fm2d_golden()
     fm2d_rays_golden()
+    formats_golden()
     print("wrote golden vectors to", OUT)
