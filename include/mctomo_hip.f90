@@ -24,6 +24,7 @@ module mctomo_hip
         integer(c_int) :: datatype, order, sigdep, nsrc, nrev
         type(c_ptr) :: rev, ttime, raystat
         integer(c_int) :: nrays, max_cells, max_chains
+        integer(c_int) :: dynamic
     end type
 
     type, bind(C) :: mct_surf_setup
@@ -159,6 +160,14 @@ module mctomo_hip
             import
             type(c_ptr), value :: ctx, locations, bnoise0, bnoise1, out
             integer(c_int) :: mct_body_noise_likelihood
+        end function
+
+        ! location_likelihood with dynamic /= 0 (src/likelihood.f90:162-191)
+        function mct_body_location_likelihood(ctx, locations, bnoise0, bnoise1, out) &
+                bind(C, name="mct_body_location_likelihood")
+            import
+            type(c_ptr), value :: ctx, locations, bnoise0, bnoise1, out
+            integer(c_int) :: mct_body_location_likelihood
         end function
 
         function mct_surf_noise_likelihood(ctx, snoise0, snoise1, out) &

@@ -116,6 +116,9 @@ typedef struct {
     int nrays;                /* dat%nrays */
     int max_cells;            /* largest Voronoi site count a call may pass */
     int max_chains;           /* largest batch for the _batch_dev entry */
+    int dynamic;              /* LIKE_SET%dynamic: != 0 traces from the receivers and keeps
+                                 their traveltime fields for mct_body_location_likelihood
+                                 (src/likelihood_body.F90:250-279) */
 } mct_body_setup;
 
 /* like_setup for body data (src/likelihood.f90:31-55): uploads the data to
@@ -185,6 +188,21 @@ int mct_write_sources(const char *path, int n, int ndim, const double *src);
  * (np,nsrc*nrev), raystat as in mct_read_times.  Readable by read_times_3. */
 int mct_write_times(const char *path, int np, const double *freqs, int nsrc, int nrev, const int32_t *raystat,
                     const double *phase_time, const double *sigma, const double *locations);
+
+/* location_likelihood with dynamic != 0 (src/likelihood.f90:162-191 ->
+ * body_location_likelihood, src/likelihood_body.F90:82-149): the times from the
+ * receivers' fields of the context's last likelihood call, interpolated at the
+ * new source locations (Interpolate3d, :976-1007: trilinear over the clamped
+ * cell), sigma = bnoise0*t + bnoise1, residuals with the new origin times.  No
+ * eikonal solve.  locations (nsrc,4) = Fortran locations(4,nsrc).  Needs a
+ * context created with dynamic != 0 (else MCT_ERR_ARG); with dynamic = 0 the
+ * reference runs the full likelihood, which is mct_body_likelihood. */
+int mct_body_location_likelihood(mct_ctx *ctx, const double *locations, const double *bnoise0,
+                                 const double *bnoise1, double *out);
+/* The same for the first nchain chains of the last batch: locations_dev
+ * (nchain, nsrc, 4), noise_dev (nchain, 2, np); asynchronous. */
+int mct_body_location_likelihood_batch_dev(mct_ctx *ctx, int nchain, const double *locations_dev,
+                                           const double *noise_dev, double *out_dev, int32_t *status_dev);
 
 /* Device memory for callers that hold no device allocator of their own (a
  * Fortran host, or Python without a GPU framework): plain hipMalloc / hipFree

@@ -46,7 +46,8 @@ class _BodySetup(ctypes.Structure):
     _fields_ = [("datatype", ctypes.c_int), ("order", ctypes.c_int), ("sigdep", ctypes.c_int),
                 ("nsrc", ctypes.c_int), ("nrev", ctypes.c_int),
                 ("rev", ctypes.c_void_p), ("ttime", ctypes.c_void_p), ("raystat", ctypes.c_void_p),
-                ("nrays", ctypes.c_int), ("max_cells", ctypes.c_int), ("max_chains", ctypes.c_int)]
+                ("nrays", ctypes.c_int), ("max_cells", ctypes.c_int), ("max_chains", ctypes.c_int),
+                ("dynamic", ctypes.c_int)]
 
 
 def lib():
@@ -71,7 +72,8 @@ def lib():
                      "mct_surf_noise_likelihood", "mct_surf_noise_likelihood_batch_dev",
                      "mct_surf_likelihood_box", "mct_surf_box_columns", "mct_surf_likelihood_box_batch_dev",
                      "mct_surf_accept", "mct_surf_accept_batch", "mct_surf_group_velocities",
-                     "mct_read_sources", "mct_read_times", "mct_write_sources", "mct_write_times"):
+                     "mct_read_sources", "mct_read_times", "mct_write_sources", "mct_write_times",
+                     "mct_body_location_likelihood", "mct_body_location_likelihood_batch_dev"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -321,7 +323,7 @@ class BodyLikelihood:
     """
 
     def __init__(self, grid, rev, ttime, raystat, nsrc, datatype=0, order=0, sigdep=0,
-                 nrays=None, max_cells=1024, max_chains=1):
+                 nrays=None, max_cells=1024, max_chains=1, dynamic=0):
         self.grid = grid
         self.np = 1 if datatype == 0 else 2
         self.nsrc = int(nsrc)
@@ -337,7 +339,7 @@ class BodyLikelihood:
         self.max_chains = max_chains
         st = _BodySetup(datatype, order, sigdep, self.nsrc, self.nrev,
                         self.rev.ctypes.data, self.ttime.ctypes.data, self.raystat.ctypes.data,
-                        nrays, max_cells, max_chains)
+                        nrays, max_cells, max_chains, int(dynamic))
         self._g = grid.c()
         h = ctypes.c_void_p()
         _check(lib().mct_body_create(ctypes.byref(self._g), ctypes.byref(st), ctypes.byref(h)),
@@ -397,6 +399,25 @@ class BodyLikelihood:
         rc = lib().mct_body_noise_likelihood(self._h, _p(loc), _p(b0), _p(b1), _p(out))
         _check(rc, "mct_body_noise_likelihood")
         return rc, out
+
+    def location_likelihood(self, locations, bnoise0, bnoise1):
+        """location_likelihood with dynamic != 0: the last call's receiver
+        fields interpolated at new source locations (nsrc, 4).  Returns
+        (status, out[3])."""
+        b0 = _f64(np.atleast_1d(bnoise0))
+        b1 = _f64(np.atleast_1d(bnoise1))
+        loc = _f64(locations)
+        out = np.zeros(3)
+        rc = lib().mct_body_location_likelihood(self._h, _p(loc), _p(b0), _p(b1), _p(out))
+        _check(rc, "mct_body_location_likelihood")
+        return rc, out
+
+    def location_likelihood_batch_dev(self, nchain, locations_ptr, noise_ptr, out_ptr, status_ptr):
+        rc = lib().mct_body_location_likelihood_batch_dev(
+            self._h, int(nchain), ctypes.c_void_p(locations_ptr), ctypes.c_void_p(noise_ptr),
+            ctypes.c_void_p(out_ptr), ctypes.c_void_p(status_ptr) if status_ptr else None)
+        _check(rc, "mct_body_location_likelihood_batch_dev")
+        return rc
 
     def noise_likelihood_batch_dev(self, nchain, locations_ptr, noise_ptr, out_ptr, status_ptr):
         rc = lib().mct_body_noise_likelihood_batch_dev(

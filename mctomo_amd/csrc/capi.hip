@@ -221,10 +221,11 @@ static int body_pipeline(mct_ctx *C, int nchain, const double *d_points, const d
 
     // (d) picks + misfit
     mct_pick_args pa;
+    memset(&pa, 0, sizeof pa);
     pa.G = G; pa.T = E.T; pa.t64 = E.t64; pa.nchain = nchain; pa.nwave = C->nwave; pa.nsrc_eff = C->nsrc_eff;
     pa.npick = C->npick; pa.pick_pts = pick_pts; pa.pick_stride_chain = pchain;
     pa.pick_point_stride = pstride; pa.xmin = G.xmin; pa.ymin = G.ymin; pa.zmin = zmin;
-    pa.lx = G.dx; pa.ly = G.dy; pa.lz = lz; pa.out = P<double>(C->picks);
+    pa.lx = G.dx; pa.ly = G.dy; pa.lz = lz; pa.out = P<double>(C->picks); pa.interp3d = 0;
     if ((rc = mct_launch_picks(pa, s))) return rc;
     mct_misfit_args ma;
     memset(&ma, 0, sizeof ma);
@@ -262,6 +263,33 @@ static int body_noise(mct_ctx *C, int nchain, const double *d_locations, const d
     return mct_launch_misfit(ma, C->stream);
 }
 
+// body_location_likelihood with dynamic != 0 (src/likelihood_body.F90:82-149):
+// each receiver's field of the last call interpolated at the new source
+// locations (Interpolate3d), then the misfit with srdist = that time, sigma =
+// bnoise0*t + bnoise1 and the new origin times (no lower bound on sigma).
+static int body_location(mct_ctx *C, int nchain, const double *d_locations, const double *d_noise,
+                         double *d_out, int32_t *d_status)
+{
+    if (!C->S.dynamic) return MCT_ERR_ARG;
+    mct_pick_args pa;
+    memset(&pa, 0, sizeof pa);
+    pa.G = C->G; pa.T = C->eik.T.p; pa.t64 = C->t64; pa.nchain = nchain; pa.nwave = C->nwave;
+    pa.nsrc_eff = C->nsrc_eff; pa.npick = C->npick; pa.pick_pts = d_locations;
+    pa.pick_stride_chain = C->S.nsrc * 4; pa.pick_point_stride = 4;
+    pa.out = P<double>(C->picks); pa.interp3d = 1;
+    int rc = mct_launch_picks(pa, C->stream);
+    if (rc) return rc;
+    mct_misfit_args ma;
+    memset(&ma, 0, sizeof ma);
+    ma.nchain = nchain; ma.nsrc = C->S.nsrc; ma.nrev = C->S.nrev; ma.np = C->nwave;
+    ma.reciprocal = C->reciprocal; ma.picks = P<double>(C->picks);
+    ma.ttime = P<double>(C->ttime); ma.raystat = P<int32_t>(C->raystat);
+    ma.locations = d_locations; ma.noise = d_noise; ma.sigdep = 1; ma.nrays = C->S.nrays;
+    ma.eps = -DBL_MAX;
+    ma.out = d_out; ma.status = d_status;
+    return mct_launch_misfit(ma, C->stream);
+}
+
 extern "C" {
 
 const char *mct_version(void) { return MCT_VERSION; }
@@ -281,7 +309,9 @@ int mct_body_create(const mct_grid *grid, const mct_body_setup *setup, mct_ctx *
     C->S = *setup;
     C->S.rev = nullptr; C->S.ttime = nullptr; C->S.raystat = nullptr;
     C->nwave = setup->datatype == 0 ? 1 : 2;
-    C->reciprocal = setup->nsrc > setup->nrev;
+    // from the receivers when they are fewer (likelihood_body.F90:226-249), and
+    // always with dynamic fields (:250-279)
+    C->reciprocal = setup->dynamic != 0 || setup->nsrc > setup->nrev;
     C->nsrc_eff = C->reciprocal ? setup->nrev : setup->nsrc;
     C->npick = C->reciprocal ? setup->nsrc : setup->nrev;
     C->band_tiles = mct_default_band_tiles();
@@ -429,6 +459,38 @@ int mct_body_noise_likelihood_batch_dev(mct_ctx *C, int nchain, const double *lo
     if (!C || nchain < 1 || nchain > C->S.max_chains || !noise_dev || !out_dev) return MCT_ERR_ARG;
     if (status_dev) MCT_CHECK(hipMemsetAsync(status_dev, 0, sizeof(int32_t) * nchain, C->stream));
     return body_noise(C, nchain, locations_dev, noise_dev, out_dev, status_dev);
+}
+
+int mct_body_location_likelihood(mct_ctx *C, const double *locations, const double *bnoise0,
+                                 const double *bnoise1, double *out)
+{
+    if (!C || !locations || !bnoise0 || !bnoise1 || !out) return MCT_ERR_ARG;
+    if (!C->S.dynamic) return MCT_ERR_ARG;
+    hipStream_t s = C->stream;
+    const int np = C->nwave;
+    MCT_CHECK(hipStreamSynchronize(s));
+    double *hl = C->h_in, *hn = hl + 4 * (size_t)C->S.nsrc;
+    for (int j = 0; j < 4 * C->S.nsrc; j++) hl[j] = locations[j];
+    for (int i = 0; i < np; i++) { hn[i] = bnoise0[i]; hn[np + i] = bnoise1[i]; }
+    MCT_CHECK(hipMemcpyAsync(C->locations.p, hl, sizeof(double) * 4 * C->S.nsrc, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemcpyAsync(C->noise.p, hn, sizeof(double) * 2 * np, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemsetAsync(C->status.p, 0, sizeof(int32_t), s));
+    int rc = body_location(C, 1, P<double>(C->locations), P<double>(C->noise), P<double>(C->out),
+                           P<int32_t>(C->status));
+    if (rc) return rc;
+    MCT_CHECK(hipMemcpyAsync(C->h_out, C->out.p, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+    MCT_CHECK(hipMemcpyAsync(C->h_status, C->status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    MCT_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < 3; i++) out[i] = C->h_out[i];
+    return *C->h_status;
+}
+
+int mct_body_location_likelihood_batch_dev(mct_ctx *C, int nchain, const double *locations_dev,
+                                           const double *noise_dev, double *out_dev, int32_t *status_dev)
+{
+    if (!C || nchain < 1 || nchain > C->S.max_chains || !locations_dev || !noise_dev || !out_dev) return MCT_ERR_ARG;
+    if (status_dev) MCT_CHECK(hipMemsetAsync(status_dev, 0, sizeof(int32_t) * nchain, C->stream));
+    return body_location(C, nchain, locations_dev, noise_dev, out_dev, status_dev);
 }
 
 int mct_dev_alloc(void **ptr, int64_t bytes)

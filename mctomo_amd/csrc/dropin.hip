@@ -149,6 +149,7 @@ extern "C" int fastMarching3D(int nx, int ny, int nz, double dx, double dy, doub
     if ((rc = eik_rounds(E, W.h_count, (float)dx, (float)dy, (float)dz, s))) return rc;
     if (nrev > 0) {
         mct_pick_args pa;
+        memset(&pa, 0, sizeof pa);
         pa.G = G; pa.T = E.T; pa.t64 = E.t64; pa.nchain = 1; pa.nwave = 1; pa.nsrc_eff = 1; pa.npick = nrev;
         pa.pick_pts = (const double *)W.rev.p; pa.pick_stride_chain = 0; pa.pick_point_stride = 3;
         pa.xmin = xmin; pa.ymin = ymin; pa.zmin = zmin; pa.lx = dx; pa.ly = dy; pa.lz = dz;

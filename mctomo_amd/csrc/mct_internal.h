@@ -128,6 +128,7 @@ struct mct_pick_args {
     int pick_point_stride;         // doubles between points (3 or 4)
     double xmin, ymin, zmin, lx, ly, lz;
     double *out;                   // (nchain, nwave, nsrc_eff, npick)
+    int interp3d;                  // Interpolate3d (likelihood_body.F90:976-1007) instead of getTime
 };
 int mct_launch_picks(const mct_pick_args &a, hipStream_t s);
 

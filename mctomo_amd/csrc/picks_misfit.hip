@@ -45,6 +45,31 @@ __global__ void k_picks(mct_pick_args a)
     const long toff = prob * (long)G.ntiles * MCT_TILE_N;
     const float *Tf = (const float *)a.T + toff;
     const double *Td = (const double *)a.T + toff;
+    if (a.interp3d) {
+        // Interpolate3d / point2idx (src/likelihood_body.F90:976-1007,1038-1054):
+        // the cell floor((p - min)/d) + 1 clamped to [1, n-1] (NaN to 1), z in
+        // unscaled units, trilinear weights in x, y, z loop order
+        const double dzs = G.dz / G.scaling;
+        const double fq[3] = {floor((pp[0] - G.xmin) / G.dx) + 1, floor((pp[1] - G.ymin) / G.dy) + 1,
+                              floor((pp[2] - G.zmin / G.scaling) / dzs) + 1};
+        int ic[3];
+        for (int d = 0; d < 3; d++) ic[d] = !(fq[d] >= 1.0) ? 1 : (fq[d] >= (double)dims[d] ? dims[d] - 1 : (int)fq[d]);
+        const double dsx = pp[0] - (G.xmin + (ic[0] - 1) * G.dx);
+        const double dsy = pp[1] - (G.ymin + (ic[1] - 1) * G.dy);
+        const double dsz = pp[2] - (G.zmin + (ic[2] - 1) * G.dz) / G.scaling;
+        double qv = 0;
+        for (int i = 1; i <= 2; i++)
+            for (int j = 1; j <= 2; j++)
+                for (int k = 1; k <= 2; k++) {
+                    double weight = (1.0 - fabs((i - 1) * G.dx - dsx) / G.dx) *
+                                    (1.0 - fabs((j - 1) * G.dy - dsy) / G.dy) *
+                                    (1.0 - fabs((k - 1) * G.dz / G.scaling - dsz) / dzs);
+                    long ti = tl_of(G, ic[0] + i - 2, ic[1] + j - 2, ic[2] + k - 2);
+                    qv = qv + weight * (a.t64 ? Td[ti] : (double)Tf[ti]);
+                }
+        a.out[i] = qv;
+        return;
+    }
     double acc = 0.0;
     for (int v = 0; v < 8; v++) {
         int vc[3];

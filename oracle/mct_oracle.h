@@ -116,6 +116,11 @@ int orc_body_misfit(int nsrc, int nrev, int np, const double *phase_time,
                     double *srdist, double *sigma, double *out);
 
 
+/* Interpolate3d (src/likelihood_body.F90:976-1007): the trilinear value of a
+ * traveltime field (Fortran (nz,ny,nx)) at point[3] (z unscaled), misfit.c. */
+double orc_interpolate3d(int nx, int ny, int nz, double xmin, double ymin, double zmin, double dx,
+                         double dy, double dz, double scaling, const double *field, const double *point);
+
 /* Surface-wave likelihood (datatype 2), surf.c.  Grid as
  * T_GRID (dz scaled, scaling = grid%scaling); points/params (ncells,3);
  * freqs (np) Hz; raylov 1 Rayleigh, 0 Love; src (nsrc,2), rev (nrev,2) as

@@ -55,3 +55,33 @@ int orc_body_misfit(int nsrc, int nrev, int np, const double *phase_time,
     out[0] = like; out[1] = misfit; out[2] = unw;
     return fatal ? -1 : 0;
 }
+
+/* Interpolate3d with point2idx (src/likelihood_body.F90:976-1007,1038-1054):
+ * field Fortran (nz,ny,nx) (C: field[(ix*ny + iy)*nz + iz]); grid as T_GRID
+ * with zmin/dz scaled; the point in unscaled units.  The cell is
+ * floor((p - min)/d) + 1 clamped to [1, n-1]. */
+double orc_interpolate3d(int nx, int ny, int nz, double xmin, double ymin, double zmin, double dx,
+                         double dy, double dz, double scaling, const double *field, const double *point)
+{
+    int ix = (int)floor((point[0] - xmin) / dx) + 1;
+    int iy = (int)floor((point[1] - ymin) / dy) + 1;
+    int iz = (int)floor((point[2] - zmin / scaling) / (dz / scaling)) + 1;
+    if (ix < 1) ix = 1;
+    if (iy < 1) iy = 1;
+    if (iz < 1) iz = 1;
+    if (ix >= nx) ix = nx - 1;
+    if (iy >= ny) iy = ny - 1;
+    if (iz >= nz) iz = nz - 1;
+    double dsx = point[0] - (xmin + (ix - 1) * dx);
+    double dsy = point[1] - (ymin + (iy - 1) * dy);
+    double dsz = point[2] - (zmin + (iz - 1) * dz) / scaling;
+    double qv = 0;
+    for (int i = 1; i <= 2; i++)
+        for (int j = 1; j <= 2; j++)
+            for (int k = 1; k <= 2; k++) {
+                double weight = (1.0 - fabs((i - 1) * dx - dsx) / dx) * (1.0 - fabs((j - 1) * dy - dsy) / dy) *
+                                (1.0 - fabs((k - 1) * dz / scaling - dsz) / (dz / scaling));
+                qv = qv + weight * field[((long)(ix + i - 2) * ny + (iy + j - 2)) * nz + (iz + k - 2)];
+            }
+    return qv;
+}

@@ -154,6 +154,20 @@ def body_misfit(phase_time, ttime, raystat, locations, sigdep=0, bnoise0=None, b
     return out, srd, sig, rc
 
 
+def interpolate3d(grid, field, point):
+    """Interpolate3d (src/likelihood_body.F90:976-1007).  grid: dict with zmin/dz
+    scaled and scaling; field (nx,ny,nz) C array; point (3,) unscaled."""
+    f = c64(field)
+    nx, ny, nz = f.shape
+    L = lib()
+    L.orc_interpolate3d.restype = ctypes.c_double
+    return L.orc_interpolate3d(ctypes.c_int(nx), ctypes.c_int(ny), ctypes.c_int(nz),
+                               ctypes.c_double(grid["xmin"]), ctypes.c_double(grid["ymin"]),
+                               ctypes.c_double(grid["zmin"]), ctypes.c_double(grid["dx"]),
+                               ctypes.c_double(grid["dy"]), ctypes.c_double(grid["dz"]),
+                               ctypes.c_double(grid.get("scaling", 1.0)), _p(f), _p(c64(point)))
+
+
 # ---------------------------------------------------------------- dispersion
 _F = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 

@@ -221,3 +221,45 @@ def test_noise_likelihood_reuses_the_last_traveltimes():
     ref, _, _, _ = po.body_misfit(ptg, w.ttime, w.raystat, loc2, sigdep=1, bnoise0=c0, bnoise1=c1)
     assert abs(out2[0] - ref[0]) <= 1e-12 * abs(ref[0]), (out2, ref)
     np.testing.assert_allclose(out2[1:], ref[1:], rtol=1e-12)
+
+
+@pytest.mark.parametrize("datatype", [0, 1])
+def test_location_likelihood_dynamic_fields(datatype):
+    """location_likelihood with dynamic fields (likelihood_body.F90:82-149):
+    the receivers' fields of the last call interpolated (Interpolate3d) at new
+    source locations, sigma = b0*t + b1, the new origin times; against the
+    oracle's fields from each receiver."""
+    say("body location likelihood, dynamic fields, datatype", datatype)
+    nsrc, nrev = 6, 5
+    w = body_workload(n=29, h=0.3, ncells=40, nsrc=nsrc, nrev=nrev, datatype=datatype, seed=17)
+    grid = dict(_g(w.grid), scaling=1.0)
+    npp = 1 if datatype == 0 else 2
+    like = mct.BodyLikelihood(w.grid, w.rev, w.ttime, w.raystat, nsrc, datatype=datatype, order=0,
+                              max_cells=64, dynamic=1)
+    rc, out = like.likelihood(w.points, w.params, w.locations)
+    assert rc == 0
+    rng = np.random.default_rng(3)
+    loc2 = w.locations.copy()
+    L = (w.grid.nx - 1) * w.grid.dx
+    loc2[:, :3] = np.clip(loc2[:, :3] + rng.uniform(-0.6, 0.6, (nsrc, 3)), 0.2, L - 0.2)
+    loc2[:, 3] = rng.uniform(-0.3, 0.3, nsrc)
+    b0, b1 = np.full(npp, 0.02), np.full(npp, 0.05)
+    rc2, out2 = like.location_likelihood(loc2, b0, b1)
+    assert rc2 == 0
+    _, vp, vs, _ = po.kdtree_to_grid(grid, w.points, w.params)
+    pt = np.zeros((npp, nsrc, nrev))
+    for k, vel in enumerate([vp, vs][:npp]):
+        for j in range(nrev):
+            _, f = po.fast_marching_3d(vel, grid["dx"], grid["dy"], grid["dz"], grid["xmin"], grid["ymin"],
+                                       grid["zmin"], w.rev[j], loc2[:, :3], 0, field=True)
+            for i in range(nsrc):
+                pt[k, i, j] = po.interpolate3d(grid, f, loc2[i, :3])
+    ref, _, _, rco = po.body_misfit(pt, w.ttime, w.raystat, loc2, sigdep=1, bnoise0=b0, bnoise1=b1)
+    say("  like", out2[0], "oracle", ref[0])
+    assert abs(out2[0] - ref[0]) <= 1e-6 * abs(ref[0]), (out2, ref)
+    np.testing.assert_allclose(out2[1:], ref[1:], rtol=1e-4)
+    # a context without dynamic fields refuses the call (the reference then runs
+    # the full likelihood, which is mct_body_likelihood)
+    plain = mct.BodyLikelihood(w.grid, w.rev, w.ttime, w.raystat, nsrc, datatype=datatype, order=0, max_cells=64)
+    with pytest.raises(mct.MCTError):
+        plain.location_likelihood(loc2, b0, b1)
