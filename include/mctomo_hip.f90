@@ -37,6 +37,7 @@ module mctomo_hip
         integer(c_int) :: sigdep, nrays, gridx, gridy, sgref, sgdic, sgext, order
         real(c_double) :: band
         integer(c_int) :: max_cells, max_chains
+        integer(c_int) :: straight
     end type
 
     interface

@@ -267,6 +267,9 @@ typedef struct {
     double band;              /* LIKE_SET%band (narrow-band size fraction) */
     int max_cells;            /* largest Voronoi site count a call may pass */
     int max_chains;           /* largest batch for the _batch_dev entry */
+    int straight;             /* LIKE_SET%isStraight: 1 = times along straight source-receiver
+                                 rays (src/likelihood_surf.F90:233-243,412-452), no 2-D solve;
+                                 srdist = the distances */
 } mct_surf_setup;
 
 typedef struct mct_surf_ctx mct_surf_ctx;

@@ -451,7 +451,7 @@ class _SurfSetup(ctypes.Structure):
                 ("nrays", ctypes.c_int), ("gridx", ctypes.c_int), ("gridy", ctypes.c_int),
                 ("sgref", ctypes.c_int), ("sgdic", ctypes.c_int), ("sgext", ctypes.c_int),
                 ("order", ctypes.c_int), ("band", ctypes.c_double), ("max_cells", ctypes.c_int),
-                ("max_chains", ctypes.c_int)]
+                ("max_chains", ctypes.c_int), ("straight", ctypes.c_int)]
 
 
 def _surf_setup(grid, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3, sigdep=0, nrays=None,
@@ -486,7 +486,7 @@ class SurfLikelihood:
 
     def __init__(self, grid, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3, sigdep=0,
                  nrays=None, gridx=1, gridy=1, sgref=1, sgdic=4, sgext=8, order=1, band=0.5,
-                 max_cells=1024, max_chains=1, phase_group=0):
+                 max_cells=1024, max_chains=1, phase_group=0, straight=0):
         self.grid = grid
         self.freqs = _f64(freqs).reshape(-1)
         self.np = self.freqs.shape[0]
@@ -505,7 +505,7 @@ class SurfLikelihood:
                         self.nrev, self.src.ctypes.data, self.rev.ctypes.data, self.ttime.ctypes.data,
                         self.raystat.ctypes.data, int(sigdep), int(nrays), int(gridx), int(gridy),
                         int(sgref), int(sgdic), int(sgext), int(order), float(band), int(max_cells),
-                        int(max_chains))
+                        int(max_chains), int(straight))
         self._g = grid.c()
         h = ctypes.c_void_p()
         _check(lib().mct_surf_create(ctypes.byref(self._g), ctypes.byref(st), ctypes.byref(h)),

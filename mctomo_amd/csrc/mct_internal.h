@@ -199,6 +199,9 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
 // rpaths + ray length + CalGroupTime for the chunk [b.prob0, b.prob0 + b.nrun):
 // one thread per (problem, receiver)
 int mct_launch_fm2d_rays(const mct_fm2d_args &b, hipStream_t s);
+// straight rays (isStraight = 1): group delays along the source-receiver
+// segments for a.nchain chains into a.ttime, through a.gvel
+int mct_launch_straight_rays(const mct_fm2d_args &a, hipStream_t s);
 // Bytes of fm2d workspace per problem slot.
 static inline long mct_fm2d_slot_bytes(long ld, long nc)
 {

@@ -214,6 +214,67 @@ __global__ __launch_bounds__(64) void k_fm2d_rays(mct_fm2d_args A)
     A.rlen[q] = R.len;
 }
 
+// Straight rays (isStraight = 1): setup_straightRays' points (likelihood_surf.F90
+// :412-452) generated on the fly and CalGroupTime over them (:454-489), one
+// thread per (chain, period, source, receiver) of a.nchain chains.  The
+// distances (srdist) are fixed by the geometry and set at create time.
+__global__ __launch_bounds__(256) void k_straight_rays(mct_fm2d_args A)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long npair = (long)A.nsrc * A.nrc;
+    if (t >= (long)A.nchain * A.np * npair) return;
+    const long q = t % npair;
+    const long cp = t / npair;
+    const int p = (int)(cp % A.np);
+    const int c = (int)(cp / A.np);
+    if (A.status[c] > 0) return;            // model already rejected
+    const int j = (int)(q / A.nrc), k = (int)(q % A.nrc);
+    if (A.srs[(long)p * A.srs_pstride + q] != 1) { A.ttime[t] = 0.0; return; }
+    const double *gv = A.gvel + (long)c * A.nvx * A.nvz * A.np + p;
+    const double xmax = A.gox + (A.nvx - 1) * A.dvx, ymax = A.goz + (A.nvz - 1) * A.dvz;
+    double dl = A.dvx < A.dvz ? A.dvx : A.dvz;
+    dl = dl / 2;
+    const double sx = A.scx[j], sy = A.scz[j], rx = A.rcx[k], ry = A.rcz[k];
+    const double dx = rx - sx, dy = ry - sy;
+    const double ds = sqrt(dx * dx + dy * dy);
+    const double fn = floor(ds / dl) + 1;
+    const long npoints = fn < 1.0 ? 1 : (long)fn;
+    ray_sum R;
+    R.n = 0; R.px = 0; R.py = 0; R.len = 0; R.gt = 0; R.vhead = 0;
+    auto point = [&](double x, double y) {
+        const double vt = ray_velocity(gv, A.np, A.nvx, A.nvz, A.gox, A.goz, A.dvx, A.dvz, x, y);
+        if (R.n > 0) {
+            double dist = (x - R.px) * (x - R.px) + (y - R.py) * (y - R.py);
+            dist = sqrt(dist);
+            R.gt = R.gt + dist * 2 / (R.vhead + vt);
+        }
+        R.vhead = vt;
+        R.px = x; R.py = y;
+        R.n++;
+    };
+    for (long l = 1; l <= npoints - 1; l++) {
+        const double px = sx + (l - 1) * dl * dx / ds;
+        const double py = sy + (l - 1) * dl * dy / ds;
+        if (px < A.gox || px > xmax || py < A.goz || py > ymax) {   // 'Straight ray out of area'
+            atomicMin(&A.status[c], (int)MCT_ERR_OUTSIDE);
+            return;
+        }
+        point(px, py);
+    }
+    point(rx, ry);
+    A.ttime[t] = R.n >= 2 ? R.gt : 0.0;
+}
+
+int mct_launch_straight_rays(const mct_fm2d_args &a, hipStream_t s)
+{
+    if (!a.gvel) return MCT_ERR_ARG;
+    const long n = (long)a.nchain * a.np * a.nsrc * a.nrc;
+    if (n <= 0) return MCT_OK;
+    hipLaunchKernelGGL(k_straight_rays, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+    MCT_CHECK(hipGetLastError());
+    return MCT_OK;
+}
+
 int mct_launch_fm2d_rays(const mct_fm2d_args &b, hipStream_t s)
 {
     if (!b.rays) return MCT_OK;

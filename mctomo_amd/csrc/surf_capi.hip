@@ -198,6 +198,21 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     MCT_CHECK(hipGetLastError());
     if (timing) MCT_CHECK(hipEventRecord(C->ev[2], s));
     if (ev_dsp) MCT_CHECK(hipEventRecord(ev_dsp, s));
+    double *phase_time = C->phase_time.as<double>() + (long)c0 * npair * np;
+    double *rlen = (S.phase_group || S.straight) ? C->rlen.as<double>() + (long)c0 * npair * np : nullptr;
+    if (S.straight) {
+        // (c') straight rays (likelihood_surf.F90:233-243): the phase (or group)
+        // velocities integrated along the source-receiver segments; no 2-D solve
+        mct_fm2d_args fa = C->F;
+        fa.nchain = nchain;
+        fa.scx = C->scx.as<double>(); fa.scz = C->scz.as<double>();
+        fa.rcx = C->rcx.as<double>(); fa.rcz = C->rcz.as<double>();
+        fa.srs = C->raystat.as<int32_t>(); fa.srs_pstride = 2L * S.nsrc * S.nrev;
+        fa.ttime = phase_time; fa.status = d_status;
+        fa.gvel = S.phase_group ? da.group : da.phase;
+        if (timing) C->sweep_rec = false;
+        if ((rc = mct_launch_straight_rays(fa, s))) return rc;
+    } else {
     // (c) padded phase-velocity grids, then modrays per (chain, period, source)
     double *velv = C->velv.as<double>() + (long)c0 * (G.nx + 2) * (G.ny + 2) * np;
     {
@@ -219,13 +234,11 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     fa.vr = C->vr.as<double>() + soff * wsz;
     fa.nsts = C->nsts.as<int32_t>() + soff * wsz; fa.nstr = C->nstr.as<int32_t>() + soff * wsz;
     fa.heap = C->heap.as<int32_t>() + soff * (wsz + 8);
-    double *phase_time = C->phase_time.as<double>() + (long)c0 * npair * np;
     fa.ttime = phase_time; fa.status = d_status;
     fa.rounds_out = C->rounds.as<int32_t>() + 4 * soff;
     fa.ev_sweep = timing ? C->ev_sweep : nullptr;
     // group velocities: rays, group delays (CalGroupTime) and ray lengths
     // (likelihood_surf.F90:245,311-351)
-    double *rlen = S.phase_group ? C->rlen.as<double>() + (long)c0 * npair * np : nullptr;
     fa.rays = S.phase_group; fa.gvel = da.group; fa.rlen = rlen;
     const bool fast = C->fm2d_method == 0 && mct_fm2d_fast_ok(fa);
     if (timing) C->sweep_rec = fast;
@@ -234,6 +247,7 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     else
         rc = mct_launch_fm2d(fa, chunk, s);
     if (rc) return rc;
+    }
     if (timing) MCT_CHECK(hipEventRecord(C->ev[3], s));
     // (d) misfit: srdist = phaseTime, or the ray length for group times
     // (src/likelihood_surf.F90:324-331)
@@ -269,7 +283,7 @@ static int surf_noise(mct_surf_ctx *C, int nchain, const double *d_noise, double
     ma.sigdep = 1; ma.nrays = S.nrays;
     ma.eps = (double)1.0e-10f;   // EPS, src/likelihood_surf.F90:37
     ma.out = d_out; ma.status = d_status;
-    ma.dist = S.phase_group ? C->rlen.as<double>() : nullptr;
+    ma.dist = (S.phase_group || S.straight) ? C->rlen.as<double>() : nullptr;
     return mct_launch_misfit(ma, C->stream);
 }
 
@@ -295,8 +309,9 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     if (S.raylov != 0 && S.raylov != 1) return MCT_ERR_ARG;
     // group times trace rays from the refined source grid (rpaths reads it)
     if (S.phase_group != 0 && S.phase_group != 1) return MCT_ERR_ARG;
-    if (S.phase_group == 1 && S.sgref != 1) return MCT_ERR_ARG;
-    if (S.phase_group == 1) {
+    if (S.straight != 0 && S.straight != 1) return MCT_ERR_ARG;
+    if (S.phase_group == 1 && S.sgref != 1 && !S.straight) return MCT_ERR_ARG;
+    if (S.phase_group == 1 && !S.straight) {
         // rpaths stores pair n's ray at slot raystat(n,2) and CalGroupTime reads
         // slot n for pair n (fm2dray_cartesian.f90:1403, likelihood_surf.F90:477):
         // the data readers set raystat(n,2) = n (likelihood_settings.f90:401)
@@ -400,10 +415,8 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     ALLOC(nlayer, sizeof(int32_t) * ncol * M);
     ALLOC(lay, sizeof(float) * mct_dispersion_lay_floats((int)ncol, grid->nz, M));
     ALLOC(phase_keep, sizeof(double) * ncol * np * M);
-    if (S.phase_group) {
-        ALLOC(rlen, sizeof(double) * npair * np * M);
-        ALLOC(group_keep, sizeof(double) * ncol * np * M);
-    }
+    if (S.phase_group || S.straight) ALLOC(rlen, sizeof(double) * npair * np * M);
+    if (S.phase_group) ALLOC(group_keep, sizeof(double) * ncol * np * M);
     ALLOC(box, sizeof(int32_t) * (5 * (size_t)M + 1));
     ALLOC(dsp_queue, 256);
     ALLOC(dsp_jobs, sizeof(int32_t) * ncol * M);
@@ -449,6 +462,18 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
                 rc = MCT_ERR_HIP;
         }
         free(per); free(xy);
+    }
+    if (!rc && S.straight) {
+        // setup_straightRays' distances (likelihood_surf.F90:426-432) are srdist for
+        // every pair, period and chain
+        std::vector<double> d((size_t)npair * np * M);
+        for (int j = 0; j < S.nsrc; j++)
+            for (int k = 0; k < S.nrev; k++) {
+                double dx = S.rev[2 * k] - S.src[2 * j], dy = S.rev[2 * k + 1] - S.src[2 * j + 1];
+                double ds = sqrt(dx * dx + dy * dy);
+                for (long m = 0; m < (long)np * M; m++) d[m * npair + (long)j * S.nrev + k] = ds;
+            }
+        if (hipMemcpy(C->rlen.p, d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice) != hipSuccess) rc = MCT_ERR_HIP;
     }
     // like%phaseTime starts at zero; pairs that are never traced keep it
     if (!rc && hipMemset(C->phase_time.p, 0, sizeof(double) * npair * np * M) != hipSuccess) rc = MCT_ERR_HIP;

@@ -785,3 +785,42 @@ int orc_fm2d_rays(int nvx, int nvz, double goxd, double gozd, double dvxd, doubl
     free(velnc); free(S.ttn); free(S.veln); free(S.nsts); free(ttnr); free(nstsr); free(S.btg);
     return rc;
 }
+
+/* Straight rays (isStraight = 1): setup_straightRays (src/likelihood_surf.F90
+ * :412-452) and CalGroupTime over them (:454-489).  dist (nsrc,nrc) = the
+ * source-receiver distance for every pair; for a used pair the points are the
+ * source + (l-1)*dl along the segment, l = 1..floor(dist/dl), then the
+ * receiver, dl = min(dx,dy)/2; gtime (nsrc,nrc) the group delay along them
+ * (0 for unused pairs).  Returns -1 when a point leaves [xmin,xmax]x[ymin,ymax]
+ * (the reference stops: 'Straight ray out of area'). */
+int orc_straight_rays(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvzd, const double *gvel,
+                      int nsrc, const double *scx, const double *scz, int nrc, const double *rcx,
+                      const double *rcz, const int32_t *srs, double *gtime, double *dist)
+{
+    const double xmax = goxd + (nvx - 1) * dvxd, ymax = gozd + (nvz - 1) * dvzd;
+    double dl = dvxd < dvzd ? dvxd : dvzd;
+    dl = dl / 2;
+    for (int j = 1; j <= nsrc; j++)
+        for (int k = 1; k <= nrc; k++) {
+            const long q = (long)(j - 1) * nrc + (k - 1);
+            double dx = rcx[k - 1] - scx[j - 1];
+            double dy = rcz[k - 1] - scz[j - 1];
+            double ds = sqrt(dx * dx + dy * dy);
+            dist[q] = ds;
+            gtime[q] = 0.0;
+            if (srs[q] != 1) continue;
+            long npoints = (long)floor(ds / dl) + 1;
+            ray_acc A;
+            memset(&A, 0, sizeof A);
+            A.gv = gvel; A.nx = nvx; A.ny = nvz; A.xmin = goxd; A.ymin = gozd; A.dx = dvxd; A.dy = dvzd;
+            for (long l = 1; l <= npoints - 1; l++) {
+                double px = scx[j - 1] + (l - 1) * dl * dx / ds;
+                double py = scz[j - 1] + (l - 1) * dl * dy / ds;
+                if (px < goxd || px > xmax || py < gozd || py > ymax) return -1;
+                ray_point(&A, px, py);
+            }
+            ray_point(&A, rcx[k - 1], rcz[k - 1]);
+            gtime[q] = A.n >= 2 ? A.gt : 0.0;
+        }
+    return 0;
+}

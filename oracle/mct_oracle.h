@@ -80,6 +80,13 @@ int orc_fm2d_rays(int nvx, int nvz, double goxd, double gozd, double dvxd, doubl
                   double *tfield, int rays, const double *gvel, double *gtime, double *rlen,
                   int32_t *crazy, int maxpts, int32_t *npts, double *pts);
 
+/* Straight rays (isStraight = 1, src/likelihood_surf.F90:412-489): dist and
+ * group delays gtime (nsrc,nrc) along the source-receiver segments through
+ * gvel (Fortran gvel(nvz,nvx)).  Returns -1 for a point outside the model. */
+int orc_straight_rays(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvzd, const double *gvel,
+                      int nsrc, const double *scx, const double *scz, int nrc, const double *rcx,
+                      const double *rcz, const int32_t *srs, double *gtime, double *dist);
+
 /* ---- (c) 3-D fast marching (SFMM) --------------------------------------- */
 /* Same signature and semantics as the reference C ABI fastMarching3D
  * (fastmarching/fastMarching.cpp:26-129). vel/tfield are Fortran (nz,ny,nx). */
@@ -143,7 +150,7 @@ int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double
                         double band, int nthreads,
                         double *out, double *phase_time, double *srdist, double *sigma,
                         double *phase_vel, int32_t *sites_id, int phase_group,
-                        double *group_vel);
+                        double *group_vel, int straight);
 
 #ifdef __cplusplus
 }

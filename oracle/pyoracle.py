@@ -299,7 +299,8 @@ def ref_read_data(datatype, sources_file, receivers_file, data_file):
 # ---------------------------------------------------------------- surface likelihood
 def surf_likelihood(grid, points, params, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3,
                     sigdep=0, snoise0=None, snoise1=None, nrays=None, gridx=1, gridy=1, sgref=1,
-                    sgdic=4, sgext=8, order=1, band=0.5, nthreads=0, phase_group=0, group_vel=None):
+                    sgdic=4, sgext=8, order=1, band=0.5, nthreads=0, phase_group=0, group_vel=None,
+                    straight=0):
     """orc_surf_likelihood.  grid: dict nx..dz (+ scaling).  Returns (rc, out(3),
     phase_time (np,nsrc,nrev), srdist, sigma (np,npair), phase_vel (nx,ny,np),
     sites_id (nx,ny,nz)).  phase_group=1: group times along rays; group_vel, an
@@ -334,5 +335,5 @@ def surf_likelihood(grid, points, params, freqs, src, rev, ttime, raystat, raylo
         _p(s0), _p(s1), ctypes.c_int(nrays), ctypes.c_int(gridx), ctypes.c_int(gridy),
         ctypes.c_int(sgref), ctypes.c_int(sgdic), ctypes.c_int(sgext), ctypes.c_int(order),
         ctypes.c_double(band), ctypes.c_int(nthreads), _p(out), _p(pt), _p(sd), _p(sg), _p(pv), _p(sid),
-        ctypes.c_int(phase_group), _p(group_vel))
+        ctypes.c_int(phase_group), _p(group_vel), ctypes.c_int(straight))
     return rc, out, pt, sd, sg, pv, sid

@@ -64,7 +64,8 @@ int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double
                         int gridx, int gridy, int sgref, int sgdic, int sgext, int order,
                         double band, int nthreads,
                         double *out, double *phase_time, double *srdist, double *sigma,
-                        double *phase_vel, int32_t *sites_id, int phase_group, double *group_vel)
+                        double *phase_vel, int32_t *sites_id, int phase_group, double *group_vel,
+                        int straight)
 {
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -98,7 +99,25 @@ int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double
         }
         if (bad) rc = 1;
     }
-    if (!rc) {
+    if (!rc && straight) {
+        /* straight rays (:233-243): the phase or group velocities integrated
+         * along the source-receiver segments; srdist = the distances */
+        double *gvp = malloc(sizeof(double) * C);
+        double *scx = malloc(sizeof(double) * nsrc), *scz = malloc(sizeof(double) * nsrc);
+        double *rcx = malloc(sizeof(double) * nrev), *rcz = malloc(sizeof(double) * nrev);
+        for (int j = 0; j < nsrc; j++) { scx[j] = src[2 * j]; scz[j] = src[2 * j + 1]; }
+        for (int k = 0; k < nrev; k++) { rcx[k] = rev[2 * k]; rcz[k] = rev[2 * k + 1]; }
+        const double *vsel = phase_group ? gvel : pvel;
+        for (int p = 0; p < np && !rc; p++) {
+            for (long c = 0; c < C; c++) gvp[c] = vsel[c * np + p];
+            if (orc_straight_rays(nx, ny, xmin, ymin, dx, dy, gvp, nsrc, scx, scz, nrev, rcx, rcz,
+                                  raystat + (long)p * 2 * npair, phase_time + (long)p * npair,
+                                  srdist + (long)p * npair))
+                rc = -6;
+        }
+        free(gvp); free(scx); free(scz); free(rcx); free(rcz);
+    }
+    if (!rc && !straight) {
         /* like%vel(np, ny+2, nx+2) with replicated edges; per period a Fortran
          * velv(0:ny+1, 0:nx+1) grid */
         const int ldv = ny + 2;

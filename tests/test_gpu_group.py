@@ -201,3 +201,35 @@ def test_group_perturbed_box_call_equals_the_whole_grid_call():
     # outside the changed columns the kept group velocities are the accepted ones
     np.testing.assert_array_equal(gvm[~chg], gvf[~chg])
     L.close()
+
+
+@pytest.mark.parametrize("phase_group", [0, 1])
+def test_straight_rays_match_oracle(phase_group):
+    """isStraight = 1 (likelihood_surf.F90:233-243,412-489): phase or group
+    velocities integrated along the source-receiver segments, srdist = the
+    distances, no 2-D solve; equal to the oracle bit for bit."""
+    w = W.surface_workload(n=41, nz=21, ncells=30, np_=10, nsrc=16, nrev=16, seed=20261202, order=1)
+    say("straight rays, phase_group", phase_group)
+    L = mct.SurfLikelihood(w.grid, w.freqs, w.src, w.rev, w.ttime, w.raystat, raylov=w.raylov,
+                           order=w.order, phase_group=phase_group, straight=1, sigdep=1)
+    npp = w.freqs.shape[0]
+    s0, s1 = np.full(npp, 0.01), np.full(npp, 0.04)
+    rc, out, pt, sd, sg, pv = L.likelihood(w.points, w.params, snoise0=s0, snoise1=s1, arrays=True)
+    g = w.grid
+    res = po.surf_likelihood(gdict(g), w.points, w.params, w.freqs, w.src, w.rev, w.ttime, w.raystat,
+                             raylov=w.raylov, order=w.order, sigdep=1, snoise0=s0, snoise1=s1,
+                             phase_group=phase_group, straight=1)
+    orc, oout, opt, osd, osg, opv, _ = res
+    assert rc == orc == 0
+    say("  like", out[0], "oracle", oout[0])
+    np.testing.assert_array_equal(pv, opv)
+    np.testing.assert_array_equal(pt, opt)
+    np.testing.assert_array_equal(sd, osd)
+    assert abs(out[0] - oout[0]) <= 1e-12 * abs(oout[0])
+    dist = np.linalg.norm(w.rev[None, :, :] - w.src[:, None, :], axis=2).reshape(-1)
+    np.testing.assert_allclose(sd[0], dist, rtol=1e-15)
+    # noise_likelihood uses the distances as srdist
+    rc2, cheap = L.noise_likelihood(s0 * 2, s1)
+    rc3, full = L.likelihood(w.points, w.params, snoise0=s0 * 2, snoise1=s1)
+    assert rc2 == rc3 == 0 and np.array_equal(cheap[:2], full[:2])
+    L.close()

@@ -129,3 +129,50 @@ def test_matches_compiled_reference_live():
             np.testing.assert_array_equal(a["npts"], b["npts"])
             np.testing.assert_array_equal(a["pts"], b["pts"])
             assert a["crazy"] == b["crazy"]
+
+
+def _straight(gv, src, rec, srs, h=0.5):
+    nvx, nvz = gv.shape
+    ns, nr = src.shape[0], rec.shape[0]
+    gt, d = np.zeros(ns * nr), np.zeros(ns * nr)
+    import ctypes
+    L = po.lib()
+    rc = L.orc_straight_rays(nvx, nvz, ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_double(h),
+                             ctypes.c_double(h), po._p(po.c64(gv)), ns, po._p(po.c64(src[:, 0])),
+                             po._p(po.c64(src[:, 1])), nr, po._p(po.c64(rec[:, 0])), po._p(po.c64(rec[:, 1])),
+                             po._p(np.ascontiguousarray(srs, np.int32)), po._p(gt), po._p(d))
+    return rc, gt.reshape(ns, nr), d.reshape(ns, nr)
+
+
+def test_straight_rays_closed_forms():
+    """setup_straightRays + CalGroupTime (likelihood_surf.F90:412-489): the
+    distance is |rec - src|; with a constant velocity the time is distance / v;
+    with a velocity linear along x the trapezoid over the half-cell points
+    approaches the exact integral of 1/v (log form) to second order; unused
+    pairs get 0."""
+    rng = np.random.default_rng(12)
+    nx, ny, h = 30, 24, 0.5
+    src = rng.uniform(0.5, 11.0, (4, 2))
+    rec = rng.uniform(0.5, 11.0, (6, 2))
+    srs = np.ones(24, np.int32)
+    srs[5] = 0
+    rc, gt, d = _straight(np.full((nx, ny), 3.2), src, rec, srs, h)
+    assert rc == 0
+    dist = np.linalg.norm(rec[None, :, :] - src[:, None, :], axis=2)
+    np.testing.assert_allclose(d, dist, rtol=1e-15)
+    used = srs.reshape(4, 6) == 1
+    np.testing.assert_allclose(gt[used], dist[used] / 3.2, rtol=1e-13)
+    assert gt.reshape(-1)[5] == 0.0
+    a, b = 2.0, 0.08
+    gv = a + b * (np.arange(nx) * h)[:, None] + 0.0 * np.arange(ny)[None, :]
+    rc, gt, d = _straight(gv, src, rec, srs, h)
+    dx = rec[None, :, 0] - src[:, None, 0]
+    vs, vr = a + b * src[:, None, 0], a + b * rec[None, :, 0]
+    exact = np.where(np.abs(dx) > 1e-9, dist * np.log(vr / vs) / (b * dx + 1e-300), dist / vs)
+    np.testing.assert_allclose(gt[used], exact[used], rtol=2e-4)
+
+
+def test_straight_ray_leaving_the_model_is_fatal():
+    rc, _, _ = _straight(np.full((10, 10), 3.0), np.array([[1.0, 1.0]]), np.array([[6.0, 1.0]]),
+                         np.ones(1, np.int32), 0.5)
+    assert rc == -1
