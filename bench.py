@@ -231,7 +231,7 @@ def run_surface(args):
     # ---- served: Poisson arrivals into the continuous-batching server -----------
     served = None
     if args.served > 0:
-        srv = mct.SurfServer(w.grid, w.freqs, w.src, w.rev, w.ttime, w.raystat, max_batch=B, nlanes=2,
+        srv = mct.SurfServer(w.grid, w.freqs, w.src, w.rev, w.ttime, w.raystat, max_batch=min(B, 32), nlanes=2,
                              raylov=1, order=args.order, max_cells=cmax)
         reqs = [(P[c % B, :N[c % B]], Q[c % B, :N[c % B]]) for c in range(args.served)]
         for t in [srv.submit(*reqs[i]) for i in range(min(B, len(reqs)))]:
@@ -257,7 +257,7 @@ def run_surface(args):
         served = {"requests": len(reqs), "arrival_rate_per_s": round(rate, 3),
                   "evals_per_s": round(len(reqs) / t_all, 3),
                   "p50_ms": round(float(np.median(lat_s)), 3), "p99_ms": round(float(np.quantile(lat_s, 0.99)), 3),
-                  "batches": int(nb - 1), "lanes": 2, "max_batch": B}
+                  "batches": int(nb - 1), "lanes": 2, "max_batch": min(B, 32)}
         log("served:", served)
 
     # ---- roofline of the dominant kernel, live HIP events ------------------------
@@ -459,7 +459,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--order", type=int, default=0)
     ap.add_argument("--workload", default="surf201", choices=["surf201", "body101"])
     ap.add_argument("--fm2d-method", type=int, default=0)

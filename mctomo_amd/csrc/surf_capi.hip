@@ -343,10 +343,10 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     const long npair = (long)S.nsrc * S.nrev, gsz = (long)grid->nx * grid->ny * grid->nz;
     const long ncol = (long)grid->nx * grid->ny;
     const long nprob = (long)M * np * S.nsrc;
-    // fm2d workspace: problem slots for at most MCT_FM2D_WS_GB GiB (default 32),
+    // fm2d workspace: problem slots for at most MCT_FM2D_WS_GB GiB (default 64),
     // whole waves of 64; more problems run in chunks
     const long slot = mct_fm2d_slot_bytes(F.ld, F.nc);
-    long ws_gb = 32;
+    long ws_gb = 64;
     {
         const char *e = getenv("MCT_FM2D_WS_GB");
         if (e && atol(e) > 0) ws_gb = atol(e);
