@@ -260,6 +260,66 @@ def run_surface(args):
                   "batches": int(nb - 1), "lanes": 2, "max_batch": min(B, 32)}
         log("served:", served)
 
+    # ---- served mix: three grids in one server -----------------------------------
+    # Requests for three data sets (51x51x21 with 10 periods and 16x16 pairs,
+    # 101x101x41 with 20 periods and 32x32, and the 201x201x61 one above),
+    # Voronoi cell counts U{10..800}, the model of each request uniform at
+    # random; Poisson arrivals at 0.8x the capacity a closed burst measures.
+    mixres = None
+    if args.served_mix > 0:
+        from mctomo_amd.workloads import surface_workload as sw_
+        wm = [sw_(51, 21, 25.0, 10.0, 50, 10, (1.0, 5.0), 16, 16, seed=20261210, order=args.order),
+              sw_(101, 41, 50.0, 20.0, 100, 20, (1.0, 10.0), 32, 32, seed=20261211, order=args.order), w]
+        mixc = 800
+        specs = [dict(grid=x.grid, freqs=x.freqs, src=x.src, rev=x.rev, ttime=x.ttime, raystat=x.raystat,
+                      raylov=1, order=args.order, max_cells=mixc, max_batch=16) for x in wm]
+        srvm = mct.SurfServer.mix(specs, max_lanes=2)
+        rng_m = np.random.default_rng(99)
+
+        def draw(m):
+            g_ = wm[m].grid
+            k = int(rng_m.integers(10, mixc + 1))
+            return _surface_sites(rng_m, k, g_.dx * (g_.nx - 1), g_.dz * (g_.nz - 1), (2.5, 4.5))
+
+        nreq = args.served_mix
+        mreq = [int(v) for v in rng_m.integers(0, 3, 2 * nreq)]
+        reqs_m = [(m,) + draw(m) for m in mreq]
+        burst, live = reqs_m[:nreq], reqs_m[nreq:]
+        t0 = time.perf_counter()
+        for t in [srvm.submit(p_, q_, model=m) for m, p_, q_ in burst]:
+            srvm.wait(t)
+        cap = len(burst) / (time.perf_counter() - t0)
+        rate_m = 0.8 * cap
+        gaps = rng_m.exponential(1.0 / rate_m, len(live))
+        tickets = []
+        t0 = time.perf_counter()
+        for i, (m, p_, q_) in enumerate(live):
+            t_next = t0 + gaps[:i + 1].sum()
+            now = time.perf_counter()
+            if t_next > now:
+                time.sleep(t_next - now)
+            tickets.append((m, srvm.submit(p_, q_, model=m)))
+        res_m = [(m, srvm.wait(t)) for m, t in tickets]
+        t_all = time.perf_counter() - t0
+        used_b, peak_b, lanes_m, rel_m = srvm.memory()
+        nb_m, ne_m, rc_m = srvm.stats()
+        srvm.close()
+        if rc_m or any(r[0] < 0 for _, r in res_m):
+            raise RuntimeError("served mix rc %d" % rc_m)
+        lat_m = np.array([r[2] for _, r in res_m])
+        per = {}
+        for mm, name in enumerate(("51x51x21", "101x101x41", "201x201x61")):
+            lm = np.array([r[2] for m, r in res_m if m == mm])
+            if lm.size:
+                per[name] = {"requests": int(lm.size), "p50_ms": round(float(np.median(lm)), 3),
+                             "p99_ms": round(float(np.quantile(lm, 0.99)), 3)}
+        mixres = {"requests": len(live), "closed_burst_capacity_per_s": round(cap, 3),
+                  "arrival_rate_per_s": round(rate_m, 3), "evals_per_s": round(len(live) / t_all, 3),
+                  "p50_ms": round(float(np.median(lat_m)), 3), "p99_ms": round(float(np.quantile(lat_m, 0.99)), 3),
+                  "rejected_models": int(sum(r[0] > 0 for _, r in res_m)), "per_grid": per,
+                  "lanes": lanes_m, "device_bytes_peak": int(peak_b), "batches": int(nb_m)}
+        log("served mix:", mixres)
+
     # ---- roofline of the dominant kernel, live HIP events ------------------------
     # Two candidates, timed by events on the stream they run on: the dispersion
     # launch k_dispersion_sm (FP64 VALU; algorithmic flops = 170 per dltar
@@ -361,6 +421,7 @@ def run_surface(args):
         "batched_vs_plain": round(batched_eps / plain_eps, 3),
         "plain_stage_ms": [round(v, 3) for v in st_plain],
         "served": served,
+        "served_mix": mixres,
         "perturbed_box": pbox,
         "group_velocity": group,
         "roofline": roof,
@@ -468,6 +529,8 @@ def main():
     ap.add_argument("--served", type=int, default=48, help="requests for the served measurement (0: skip)")
     ap.add_argument("--box-steps", type=int, default=3,
                     help="timed perturbed-box batches (secondary unit; 0: skip)")
+    ap.add_argument("--served-mix", type=int, default=48,
+                    help="requests of the three-grid served mix (0: skip)")
     ap.add_argument("--group-steps", type=int, default=2,
                     help="timed group-velocity batches (secondary figure; 0: skip)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,

@@ -406,8 +406,28 @@ int mct_serve_submit(mct_server *srv, const double *points, const double *params
 int mct_serve_wait(mct_server *srv, int64_t ticket, double *out, int32_t *status, double *latency_ms);
 /* Batches launched and evaluations finished so far; returns the first error. */
 int mct_serve_stats(mct_server *srv, int64_t *batches, int64_t *evals);
+/* A server over a mix of models (no counterpart in the reference): model m is
+ * a data set on its own grid (grids[m], setups[m]) with lanes of max_batch[m]
+ * chains, at most max_lanes lanes per model.  Lanes are admitted by device
+ * footprint: each model starts with one lane (MCT_ERR_NOMEM if those exceed
+ * hbm_budget bytes); more are created on demand while the bytes of all lanes
+ * stay within hbm_budget (0: no limit), and when a model has requests waiting
+ * and no room, idle lanes of models with nothing pending are released, least
+ * recently used first.  The setups' arrays are copied. */
+int mct_serve_create_mix(int nmodels, const mct_grid *grids, const mct_surf_setup *setups, const int *max_batch,
+                         int max_lanes, int64_t hbm_budget, mct_server **srv);
+/* mct_serve_submit for model `model` of a mix (mct_serve_submit is model 0). */
+int mct_serve_submit_to(mct_server *srv, int model, const double *points, const double *params, int ncells,
+                        const double *snoise0, const double *snoise1, int64_t *ticket);
+/* Device bytes held by the lanes now and at most so far, the lanes now, and
+ * how many lanes were released to make room. */
+int mct_serve_memory(mct_server *srv, int64_t *bytes_in_use, int64_t *peak_bytes, int32_t *lanes,
+                     int64_t *lane_releases);
 /* Finishes the pending requests and frees everything. */
 int mct_serve_destroy(mct_server *srv);
+
+/* Device bytes a surface context holds (all its buffers). */
+int mct_surf_device_bytes(mct_surf_ctx *ctx, int64_t *bytes);
 
 /* Device pointers of the context's phase_time (nchain,np,nsrc,nrev) and
  * phase velocity (nchain,nx,ny,np) buffers (valid until destroy). */

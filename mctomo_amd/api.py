@@ -73,7 +73,8 @@ def lib():
                      "mct_surf_likelihood_box", "mct_surf_box_columns", "mct_surf_likelihood_box_batch_dev",
                      "mct_surf_accept", "mct_surf_accept_batch", "mct_surf_group_velocities",
                      "mct_read_sources", "mct_read_times", "mct_write_sources", "mct_write_times",
-                     "mct_body_location_likelihood", "mct_body_location_likelihood_batch_dev"):
+                     "mct_body_location_likelihood", "mct_body_location_likelihood_batch_dev",
+                     "mct_serve_create_mix", "mct_serve_submit_to", "mct_serve_memory", "mct_surf_device_bytes"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -456,7 +457,7 @@ class _SurfSetup(ctypes.Structure):
 
 def _surf_setup(grid, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3, sigdep=0, nrays=None,
                 gridx=1, gridy=1, sgref=1, sgdic=4, sgext=8, order=1, band=0.5, max_cells=1024,
-                max_chains=1):
+                max_chains=1, phase_group=0, straight=0):
     """A filled mct_surf_setup and the arrays it points to (keep them alive)."""
     fr = _f64(freqs).reshape(-1)
     npd = fr.shape[0]
@@ -466,10 +467,10 @@ def _surf_setup(grid, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3, si
     rs = np.ascontiguousarray(raystat, dtype=np.int32).reshape(npd, 2, npair)
     if nrays is None:
         nrays = int((rs[:, 0, :] == 1).sum())
-    st = _SurfSetup(int(raylov), 0, float(dphase), npd, fr.ctypes.data, sr.shape[0], rv.shape[0],
+    st = _SurfSetup(int(raylov), int(phase_group), float(dphase), npd, fr.ctypes.data, sr.shape[0], rv.shape[0],
                     sr.ctypes.data, rv.ctypes.data, tt.ctypes.data, rs.ctypes.data, int(sigdep), int(nrays),
                     int(gridx), int(gridy), int(sgref), int(sgdic), int(sgext), int(order), float(band),
-                    int(max_cells), int(max_chains))
+                    int(max_cells), int(max_chains), int(straight))
     return st, (fr, sr, rv, tt, rs)
 
 
@@ -683,7 +684,7 @@ class SurfServer:
     until that evaluation is done and returns (status, out[3], latency_ms)."""
 
     def __init__(self, grid, freqs, src, rev, ttime, raystat, max_batch=16, nlanes=2, **kw):
-        self.np = len(freqs)
+        self.nps = [len(freqs)]
         st, keep = _surf_setup(grid, freqs, src, rev, ttime, raystat, **kw)
         self._keep = keep
         self._g = grid.c()
@@ -692,14 +693,52 @@ class SurfServer:
                                       ctypes.byref(h)), "mct_serve_create")
         self._h = h
 
-    def submit(self, points, params, snoise0=None, snoise1=None):
+    @classmethod
+    def mix(cls, models, max_lanes=2, hbm_budget=0):
+        """A server over several models: models is a list of dicts with keys
+        grid, freqs, src, rev, ttime, raystat, max_batch and any SurfLikelihood
+        setting.  Lanes are admitted within hbm_budget device bytes (0: no
+        limit).  submit(..., model=m) names the model."""
+        self = cls.__new__(cls)
+        setups, keeps, grids = [], [], []
+        for m in models:
+            m = dict(m)
+            g = m.pop("grid")
+            mb = m.pop("max_batch", 16)
+            st, keep = _surf_setup(g, m.pop("freqs"), m.pop("src"), m.pop("rev"), m.pop("ttime"),
+                                   m.pop("raystat"), **m)
+            setups.append((st, mb))
+            keeps.append(keep)
+            grids.append(g.c())
+        n = len(models)
+        ST = (_SurfSetup * n)(*[s for s, _ in setups])
+        GR = (type(grids[0]) * n)(*grids)
+        MB = (ctypes.c_int * n)(*[int(b) for _, b in setups])
+        self.nps = [int(s.np) for s, _ in setups]
+        self._keep = (keeps, ST, GR, MB)
+        h = ctypes.c_void_p()
+        _check(lib().mct_serve_create_mix(n, GR, ST, MB, int(max_lanes), ctypes.c_int64(int(hbm_budget)),
+                                          ctypes.byref(h)), "mct_serve_create_mix")
+        self._h = h
+        return self
+
+    def submit(self, points, params, snoise0=None, snoise1=None, model=0):
         pts, prm = _f64(points), _f64(params)
-        s0 = None if snoise0 is None else _f64(np.broadcast_to(np.atleast_1d(snoise0), (self.np,)))
-        s1 = None if snoise1 is None else _f64(np.broadcast_to(np.atleast_1d(snoise1), (self.np,)))
+        npm = self.nps[model]
+        s0 = None if snoise0 is None else _f64(np.broadcast_to(np.atleast_1d(snoise0), (npm,)))
+        s1 = None if snoise1 is None else _f64(np.broadcast_to(np.atleast_1d(snoise1), (npm,)))
         t = ctypes.c_int64()
-        _check(lib().mct_serve_submit(self._h, _p(pts), _p(prm), pts.shape[0], _p(s0), _p(s1), ctypes.byref(t)),
-               "mct_serve_submit")
+        _check(lib().mct_serve_submit_to(self._h, int(model), _p(pts), _p(prm), pts.shape[0], _p(s0), _p(s1),
+                                         ctypes.byref(t)), "mct_serve_submit_to")
         return t.value
+
+    def memory(self):
+        """(device bytes in use, peak bytes, lanes, lane releases)."""
+        u, pk, r = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        ln = ctypes.c_int32()
+        _check(lib().mct_serve_memory(self._h, ctypes.byref(u), ctypes.byref(pk), ctypes.byref(ln), ctypes.byref(r)),
+               "mct_serve_memory")
+        return u.value, pk.value, ln.value, r.value
 
     def wait(self, ticket):
         out = np.zeros(3)

@@ -15,13 +15,23 @@
 
 namespace {
 
+// floor(v) + 1 as an int for a cell index (Fortran floor(...) + 1); a
+// non-finite or huge v (a ray step from a zero gradient) gives an index far
+// outside the grid, which every caller clamps or rejects, instead of an
+// undefined conversion.
+__device__ __forceinline__ int cell1(double v)
+{
+    const double f = floor(v);
+    return (f >= -1.0e9 && f <= 1.0e9) ? (int)f + 1 : (f > 0 ? 1000000001 : -1000000000);
+}
+
 // GetVelocity (likelihood_surf.F90:491-521): bilinear value of one period's
 // group velocities, Fortran gvel(ny,nx) strided by np in the dispersion output.
 __device__ __forceinline__ double ray_velocity(const double *gv, int np, int nx, int ny, double xmin,
                                                double ymin, double dx, double dy, double px, double py)
 {
-    int ix = (int)floor((px - xmin) / dx) + 1;
-    int iy = (int)floor((py - ymin) / dy) + 1;
+    int ix = cell1((px - xmin) / dx);
+    int iy = cell1((py - ymin) / dy);
     if (ix < 1) ix = 1;
     if (iy < 1) iy = 1;
     if (ix >= nx) ix = nx - 1;
@@ -107,8 +117,8 @@ __global__ __launch_bounds__(64) void k_fm2d_rays(mct_fm2d_args A)
         R.n++;
     };
     auto refined_cell = [&](double x, double z, int &ipxr, int &ipzr) -> int {
-        ipxr = (int)floor((x - goxr) / dnxr) + 1;
-        ipzr = (int)floor((z - gozr) / dnzr) + 1;
+        ipxr = cell1((x - goxr) / dnxr);
+        ipzr = cell1((z - gozr) / dnzr);
         int ig = 1;
         if (ipxr < 1 || ipxr >= nnxr) ig = 0;
         if (ipzr < 1 || ipzr >= nnzr) ig = 0;
@@ -124,14 +134,14 @@ __global__ __launch_bounds__(64) void k_fm2d_rays(mct_fm2d_args A)
     dpl = 0.5 * dpl;
     int isx, isz;
     if (asgr == 1) {
-        isx = (int)floor((scx - goxr) / dnxr) + 1;
-        isz = (int)floor((scz - gozr) / dnzr) + 1;
+        isx = cell1((scx - goxr) / dnxr);
+        isz = cell1((scz - gozr) / dnzr);
     } else {
-        isx = (int)floor((scx - gox) / dnx) + 1;
-        isz = (int)floor((scz - goz) / dnz) + 1;
+        isx = cell1((scx - gox) / dnx);
+        isz = cell1((scz - goz) / dnz);
     }
-    int ipx = (int)floor((rx - gox) / dnx) + 1;
-    int ipz = (int)floor((rz - goz) / dnz) + 1;
+    int ipx = cell1((rx - gox) / dnx);
+    int ipz = cell1((rz - goz) / dnz);
     if (ipx < 1 || ipx >= nnx || ipz < 1 || ipz >= nnz) {   // the reference stops (:953-960)
         atomicMin(&A.status[c], (int)MCT_ERR_OUTSIDE);
         return;
@@ -179,8 +189,8 @@ __global__ __launch_bounds__(64) void k_fm2d_rays(mct_fm2d_args A)
         double nz_ = cz - dpl * dtz / rd1;
         if (asgr == 1) igref = refined_cell(nx_, nz_, ipxr, ipzr);
         else igref = 0;
-        ipx = (int)floor((nx_ - gox) / dnx) + 1;
-        ipz = (int)floor((nz_ - goz) / dnz) + 1;
+        ipx = cell1((nx_ - gox) / dnx);
+        ipz = cell1((nz_ - goz) / dnz);
         sred = (scx - nx_) * (scx - nx_);
         sred = sred + (scz - nz_) * (scz - nz_);
         sred = sqrt(sred);

@@ -484,18 +484,36 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     return MCT_OK;
 }
 
+}   // extern "C"
+
+// every device buffer of a context
+static std::vector<sbuf *> surf_bufs(mct_surf_ctx *C)
+{
+    return {&C->periods, &C->scx, &C->scz, &C->rcx, &C->rcz, &C->ttime, &C->raystat,
+            &C->points, &C->params, &C->noise, &C->out, &C->status, &C->vs, &C->flag_count,
+            &C->flag_nodes, &C->kd_nodes, &C->kd_ind, &C->kd_rdata, &C->phase, &C->group,
+            &C->ierr, &C->bad_model, &C->ndltar, &C->nlayer, &C->lay, &C->dsp_queue, &C->dsp_jobs,
+            &C->dsp_keys, &C->dsp_hist, &C->velv, &C->veln, &C->phase_time, &C->srdist, &C->sigma,
+            &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap, &C->pflag, &C->rounds,
+            &C->phase_keep, &C->box, &C->rlen, &C->group_keep};
+}
+
+extern "C" {
+
+int mct_surf_device_bytes(mct_surf_ctx *C, int64_t *bytes)
+{
+    if (!C || !bytes) return MCT_ERR_ARG;
+    int64_t b = 0;
+    for (sbuf *s : surf_bufs(C)) b += (int64_t)s->n;
+    *bytes = b;
+    return MCT_OK;
+}
+
 int mct_surf_destroy(mct_surf_ctx *C)
 {
     if (!C) return MCT_OK;
     if (C->stream) (void)hipStreamSynchronize(C->stream);
-    sbuf *bufs[] = {&C->periods, &C->scx, &C->scz, &C->rcx, &C->rcz, &C->ttime, &C->raystat,
-                    &C->points, &C->params, &C->noise, &C->out, &C->status, &C->vs, &C->flag_count,
-                    &C->flag_nodes, &C->kd_nodes, &C->kd_ind, &C->kd_rdata, &C->phase, &C->group,
-                    &C->ierr, &C->bad_model, &C->ndltar, &C->nlayer, &C->lay, &C->dsp_queue, &C->dsp_jobs,
-                    &C->dsp_keys, &C->dsp_hist, &C->velv, &C->veln, &C->phase_time, &C->srdist, &C->sigma,
-                    &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap, &C->pflag, &C->rounds,
-                    &C->phase_keep, &C->box, &C->rlen, &C->group_keep};
-    for (sbuf *b : bufs) b->release();
+    for (sbuf *b : surf_bufs(C)) b->release();
     if (C->h_in) (void)hipHostFree(C->h_in);
     if (C->h_out) (void)hipHostFree(C->h_out);
     if (C->h_status) (void)hipHostFree(C->h_status);

@@ -70,3 +70,88 @@ def test_results_alone_equal_results_in_the_mix():
         if st_a == 0:
             np.testing.assert_array_equal(out_a, out_m, err_msg="request %d" % i)
         assert lat > 0
+
+
+def _two_models():
+    wa = W.surface_workload(n=33, nz=17, L=16.0, Z=8.0, ncells=30, np_=6, nsrc=8, nrev=8, seed=31)
+    wb = W.surface_workload(n=25, nz=13, L=12.0, Z=6.0, ncells=20, np_=4, nsrc=6, nrev=6, seed=32)
+    return wa, wb
+
+
+def _spec(w, max_batch):
+    return dict(grid=w.grid, freqs=w.freqs, src=w.src, rev=w.rev, ttime=w.ttime, raystat=w.raystat,
+                order=1, max_cells=60, max_batch=max_batch)
+
+
+def _alone(w, ms):
+    plain = mct.SurfLikelihood(w.grid, w.freqs, w.src, w.rev, w.ttime, w.raystat, order=1, max_cells=60)
+    out = [plain.likelihood(p, q) for p, q in ms]
+    plain.close()
+    return out
+
+
+def test_mixed_models_alone_equal_mix():
+    """Two data sets on different grids (33x33x17 with 6 periods, 25x25x13 with
+    4) served by one server: requests of both arrive interleaved from several
+    threads; each result equals its model's plain call bit for bit."""
+    wa, wb = _two_models()
+    ma, mb = models(wa, 24, 11), models(wb, 24, 12)
+    alone = {0: _alone(wa, ma), 1: _alone(wb, mb)}
+    srv = mct.SurfServer.mix([_spec(wa, 6), _spec(wb, 6)], max_lanes=2)
+    jobs = [(0, i) for i in range(len(ma))] + [(1, i) for i in range(len(mb))]
+    np.random.default_rng(5).shuffle(jobs)
+    tickets = {}
+
+    def client(part):
+        for m, i in part:
+            p, q = (ma if m == 0 else mb)[i]
+            tickets[(m, i)] = srv.submit(p, q, model=m)
+
+    th = [threading.Thread(target=client, args=(jobs[k::3],)) for k in range(3)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    res = {k: srv.wait(t) for k, t in tickets.items()}
+    b, e, rc = srv.stats()
+    used, peak, lanes, rel = srv.memory()
+    srv.close()
+    say("mix: batches", b, "evals", e, "lanes", lanes, "peak MB %.1f" % (peak / 2**20))
+    assert rc == 0 and e == len(jobs)
+    for (m, i), (st, out, lat) in res.items():
+        st_a, out_a = alone[m][i]
+        assert st == st_a, (m, i)
+        if st == 0:
+            np.testing.assert_array_equal(out, out_a, err_msg="model %d request %d" % (m, i))
+
+
+def test_hbm_budget_moves_lanes_between_models():
+    """With a device budget of exactly one lane per model, a burst for one
+    model takes a second lane only by releasing the other model's idle lane;
+    the footprint never exceeds the budget and every result is still the plain
+    call's."""
+    wa, wb = _two_models()
+    probe = mct.SurfServer.mix([_spec(wa, 4), _spec(wb, 4)], max_lanes=2)
+    budget, _, lanes0, _ = probe.memory()
+    probe.close()
+    assert lanes0 == 2 and budget > 0
+    ma, mb = models(wa, 20, 21), models(wb, 20, 22)
+    alone = {0: _alone(wa, ma), 1: _alone(wb, mb)}
+    srv = mct.SurfServer.mix([_spec(wa, 4), _spec(wb, 4)], max_lanes=2, hbm_budget=budget)
+    # a burst for model 0 (more than one lane's batch), then one for model 1
+    ta = [srv.submit(p, q, model=0) for p, q in ma]
+    ra = [srv.wait(t) for t in ta]
+    tb = [srv.submit(p, q, model=1) for p, q in mb]
+    rb = [srv.wait(t) for t in tb]
+    used, peak, lanes, rel = srv.memory()
+    b, e, rc = srv.stats()
+    srv.close()
+    say("budget %.1f MB: peak %.1f MB, lanes %d, releases %d, batches %d" % (budget / 2**20, peak / 2**20, lanes, rel, b))
+    assert rc == 0 and e == 40
+    assert peak <= budget
+    for m, rs in ((0, ra), (1, rb)):
+        for i, (st, out, lat) in enumerate(rs):
+            st_a, out_a = alone[m][i]
+            assert st == st_a
+            if st == 0:
+                np.testing.assert_array_equal(out, out_a)
