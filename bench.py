@@ -141,6 +141,17 @@ def run_surface(args):
     log("2-D sweep (plain): slots %d, per problem: rounds %.2f, diagonal steps %.0f, node updates %.0f"
         % like.fm2d_stats())
 
+    # ---- launch shape picked by the autotuner (results unchanged, checked) ----------
+    tuned = None
+    if not args.no_autotune:
+        from mctomo_amd import autotune
+        # one 2-D launch per batch, so that the roofline below times a whole launch
+        rep = autotune.tune(like, lambda: run(B), lambda: [dO.numpy(), dS.numpy()], B, repeats=1, log=log,
+                            chunks=False)
+        tuned = {k: rep[k] for k in ("dispersion", "sweep", "chunk")}
+        tuned["stage_ms"] = {st: {k: round(float(np.median(v)), 2) for k, v in tab.items()}
+                             for st, tab in rep["times_ms"].items()}
+
     # ---- batched: the timed region -----------------------------------------------
     for _ in range(args.warmup):
         run(B)
@@ -422,6 +433,7 @@ def run_surface(args):
         "plain_stage_ms": [round(v, 3) for v in st_plain],
         "served": served,
         "served_mix": mixres,
+        "autotune": tuned,
         "perturbed_box": pbox,
         "group_velocity": group,
         "roofline": roof,
@@ -529,6 +541,7 @@ def main():
     ap.add_argument("--served", type=int, default=48, help="requests for the served measurement (0: skip)")
     ap.add_argument("--box-steps", type=int, default=3,
                     help="timed perturbed-box batches (secondary unit; 0: skip)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--served-mix", type=int, default=48,
                     help="requests of the three-grid served mix (0: skip)")
     ap.add_argument("--group-steps", type=int, default=2,

@@ -371,6 +371,17 @@ int mct_surf_set_method(mct_surf_ctx *ctx, int fm2d_method);
  * columns taken from a queue in layer-count order.  Results are the same. */
 int mct_surf_set_dispersion_mode(mct_surf_ctx *ctx, int mode);
 
+/* Launch shape of a context, for tuning (no counterpart in the reference;
+ * results do not depend on it): MCT_TUNE_DSP_WAVES waves of the queue
+ * dispersion kernel (0: one per 64 columns), MCT_TUNE_SWEEP_KIND the 2-D
+ * sweep (0 default, 1 strip-pipelined, 2 anti-diagonal), MCT_TUNE_SWEEP_THREADS
+ * threads per problem of the anti-diagonal sweep (0 default, 64, 128, 256),
+ * MCT_TUNE_FM2D_CHUNK 2-D problems per launch (0: all the workspace holds).
+ * MCT_ERR_ARG for an unknown key or value.  get returns the value in use. */
+enum { MCT_TUNE_DSP_WAVES = 1, MCT_TUNE_SWEEP_KIND = 2, MCT_TUNE_SWEEP_THREADS = 3, MCT_TUNE_FM2D_CHUNK = 4 };
+int mct_surf_set_tuning(mct_surf_ctx *ctx, int key, int64_t value);
+int mct_surf_get_tuning(mct_surf_ctx *ctx, int key, int64_t *value);
+
 /* Work of the last call's dispersion stage over its first nchain chains:
  * evals = period-equation evaluations of surfdisp96's sequential root walk
  * (dltar calls), layer_evals = the same weighted by (layers - 1), the

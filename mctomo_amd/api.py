@@ -74,7 +74,8 @@ def lib():
                      "mct_surf_accept", "mct_surf_accept_batch", "mct_surf_group_velocities",
                      "mct_read_sources", "mct_read_times", "mct_write_sources", "mct_write_times",
                      "mct_body_location_likelihood", "mct_body_location_likelihood_batch_dev",
-                     "mct_serve_create_mix", "mct_serve_submit_to", "mct_serve_memory", "mct_surf_device_bytes"):
+                     "mct_serve_create_mix", "mct_serve_submit_to", "mct_serve_memory", "mct_surf_device_bytes",
+                     "mct_surf_set_tuning", "mct_surf_get_tuning"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -663,6 +664,18 @@ class SurfLikelihood:
         """Dispersion launch shape: 0 by size, 1 lane per column, 2 wave per column,
         3 lane per column with columns taken from a queue."""
         _check(lib().mct_surf_set_dispersion_mode(self._h, int(mode)), "mct_surf_set_dispersion_mode")
+
+    TUNE_DSP_WAVES, TUNE_SWEEP_KIND, TUNE_SWEEP_THREADS, TUNE_FM2D_CHUNK = 1, 2, 3, 4
+
+    def set_tuning(self, key, value):
+        """Launch shape (results do not depend on it): TUNE_DSP_WAVES, TUNE_SWEEP_KIND
+        (0 default, 1 strip, 2 anti-diagonal), TUNE_SWEEP_THREADS, TUNE_FM2D_CHUNK."""
+        _check(lib().mct_surf_set_tuning(self._h, int(key), ctypes.c_int64(int(value))), "mct_surf_set_tuning")
+
+    def get_tuning(self, key):
+        v = ctypes.c_int64()
+        _check(lib().mct_surf_get_tuning(self._h, int(key), ctypes.byref(v)), "mct_surf_get_tuning")
+        return v.value
 
     def fm2d_stats(self):
         """(slots, mean rounds, mean diagonal steps, mean node updates) of the last 2-D chunk."""

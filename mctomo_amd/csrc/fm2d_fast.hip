@@ -1723,15 +1723,17 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
     const size_t dirty_lds = use_dirty ? dirty_bytes : 0;
     // threads per 2-D sweep (one problem per workgroup; one wave measured fastest):
     // environment MCT_SWEEP_THREADS
-    static int sweep_threads = 0;
-    if (!sweep_threads) {
+    static int env_threads = 0;
+    if (!env_threads) {
         const char *e = getenv("MCT_SWEEP_THREADS");
         int v = e ? atoi(e) : 64;
-        sweep_threads = (v == 64 || v == 128 || v == 256) ? v : 64;
+        env_threads = (v == 64 || v == 128 || v == 256) ? v : 64;
     }
-    // the strip-pipelined sweep unless disabled (MCT_SWEEP_KIND=diag) or too large for LDS
+    const int sweep_threads = a.sweep_threads ? a.sweep_threads : env_threads;
+    // the strip-pipelined sweep unless disabled (MCT_SWEEP_KIND=diag, or the
+    // context's setting) or too large for LDS
     const char *sk = getenv("MCT_SWEEP_KIND");
-    const int sweep_kind = (sk && strcmp(sk, "diag") == 0) ? 0 : 1;
+    const int sweep_kind = a.sweep_kind ? (a.sweep_kind == 1 ? 1 : 0) : ((sk && strcmp(sk, "diag") == 0) ? 0 : 1);
     const size_t strip_lds = sweep_kind ? ff_strip_lds(a) : 0;
     const long nprob = (long)a.nchain * a.np * a.nsrc;
     if (chunk < 1) return MCT_ERR_ARG;

@@ -182,6 +182,9 @@ struct mct_fm2d_args {
     int rays;
     const double *gvel;         // (nchain, nvx*nvz, np) group velocities, column iy + nvz*ix (0-based)
     double *rlen;               // (nchain, np, nsrc, nrc) ray lengths, receiver fastest
+    // launch shape of the parallel path (0: the environment's or the default)
+    int sweep_kind;             // 1 strip-pipelined, 2 anti-diagonal
+    int sweep_threads;          // anti-diagonal sweep: 64, 128 or 256 threads per problem
 };
 // Coarse B-spline velocity per (chain, period), then one lane per
 // (chain, period, source) running the sequential modrays phase solve.

@@ -84,6 +84,7 @@ struct mct_surf_ctx {
     long fm2d_chunk;
     int fm2d_method;                 // 0 parallel sweeps, 1 sequential heap loop
     long dsp_waves;                  // queue mode: at most this many waves (0: one per 64 columns)
+    long fm2d_run_chunk = 0;         // 2-D problems per launch (0: the workspace's capacity)
     int dsp_unsorted;                // queue mode: columns in grid order instead of by layer count
     int dsp_mode;                    // 0 by size, 1 lane per column, 2 wave per column,
                                      // 3 lane per column from a column queue
@@ -223,7 +224,8 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     }
     mct_fm2d_args fa = C->F;
     const long wsz = (long)fa.ld * fa.nc;
-    const long chunk = C->fm2d_chunk / nparts, soff = chunk * part;
+    const long run = C->fm2d_run_chunk > 0 && C->fm2d_run_chunk < C->fm2d_chunk ? C->fm2d_run_chunk : C->fm2d_chunk;
+    const long chunk = run / nparts, soff = chunk * part;
     fa.nchain = nchain;
     fa.velv = velv;
     fa.veln = C->veln.as<double>() + (long)c0 * fa.nnx * fa.nnz * np;
@@ -824,6 +826,43 @@ int mct_surf_set_dispersion_mode(mct_surf_ctx *C, int mode)
     if (!C || mode < 0 || mode > 3) return MCT_ERR_ARG;
     C->dsp_mode = mode;
     return MCT_OK;
+}
+
+int mct_surf_set_tuning(mct_surf_ctx *C, int key, int64_t value)
+{
+    if (!C) return MCT_ERR_ARG;
+    switch (key) {
+    case MCT_TUNE_DSP_WAVES:
+        if (value < 0) return MCT_ERR_ARG;
+        C->dsp_waves = (long)value;
+        return MCT_OK;
+    case MCT_TUNE_SWEEP_KIND:
+        if (value < 0 || value > 2) return MCT_ERR_ARG;
+        C->F.sweep_kind = (int)value;
+        return MCT_OK;
+    case MCT_TUNE_SWEEP_THREADS:
+        if (value != 0 && value != 64 && value != 128 && value != 256) return MCT_ERR_ARG;
+        C->F.sweep_threads = (int)value;
+        return MCT_OK;
+    case MCT_TUNE_FM2D_CHUNK:
+        if (value < 0) return MCT_ERR_ARG;
+        C->fm2d_run_chunk = (long)value;
+        return MCT_OK;
+    default:
+        return MCT_ERR_ARG;
+    }
+}
+
+int mct_surf_get_tuning(mct_surf_ctx *C, int key, int64_t *value)
+{
+    if (!C || !value) return MCT_ERR_ARG;
+    switch (key) {
+    case MCT_TUNE_DSP_WAVES: *value = C->dsp_waves; return MCT_OK;
+    case MCT_TUNE_SWEEP_KIND: *value = C->F.sweep_kind; return MCT_OK;
+    case MCT_TUNE_SWEEP_THREADS: *value = C->F.sweep_threads; return MCT_OK;
+    case MCT_TUNE_FM2D_CHUNK: *value = C->fm2d_run_chunk > 0 ? C->fm2d_run_chunk : C->fm2d_chunk; return MCT_OK;
+    default: return MCT_ERR_ARG;
+    }
 }
 
 int mct_surf_dispersion_work(mct_surf_ctx *C, int nchain, int64_t *evals, int64_t *layer_evals)

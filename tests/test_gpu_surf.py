@@ -401,3 +401,39 @@ def test_perturbed_box_batch_equals_whole_grid_batch():
     np.testing.assert_array_equal(tb, dT.numpy())
     np.testing.assert_array_equal(ob, dO.numpy())
     L.close()
+
+
+def test_autotune_keeps_results_and_caches_its_choice(tmp_path):
+    """The launch-shape tuner on a small batch: every shape it times gives the
+    same results (it checks), the chosen shape stays set and the batch still
+    equals the untuned one; the cache reapplies the choice without timing."""
+    from mctomo_amd import autotune
+    base = W.surface_workload(n=33, nz=17, L=16.0, Z=8.0, ncells=40, np_=6, nsrc=8, nrev=8, seed=21)
+    nchain, maxc = 4, 40
+    pts = np.zeros((nchain, maxc, 3))
+    prm = np.zeros((nchain, maxc, 3))
+    ncl = np.zeros(nchain, np.int32)
+    for c in range(nchain):
+        w = W.surface_workload(n=33, nz=17, L=16.0, Z=8.0, ncells=30, np_=6, nsrc=8, nrev=8, seed=300 + c)
+        pts[c, :30], prm[c, :30], ncl[c] = w.points, w.params, 30
+    L = ctx_for(base, max_cells=maxc, max_chains=nchain)
+    dP, dQ, dN = (mct.DeviceArray.from_numpy(a) for a in (pts, prm, ncl))
+    dS = mct.DeviceArray.from_numpy(np.zeros((nchain, 2, 6)))
+    dO, dT = mct.DeviceArray((nchain, 3), np.float64), mct.DeviceArray((nchain,), np.int32)
+
+    def run():
+        L.likelihood_batch_dev(nchain, dP.ptr, dQ.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
+        L.sync()
+
+    run()
+    before = dO.numpy().copy()
+    path = str(tmp_path / "tune.json")
+    rep = autotune.tune(L, run, lambda: [dO.numpy(), dT.numpy()], nchain, repeats=1, cache=path, log=say)
+    say("autotune", {k: rep[k] for k in ("dispersion", "sweep", "chunk")})
+    run()
+    np.testing.assert_array_equal(dO.numpy(), before)
+    L2 = ctx_for(base, max_cells=maxc, max_chains=nchain)
+    rep2 = autotune.tune(L2, None, None, nchain, cache=path)
+    assert rep2["cached"] and rep2["sweep"] == rep["sweep"]
+    L.close()
+    L2.close()
