@@ -296,6 +296,41 @@ def ref_read_data(datatype, sources_file, receivers_file, data_file):
     return dict(src=src, rev=rev, freqs=fr, ttime=tt, raystat=rs, nrays=nrays, nmodes=nmodes)
 
 
+def ref_surf_forward(grid, points, params, freqs, src, rev, raystat, raylov=1, dphase=1e-3, gridx=1,
+                     gridy=1, sgref=1, sgdic=4, sgext=8, order=1, band=0.5, nthreads=0):
+    """The surface-wave forward model with the reference's own compiled stages
+    doing the work: nearest site and convert_to_layer from the oracle, then
+    surfdisp96 per column and modrays per period from oracle/_ref (OpenMP over
+    columns and over periods, as surf_likelihood).  Returns (nbad, pvel
+    (nx,ny,np), phase_time (np,nsrc,nrev)) or None where oracle/_ref is not built."""
+    R = globals()["ref"]()
+    if R is None:
+        return None
+    nx, ny, nz = grid["nx"], grid["ny"], grid["nz"]
+    _, _, vs, _ = kdtree_to_grid(grid, points, params)
+    ncol = nx * ny
+    maxl = nz
+    lay = np.zeros((ncol, maxl, 4))
+    nl = np.zeros(ncol, np.int32)
+    if lib().orc_column_layers_all(ctypes.c_long(ncol), nz, _p(c64(vs)), ctypes.c_double(grid["dz"]),
+                                   ctypes.c_double(grid.get("scaling", 1.0)), maxl, _p(lay), _p(nl)):
+        raise RuntimeError("layer count above nz")
+    fr = c64(freqs).reshape(-1)
+    npp = fr.shape[0]
+    s, r = c64(src).reshape(-1, 2), c64(rev).reshape(-1, 2)
+    nsrc, nrev = s.shape[0], r.shape[0]
+    rs = np.ascontiguousarray(raystat, dtype=np.int32).reshape(npp, 2, nsrc * nrev)
+    pv = np.zeros((ncol, npp))
+    pt = np.zeros((npp, nsrc, nrev))
+    nbad = ctypes.c_int()
+    R.ref_surf_forward(int(nthreads), ncol, maxl, _p(nl), _p(lay), npp, _p(fr), 2 if raylov == 1 else 1,
+                       ctypes.c_double(dphase), nx, ny, ctypes.c_double(grid["xmin"]), ctypes.c_double(grid["ymin"]),
+                       ctypes.c_double(grid["dx"]), ctypes.c_double(grid["dy"]), nsrc, _p(c64(s[:, 0])),
+                       _p(c64(s[:, 1])), nrev, _p(c64(r[:, 0])), _p(c64(r[:, 1])), _p(rs), gridx, gridy, sgref,
+                       sgdic, sgext, order, ctypes.c_double(band), _p(pv), _p(pt), ctypes.byref(nbad))
+    return nbad.value, pv.reshape(nx, ny, npp), pt
+
+
 # ---------------------------------------------------------------- surface likelihood
 def surf_likelihood(grid, points, params, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3,
                     sigdep=0, snoise0=None, snoise1=None, nrays=None, gridx=1, gridy=1, sgref=1,

@@ -175,6 +175,70 @@ contains
         enddo
     end subroutine
 
+    ! The reference's surface-wave forward stages composed as surf_likelihood
+    ! composes them (src/likelihood_surf.F90:185-330), OpenMP over columns and
+    ! over periods as there: surfmodes' surfdisp96 call per column on the layers
+    ! convert_to_layer produced (lay(4,maxl,ncol) = thick, vp, vs, rho; nl(ncol)),
+    ! the like%vel padding, modrays per period (uar = 1).  pvel(np,ncol) and
+    ! ptime(nrev,nsrc,np); nbad counts columns with ierr /= 0 (then no modrays).
+    subroutine ref_surf_forward(nthreads, ncol, maxl, nl, lay, np, freqs, iwave, dphase, nx, ny, &
+                                xmin, ymin, dx, dy, nsrc, scx, scz, nrev, rcx, rcz, srs, gdx, gdz, asgr, &
+                                sgdl, sgs, fom, snb, pvel, ptime, nbad) bind(C, name="ref_surf_forward")
+        use m_fm2d, only : modrays, T_RAY
+        use omp_lib
+        integer(c_int), value :: nthreads, ncol, maxl, np, iwave, nx, ny, nsrc, nrev
+        integer(c_int), value :: gdx, gdz, asgr, sgdl, sgs, fom
+        integer(c_int), intent(in) :: nl(ncol), srs(nsrc*nrev, 2, np)
+        real(c_double), intent(in) :: lay(4, maxl, ncol), freqs(np)
+        real(c_double), value :: dphase, xmin, ymin, dx, dy, snb
+        real(c_double), intent(in) :: scx(nsrc), scz(nsrc), rcx(nrev), rcz(nrev)
+        real(c_double), intent(out) :: pvel(np, ncol), ptime(nrev, nsrc, np)
+        integer(c_int), intent(out) :: nbad
+        real(4) :: d4(200), a4(200), b4(200), r4(200)
+        real(8) :: tt(60), cg(60)
+        real(8), allocatable :: velv(:,:,:), vv(:,:)
+        type(T_RAY), allocatable :: rays(:)
+        integer :: c, n, ie, i, ix, iy, cr, srsl(nsrc*nrev, 2)
+        if (nthreads > 0) call omp_set_num_threads(nthreads)
+        tt = 0
+        tt(1:np) = dble(1/freqs)
+        nbad = 0
+        !$omp parallel do private(c, n, d4, a4, b4, r4, cg, ie) reduction(+:nbad) schedule(dynamic, 16)
+        do c = 1, ncol
+            n = nl(c)
+            d4 = 0; a4 = 0; b4 = 0; r4 = 0
+            d4(1:n) = real(lay(1, 1:n, c), 4)
+            a4(1:n) = real(lay(2, 1:n, c), 4)
+            b4(1:n) = real(lay(3, 1:n, c), 4)
+            r4(1:n) = real(lay(4, 1:n, c), 4)
+            call surfdisp96(d4, a4, b4, r4, n, 0, iwave, 1, 0, np, tt, dphase, pvel(:, c), cg, ie)
+            if (ie /= 0) nbad = nbad + 1
+        enddo
+        !$omp end parallel do
+        if (nbad > 0) return
+        allocate(velv(np, ny + 2, nx + 2))
+        do ix = 1, nx + 2
+            do iy = 1, ny + 2
+                velv(:, iy, ix) = pvel(:, min(max(iy - 1, 1), ny) + ny * (min(max(ix - 1, 1), nx) - 1))
+            enddo
+        enddo
+        !$omp parallel do private(i, vv, rays, cr, srsl) schedule(dynamic, 1)
+        do i = 1, np
+            allocate(vv(ny + 2, nx + 2), rays(nsrc * nrev))
+            vv = velv(i, :, :)
+            rays%srcid = 0
+            rays%revid = 0
+            rays%npoints = 0
+            cr = 0
+            srsl = srs(:, :, i)
+            call modrays(nsrc, scx, scz, nrev, rcx, rcz, srsl, 0, nx, ny, xmin, ymin, dx, dy, vv, &
+                         gdx, gdz, asgr, sgdl, sgs, 6371.0d0, fom, snb, ptime(:, :, i), rays, cr, 1)
+            deallocate(vv, rays)
+        enddo
+        !$omp end parallel do
+        deallocate(velv)
+    end subroutine
+
     function cstr(c) result(f)
         character(kind=c_char), intent(in) :: c(*)
         character(len=100) :: f

@@ -54,6 +54,24 @@ static int column_layers(const double *vs, int nz, double dz, double scaling,
     return nl + 1;
 }
 
+int orc_column_layers_all(long ncol, int nz, const double *vs, double dz, double scaling, int maxl,
+                          double *lay, int32_t *nl)
+{
+    int bad = 0;
+    #pragma omp parallel for schedule(static) reduction(|:bad)
+    for (long c = 0; c < ncol; c++) {
+        double th[SURF_NMAX + 1], a[SURF_NMAX + 1], b[SURF_NMAX + 1], r[SURF_NMAX + 1];
+        int n = column_layers(vs + c * nz, nz, dz, scaling, th, a, b, r);
+        if (n > maxl) { bad = 1; continue; }
+        nl[c] = n;
+        for (int k = 0; k < n; k++) {
+            double *o = lay + ((size_t)c * maxl + k) * 4;
+            o[0] = th[k]; o[1] = a[k]; o[2] = b[k]; o[3] = r[k];
+        }
+    }
+    return bad ? -1 : 0;
+}
+
 int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double zmin,
                         double dx, double dy, double dz, double scaling,
                         const double *points, const double *params, int ncells,

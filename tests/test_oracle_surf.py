@@ -118,3 +118,21 @@ def test_vs_only_vp_and_rho_of_the_sites_are_ignored(small):
                                        w.ttime, w.raystat)
     assert rc2 == 0
     np.testing.assert_array_equal(out, out2)
+
+
+def test_reference_stages_equal_the_oracle_live():
+    """surfdisp96 per column and modrays per period from the reference's own
+    compiled Fortran (oracle/_ref, OpenMP as its OMP build), composed as
+    surf_likelihood composes them, equal the oracle's phase velocities and
+    phase times bit for bit (orders 0 and 1)."""
+    if po.ref() is None:
+        pytest.skip("oracle/_ref not built here")
+    for order in (0, 1):
+        w = W.surface_workload(n=41, nz=21, ncells=30, np_=10, nsrc=16, nrev=16, seed=20261202, order=order)
+        g = gdict(w.grid)
+        nbad, pv, pt = po.ref_surf_forward(g, w.points, w.params, w.freqs, w.src, w.rev, w.raystat,
+                                           order=order, nthreads=4)
+        res = po.surf_likelihood(g, w.points, w.params, w.freqs, w.src, w.rev, w.ttime, w.raystat, order=order)
+        assert nbad == 0 and res[0] == 0
+        np.testing.assert_array_equal(pv, res[5])
+        np.testing.assert_array_equal(pt, res[2])
