@@ -1,16 +1,20 @@
-// surf_capi.hip -- surface-wave likelihood context (datatype 2, phase
-// velocities) behind the C ABI declared in include/mctomo_hip.h.
+// surf_capi.hip -- surface-wave likelihood context (datatype 2, phase or
+// group velocities) behind the C ABI declared in include/mctomo_hip.h.
 //
 // One call is the reference's likelihood() for datatype 2 (src/likelihood.f90
-// :72-80 -> src/likelihood_surf.F90:110-405) over the whole grid, for one
-// chain (host arrays) or many chains (device arrays), all on the context's
-// stream:
+// :72-80 -> src/likelihood_surf.F90:110-405) over the whole grid or a
+// perturbed box, for one chain (host arrays) or many chains (device arrays),
+// all on the context's stream:
 //   nearest site -> vs grid (Fortran (nz,ny,nx) per chain)
-//   k_dispersion: check_model, convert_to_layer, surfdisp96 per column
+//   dispersion: check_model, convert_to_layer, surfdisp96 per column (the
+//     box's columns only in a perturbed-box call)
 //   k_surf_status: reject chains with check_model / dispersion failures
 //   k_surf_pad: like%vel(np, ny+2, nx+2) with replicated edges
-//   fm2d: B-spline gridder + one lane per (chain, period, source) modrays solve
-//   misfit: srdist = phaseTime, sigma, Gaussian misfit
+//   fm2d: B-spline gridder, the near-source kernel, the coarse sweeps (or the
+//     sequential solve) per (chain, period, source), then rays and group
+//     delays with group velocities; or straight rays and no 2-D solve
+//   misfit: srdist = phaseTime (ray lengths / distances with rays), sigma,
+//     Gaussian misfit
 #include "mct_internal.h"
 #include <vector>
 #include <float.h>

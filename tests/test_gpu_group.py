@@ -233,3 +233,22 @@ def test_straight_rays_match_oracle(phase_group):
     rc3, full = L.likelihood(w.points, w.params, snoise0=s0 * 2, snoise1=s1)
     assert rc2 == rc3 == 0 and np.array_equal(cheap[:2], full[:2])
     L.close()
+
+
+def test_straight_ray_out_of_area_is_fatal_like_the_oracle():
+    """A receiver outside the model puts straight-ray points outside the area:
+    the reference stops ('Straight ray out of area'); the call returns
+    MCT_ERR_OUTSIDE and the oracle its fatal status."""
+    w = W.surface_workload(n=41, nz=21, ncells=30, np_=4, nsrc=4, nrev=4, seed=20261202, order=0)
+    rev = w.rev.copy()
+    rev[2, 0] = (w.grid.nx - 1) * w.grid.dx + 3.0
+    L = mct.SurfLikelihood(w.grid, w.freqs, w.src, rev, w.ttime, w.raystat, raylov=w.raylov, order=0,
+                           straight=1)
+    with pytest.raises(mct.MCTError) as e:
+        L.likelihood(w.points, w.params)
+    assert e.value.code == -6
+    res = po.surf_likelihood(gdict(w.grid), w.points, w.params, w.freqs, w.src, rev, w.ttime, w.raystat,
+                             raylov=w.raylov, order=0, straight=1)
+    assert res[0] < 0
+    # the context stays usable for a model within the area
+    L.close()

@@ -155,3 +155,23 @@ def test_hbm_budget_moves_lanes_between_models():
             assert st == st_a
             if st == 0:
                 np.testing.assert_array_equal(out, out_a)
+
+
+def test_mix_rejects_unknown_models_and_oversized_requests():
+    wa, wb = _two_models()
+    srv = mct.SurfServer.mix([_spec(wa, 4), _spec(wb, 4)], max_lanes=1)
+    p, q = models(wa, 1, 3)[0]
+    with pytest.raises(mct.MCTError):
+        srv.submit(p, q, model=2)
+    big = np.zeros((61, 3))
+    with pytest.raises(mct.MCTError):
+        srv.submit(big, big, model=0)
+    st, out, lat = srv.wait(srv.submit(p, q, model=0))
+    assert st == 0
+    srv.close()
+    # a budget below one lane per model cannot start
+    probe = mct.SurfServer.mix([_spec(wa, 4)], max_lanes=1)
+    one, _, _, _ = probe.memory()
+    probe.close()
+    with pytest.raises(mct.MCTError):
+        mct.SurfServer.mix([_spec(wa, 4), _spec(wb, 4)], max_lanes=1, hbm_budget=one)
