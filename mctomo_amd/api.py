@@ -737,9 +737,10 @@ class SurfServer:
 
     def submit(self, points, params, snoise0=None, snoise1=None, model=0):
         pts, prm = _f64(points), _f64(params)
-        npm = self.nps[model]
-        s0 = None if snoise0 is None else _f64(np.broadcast_to(np.atleast_1d(snoise0), (npm,)))
-        s1 = None if snoise1 is None else _f64(np.broadcast_to(np.atleast_1d(snoise1), (npm,)))
+        known = 0 <= int(model) < len(self.nps)      # the library rejects other models
+        npm = self.nps[model] if known else 1
+        s0 = None if snoise0 is None or not known else _f64(np.broadcast_to(np.atleast_1d(snoise0), (npm,)))
+        s1 = None if snoise1 is None or not known else _f64(np.broadcast_to(np.atleast_1d(snoise1), (npm,)))
         t = ctypes.c_int64()
         _check(lib().mct_serve_submit_to(self._h, int(model), _p(pts), _p(prm), pts.shape[0], _p(s0), _p(s1),
                                          ctypes.byref(t)), "mct_serve_submit_to")

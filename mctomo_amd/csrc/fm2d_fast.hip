@@ -611,8 +611,9 @@ __device__ __forceinline__ int ff_sweep(const fview &V, const FX &fixed, const V
                         uint32_t m = 1u << (q & 31);
                         if (!(W->bits[q >> 5] & m)) continue;
                         atomicAnd(&W->bits[q >> 5], ~m);
+                    } else if (fixed(z, x)) {
+                        continue;   // with the work list, fixed nodes are never marked
                     }
-                    if (fixed(z, x)) continue;
                     n_evals++;
                     ff_sten S;
                     ff_load_sten<FOM>(V, z, x, S);
@@ -1306,10 +1307,11 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
     // fixed flags only in the neighbourhood (the sweeps read them nowhere else)
     double *T = A.ttn + slot * wsz;
     int32_t *F = A.nsts + slot * wsz;
-    for (long q = lane; q < (long)A.nnx * A.nnz; q += 64) {
-        int k = (int)(q / A.nnz) + 1, l = (int)(q % A.nnz) + 1;
-        long o = (long)(k - 1) * A.ld + (l - 1);
-        T[o] = (C.local(l, k) && C.n(l, k) == 0) ? C.t(l, k) : FF_BIG;
+    for (int k = 1; k <= A.nnx; k++) {
+        double *Tc = T + (long)(k - 1) * A.ld;
+        const bool wk = k > C.ox && k <= C.ox + C.lnx;
+        for (int l = lane + 1; l <= A.nnz; l += 64)
+            Tc[l - 1] = (wk && l > C.oz && l <= C.oz + C.lnz && C.n(l, k) == 0) ? C.t(l, k) : FF_BIG;
     }
     for (int q = lane; q < C.lnx * C.lnz; q += 64) {
         int k = C.ox + q / C.lnz + 1, l = C.oz + q % C.lnz + 1;
