@@ -302,6 +302,17 @@ int mct_surf_likelihood_batch_dev(mct_surf_ctx *ctx, int nchain, const double *p
                                   const double *params_dev, const int32_t *ncells_dev,
                                   const double *noise_dev, double *out_dev, int32_t *status_dev);
 int mct_surf_sync(mct_surf_ctx *ctx);
+/* mct_surf_likelihood_batch_dev replayed from a HIP graph (no counterpart in
+ * the reference): the first call with a given nchain and set of device
+ * pointers captures the context's launch sequence for one batch (its flag
+ * clears are kernels, not memsets) and later calls with the same arguments
+ * replay it with one launch (a context keeps up to 64 such graphs; the
+ * served form's lanes replay one per batch size).  Stage timings are not recorded in graph mode.
+ * A batch that runs in parts (MCT_SURF_SPLIT) is MCT_ERR_ARG.  Results equal
+ * the eager call's. */
+int mct_surf_likelihood_batch_graph(mct_surf_ctx *ctx, int nchain, const double *points_dev,
+                                    const double *params_dev, const int32_t *ncells_dev,
+                                    const double *noise_dev, double *out_dev, int32_t *status_dev);
 
 /* likelihood() for a proposal that changed the model only inside
  * perturbed_box (src/likelihood_surf.F90:110-270): convert_to_layer and

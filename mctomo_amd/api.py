@@ -75,7 +75,7 @@ def lib():
                      "mct_read_sources", "mct_read_times", "mct_write_sources", "mct_write_times",
                      "mct_body_location_likelihood", "mct_body_location_likelihood_batch_dev",
                      "mct_serve_create_mix", "mct_serve_submit_to", "mct_serve_memory", "mct_surf_device_bytes",
-                     "mct_surf_set_tuning", "mct_surf_get_tuning"):
+                     "mct_surf_set_tuning", "mct_surf_get_tuning", "mct_surf_likelihood_batch_graph"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -555,6 +555,16 @@ class SurfLikelihood:
             ctypes.c_void_p(ncells_ptr), ctypes.c_void_p(noise_ptr), ctypes.c_void_p(out_ptr),
             ctypes.c_void_p(status_ptr))
         return _check(rc, "mct_surf_likelihood_batch_dev")
+
+    def likelihood_batch_graph(self, nchain, points_ptr, params_ptr, ncells_ptr, noise_ptr, out_ptr,
+                               status_ptr):
+        """likelihood_batch_dev replayed from a HIP graph captured on the first call
+        with these arguments."""
+        rc = lib().mct_surf_likelihood_batch_graph(
+            self._h, int(nchain), ctypes.c_void_p(points_ptr), ctypes.c_void_p(params_ptr),
+            ctypes.c_void_p(ncells_ptr), ctypes.c_void_p(noise_ptr), ctypes.c_void_p(out_ptr),
+            ctypes.c_void_p(status_ptr))
+        return _check(rc, "mct_surf_likelihood_batch_graph")
 
     def likelihood_box(self, points, params, box, snoise0=None, snoise1=None, arrays=False):
         """likelihood() of a proposal that changed the model only inside box =

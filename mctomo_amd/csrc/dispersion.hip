@@ -1105,7 +1105,10 @@ __global__ __launch_bounds__(64) void k_dispersion_sm(mct_dsp_args A, float *lay
 int mct_launch_dispersion(const mct_dsp_args &a, hipStream_t s)
 {
     if (a.np < 1 || a.np > DSP_MAXP || a.nz < 1 || a.nz > DSP_NMAX) return MCT_ERR_ARG;
-    MCT_CHECK(hipMemsetAsync(a.bad_model, 0, sizeof(int32_t) * a.nchain, s));
+    {
+        int rc0 = mct_fill_i32(a.bad_model, a.nchain, 0, s);
+        if (rc0) return rc0;
+    }
     if (a.lay && a.queue) {
         // persistent waves, columns from a queue (ordered by layer count when
         // the order's buffers are given)

@@ -132,6 +132,10 @@ struct mct_pick_args {
 };
 int mct_launch_picks(const mct_pick_args &a, hipStream_t s);
 
+// n int32 set to v by a kernel (a stream's memset as a kernel node, so that
+// launch sequences that clear flags can be captured in a HIP graph)
+int mct_fill_i32(int32_t *p, long n, int32_t v, hipStream_t s);
+
 struct mct_misfit_args {
     int nchain, nsrc, nrev, np;
     int reciprocal;                // picks are (rev-major) when traced from receivers

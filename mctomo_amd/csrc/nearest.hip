@@ -357,7 +357,8 @@ int mct_launch_nearest(const mct_nearest_args &a, hipStream_t s)
 {
     long ntl = (long)a.G.ntiles * MCT_TILE_N;
     dim3 grid((unsigned)((ntl + NEAR_BLOCK - 1) / NEAR_BLOCK), (unsigned)a.nchain);
-    MCT_CHECK(hipMemsetAsync(a.flag_count, 0, sizeof(int32_t) * a.nchain, s));
+    int rc = mct_fill_i32(a.flag_count, a.nchain, 0, s);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_nearest_brute, grid, dim3(NEAR_BLOCK), 0, s, a);
     MCT_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_nearest_replay, dim3(a.nchain), dim3(256), 0, s, a);

@@ -170,6 +170,20 @@ __global__ __launch_bounds__(MIS_BLOCK) void k_misfit(mct_misfit_args a)
     }
 }
 
+__global__ void k_fill_i32(int32_t *p, long n, int32_t v)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int mct_fill_i32(int32_t *p, long n, int32_t v, hipStream_t s)
+{
+    if (n <= 0) return MCT_OK;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    MCT_CHECK(hipGetLastError());
+    return MCT_OK;
+}
+
 int mct_launch_misfit(const mct_misfit_args &a, hipStream_t s)
 {
     hipLaunchKernelGGL(k_misfit, dim3(a.nchain), dim3(MIS_BLOCK), 0, s, a);

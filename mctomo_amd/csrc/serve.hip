@@ -149,7 +149,11 @@ static int lane_launch(const model &M, lane *L)
         hipMemcpyAsync(L->d_noise, L->h_noise, sizeof(double) * 2 * np * B, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(L->d_ncells, L->h_ncells, sizeof(int32_t) * B, hipMemcpyHostToDevice, s) != hipSuccess)
         return MCT_ERR_HIP;
-    int rc = mct_surf_likelihood_batch_dev(L->ctx, B, L->d_points, L->d_params, L->d_ncells, L->d_noise, L->d_out,
+    // a lane's buffers never move, so each batch size replays its own captured graph
+    int rc = mct_surf_likelihood_batch_graph(L->ctx, B, L->d_points, L->d_params, L->d_ncells, L->d_noise, L->d_out,
+                                             L->d_status);
+    if (rc == MCT_ERR_ARG)   // a batch in parts has no graph form
+        rc = mct_surf_likelihood_batch_dev(L->ctx, B, L->d_points, L->d_params, L->d_ncells, L->d_noise, L->d_out,
                                            L->d_status);
     if (rc) return rc;
     if (hipMemcpyAsync(L->h_out, L->d_out, sizeof(double) * 3 * B, hipMemcpyDeviceToHost, s) != hipSuccess ||

@@ -89,6 +89,15 @@ struct mct_surf_ctx {
     int fm2d_method;                 // 0 parallel sweeps, 1 sequential heap loop
     long dsp_waves;                  // queue mode: at most this many waves (0: one per 64 columns)
     long fm2d_run_chunk = 0;         // 2-D problems per launch (0: the workspace's capacity)
+    // the batched call captured in HIP graphs, one per set of arguments (a
+    // server lane replays one per batch size), oldest evicted beyond 64
+    bool capturing = false;
+    struct graph_entry {
+        const void *key[7];
+        int nchain;
+        hipGraphExec_t exec;
+    };
+    std::vector<graph_entry> graphs;
     int dsp_unsorted;                // queue mode: columns in grid order instead of by layer count
     int dsp_mode;                    // 0 by size, 1 lane per column, 2 wave per column,
                                      // 3 lane per column from a column queue
@@ -142,7 +151,8 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     const long wpc = (ncol + 63) / 64;          // waves of lane-per-column scratch per chain
     const bool timing = part == 0;
     int rc;
-    if (nparts == 1) MCT_CHECK(hipEventRecord(C->ev[0], s));
+    const bool events = !C->capturing;   // no timing events inside a captured graph
+    if (nparts == 1 && events) MCT_CHECK(hipEventRecord(C->ev[0], s));
     // (a) nearest site -> vs
     double *vs = C->vs.as<double>() + c0 * gsz;
     mct_nearest_args na;
@@ -157,7 +167,7 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     na.kd_ind = C->kd_ind.as<int32_t>() + (long)c0 * (NC + 1);
     na.kd_rdata = C->kd_rdata.as<double>() + (long)c0 * 3 * NC;
     if ((rc = mct_launch_nearest(na, s))) return rc;
-    if (timing) MCT_CHECK(hipEventRecord(C->ev[1], s));
+    if (timing && events) MCT_CHECK(hipEventRecord(C->ev[1], s));
     // (b) dispersion per column
     mct_dsp_args da;
     memset(&da, 0, sizeof da);
@@ -183,7 +193,7 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
         if (S.phase_group)   // like%gvel keeps the accepted group velocities too (:227-229)
             MCT_CHECK(hipMemcpyAsync(da.group, C->group_keep.as<double>() + (long)c0 * ncol * np,
                                      sizeof(double) * ncol * np * nchain, hipMemcpyDeviceToDevice, s));
-        MCT_CHECK(hipMemsetAsync(da.ierr, 0, sizeof(int32_t) * ncol * nchain, s));
+        if ((rc = mct_fill_i32(da.ierr, (long)ncol * nchain, 0, s))) return rc;
         da.box = d_box; da.box_off = d_box_off; da.box_total = box_total; da.ny = G.ny;
     }
     da.lay = (dmode == 1 || dmode == 3) ? C->lay.as<float>() + (long)c0 * wpc * 4 * G.nz * 64 : nullptr;
@@ -201,7 +211,7 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     hipLaunchKernelGGL(k_surf_status, dim3(nchain), dim3(256), 0, s, (const int32_t *)da.bad_model,
                        (const int32_t *)da.ierr, ncol, d_status);
     MCT_CHECK(hipGetLastError());
-    if (timing) MCT_CHECK(hipEventRecord(C->ev[2], s));
+    if (timing && events) MCT_CHECK(hipEventRecord(C->ev[2], s));
     if (ev_dsp) MCT_CHECK(hipEventRecord(ev_dsp, s));
     double *phase_time = C->phase_time.as<double>() + (long)c0 * npair * np;
     double *rlen = (S.phase_group || S.straight) ? C->rlen.as<double>() + (long)c0 * npair * np : nullptr;
@@ -242,19 +252,19 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     fa.heap = C->heap.as<int32_t>() + soff * (wsz + 8);
     fa.ttime = phase_time; fa.status = d_status;
     fa.rounds_out = C->rounds.as<int32_t>() + 4 * soff;
-    fa.ev_sweep = timing ? C->ev_sweep : nullptr;
+    fa.ev_sweep = timing && events ? C->ev_sweep : nullptr;
     // group velocities: rays, group delays (CalGroupTime) and ray lengths
     // (likelihood_surf.F90:245,311-351)
     fa.rays = S.phase_group; fa.gvel = da.group; fa.rlen = rlen;
     const bool fast = C->fm2d_method == 0 && mct_fm2d_fast_ok(fa);
-    if (timing) C->sweep_rec = fast;
+    if (timing && events) C->sweep_rec = fast;
     if (fast)
         rc = mct_launch_fm2d_fast(fa, chunk, C->pflag.as<int32_t>() + soff, s);
     else
         rc = mct_launch_fm2d(fa, chunk, s);
     if (rc) return rc;
     }
-    if (timing) MCT_CHECK(hipEventRecord(C->ev[3], s));
+    if (timing && events) MCT_CHECK(hipEventRecord(C->ev[3], s));
     // (d) misfit: srdist = phaseTime, or the ray length for group times
     // (src/likelihood_surf.F90:324-331)
     mct_misfit_args ma;
@@ -269,7 +279,7 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     ma.sigma = want_arrays ? C->sigma.as<double>() : nullptr;
     ma.dist = rlen;
     if ((rc = mct_launch_misfit(ma, s))) return rc;
-    if (nparts == 1) MCT_CHECK(hipEventRecord(C->ev[4], s));
+    if (nparts == 1 && events) MCT_CHECK(hipEventRecord(C->ev[4], s));
     return MCT_OK;
 }
 
@@ -526,6 +536,8 @@ int mct_surf_destroy(mct_surf_ctx *C)
     if (C->h_box) (void)hipHostFree(C->h_box);
     for (int i = 0; i < 5; i++) if (C->ev[i]) (void)hipEventDestroy(C->ev[i]);
     if (C->ev_sweep) (void)hipEventDestroy(C->ev_sweep);
+    for (auto &g : C->graphs) (void)hipGraphExecDestroy(g.exec);
+    C->graphs.clear();
     for (int p = 0; p < MCT_SURF_SPLIT_MAX; p++) {
         if (C->sub[p]) { (void)hipStreamSynchronize(C->sub[p]); (void)hipStreamDestroy(C->sub[p]); }
         for (int k = 0; k < 2; k++) if (C->sev[p][k]) (void)hipEventDestroy(C->sev[p][k]);
@@ -718,6 +730,51 @@ int mct_surf_accept_batch(mct_surf_ctx *C, int nchain, const int32_t *accepted)
         int rc = mct_surf_accept(C, c);
         if (rc) return rc;
     }
+    return MCT_OK;
+}
+
+int mct_surf_likelihood_batch_graph(mct_surf_ctx *C, int nchain, const double *points_dev,
+                                    const double *params_dev, const int32_t *ncells_dev,
+                                    const double *noise_dev, double *out_dev, int32_t *status_dev)
+{
+    if (!C || nchain < 1 || nchain > C->S.max_chains || !points_dev || !params_dev || !ncells_dev ||
+        !noise_dev || !out_dev || !status_dev)
+        return MCT_ERR_ARG;
+    if (surf_parts(C, nchain) > 1) return MCT_ERR_ARG;   // one stream only
+    const void *key[7] = {points_dev, params_dev, ncells_dev, noise_dev, out_dev, status_dev, (const void *)C->S.np};
+    hipGraphExec_t exec = nullptr;
+    for (auto &g : C->graphs) {
+        bool same = g.nchain == nchain;
+        for (int i = 0; i < 7 && same; i++) same = g.key[i] == key[i];
+        if (same) { exec = g.exec; break; }
+    }
+    if (!exec) {
+        if (C->graphs.size() >= 64) {
+            (void)hipGraphExecDestroy(C->graphs.front().exec);
+            C->graphs.erase(C->graphs.begin());
+        }
+        MCT_CHECK(hipStreamSynchronize(C->stream));
+        MCT_CHECK(hipStreamBeginCapture(C->stream, hipStreamCaptureModeThreadLocal));
+        C->capturing = true;
+        int rc = mct_fill_i32(status_dev, nchain, 0, C->stream);
+        if (!rc)
+            rc = surf_pipeline(C, 0, nchain, 0, 1, C->stream, points_dev, params_dev, ncells_dev, 0, noise_dev,
+                               out_dev, status_dev, false, nullptr);
+        C->capturing = false;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(C->stream, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess || !g) return MCT_ERR_HIP;
+        e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return MCT_ERR_HIP;
+        mct_surf_ctx::graph_entry ge;
+        for (int i = 0; i < 7; i++) ge.key[i] = key[i];
+        ge.nchain = nchain;
+        ge.exec = exec;
+        C->graphs.push_back(ge);
+    }
+    MCT_CHECK(hipGraphLaunch(exec, C->stream));
     return MCT_OK;
 }
 

@@ -437,3 +437,54 @@ def test_autotune_keeps_results_and_caches_its_choice(tmp_path):
     assert rep2["cached"] and rep2["sweep"] == rep["sweep"]
     L.close()
     L2.close()
+
+
+def test_graph_replays_equal_the_eager_batch():
+    """The batched call captured in a HIP graph: every replay, with eager work
+    and synchronisations in between, equals the eager batch bit for bit
+    (statuses included, a rejected chain among them); new inputs in the same
+    buffers replay correctly; other buffers capture a new graph."""
+    base = W.surface_workload(n=33, nz=17, L=16.0, Z=8.0, ncells=40, np_=6, nsrc=8, nrev=8, seed=21)
+    nchain, maxc = 5, 40
+    pts = np.zeros((nchain, maxc, 3))
+    prm = np.zeros((nchain, maxc, 3))
+    ncl = np.zeros(nchain, np.int32)
+    for c in range(nchain):
+        w = W.surface_workload(n=33, nz=17, L=16.0, Z=8.0, ncells=30, np_=6, nsrc=8, nrev=8, seed=400 + c)
+        pts[c, :30], prm[c, :30], ncl[c] = w.points, w.params, 30
+    prm[2, int(np.argmax(pts[2, :30, 2])), 1] = prm[2, :30, 1].min() - 0.3   # rejected by check_model
+    L = ctx_for(base, max_cells=maxc, max_chains=nchain)
+    dP, dQ, dN = (mct.DeviceArray.from_numpy(a) for a in (pts, prm, ncl))
+    dS = mct.DeviceArray.from_numpy(np.zeros((nchain, 2, 6)))
+    dO, dT = mct.DeviceArray((nchain, 3), np.float64), mct.DeviceArray((nchain,), np.int32)
+    L.likelihood_batch_dev(nchain, dP.ptr, dQ.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
+    L.sync()
+    ref_o, ref_t = dO.numpy().copy(), dT.numpy().copy()
+    assert ref_t[2] == mct.REJECT_MODEL and (np.delete(ref_t, 2) == 0).all()
+    for rep in range(4):
+        dO.upload(np.full((nchain, 3), -7.0))
+        dT.upload(np.full(nchain, 99, np.int32))
+        L.likelihood_batch_graph(nchain, dP.ptr, dQ.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
+        L.sync()
+        say("graph replay", rep, dT.numpy())
+        np.testing.assert_array_equal(dO.numpy(), ref_o)
+        np.testing.assert_array_equal(dT.numpy(), ref_t)
+        # eager work between replays
+        rc, _ = L.likelihood(pts[0, :30], prm[0, :30])
+        assert rc == 0
+    # new inputs in the same buffers: the replay computes them
+    prm2 = prm.copy()
+    prm2[:, :, 1] += 0.05
+    prm2[:, :, 0] = 1.73 * prm2[:, :, 1]
+    dQ.upload(prm2)
+    L.likelihood_batch_dev(nchain, dP.ptr, dQ.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
+    L.sync()
+    ref2 = dO.numpy().copy()
+    L.likelihood_batch_graph(nchain, dP.ptr, dQ.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
+    L.sync()
+    np.testing.assert_array_equal(dO.numpy(), ref2)
+    # a smaller batch in the same buffers captures a new graph
+    L.likelihood_batch_graph(3, dP.ptr, dQ.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
+    L.sync()
+    np.testing.assert_array_equal(dO.numpy()[:3], ref2[:3])
+    L.close()
