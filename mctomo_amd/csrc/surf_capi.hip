@@ -93,7 +93,7 @@ struct mct_surf_ctx {
     // server lane replays one per batch size), oldest evicted beyond 64
     bool capturing = false;
     struct graph_entry {
-        const void *key[7];
+        const void *key[6];
         int nchain;
         hipGraphExec_t exec;
     };
@@ -741,11 +741,11 @@ int mct_surf_likelihood_batch_graph(mct_surf_ctx *C, int nchain, const double *p
         !noise_dev || !out_dev || !status_dev)
         return MCT_ERR_ARG;
     if (surf_parts(C, nchain) > 1) return MCT_ERR_ARG;   // one stream only
-    const void *key[7] = {points_dev, params_dev, ncells_dev, noise_dev, out_dev, status_dev, (const void *)C->S.np};
+    const void *key[6] = {points_dev, params_dev, ncells_dev, noise_dev, out_dev, status_dev};
     hipGraphExec_t exec = nullptr;
     for (auto &g : C->graphs) {
         bool same = g.nchain == nchain;
-        for (int i = 0; i < 7 && same; i++) same = g.key[i] == key[i];
+        for (int i = 0; i < 6 && same; i++) same = g.key[i] == key[i];
         if (same) { exec = g.exec; break; }
     }
     if (!exec) {
@@ -769,7 +769,7 @@ int mct_surf_likelihood_batch_graph(mct_surf_ctx *C, int nchain, const double *p
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return MCT_ERR_HIP;
         mct_surf_ctx::graph_entry ge;
-        for (int i = 0; i < 7; i++) ge.key[i] = key[i];
+        for (int i = 0; i < 6; i++) ge.key[i] = key[i];
         ge.nchain = nchain;
         ge.exec = exec;
         C->graphs.push_back(ge);

@@ -263,3 +263,56 @@ def test_location_likelihood_dynamic_fields(datatype):
     plain = mct.BodyLikelihood(w.grid, w.rev, w.ttime, w.raystat, nsrc, datatype=datatype, order=0, max_cells=64)
     with pytest.raises(mct.MCTError):
         plain.location_likelihood(loc2, b0, b1)
+
+
+def test_location_likelihood_batch_equals_single_chain_calls():
+    """The batched location likelihood (dynamic fields kept per chain) matches
+    each chain's own location likelihood after its own likelihood call."""
+    say("body location likelihood batch")
+    nchain, nsrc, nrev = 3, 4, 6
+    ws = [body_workload(n=21, h=0.4, ncells=int(8 + 5 * c), nsrc=nsrc, nrev=nrev, seed=200 + c)
+          for c in range(nchain)]
+    base = ws[0]
+    like = mct.BodyLikelihood(base.grid, base.rev, base.ttime, base.raystat, nsrc, max_cells=32,
+                              max_chains=nchain, dynamic=1)
+    rng = np.random.default_rng(9)
+    L = (base.grid.nx - 1) * base.grid.dx
+    new_loc = []
+    for w in ws:
+        l2 = w.locations.copy()
+        l2[:, :3] = np.clip(l2[:, :3] + rng.uniform(-0.5, 0.5, (nsrc, 3)), 0.2, L - 0.2)
+        l2[:, 3] = rng.uniform(-0.2, 0.2, nsrc)
+        new_loc.append(l2)
+    b0, b1 = np.array([0.03]), np.array([0.04])
+    single = []
+    for w, l2 in zip(ws, new_loc):
+        rc, _ = like.likelihood(w.points, w.params, w.locations)
+        assert rc == 0
+        rc2, out = like.location_likelihood(l2, b0, b1)
+        assert rc2 == 0
+        single.append(out)
+    P = np.zeros((nchain, 32, 3))
+    Q = np.zeros((nchain, 32, 3))
+    nc = np.zeros(nchain, np.int32)
+    loc = np.zeros((nchain, nsrc, 4))
+    for c, w in enumerate(ws):
+        P[c, :w.points.shape[0]] = w.points
+        Q[c, :w.points.shape[0]] = w.params
+        nc[c] = w.points.shape[0]
+        loc[c] = w.locations
+    noise = np.zeros((nchain, 2, 1))
+    noise[:, 0, 0], noise[:, 1, 0] = b0[0], b1[0]
+    dP, dQ, dn, dl = (mct.DeviceArray.from_numpy(a) for a in (P, Q, nc, loc))
+    dl2 = mct.DeviceArray.from_numpy(np.array(new_loc))
+    dnz = mct.DeviceArray.from_numpy(noise)
+    dout = mct.DeviceArray((nchain, 3), np.float64)
+    dst = mct.DeviceArray((nchain,), np.int32)
+    like.likelihood_batch_dev(nchain, dP.ptr, dQ.ptr, dn.ptr, dl.ptr, dnz.ptr, dout.ptr, dst.ptr)
+    like.sync()
+    assert (dst.numpy() == 0).all()
+    like.location_likelihood_batch_dev(nchain, dl2.ptr, dnz.ptr, dout.ptr, dst.ptr)
+    like.sync()
+    assert (dst.numpy() == 0).all()
+    # the batch's eikonal rounds run until every chain's field has converged, so
+    # a field may take more rounds than alone: equal within the solver's tolerance
+    np.testing.assert_allclose(dout.numpy(), np.array(single), rtol=1e-6)
