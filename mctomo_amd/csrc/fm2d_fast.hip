@@ -555,10 +555,25 @@ struct ff_work {
     int *cs, *cd;     // (nnx+nnz-1) flags: sum diagonals (x-1)+(z-1), difference diagonals (nnx-x)+(z-1)
 };
 
+// The refined grid's stop (fm2d_ttime.f90:143-163) freezes only nodes below the
+// smallest time on an inner edge.  For first-order updates, which only lower a
+// node's time, any inner-edge node's current time bounds that stop from above;
+// a node whose update lands above the bound is left alone (its time cannot
+// reach below the stop, and nodes below the stop depend only on smaller ones),
+// so the sweep stops spreading into the refined grid's far corners.
+struct ff_edges {
+    unsigned long long *bound;       // LDS: the bound's bits (positive doubles order as integers)
+    bool left, right, top, bottom;   // which edges are inner edges
+};
+
 template <int FOM, bool FAST, class FX, class VL>
 __device__ __forceinline__ int ff_sweep(const fview &V, const FX &fixed, const VL &vel, int *sh_flag,
-                        const ff_work *W, int *stat = nullptr)
+                        const ff_work *W, int *stat = nullptr, const ff_edges *E = nullptr)
 {
+    const bool prune = FOM == 0 && E != nullptr;
+    auto on_edge = [&](int iz, int ix) -> bool {
+        return (ix == 1 && E->left) || (ix == V.nnx && E->right) || (iz == 1 && E->top) || (iz == V.nnz && E->bottom);
+    };
     const ff_k0 K0 = ff_k0_make(V.dnx, V.dnz);
     int n_steps = 0, n_evals = 0;
     const int nt = blockDim.x, tid = threadIdx.x;
@@ -621,6 +636,8 @@ __device__ __forceinline__ int ff_sweep(const fview &V, const FX &fixed, const V
                     double slown = 1.0 / vel(z, x);
                     double c = (FAST && FOM == 0) ? ff_cands0f(S, K0, slown) : ff_cands<FOM>(S, V.dnx, V.dnz, slown);
                     if (c < FF_THR0 && c != t) {
+                        if (prune && c > __longlong_as_double((long long)*E->bound)) continue;
+                        if (prune && on_edge(z, x)) atomicMin(E->bound, (unsigned long long)__double_as_longlong(c));
                         if (fabs(c - t) > 1e-14 * c) changed = 1;
                         V.t(z, x) = c;
                         if (W) {
@@ -1164,7 +1181,14 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
             __syncthreads();
             auto rfixed = [&](int iz, int ix) -> bool { return R.n(iz, ix) == 0; };
             RW.cs = DC; RW.cd = DC + (R.nnx + R.nnz - 1);
-            int rounds = A.fom == 0 ? ff_sweep<0, true>(R, rfixed, rvel, &sh_flag, &RW)
+            // inner edges as the stop below tests them (fm2d_ttime.f90:143-163)
+            __shared__ unsigned long long sh_bound;
+            if (lane == 0) sh_bound = (unsigned long long)__double_as_longlong(FF_BIG);
+            __syncthreads();
+            ff_edges RE;
+            RE.bound = &sh_bound;
+            RE.left = g.vnl != 1; RE.right = g.vnr != R.nnx; RE.top = g.vnt != 1; RE.bottom = g.vnb != R.nnz;
+            int rounds = A.fom == 0 ? ff_sweep<0, true>(R, rfixed, rvel, &sh_flag, &RW, nullptr, &RE)
                                     : ff_sweep<1, true>(R, rfixed, rvel, &sh_flag, &RW);
             if (rounds < 0) {
                 if (lane == 0) pflag[slot] = 1;
