@@ -562,11 +562,11 @@ def main():
     ap.add_argument("--fm2d-method", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-evals", type=int, default=10)
-    ap.add_argument("--served", type=int, default=48, help="requests for the served measurement (0: skip)")
+    ap.add_argument("--served", type=int, default=192, help="requests for the served measurement (0: skip)")
     ap.add_argument("--box-steps", type=int, default=3,
                     help="timed perturbed-box batches (secondary unit; 0: skip)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
-    ap.add_argument("--served-mix", type=int, default=48,
+    ap.add_argument("--served-mix", type=int, default=96,
                     help="requests of the three-grid served mix (0: skip)")
     ap.add_argument("--group-steps", type=int, default=2,
                     help="timed group-velocity batches (secondary figure; 0: skip)")
