@@ -340,6 +340,7 @@ def run_surface(args):
     # may run in parts; the stage times follow the first part.
     nparts, n0 = like.split_parts(B)
     ev0, lev0 = like.dispersion_work(n0)
+    sc0, slev0 = like.dispersion_solved(n0)
     G = w.grid.nx * w.grid.ny * w.grid.nz
     C = w.grid.nx * w.grid.ny
     npd = w.freqs.shape[0]
@@ -358,16 +359,18 @@ def run_surface(args):
         kt = traffic.get(kernel)
         return int(kt["bytes_per_chain"] * chains) if kt else None
 
-    flops = 170.0 * lev0
+    flops = 170.0 * slev0
     a_disp = flops / (st_b[1] * 1e-3) / 1e12
     r_disp = {"bound": "mfma", "achieved": round(a_disp, 4), "peak": FP64_PEAK_TFLOPS,
               "unit": "TFLOP/s", "frac": round(a_disp / FP64_PEAK_TFLOPS, 6),
               "traffic": traffic_of("k_dispersion_sm", n0),
               "kernel": "k_dispersion_sm", "launch_chains": int(n0), "launch_ms": round(st_b[1], 3),
               "note": "FP64 vector arithmetic, no MFMA: the peak quoted is MI355X's FP64 figure "
-                      "(78.6 TFLOP/s, vector and matrix alike); flops = 170 x sum over columns of "
-                      "dltar evaluations x (layers-1) (the sequential root walk's count)",
-              "dltar_evals": int(ev0), "layer_evals": int(lev0),
+                      "(78.6 TFLOP/s, vector and matrix alike); flops = 170 x sum over the columns "
+                      "solved of dltar evaluations x (layers-1) (the sequential root walk's count); "
+                      "columns whose Vs profile repeats an earlier column of the chain are copied",
+              "columns": int(n0 * C), "columns_solved": int(sc0),
+              "dltar_evals": int(ev0), "layer_evals": int(lev0), "layer_evals_solved": int(slev0),
               "alg_bytes": int(n0 * (4 * G + 8 * C * npd))}
     solves = n0 * npd * nsrc
     alg_sweep = solves * (16.0 * C + 8.0 * nrev)

@@ -387,9 +387,13 @@ int mct_surf_set_dispersion_mode(mct_surf_ctx *ctx, int mode);
  * dispersion kernel (0: one per 64 columns), MCT_TUNE_SWEEP_KIND the 2-D
  * sweep (0 default, 1 strip-pipelined, 2 anti-diagonal), MCT_TUNE_SWEEP_THREADS
  * threads per problem of the anti-diagonal sweep (0 default, 64, 128, 256),
- * MCT_TUNE_FM2D_CHUNK 2-D problems per launch (0: all the workspace holds).
- * MCT_ERR_ARG for an unknown key or value.  get returns the value in use. */
-enum { MCT_TUNE_DSP_WAVES = 1, MCT_TUNE_SWEEP_KIND = 2, MCT_TUNE_SWEEP_THREADS = 3, MCT_TUNE_FM2D_CHUNK = 4 };
+ * MCT_TUNE_FM2D_CHUNK 2-D problems per launch (0: all the workspace holds),
+ * MCT_TUNE_DSP_DEDUP 1 (default) to solve each distinct Vs column profile of
+ * a chain once and copy its curves to the identical columns, 0 to solve
+ * every column.  MCT_ERR_ARG for an unknown key or value.  get returns the
+ * value in use. */
+enum { MCT_TUNE_DSP_WAVES = 1, MCT_TUNE_SWEEP_KIND = 2, MCT_TUNE_SWEEP_THREADS = 3, MCT_TUNE_FM2D_CHUNK = 4,
+       MCT_TUNE_DSP_DEDUP = 5 };
 int mct_surf_set_tuning(mct_surf_ctx *ctx, int key, int64_t value);
 int mct_surf_get_tuning(mct_surf_ctx *ctx, int key, int64_t *value);
 
@@ -398,6 +402,11 @@ int mct_surf_get_tuning(mct_surf_ctx *ctx, int key, int64_t *value);
  * (dltar calls), layer_evals = the same weighted by (layers - 1), the
  * per-evaluation layer recursion count. */
 int mct_surf_dispersion_work(mct_surf_ctx *ctx, int nchain, int64_t *evals, int64_t *layer_evals);
+
+/* The part of that work the last call actually solved: columns solved (the
+ * others copied an identical column of the same chain) and their evaluations
+ * weighted by (layers - 1).  No counterpart in the reference: a measurement. */
+int mct_surf_dispersion_solved(mct_surf_ctx *ctx, int nchain, int64_t *columns, int64_t *layer_evals);
 
 /* Work of the parallel 2-D path in the last launch chunk: {slots swept, mean
  * sweep rounds (four directions each), mean diagonal steps executed, mean

@@ -65,7 +65,7 @@ def lib():
                      "mct_copy_d2h", "mct_copy_d2d", "mct_surfdisp96_batch", "mct_modrays", "mct_surf_create",
                      "mct_surf_destroy", "mct_surf_likelihood", "mct_surf_likelihood_batch_dev",
                      "mct_surf_sync", "mct_surf_last_timings", "mct_surf_dev_arrays",
-                     "mct_surf_set_method", "mct_surf_dispersion_work", "mct_surf_fm2d_stats",
+                     "mct_surf_set_method", "mct_surf_dispersion_work", "mct_surf_dispersion_solved", "mct_surf_fm2d_stats",
                      "mct_surf_set_dispersion_mode", "mct_surf_split_parts", "mct_surf_last_timings2d", "mct_surf_stream", "mct_serve_create",
                      "mct_serve_submit", "mct_serve_wait", "mct_serve_stats", "mct_serve_destroy",
                      "mct_body_noise_likelihood", "mct_body_noise_likelihood_batch_dev",
@@ -657,6 +657,14 @@ class SurfLikelihood:
                "mct_surf_dispersion_work")
         return a.value, b.value
 
+    def dispersion_solved(self, nchain=1):
+        """(columns solved, their evaluations x (layers-1)) of the last call's first
+        nchain chains; columns identical to an earlier one of the chain are copied."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().mct_surf_dispersion_solved(self._h, int(nchain), ctypes.byref(a), ctypes.byref(b)),
+               "mct_surf_dispersion_solved")
+        return a.value, b.value
+
     def timings2d(self):
         """(near-source ms, sweep ms) of the last synchronised call's 2-D stage."""
         a, b = ctypes.c_double(), ctypes.c_double()
@@ -675,11 +683,12 @@ class SurfLikelihood:
         3 lane per column with columns taken from a queue."""
         _check(lib().mct_surf_set_dispersion_mode(self._h, int(mode)), "mct_surf_set_dispersion_mode")
 
-    TUNE_DSP_WAVES, TUNE_SWEEP_KIND, TUNE_SWEEP_THREADS, TUNE_FM2D_CHUNK = 1, 2, 3, 4
+    TUNE_DSP_WAVES, TUNE_SWEEP_KIND, TUNE_SWEEP_THREADS, TUNE_FM2D_CHUNK, TUNE_DSP_DEDUP = 1, 2, 3, 4, 5
 
     def set_tuning(self, key, value):
         """Launch shape (results do not depend on it): TUNE_DSP_WAVES, TUNE_SWEEP_KIND
-        (0 default, 1 strip, 2 anti-diagonal), TUNE_SWEEP_THREADS, TUNE_FM2D_CHUNK."""
+        (0 default, 1 strip, 2 anti-diagonal), TUNE_SWEEP_THREADS, TUNE_FM2D_CHUNK,
+        TUNE_DSP_DEDUP (1: identical column profiles solved once, 0: every column)."""
         _check(lib().mct_surf_set_tuning(self._h, int(key), ctypes.c_int64(int(value))), "mct_surf_set_tuning")
 
     def get_tuning(self, key):

@@ -752,6 +752,73 @@ __device__ __forceinline__ int dsp_layer_bin(const mct_dsp_args &A, long g)
     return A.nz - 1 - nl;     // mmax = nl + 1 in [1, nz]: bin 0 holds the most layers
 }
 
+// ---- columns with identical profiles ------------------------------------------
+// A column's dispersion curves depend on nothing but its Vs profile (check_model,
+// convert_to_layer and surfdisp96 read only vs(:), dz and the periods), so
+// columns of one chain whose profiles are equal bit for bit get the same
+// phase and group velocities, ierr and walk counts.  Each chain keeps an
+// open-addressing table of profile hashes; the smallest queue position holding
+// a hash is its representative, a column whose profile equals its
+// representative's copies that column's results, and a hash collision (a
+// profile that differs) is solved on its own.
+__device__ __forceinline__ unsigned long long dsp_profile_hash(const double *v, int nz)
+{
+    unsigned long long h = 0x9e3779b97f4a7c15ull;
+    for (int k = 0; k < nz; k++) {
+        h ^= (unsigned long long)__double_as_longlong(v[k]);
+        h *= 0xff51afd7ed558ccdull;
+        h ^= h >> 31;
+    }
+    return h | 1ull;                         // 0 marks an empty slot
+}
+
+__device__ __forceinline__ bool dsp_same_profile(const double *u, const double *v, int nz)
+{
+    for (int k = 0; k < nz; k++)
+        if (__double_as_longlong(u[k]) != __double_as_longlong(v[k])) return false;
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_dsp_hash_insert(mct_dsp_args A, long total)
+{
+    const long base = (long)blockIdx.x * 256 * DSP_SORT_IPT;
+    const unsigned mask = (1u << A.hlog) - 1u;
+#pragma unroll
+    for (int i = 0; i < DSP_SORT_IPT; i++) {
+        const long idx = base + i * 256 + threadIdx.x;
+        if (idx >= total) continue;
+        const long g = dsp_job_column(A, idx);
+        const long t0 = (g / A.ncol) << A.hlog;
+        const unsigned long long h = dsp_profile_hash(A.vs + g * A.nz, A.nz);
+        unsigned s = (unsigned)(h >> 17) & mask;
+        // at most ncol keys per chain in >= 2 * ncol slots: an empty slot is always found
+        for (;;) {
+            const unsigned long long prev = atomicCAS(&A.hkey[t0 + s], 0ull, h);
+            if (prev == 0ull || prev == h) break;
+            s = (s + 1u) & mask;
+        }
+        atomicMin(&A.hrep[t0 + s], (int32_t)idx);
+        A.dup[idx] = (int32_t)s;
+    }
+}
+
+// results of the solved representative copied to each duplicate column
+__global__ __launch_bounds__(256) void k_dsp_dup_copy(mct_dsp_args A, long total)
+{
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= total) return;
+    const int d = A.dup[idx];
+    if (d < 0) return;
+    const long g = dsp_job_column(A, idx), r = dsp_job_column(A, d);
+    for (int p = 0; p < A.np; p++) {
+        A.phase[g * A.np + p] = A.phase[r * A.np + p];
+        A.group[g * A.np + p] = A.group[r * A.np + p];
+    }
+    A.ierr[g] = A.ierr[r];
+    if (A.ndltar) A.ndltar[g] = A.ndltar[r];
+    if (A.nlayer) A.nlayer[g] = A.nlayer[r];
+}
+
 __global__ __launch_bounds__(256) void k_dsp_sort_count(mct_dsp_args A, long total)
 {
     __shared__ int h[128];
@@ -763,9 +830,22 @@ __global__ __launch_bounds__(256) void k_dsp_sort_count(mct_dsp_args A, long tot
     for (int i = 0; i < DSP_SORT_IPT; i++) {
         const long idx = base + i * 256 + threadIdx.x;
         if (idx < total) {
-            const int b = dsp_layer_bin(A, dsp_job_column(A, idx));
+            const long g = dsp_job_column(A, idx);
+            int b = -1;
+            if (A.dup) {
+                // the slot k_dsp_hash_insert left here becomes the copy source or -1
+                const long t0 = (g / A.ncol) << A.hlog;
+                const int rep = A.hrep[t0 + A.dup[idx]];
+                int d = -1;
+                if (rep != idx && dsp_same_profile(A.vs + g * A.nz, A.vs + dsp_job_column(A, rep) * A.nz, A.nz))
+                    d = rep;
+                A.dup[idx] = d;
+                if (d < 0) b = dsp_layer_bin(A, g);
+            } else {
+                b = dsp_layer_bin(A, g);
+            }
             A.keys[idx] = b;
-            atomicAdd(&h[b], 1);
+            if (b >= 0) atomicAdd(&h[b], 1);
         }
     }
     __syncthreads();
@@ -774,12 +854,13 @@ __global__ __launch_bounds__(256) void k_dsp_sort_count(mct_dsp_args A, long tot
 }
 
 // exclusive prefix sum of the bin counts: the bins' first positions
-__global__ void k_dsp_sort_scan(int32_t *hist, int nb, unsigned long long *queue)
+__global__ void k_dsp_sort_scan(int32_t *hist, int nb, unsigned long long *queue, int32_t *njobs)
 {
     if (threadIdx.x == 0) {
         int acc = 0;
         for (int b = 0; b < nb; b++) { const int c = hist[b]; hist[b] = acc; acc += c; }
         queue[0] = 0ull;
+        if (njobs) njobs[0] = acc;
     }
 }
 
@@ -818,6 +899,7 @@ __global__ void k_dsp_queue_reset(unsigned long long *q)
 __global__ __launch_bounds__(64) void k_dispersion_sm(mct_dsp_args A, float *lay, int nlmax, long total)
 {
     const int lane = threadIdx.x;
+    const long ntot = A.njobs ? (long)A.njobs[0] : total;   // duplicates are not queued
     const long plane = (long)nlmax * 64;
     float *base = lay + (long)blockIdx.x * 4 * plane + lane;
     float *sd = base, *sa = base + plane, *sb = base + 2 * plane, *sr = base + 3 * plane;
@@ -862,7 +944,7 @@ __global__ __launch_bounds__(64) void k_dispersion_sm(mct_dsp_args A, float *lay
             switch (pc) {
             case SM_NEXT: {
                 job = (long)atomicAdd((unsigned long long *)A.queue, 1ull);
-                if (job >= total) { pc = SM_DONE; break; }
+                if (job >= ntot) { pc = SM_DONE; break; }
                 if (A.jobs) job = A.jobs[job];
                 c = (int)(job / A.ncol);
                 col = job - (long)c * A.ncol;
@@ -1118,18 +1200,33 @@ int mct_launch_dispersion(const mct_dsp_args &a, hipStream_t s)
         if (total < 1) return MCT_OK;
         if (b.jobs && b.keys && b.hist && b.nz <= 128 && (long)a.ncol * a.nchain < 0x7fffffffL) {
             const unsigned nblk = (unsigned)((total + 256 * DSP_SORT_IPT - 1) / (256 * DSP_SORT_IPT));
+            const bool dedup = b.hkey && b.hrep && b.dup && b.njobs && b.hlog > 0 && b.hlog < 31 &&
+                               (2L * b.ncol) <= (1L << b.hlog);
+            if (dedup) {
+                const long slots = (long)b.nchain << b.hlog;
+                int rc = mct_fill_i32((int32_t *)b.hkey, 2 * slots, 0, s);
+                if (!rc) rc = mct_fill_i32(b.hrep, slots, 0x7fffffff, s);
+                if (rc) return rc;
+                hipLaunchKernelGGL(k_dsp_hash_insert, dim3(nblk), dim3(256), 0, s, b, total);
+            } else {
+                b.dup = nullptr; b.njobs = nullptr;
+            }
             hipLaunchKernelGGL(k_dsp_sort_reset, dim3(1), dim3(128), 0, s, b.hist, b.nz);
             hipLaunchKernelGGL(k_dsp_sort_count, dim3(nblk), dim3(256), 0, s, b, total);
-            hipLaunchKernelGGL(k_dsp_sort_scan, dim3(1), dim3(64), 0, s, b.hist, b.nz, b.queue);
+            hipLaunchKernelGGL(k_dsp_sort_scan, dim3(1), dim3(64), 0, s, b.hist, b.nz, b.queue, b.njobs);
             hipLaunchKernelGGL(k_dsp_sort_place, dim3(nblk), dim3(256), 0, s, b, total);
             MCT_CHECK(hipGetLastError());
         } else {
-            b.jobs = nullptr;
+            b.jobs = nullptr; b.dup = nullptr; b.njobs = nullptr;
             hipLaunchKernelGGL(k_dsp_queue_reset, dim3(1), dim3(64), 0, s, b.queue);
         }
         long nw = (total + 63) / 64;
         if (nw > b.lay_waves) nw = b.lay_waves;
         hipLaunchKernelGGL(k_dispersion_sm, dim3((unsigned)nw), dim3(64), 0, s, b, b.lay, b.nz, total);
+        if (b.dup) {
+            MCT_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(k_dsp_dup_copy, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, b, total);
+        }
     } else if (a.lay) {
         const unsigned nb = (unsigned)((a.ncol + 63) / 64);
         hipLaunchKernelGGL(k_dispersion_lane, dim3(nb, a.nchain), dim3(64), 0, s, a, a.lay, a.nz);

@@ -239,6 +239,13 @@ struct mct_dsp_args {
     long lay_waves;               // waves the scratch holds (queue mode)
     int32_t *jobs, *keys;         // queue mode: (nchain*ncol) columns ordered by layer count, or NULL
     int32_t *hist;                // (nz + 1) bins for that order
+    // sorted queue mode, optional: columns whose Vs profiles are bit-identical
+    // to an earlier column of the same chain are not solved but copied from it
+    unsigned long long *hkey;     // (nchain, 1 << hlog) profile-hash table keys, or NULL
+    int32_t *hrep;                // (nchain, 1 << hlog) smallest queue position per key
+    int32_t *dup;                 // per queue position: the position it copies, or -1
+    int32_t *njobs;               // (1) queue positions solved
+    int hlog;                     // table slots per chain = 1 << hlog >= 2 * ncol
     // queue mode: only the columns of a box per chain (a perturbed-box call), or NULL
     const int32_t *box;           // (nchain, 4): ix0, ix1, iy0, iy1, 1-based inclusive
     const int32_t *box_off;       // (nchain + 1): first compact index of each chain's box

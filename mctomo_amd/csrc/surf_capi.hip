@@ -99,6 +99,9 @@ struct mct_surf_ctx {
     };
     std::vector<graph_entry> graphs;
     int dsp_unsorted;                // queue mode: columns in grid order instead of by layer count
+    int dsp_dedup;                   // sorted queue mode: solve each distinct column profile once
+    int dsp_hlog;                    // profile-hash slots per chain = 1 << dsp_hlog
+    int dedup_last;                  // the last whole-grid call copied duplicate columns
     int dsp_mode;                    // 0 by size, 1 lane per column, 2 wave per column,
                                      // 3 lane per column from a column queue
     hipStream_t stream = nullptr;
@@ -119,6 +122,7 @@ struct mct_surf_ctx {
     // stage buffers
     sbuf vs, flag_count, flag_nodes, kd_nodes, kd_ind, kd_rdata;
     sbuf phase, group, ierr, bad_model, ndltar, nlayer, lay, dsp_queue, dsp_jobs, dsp_keys, dsp_hist, velv, veln, phase_time, srdist, sigma;
+    sbuf dsp_hkey, dsp_hrep, dsp_dup, dsp_njobs;
     sbuf ttn, ttr, vr, nsts, nstr, heap, pflag, rounds;
     // perturbed-box calls: accepted phase velocities per chain, the boxes of a call
     sbuf phase_keep, box;
@@ -205,9 +209,17 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
             da.jobs = C->dsp_jobs.as<int32_t>() + (long)c0 * ncol;
             da.keys = C->dsp_keys.as<int32_t>() + (long)c0 * ncol;
             da.hist = C->dsp_hist.as<int32_t>() + 128 * part;
+            if (C->dsp_dedup) {
+                da.hlog = C->dsp_hlog;
+                da.hkey = C->dsp_hkey.as<unsigned long long>() + ((long)c0 << C->dsp_hlog);
+                da.hrep = C->dsp_hrep.as<int32_t>() + ((long)c0 << C->dsp_hlog);
+                da.dup = C->dsp_dup.as<int32_t>() + (long)c0 * ncol;
+                da.njobs = C->dsp_njobs.as<int32_t>() + part;
+            }
         }
     }
     if ((rc = mct_launch_dispersion(da, s))) return rc;
+    if (timing) C->dedup_last = !d_box && da.dup != nullptr;
     hipLaunchKernelGGL(k_surf_status, dim3(nchain), dim3(256), 0, s, (const int32_t *)da.bad_model,
                        (const int32_t *)da.ierr, ncol, d_status);
     MCT_CHECK(hipGetLastError());
@@ -385,6 +397,8 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
         C->dsp_waves = w ? atol(w) : 0;
         const char *u = getenv("MCT_DSP_UNSORTED");     // columns in grid order (tuning)
         C->dsp_unsorted = u && atoi(u) != 0;
+        const char *dd = getenv("MCT_DSP_DEDUP");       // 0: solve every column (tuning)
+        C->dsp_dedup = !(dd && atoi(dd) == 0);
     }
     int rc = MCT_OK;
     if (hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking) != hipSuccess) { delete C; return MCT_ERR_HIP; }
@@ -438,6 +452,12 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     ALLOC(dsp_jobs, sizeof(int32_t) * ncol * M);
     ALLOC(dsp_keys, sizeof(int32_t) * ncol * M);
     ALLOC(dsp_hist, sizeof(int32_t) * 128 * MCT_SURF_SPLIT_MAX);
+    C->dsp_hlog = 1;
+    while ((1L << C->dsp_hlog) < 2L * ncol) C->dsp_hlog++;
+    ALLOC(dsp_hkey, sizeof(unsigned long long) * ((size_t)M << C->dsp_hlog));
+    ALLOC(dsp_hrep, sizeof(int32_t) * ((size_t)M << C->dsp_hlog));
+    ALLOC(dsp_dup, sizeof(int32_t) * ncol * M);
+    ALLOC(dsp_njobs, sizeof(int32_t) * MCT_SURF_SPLIT_MAX);
     ALLOC(velv, sizeof(double) * (grid->nx + 2) * (grid->ny + 2) * np * M);
     ALLOC(veln, sizeof(double) * nnx * nnz * np * M);
     ALLOC(phase_time, sizeof(double) * npair * np * M);
@@ -511,7 +531,8 @@ static std::vector<sbuf *> surf_bufs(mct_surf_ctx *C)
             &C->ierr, &C->bad_model, &C->ndltar, &C->nlayer, &C->lay, &C->dsp_queue, &C->dsp_jobs,
             &C->dsp_keys, &C->dsp_hist, &C->velv, &C->veln, &C->phase_time, &C->srdist, &C->sigma,
             &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap, &C->pflag, &C->rounds,
-            &C->phase_keep, &C->box, &C->rlen, &C->group_keep};
+            &C->phase_keep, &C->box, &C->rlen, &C->group_keep, &C->dsp_hkey, &C->dsp_hrep,
+            &C->dsp_dup, &C->dsp_njobs};
 }
 
 extern "C" {
@@ -909,6 +930,10 @@ int mct_surf_set_tuning(mct_surf_ctx *C, int key, int64_t value)
         if (value < 0) return MCT_ERR_ARG;
         C->fm2d_run_chunk = (long)value;
         return MCT_OK;
+    case MCT_TUNE_DSP_DEDUP:
+        if (value != 0 && value != 1) return MCT_ERR_ARG;
+        C->dsp_dedup = (int)value;
+        return MCT_OK;
     default:
         return MCT_ERR_ARG;
     }
@@ -922,6 +947,7 @@ int mct_surf_get_tuning(mct_surf_ctx *C, int key, int64_t *value)
     case MCT_TUNE_SWEEP_KIND: *value = C->F.sweep_kind; return MCT_OK;
     case MCT_TUNE_SWEEP_THREADS: *value = C->F.sweep_threads; return MCT_OK;
     case MCT_TUNE_FM2D_CHUNK: *value = C->fm2d_run_chunk > 0 ? C->fm2d_run_chunk : C->fm2d_chunk; return MCT_OK;
+    case MCT_TUNE_DSP_DEDUP: *value = C->dsp_dedup; return MCT_OK;
     default: return MCT_ERR_ARG;
     }
 }
@@ -942,6 +968,32 @@ int mct_surf_dispersion_work(mct_surf_ctx *C, int nchain, int64_t *evals, int64_
     free(a); free(b);
     if (rc) return rc;
     if (evals) *evals = e;
+    if (layer_evals) *layer_evals = le;
+    return MCT_OK;
+}
+
+int mct_surf_dispersion_solved(mct_surf_ctx *C, int nchain, int64_t *columns, int64_t *layer_evals)
+{
+    if (!C || nchain < 1 || nchain > C->S.max_chains) return MCT_ERR_ARG;
+    const long n = (long)nchain * C->G.nx * C->G.ny;
+    int32_t *a = (int32_t *)malloc(sizeof(int32_t) * n), *b = (int32_t *)malloc(sizeof(int32_t) * n);
+    int32_t *d = (int32_t *)malloc(sizeof(int32_t) * n);
+    if (!a || !b || !d) { free(a); free(b); free(d); return MCT_ERR_NOMEM; }
+    int rc = MCT_OK;
+    if (hipStreamSynchronize(C->stream) != hipSuccess ||
+        hipMemcpy(a, C->ndltar.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(b, C->nlayer.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+        (C->dedup_last && hipMemcpy(d, C->dsp_dup.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost) != hipSuccess))
+        rc = MCT_ERR_HIP;
+    int64_t cols = 0, le = 0;
+    for (long i = 0; !rc && i < n; i++) {
+        if (C->dedup_last && d[i] >= 0) continue;       // copied from an identical column
+        cols++;
+        le += (int64_t)a[i] * (b[i] > 1 ? b[i] - 1 : 0);
+    }
+    free(a); free(b); free(d);
+    if (rc) return rc;
+    if (columns) *columns = cols;
     if (layer_evals) *layer_evals = le;
     return MCT_OK;
 }

@@ -488,3 +488,56 @@ def test_graph_replays_equal_the_eager_batch():
     L.sync()
     np.testing.assert_array_equal(dO.numpy()[:3], ref2[:3])
     L.close()
+
+
+@pytest.mark.parametrize("phase_group", [0, 1])
+def test_identical_columns_solved_once_match_every_column(phase_group):
+    """Sorted queue dispersion with identical column profiles solved once per
+    chain (the default) gives the every-column results bit for bit: phase and
+    group velocities, statuses, likelihoods and per-column period-equation
+    counts; with few cells most columns are copies, and the rejected chain
+    (a fast top) stays rejected."""
+    nchain, maxc, n, npd = 5, 12, 29, 8
+    rng = np.random.default_rng(41)
+    pts = np.zeros((nchain, maxc, 3))
+    prm = np.zeros((nchain, maxc, 3))
+    ncl = np.zeros(nchain, np.int32)
+    for c in range(nchain):
+        k = int(rng.integers(3, maxc + 1))
+        w = W.surface_workload(n=n, nz=15, L=14.0, Z=7.0, ncells=k, np_=npd, nsrc=5, nrev=6, seed=700 + c)
+        pts[c, :k], prm[c, :k], ncl[c] = w.points, w.params, k
+    k = ncl[2]
+    prm[2, int(np.argmax(pts[2, :k, 2])), 1] = prm[2, :k, 1].min() - 0.3
+    L = mct.SurfLikelihood(w.grid, w.freqs, w.src, w.rev, w.ttime, w.raystat, raylov=w.raylov,
+                           order=w.order, max_cells=maxc, max_chains=nchain, phase_group=phase_group)
+    L.set_dispersion_mode(3)
+    dP, dQ = mct.DeviceArray.from_numpy(pts), mct.DeviceArray.from_numpy(prm)
+    dN = mct.DeviceArray.from_numpy(ncl)
+    dS = mct.DeviceArray.from_numpy(np.zeros((nchain, 2, npd)))
+    res = {}
+    for dedup in (0, 1):
+        say("identical columns, phase_group", phase_group, "dedup", dedup)
+        L.set_tuning(L.TUNE_DSP_DEDUP, dedup)
+        assert L.get_tuning(L.TUNE_DSP_DEDUP) == dedup
+        dO = mct.DeviceArray((nchain, 3), np.float64)
+        dT = mct.DeviceArray((nchain,), np.int32)
+        L.likelihood_batch_dev(nchain, dP.ptr, dQ.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
+        L.sync()
+        ph = np.empty((nchain, n * n, npd))
+        assert mct.lib().mct_copy_d2h(ctypes.c_void_p(ph.ctypes.data), ctypes.c_void_p(L.dev_arrays()[1]),
+                                      ctypes.c_int64(ph.nbytes)) == 0
+        gv = [L.group_velocities(c) for c in range(nchain)] if phase_group else []
+        res[dedup] = (dO.numpy(), dT.numpy(), ph, gv, L.dispersion_work(nchain), L.dispersion_solved(nchain))
+    o0, t0, p0, g0, w0, s0 = res[0]
+    o1, t1, p1, g1, w1, s1 = res[1]
+    assert t0[2] == mct.REJECT_MODEL
+    np.testing.assert_array_equal(t0, t1)
+    np.testing.assert_array_equal(o0[t0 == 0], o1[t1 == 0])
+    np.testing.assert_array_equal(p0, p1)
+    for a, b in zip(g0, g1):
+        np.testing.assert_array_equal(a, b)
+    assert w0 == w1
+    assert s0[0] == nchain * n * n and s0[1] == w0[1]
+    say("  columns solved %d of %d" % (s1[0], s0[0]))
+    assert 0 < s1[0] < s0[0] // 2 and s1[1] < w0[1]
+    L.close()
