@@ -22,7 +22,7 @@
 #pragma clang fp contract(off)
 
 #define NEAR_BLOCK 256
-#define NEAR_CHUNK 2048        // sites staged in LDS per pass (48 KiB)
+#define NEAR_CHUNK 1024        // sites staged in LDS per pass (24 KiB + 8 KiB of lists)
 
 __device__ __forceinline__ void node_of_tile_lane(const mct_geom &G, long e, int &x, int &y, int &z)
 {
@@ -78,6 +78,7 @@ __device__ __forceinline__ bool pm_selects(const mct_nearest_args &a, int c, int
 __global__ __launch_bounds__(NEAR_BLOCK) void k_nearest_brute(mct_nearest_args a)
 {
     __shared__ double sx[NEAR_CHUNK], sy[NEAR_CHUNK], sz[NEAR_CHUNK];
+    __shared__ unsigned short cand[NEAR_BLOCK / 64][NEAR_CHUNK];
     const mct_geom &G = a.G;
     const int c = blockIdx.y;
     const int S = a.ncells ? a.ncells[c] : a.ncells_const;
@@ -101,6 +102,27 @@ __global__ __launch_bounds__(NEAR_BLOCK) void k_nearest_brute(mct_nearest_args a
     double best = DBL_MAX, second = DBL_MAX;
     int bi = 1;
     const double *P = a.points + (long)c * a.max_cells * 3;
+    // Site pruning per wave: the wave's nodes form one 4x4x4 tile, whose node
+    // coordinates lie in the box [lo, hi].  ub = min over sites of the squared
+    // distance to the box's farthest point bounds every node's best distance;
+    // a site whose squared distance to the box's nearest point exceeds
+    // ub (1 + 1e-12) is farther than every node's best by more than the
+    // near-tie tolerance, so it can be neither the nearest site nor a tie.
+    // Rounding is monotone in each operand, so the bounds computed in FP64 hold
+    // for the distances computed in FP64 (same order of operations).
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    const long tile = ((long)blockIdx.x * NEAR_BLOCK + wv * 64) >> 6;
+    const bool wlive = tile < G.ntiles && __ballot(live) != 0ull;      // wave-uniform
+    double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};
+    if (wlive) {
+        const int t = (int)tile;
+        const int tx = t % G.ntx, r = t / G.ntx, ty = r % G.nty, tz = r / G.nty;
+        const int x0 = tx * MCT_TILE, y0 = ty * MCT_TILE, z0 = tz * MCT_TILE;
+        lo[0] = G.xmin + (double)x0 * G.dx; hi[0] = G.xmin + (double)min(x0 + MCT_TILE - 1, G.nx - 1) * G.dx;
+        lo[1] = G.ymin + (double)y0 * G.dy; hi[1] = G.ymin + (double)min(y0 + MCT_TILE - 1, G.ny - 1) * G.dy;
+        lo[2] = G.zmin + (double)z0 * G.dz; hi[2] = G.zmin + (double)min(z0 + MCT_TILE - 1, G.nz - 1) * G.dz;
+    }
+    double ub = DBL_MAX;
     for (int base = 0; base < S; base += NEAR_CHUNK) {
         int n = min(NEAR_CHUNK, S - base);
         __syncthreads();
@@ -110,8 +132,42 @@ __global__ __launch_bounds__(NEAR_BLOCK) void k_nearest_brute(mct_nearest_args a
             sz[i] = P[3 * (base + i) + 2];
         }
         __syncthreads();
+        int ncand = 0;
+        if (wlive) {
+            double m = DBL_MAX;
+            for (int i = ln; i < n; i += 64) {
+                double f0 = fmax(fabs(sx[i] - lo[0]), fabs(sx[i] - hi[0]));
+                double f1 = fmax(fabs(sy[i] - lo[1]), fabs(sy[i] - hi[1]));
+                double f2 = fmax(fabs(sz[i] - lo[2]), fabs(sz[i] - hi[2]));
+                double u = f0 * f0;
+                u = u + f1 * f1;
+                u = u + f2 * f2;
+                m = fmin(m, u);
+            }
+            for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+            ub = fmin(ub, m);
+            const double lim = ub + ub * 1e-12;
+            for (int i0 = 0; i0 < n; i0 += 64) {
+                const int i = i0 + ln;
+                bool take = false;
+                if (i < n) {
+                    double g0 = fmax(fmax(lo[0] - sx[i], sx[i] - hi[0]), 0.0);
+                    double g1 = fmax(fmax(lo[1] - sy[i], sy[i] - hi[1]), 0.0);
+                    double g2 = fmax(fmax(lo[2] - sz[i], sz[i] - hi[2]), 0.0);
+                    double l = g0 * g0;
+                    l = l + g1 * g1;
+                    l = l + g2 * g2;
+                    take = l <= lim;
+                }
+                const unsigned long long bal = __ballot(take);
+                if (take) cand[wv][ncand + __popcll(bal & ((1ull << ln) - 1ull))] = (unsigned short)i;
+                ncand += __popcll(bal);
+            }
+        }
+        __syncthreads();            // the lists are read by every lane of the wave
         if (live) {
-            for (int i = 0; i < n; i++) {
+            for (int j = 0; j < ncand; j++) {
+                const int i = cand[wv][j];
                 double d0 = sx[i] - qx;
                 double sd = d0 * d0;
                 double d1 = sy[i] - qy;

@@ -29,7 +29,7 @@ def _g(grid):
                 zmin=grid.zmin, dx=grid.dx, dy=grid.dy, dz=grid.dz)
 
 
-@pytest.mark.parametrize("seed,n,ncell", [(1, 21, 7), (2, 41, 100), (3, 33, 800), (4, 50, 1)])
+@pytest.mark.parametrize("seed,n,ncell", [(1, 21, 7), (2, 41, 100), (3, 33, 800), (4, 50, 1), (7, 23, 2500)])
 def test_nearest_random_bit_exact(seed, n, ncell):
     say("nearest random", seed, n, ncell)
     rng = np.random.default_rng(seed)
