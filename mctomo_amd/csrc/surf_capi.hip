@@ -896,9 +896,25 @@ int mct_surf_fm2d_stats(mct_surf_ctx *C, double *stats4)
     return MCT_OK;
 }
 
+}   // extern "C"
+
+// Captured batch graphs hold the launch shape of their capture: a setter that
+// changes the shape drops them (after the context's queued work has finished).
+static int surf_drop_graphs(mct_surf_ctx *C)
+{
+    if (C->graphs.empty()) return MCT_OK;
+    MCT_CHECK(hipStreamSynchronize(C->stream));
+    for (auto &g : C->graphs) (void)hipGraphExecDestroy(g.exec);
+    C->graphs.clear();
+    return MCT_OK;
+}
+
+extern "C" {
+
 int mct_surf_set_method(mct_surf_ctx *C, int fm2d_method)
 {
     if (!C || (fm2d_method != 0 && fm2d_method != 1)) return MCT_ERR_ARG;
+    if (surf_drop_graphs(C)) return MCT_ERR_HIP;
     C->fm2d_method = fm2d_method;
     return MCT_OK;
 }
@@ -906,6 +922,7 @@ int mct_surf_set_method(mct_surf_ctx *C, int fm2d_method)
 int mct_surf_set_dispersion_mode(mct_surf_ctx *C, int mode)
 {
     if (!C || mode < 0 || mode > 3) return MCT_ERR_ARG;
+    if (surf_drop_graphs(C)) return MCT_ERR_HIP;
     C->dsp_mode = mode;
     return MCT_OK;
 }
@@ -913,6 +930,7 @@ int mct_surf_set_dispersion_mode(mct_surf_ctx *C, int mode)
 int mct_surf_set_tuning(mct_surf_ctx *C, int key, int64_t value)
 {
     if (!C) return MCT_ERR_ARG;
+    if (surf_drop_graphs(C)) return MCT_ERR_HIP;
     switch (key) {
     case MCT_TUNE_DSP_WAVES:
         if (value < 0) return MCT_ERR_ARG;

@@ -487,6 +487,17 @@ def test_graph_replays_equal_the_eager_batch():
     L.likelihood_batch_graph(3, dP.ptr, dQ.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
     L.sync()
     np.testing.assert_array_equal(dO.numpy()[:3], ref2[:3])
+    # a launch-shape change drops the captured graphs: replays follow the new shape
+    L.set_method(1)
+    L.set_tuning(L.TUNE_DSP_DEDUP, 0)
+    L.likelihood_batch_dev(nchain, dP.ptr, dQ.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
+    L.sync()
+    ref3 = dO.numpy().copy()
+    for rep in range(2):
+        dO.upload(np.full((nchain, 3), -7.0))
+        L.likelihood_batch_graph(nchain, dP.ptr, dQ.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
+        L.sync()
+        np.testing.assert_array_equal(dO.numpy(), ref3)
     L.close()
 
 
