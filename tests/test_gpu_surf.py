@@ -200,11 +200,12 @@ def test_column_queue_dispersion_matches_lane_per_column(waves, unsorted, monkey
     L.close()
 
 
-@pytest.mark.parametrize("split", [1, 3])
-def test_batch_in_parts_matches_one_part(split, monkeypatch):
+@pytest.mark.parametrize("split,mode", [(1, 0), (3, 0), (3, 3)])
+def test_batch_in_parts_matches_one_part(split, mode, monkeypatch):
     """A batch run in parts on the context's streams (one part's 2-D solves
     overlapping the next part's dispersion) gives the one-part results bit for
-    bit, statuses included."""
+    bit, statuses included; with the column queue (mode 3) each part keeps its
+    own profile-hash tables, copy list and job count."""
     nchain, maxc = 7, 30
     rng = np.random.default_rng(29)
     pts = np.zeros((nchain, maxc, 3))
@@ -224,17 +225,26 @@ def test_batch_in_parts_matches_one_part(split, monkeypatch):
         monkeypatch.setenv("MCT_SURF_SPLIT", str(parts))
         L = ctx_for(w, max_cells=maxc, max_chains=nchain)
         assert L.split_parts(nchain)[0] == parts
-        say("surf batch in parts", parts)
+        if mode:
+            L.set_dispersion_mode(mode)
+        say("surf batch in parts", parts, "dispersion mode", mode)
         dO = mct.DeviceArray((nchain, 3), np.float64)
         dT = mct.DeviceArray((nchain,), np.int32)
         L.likelihood_batch_dev(nchain, dP.ptr, dQ.ptr, dN.ptr, dS.ptr, dO.ptr, dT.ptr)
         L.sync()
-        res[parts] = (dO.numpy(), dT.numpy())
+        ph = np.empty((nchain, 25 * 25, 6))
+        assert mct.lib().mct_copy_d2h(ctypes.c_void_p(ph.ctypes.data), ctypes.c_void_p(L.dev_arrays()[1]),
+                                      ctypes.c_int64(ph.nbytes)) == 0
+        res[parts] = (dO.numpy(), dT.numpy(), ph, L.dispersion_solved(nchain)[0])
         L.close()
-    (o1, t1), (o2, t2) = res[1], res[split]
+    (o1, t1, p1, s1), (o2, t2, p2, s2) = res[1], res[split]
     assert t1[4] == mct.REJECT_MODEL
     np.testing.assert_array_equal(t1, t2)
     np.testing.assert_array_equal(o1[t1 == 0], o2[t2 == 0])
+    np.testing.assert_array_equal(p1, p2)
+    assert s1 == s2
+    if mode == 3:
+        assert s1 < nchain * 25 * 25
 
 
 def test_noise_likelihood_reuses_the_last_phase_times():
