@@ -449,11 +449,14 @@ def test_autotune_keeps_results_and_caches_its_choice(tmp_path):
     L2.close()
 
 
-def test_graph_replays_equal_the_eager_batch():
+@pytest.mark.parametrize("mode", [0, 3])
+def test_graph_replays_equal_the_eager_batch(mode):
     """The batched call captured in a HIP graph: every replay, with eager work
     and synchronisations in between, equals the eager batch bit for bit
     (statuses included, a rejected chain among them); new inputs in the same
-    buffers replay correctly; other buffers capture a new graph."""
+    buffers replay correctly; other buffers capture a new graph.  Mode 3 (the
+    column queue) puts the profile-hash tables, cleared by fill kernels, in the
+    graph."""
     base = W.surface_workload(n=33, nz=17, L=16.0, Z=8.0, ncells=40, np_=6, nsrc=8, nrev=8, seed=21)
     nchain, maxc = 5, 40
     pts = np.zeros((nchain, maxc, 3))
@@ -464,6 +467,8 @@ def test_graph_replays_equal_the_eager_batch():
         pts[c, :30], prm[c, :30], ncl[c] = w.points, w.params, 30
     prm[2, int(np.argmax(pts[2, :30, 2])), 1] = prm[2, :30, 1].min() - 0.3   # rejected by check_model
     L = ctx_for(base, max_cells=maxc, max_chains=nchain)
+    if mode:
+        L.set_dispersion_mode(mode)
     dP, dQ, dN = (mct.DeviceArray.from_numpy(a) for a in (pts, prm, ncl))
     dS = mct.DeviceArray.from_numpy(np.zeros((nchain, 2, 6)))
     dO, dT = mct.DeviceArray((nchain, 3), np.float64), mct.DeviceArray((nchain,), np.int32)
