@@ -739,17 +739,21 @@ __device__ __forceinline__ long dsp_job_column(const mct_dsp_args &A, long idx)
     return (long)lo * A.ncol + (long)(ix - 1) * A.ny + (iy - 1);
 }
 
-__device__ __forceinline__ int dsp_layer_bin(const mct_dsp_args &A, long g)
+__device__ __forceinline__ int dsp_layer_bin_v(const double *vs, int nz)
 {
     const double EPS = (double)1.0e-10f;       // src/likelihood_surf.F90:37
-    const double *vs = A.vs + g * A.nz;
     int nl = 0;
     double last = vs[0];
-    for (int k = 1; k < A.nz; k++) {
+    for (int k = 1; k < nz; k++) {
         const double v = vs[k];
         if (fabs(v - last) > EPS) { nl++; last = v; }
     }
-    return A.nz - 1 - nl;     // mmax = nl + 1 in [1, nz]: bin 0 holds the most layers
+    return nz - 1 - nl;       // mmax = nl + 1 in [1, nz]: bin 0 holds the most layers
+}
+
+__device__ __forceinline__ int dsp_layer_bin(const mct_dsp_args &A, long g)
+{
+    return dsp_layer_bin_v(A.vs + g * A.nz, A.nz);
 }
 
 // ---- columns with identical profiles ------------------------------------------
@@ -779,27 +783,51 @@ __device__ __forceinline__ bool dsp_same_profile(const double *u, const double *
     return true;
 }
 
+// the profile of queue position idx (global column g) enters its chain's table;
+// its layer-count bin is kept in keys[idx] for the sort
+__device__ __forceinline__ void dsp_hash_one(const mct_dsp_args &A, long idx, long g, const double *v)
+{
+    const unsigned mask = (1u << A.hlog) - 1u;
+    const long t0 = (g / A.ncol) << A.hlog;
+    const unsigned long long h = dsp_profile_hash(v, A.nz);
+    unsigned s = (unsigned)(h >> 17) & mask;
+    // at most ncol keys per chain in >= 2 * ncol slots: an empty slot is always found
+    for (;;) {
+        const unsigned long long prev = atomicCAS(&A.hkey[t0 + s], 0ull, h);
+        if (prev == 0ull || prev == h) break;
+        s = (s + 1u) & mask;
+    }
+    atomicMin(&A.hrep[t0 + s], (int32_t)idx);
+    A.dup[idx] = (int32_t)s;
+    A.keys[idx] = dsp_layer_bin_v(v, A.nz);
+}
+
+// box calls: one lane per queue position reading its column in place
 __global__ __launch_bounds__(256) void k_dsp_hash_insert(mct_dsp_args A, long total)
 {
     const long base = (long)blockIdx.x * 256 * DSP_SORT_IPT;
-    const unsigned mask = (1u << A.hlog) - 1u;
 #pragma unroll
     for (int i = 0; i < DSP_SORT_IPT; i++) {
         const long idx = base + i * 256 + threadIdx.x;
         if (idx >= total) continue;
         const long g = dsp_job_column(A, idx);
-        const long t0 = (g / A.ncol) << A.hlog;
-        const unsigned long long h = dsp_profile_hash(A.vs + g * A.nz, A.nz);
-        unsigned s = (unsigned)(h >> 17) & mask;
-        // at most ncol keys per chain in >= 2 * ncol slots: an empty slot is always found
-        for (;;) {
-            const unsigned long long prev = atomicCAS(&A.hkey[t0 + s], 0ull, h);
-            if (prev == 0ull || prev == h) break;
-            s = (s + 1u) & mask;
-        }
-        atomicMin(&A.hrep[t0 + s], (int32_t)idx);
-        A.dup[idx] = (int32_t)s;
+        dsp_hash_one(A, idx, g, A.vs + g * A.nz);
     }
+}
+
+// whole-grid calls (queue position = global column): a wave's 64 columns are
+// contiguous, so they are staged through LDS with coalesced loads
+__global__ __launch_bounds__(64) void k_dsp_hash_insert_grid(mct_dsp_args A, long total)
+{
+    extern __shared__ double dsp_sv[];         // 64 * nz
+    const long idx0 = (long)blockIdx.x * 64;
+    const int n = (int)(total - idx0 < 64 ? total - idx0 : 64);
+    const double *src = A.vs + idx0 * A.nz;
+    for (int i = threadIdx.x; i < n * A.nz; i += 64) dsp_sv[i] = src[i];
+    __syncthreads();
+    if ((int)threadIdx.x >= n) return;
+    const long idx = idx0 + threadIdx.x;
+    dsp_hash_one(A, idx, idx, dsp_sv + threadIdx.x * A.nz);
 }
 
 // results of the solved representative copied to each duplicate column
@@ -840,7 +868,7 @@ __global__ __launch_bounds__(256) void k_dsp_sort_count(mct_dsp_args A, long tot
                 if (rep != idx && dsp_same_profile(A.vs + g * A.nz, A.vs + dsp_job_column(A, rep) * A.nz, A.nz))
                     d = rep;
                 A.dup[idx] = d;
-                if (d < 0) b = dsp_layer_bin(A, g);
+                if (d < 0) b = A.keys[idx];         // the bin k_dsp_hash_insert kept
             } else {
                 b = dsp_layer_bin(A, g);
             }
@@ -1207,7 +1235,11 @@ int mct_launch_dispersion(const mct_dsp_args &a, hipStream_t s)
                 int rc = mct_fill_i32((int32_t *)b.hkey, 2 * slots, 0, s);
                 if (!rc) rc = mct_fill_i32(b.hrep, slots, 0x7fffffff, s);
                 if (rc) return rc;
-                hipLaunchKernelGGL(k_dsp_hash_insert, dim3(nblk), dim3(256), 0, s, b, total);
+                if (b.box_off)
+                    hipLaunchKernelGGL(k_dsp_hash_insert, dim3(nblk), dim3(256), 0, s, b, total);
+                else
+                    hipLaunchKernelGGL(k_dsp_hash_insert_grid, dim3((unsigned)((total + 63) / 64)), dim3(64),
+                                       (size_t)64 * b.nz * sizeof(double), s, b, total);
             } else {
                 b.dup = nullptr; b.njobs = nullptr;
             }
