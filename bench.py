@@ -6,8 +6,10 @@ Default workload (--workload surf201): the surface-wave likelihood at the
 sites with Vs in [2.5,4.5] km/s (slower sites shallower so every model passes
 check_model), Rayleigh fundamental-mode phase velocity per column at 20
 periods log-spaced 1-10 s, then 2-D phase traveltimes per period from 32
-sources to 32 receivers (order 1, refined source grid 4 x +-8), Gaussian
-misfit.  One evaluation = one likelihood() call with the whole grid as the
+sources to 32 receivers (refined source grid 4 x +-8), Gaussian misfit.
+--order 0 (the default) is the first-order upwind update, --order 1 the
+mixed-order update the reference's example settings use; the JSON line
+carries an order-1 batched figure beside the order-0 `value`.  One evaluation = one likelihood() call with the whole grid as the
 perturbed box: nearest site (2.46 M nodes), 40,401 dispersion curves, 640
 2-D solves of 40,401 nodes, 20,480 picks.
 

@@ -181,11 +181,12 @@ module mctomo_hip
         ! data files: read_sources / read_times_3 (src/likelihood_settings.f90:198-420),
         ! write_sources, write_atimes (src/read_write.f90:484-530); paths are
         ! NUL-terminated character(kind=c_char) strings
-        function mct_read_sources(path, bnd, cap, n, ndim, src) bind(C, name="mct_read_sources")
+        function mct_read_sources(path, bnd, nbnd, cap, n, ndim, src) bind(C, name="mct_read_sources")
             import
             character(kind=c_char), dimension(*) :: path
             type(c_ptr), value :: bnd, src
-            integer(c_int), value :: cap
+            integer(c_int), value :: nbnd
+            integer(c_long), value :: cap
             integer(c_int) :: n, ndim
             integer(c_int) :: mct_read_sources
         end function

@@ -67,6 +67,20 @@ int mct_surfdisp96_batch(int nmod, int nl_max, const int32_t *nlayer, const floa
                          int np, const double *periods, int iwave, int igr, double dphase,
                          double *phase, double *group, int32_t *ierr);
 
+/* The surface likelihood's dispersion stage on explicit columns: check_model,
+ * convert_to_layer (thickness (k-last_k)*dz/scaling, vp = POISSON*vs, rho =
+ * 1.74*vp^0.25; src/likelihood_surf.F90:523-646, src/utils.f90:102-134) and
+ * surfdisp96 per column (surfmodes/surfmodes.f90:79-95).  vs (ncol, nz) holds
+ * Fortran vs(nz) per column; phase/group (ncol, np); ierr (ncol); bad_model
+ * (1): check_model failed for some column.  mode picks the kernel the
+ * likelihood uses: 1 one lane per column, 2 one wave per column, 3 the column
+ * queue in layer-count order with identical columns solved once (the default
+ * for large batches).  nz <= 100, np <= 60. */
+int mct_dispersion_columns(int ncol, int nz, double dz, double scaling, const double *vs,
+                           int np, const double *periods, int iwave, int igr, double dphase,
+                           int mode, double *phase, double *group, int32_t *ierr,
+                           int32_t *bad_model);
+
 /* ---- (c) 2-D phase traveltimes ---------------------------------------------- */
 /* modrays' phase-time path (fm2d/fm2dray_cartesian.f90:67-480, with
  * crazyray = 0, uar = 1, earth = 6371) for np periods at once.  velv holds np
@@ -81,13 +95,20 @@ int mct_surfdisp96_batch(int nmod, int nl_max, const int32_t *nlayer, const floa
  * only where its history shows (the source seed, the refined grid's stop and
  * the coarse restart around the source), the rest of the grid by parallel
  * sweeps to the same fixed point; problems that do not fit fall back to
- * method 1.  Returns MCT_ERR_OUTSIDE where modrays would stop on a source or
- * receiver outside the grid. */
+ * method 1, and so do problems whose converged field fails the check that the
+ * heap loop would freeze it in time order (on for fom 1; the environment's
+ * MCT_FM2D_VERIFY=0/1 overrides).  Returns MCT_ERR_OUTSIDE where modrays
+ * would stop on a source or receiver outside the grid. */
 int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvzd,
                 int np, const double *velv, int gdx, int gdz, int asgr, int sgdl, int sgs,
                 int fom, int nsrc, const double *scx, const double *scz, int nrc,
                 const double *rcx, const double *rcz, const int32_t *srs, double *ttime,
                 int method);
+
+/* Problems (period, source) the calling thread's last mct_modrays call handed
+ * from the parallel path to the sequential solver.  No counterpart in the
+ * reference: a measurement. */
+int mct_fm2d_last_handover(long long *n);
 
 /* ---- (c) 3-D eikonal: link-level drop-in ---------------------------------- */
 /* Same symbol, signature and semantics as the reference C ABI
@@ -165,10 +186,12 @@ int mct_body_noise_likelihood_batch_dev(mct_ctx *ctx, int nchain, const double *
 /* ---- data files (host only) ------------------------------------------------ */
 /* read_sources (src/likelihood_settings.f90:198-228): "nsrc ndim", then one
  * record of ndim values per source, Fortran list-directed.  With src NULL only
- * n and ndim are returned; otherwise src (n, ndim) (= Fortran src(ndim,n)) of
- * capacity cap sources.  bnd (2*ndim: min, max per coordinate) may be NULL;
- * a point outside it is MCT_ERR_OUTSIDE (out_bnd, src/settings.f90:143-154). */
-int mct_read_sources(const char *path, const double *bnd, int cap, int *n, int *ndim, double *src);
+ * n and ndim are returned; otherwise src (n, ndim) (= Fortran src(ndim,n)),
+ * which holds cap doubles: MCT_ERR_ARG before anything is written when
+ * n*ndim > cap.  bnd (nbnd doubles: min, max per coordinate) may be NULL; when
+ * given, nbnd must be 2*ndim (out_bnd compares coordinate i with pair i,
+ * src/settings.f90:143-154), and a point outside it is MCT_ERR_OUTSIDE. */
+int mct_read_sources(const char *path, const double *bnd, int nbnd, long cap, int *n, int *ndim, double *src);
 /* read_times_3 (src/likelihood_settings.f90:332-420) as read_data calls it (no
  * ds file): the number of periods (or wave types) np and their values, then per
  * (source, receiver) pair a validity flag and, for a valid pair, np records
@@ -412,6 +435,12 @@ int mct_surf_dispersion_solved(mct_surf_ctx *ctx, int nchain, int64_t *columns, 
  * sweep rounds (four directions each), mean diagonal steps executed, mean
  * node updates} per (period, source) problem. */
 int mct_surf_fm2d_stats(mct_surf_ctx *ctx, double *stats4);
+
+/* Problems of the last launch chunk that the parallel 2-D path handed to the
+ * sequential solver: {failed the heap-order check, all hand-overs (the check
+ * and a heap loop that outgrew its LDS neighbourhood)}.  No counterpart in
+ * the reference: a measurement. */
+int mct_surf_fm2d_handover(mct_surf_ctx *ctx, int64_t *handover2);
 
 /* The context's HIP stream (hipStream_t) for callers that queue their own
  * copies in order with the context's work. */

@@ -75,7 +75,8 @@ def lib():
                      "mct_read_sources", "mct_read_times", "mct_write_sources", "mct_write_times",
                      "mct_body_location_likelihood", "mct_body_location_likelihood_batch_dev",
                      "mct_serve_create_mix", "mct_serve_submit_to", "mct_serve_memory", "mct_surf_device_bytes",
-                     "mct_surf_set_tuning", "mct_surf_get_tuning", "mct_surf_likelihood_batch_graph"):
+                     "mct_surf_set_tuning", "mct_surf_get_tuning", "mct_surf_likelihood_batch_graph",
+                     "mct_fm2d_last_handover", "mct_surf_fm2d_handover", "mct_dispersion_columns"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -94,12 +95,12 @@ def read_sources(path, bnd=None):
     """read_sources (src/likelihood_settings.f90:198-228): (n, ndim) array.
     bnd: (min, max) per coordinate, or None."""
     n, nd = ctypes.c_int(), ctypes.c_int()
-    _check(lib().mct_read_sources(_path(path), None, 0, ctypes.byref(n), ctypes.byref(nd), None),
-           "mct_read_sources")
+    _check(lib().mct_read_sources(_path(path), None, 0, ctypes.c_long(0), ctypes.byref(n), ctypes.byref(nd),
+                                  None), "mct_read_sources")
     out = np.zeros((n.value, nd.value))
     b = None if bnd is None else _f64(bnd).reshape(-1)
-    _check(lib().mct_read_sources(_path(path), _p(b), n.value, ctypes.byref(n), ctypes.byref(nd), _p(out)),
-           "mct_read_sources")
+    _check(lib().mct_read_sources(_path(path), _p(b), 0 if b is None else b.size, ctypes.c_long(out.size),
+                                  ctypes.byref(n), ctypes.byref(nd), _p(out)), "mct_read_sources")
     return out
 
 
@@ -267,6 +268,25 @@ def surfdisp96_batch(models, periods, iwave, igr, dphase=1e-3):
     return ph, gr, ie
 
 
+def dispersion_columns(vs, dz, periods, iwave, igr, mode=3, dphase=1e-3, scaling=1.0):
+    """The surface likelihood's dispersion stage on explicit Vs columns
+    (check_model, convert_to_layer, surfdisp96).  vs: (ncol, nz).  mode 1 lane
+    per column, 2 wave per column, 3 column queue.  Returns phase (ncol, np),
+    group (ncol, np), ierr (ncol,), bad_model."""
+    vs = _f64(vs)
+    ncol, nz = vs.shape
+    per = _f64(periods)
+    ph = np.zeros((ncol, per.shape[0]))
+    gr = np.zeros((ncol, per.shape[0]))
+    ie = np.zeros(ncol, np.int32)
+    bad = np.zeros(1, np.int32)
+    rc = lib().mct_dispersion_columns(ncol, nz, ctypes.c_double(dz), ctypes.c_double(scaling), _p(vs),
+                                      per.shape[0], _p(per), int(iwave), int(igr), ctypes.c_double(dphase),
+                                      int(mode), _p(ph), _p(gr), _p(ie), _p(bad))
+    _check(rc, "mct_dispersion_columns")
+    return ph, gr, ie, int(bad[0])
+
+
 def modrays(velv, goxd, gozd, dvxd, dvzd, scx, scz, rcx, rcz, srs=None, gdx=1, gdz=1, asgr=1,
             sgdl=4, sgs=8, fom=1, ttime=None, method=0):
     """modrays phase times (fm2d/fm2dray_cartesian.f90:67-480) on the GPU.
@@ -296,6 +316,14 @@ def modrays(velv, goxd, gozd, dvxd, dvzd, scx, scz, rcx, rcz, srs=None, gdx=1, g
                            _p(srs), _p(tt), int(method))
     _check(rc, "mct_modrays")
     return tt[0] if single else tt
+
+
+def modrays_last_handover():
+    """Problems the calling thread's last modrays() call handed from the
+    parallel path to the sequential solver."""
+    n = ctypes.c_longlong()
+    _check(lib().mct_fm2d_last_handover(ctypes.byref(n)), "mct_fm2d_last_handover")
+    return n.value
 
 
 def kdtree_to_grid(grid, points, params, box=None, pm=None, vp=None, vs=None, rho=None):
@@ -701,6 +729,13 @@ class SurfLikelihood:
         t = np.zeros(4)
         _check(lib().mct_surf_fm2d_stats(self._h, _p(t)), "mct_surf_fm2d_stats")
         return tuple(t)
+
+    def fm2d_handover(self):
+        """(problems that failed the heap-order check, all problems handed to
+        the sequential solver) in the last 2-D chunk."""
+        t = np.zeros(2, np.int64)
+        _check(lib().mct_surf_fm2d_handover(self._h, _p(t)), "mct_surf_fm2d_handover")
+        return int(t[0]), int(t[1])
 
     def dev_arrays(self):
         """Device addresses of the phase_time and phase-velocity buffers."""

@@ -20,8 +20,11 @@
 // bisection refinement (:903-1020) is sequential and runs in all lanes in
 // lockstep.  Arithmetic types follow the Fortran (REAL*4 where the reference
 // is implicitly typed real, FP64 in the period equations); FP contraction is
-// off.  The period equations call the device libm (sin, cos, exp, sqrt).
+// off.  The period equations take exp, sin and cos from libm_glibc.h (the host
+// C library's numbers, which the reference's build uses) and sqrt and division
+// from the hardware's correctly rounded operations.
 #include "mct_internal.h"
+#include "libm_glibc.h"
 #include <float.h>
 
 #pragma clang fp contract(off)
@@ -81,17 +84,16 @@ __device__ static double d_dltar1(double wvno, double omega, const MD &M)
         double q = (double)M.d[M.st * (m - 1)] * rb;
         double sinq, y, z, cosq;
         if (wvno < xkb) {
-            sinq = sin(q);
+            mct_libm::libm_sincos(q, &sinq, &cosq);
             y = sinq / rb;
             z = -(rb * sinq);
-            cosq = cos(q);
         } else if (wvno == xkb) {
             cosq = 1.0;
             y = (double)M.d[M.st * (m - 1)];
             z = 0.0;
         } else {
             double fac = 0.0;
-            if (q < 16) fac = exp(-2.0 * q);
+            if (q < 16) fac = mct_libm::libm_exp(-2.0 * q);
             cosq = (1.0 + fac) * 0.5;
             sinq = (1.0 - fac) * 0.5;
             y = sinq / rb;
@@ -153,10 +155,9 @@ __device__ static double d_dltar4(double wvno, double omga, const MD &M)
         // ---- var
         double pex = 0.0, sex = 0.0, sinp, w = 0, x = 0, cosp = 0, sinq, y = 0, z = 0, cosq = 0;
         if (wvno < xka) {
-            sinp = sin(p);
+            mct_libm::libm_sincos(p, &sinp, &cosp);
             w = sinp / ra;
             x = -(ra * sinp);
-            cosp = cos(p);
         } else if (wvno == xka) {
             cosp = 1.0;
             w = dpth;
@@ -164,17 +165,16 @@ __device__ static double d_dltar4(double wvno, double omga, const MD &M)
         } else if (wvno > xka) {
             pex = p;
             double fac = 0.0;
-            if (p < 16) fac = exp(-2.0 * p);
+            if (p < 16) fac = mct_libm::libm_exp(-2.0 * p);
             cosp = (1.0 + fac) * 0.5;
             sinp = (1.0 - fac) * 0.5;
             w = sinp / ra;
             x = ra * sinp;
         }
         if (wvno < xkb) {
-            sinq = sin(q);
+            mct_libm::libm_sincos(q, &sinq, &cosq);
             y = sinq / rb;
             z = -(rb * sinq);
-            cosq = cos(q);
         } else if (wvno == xkb) {
             cosq = 1.0;
             y = dpth;
@@ -182,7 +182,7 @@ __device__ static double d_dltar4(double wvno, double omga, const MD &M)
         } else if (wvno > xkb) {
             sex = q;
             double fac = 0.0;
-            if (q < 16) fac = exp(-2.0 * q);
+            if (q < 16) fac = mct_libm::libm_exp(-2.0 * q);
             cosq = (1.0 + fac) * 0.5;
             sinq = (1.0 - fac) * 0.5;
             y = sinq / rb;
@@ -190,7 +190,7 @@ __device__ static double d_dltar4(double wvno, double omga, const MD &M)
         }
         double exa = pex + sex;
         double a0 = 0.0;
-        if (exa < 60.0) a0 = exp(-exa);
+        if (exa < 60.0) a0 = mct_libm::libm_exp(-exa);
         double cpcq = cosp * cosq, cpy = cosp * y, cpz = cosp * z, cqw = cosq * w, cqx = cosq * x;
         double xy = x * y, xz = x * z, wy = w * y, wz = w * z;
         // ---- dnka
@@ -253,11 +253,11 @@ __device__ static double d_dltar4(double wvno, double omga, const MD &M)
         rho1 = (double)M.rho[M.st * (0)];
         double p = ra * dpth;
         double w = 0, cosp = 0;
-        if (wvno < xka) { double sp = sin(p); w = sp / ra; cosp = cos(p); }
+        if (wvno < xka) { double sp; mct_libm::libm_sincos(p, &sp, &cosp); w = sp / ra; }
         else if (wvno == xka) { cosp = 1.0; w = dpth; }
         else {
             double fac = 0.0;
-            if (p < 16) fac = exp(-2.0 * p);
+            if (p < 16) fac = mct_libm::libm_exp(-2.0 * p);
             cosp = (1.0 + fac) * 0.5;
             w = ((1.0 - fac) * 0.5) / ra;
         }
@@ -1409,5 +1409,64 @@ extern "C" int mct_surfdisp96_batch(int nmod, int nl_max, const int32_t *nlayer,
         rc = MCT_ERR_HIP;
     void *bufs[] = {dn, dl, dp, dph, dg, de};
     for (void *b : bufs) if (b) (void)hipFree(b);
+    return rc;
+}
+
+// ---- C ABI: the likelihood's dispersion stage on explicit Vs columns -------------
+extern "C" int mct_dispersion_columns(int ncol, int nz, double dz, double scaling, const double *vs,
+                                      int np, const double *periods, int iwave, int igr, double dphase,
+                                      int mode, double *phase, double *group, int32_t *ierr,
+                                      int32_t *bad_model)
+{
+    if (ncol < 1 || nz < 1 || nz > DSP_NMAX || np < 1 || np > DSP_MAXP || !vs || !periods || !phase ||
+        !group || !ierr || !bad_model || (iwave != 1 && iwave != 2) || mode < 1 || mode > 3 ||
+        !(dz > 0) || !(scaling > 0))
+        return MCT_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MCT_ERR_HIP;
+    int hlog = 1;
+    while ((1L << hlog) < 2L * ncol) hlog++;
+    const long lay = mct_dispersion_lay_floats(ncol, nz, 1);
+    const size_t sizes[] = {sizeof(double) * (size_t)ncol * nz, sizeof(double) * np,
+                            sizeof(double) * (size_t)ncol * np, sizeof(double) * (size_t)ncol * np,
+                            sizeof(int32_t) * (size_t)ncol, sizeof(int32_t) * 4, sizeof(float) * (size_t)lay,
+                            256, sizeof(int32_t) * (size_t)ncol, sizeof(int32_t) * (size_t)ncol,
+                            sizeof(int32_t) * 128, 8 * ((size_t)1 << hlog), 4 * ((size_t)1 << hlog),
+                            sizeof(int32_t) * (size_t)ncol, 4 * 4};
+    const int nb = (int)(sizeof sizes / sizeof sizes[0]);
+    void *b[16] = {nullptr};
+    int rc = MCT_OK;
+    for (int i = 0; i < nb && !rc; i++)
+        if (hipMalloc(&b[i], sizes[i]) != hipSuccess) rc = MCT_ERR_NOMEM;
+    if (!rc && (hipMemcpy(b[0], vs, sizes[0], hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(b[1], periods, sizes[1], hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemset(b[7], 0, 256) != hipSuccess))
+        rc = MCT_ERR_HIP;
+    if (!rc) {
+        mct_dsp_args a;
+        memset(&a, 0, sizeof a);
+        a.ncol = ncol; a.nz = nz; a.nchain = 1; a.vs = (const double *)b[0];
+        a.dz = dz; a.scaling = scaling; a.periods = (const double *)b[1]; a.np = np;
+        a.iwave = iwave; a.igr = igr; a.dphase = dphase;
+        a.phase = (double *)b[2]; a.group = (double *)b[3]; a.ierr = (int32_t *)b[4];
+        a.bad_model = (int32_t *)b[5];
+        if (mode == 1 || mode == 3) a.lay = (float *)b[6];
+        if (mode == 3) {
+            a.queue = (unsigned long long *)b[7];
+            a.lay_waves = (ncol + 63) / 64;
+            a.jobs = (int32_t *)b[8]; a.keys = (int32_t *)b[9]; a.hist = (int32_t *)b[10];
+            a.hlog = hlog; a.hkey = (unsigned long long *)b[11]; a.hrep = (int32_t *)b[12];
+            a.dup = (int32_t *)b[13]; a.njobs = (int32_t *)b[14];
+        }
+        rc = mct_launch_dispersion(a, 0);
+        if (!rc && hipDeviceSynchronize() != hipSuccess) rc = MCT_ERR_HIP;
+    }
+    if (!rc && (hipMemcpy(phase, b[2], sizes[2], hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(group, b[3], sizes[3], hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(ierr, b[4], sizes[4], hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(bad_model, b[5], sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess))
+        rc = MCT_ERR_HIP;
+    for (int i = 0; i < nb; i++)
+        if (b[i]) (void)hipFree(b[i]);
     return rc;
 }

@@ -613,6 +613,17 @@ int mct_launch_fm2d(const mct_fm2d_args &a, long chunk, hipStream_t s)
     return MCT_OK;
 }
 
+// problems the calling thread's last mct_modrays call handed from the parallel
+// path to the sequential solver
+static thread_local long long g_modrays_handover = 0;
+
+extern "C" int mct_fm2d_last_handover(long long *n)
+{
+    if (!n) return MCT_ERR_ARG;
+    *n = g_modrays_handover;
+    return MCT_OK;
+}
+
 // ---- C ABI: modrays' phase times for one or more periods --------------------------
 extern "C" int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvzd,
                            int np, const double *velv, int gdx, int gdz, int asgr, int sgdl, int sgs,
@@ -673,6 +684,13 @@ extern "C" int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dv
     if (!rc && (hipMemcpy(ttime, bufs[13], 8 * (size_t)nprob * nrc, hipMemcpyDeviceToHost) != hipSuccess ||
                 hipMemcpy(&st, bufs[14], 4, hipMemcpyDeviceToHost) != hipSuccess))
         rc = MCT_ERR_HIP;
+    g_modrays_handover = 0;
+    if (!rc && fast) {
+        int32_t *pf = (int32_t *)malloc(4 * (size_t)nprob);
+        if (pf && hipMemcpy(pf, bufs[15], 4 * (size_t)nprob, hipMemcpyDeviceToHost) == hipSuccess)
+            for (long i = 0; i < nprob; i++) g_modrays_handover += pf[i] == 1;
+        free(pf);
+    }
     for (int i = 0; i < 16; i++) if (bufs[i]) (void)hipFree(bufs[i]);
     if (!rc && st < 0) rc = st;
     return rc;

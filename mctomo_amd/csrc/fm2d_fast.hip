@@ -1072,6 +1072,7 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
     const int lane = threadIdx.x;
     near_geom g;
     const long slot = ff_xcd_slot(blockIdx.x, gridDim.x);
+    if (lane == 0 && A.rounds_out) A.rounds_out[4 * slot + 3] = 0;   // set when the check hands it over
     if (!ff_problem(A, slot, g, true)) {
         if (lane == 0) pflag[slot] = 2;   // nothing to do
         return;
@@ -1323,8 +1324,21 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
         if (lane == 0) sh_ovf = S.overflow || H.full;
         __syncthreads();
     }
+    if (A.verify > 0 && lane == 0 && !sh_ovf) {
+        // the band handed to the sweep must be a valid heap: fouds overwrites a
+        // key and updtree only sifts up, so a raised key can leave it out of
+        // order, and then the pops that follow are not in time order
+        for (int q = 2; q <= H.ntr; q++) {
+            const int ec = H.btg[q], ep = H.btg[q / 2];
+            if (C.t(HPZ(ec), HPX(ec)) < C.t(HPZ(ep), HPX(ep))) { sh_ovf = 4; break; }
+        }
+    }
+    __syncthreads();
     if (sh_ovf) {
-        if (lane == 0) pflag[slot] = 1;
+        if (lane == 0) {
+            pflag[slot] = 1;
+            if (A.rounds_out && sh_ovf == 4) A.rounds_out[4 * slot + 3] = 1;
+        }
         return;
     }
     // write the coarse grid: frozen nodes with their times, the rest open; the
@@ -1340,6 +1354,15 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
     for (int q = lane; q < C.lnx * C.lnz; q += 64) {
         int k = C.ox + q / C.lnz + 1, l = C.oz + q % C.lnz + 1;
         F[(long)(k - 1) * A.ld + (l - 1)] = C.n(l, k) == 0 ? 1 : 0;
+    }
+    if (A.verify > 0) {
+        // the band's keys at the hand-over, for k_fm2d_verify (the refined
+        // velocities in the slot's vr are no longer read)
+        double *Wb = A.vr + slot * wsz;
+        for (int q = lane; q < C.lnx * C.lnz; q += 64) {
+            int k = C.ox + q / C.lnz + 1, l = C.oz + q % C.lnz + 1;
+            Wb[(long)(k - 1) * A.ld + (l - 1)] = C.n(l, k) > 0 ? C.t(l, k) : FF_BIG;
+        }
     }
     if (lane == 0) pflag[slot] = 0;
 }
@@ -1685,6 +1708,119 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, const int32_
     ff_srtimes(A, g, V, vel);
 }
 
+// ---- the heap loop's answer, checked -------------------------------------------
+// The sweep's field T satisfies T(n) = fouds(n; alive = {T < T(n)}).  The
+// reference's heap loop gives the same field if, from the hand-over on, it
+// freezes the open nodes in the order of T.  Its key for an open node n is
+// the band key W(n) at the hand-over (or none), then the update after each
+// of n's open neighbours d_1 < d_2 < ... < d_r (the ones with T below T(n))
+// freezes, over the alive set then: the fixed nodes and the open nodes with
+// T <= T(d_i).  Call those keys V_0 = W(n), V_1, ..., V_r.  If for every open
+// node
+//   (a) V_i <= V_{i-1}: keys only fall, so updtree's sift-up keeps the heap
+//       valid (fouds overwrites a key, it may rise);
+//   (b) V_{i-1} > T(d_i): n is not popped before the neighbour that updates
+//       it next;
+//   (c) T(n) differs from every neighbour's T, and the T(d_i) from each
+//       other: the freeze order is not left to the heap's tie positions;
+//   (e) V_r = T(n);
+// and the band at the hand-over is a valid heap (checked by k_fm2d_near),
+// then by induction over T the heap loop pops the open nodes in the order of
+// T, each with key T(n): the node with the smallest T of the unfrozen ones
+// holds key T(n) by (e), every other one a larger key by (b) and (c).  So a
+// problem that passes has the reference's times; one that fails goes to the
+// sequential solver.  The sweep stops at 1e-14 relative change, so (a) and
+// (e) compare within 1e-12 relative.
+template <int FOM, class FX>
+__device__ __forceinline__ bool ff_verify_node(const fview &V, const FX &fixed, double w0, int z, int x,
+                                               double slown)
+{
+    const double tn = V.t(z, x);
+    if (!(tn < FF_FIN)) return false;
+    double d[4];
+    int r = 0;
+    const int nz_[4] = {z, z, z - 1, z + 1}, nx_[4] = {x - 1, x + 1, x, x};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int zz = nz_[k], xx = nx_[k];
+        if (zz < 1 || zz > V.nnz || xx < 1 || xx > V.nnx) continue;
+        const double t = V.t(zz, xx);
+        if (t == tn) return false;
+        if (t < tn && !fixed(zz, xx)) {
+            int i = r++;
+            while (i > 0 && d[i - 1] > t) { d[i] = d[i - 1]; i--; }
+            d[i] = t;
+        }
+    }
+    for (int i = 1; i < r; i++)
+        if (d[i] == d[i - 1]) return false;
+    double v = w0;
+    for (int i = 0; i < r; i++) {
+        if (!(v > d[i])) return false;
+        const double thr = d[i];
+        auto al = [&](int zz, int xx) -> bool { return fixed(zz, xx) || V.t(zz, xx) <= thr; };
+        const double vn = FOM == 0 ? ff_fouds1(V, al, z, x, slown) : ff_fouds2(V, al, z, x, slown);
+        if (vn > v && vn - v > 1e-12 * vn) return false;
+        v = vn;
+    }
+    return v == tn || fabs(v - tn) <= 1e-12 * tn;
+}
+
+// One workgroup per problem over the converged coarse field; a problem that
+// fails the check is flagged for the sequential solver (pflag 1).
+template <int FOM>
+__global__ __launch_bounds__(256) void k_fm2d_verify(mct_fm2d_args A, int32_t *pflag)
+{
+    extern __shared__ uint32_t ff_vfb[];
+    __shared__ int bad;
+    const long slot = ff_xcd_slot(blockIdx.x, gridDim.x);
+    if (pflag[slot] != 0) return;
+    near_geom g;
+    if (!ff_problem(A, slot, g, false)) return;
+    const int nnx = A.nnx, nnz = A.nnz, ld = A.ld;
+    const long wsz = (long)ld * A.nc;
+    const long cp = g.prob / A.nsrc;
+    const double *velnc = A.veln + cp * (long)nnx * nnz;
+    fview V;
+    V.nnx = nnx; V.nnz = nnz; V.dnx = A.dvx / A.gdx; V.dnz = A.dvz / A.gdz;
+    V.T = A.ttn + slot * wsz; V.ns = nullptr; V.ox = 0; V.oz = 0; V.lnx = nnx; V.lnz = nnz; V.ld = ld;
+    const int32_t *F = A.nsts + slot * wsz;
+    const double *Wb = A.vr + slot * wsz;
+    const ff_win win = ff_window(A, g);
+    const int wn = win.nodes(), wnz = win.nz(), fw = (wn + 31) / 32;
+    for (int w = threadIdx.x; w < fw; w += blockDim.x) {
+        uint32_t m = 0;
+        for (int b = 0; b < 32; b++) {
+            const int q = w * 32 + b;
+            if (q < wn) {
+                const int x = win.x0 + q / wnz, z = win.z0 + q % wnz;
+                if (F[(long)(x - 1) * ld + (z - 1)] != 0) m |= 1u << b;
+            }
+        }
+        ff_vfb[w] = m;
+    }
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    auto fixed = [&](int z, int x) -> bool {
+        if (!win.in(z, x)) return false;
+        const int q = (x - win.x0) * wnz + (z - win.z0);
+        return (ff_vfb[q >> 5] >> (q & 31)) & 1u;
+    };
+    const long nn = (long)nnx * nnz;
+    for (long q = threadIdx.x; q < nn; q += blockDim.x) {
+        if (*(volatile int *)&bad) break;
+        const int x = (int)(q / nnz) + 1, z = (int)(q % nnz) + 1;
+        if (fixed(z, x)) continue;
+        const double w0 = win.in(z, x) ? Wb[(long)(x - 1) * ld + (z - 1)] : FF_BIG;
+        if (!ff_verify_node<FOM>(V, fixed, w0, z, x, 1.0 / velnc[(long)(x - 1) * nnz + (z - 1)])) bad = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && bad) {
+        pflag[slot] = 1;
+        if (A.rounds_out) A.rounds_out[4 * slot + 3] = 1;
+    }
+}
+
 // LDS bytes of k_fm2d_strip, or 0 when the grid needs more than the limit.
 static size_t ff_strip_lds(const mct_fm2d_args &a)
 {
@@ -1763,8 +1899,15 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
     const size_t strip_lds = sweep_kind ? ff_strip_lds(a) : 0;
     const long nprob = (long)a.nchain * a.np * a.nsrc;
     if (chunk < 1) return MCT_ERR_ARG;
+    // the heap-order check (k_fm2d_verify): on by default for the mixed-order update
+    int verify = a.verify;
+    if (verify == 0) {
+        const char *ev = getenv("MCT_FM2D_VERIFY");
+        verify = ev ? (atoi(ev) ? 1 : -1) : (a.fom == 1 ? 1 : -1);
+    }
     for (long p0 = 0; p0 < nprob; p0 += chunk) {
         mct_fm2d_args b = a;
+        b.verify = verify > 0 ? 1 : -1;
         b.prob0 = p0;
         b.nrun = nprob - p0 < chunk ? nprob - p0 : chunk;
         hipLaunchKernelGGL(k_fm2d_near, dim3((unsigned)b.nrun), dim3(64), lds, s, b, region, heap_cap, win, pflag);
@@ -1781,7 +1924,16 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
             hipLaunchKernelGGL(k_fm2d_sweep<1>, dim3((unsigned)b.nrun), dim3(sweep_threads), dirty_lds, s, b,
                                (const int32_t *)pflag, use_dirty);
         MCT_CHECK(hipGetLastError());
-        // problems whose heap loop left the LDS neighbourhood: the sequential solver
+        if (verify > 0) {
+            const size_t vlds = 4 * (size_t)ff_win_words(a);
+            if (a.fom == 0)
+                hipLaunchKernelGGL(k_fm2d_verify<0>, dim3((unsigned)b.nrun), dim3(256), vlds, s, b, pflag);
+            else
+                hipLaunchKernelGGL(k_fm2d_verify<1>, dim3((unsigned)b.nrun), dim3(256), vlds, s, b, pflag);
+            MCT_CHECK(hipGetLastError());
+        }
+        // problems whose heap loop left the LDS neighbourhood, or that the check
+        // above failed: the sequential solver
         b.only_flag = pflag;
         if ((rc = mct_launch_fm2d_solve(b, s))) return rc;
         if ((rc = mct_launch_fm2d_rays(b, s))) return rc;

@@ -122,9 +122,9 @@ void put_real(FILE *f, double v)
 
 extern "C" {
 
-int mct_read_sources(const char *path, const double *bnd, int cap, int *n, int *ndim, double *src)
+int mct_read_sources(const char *path, const double *bnd, int nbnd, long cap, int *n, int *ndim, double *src)
 {
-    if (!path || !n || !ndim) return MCT_ERR_ARG;
+    if (!path || !n || !ndim || (bnd && nbnd < 0)) return MCT_ERR_ARG;
     ld_reader R;
     R.f = fopen(path, "r");
     if (!R.f) return MCT_ERR_IO;
@@ -135,7 +135,9 @@ int mct_read_sources(const char *path, const double *bnd, int cap, int *n, int *
     *n = ns;
     *ndim = nd;
     if (!src) return MCT_OK;                    // sizes only
-    if (ns > cap) return MCT_ERR_ARG;
+    // src holds cap doubles; out_bnd compares each coordinate with its (min,
+    // max) pair, so bnd must hold one pair per coordinate of the file
+    if ((long)ns * nd > cap || (bnd && nbnd != 2 * nd)) return MCT_ERR_ARG;
     for (int i = 0; i < ns; i++) {
         if (!R.read(nd, t)) return MCT_ERR_IO;
         for (int k = 0; k < (int)t.size(); k++)

@@ -189,6 +189,11 @@ struct mct_fm2d_args {
     // launch shape of the parallel path (0: the environment's or the default)
     int sweep_kind;             // 1 strip-pipelined, 2 anti-diagonal
     int sweep_threads;          // anti-diagonal sweep: 64, 128 or 256 threads per problem
+    // parallel path: check each converged field against the heap loop's freeze
+    // order and hand the problems that fail to the sequential solver (1 on,
+    // -1 off, 0 the default: the environment's MCT_FM2D_VERIFY, else on for
+    // the mixed-order update, fom 1)
+    int verify;
 };
 // Coarse B-spline velocity per (chain, period), then one lane per
 // (chain, period, source) running the sequential modrays phase solve.

@@ -860,6 +860,29 @@ int mct_surf_noise_likelihood_batch_dev(mct_surf_ctx *C, int nchain, const doubl
     return surf_noise(C, nchain, noise_dev, out_dev, status_dev);
 }
 
+int mct_surf_fm2d_handover(mct_surf_ctx *C, int64_t *handover2)
+{
+    if (!C || !handover2) return MCT_ERR_ARG;
+    const long n = C->fm2d_chunk;
+    int32_t *r = (int32_t *)malloc(sizeof(int32_t) * 5 * n);
+    if (!r) return MCT_ERR_NOMEM;
+    if (hipStreamSynchronize(C->stream) != hipSuccess ||
+        hipMemcpy(r, C->rounds.p, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(r + 4 * n, C->pflag.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+        free(r);
+        return MCT_ERR_HIP;
+    }
+    int64_t chk = 0, all = 0;
+    for (long i = 0; i < n; i++) {
+        chk += r[4 * i + 3] == 1;
+        all += r[4 * n + i] == 1;
+    }
+    free(r);
+    handover2[0] = chk;
+    handover2[1] = all;
+    return MCT_OK;
+}
+
 int mct_surf_split_parts(mct_surf_ctx *C, int nchain, int *first_part_chains)
 {
     if (!C || nchain < 1 || !first_part_chains) return MCT_ERR_ARG;

@@ -145,3 +145,28 @@ def test_read_sources_bounds_and_errors(tmp_path):
     with pytest.raises(mct.MCTError) as e:
         mct.read_sources(str(tmp_path / "missing.dat"))
     assert e.value.code == -7
+
+
+def test_read_sources_checks_the_callers_capacity(tmp_path):
+    """A file with more coordinates per source than the caller sized for is an
+    argument error before anything is written: src holds `cap` doubles, and bnd
+    must hold one (min, max) pair per coordinate of the file."""
+    import ctypes
+    p = str(tmp_path / "s4.dat")
+    with open(p, "w") as f:
+        f.write("2 4\n1 2 3 4\n5 6 7 8\n")
+    L = mct.lib()
+    n, nd = ctypes.c_int(), ctypes.c_int()
+    guard = np.full(8, -7.0)                     # room for 2 sources x 3 coordinates, then a guard
+    rc = L.mct_read_sources(p.encode(), None, 0, ctypes.c_long(6), ctypes.byref(n), ctypes.byref(nd),
+                            guard.ctypes.data_as(ctypes.c_void_p))
+    assert rc == -1 and n.value == 2 and nd.value == 4
+    assert (guard == -7.0).all()
+    bnd = np.array([0, 10, 0, 10, 0, 10.0])      # three pairs for a four-coordinate file
+    out = np.zeros(8)
+    rc = L.mct_read_sources(p.encode(), bnd.ctypes.data_as(ctypes.c_void_p), 6, ctypes.c_long(8),
+                            ctypes.byref(n), ctypes.byref(nd), out.ctypes.data_as(ctypes.c_void_p))
+    assert rc == -1
+    rc = L.mct_read_sources(p.encode(), None, 0, ctypes.c_long(8), ctypes.byref(n), ctypes.byref(nd),
+                            out.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0 and out.tolist() == [1, 2, 3, 4, 5, 6, 7, 8]
