@@ -38,6 +38,7 @@ module mctomo_hip
         real(c_double) :: band
         integer(c_int) :: max_cells, max_chains
         integer(c_int) :: straight
+        integer(c_int) :: datatype
     end type
 
     interface
@@ -176,6 +177,53 @@ module mctomo_hip
             import
             type(c_ptr), value :: ctx, snoise0, snoise1, out
             integer(c_int) :: mct_surf_noise_likelihood
+        end function
+
+        ! noise_likelihood over the caller's like%phaseTime (and like%srdist)
+        function mct_body_noise_likelihood_from(ctx, phase_time, locations, bnoise0, bnoise1, out, sigma) &
+                bind(C, name="mct_body_noise_likelihood_from")
+            import
+            type(c_ptr), value :: ctx, phase_time, locations, bnoise0, bnoise1, out, sigma
+            integer(c_int) :: mct_body_noise_likelihood_from
+        end function
+
+        function mct_surf_noise_likelihood_from(ctx, phase_time, srdist, snoise0, snoise1, out, sigma) &
+                bind(C, name="mct_surf_noise_likelihood_from")
+            import
+            type(c_ptr), value :: ctx, phase_time, srdist, snoise0, snoise1, out, sigma
+            integer(c_int) :: mct_surf_noise_likelihood_from
+        end function
+
+        ! datatype 3: the joint body + surface context (src/likelihood.f90:83-88)
+        function mct_joint_create(grid, body, surf, ctx) bind(C, name="mct_joint_create")
+            import
+            type(mct_grid), intent(in) :: grid
+            type(mct_body_setup), intent(in) :: body
+            type(mct_surf_setup), intent(in) :: surf
+            type(c_ptr) :: ctx
+            integer(c_int) :: mct_joint_create
+        end function
+
+        function mct_joint_destroy(ctx) bind(C, name="mct_joint_destroy")
+            import
+            type(c_ptr), value :: ctx
+            integer(c_int) :: mct_joint_destroy
+        end function
+
+        function mct_joint_parts(ctx, body, surf) bind(C, name="mct_joint_parts")
+            import
+            type(c_ptr), value :: ctx
+            type(c_ptr) :: body, surf
+            integer(c_int) :: mct_joint_parts
+        end function
+
+        function mct_joint_likelihood(ctx, points, params, ncells, locations, bnoise0, bnoise1, snoise0, &
+                snoise1, out, out_parts) bind(C, name="mct_joint_likelihood")
+            import
+            type(c_ptr), value :: ctx, points, params
+            integer(c_int), value :: ncells
+            type(c_ptr), value :: locations, bnoise0, bnoise1, snoise0, snoise1, out, out_parts
+            integer(c_int) :: mct_joint_likelihood
         end function
 
         ! data files: read_sources / read_times_3 (src/likelihood_settings.f90:198-420),

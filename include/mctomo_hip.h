@@ -127,7 +127,8 @@ typedef struct mct_ctx mct_ctx;
  * body-wave data set and the T_LIKE_SET knobs that the body path reads
  * (src/likelihood_settings.f90:47-79). */
 typedef struct {
-    int datatype;             /* 0: P only (np=1), 1: P & S (np=2) */
+    int datatype;             /* 0: P only (np=1), 1: P & S (np=2), 3: P & S as the body half of
+                                 the joint likelihood (dat(1)) */
     int order;                /* 0 first order, 1 mixed order (LIKE_SET%ORDER) */
     int sigdep;               /* MCMC_SET%SIGDEP: sigma = b0*t + b1 when != 0 */
     int nsrc, nrev;
@@ -170,6 +171,9 @@ int mct_body_likelihood_batch_dev(mct_ctx *ctx, int nchain, const double *points
                                   const double *params_dev, const int32_t *ncells_dev,
                                   const double *locations_dev, const double *noise_dev,
                                   double *out_dev, int32_t *status_dev);
+/* The context's HIP stream (hipStream_t), for callers that order their own
+ * work with the context's. */
+int mct_body_stream(mct_ctx *ctx, void **stream);
 
 /* noise_likelihood for datatype 0, 1 (src/likelihood.f90:131-146 ->
  * body_noise_likelihood, src/likelihood_body.F90:38-78): the likelihood of the
@@ -180,6 +184,13 @@ int mct_body_noise_likelihood(mct_ctx *ctx, const double *locations, const doubl
                               const double *bnoise1, double *out);
 /* The same for the first nchain chains of the last batch: locations_dev
  * (nchain, nsrc, 4) or NULL, noise_dev (nchain, 2, np); asynchronous. */
+/* The same over traveltimes the caller passes: phase_time (nrev,nsrc,np)
+ * Fortran = like%likelihoods(1)%phaseTime, as the likelihood call wrote it.
+ * The reference's body_noise_likelihood reads that array (copied with
+ * like_copy, so restored on reject); this form keeps the drop-in stateless.
+ * sigma (nsrc*nrev,np) may be NULL; else it receives like%sigma. */
+int mct_body_noise_likelihood_from(mct_ctx *ctx, const double *phase_time, const double *locations,
+                                   const double *bnoise0, const double *bnoise1, double *out, double *sigma);
 int mct_body_noise_likelihood_batch_dev(mct_ctx *ctx, int nchain, const double *locations_dev,
                                         const double *noise_dev, double *out_dev, int32_t *status_dev);
 
@@ -293,6 +304,11 @@ typedef struct {
     int straight;             /* LIKE_SET%isStraight: 1 = times along straight source-receiver
                                  rays (src/likelihood_surf.F90:233-243,412-452), no 2-D solve;
                                  srdist = the distances */
+    int datatype;             /* LIKE_SET%datatype: 2 (or 0) surface waves alone, vp = 1.73 vs and
+                                 rho = 1.74 vp^0.25 (vs2vp_3d / vp2rho_3d, src/likelihood.f90:
+                                 75-76); 3 the surface half of the joint likelihood: vp and rho
+                                 are the sites' own parameters (RTI%parameters(1,:), (3,:);
+                                 src/likelihood.f90:83-88) */
 } mct_surf_setup;
 
 typedef struct mct_surf_ctx mct_surf_ctx;
@@ -377,6 +393,13 @@ int mct_surf_group_velocities(mct_surf_ctx *ctx, int chain, double *group_vel);
  * parameters, sigma = snoise0*t + snoise1 (np each).  No dispersion or 2-D
  * solve.  MCT_ERR_SIGMA where the reference stops on a zero noise level. */
 int mct_surf_noise_likelihood(mct_surf_ctx *ctx, const double *snoise0, const double *snoise1, double *out);
+/* surf_noise_likelihood over traveltimes the caller passes (phase_time
+ * (nrev,nsrc,np) Fortran = like%phaseTime; srdist (nsrc*nrev,np) = like%srdist,
+ * needed with group velocities or straight rays, else NULL), so a sampler
+ * that restored like_copy after a reject gets the accepted state's misfit.
+ * sigma may be NULL. */
+int mct_surf_noise_likelihood_from(mct_surf_ctx *ctx, const double *phase_time, const double *srdist,
+                                   const double *snoise0, const double *snoise1, double *out, double *sigma);
 /* The same for the first nchain chains of the last batch: noise_dev
  * (nchain, 2, np); asynchronous on the context's stream. */
 int mct_surf_noise_likelihood_batch_dev(mct_surf_ctx *ctx, int nchain, const double *noise_dev,
@@ -445,6 +468,39 @@ int mct_surf_fm2d_handover(mct_surf_ctx *ctx, int64_t *handover2);
 /* The context's HIP stream (hipStream_t) for callers that queue their own
  * copies in order with the context's work. */
 int mct_surf_stream(mct_surf_ctx *ctx, void **stream);
+
+/* ---- joint body + surface-wave likelihood (datatype 3) ---------------------- */
+/* like_setup / likelihood for LIKE_SET%datatype = 3 (src/likelihood.f90:44-52,
+ * 83-88): body_likelihood on dat(1) with P and S plus surf_likelihood on
+ * dat(2), whose layers take the sites' own vp and rho, summed.  body must have
+ * datatype 3, surf datatype 3, and both the same max_cells and max_chains; the
+ * grid is shared.  A Fortran caller can equally keep the two contexts itself
+ * and add the results, as the reference does; this context runs the two
+ * pipelines of a batch at once on their own streams. */
+typedef struct mct_joint_ctx mct_joint_ctx;
+int mct_joint_create(const mct_grid *grid, const mct_body_setup *body, const mct_surf_setup *surf,
+                     mct_joint_ctx **ctx);
+int mct_joint_destroy(mct_joint_ctx *ctx);
+/* The joint context's body and surface contexts (for their noise and
+ * location likelihoods, arrays and timings; owned by the joint context). */
+int mct_joint_parts(mct_joint_ctx *ctx, mct_ctx **body, mct_surf_ctx **surf);
+/* One chain: points/params (ncells,3), locations (nsrc,4), bnoise0/bnoise1
+ * (2: P, S), snoise0/snoise1 (np surface periods); any noise may be NULL
+ * (zero).  out = {like, misfit, unweighted} summed; out_parts (6, may be NULL)
+ * = the body then the surface triple.  Returns the status: 0, > 0 the surface
+ * half rejected the model (like = huge), < 0 fatal. */
+int mct_joint_likelihood(mct_joint_ctx *ctx, const double *points, const double *params, int ncells,
+                         const double *locations, const double *bnoise0, const double *bnoise1,
+                         const double *snoise0, const double *snoise1, double *out, double *out_parts);
+/* nchain chains on device arrays: points/params (nchain,max_cells,3), ncells
+ * (nchain), locations (nchain,nsrc,4), bnoise (nchain,2,2), snoise
+ * (nchain,2,np); out (nchain,3), status (nchain).  Asynchronous; mct_joint_sync
+ * waits. */
+int mct_joint_likelihood_batch_dev(mct_joint_ctx *ctx, int nchain, const double *points_dev,
+                                   const double *params_dev, const int32_t *ncells_dev,
+                                   const double *locations_dev, const double *bnoise_dev,
+                                   const double *snoise_dev, double *out_dev, int32_t *status_dev);
+int mct_joint_sync(mct_joint_ctx *ctx);
 
 /* ---- served form: continuous batching -------------------------------------- */
 /* No counterpart in the reference (its sampler calls likelihood() one chain

@@ -19,6 +19,6 @@ falls back to CPU code: a missing library raises.
 """
 from .api import (  # noqa: F401
     MCTError, Grid, DeviceArray, lib, lib_path, version, fast_marching_3d, kdtree_to_grid, surfdisp96_batch, dispersion_columns, modrays, modrays_last_handover,
-    BodyLikelihood, SurfLikelihood, SurfServer, REJECT_MODEL, REJECT_RAY,
+    BodyLikelihood, SurfLikelihood, SurfServer, JointLikelihood, REJECT_MODEL, REJECT_RAY,
     read_sources, read_times, read_data, write_sources, write_times,
 )

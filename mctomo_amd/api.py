@@ -76,7 +76,10 @@ def lib():
                      "mct_body_location_likelihood", "mct_body_location_likelihood_batch_dev",
                      "mct_serve_create_mix", "mct_serve_submit_to", "mct_serve_memory", "mct_surf_device_bytes",
                      "mct_surf_set_tuning", "mct_surf_get_tuning", "mct_surf_likelihood_batch_graph",
-                     "mct_fm2d_last_handover", "mct_surf_fm2d_handover", "mct_dispersion_columns"):
+                     "mct_fm2d_last_handover", "mct_surf_fm2d_handover", "mct_dispersion_columns",
+                     "mct_body_stream", "mct_joint_create", "mct_joint_destroy", "mct_joint_parts",
+                     "mct_joint_likelihood", "mct_joint_likelihood_batch_dev", "mct_joint_sync",
+                     "mct_body_noise_likelihood_from", "mct_surf_noise_likelihood_from"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -481,12 +484,12 @@ class _SurfSetup(ctypes.Structure):
                 ("nrays", ctypes.c_int), ("gridx", ctypes.c_int), ("gridy", ctypes.c_int),
                 ("sgref", ctypes.c_int), ("sgdic", ctypes.c_int), ("sgext", ctypes.c_int),
                 ("order", ctypes.c_int), ("band", ctypes.c_double), ("max_cells", ctypes.c_int),
-                ("max_chains", ctypes.c_int), ("straight", ctypes.c_int)]
+                ("max_chains", ctypes.c_int), ("straight", ctypes.c_int), ("datatype", ctypes.c_int)]
 
 
 def _surf_setup(grid, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3, sigdep=0, nrays=None,
                 gridx=1, gridy=1, sgref=1, sgdic=4, sgext=8, order=1, band=0.5, max_cells=1024,
-                max_chains=1, phase_group=0, straight=0):
+                max_chains=1, phase_group=0, straight=0, datatype=2):
     """A filled mct_surf_setup and the arrays it points to (keep them alive)."""
     fr = _f64(freqs).reshape(-1)
     npd = fr.shape[0]
@@ -499,7 +502,7 @@ def _surf_setup(grid, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3, si
     st = _SurfSetup(int(raylov), int(phase_group), float(dphase), npd, fr.ctypes.data, sr.shape[0], rv.shape[0],
                     sr.ctypes.data, rv.ctypes.data, tt.ctypes.data, rs.ctypes.data, int(sigdep), int(nrays),
                     int(gridx), int(gridy), int(sgref), int(sgdic), int(sgext), int(order), float(band),
-                    int(max_cells), int(max_chains), int(straight))
+                    int(max_cells), int(max_chains), int(straight), int(datatype))
     return st, (fr, sr, rv, tt, rs)
 
 
@@ -512,11 +515,13 @@ class SurfLikelihood:
     raystat (np,2,nsrc*nrev).  Settings as LIKE_SET: raylov (1 Rayleigh,
     0 Love), phase_group (0 phase, 1 group velocities: group delays along
     rays), dphase (dPhaseVel), gridx/gridy, sgref/sgdic/sgext, order, band.
+    datatype 2: vp and rho follow from Vs; 3: the surface half of the joint
+    likelihood, vp and rho are the sites' own (params columns 0 and 2).
     """
 
     def __init__(self, grid, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3, sigdep=0,
                  nrays=None, gridx=1, gridy=1, sgref=1, sgdic=4, sgext=8, order=1, band=0.5,
-                 max_cells=1024, max_chains=1, phase_group=0, straight=0):
+                 max_cells=1024, max_chains=1, phase_group=0, straight=0, datatype=2):
         self.grid = grid
         self.freqs = _f64(freqs).reshape(-1)
         self.np = self.freqs.shape[0]
@@ -535,7 +540,7 @@ class SurfLikelihood:
                         self.nrev, self.src.ctypes.data, self.rev.ctypes.data, self.ttime.ctypes.data,
                         self.raystat.ctypes.data, int(sigdep), int(nrays), int(gridx), int(gridy),
                         int(sgref), int(sgdic), int(sgext), int(order), float(band), int(max_cells),
-                        int(max_chains), int(straight))
+                        int(max_chains), int(straight), int(datatype))
         self._g = grid.c()
         h = ctypes.c_void_p()
         _check(lib().mct_surf_create(ctypes.byref(self._g), ctypes.byref(st), ctypes.byref(h)),
@@ -742,6 +747,77 @@ class SurfLikelihood:
         a, b = ctypes.c_void_p(), ctypes.c_void_p()
         lib().mct_surf_dev_arrays(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+
+class JointLikelihood:
+    """Joint body + surface-wave likelihood (datatype 3, src/likelihood.f90:83-88):
+    body data (P and S) and surface data on one grid, the surface layers taking
+    the sites' own vp and rho.  body: dict(rev, ttime, raystat, nsrc[, order,
+    sigdep, nrays]); surf: dict(freqs, src, rev, ttime, raystat[, raylov,
+    order, ...]) as SurfLikelihood takes them."""
+
+    def __init__(self, grid, body, surf, max_cells=1024, max_chains=1):
+        self.grid = grid
+        b = dict(body)
+        self.nsrc = int(b["nsrc"])
+        self._brev = _f64(b["rev"]).reshape(-1, 3)
+        nrev = self._brev.shape[0]
+        self._btt = _f64(b["ttime"]).reshape(2, 3, self.nsrc * nrev)
+        self._brs = np.ascontiguousarray(b["raystat"], dtype=np.int32).reshape(2, 2, -1)
+        nrays = b.get("nrays")
+        if nrays is None:
+            nrays = int((self._brs[:, 0, :] == 1).sum())
+        bst = _BodySetup(3, int(b.get("order", 0)), int(b.get("sigdep", 0)), self.nsrc, nrev,
+                         self._brev.ctypes.data, self._btt.ctypes.data, self._brs.ctypes.data,
+                         int(nrays), int(max_cells), int(max_chains), 0)
+        sk = dict(surf)
+        args = [sk.pop(k) for k in ("freqs", "src", "rev", "ttime", "raystat")]
+        sst, self._keep = _surf_setup(grid, *args, max_cells=max_cells, max_chains=max_chains, datatype=3, **sk)
+        self.np = sst.np
+        self.max_cells, self.max_chains = max_cells, max_chains
+        self._g = grid.c()
+        h = ctypes.c_void_p()
+        _check(lib().mct_joint_create(ctypes.byref(self._g), ctypes.byref(bst), ctypes.byref(sst),
+                                      ctypes.byref(h)), "mct_joint_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mct_joint_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def likelihood(self, points, params, locations, bnoise0=None, bnoise1=None, snoise0=None, snoise1=None):
+        """One chain.  Returns (status, (like, misfit, unweighted), body triple,
+        surface triple)."""
+        pts, prm, loc = _f64(points), _f64(params), _f64(locations)
+        out = np.zeros(3)
+        parts = np.zeros(6)
+        z2, zp = np.zeros(2), np.zeros(self.np)
+        rc = lib().mct_joint_likelihood(self._h, _p(pts), _p(prm), pts.shape[0], _p(loc),
+                                        _p(z2 if bnoise0 is None else _f64(bnoise0)),
+                                        _p(z2 if bnoise1 is None else _f64(bnoise1)),
+                                        _p(zp if snoise0 is None else _f64(snoise0)),
+                                        _p(zp if snoise1 is None else _f64(snoise1)), _p(out), _p(parts))
+        if rc < 0:
+            _check(rc, "mct_joint_likelihood")
+        return rc, out, parts[:3], parts[3:]
+
+    def likelihood_batch_dev(self, nchain, points_ptr, params_ptr, ncells_ptr, locations_ptr, bnoise_ptr,
+                             snoise_ptr, out_ptr, status_ptr):
+        _check(lib().mct_joint_likelihood_batch_dev(self._h, int(nchain), ctypes.c_void_p(points_ptr),
+                                                    ctypes.c_void_p(params_ptr), ctypes.c_void_p(ncells_ptr),
+                                                    ctypes.c_void_p(locations_ptr), ctypes.c_void_p(bnoise_ptr),
+                                                    ctypes.c_void_p(snoise_ptr), ctypes.c_void_p(out_ptr),
+                                                    ctypes.c_void_p(status_ptr)), "mct_joint_likelihood_batch_dev")
+
+    def sync(self):
+        _check(lib().mct_joint_sync(self._h), "mct_joint_sync")
 
 
 class SurfServer:

@@ -300,7 +300,9 @@ int mct_body_create(const mct_grid *grid, const mct_body_setup *setup, mct_ctx *
     if (grid->nx < 2 || grid->ny < 2 || grid->nz < 2 || setup->nsrc < 1 || setup->nrev < 1 ||
         setup->max_cells < 1 || setup->max_chains < 1 || !setup->rev || !setup->ttime || !setup->raystat)
         return MCT_ERR_ARG;
-    if (setup->datatype != 0 && setup->datatype != 1) return MCT_ERR_ARG;
+    // datatype 3 (the body half of the joint likelihood) traces P and S as 1
+    // does (src/likelihood_body.F90:204-216, the branch for datatype != 0)
+    if (setup->datatype != 0 && setup->datatype != 1 && setup->datatype != 3) return MCT_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MCT_ERR_HIP;
     mct_ctx *C = new (std::nothrow) mct_ctx();
@@ -453,6 +455,41 @@ int mct_body_noise_likelihood(mct_ctx *C, const double *locations, const double 
     return *C->h_status;
 }
 
+int mct_body_noise_likelihood_from(mct_ctx *C, const double *phase_time, const double *locations,
+                                   const double *bnoise0, const double *bnoise1, double *out, double *sigma)
+{
+    if (!C || !phase_time || !bnoise0 || !bnoise1 || !out) return MCT_ERR_ARG;
+    hipStream_t s = C->stream;
+    const int np = C->nwave;
+    const long npt = (long)np * C->S.nsrc * C->S.nrev;
+    MCT_CHECK(hipStreamSynchronize(s));
+    double *hl = C->h_in, *hn = hl + 4 * (size_t)C->S.nsrc;
+    for (int j = 0; j < 4 * C->S.nsrc; j++) hl[j] = locations ? locations[j] : 0.0;
+    for (int i = 0; i < np; i++) { hn[i] = bnoise0[i]; hn[np + i] = bnoise1[i]; }
+    MCT_CHECK(hipMemcpyAsync(C->locations.p, hl, sizeof(double) * 4 * C->S.nsrc, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemcpyAsync(C->noise.p, hn, sizeof(double) * 2 * np, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemcpyAsync(C->picks.p, phase_time, sizeof(double) * npt, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemsetAsync(C->status.p, 0, sizeof(int32_t), s));
+    mct_misfit_args ma;
+    memset(&ma, 0, sizeof ma);
+    ma.nchain = 1; ma.nsrc = C->S.nsrc; ma.nrev = C->S.nrev; ma.np = np;
+    ma.reciprocal = 0;                 // phaseTime(nrev,nsrc,np) as the caller holds it
+    ma.picks = P<double>(C->picks); ma.ttime = P<double>(C->ttime); ma.raystat = P<int32_t>(C->raystat);
+    ma.locations = P<double>(C->locations); ma.noise = P<double>(C->noise); ma.sigdep = 1;
+    ma.nrays = C->S.nrays; ma.eps = -DBL_MAX;
+    ma.out = P<double>(C->out); ma.status = P<int32_t>(C->status);
+    ma.sigma = sigma ? P<double>(C->sigma) : nullptr;
+    int rc = mct_launch_misfit(ma, s);
+    if (rc) return rc;
+    MCT_CHECK(hipMemcpyAsync(C->h_out, C->out.p, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+    MCT_CHECK(hipMemcpyAsync(C->h_status, C->status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (sigma)
+        MCT_CHECK(hipMemcpyAsync(sigma, C->sigma.p, sizeof(double) * npt, hipMemcpyDeviceToHost, s));
+    MCT_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < 3; i++) out[i] = C->h_out[i];
+    return *C->h_status;
+}
+
 int mct_body_noise_likelihood_batch_dev(mct_ctx *C, int nchain, const double *locations_dev,
                                         const double *noise_dev, double *out_dev, int32_t *status_dev)
 {
@@ -525,6 +562,13 @@ int mct_copy_d2h(void *dst_host, const void *src_dev, int64_t bytes)
 {
     if (bytes < 0 || (bytes > 0 && (!dst_host || !src_dev))) return MCT_ERR_ARG;
     if (bytes) MCT_CHECK(hipMemcpy(dst_host, src_dev, (size_t)bytes, hipMemcpyDeviceToHost));
+    return MCT_OK;
+}
+
+int mct_body_stream(mct_ctx *C, void **stream)
+{
+    if (!C || !stream) return MCT_ERR_ARG;
+    *stream = (void *)C->stream;
     return MCT_OK;
 }
 

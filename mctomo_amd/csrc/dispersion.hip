@@ -32,6 +32,23 @@
 #define DSP_NMAX 100           // NMAX layers per column (src/likelihood_surf.F90:30)
 #define DSP_MAXP 60            // NP periods (surfmodes/surfdisp96.f:59)
 
+// vp and rho of the layer that starts at node k0 (0-based) of a column whose
+// Vs there is vsk: the Voronoi sites' own values (the surface half of the
+// joint likelihood, datatype 3: src/likelihood.f90:83-88 reads model%vp and
+// model%rho as kdtree_to_grid set them) or vs2vp_3d / vp2rho_3d of Vs
+// (datatype 2: src/likelihood.f90:75-76, src/utils.f90:102-134).
+__device__ __forceinline__ void dsp_vprho(const mct_dsp_args &A, long cbase, int k0, double vsk, double &vp,
+                                          double &rho)
+{
+    if (A.vp) {
+        vp = A.vp[cbase + k0];
+        rho = A.rho[cbase + k0];
+        return;
+    }
+    vp = vsk * (double)1.730f;   // POISSON, a default-real literal
+    rho = (double)1.74f * pow(vp, 0.25);
+}
+
 // A column's layer model; ST is the element stride of the layer arrays (1 in
 // LDS, 64 when lanes interleave their models).
 template <int ST> struct dsp_model_t {
@@ -408,41 +425,36 @@ __global__ __launch_bounds__(64) void k_dispersion(mct_dsp_args A)
     __shared__ float sd[DSP_NMAX], sa[DSP_NMAX], sb[DSP_NMAX], sr[DSP_NMAX];
     __shared__ int s_nl;
     const int col = blockIdx.x, c = blockIdx.y, lane = threadIdx.x;
-    const double *vs = A.vs + ((long)c * A.ncol + col) * A.nz;
-    const double POISSON = (double)1.730f;     // src/utils.f90:97-112 (vs2vp_3d)
+    const long cb = ((long)c * A.ncol + col) * A.nz;
+    const double *vs = A.vs + cb;
     const double EPS = (double)1.0e-10f;       // src/likelihood_surf.F90:37
     // check_model (src/likelihood_surf.F90:631-646): a deeper vs below the top one
     bool below = false;
     for (int k = 1 + lane; k < A.nz; k += 64) below |= vs[k] < vs[0];
     if (__ballot(below) != 0ull && lane == 0) A.bad_model[c] = 1;
-    // convert_to_layer (src/likelihood_surf.F90:523-629), no water layer; vp and
-    // rho from vs2vp_3d / vp2rho_3d (src/likelihood.f90:75-76)
+    // convert_to_layer (src/likelihood_surf.F90:523-629), no water layer
     if (lane == 0) {
         int nl = 0, last_k = 1;
         double last_vs = vs[0];
         for (int k = 2; k <= A.nz; k++) {
             if (fabs(vs[k - 1] - last_vs) > EPS) {
-                double vpk = last_vs * POISSON;
+                double vpk, rhok;
+                dsp_vprho(A, cb, last_k - 1, last_vs, vpk, rhok);
                 sa[nl] = (float)vpk;
                 sb[nl] = (float)last_vs;
-                sr[nl] = (float)((double)1.74f * pow(vpk, 0.25));
+                sr[nl] = (float)rhok;
                 sd[nl] = (float)(((double)(k - last_k) * A.dz) / A.scaling);
                 nl++;
                 last_vs = vs[k - 1];
                 last_k = k;
             }
         }
-        if (last_k != A.nz) {
-            double vpk = vs[A.nz - 1] * POISSON;
+        {   // the half-space: the bottom node's values, thickness 0 (either branch of :575-602)
+            double vpk, rhok;
+            dsp_vprho(A, cb, A.nz - 1, vs[A.nz - 1], vpk, rhok);
             sa[nl] = (float)vpk;
             sb[nl] = (float)vs[A.nz - 1];
-            sr[nl] = (float)((double)1.74f * pow(vpk, 0.25));
-            sd[nl] = 0.0f;
-        } else {
-            double vpk = vs[A.nz - 1] * POISSON;
-            sa[nl] = (float)vpk;
-            sb[nl] = (float)vs[A.nz - 1];
-            sr[nl] = (float)((double)1.74f * pow(vpk, 0.25));
+            sr[nl] = (float)rhok;
             sd[nl] = 0.0f;
         }
         s_nl = nl + 1;
@@ -587,27 +599,28 @@ __global__ __launch_bounds__(64) void k_dispersion_lane(mct_dsp_args A, float *l
     const long col = (long)blockIdx.x * 64 + lane;
     const int c = blockIdx.y;
     if (col >= A.ncol) return;
-    const double *vs = A.vs + ((long)c * A.ncol + col) * A.nz;
+    const long cb = ((long)c * A.ncol + col) * A.nz;
+    const double *vs = A.vs + cb;
     const long plane = (long)nlmax * 64;
     float *base = lay + ((long)c * gridDim.x + blockIdx.x) * 4 * plane + lane;
     float *sd = base, *sa = base + plane, *sb = base + 2 * plane, *sr = base + 3 * plane;
-    const double POISSON = (double)1.730f;     // src/utils.f90:97-112 (vs2vp_3d)
     const double EPS = (double)1.0e-10f;       // src/likelihood_surf.F90:37
     // check_model (src/likelihood_surf.F90:631-646)
     bool below = false;
     const double top = vs[0];
     for (int k = 1; k < A.nz; k++) below |= vs[k] < top;
     if (below) A.bad_model[c] = 1;
-    // convert_to_layer (src/likelihood_surf.F90:523-629), vp and rho as vs2vp_3d / vp2rho_3d
+    // convert_to_layer (src/likelihood_surf.F90:523-629)
     int nl = 0, last_k = 1;
     double last_vs = vs[0];
     for (int k = 2; k <= A.nz; k++) {
         double vk = vs[k - 1];
         if (fabs(vk - last_vs) > EPS) {
-            double vpk = last_vs * POISSON;
+            double vpk, rhok;
+            dsp_vprho(A, cb, last_k - 1, last_vs, vpk, rhok);
             sa[64 * nl] = (float)vpk;
             sb[64 * nl] = (float)last_vs;
-            sr[64 * nl] = (float)((double)1.74f * pow(vpk, 0.25));
+            sr[64 * nl] = (float)rhok;
             sd[64 * nl] = (float)(((double)(k - last_k) * A.dz) / A.scaling);
             nl++;
             last_vs = vk;
@@ -615,10 +628,11 @@ __global__ __launch_bounds__(64) void k_dispersion_lane(mct_dsp_args A, float *l
         }
     }
     {
-        double vpk = vs[A.nz - 1] * POISSON;
+        double vpk, rhok;
+        dsp_vprho(A, cb, A.nz - 1, vs[A.nz - 1], vpk, rhok);
         sa[64 * nl] = (float)vpk;
         sb[64 * nl] = (float)vs[A.nz - 1];
-        sr[64 * nl] = (float)((double)1.74f * pow(vpk, 0.25));
+        sr[64 * nl] = (float)rhok;
         sd[64 * nl] = 0.0f;
     }
     dsp_model_t<64> M;
@@ -931,7 +945,6 @@ __global__ __launch_bounds__(64) void k_dispersion_sm(mct_dsp_args A, float *lay
     const long plane = (long)nlmax * 64;
     float *base = lay + (long)blockIdx.x * 4 * plane + lane;
     float *sd = base, *sa = base + plane, *sb = base + 2 * plane, *sr = base + 3 * plane;
-    const double POISSON = (double)1.730f;     // src/utils.f90:97-112 (vs2vp_3d)
     const double EPS = (double)1.0e-10f;       // src/likelihood_surf.F90:37
     const double twopi = 2.0 * 3.141592653589793;
     const float sone = 1.500f, h = 0.005f;
@@ -976,7 +989,8 @@ __global__ __launch_bounds__(64) void k_dispersion_sm(mct_dsp_args A, float *lay
                 if (A.jobs) job = A.jobs[job];
                 c = (int)(job / A.ncol);
                 col = job - (long)c * A.ncol;
-                const double *vs = A.vs + ((long)c * A.ncol + col) * A.nz;
+                const long cb = ((long)c * A.ncol + col) * A.nz;
+                const double *vs = A.vs + cb;
                 // check_model (src/likelihood_surf.F90:631-646)
                 bool below = false;
                 const double top = vs[0];
@@ -988,10 +1002,11 @@ __global__ __launch_bounds__(64) void k_dispersion_sm(mct_dsp_args A, float *lay
                 for (int kk = 2; kk <= A.nz; kk++) {
                     double vk = vs[kk - 1];
                     if (fabs(vk - last_vs) > EPS) {
-                        double vpk = last_vs * POISSON;
+                        double vpk, rhok;
+                        dsp_vprho(A, cb, last_k - 1, last_vs, vpk, rhok);
                         sa[64 * nl] = (float)vpk;
                         sb[64 * nl] = (float)last_vs;
-                        sr[64 * nl] = (float)((double)1.74f * pow(vpk, 0.25));
+                        sr[64 * nl] = (float)rhok;
                         sd[64 * nl] = (float)(((double)(kk - last_k) * A.dz) / A.scaling);
                         nl++;
                         last_vs = vk;
@@ -999,10 +1014,11 @@ __global__ __launch_bounds__(64) void k_dispersion_sm(mct_dsp_args A, float *lay
                     }
                 }
                 {
-                    double vpk = vs[A.nz - 1] * POISSON;
+                    double vpk, rhok;
+                    dsp_vprho(A, cb, A.nz - 1, vs[A.nz - 1], vpk, rhok);
                     sa[64 * nl] = (float)vpk;
                     sb[64 * nl] = (float)vs[A.nz - 1];
-                    sr[64 * nl] = (float)((double)1.74f * pow(vpk, 0.25));
+                    sr[64 * nl] = (float)rhok;
                     sd[64 * nl] = 0.0f;
                 }
                 M.mmax = nl + 1;

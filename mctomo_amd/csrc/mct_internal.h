@@ -256,6 +256,9 @@ struct mct_dsp_args {
     const int32_t *box_off;       // (nchain + 1): first compact index of each chain's box
     long box_total;               // box columns over all chains
     int ny;                       // columns are (ix-1)*ny + (iy-1)
+    // joint likelihood (datatype 3): the sites' vp and rho per node, laid out
+    // as vs; NULL derives them from vs (datatype 2)
+    const double *vp, *rho;
 };
 // One wave per (column, chain): check_model, convert_to_layer, surfdisp96.
 // Requires np <= 60 and nz <= 100.

@@ -128,6 +128,8 @@ struct mct_surf_ctx {
     sbuf phase_keep, box;
     // group velocities: ray lengths (srdist) per chain and the accepted group velocities
     sbuf rlen, group_keep;
+    // joint likelihood (datatype 3): the sites' vp and rho on the grid
+    sbuf vpg, rhog;
     int32_t *h_box = nullptr;        // pinned: (nchain, 4) boxes then (nchain + 1) offsets
     std::vector<char> keep_valid;
     double *h_in = nullptr, *h_out = nullptr;
@@ -165,6 +167,11 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     na.points = d_points; na.params = d_params; na.ncells = d_ncells; na.ncells_const = ncells_const;
     na.ix0 = 1; na.ix1 = G.nx; na.iy0 = 1; na.iy1 = G.ny; na.iz0 = 1; na.iz1 = G.nz;
     na.vs = vs; na.nwave = 1;
+    const bool joint = S.datatype == 3;
+    if (joint) {   // the sites' own vp and rho (src/likelihood.f90:83-88)
+        na.vp = C->vpg.as<double>() + c0 * gsz;
+        na.rho = C->rhog.as<double>() + c0 * gsz;
+    }
     na.flag_count = C->flag_count.as<int32_t>() + c0;
     na.flag_nodes = C->flag_nodes.as<int32_t>() + (long)c0 * MCT_MAX_FLAGGED;
     na.kd_nodes = C->kd_nodes.as<mct_kd_node>() + (long)c0 * (2 * NC + 2);
@@ -184,6 +191,7 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     da.bad_model = C->bad_model.as<int32_t>() + c0;
     da.ndltar = C->ndltar.as<int32_t>() + (long)c0 * ncol;
     da.nlayer = C->nlayer.as<int32_t>() + (long)c0 * ncol;
+    if (joint) { da.vp = na.vp; da.rho = na.rho; }
     // many columns: one lane per column from a column queue (throughput); few:
     // one wave per column (latency)
     int dmode = C->dsp_mode;
@@ -209,7 +217,9 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
             da.jobs = C->dsp_jobs.as<int32_t>() + (long)c0 * ncol;
             da.keys = C->dsp_keys.as<int32_t>() + (long)c0 * ncol;
             da.hist = C->dsp_hist.as<int32_t>() + 128 * part;
-            if (C->dsp_dedup) {
+            // identical-column detection hashes the Vs profile, which decides the
+            // column's layers only when vp and rho follow from Vs
+            if (C->dsp_dedup && !joint) {
                 da.hlog = C->dsp_hlog;
                 da.hkey = C->dsp_hkey.as<unsigned long long>() + ((long)c0 << C->dsp_hlog);
                 da.hrep = C->dsp_hrep.as<int32_t>() + ((long)c0 << C->dsp_hlog);
@@ -338,6 +348,7 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     // group times trace rays from the refined source grid (rpaths reads it)
     if (S.phase_group != 0 && S.phase_group != 1) return MCT_ERR_ARG;
     if (S.straight != 0 && S.straight != 1) return MCT_ERR_ARG;
+    if (S.datatype != 0 && S.datatype != 2 && S.datatype != 3) return MCT_ERR_ARG;
     if (S.phase_group == 1 && S.sgref != 1 && !S.straight) return MCT_ERR_ARG;
     if (S.phase_group == 1 && !S.straight) {
         // rpaths stores pair n's ray at slot raystat(n,2) and CalGroupTime reads
@@ -432,6 +443,10 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     ALLOC(out, sizeof(double) * 3 * M);
     ALLOC(status, sizeof(int32_t) * M);
     ALLOC(vs, sizeof(double) * gsz * M);
+    if (S.datatype == 3) {
+        ALLOC(vpg, sizeof(double) * gsz * M);
+        ALLOC(rhog, sizeof(double) * gsz * M);
+    }
     ALLOC(flag_count, sizeof(int32_t) * M);
     ALLOC(flag_nodes, sizeof(int32_t) * MCT_MAX_FLAGGED * M);
     ALLOC(kd_nodes, sizeof(mct_kd_node) * (2 * NC + 2) * M);
@@ -532,7 +547,7 @@ static std::vector<sbuf *> surf_bufs(mct_surf_ctx *C)
             &C->dsp_keys, &C->dsp_hist, &C->velv, &C->veln, &C->phase_time, &C->srdist, &C->sigma,
             &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap, &C->pflag, &C->rounds,
             &C->phase_keep, &C->box, &C->rlen, &C->group_keep, &C->dsp_hkey, &C->dsp_hrep,
-            &C->dsp_dup, &C->dsp_njobs};
+            &C->dsp_dup, &C->dsp_njobs, &C->vpg, &C->rhog};
 }
 
 extern "C" {
@@ -847,6 +862,44 @@ int mct_surf_noise_likelihood(mct_surf_ctx *C, const double *snoise0, const doub
     if (rc) return rc;
     MCT_CHECK(hipMemcpyAsync(C->h_out, C->out.p, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
     MCT_CHECK(hipMemcpyAsync(C->h_status, C->status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    MCT_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < 3; i++) out[i] = C->h_out[i];
+    return *C->h_status;
+}
+
+int mct_surf_noise_likelihood_from(mct_surf_ctx *C, const double *phase_time, const double *srdist,
+                                   const double *snoise0, const double *snoise1, double *out, double *sigma)
+{
+    if (!C || !phase_time || !snoise0 || !snoise1 || !out) return MCT_ERR_ARG;
+    const bool dist = C->S.phase_group || C->S.straight;
+    if (dist && !srdist) return MCT_ERR_ARG;
+    hipStream_t s = C->stream;
+    const int np = C->S.np;
+    const long npt = (long)np * C->S.nsrc * C->S.nrev;
+    MCT_CHECK(hipStreamSynchronize(s));
+    double *hn = C->h_in;
+    for (int i = 0; i < np; i++) { hn[i] = snoise0[i]; hn[np + i] = snoise1[i]; }
+    MCT_CHECK(hipMemcpyAsync(C->noise.p, hn, sizeof(double) * 2 * np, hipMemcpyHostToDevice, s));
+    // the given traveltimes (and ray lengths) become chain 0's kept state
+    MCT_CHECK(hipMemcpyAsync(C->phase_time.p, phase_time, sizeof(double) * npt, hipMemcpyHostToDevice, s));
+    if (dist) MCT_CHECK(hipMemcpyAsync(C->rlen.p, srdist, sizeof(double) * npt, hipMemcpyHostToDevice, s));
+    MCT_CHECK(hipMemsetAsync(C->status.p, 0, sizeof(int32_t), s));
+    const mct_surf_setup &S = C->S;
+    mct_misfit_args ma;
+    memset(&ma, 0, sizeof ma);
+    ma.nchain = 1; ma.nsrc = S.nsrc; ma.nrev = S.nrev; ma.np = np; ma.reciprocal = 0;
+    ma.picks = C->phase_time.as<double>(); ma.ttime = C->ttime.as<double>();
+    ma.raystat = C->raystat.as<int32_t>(); ma.noise = C->noise.as<double>();
+    ma.sigdep = 1; ma.nrays = S.nrays;
+    ma.eps = (double)1.0e-10f;   // EPS, src/likelihood_surf.F90:37
+    ma.out = C->out.as<double>(); ma.status = C->status.as<int32_t>();
+    ma.dist = dist ? C->rlen.as<double>() : nullptr;
+    ma.sigma = sigma ? C->sigma.as<double>() : nullptr;
+    int rc = mct_launch_misfit(ma, s);
+    if (rc) return rc;
+    MCT_CHECK(hipMemcpyAsync(C->h_out, C->out.p, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+    MCT_CHECK(hipMemcpyAsync(C->h_status, C->status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (sigma) MCT_CHECK(hipMemcpyAsync(sigma, C->sigma.p, sizeof(double) * npt, hipMemcpyDeviceToHost, s));
     MCT_CHECK(hipStreamSynchronize(s));
     for (int i = 0; i < 3; i++) out[i] = C->h_out[i];
     return *C->h_status;

@@ -171,3 +171,89 @@ def surface_201(seed=20261206, order=1, ncells=200, np_=20, nsrc=16, nrev=16):
     at 0.5 km, Rayleigh phase velocity at np_ periods 1-10 s."""
     return surface_workload(201, 61, 100.0, 30.0, ncells, np_, (1.0, 10.0), nsrc, nrev, seed=seed,
                             order=order)
+
+
+@dataclass
+class JointWorkload:
+    grid: Grid
+    points: np.ndarray      # (ncells, 3)   RTI%points
+    params: np.ndarray      # (ncells, 3)   (vp, vs, rho), each the site's own (datatype 3)
+    locations: np.ndarray   # (nsrc, 4)     RTI%locations (x, y, z, t0) of the body-wave sources
+    body: dict              # BodyLikelihood-style data: rev (nrev,3), ttime (2,3,npair), raystat, nsrc
+    surf: dict              # SurfLikelihood-style data: freqs, src, rev, ttime, raystat
+    rejected_draws: int = 0
+
+
+def _joint_sites(rng, ncells, L, Z, vs_range, pert):
+    """Sites uniform in the box; Vs a linear depth gradient over vs_range plus a
+    uniform perturbation of +-pert, clipped to vs_range and ranked by site depth
+    (slower sites shallower); vp = Vs times U[1.65, 1.85]; rho = 0.32 vp + 0.77
+    plus U[-0.05, 0.05] g/cc: three independent site parameters."""
+    pts = np.column_stack([rng.uniform(0.0, L, ncells), rng.uniform(0.0, L, ncells),
+                           rng.uniform(0.0, Z, ncells)])
+    vs = vs_range[0] + (vs_range[1] - vs_range[0]) * pts[:, 2] / Z + rng.uniform(-pert, pert, ncells)
+    vs = np.clip(vs, *vs_range)
+    vs = np.sort(vs)[np.argsort(np.argsort(pts[:, 2]))]
+    vp = vs * rng.uniform(1.65, 1.85, ncells)
+    rho = 0.32 * vp + 0.77 + rng.uniform(-0.05, 0.05, ncells)
+    return pts, np.column_stack([vp, vs, rho])
+
+
+def joint_workload(n=41, nz=21, L=20.0, Z=10.0, ncells=40, nsrc=6, nrev=48, np_=6, periods=(1.0, 5.0),
+                   nsta=8, vs_range=(1.5, 4.8), pert=0.3, sigma=0.05, order=0, seed=20261205,
+                   check=True):
+    """Joint body + surface-wave inputs (datatype 3): ncells sites in an L x L x
+    Z km box on n x n x nz nodes; nsrc body-wave sources in the volume (depth
+    10-90% of Z) and nrev receivers at the surface, P and S; surface-wave
+    station pairs on nsta of those receivers (sources = receivers = the
+    stations, as the reference's example surface data) at np_ periods
+    log-spaced over `periods`.  Observed times: straight rays at 6 / 3.5 km/s
+    (body) and 3 km/s (surface) plus Gaussian noise.  check: redraw site sets
+    that fail check_model (brute-force nearest site on the host)."""
+    rng = np.random.default_rng(seed)
+    h, hz = L / (n - 1), Z / (nz - 1)
+    grid = Grid(n, n, nz, 0.0, 0.0, 0.0, h, h, hz, 1.0)
+    rejected = 0
+    while True:
+        points, params = _joint_sites(rng, ncells, L, Z, vs_range, pert)
+        if not check or top_is_slowest(grid, points, params):
+            break
+        rejected += 1
+        if rejected > 100:
+            raise RuntimeError("no site set passed check_model")
+    loc = np.zeros((nsrc, 4))
+    loc[:, 0:2] = rng.uniform(h, L - h, (nsrc, 2))
+    loc[:, 2] = rng.uniform(0.1 * Z, 0.9 * Z, nsrc)
+    rev = np.zeros((nrev, 3))
+    rev[:, 0:2] = rng.uniform(h, L - h, (nrev, 2))
+    d = np.linalg.norm(loc[:, None, 0:3] - rev[None, :, :], axis=2).reshape(-1)
+    npair = nsrc * nrev
+    btt = np.zeros((2, 3, npair))
+    for i, v in enumerate((6.0, 3.5)):
+        btt[i, 0] = d / v + rng.normal(0.0, sigma, npair)
+    btt[:, 1] = sigma
+    brs = np.ones((2, 2, npair), np.int32)
+    sta = rev[:nsta, 0:2].copy()
+    T = np.geomspace(periods[0], periods[1], np_)
+    sp = nsta * nsta
+    ds = np.linalg.norm(sta[:, None, :] - sta[None, :, :], axis=2).reshape(-1)
+    stt = np.zeros((np_, 3, sp))
+    srs = np.zeros((np_, 2, sp), np.int32)
+    used = ds > 0.0                         # a station is not paired with itself
+    for i in range(np_):
+        stt[i, 0] = ds / 3.0 + rng.normal(0.0, sigma, sp)
+        srs[i, 0] = used
+        srs[i, 1] = np.where(used, np.arange(1, sp + 1), 0)
+    stt[:, 1] = sigma
+    body = dict(rev=rev, ttime=btt, raystat=brs, nsrc=nsrc, order=order)
+    surf = dict(freqs=1.0 / T, src=sta, rev=sta.copy(), ttime=stt, raystat=srs, order=order)
+    return JointWorkload(grid, points, params, loc, body, surf, rejected)
+
+
+def c5_joint(seed=20261205, order=0, check=False):
+    """The joint config: 201x201x81 (100 x 100 x 40 km at 0.5 km), 600 sites, Vs
+    1.5-4.8 km/s with a linear depth gradient plus +-0.3 km/s, 400 sources in
+    the volume and 2000 surface receivers (P and S), surface waves at 30
+    periods 0.2-8 s on 64 of the receivers as stations (64 x 63 pairs)."""
+    return joint_workload(201, 81, 100.0, 40.0, 600, 400, 2000, 30, (0.2, 8.0), 64, order=order, seed=seed,
+                          check=check)

@@ -143,9 +143,10 @@ double orc_interpolate3d(int nx, int ny, int nz, double xmin, double ymin, doubl
  * (np,nsrc,nrev), srdist and sigma (np,nsrc*nrev), phase_vel (nx,ny,np) (may
  * be NULL), sites_id (nx,ny,nz) (may be NULL).  phase_group 1 computes group
  * velocities and times: group delays along the rays (CalGroupTime), srdist =
- * ray lengths; group_vel (nx,ny,np) may be NULL.  Returns 0, 1 (model
- * rejected: check_model or a dispersion failure), 2 (a crazy ray), or < 0
- * (fatal). */
+ * ray lengths; group_vel (nx,ny,np) may be NULL.  datatype 2: vp and rho of
+ * the layers from vs2vp_3d / vp2rho_3d; 3 (the surface half of the joint
+ * likelihood): the sites' own vp and rho.  Returns 0, 1 (model rejected:
+ * check_model or a dispersion failure), 2 (a crazy ray), or < 0 (fatal). */
 int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double zmin,
                         double dx, double dy, double dz, double scaling,
                         const double *points, const double *params, int ncells,
@@ -157,7 +158,7 @@ int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double
                         double band, int nthreads,
                         double *out, double *phase_time, double *srdist, double *sigma,
                         double *phase_vel, int32_t *sites_id, int phase_group,
-                        double *group_vel, int straight);
+                        double *group_vel, int straight, int datatype);
 
 #ifdef __cplusplus
 }

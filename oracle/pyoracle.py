@@ -335,11 +335,12 @@ def ref_surf_forward(grid, points, params, freqs, src, rev, raystat, raylov=1, d
 def surf_likelihood(grid, points, params, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3,
                     sigdep=0, snoise0=None, snoise1=None, nrays=None, gridx=1, gridy=1, sgref=1,
                     sgdic=4, sgext=8, order=1, band=0.5, nthreads=0, phase_group=0, group_vel=None,
-                    straight=0):
+                    straight=0, datatype=2):
     """orc_surf_likelihood.  grid: dict nx..dz (+ scaling).  Returns (rc, out(3),
     phase_time (np,nsrc,nrev), srdist, sigma (np,npair), phase_vel (nx,ny,np),
     sites_id (nx,ny,nz)).  phase_group=1: group times along rays; group_vel, an
-    (nx,ny,np) array, receives the group velocities."""
+    (nx,ny,np) array, receives the group velocities.  datatype 3: the layers take
+    the sites' own vp and rho (the surface half of the joint likelihood)."""
     nx, ny, nz = grid["nx"], grid["ny"], grid["nz"]
     pts, prm = c64(points), c64(params)
     fr = c64(freqs).reshape(-1)
@@ -370,5 +371,5 @@ def surf_likelihood(grid, points, params, freqs, src, rev, ttime, raystat, raylo
         _p(s0), _p(s1), ctypes.c_int(nrays), ctypes.c_int(gridx), ctypes.c_int(gridy),
         ctypes.c_int(sgref), ctypes.c_int(sgdic), ctypes.c_int(sgext), ctypes.c_int(order),
         ctypes.c_double(band), ctypes.c_int(nthreads), _p(out), _p(pt), _p(sd), _p(sg), _p(pv), _p(sid),
-        ctypes.c_int(phase_group), _p(group_vel), ctypes.c_int(straight))
+        ctypes.c_int(phase_group), _p(group_vel), ctypes.c_int(straight), ctypes.c_int(datatype))
     return rc, out, pt, sd, sg, pv, sid

@@ -1,10 +1,11 @@
 /*
- * surf.c -- oracle for the surface-wave likelihood (datatype 2, phase
- * velocities).  TEST INFRASTRUCTURE ONLY (see mct_oracle.h).
+ * surf.c -- oracle for the surface-wave likelihood (datatype 2, and the
+ * surface half of datatype 3).  TEST INFRASTRUCTURE ONLY (see mct_oracle.h).
  *
  * Composes the restated stages in the order of likelihood (src/likelihood.f90
- * :72-80) and surf_likelihood (src/likelihood_surf.F90:110-405):
- *   kdtree_to_grid (nearest.c) -> vs2vp_3d / vp2rho_3d (src/utils.f90:102-134)
+ * :72-88) and surf_likelihood (src/likelihood_surf.F90:110-405):
+ *   kdtree_to_grid (nearest.c) -> vs2vp_3d / vp2rho_3d (src/utils.f90:102-134;
+ *   datatype 2 only, datatype 3 keeps the sites' vp and rho)
  *   -> check_model (:631-646) -> convert_to_layer (:523-629) -> surfmodes per
  *   column (surfdisp.c) -> ierr reject (:212-224) -> like%vel padding
  *   (:259-264) -> modrays per period (fm2d.c) -> srdist = phaseTime (:329-330)
@@ -27,29 +28,41 @@
 #define SURF_PI2 ((double)6.283185f)     /* :36 */
 #define SURF_NMAX 100                    /* :30 */
 
-/* convert_to_layer for one column (no water layer, no vbnd): vs(nz) -> layers.
- * vp and rho follow vs2vp_3d / vp2rho_3d.  Returns the layer count. */
-static int column_layers(const double *vs, int nz, double dz, double scaling,
-                         double *thick, double *vp, double *vsl, double *rho)
+/* vp and rho at node k of a column: the model's own (gvp/grho non-NULL, the
+ * joint likelihood, datatype 3: src/likelihood.f90:83-88) or vs2vp_3d /
+ * vp2rho_3d of vs (datatype 2, src/likelihood.f90:75-76, src/utils.f90:102-134) */
+static void node_vprho(const double *vs, const double *gvp, const double *grho, int k, double *vp, double *rho)
 {
-    const double POISSON = (double)1.730f;
+    if (gvp) {
+        *vp = gvp[k];
+        *rho = grho[k];
+        return;
+    }
+    *vp = vs[k] * (double)1.730f;   /* POISSON */
+    *rho = (double)1.74f * pow(*vp, 0.25);
+}
+
+/* convert_to_layer for one column (no water layer, no vbnd): a layer starts
+ * where vs changes by more than EPS and takes the values of its first node;
+ * the half-space (thickness 0) takes the bottom node's.  Returns the layer
+ * count. */
+static int column_layers(const double *vs, const double *gvp, const double *grho, int nz, double dz,
+                         double scaling, double *thick, double *vp, double *vsl, double *rho)
+{
     int nl = 0, last_k = 1;
     double last_vs = vs[0];
     for (int k = 2; k <= nz; k++) {
         if (fabs(vs[k - 1] - last_vs) > SURF_EPS) {
             vsl[nl] = last_vs;
-            vp[nl] = last_vs * POISSON;
-            rho[nl] = (double)1.74f * pow(vp[nl], 0.25);
+            node_vprho(vs, gvp, grho, last_k - 1, &vp[nl], &rho[nl]);
             thick[nl] = ((double)(k - last_k) * dz) / scaling;
             nl++;
             last_vs = vs[k - 1];
             last_k = k;
         }
     }
-    /* the last layer is the half-space (thickness 0) with the bottom node's values */
     vsl[nl] = vs[nz - 1];
-    vp[nl] = vs[nz - 1] * POISSON;
-    rho[nl] = (double)1.74f * pow(vp[nl], 0.25);
+    node_vprho(vs, gvp, grho, nz - 1, &vp[nl], &rho[nl]);
     thick[nl] = 0.0;
     return nl + 1;
 }
@@ -61,7 +74,7 @@ int orc_column_layers_all(long ncol, int nz, const double *vs, double dz, double
     #pragma omp parallel for schedule(static) reduction(|:bad)
     for (long c = 0; c < ncol; c++) {
         double th[SURF_NMAX + 1], a[SURF_NMAX + 1], b[SURF_NMAX + 1], r[SURF_NMAX + 1];
-        int n = column_layers(vs + c * nz, nz, dz, scaling, th, a, b, r);
+        int n = column_layers(vs + c * nz, NULL, NULL, nz, dz, scaling, th, a, b, r);
         if (n > maxl) { bad = 1; continue; }
         nl[c] = n;
         for (int k = 0; k < n; k++) {
@@ -83,7 +96,7 @@ int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double
                         double band, int nthreads,
                         double *out, double *phase_time, double *srdist, double *sigma,
                         double *phase_vel, int32_t *sites_id, int phase_group, double *group_vel,
-                        int straight)
+                        int straight, int datatype)
 {
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -110,7 +123,8 @@ int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double
         #pragma omp parallel for schedule(dynamic, 16) reduction(|:bad)
         for (long c = 0; c < C; c++) {
             double th[SURF_NMAX + 1], a[SURF_NMAX + 1], b[SURF_NMAX + 1], r[SURF_NMAX + 1];
-            int nl = column_layers(vs + c * nz, nz, dz, scaling, th, a, b, r);
+            int nl = datatype == 3 ? column_layers(vs + c * nz, vp + c * nz, rho + c * nz, nz, dz, scaling, th, a, b, r)
+                                   : column_layers(vs + c * nz, NULL, NULL, nz, dz, scaling, th, a, b, r);
             int ierr = orc_surfmodes(th, a, b, r, nl, freqs, np, raylov, phase_group, dphase,
                                      pvel + c * np, gvel + c * np, NULL);
             if (ierr) bad = 1;
