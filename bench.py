@@ -37,6 +37,7 @@ sys.path.insert(0, ROOT)
 METRIC = "likelihood evals/sec, 1 GPU (batched/served vs plain) + p99 eval latency"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
 
 def ctypes_vp(p):
@@ -170,6 +171,35 @@ def run_surface(args):
     log("batched: B=%d %.1f ms/step %.2f evals/s  stages(ms) nearest %.2f dispersion %.2f "
         "2-D %.2f misfit %.2f" % (B, ms_step, batched_eps, *st_b))
     log("2-D stage: near-source %.2f ms, sweep %.2f ms" % like.timings2d())
+
+    # ---- the same batch with the mixed-order update (the reference examples'
+    # LIKE_SET%ORDER = 1): the parallel 2-D path checks each converged field
+    # against the heap loop's freeze order and hands the problems that fail to
+    # the sequential solver, so the times are the reference's
+    order1 = None
+    if args.order == 0 and args.order1_steps > 0:
+        like1 = mct.SurfLikelihood(w.grid, w.freqs, w.src, w.rev, w.ttime, w.raystat, raylov=1, order=1,
+                                   max_cells=cmax, max_chains=B)
+        dO1 = mct.DeviceArray((B, 3), np.float64)
+        dS1 = mct.DeviceArray((B,), np.int32)
+
+        def run1():
+            like1.likelihood_batch_dev(B, dP.ptr, dQ.ptr, dN.ptr, dS0.ptr, dO1.ptr, dS1.ptr)
+            like1.sync()
+        run1()
+        t0 = time.perf_counter()
+        for _ in range(args.order1_steps):
+            run1()
+        dt1 = time.perf_counter() - t0
+        st1 = dS1.numpy()
+        checked, handed = like1.fm2d_handover()
+        order1 = {"evals_per_s": round(B * args.order1_steps / dt1, 3), "steps": args.order1_steps,
+                  "ms_per_step": round(dt1 / args.order1_steps * 1e3, 3),
+                  "statuses": {str(k): int(v) for k, v in zip(*np.unique(st1, return_counts=True))},
+                  "stage_ms": [round(v, 3) for v in like1.timings()],
+                  "problems": int(B * w.freqs.shape[0] * w.nsrc),
+                  "handed_to_sequential": int(handed), "failed_heap_order_check": int(checked)}
+        log("order 1 batched:", order1)
 
     # ---- perturbed-box proposals (the secondary unit) ----------------------------
     # One proposal per chain: the Vs of the chain's deepest cell rises by 0.05
@@ -370,19 +400,22 @@ def run_surface(args):
 
     flops = 170.0 * slev0
     a_disp = flops / (st_b[1] * 1e-3) / 1e12
-    r_disp = {"bound": "mfma", "achieved": round(a_disp, 4), "peak": FP64_PEAK_TFLOPS,
+    r_disp = {"bound": "fp64", "achieved": round(a_disp, 4), "peak": FP64_PEAK_TFLOPS,
               "unit": "TFLOP/s", "frac": round(a_disp / FP64_PEAK_TFLOPS, 6),
               "traffic": traffic_of("k_dispersion_sm", n0),
               "kernel": "k_dispersion_sm", "launch_chains": int(n0), "launch_ms": round(st_b[1], 3),
-              "note": "FP64 vector arithmetic, no MFMA: the peak quoted is MI355X's FP64 figure "
-                      "(78.6 TFLOP/s, vector and matrix alike); flops = 170 x sum over the columns "
+              "note": "FP64 vector arithmetic (no MFMA; VALU-bound): the peak quoted is MI355X's FP64 "
+                      "figure (78.6 TFLOP/s, vector and matrix alike); flops = 170 x sum over the columns "
                       "solved of dltar evaluations x (layers-1) (the sequential root walk's count); "
                       "columns whose Vs profile repeats an earlier column of the chain are copied",
               "columns": int(n0 * C), "columns_solved": int(sc0),
               "dltar_evals": int(ev0), "layer_evals": int(lev0), "layer_evals_solved": int(slev0),
               "alg_bytes": int(n0 * (4 * G + 8 * C * npd))}
     solves = n0 * npd * nsrc
-    alg_sweep = solves * (16.0 * C + 8.0 * nrev)
+    # the survey's fixed definition for a 2-D solve: 4 (C + R) + 4 Nr bytes (FP32
+    # times over the coarse and refined nodes, and the picks); the strip kernel
+    # sweeps the coarse grid and picks, so its share is 4 C + 4 Nr
+    alg_sweep = solves * (4.0 * C + 4.0 * nrev)
     r_sweep = None
     if t_sweep > 0:
         a_sweep = alg_sweep / (t_sweep * 1e-3) / 1e9
@@ -391,9 +424,10 @@ def run_surface(args):
                    "kernel": "k_fm2d_strip", "launch_solves": int(solves), "launch_ms": round(t_sweep, 3),
                    "alg_bytes": int(alg_sweep),
                    "note": "Gauss-Seidel sweeps of the coarse grid, one wave per (chain, period, source); "
-                           "the traffic counts every sweep's re-read of the grid, the algorithmic bytes "
-                           "one read of the velocities and one write of the times; bound in practice by "
-                           "FP64 VALU issue and latency, not HBM"}
+                           "algorithmic bytes per solve 4 C + 4 Nr (the survey's 2-D definition without the "
+                           "refined nodes, which the near-source kernel handles); the traffic counts every "
+                           "sweep's re-read of the grid (the times and velocities are FP64 here); bound in "
+                           "practice by FP64 VALU issue and latency, not HBM"}
     roof, other = r_disp, r_sweep
     if r_sweep and t_sweep > st_b[1]:
         roof, other = r_sweep, r_disp
@@ -404,12 +438,35 @@ def run_surface(args):
     roof["fm2d_split_ms"] = {"near_source": round(t_near, 3), "sweep": round(t_sweep, 3)}
     if other:
         roof["next_kernel"] = other
+    # whole evaluation against the survey's per-stage roofline:
+    # t_roof = sum over stages of max(B_s / 8 TB/s, F_s / peak_s), per eval
+    S_ = int(np.mean(N))
+    R_ = (2 * 8 * 4 + 1) ** 2          # refined nodes, sgext 8, sgdic 4
+    stages = {
+        "nearest": (4.0 * G + 24.0 * S_, 8.0 * G * S_, FP64_PEAK_TFLOPS),
+        "dispersion": (4.0 * G + 8.0 * C * npd, 170.0 * levB / B, FP64_PEAK_TFLOPS),
+        "fm2d": (npd * nsrc * (4.0 * (C + R_) + 4.0 * nrev), npd * nsrc * 20.0 * (C + R_), FP32_PEAK_TFLOPS),
+        "misfit": (20.0 * nsrc * nrev * npd, 8.0 * nsrc * nrev * npd, FP64_PEAK_TFLOPS),
+    }
+    t_roof = sum(max(b_ / (HBM_PEAK_GBS * 1e9), f_ / (pk * 1e12)) for b_, f_, pk in stages.values())
+    t_eval = ms_step * 1e-3 / B
+    roof["whole_eval"] = {
+        "t_roof_ms": round(t_roof * 1e3, 4), "t_measured_ms": round(t_eval * 1e3, 4),
+        "frac": round(t_roof / t_eval, 6),
+        "hbm_only_frac": round(sum(v[0] for v in stages.values()) / t_eval / (HBM_PEAK_GBS * 1e9), 8),
+        "stage_bound_ms": {k: round(max(b_ / (HBM_PEAK_GBS * 1e9), f_ / (pk * 1e12)) * 1e3, 5)
+                           for k, (b_, f_, pk) in stages.items()},
+        "note": "per evaluation: nearest B = 4G + 24S, F = 8GS (FP64); dispersion B = 4G + 8CP, F = 170 x "
+                "dltar evaluations x (layers-1) of the sequential walk over every column (FP64); 2-D per "
+                "solve B = 4(C+R) + 4Nr, F = 20(C+R) (FP32), R = 65^2; misfit B = 20 Ns Nr P, F = 8 Ns Nr P; "
+                "S = the batch's mean cell count"}
 
     # ---- parity of this run vs the oracle, and the CPU baseline ------------------
     cpu = err = None
     if not args.no_cpu:
         from oracle import pyoracle as po
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        # every host core this process may run on (the GPU box's CPU share)
+        threads = len(os.sched_getaffinity(0))
         t0 = time.perf_counter()
         orc = po.surf_likelihood(gdict(w.grid), w.points, w.params, w.freqs, w.src, w.rev, w.ttime,
                                  w.raystat, order=args.order, nthreads=threads)
@@ -432,19 +489,37 @@ def run_surface(args):
         except OSError:
             ref_ok = False
         if ref_ok:
-            t0 = time.perf_counter()
-            rr = po.ref_surf_forward(gdict(w.grid), w.points, w.params, w.freqs, w.src, w.rev, w.raystat,
-                                     order=args.order, nthreads=threads)
-            po.body_misfit(rr[2], w.ttime, w.raystat, np.zeros((w.src.shape[0], 4)))
-            t_ref = time.perf_counter() - t0
+            def ref_eval(nth):
+                t0 = time.perf_counter()
+                rr = po.ref_surf_forward(gdict(w.grid), w.points, w.params, w.freqs, w.src, w.rev, w.raystat,
+                                         order=args.order, nthreads=nth)
+                po.body_misfit(rr[2], w.ttime, w.raystat, np.zeros((w.src.shape[0], 4)))
+                return time.perf_counter() - t0, rr
+            ref_eval(threads)                                   # warm-up
+            runs = [ref_eval(threads) for _ in range(3)]
+            t_ref = float(np.median([r[0] for r in runs]))
+            rr = runs[-1][1]
             err["reference_stages_equal_oracle"] = bool(rr[0] == 0 and np.array_equal(rr[1], orc[5]) and
                                                         np.array_equal(rr[2], orc[2]))
+            t_ref1 = ref_eval(1)[0] if not args.no_cpu_1thread else None
+            cpu_model = "unknown"
+            try:
+                with open("/proc/cpuinfo") as f:
+                    for line in f:
+                        if line.startswith("model name"):
+                            cpu_model = line.split(":", 1)[1].strip()
+                            break
+            except OSError:
+                pass
             cpu = {"value": round(1.0 / t_ref, 5), "unit": "evals/s", "cores": threads, "kind": "reference",
-                   "sample": "one full evaluation of the plain chain's model: surfdisp96 on 40,401 columns "
+                   "sample": "full evaluations of the plain chain's model: surfdisp96 on 40,401 columns "
                              "and modrays for 20 periods x 32 sources by the reference's own Fortran "
                              "(amdflang -O3 -fopenmp, OpenMP over columns and periods), nearest site, "
-                             "convert_to_layer and misfit by the C oracle; %.1f s (the C oracle alone: "
-                             "%.1f s)" % (t_ref, t_cpu),
+                             "convert_to_layer and misfit by the C oracle; median of 3 after a warm-up, "
+                             "%.2f s on %d threads (the C oracle alone: %.2f s)" % (t_ref, threads, t_cpu),
+                   "cpu_model": cpu_model, "host_cpus_visible": os.cpu_count(),
+                   "one_thread_value": None if t_ref1 is None else round(1.0 / t_ref1, 5),
+                   "one_thread_s": None if t_ref1 is None else round(t_ref1, 3),
                    "port_value": round(1.0 / t_cpu, 5)}
         log("cpu baseline", cpu, "err", err)
 
@@ -472,6 +547,7 @@ def run_surface(args):
         "autotune": tuned,
         "perturbed_box": pbox,
         "group_velocity": group,
+        "order1_batched": order1,
         "roofline": roof,
         "cpu_baseline": cpu,
         "max_error_vs_oracle": err,
@@ -538,7 +614,7 @@ def run_body(args):
     cpu = None
     if not args.no_cpu:
         from oracle import pyoracle as po
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        threads = len(os.sched_getaffinity(0))
         grid = gdict(w.grid)
         t0 = time.perf_counter()
         v = po.kdtree_to_grid(grid, w.points, w.params)[1]
@@ -582,6 +658,9 @@ def main():
                     help="requests of the three-grid served mix (0: skip)")
     ap.add_argument("--group-steps", type=int, default=2,
                     help="timed group-velocity batches (secondary figure; 0: skip)")
+    ap.add_argument("--order1-steps", type=int, default=3,
+                    help="timed batches with the mixed-order update beside the order-0 value (0: skip)")
+    ap.add_argument("--no-cpu-1thread", action="store_true", help="skip the one-thread CPU reference eval")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
