@@ -254,7 +254,11 @@ int mct_copy_d2d(void *dst_dev, const void *src_dev, int64_t bytes);
 /* Eikonal schedule and precision of a context: band_tiles is how many 4^3
  * tiles the relaxation band advances per round (smaller = fewer re-visits,
  * more rounds); t64 = 1 stores arrival times in FP64 (the parity setting),
- * 0 in FP32.  Values <= 0 / < 0 leave the setting unchanged. */
+ * 0 in FP32.  A context starts with FP64 times, velocities and local solves
+ * (the reference's precision; the environment variable MCT_EIK_F64=0 at
+ * create keeps FP64 times with FP32 velocities and solves); t64 = 0 switches
+ * all three to FP32 for the rest of the context's life.  Values <= 0 / < 0
+ * leave the setting unchanged. */
 int mct_set_tuning(mct_ctx *ctx, float band_tiles, int t64);
 
 /* Wait for the context's stream. */
@@ -460,10 +464,13 @@ int mct_surf_dispersion_solved(mct_surf_ctx *ctx, int nchain, int64_t *columns, 
 int mct_surf_fm2d_stats(mct_surf_ctx *ctx, double *stats4);
 
 /* Problems of the last launch chunk that the parallel 2-D path handed to the
- * sequential solver: {failed the heap-order check, all hand-overs (the check
- * and a heap loop that outgrew its LDS neighbourhood)}.  No counterpart in
- * the reference: a measurement. */
-int mct_surf_fm2d_handover(mct_surf_ctx *ctx, int64_t *handover2);
+ * sequential solver: [0] failed the heap-order check, [1] all hand-overs (the
+ * check and a heap loop that outgrew its LDS neighbourhood), [2..8] problems
+ * per reason of the check: no finite time, tied times, a key at or below the
+ * next neighbour's time, a rising key, a last key that is not the time, an
+ * invalid band heap at the hand-over, a sweep that did not settle.  No
+ * counterpart in the reference: a measurement. */
+int mct_surf_fm2d_handover(mct_surf_ctx *ctx, int64_t *counts9);
 
 /* The context's HIP stream (hipStream_t) for callers that queue their own
  * copies in order with the context's work. */

@@ -735,11 +735,16 @@ class SurfLikelihood:
         _check(lib().mct_surf_fm2d_stats(self._h, _p(t)), "mct_surf_fm2d_stats")
         return tuple(t)
 
-    def fm2d_handover(self):
+    HANDOVER_REASONS = ("no time", "tie", "early", "rising key", "final key", "band heap", "no convergence")
+
+    def fm2d_handover(self, reasons=False):
         """(problems that failed the heap-order check, all problems handed to
-        the sequential solver) in the last 2-D chunk."""
-        t = np.zeros(2, np.int64)
+        the sequential solver) in the last 2-D chunk; with reasons, also a dict
+        of problems per reason of the check."""
+        t = np.zeros(9, np.int64)
         _check(lib().mct_surf_fm2d_handover(self._h, _p(t)), "mct_surf_fm2d_handover")
+        if reasons:
+            return int(t[0]), int(t[1]), dict(zip(self.HANDOVER_REASONS, (int(x) for x in t[2:])))
         return int(t[0]), int(t[1])
 
     def dev_arrays(self):

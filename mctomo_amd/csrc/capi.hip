@@ -114,7 +114,7 @@ struct round_timer {
 };
 
 // Run eikonal rounds until no tile is active.  Returns rounds used or <0.
-int eik_solve(const mct_eik_problem_set &Pset, eik_ws &ws, float hx, float hy, float hz,
+int eik_solve(const mct_eik_problem_set &Pset, eik_ws &ws, double hx, double hy, double hz,
               hipStream_t s, int *rounds_out, round_timer *tm)
 {
     const mct_geom &G = Pset.G;
@@ -149,6 +149,7 @@ struct mct_ctx {
     int nwave, nsrc_eff, npick, reciprocal;
     float band_tiles;                // eikonal schedule: tiles the band advances per round
     int t64;                         // eikonal arrival times stored as double
+    int f64;                         // eikonal velocities and local solves in FP64 (with t64)
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double ms[4] = {0, 0, 0, 0};
@@ -180,7 +181,9 @@ static int body_pipeline(mct_ctx *C, int nchain, const double *d_points, const d
     na.G = G; na.nchain = nchain; na.max_cells = C->S.max_cells;
     na.points = d_points; na.params = d_params; na.ncells = d_ncells; na.ncells_const = ncells_const;
     na.ix0 = 1; na.ix1 = G.nx; na.iy0 = 1; na.iy1 = G.ny; na.iz0 = 1; na.iz1 = G.nz;
-    na.vel_tiles = P<float>(C->vel); na.nwave = C->nwave;
+    na.vel_tiles = C->f64 ? nullptr : P<float>(C->vel);
+    na.vel_tiles64 = C->f64 ? P<double>(C->vel) : nullptr;
+    na.nwave = C->nwave;
     na.flag_count = P<int32_t>(C->flag_count); na.flag_nodes = P<int32_t>(C->flag_nodes);
     na.kd_nodes = P<mct_kd_node>(C->kd_nodes); na.kd_ind = P<int32_t>(C->kd_ind);
     na.kd_rdata = P<double>(C->kd_rdata);
@@ -193,13 +196,15 @@ static int body_pipeline(mct_ctx *C, int nchain, const double *d_points, const d
     memset(&E, 0, sizeof E);
     E.G = G; E.nsrc_eff = C->nsrc_eff; E.nwave = C->nwave; E.order = C->S.order;
     E.nprob = nchain * C->nwave * C->nsrc_eff;
-    E.T = C->eik.T.p; E.t64 = C->t64; E.vel = P<float>(C->vel);
+    E.T = C->eik.T.p; E.t64 = C->t64;
+    E.vel = C->f64 ? nullptr : P<float>(C->vel);
+    E.vel64 = C->f64 ? P<double>(C->vel) : nullptr;
         E.flags[0] = P<uint8_t>(C->eik.flags0); E.flags[1] = P<uint8_t>(C->eik.flags1);
     E.any = P<uint32_t>(C->eik.counts);
     E.key[0] = P<uint32_t>(C->eik.key0); E.key[1] = P<uint32_t>(C->eik.key1);
     E.band_dt = P<float>(C->eik.band);
     E.band_rounds = mct_band_rounds(G, C->band_tiles);
-    if ((rc = mct_launch_band_dt(E.vel, (long)G.ntiles * MCT_TILE_N, nchain * C->nwave,
+    if ((rc = mct_launch_band_dt(E.vel, E.vel64, (long)G.ntiles * MCT_TILE_N, nchain * C->nwave,
                                  (float)fmin(fmin(G.dx, G.dy), lz), C->band_tiles,
                                  P<float>(C->eik.band), s)))
         return rc;
@@ -216,7 +221,7 @@ static int body_pipeline(mct_ctx *C, int nchain, const double *d_points, const d
         return rc;
     int rounds = 0;
     if (C->timer) C->timer->used = 0;
-    if ((rc = eik_solve(E, C->eik, (float)G.dx, (float)G.dy, (float)lz, s, &rounds, C->timer))) return rc;
+    if ((rc = eik_solve(E, C->eik, G.dx, G.dy, lz, s, &rounds, C->timer))) return rc;
     MCT_CHECK(hipEventRecord(C->ev[2], s));
 
     // (d) picks + misfit
@@ -317,7 +322,8 @@ int mct_body_create(const mct_grid *grid, const mct_body_setup *setup, mct_ctx *
     C->nsrc_eff = C->reciprocal ? setup->nrev : setup->nsrc;
     C->npick = C->reciprocal ? setup->nsrc : setup->nrev;
     C->band_tiles = mct_default_band_tiles();
-    C->t64 = mct_default_t64();
+    C->f64 = mct_default_f64();
+    C->t64 = C->f64 ? 1 : mct_default_t64();
     int rc = MCT_OK;
     const long npair = (long)setup->nsrc * setup->nrev;
     const int np = C->nwave, M = setup->max_chains, S = setup->max_cells;
@@ -335,7 +341,7 @@ int mct_body_create(const mct_grid *grid, const mct_body_setup *setup, mct_ctx *
     ALLOC(noise, sizeof(double) * 2 * np * M);
     ALLOC(out, sizeof(double) * 3 * M);
     ALLOC(status, sizeof(int32_t) * M);
-    ALLOC(vel, sizeof(float) * gsz * np * M);
+    ALLOC(vel, (C->f64 ? sizeof(double) : sizeof(float)) * gsz * np * M);
     ALLOC(flag_count, sizeof(int32_t) * M);
     ALLOC(flag_nodes, sizeof(int32_t) * MCT_MAX_FLAGGED * M);
     ALLOC(kd_nodes, sizeof(mct_kd_node) * (2 * S + 2) * M);
@@ -588,6 +594,7 @@ int mct_set_tuning(mct_ctx *C, float band_tiles, int t64)
     if (!C) return MCT_ERR_ARG;
     if (band_tiles > 0.f) C->band_tiles = band_tiles;
     if (t64 >= 0) C->t64 = t64 ? 1 : 0;
+    if (!C->t64) C->f64 = 0;   // FP32 arrival times: FP32 velocities and local solves too
     return MCT_OK;
 }
 

@@ -51,7 +51,8 @@ dropin_ws &WS()
 }
 
 // Fortran (nz,ny,nx) double velocity -> FP32 eikonal tiles (padding lanes 0)
-__global__ void k_fortran_to_tiles(const double *v, float *vt, mct_geom G)
+template <class VT>
+__global__ void k_fortran_to_tiles(const double *v, VT *vt, mct_geom G)
 {
     long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long n = (long)G.ntiles * MCT_TILE_N;
@@ -59,9 +60,9 @@ __global__ void k_fortran_to_tiles(const double *v, float *vt, mct_geom G)
     int tile = (int)(e >> 6), lane = (int)(e & 63);
     int tx = tile % G.ntx, ty = (tile / G.ntx) % G.nty, tz = tile / (G.ntx * G.nty);
     int x = tx * 4 + (lane & 3), y = ty * 4 + ((lane >> 2) & 3), z = tz * 4 + (lane >> 4);
-    float out = 0.0f;
+    VT out = 0;
     if (x < G.nx && y < G.ny && z < G.nz)
-        out = (float)v[(long)z + (long)G.nz * ((long)y + (long)G.ny * x)];
+        out = (VT)v[(long)z + (long)G.nz * ((long)y + (long)G.ny * x)];
     vt[e] = out;
 }
 
@@ -78,7 +79,7 @@ __global__ void k_tiles_to_fortran(const void *tv, int t64, double *f, mct_geom 
     f[g] = t64 ? ((const double *)tv)[i] : (double)((const float *)tv)[i];
 }
 
-int eik_rounds(const mct_eik_problem_set &E, uint32_t *h_count, float hx, float hy, float hz,
+int eik_rounds(const mct_eik_problem_set &E, uint32_t *h_count, double hx, double hy, double hz,
                hipStream_t s)
 {
     const mct_geom &G = E.G;
@@ -117,7 +118,8 @@ extern "C" int fastMarching3D(int nx, int ny, int nz, double dx, double dy, doub
     mct_grid g = {nx, ny, nz, xmin, ymin, zmin, dx, dy, dz, 1.0};
     mct_geom G = mct_make_geom(&g);
     const long gn = (long)nx * ny * nz, tn = (long)G.ntiles * MCT_TILE_N;
-    if ((rc = W.vel_d.need(sizeof(double) * gn)) || (rc = W.vel_t.need(sizeof(float) * tn)) ||
+    const int f64 = mct_default_f64();
+    if ((rc = W.vel_d.need(sizeof(double) * gn)) || (rc = W.vel_t.need((f64 ? sizeof(double) : sizeof(float)) * tn)) ||
         (rc = W.T.need(sizeof(double) * tn)) || (rc = W.fl0.need((G.ntiles + 3) & ~3)) ||
         (rc = W.fl1.need((G.ntiles + 3) & ~3)) || (rc = W.counts.need(4 * sizeof(uint32_t))) ||
         (rc = W.k0.need(sizeof(uint32_t) * G.ntiles)) || (rc = W.k1.need(sizeof(uint32_t) * G.ntiles)) ||
@@ -130,23 +132,29 @@ extern "C" int fastMarching3D(int nx, int ny, int nz, double dx, double dy, doub
     MCT_CHECK(hipMemcpyAsync(W.vel_d.p, vel, sizeof(double) * gn, hipMemcpyHostToDevice, s));
     MCT_CHECK(hipMemcpyAsync(W.src.p, src, 3 * sizeof(double), hipMemcpyHostToDevice, s));
     if (nrev > 0) MCT_CHECK(hipMemcpyAsync(W.rev.p, rev, sizeof(double) * 3 * nrev, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_fortran_to_tiles, dim3((unsigned)((tn + 255) / 256)), dim3(256), 0, s,
-                       (const double *)W.vel_d.p, (float *)W.vel_t.p, G);
+    if (f64)
+        hipLaunchKernelGGL(k_fortran_to_tiles<double>, dim3((unsigned)((tn + 255) / 256)), dim3(256), 0, s,
+                           (const double *)W.vel_d.p, (double *)W.vel_t.p, G);
+    else
+        hipLaunchKernelGGL(k_fortran_to_tiles<float>, dim3((unsigned)((tn + 255) / 256)), dim3(256), 0, s,
+                           (const double *)W.vel_d.p, (float *)W.vel_t.p, G);
     MCT_CHECK(hipGetLastError());
     mct_eik_problem_set E;
     memset(&E, 0, sizeof E);
     E.G = G; E.nprob = 1; E.nsrc_eff = 1; E.nwave = 1; E.order = order == 1 ? 1 : 0;
-    E.T = W.T.p; E.t64 = mct_default_t64(); E.vel = (const float *)W.vel_t.p;
+    E.T = W.T.p; E.t64 = f64 ? 1 : mct_default_t64();
+    E.vel = f64 ? nullptr : (const float *)W.vel_t.p;
+    E.vel64 = f64 ? (const double *)W.vel_t.p : nullptr;
         E.flags[0] = (uint8_t *)W.fl0.p; E.flags[1] = (uint8_t *)W.fl1.p;
     E.any = (uint32_t *)W.counts.p;
     E.key[0] = (uint32_t *)W.k0.p; E.key[1] = (uint32_t *)W.k1.p;
     E.band_dt = (const float *)W.band.p;
     E.band_rounds = mct_band_rounds(G, mct_default_band_tiles());
-    if ((rc = mct_launch_band_dt(E.vel, tn, 1, (float)fmin(fmin(dx, dy), dz), mct_default_band_tiles(),
+    if ((rc = mct_launch_band_dt(E.vel, E.vel64, tn, 1, (float)fmin(fmin(dx, dy), dz), mct_default_band_tiles(),
                                  (float *)W.band.p, s)))
         return rc;
     if ((rc = mct_launch_eik_init(E, (const double *)W.src.p, 3, 0, xmin, ymin, zmin, dx, dy, dz, s))) return rc;
-    if ((rc = eik_rounds(E, W.h_count, (float)dx, (float)dy, (float)dz, s))) return rc;
+    if ((rc = eik_rounds(E, W.h_count, dx, dy, dz, s))) return rc;
     if (nrev > 0) {
         mct_pick_args pa;
         memset(&pa, 0, sizeof pa);

@@ -29,62 +29,69 @@ template <> __device__ __forceinline__ double tinf<double>() { return __builtin_
 // LDS box: the 4^3 tile at offset 2 inside an 8^3 box (2-deep face halo).
 #define BX(x, y, z) ((((z) + 2) << 6) | (((y) + 2) << 3) | ((x) + 2))
 
+// The local solves below run in R = float (velocities and increments in FP32)
+// or R = double (the FP64 mode: the reference's own precision).
+__device__ __forceinline__ float lsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double lsqrt(double x) { return sqrt(x); }
+
 // Compare-and-swap of (time, 1/h^2, h) triples by time; the swap keeps the
 // earlier dimension first on equal times (std::sort on <= 16 elements is an
 // insertion sort, which is stable).
-__device__ __forceinline__ void cas3(float &t0, float &a0, float &h0, float &t1, float &a1, float &h1)
+template <class R>
+__device__ __forceinline__ void cas3(R &t0, R &a0, R &h0, R &t1, R &a1, R &h1)
 {
     bool sw = t1 < t0;
-    float tt = sw ? t1 : t0, aa = sw ? a1 : a0, hh = sw ? h1 : h0;
+    R tt = sw ? t1 : t0, aa = sw ? a1 : a0, hh = sw ? h1 : h0;
     t1 = sw ? t0 : t1; a1 = sw ? a0 : a1; h1 = sw ? h0 : h1;
     t0 = tt; a0 = aa; h0 = hh;
 }
 
-__device__ __forceinline__ void cas4(float &t0, float &u0, float &a0, float &h0,
-                                     float &t1, float &u1, float &a1, float &h1)
+template <class R>
+__device__ __forceinline__ void cas4(R &t0, R &u0, R &a0, R &h0, R &t1, R &u1, R &a1, R &h1)
 {
     bool sw = t1 < t0;
-    float tt = sw ? t1 : t0, uu = sw ? u1 : u0, aa = sw ? a1 : a0, hh = sw ? h1 : h0;
+    R tt = sw ? t1 : t0, uu = sw ? u1 : u0, aa = sw ? a1 : a0, hh = sw ? h1 : h0;
     t1 = sw ? t0 : t1; u1 = sw ? u0 : u1; a1 = sw ? a0 : a1; h1 = sw ? h0 : h1;
     t0 = tt; u0 = uu; a0 = aa; h0 = hh;
 }
 
 // root of sum_i al_i (tau - dl_i)^2 = s2 (larger root), INF if none
-__device__ __forceinline__ float root2(float a0, float d0, float a1, float d1, float s2)
+template <class R>
+__device__ __forceinline__ R root2(R a0, R d0, R a1, R d1, R s2)
 {
-    float A = a0 + a1, B = a0 * d0 + a1 * d1, C = a0 * d0 * d0 + a1 * d1 * d1 - s2;
-    float disc = B * B - A * C;
-    return disc < 0.f ? EIK_INF : (B + sqrtf(disc)) / A;
+    R A = a0 + a1, B = a0 * d0 + a1 * d1, C = a0 * d0 * d0 + a1 * d1 * d1 - s2;
+    R disc = B * B - A * C;
+    return disc < R(0) ? tinf<R>() : (B + lsqrt(disc)) / A;
 }
 
-__device__ __forceinline__ float root3(float a0, float d0, float a1, float d1, float a2, float d2,
-                                       float s2)
+template <class R>
+__device__ __forceinline__ R root3(R a0, R d0, R a1, R d1, R a2, R d2, R s2)
 {
-    float A = a0 + a1 + a2, B = a0 * d0 + a1 * d1 + a2 * d2;
-    float C = a0 * d0 * d0 + a1 * d1 * d1 + a2 * d2 * d2 - s2;
-    float disc = B * B - A * C;
-    return disc < 0.f ? EIK_INF : (B + sqrtf(disc)) / A;
+    R A = a0 + a1 + a2, B = a0 * d0 + a1 * d1 + a2 * d2;
+    R C = a0 * d0 * d0 + a1 * d1 * d1 + a2 * d2 * d2 - s2;
+    R disc = B * B - A * C;
+    return disc < R(0) ? tinf<R>() : (B + lsqrt(disc)) / A;
 }
 
 // solveEikonal (first order): dimensions whose smaller neighbour beats the
 // cell's own value, sorted, grown while the next one is below the solution.
 // Written relative to the smallest neighbour time t0.
-__device__ __forceinline__ float update_order0(const float m[3], float tself, const float h[3],
-                                               const float ih2[3], float inv_v, float s2)
+template <class R>
+__device__ __forceinline__ R update_order0(const R m[3], R tself, const R h[3], const R ih2[3], R inv_v, R s2)
 {
-    float t0 = m[0] < tself ? m[0] : EIK_INF, t1 = m[1] < tself ? m[1] : EIK_INF,
-          t2 = m[2] < tself ? m[2] : EIK_INF;
-    float a0 = ih2[0], a1 = ih2[1], a2 = ih2[2], h0 = h[0], h1 = h[1], h2 = h[2];
+    const R INF = tinf<R>();
+    R t0 = m[0] < tself ? m[0] : INF, t1 = m[1] < tself ? m[1] : INF, t2 = m[2] < tself ? m[2] : INF;
+    R a0 = ih2[0], a1 = ih2[1], a2 = ih2[2], h0 = h[0], h1 = h[1], h2 = h[2];
     cas3(t0, a0, h0, t1, a1, h1);
     cas3(t1, a1, h1, t2, a2, h2);
     cas3(t0, a0, h0, t1, a1, h1);
-    if (t0 == EIK_INF) return EIK_INF;
-    float u1 = t0 + h0 * inv_v;
+    if (t0 == INF) return INF;
+    R u1 = t0 + h0 * inv_v;
     if (!(u1 > t1)) return u1;                     // one dimension, or the next is not below
-    float d1 = t1 - t0;
-    float tau2 = root2(a0, 0.f, a1, d1, s2);
+    R d1 = t1 - t0;
+    R tau2 = root2(a0, R(0), a1, d1, s2);
     if (!(tau2 + t0 > t2)) return t0 + tau2;
-    float tau3 = root3(a0, 0.f, a1, d1, a2, t2 - t0, s2);
+    R tau3 = root3(a0, R(0), a1, d1, a2, t2 - t0, s2);
     return t0 + tau3;
 }
 
@@ -92,34 +99,35 @@ __device__ __forceinline__ float update_order0(const float m[3], float tself, co
 // the two-away time on that side is upwind (<= t), the second-order value
 // (4t - t2)/3 with weight 9/4; try all included dimensions first and drop the
 // largest while the root does not exceed that dimension's time.
-__device__ __forceinline__ float update_order1(const float m1[3], const float m2[3], float tself,
-                                               const float h[3], const float ih2[3], float inv_v,
-                                               float s2)
+template <class R>
+__device__ __forceinline__ R update_order1(const R m1[3], const R m2[3], R tself, const R h[3], const R ih2[3],
+                                           R inv_v, R s2)
 {
-    float t[3], u[3], al[3], hh[3];
+    const R INF = tinf<R>();
+    R t[3], u[3], al[3], hh[3];
     for (int d = 0; d < 3; d++) {
         bool inc = m1[d] < tself;
-        bool two = m2[d] < EIK_INF;
-        t[d] = inc ? m1[d] : EIK_INF;
-        u[d] = two ? m2[d] : EIK_INF;
-        al[d] = two ? 2.25f * ih2[d] : ih2[d];
+        bool two = m2[d] < INF;
+        t[d] = inc ? m1[d] : INF;
+        u[d] = two ? m2[d] : INF;
+        al[d] = two ? R(2.25) * ih2[d] : ih2[d];
         hh[d] = h[d];
     }
     cas4(t[0], u[0], al[0], hh[0], t[1], u[1], al[1], hh[1]);
     cas4(t[1], u[1], al[1], hh[1], t[2], u[2], al[2], hh[2]);
     cas4(t[0], u[0], al[0], hh[0], t[1], u[1], al[1], hh[1]);
-    if (t[0] == EIK_INF) return tself;
-    const float t0 = t[0];
-    float dl[3];
+    if (t[0] == INF) return tself;
+    const R t0 = t[0];
+    R dl[3];
     for (int d = 0; d < 3; d++)
-        dl[d] = u[d] < EIK_INF ? (4.f * (t[d] - t0) - (u[d] - t0)) * (1.f / 3.f) : t[d] - t0;
-    if (t[2] < EIK_INF) {
-        float tau = root3(al[0], dl[0], al[1], dl[1], al[2], dl[2], s2);
-        if (tau < EIK_INF && tau > t[2] - t0) return t0 + tau;
+        dl[d] = u[d] < INF ? (R(4) * (t[d] - t0) - (u[d] - t0)) * (R(1) / R(3)) : t[d] - t0;
+    if (t[2] < INF) {
+        R tau = root3(al[0], dl[0], al[1], dl[1], al[2], dl[2], s2);
+        if (tau < INF && tau > t[2] - t0) return t0 + tau;
     }
-    if (t[1] < EIK_INF) {
-        float tau = root2(al[0], dl[0], al[1], dl[1], s2);
-        if (tau < EIK_INF && tau > t[1] - t0) return t0 + tau;
+    if (t[1] < INF) {
+        R tau = root2(al[0], dl[0], al[1], dl[1], s2);
+        if (tau < INF && tau > t[1] - t0) return t0 + tau;
     }
     return t0 + hh[0] * inv_v;
 }
@@ -170,7 +178,8 @@ __global__ void k_eik_init(mct_eik_problem_set P, const double *src_pts, int src
     int dims[3] = {G.nx, G.ny, G.nz};
     int pc[3];
     pt_coord(G, p, leaf, pc);
-    const float *V = P.vel + (long)cw * G.ntiles * MCT_TILE_N;
+    const float *V = P.vel ? P.vel + (long)cw * G.ntiles * MCT_TILE_N : nullptr;
+    const double *V64 = P.vel64 ? P.vel64 + (long)cw * G.ntiles * MCT_TILE_N : nullptr;
     TT *T = (TT *)P.T + (long)prob * G.ntiles * MCT_TILE_N;
     int cx[8], cy[8], cz[8];
     double vn[8];
@@ -181,7 +190,8 @@ __global__ void k_eik_init(mct_eik_problem_set P, const double *src_pts, int src
             if (vc[d] >= dims[d]) vc[d] = dims[d] - 1;
         }
         cx[v] = vc[0]; cy[v] = vc[1]; cz[v] = vc[2];
-        vn[v] = (double)V[tl_index(G, vc[0], vc[1], vc[2])];
+        const long q = tl_index(G, vc[0], vc[1], vc[2]);
+        vn[v] = V64 ? V64[q] : (double)V[q];
     }
     // getVel: trilinear weights with clamped corners
     double vs = 0.0;
@@ -239,17 +249,19 @@ __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); 
 // increment of about h/v), then added to t0 in TT: with TT = double the
 // rounding of each step is that of the small increment, so errors do not
 // accumulate along a ray at the FP32 level.
-template <int ORDER, class TT>
-__global__ __launch_bounds__(64) void k_eik_round(mct_eik_problem_set P, int round, float hx, float hy, float hz)
+template <int ORDER, class TT, class CT>
+__global__ __launch_bounds__(64) void k_eik_round(mct_eik_problem_set P, int round, double hxd, double hyd,
+                                                  double hzd)
 {
     __shared__ TT box[512];
     const mct_geom &G = P.G;
     const TT TINF = tinf<TT>();
+    const CT CINF = tinf<CT>();
     const int cur = round & 1, nxt = cur ^ 1;
     const int lane = threadIdx.x;
     const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
-    const float h[3] = {hx, hy, hz};
-    const float ih2[3] = {1.f / (hx * hx), 1.f / (hy * hy), 1.f / (hz * hz)};
+    const CT h[3] = {(CT)hxd, (CT)hyd, (CT)hzd};
+    const CT ih2[3] = {CT(1) / (h[0] * h[0]), CT(1) / (h[1] * h[1]), CT(1) / (h[2] * h[2])};
     if (blockIdx.x == 0 && lane == 0) P.any[(round + 2) % 3] = 0u;   // flag of round r+2
     const uint32_t ntiles = (uint32_t)G.ntiles;
     const uint32_t nkeys = (uint32_t)P.nprob * ntiles;
@@ -258,7 +270,11 @@ __global__ __launch_bounds__(64) void k_eik_round(mct_eik_problem_set P, int rou
     uint32_t *kcur = P.key[cur], *knxt = P.key[nxt];
     unsigned int *any_next = &P.any[(round + 1) % 3];
     const bool banded = round < P.band_rounds;
-    const TT thr = sizeof(TT) == 8 ? (TT)1e-9 : (TT)1e-7;   // relative change that counts
+    // Relative change that counts.  In the FP64 mode every decrease counts, so
+    // the rounds end only at the update's fixed point, whatever order the tiles
+    // ran in (a batch gives each chain the times its own call gives); with FP32
+    // increments a change is at least an FP32 step of the increment anyway.
+    const TT thr = sizeof(CT) == 8 ? (TT)0 : sizeof(TT) == 8 ? (TT)1e-9 : (TT)1e-7;
     TT *Tall = (TT *)P.T;
     // face lane masks (bit l = lane l): the layers a neighbour reads as halo
     const int deep = ORDER == 0 ? 0 : 1;
@@ -305,7 +321,8 @@ __global__ __launch_bounds__(64) void k_eik_round(mct_eik_problem_set P, int rou
             const bool inside = gx < G.nx && gy < G.ny && gz < G.nz;
             TT *T = Tall + (size_t)prob * ((size_t)ntiles << 6);
             const uint32_t cw = prob / (uint32_t)P.nsrc_eff;
-            const float v = P.vel[(size_t)cw * ((size_t)ntiles << 6) + ((size_t)tile << 6) + lane];
+            const size_t vi = (size_t)cw * ((size_t)ntiles << 6) + ((size_t)tile << 6) + lane;
+            const CT v = sizeof(CT) == 8 ? (CT)P.vel64[vi] : (CT)P.vel[vi];
             TT t = inside ? T[((size_t)tile << 6) + lane] : TINF;
             const TT t_in = t;
             box[BX(lx, ly, lz)] = t;
@@ -331,9 +348,9 @@ __global__ __launch_bounds__(64) void k_eik_round(mct_eik_problem_set P, int rou
                 box[BX(ox, oy, oz)] = hv;
             }
             __syncthreads();
-            const float inv_v = 1.f / v;
-            const float s2 = inv_v * inv_v;
-            const bool solve = inside && v >= 2.220446e-11f;   // isOccupied(): velocity < COMP_MARGIN
+            const CT inv_v = CT(1) / v;
+            const CT s2 = inv_v * inv_v;
+            const bool solve = inside && v >= CT(2.220446e-11);   // isOccupied(): velocity < COMP_MARGIN
             const int me = BX(lx, ly, lz);
             const int st[3] = {1, 8, 64};
             bool capped = true;
@@ -352,16 +369,16 @@ __global__ __launch_bounds__(64) void k_eik_round(mct_eik_problem_set P, int rou
                 }
                 TT u = TINF;
                 if (solve && t0 < t) {
-                    // times relative to t0 in FP32; excluded/absent neighbours stay INF
-                    float m1[3], m2[3];
-                    const float fself = t < TINF ? (float)(t - t0) : __builtin_huge_valf();
+                    // times relative to t0 in CT; excluded/absent neighbours stay INF
+                    CT m1[3], m2[3];
+                    const CT fself = t < TINF ? (CT)(t - t0) : CINF;
                     for (int d = 0; d < 3; d++) {
-                        m1[d] = n1[d] < TINF ? (float)(n1[d] - t0) : __builtin_huge_valf();
-                        if (ORDER != 0) m2[d] = n2[d] < TINF ? (float)(n2[d] - t0) : __builtin_huge_valf();
+                        m1[d] = n1[d] < TINF ? (CT)(n1[d] - t0) : CINF;
+                        if (ORDER != 0) m2[d] = n2[d] < TINF ? (CT)(n2[d] - t0) : CINF;
                     }
-                    float du = ORDER == 0 ? update_order0(m1, fself, h, ih2, inv_v, s2)
-                                          : update_order1(m1, m2, fself, h, ih2, inv_v, s2);
-                    if (du < __builtin_huge_valf()) u = t0 + (TT)du;
+                    CT du = ORDER == 0 ? update_order0<CT>(m1, fself, h, ih2, inv_v, s2)
+                                       : update_order1<CT>(m1, m2, fself, h, ih2, inv_v, s2);
+                    if (du < CINF) u = t0 + (TT)du;
                 }
                 bool better = u < t;
                 bool sig = better && (u < t - thr * t || t == TINF);
@@ -426,16 +443,20 @@ int mct_launch_eik_init(const mct_eik_problem_set &P, const double *src_pts, int
     return MCT_OK;
 }
 
-int mct_launch_eik_round(const mct_eik_problem_set &P, int round, float hx, float hy, float hz,
+int mct_launch_eik_round(const mct_eik_problem_set &P, int round, double hx, double hy, double hz,
                          int grid_blocks, hipStream_t s)
 {
     dim3 g(grid_blocks), b(64);
-    if (P.t64) {
-        if (P.order == 1) hipLaunchKernelGGL((k_eik_round<1, double>), g, b, 0, s, P, round, hx, hy, hz);
-        else hipLaunchKernelGGL((k_eik_round<0, double>), g, b, 0, s, P, round, hx, hy, hz);
+    if (P.vel64) {   // FP64 velocities and local solves (arrival times FP64 too)
+        if (!P.t64) return MCT_ERR_ARG;
+        if (P.order == 1) hipLaunchKernelGGL((k_eik_round<1, double, double>), g, b, 0, s, P, round, hx, hy, hz);
+        else hipLaunchKernelGGL((k_eik_round<0, double, double>), g, b, 0, s, P, round, hx, hy, hz);
+    } else if (P.t64) {
+        if (P.order == 1) hipLaunchKernelGGL((k_eik_round<1, double, float>), g, b, 0, s, P, round, hx, hy, hz);
+        else hipLaunchKernelGGL((k_eik_round<0, double, float>), g, b, 0, s, P, round, hx, hy, hz);
     } else {
-        if (P.order == 1) hipLaunchKernelGGL((k_eik_round<1, float>), g, b, 0, s, P, round, hx, hy, hz);
-        else hipLaunchKernelGGL((k_eik_round<0, float>), g, b, 0, s, P, round, hx, hy, hz);
+        if (P.order == 1) hipLaunchKernelGGL((k_eik_round<1, float, float>), g, b, 0, s, P, round, hx, hy, hz);
+        else hipLaunchKernelGGL((k_eik_round<0, float, float>), g, b, 0, s, P, round, hx, hy, hz);
     }
     MCT_CHECK(hipGetLastError());
     return MCT_OK;
@@ -443,14 +464,14 @@ int mct_launch_eik_round(const mct_eik_problem_set &P, int round, float hx, floa
 
 // Band time step per (chain, wave): the time the front needs to cross
 // `tiles_per_round` tiles at the fastest velocity of that model.
-__global__ __launch_bounds__(256) void k_band_dt(const float *vel, long tsz, float hmin, float tpr,
-                                                 float *band_dt)
+template <class VT>
+__global__ __launch_bounds__(256) void k_band_dt(const VT *vel, long tsz, float hmin, float tpr, float *band_dt)
 {
     __shared__ float sm[4];
     const int cw = blockIdx.x;
-    const float *V = vel + (long)cw * tsz;
+    const VT *V = vel + (long)cw * tsz;
     float m = 0.f;
-    for (long i = threadIdx.x; i < tsz; i += blockDim.x) m = fmaxf(m, V[i]);
+    for (long i = threadIdx.x; i < tsz; i += blockDim.x) m = fmaxf(m, (float)V[i]);
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
     __syncthreads();
@@ -460,10 +481,13 @@ __global__ __launch_bounds__(256) void k_band_dt(const float *vel, long tsz, flo
     }
 }
 
-int mct_launch_band_dt(const float *vel, long tsz, int ncw, float hmin, float tiles_per_round,
+int mct_launch_band_dt(const float *vel, const double *vel64, long tsz, int ncw, float hmin, float tiles_per_round,
                        float *band_dt, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_band_dt, dim3(ncw), dim3(256), 0, s, vel, tsz, hmin, tiles_per_round, band_dt);
+    if (vel64)
+        hipLaunchKernelGGL(k_band_dt<double>, dim3(ncw), dim3(256), 0, s, vel64, tsz, hmin, tiles_per_round, band_dt);
+    else
+        hipLaunchKernelGGL(k_band_dt<float>, dim3(ncw), dim3(256), 0, s, vel, tsz, hmin, tiles_per_round, band_dt);
     MCT_CHECK(hipGetLastError());
     return MCT_OK;
 }
@@ -473,6 +497,12 @@ float mct_default_band_tiles()
     const char *e = getenv("MCT_BAND_TILES");
     float v = e ? (float)atof(e) : 0.5f;
     return v > 0.f ? v : 0.5f;
+}
+
+int mct_default_f64()
+{
+    const char *e = getenv("MCT_EIK_F64");
+    return e ? (atoi(e) != 0) : 1;
 }
 
 int mct_default_t64()

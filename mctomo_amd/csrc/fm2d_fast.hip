@@ -44,6 +44,16 @@
 #define FF_MARGIN 16          // coarse neighbourhood: refined window +- this many nodes
 #define FF_MAX_ROUNDS 512
 #define FF_SEED_RINGS 2       // the seed freezes the source cell and this many rings around it
+// Why the heap-order check handed a problem to the sequential solver (bits
+// of the problem's statistic, k_fm2d_verify and k_fm2d_near).
+#define FF_VR_NOTIME 1      // no finite time
+#define FF_VR_TIE 2         // equal times among the node and its open neighbours below it
+#define FF_VR_EARLY 4       // (b) a key at or below the next neighbour's time
+#define FF_VR_RISE 8        // (a) a key that rises
+#define FF_VR_FINAL 16      // (e) the last key is not the time
+#define FF_VR_HEAP 32       // the band at the hand-over is not a valid heap (k_fm2d_near)
+#define FF_VR_NOCONV 64     // the sweep did not converge
+
 
 namespace {
 
@@ -1337,7 +1347,7 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
     if (sh_ovf) {
         if (lane == 0) {
             pflag[slot] = 1;
-            if (A.rounds_out && sh_ovf == 4) A.rounds_out[4 * slot + 3] = 1;
+            if (A.rounds_out && sh_ovf == 4) A.rounds_out[4 * slot + 3] = FF_VR_HEAP;
         }
         return;
     }
@@ -1369,7 +1379,7 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
 
 // One workgroup per problem: sweep the coarse grid, then pick the receivers.
 template <int FOM>
-__global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, const int32_t *pflag, int use_dirty)
+__global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, int32_t *pflag, int use_dirty)
 {
     extern __shared__ uint32_t ff_dirty[];
     __shared__ int sh_flag;
@@ -1420,7 +1430,16 @@ __global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, const int32
     __syncthreads();
     int rounds = ff_sweep<FOM, false>(V, fixed, vel, &sh_flag, use_dirty ? &W : nullptr, stat);
     if (rounds < 0) {
-        if (threadIdx.x == 0) atomicMin(&A.status[g.c], (int)MCT_ERR_NOCONV);
+        // the mixed-order update need not settle (it is not monotone): with the
+        // heap-order check on, the sequential solver takes the problem
+        if (threadIdx.x == 0) {
+            if (A.verify > 0) {
+                pflag[slot] = 1;
+                if (A.rounds_out) A.rounds_out[4 * slot + 3] = FF_VR_NOCONV;
+            } else {
+                atomicMin(&A.status[g.c], (int)MCT_ERR_NOCONV);
+            }
+        }
         return;
     }
     if (threadIdx.x == 0 && stat) stat[0] = rounds;
@@ -1445,7 +1464,7 @@ __global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, const int32
 #define FF_PF 4
 
 template <int FOM>
-__global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, const int32_t *pflag)
+__global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, int32_t *pflag)
 {
     extern __shared__ double ff_sld[];
     const long slot = ff_xcd_slot(blockIdx.x, gridDim.x);
@@ -1698,7 +1717,14 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, const int32_
         if (lane == 0) stat[2] = tot;
     }
     if (rounds < 0) {
-        if (lane == 0) atomicMin(&A.status[g.c], (int)MCT_ERR_NOCONV);
+        if (lane == 0) {
+            if (A.verify > 0) {
+                pflag[slot] = 1;
+                if (A.rounds_out) A.rounds_out[4 * slot + 3] = FF_VR_NOCONV;
+            } else {
+                atomicMin(&A.status[g.c], (int)MCT_ERR_NOCONV);
+            }
+        }
         return;
     }
     fview V;
@@ -1732,11 +1758,11 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, const int32_
 // sequential solver.  The sweep stops at 1e-14 relative change, so (a) and
 // (e) compare within 1e-12 relative.
 template <int FOM, class FX>
-__device__ __forceinline__ bool ff_verify_node(const fview &V, const FX &fixed, double w0, int z, int x,
-                                               double slown)
+__device__ __forceinline__ int ff_verify_node(const fview &V, const FX &fixed, double w0, int z, int x,
+                                              double slown)
 {
     const double tn = V.t(z, x);
-    if (!(tn < FF_FIN)) return false;
+    if (!(tn < FF_FIN)) return FF_VR_NOTIME;
     double d[4];
     int r = 0;
     const int nz_[4] = {z, z, z - 1, z + 1}, nx_[4] = {x - 1, x + 1, x, x};
@@ -1745,7 +1771,7 @@ __device__ __forceinline__ bool ff_verify_node(const fview &V, const FX &fixed, 
         const int zz = nz_[k], xx = nx_[k];
         if (zz < 1 || zz > V.nnz || xx < 1 || xx > V.nnx) continue;
         const double t = V.t(zz, xx);
-        if (t == tn) return false;
+        if (t == tn) return FF_VR_TIE;
         if (t < tn && !fixed(zz, xx)) {
             int i = r++;
             while (i > 0 && d[i - 1] > t) { d[i] = d[i - 1]; i--; }
@@ -1753,17 +1779,17 @@ __device__ __forceinline__ bool ff_verify_node(const fview &V, const FX &fixed, 
         }
     }
     for (int i = 1; i < r; i++)
-        if (d[i] == d[i - 1]) return false;
+        if (d[i] == d[i - 1]) return FF_VR_TIE;
     double v = w0;
     for (int i = 0; i < r; i++) {
-        if (!(v > d[i])) return false;
+        if (!(v > d[i])) return FF_VR_EARLY;
         const double thr = d[i];
         auto al = [&](int zz, int xx) -> bool { return fixed(zz, xx) || V.t(zz, xx) <= thr; };
         const double vn = FOM == 0 ? ff_fouds1(V, al, z, x, slown) : ff_fouds2(V, al, z, x, slown);
-        if (vn > v && vn - v > 1e-12 * vn) return false;
+        if (vn > v && vn - v > 1e-12 * vn) return FF_VR_RISE;
         v = vn;
     }
-    return v == tn || fabs(v - tn) <= 1e-12 * tn;
+    return (v == tn || fabs(v - tn) <= 1e-12 * tn) ? 0 : FF_VR_FINAL;
 }
 
 // One workgroup per problem over the converged coarse field; a problem that
@@ -1812,12 +1838,13 @@ __global__ __launch_bounds__(256) void k_fm2d_verify(mct_fm2d_args A, int32_t *p
         const int x = (int)(q / nnz) + 1, z = (int)(q % nnz) + 1;
         if (fixed(z, x)) continue;
         const double w0 = win.in(z, x) ? Wb[(long)(x - 1) * ld + (z - 1)] : FF_BIG;
-        if (!ff_verify_node<FOM>(V, fixed, w0, z, x, 1.0 / velnc[(long)(x - 1) * nnz + (z - 1)])) bad = 1;
+        const int why = ff_verify_node<FOM>(V, fixed, w0, z, x, 1.0 / velnc[(long)(x - 1) * nnz + (z - 1)]);
+        if (why) atomicOr(&bad, why);
     }
     __syncthreads();
     if (threadIdx.x == 0 && bad) {
         pflag[slot] = 1;
-        if (A.rounds_out) A.rounds_out[4 * slot + 3] = 1;
+        if (A.rounds_out) A.rounds_out[4 * slot + 3] = bad;
     }
 }
 
@@ -1914,15 +1941,15 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
         MCT_CHECK(hipGetLastError());
         if (a.ev_sweep && p0 == 0) MCT_CHECK(hipEventRecord(a.ev_sweep, s));
         if (strip_lds && a.fom == 0)
-            hipLaunchKernelGGL(k_fm2d_strip<0>, dim3((unsigned)b.nrun), dim3(64), strip_lds, s, b, (const int32_t *)pflag);
+            hipLaunchKernelGGL(k_fm2d_strip<0>, dim3((unsigned)b.nrun), dim3(64), strip_lds, s, b, pflag);
         else if (strip_lds)
-            hipLaunchKernelGGL(k_fm2d_strip<1>, dim3((unsigned)b.nrun), dim3(64), strip_lds, s, b, (const int32_t *)pflag);
+            hipLaunchKernelGGL(k_fm2d_strip<1>, dim3((unsigned)b.nrun), dim3(64), strip_lds, s, b, pflag);
         else if (a.fom == 0)
             hipLaunchKernelGGL(k_fm2d_sweep<0>, dim3((unsigned)b.nrun), dim3(sweep_threads), dirty_lds, s, b,
-                               (const int32_t *)pflag, use_dirty);
+                               pflag, use_dirty);
         else
             hipLaunchKernelGGL(k_fm2d_sweep<1>, dim3((unsigned)b.nrun), dim3(sweep_threads), dirty_lds, s, b,
-                               (const int32_t *)pflag, use_dirty);
+                               pflag, use_dirty);
         MCT_CHECK(hipGetLastError());
         if (verify > 0) {
             const size_t vlds = 4 * (size_t)ff_win_words(a);

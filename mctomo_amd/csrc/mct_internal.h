@@ -61,6 +61,7 @@ struct mct_nearest_args {
     const double *pm;              // {vp,vs,rho} or NULL
     // outputs (any may be NULL)
     float *vel_tiles;              // (nchain, nwave, ntiles*64) eikonal velocity
+    double *vel_tiles64;           // the same in FP64 (either may be NULL)
     int nwave;                     // 1: vp only, 2: vp and vs
     int32_t *sites_id;             // (nchain, nz*ny*nx) Fortran order
     double *vp, *vs, *rho;         // (nchain, nz*ny*nx) Fortran order
@@ -83,7 +84,8 @@ struct mct_eik_problem_set {
     int order;
     void *T;                       // (nprob, ntiles*64) arrival times, float or double
     int t64;                       // 1: arrival times stored as double
-    const float *vel;              // (nchain*nwave, ntiles*64)
+    const float *vel;              // (nchain*nwave, ntiles*64), or NULL with vel64
+    const double *vel64;           // the same in FP64: local solves in FP64 (needs t64), or NULL
     uint8_t *flags[2];             // active-tile flags of the current / next round
     uint32_t *key[2];              // per tile: smallest activating time (float bits)
     uint32_t *any;                 // [3] rotating "some tile is active" flags
@@ -99,10 +101,11 @@ struct mct_eik_problem_set {
 int mct_launch_eik_init(const mct_eik_problem_set &P, const double *src_pts, int src_stride,
                         long src_chain_stride, double xmin, double ymin, double zmin,
                         double lx, double ly, double lz, hipStream_t s);
-int mct_launch_eik_round(const mct_eik_problem_set &P, int round, float hx, float hy, float hz,
+int mct_launch_eik_round(const mct_eik_problem_set &P, int round, double hx, double hy, double hz,
                          int grid_blocks, hipStream_t s);
 // band_dt[cw] = tiles_per_round * 4 * min(h) / max velocity of (chain, wave) cw
-int mct_launch_band_dt(const float *vel, long tsz, int ncw, float hmin, float tiles_per_round,
+// (from vel64 when given, else vel)
+int mct_launch_band_dt(const float *vel, const double *vel64, long tsz, int ncw, float hmin, float tiles_per_round,
                        float *band_dt, hipStream_t s);
 
 // Band schedule: the first band rounds advance tau by band_dt per round; then
@@ -116,6 +119,8 @@ static inline int mct_band_rounds(const mct_geom &G, float band_tiles)
 float mct_default_band_tiles();
 // Arrival-time storage: double unless the environment sets MCT_EIK_F32=1.
 int mct_default_t64();
+// FP64 velocities and local solves: on unless the environment sets MCT_EIK_F64=0.
+int mct_default_f64();
 
 // ---- picks + misfit -------------------------------------------------------------
 struct mct_pick_args {

@@ -57,6 +57,12 @@ __device__ __forceinline__ void write_node(const mct_nearest_args &a, int c, int
         a.vel_tiles[((long)c * a.nwave + 0) * wsz + tl] = (float)pr[0];
         if (a.nwave > 1) a.vel_tiles[((long)c * a.nwave + 1) * wsz + tl] = (float)pr[1];
     }
+    if (a.vel_tiles64) {
+        long tl = tile_lane_of_node(G, x, y, z);
+        long wsz = (long)G.ntiles * MCT_TILE_N;
+        a.vel_tiles64[((long)c * a.nwave + 0) * wsz + tl] = pr[0];
+        if (a.nwave > 1) a.vel_tiles64[((long)c * a.nwave + 1) * wsz + tl] = pr[1];
+    }
 }
 
 __device__ __forceinline__ bool in_box(const mct_nearest_args &a, int x, int y, int z)
@@ -90,10 +96,13 @@ __global__ __launch_bounds__(NEAR_BLOCK) void k_nearest_brute(mct_nearest_args a
         node_of_tile_lane(G, e, x, y, z);
         if (x < G.nx && y < G.ny && z < G.nz) {
             live = in_box(a, x, y, z) && pm_selects(a, c, x, y, z);
-        } else if (a.vel_tiles) {
+        } else {
             // padding lanes of edge tiles: never solved; zero keeps them out of max(v)
             long wsz = (long)G.ntiles * MCT_TILE_N;
-            for (int w = 0; w < a.nwave; w++) a.vel_tiles[((long)c * a.nwave + w) * wsz + e] = 0.0f;
+            if (a.vel_tiles)
+                for (int w = 0; w < a.nwave; w++) a.vel_tiles[((long)c * a.nwave + w) * wsz + e] = 0.0f;
+            if (a.vel_tiles64)
+                for (int w = 0; w < a.nwave; w++) a.vel_tiles64[((long)c * a.nwave + w) * wsz + e] = 0.0;
         }
     }
     const double qx = G.xmin + (double)x * G.dx;

@@ -925,14 +925,17 @@ int mct_surf_fm2d_handover(mct_surf_ctx *C, int64_t *handover2)
         free(r);
         return MCT_ERR_HIP;
     }
-    int64_t chk = 0, all = 0;
+    int64_t chk = 0, all = 0, why[7] = {0, 0, 0, 0, 0, 0, 0};
     for (long i = 0; i < n; i++) {
-        chk += r[4 * i + 3] == 1;
+        const int b = r[4 * i + 3];
+        chk += b != 0;
         all += r[4 * n + i] == 1;
+        for (int k = 0; k < 7; k++) why[k] += (b >> k) & 1;
     }
     free(r);
     handover2[0] = chk;
     handover2[1] = all;
+    for (int k = 0; k < 7; k++) handover2[2 + k] = why[k];
     return MCT_OK;
 }
 
