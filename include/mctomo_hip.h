@@ -33,7 +33,8 @@ enum {
     MCT_ERR_SIGMA = -4,       /* noise level below EPS ('The noise level is 0!') */
     MCT_ERR_NOCONV = -5,      /* eikonal solver hit its iteration cap */
     MCT_ERR_OUTSIDE = -6,     /* source or receiver outside the grid */
-    MCT_ERR_IO = -7           /* a data file could not be opened, read or written */
+    MCT_ERR_IO = -7,          /* a data file could not be opened, read or written */
+    MCT_ERR_SCRATCH = -8      /* the allocation-free entry's scratch cannot hold the request */
 };
 
 /* T_GRID, src/settings.f90:20-28.  zmin/dz are the SCALED values held by the
@@ -104,6 +105,30 @@ int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dvxd, double 
                 int fom, int nsrc, const double *scx, const double *scz, int nrc,
                 const double *rcx, const double *rcz, const int32_t *srs, double *ttime,
                 int method);
+
+/* Device bytes of the allocation-free entry's scratch array (one per GPU). */
+#define MCT_NOALLOC_SCRATCH_BYTES (128L << 20)
+
+/* No counterpart in the reference: mct_modrays for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The same computation as mct_modrays with
+ * every array a DEVICE pointer of the caller's: velv_dev, scx/scz_dev,
+ * rcx/rcz_dev, srs_dev as there, ttime_dev receives the marked pairs' times
+ * and status_dev (one int32) the status (0, or MCT_ERR_OUTSIDE where modrays
+ * stops).  Queued on `stream` (a hipStream_t, NULL the default stream) and
+ * returns without waiting.  The solver's workspace is one device array of
+ * MCT_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; problems
+ * run in chunks of what it holds, and MCT_ERR_SCRATCH is returned (nothing
+ * launched) when the coarse velocities and one problem's workspace do not
+ * fit.  Nothing is allocated.  Calls that overlap on one GPU share the array
+ * and race; each call leaves it holding its own state. */
+int mct_modrays_dev_noalloc(int nvx, int nvz, double goxd, double gozd, double dvxd, double dvzd,
+                            int np, const double *velv_dev, int gdx, int gdz, int asgr, int sgdl, int sgs,
+                            int fom, int nsrc, const double *scx_dev, const double *scz_dev, int nrc,
+                            const double *rcx_dev, const double *rcz_dev, const int32_t *srs_dev,
+                            double *ttime_dev, int32_t *status_dev, int method, void *stream);
+/* MCT_NOALLOC_SCRATCH_BYTES as the library was built with it. */
+int mct_noalloc_scratch_bytes(int64_t *bytes);
 
 /* Problems (period, source) the calling thread's last mct_modrays call handed
  * from the parallel path to the sequential solver.  No counterpart in the

@@ -79,7 +79,8 @@ def lib():
                      "mct_fm2d_last_handover", "mct_surf_fm2d_handover", "mct_dispersion_columns",
                      "mct_body_stream", "mct_joint_create", "mct_joint_destroy", "mct_joint_parts",
                      "mct_joint_likelihood", "mct_joint_likelihood_batch_dev", "mct_joint_sync",
-                     "mct_body_noise_likelihood_from", "mct_surf_noise_likelihood_from"):
+                     "mct_body_noise_likelihood_from", "mct_surf_noise_likelihood_from",
+                     "mct_modrays_dev_noalloc", "mct_noalloc_scratch_bytes"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
