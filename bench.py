@@ -276,6 +276,7 @@ def run_surface(args):
     if args.served > 0:
         srv = mct.SurfServer(w.grid, w.freqs, w.src, w.rev, w.ttime, w.raystat, max_batch=min(B, 32), nlanes=2,
                              raylov=1, order=args.order, max_cells=cmax)
+        srv.set_fill(args.served_fill_ms)
         reqs = [(P[c % B, :N[c % B]], Q[c % B, :N[c % B]]) for c in range(args.served)]
         for t in [srv.submit(*reqs[i]) for i in range(min(B, len(reqs)))]:
             srv.wait(t)                                   # warm-up
@@ -307,23 +308,28 @@ def run_surface(args):
                   "arrival_rate_per_s": round(rate, 3),
                   "evals_per_s": round(len(reqs) / t_all, 3),
                   "p50_ms": round(float(np.median(lat_s)), 3), "p99_ms": round(float(np.quantile(lat_s, 0.99)), 3),
-                  "batches": int(nb - 1), "lanes": 2, "max_batch": min(B, 32)}
+                  "batches": int(nb - 1), "lanes": 2, "max_batch": min(B, 32), "fill_ms": args.served_fill_ms}
         log("served:", served)
 
-    # ---- served mix: three grids in one server -----------------------------------
-    # Requests for three data sets (51x51x21 with 10 periods and 16x16 pairs,
-    # 101x101x41 with 20 periods and 32x32, and the 201x201x61 one above),
-    # Voronoi cell counts U{10..800}, the model of each request uniform at
-    # random; Poisson arrivals at 0.8x the capacity a closed burst measures.
+    # ---- served mix: three grids in one server, 1,024 chains -------------------
+    # One request from each of 1,024 chains, each chain on one of three data
+    # sets (51x51x21 with 10 periods and 16x16 pairs, 101x101x41 with 20 periods
+    # and 32x32, 201x201x61 with 30 periods and 32x32), Voronoi cell counts
+    # U{10..800}.  A closed burst of 192 requests measures the server's
+    # capacity; then the 1,024 requests arrive as a Poisson process at 0.8x it.
+    # Sustained = requests / (last result - first arrival).
     mixres = None
     if args.served_mix > 0:
         from mctomo_amd.workloads import surface_workload as sw_
         wm = [sw_(51, 21, 25.0, 10.0, 50, 10, (1.0, 5.0), 16, 16, seed=20261210, order=args.order),
-              sw_(101, 41, 50.0, 20.0, 100, 20, (1.0, 10.0), 32, 32, seed=20261211, order=args.order), w]
+              sw_(101, 41, 50.0, 20.0, 100, 20, (1.0, 10.0), 32, 32, seed=20261211, order=args.order),
+              sw_(201, 61, 100.0, 30.0, 200, 30, (1.0, 10.0), 32, 32, seed=20261212, order=args.order)]
         mixc = 800
+        mbs = (64, 64, 32)
         specs = [dict(grid=x.grid, freqs=x.freqs, src=x.src, rev=x.rev, ttime=x.ttime, raystat=x.raystat,
-                      raylov=1, order=args.order, max_cells=mixc, max_batch=16) for x in wm]
-        srvm = mct.SurfServer.mix(specs, max_lanes=2)
+                      raylov=1, order=args.order, max_cells=mixc, max_batch=mb) for x, mb in zip(wm, mbs)]
+        srvm = mct.SurfServer.mix(specs, max_lanes=2, hbm_budget=args.served_hbm_gb << 30)
+        srvm.set_fill(args.served_fill_ms)
         rng_m = np.random.default_rng(99)
 
         def draw(m):
@@ -332,23 +338,26 @@ def run_surface(args):
             return _surface_sites(rng_m, k, g_.dx * (g_.nx - 1), g_.dz * (g_.nz - 1), (2.5, 4.5))
 
         nreq = args.served_mix
-        mreq = [int(v) for v in rng_m.integers(0, 3, 2 * nreq)]
+        nburst = 192
+        mreq = [int(v) for v in rng_m.integers(0, 3, nburst + nreq)]
         reqs_m = [(m,) + draw(m) for m in mreq]
-        burst, live = reqs_m[:nreq], reqs_m[nreq:]
+        burst, live = reqs_m[:nburst], reqs_m[nburst:]
+        for t in [srvm.submit(p_, q_, model=m) for m, p_, q_ in burst[:48]]:
+            srvm.wait(t)                                  # lanes built, first graphs captured
         t0 = time.perf_counter()
         for t in [srvm.submit(p_, q_, model=m) for m, p_, q_ in burst]:
             srvm.wait(t)
         cap = len(burst) / (time.perf_counter() - t0)
         rate_m = 0.8 * cap
-        gaps = rng_m.exponential(1.0 / rate_m, len(live))
+        arrive = np.cumsum(np.random.default_rng(7).exponential(1.0 / rate_m, len(live)))
         tickets = []
         t0 = time.perf_counter()
         for i, (m, p_, q_) in enumerate(live):
-            t_next = t0 + gaps[:i + 1].sum()
             now = time.perf_counter()
-            if t_next > now:
-                time.sleep(t_next - now)
+            if t0 + arrive[i] > now:
+                time.sleep(t0 + arrive[i] - now)
             tickets.append((m, srvm.submit(p_, q_, model=m)))
+        t_sub = time.perf_counter() - t0
         res_m = [(m, srvm.wait(t)) for m, t in tickets]
         t_all = time.perf_counter() - t0
         used_b, peak_b, lanes_m, rel_m = srvm.memory()
@@ -358,16 +367,21 @@ def run_surface(args):
             raise RuntimeError("served mix rc %d" % rc_m)
         lat_m = np.array([r[2] for _, r in res_m])
         per = {}
-        for mm, name in enumerate(("51x51x21", "101x101x41", "201x201x61")):
+        for mm, name in enumerate(("51x51x21/10", "101x101x41/20", "201x201x61/30")):
             lm = np.array([r[2] for m, r in res_m if m == mm])
             if lm.size:
                 per[name] = {"requests": int(lm.size), "p50_ms": round(float(np.median(lm)), 3),
                              "p99_ms": round(float(np.quantile(lm, 0.99)), 3)}
-        mixres = {"requests": len(live), "closed_burst_capacity_per_s": round(cap, 3),
-                  "arrival_rate_per_s": round(rate_m, 3), "evals_per_s": round(len(live) / t_all, 3),
+        sustained = len(live) / t_all
+        offered = len(live) / t_sub
+        mixres = {"chains": len(live), "closed_burst_capacity_per_s": round(cap, 3),
+                  "arrival_rate_per_s": round(rate_m, 3), "offered_per_s": round(offered, 3),
+                  "evals_per_s": round(sustained, 3), "sustained_over_offered": round(sustained / offered, 4),
                   "p50_ms": round(float(np.median(lat_m)), 3), "p99_ms": round(float(np.quantile(lat_m, 0.99)), 3),
                   "rejected_models": int(sum(r[0] > 0 for _, r in res_m)), "per_grid": per,
-                  "lanes": lanes_m, "device_bytes_peak": int(peak_b), "batches": int(nb_m)}
+                  "lanes": lanes_m, "max_batch": list(mbs), "fill_ms": args.served_fill_ms,
+                  "hbm_budget_gb": args.served_hbm_gb, "device_bytes_peak": int(peak_b),
+                  "lane_releases": int(rel_m), "batches": int(nb_m)}
         log("served mix:", mixres)
 
     # ---- roofline of the dominant kernel, live HIP events ------------------------
@@ -654,8 +668,11 @@ def main():
     ap.add_argument("--box-steps", type=int, default=3,
                     help="timed perturbed-box batches (secondary unit; 0: skip)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
-    ap.add_argument("--served-mix", type=int, default=96,
-                    help="requests of the three-grid served mix (0: skip)")
+    ap.add_argument("--served-mix", type=int, default=1024,
+                    help="chains (one request each) of the three-grid served mix (0: skip)")
+    ap.add_argument("--served-fill-ms", type=float, default=25.0,
+                    help="longest wait of a partial batch while another lane runs (< 0: none)")
+    ap.add_argument("--served-hbm-gb", type=int, default=200, help="device bytes the served lanes may hold")
     ap.add_argument("--group-steps", type=int, default=2,
                     help="timed group-velocity batches (secondary figure; 0: skip)")
     ap.add_argument("--order1-steps", type=int, default=3,

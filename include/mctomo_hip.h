@@ -554,6 +554,11 @@ int mct_serve_submit(mct_server *srv, const double *points, const double *params
 int mct_serve_wait(mct_server *srv, int64_t ticket, double *out, int32_t *status, double *latency_ms);
 /* Batches launched and evaluations finished so far; returns the first error. */
 int mct_serve_stats(mct_server *srv, int64_t *batches, int64_t *evals);
+/* Batch fill (no counterpart in the reference): while some lane is running, a
+ * batch smaller than its lane's max_batch waits until its oldest request has
+ * waited fill_ms (default 25); with no lane running it is launched at once.
+ * fill_ms < 0 launches whatever is pending as soon as a lane is free. */
+int mct_serve_set_fill(mct_server *srv, double fill_ms);
 /* A server over a mix of models (no counterpart in the reference): model m is
  * a data set on its own grid (grids[m], setups[m]) with lanes of max_batch[m]
  * chains, at most max_lanes lanes per model.  Lanes are admitted by device

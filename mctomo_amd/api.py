@@ -80,7 +80,7 @@ def lib():
                      "mct_body_stream", "mct_joint_create", "mct_joint_destroy", "mct_joint_parts",
                      "mct_joint_likelihood", "mct_joint_likelihood_batch_dev", "mct_joint_sync",
                      "mct_body_noise_likelihood_from", "mct_surf_noise_likelihood_from",
-                     "mct_modrays_dev_noalloc", "mct_noalloc_scratch_bytes"):
+                     "mct_modrays_dev_noalloc", "mct_noalloc_scratch_bytes", "mct_serve_set_fill"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -881,6 +881,11 @@ class SurfServer:
         _check(lib().mct_serve_submit_to(self._h, int(model), _p(pts), _p(prm), pts.shape[0], _p(s0), _p(s1),
                                          ctypes.byref(t)), "mct_serve_submit_to")
         return t.value
+
+    def set_fill(self, fill_ms):
+        """Longest wait (ms) of a partial batch while another lane runs; < 0:
+        launch whatever is pending as soon as a lane is free."""
+        _check(lib().mct_serve_set_fill(self._h, ctypes.c_double(float(fill_ms))), "mct_serve_set_fill")
 
     def memory(self):
         """(device bytes in use, peak bytes, lanes, lane releases)."""

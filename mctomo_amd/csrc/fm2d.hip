@@ -59,9 +59,18 @@ __global__ void k_fm2d_gridder(mct_fm2d_args A)
         sumi = sumi + vi[i1 - 1] * sumj;
     }
     A.veln[(long)cp * nn + (long)(stx - 1) * A.nnz + (stz - 1)] = sumi;
+    // the slowness fouds takes (slown = 1.0 / veln), once per node instead of per update
+    if (A.sln) A.sln[(long)cp * nn + (long)(stx - 1) * A.nnz + (stz - 1)] = 1.0 / sumi;
 }
 
 // ---- per-lane sequential solver ----------------------------------------------------
+// One lane runs one problem's heap loop in global memory, so a pop is a chain
+// of dependent loads.  Two things keep that chain short: the heap holds each
+// entry's key beside its node (hk[p] == the node's time for every position p,
+// so a sift level is one load, not two), and a pop reads its four neighbours'
+// stencils before it moves the heap, all at once.  The heap operations are the
+// reference's (same comparisons, same ties, same final arrangement: a moving
+// hole instead of pairwise swaps).
 struct lane_grid {
     int nnx, nnz, ld;
     double gox, goz, dnx, dnz;
@@ -70,6 +79,7 @@ struct lane_grid {
     const double *v;        // velocity
     int vld;                // leading dimension of v
     int32_t *btg;           // heap, 1-based
+    double *hk;             // heap keys, hk[p] = time of btg[p]
     int ntr, fom;
     int vnl, vnr, vnt, vnb;
 };
@@ -80,50 +90,47 @@ struct lane_grid {
 #define BPX(e) ((e) >> 16)
 #define BPZ(e) ((e) & 0xffff)
 
-__device__ static void l_addtree(lane_grid &G, int iz, int ix)
+// addtree (fm2d_ttime.f90:527-564) for node (iz,ix) with time tv
+__device__ static void l_addtree(lane_grid &G, int iz, int ix, double tv)
 {
     G.ntr = G.ntr + 1;
-    LS(G, iz, ix) = G.ntr;
-    G.btg[G.ntr] = (ix << 16) | iz;
     int tpc = G.ntr, tpp = tpc / 2;
-    const double tv = LT(G, iz, ix);
     while (tpp > 0) {
-        int ep = G.btg[tpp];
-        if (tv < LT(G, BPZ(ep), BPX(ep))) {
-            LS(G, iz, ix) = tpp;
+        const int ep = G.btg[tpp];
+        const double kp = G.hk[tpp];
+        if (tv < kp) {           // the parent moves down into the hole
             LS(G, BPZ(ep), BPX(ep)) = tpc;
-            G.btg[tpp] = G.btg[tpc];
             G.btg[tpc] = ep;
+            G.hk[tpc] = kp;
             tpc = tpp;
             tpp = tpc / 2;
         } else {
             tpp = 0;
         }
     }
+    LS(G, iz, ix) = tpc;
+    G.btg[tpc] = (ix << 16) | iz;
+    G.hk[tpc] = tv;
 }
 
+// downtree (fm2d_ttime.f90:575-644): remove the root
 __device__ static void l_downtree(lane_grid &G)
 {
     if (G.ntr == 1) { G.ntr = G.ntr - 1; return; }
-    int el = G.btg[G.ntr];
-    LS(G, BPZ(el), BPX(el)) = 1;
-    G.btg[1] = el;
+    const int el = G.btg[G.ntr];
+    const double kl = G.hk[G.ntr];
     G.ntr = G.ntr - 1;
     int tpp = 1, tpc = 2 * tpp;
     while (tpc < G.ntr) {
-        int ec = G.btg[tpc], ec1 = G.btg[tpc + 1];
-        double rd1 = LT(G, BPZ(ec), BPX(ec));
-        double rd2 = LT(G, BPZ(ec1), BPX(ec1));
-        if (rd1 > rd2) tpc = tpc + 1;
-        ec = G.btg[tpc];
-        int ep = G.btg[tpp];
-        rd1 = LT(G, BPZ(ec), BPX(ec));
-        rd2 = LT(G, BPZ(ep), BPX(ep));
-        if (rd1 < rd2) {
-            LS(G, BPZ(ep), BPX(ep)) = tpc;
+        const int e1 = G.btg[tpc], e2 = G.btg[tpc + 1];
+        const double k1 = G.hk[tpc], k2 = G.hk[tpc + 1];
+        int ec = e1;
+        double kc = k1;
+        if (k1 > k2) { tpc = tpc + 1; ec = e2; kc = k2; }
+        if (kc < kl) {           // the smaller child moves up into the hole
             LS(G, BPZ(ec), BPX(ec)) = tpp;
-            G.btg[tpc] = ep;
             G.btg[tpp] = ec;
+            G.hk[tpp] = kc;
             tpp = tpc;
             tpc = 2 * tpp;
         } else {
@@ -131,66 +138,110 @@ __device__ static void l_downtree(lane_grid &G)
         }
     }
     if (tpc == G.ntr) {
-        int ec = G.btg[tpc], ep = G.btg[tpp];
-        double rd1 = LT(G, BPZ(ec), BPX(ec));
-        double rd2 = LT(G, BPZ(ep), BPX(ep));
-        if (rd1 < rd2) {
-            LS(G, BPZ(ep), BPX(ep)) = tpc;
+        const int ec = G.btg[tpc];
+        const double kc = G.hk[tpc];
+        if (kc < kl) {
             LS(G, BPZ(ec), BPX(ec)) = tpp;
-            G.btg[tpc] = ep;
             G.btg[tpp] = ec;
+            G.hk[tpp] = kc;
+            tpp = tpc;
         }
     }
+    LS(G, BPZ(el), BPX(el)) = tpp;
+    G.btg[tpp] = el;
+    G.hk[tpp] = kl;
 }
 
-__device__ static void l_updtree(lane_grid &G, int iz, int ix)
+// updtree (fm2d_ttime.f90:653-680): node (iz,ix), already in the heap, has the
+// new time tv; it moves up while smaller than its parent
+__device__ static void l_updtree(lane_grid &G, int iz, int ix, double tv)
 {
     int tpc = LS(G, iz, ix), tpp = tpc / 2;
-    const double tv = LT(G, iz, ix);
     while (tpp > 0) {
-        int ep = G.btg[tpp];
-        if (tv < LT(G, BPZ(ep), BPX(ep))) {
-            LS(G, iz, ix) = tpp;
+        const int ep = G.btg[tpp];
+        const double kp = G.hk[tpp];
+        if (tv < kp) {
             LS(G, BPZ(ep), BPX(ep)) = tpc;
-            G.btg[tpp] = G.btg[tpc];
             G.btg[tpc] = ep;
+            G.hk[tpc] = kp;
             tpc = tpp;
             tpp = tpc / 2;
         } else {
             tpp = 0;
         }
     }
+    LS(G, iz, ix) = tpc;
+    G.btg[tpc] = (ix << 16) | iz;
+    G.hk[tpc] = tv;
 }
 
-__device__ static void l_fouds1(lane_grid &G, int iz, int ix)
+// A node's axis stencil read ahead: times one and two nodes away on each side
+// ([0] the x-1 / z-1 side), whether each is inside the grid and alive, and
+// the node's velocity.  Out-of-grid entries read the node itself.
+struct l_sten {
+    double tx1[2], tx2[2], tz1[2], tz2[2], v;
+    bool ix1[2], iz1[2];
+    bool ax1[2], ax2[2], az1[2], az2[2];
+};
+
+__device__ __forceinline__ void l_load_sten(const lane_grid &G, int iz, int ix, l_sten &S)
+{
+    int sx1[2], sx2[2], sz1[2], sz2[2];
+#pragma unroll
+    for (int sd = 0; sd < 2; sd++) {
+        const int d = sd ? 1 : -1;
+        const int x1 = ix + d, x2 = ix + 2 * d, z1 = iz + d, z2 = iz + 2 * d;
+        S.ix1[sd] = x1 >= 1 && x1 <= G.nnx;
+        S.iz1[sd] = z1 >= 1 && z1 <= G.nnz;
+        const bool ix2 = G.fom != 0 && x2 >= 1 && x2 <= G.nnx, iz2 = G.fom != 0 && z2 >= 1 && z2 <= G.nnz;
+        const int cx1 = S.ix1[sd] ? x1 : ix, cz1 = S.iz1[sd] ? z1 : iz;
+        const int cx2 = ix2 ? x2 : ix, cz2 = iz2 ? z2 : iz;
+        S.tx1[sd] = LT(G, iz, cx1);
+        S.tz1[sd] = LT(G, cz1, ix);
+        sx1[sd] = LS(G, iz, cx1);
+        sz1[sd] = LS(G, cz1, ix);
+        S.tx2[sd] = G.fom ? LT(G, iz, cx2) : 0.0;
+        S.tz2[sd] = G.fom ? LT(G, cz2, ix) : 0.0;
+        sx2[sd] = G.fom ? LS(G, iz, cx2) : 1;
+        sz2[sd] = G.fom ? LS(G, cz2, ix) : 1;
+        S.ax1[sd] = S.ix1[sd] && sx1[sd] == 0;
+        S.az1[sd] = S.iz1[sd] && sz1[sd] == 0;
+        S.ax2[sd] = ix2 && sx2[sd] == 0;
+        S.az2[sd] = iz2 && sz2[sd] == 0;
+    }
+    S.v = LV(G, iz, ix);
+}
+
+// fouds1 (fm2d_ttime.f90:248-332) over a read-ahead stencil
+__device__ static double l_fouds1s(const l_sten &S, double dnx, double dnz)
 {
     int tsw1 = 0;
     double travm = 0.0;
-    const double slown = 1.0 / LV(G, iz, ix);
-    for (int j = ix - 1; j <= ix + 1; j += 2)
-        for (int k = iz - 1; k <= iz + 1; k += 2) {
-            if (!(j >= 1 && j <= G.nnx && k >= 1 && k <= G.nnz)) continue;
+    const double slown = 1.0 / S.v;
+    for (int sj = 0; sj < 2; sj++)
+        for (int sk = 0; sk < 2; sk++) {
+            if (!(S.ix1[sj] && S.iz1[sk])) continue;
             int swsol = 0;
             double a = 0, b = 0, c = 0, tref = 0;
-            if (LS(G, iz, j) == 0) {
+            if (S.ax1[sj]) {
                 swsol = 1;
-                if (LS(G, k, ix) == 0) {
-                    double u = G.dnx, v = G.dnz;
-                    double em = LT(G, k, ix) - LT(G, iz, j);
+                if (S.az1[sk]) {
+                    double u = dnx, v = dnz;
+                    double em = S.tz1[sk] - S.tx1[sj];
                     a = u * u + v * v;
                     b = -(2.0 * (u * u) * em);
                     c = (u * u) * (em * em - (v * v) * (slown * slown));
-                    tref = LT(G, iz, j);
+                    tref = S.tx1[sj];
                 } else {
                     a = 1.0; b = 0.0;
-                    c = -((slown * slown) * (G.dnx * G.dnx));
-                    tref = LT(G, iz, j);
+                    c = -((slown * slown) * (dnx * dnx));
+                    tref = S.tx1[sj];
                 }
-            } else if (LS(G, k, ix) == 0) {
+            } else if (S.az1[sk]) {
                 swsol = 1;
                 a = 1.0; b = 0.0;
-                c = -((slown * G.dnz) * (slown * G.dnz));
-                tref = LT(G, k, ix);
+                c = -((slown * dnz) * (slown * dnz));
+                tref = S.tz1[sk];
             }
             if (swsol == 1) {
                 double rd1 = b * b - 4.0 * a * c;
@@ -201,43 +252,32 @@ __device__ static void l_fouds1(lane_grid &G, int iz, int ix)
                 else { travm = trav; tsw1 = 1; }
             }
         }
-    LT(G, iz, ix) = travm;
+    return travm;
 }
 
-__device__ static void l_fouds2(lane_grid &G, int iz, int ix)
+// fouds2 (fm2d_ttime.f90:342-518) over a read-ahead stencil
+__device__ static double l_fouds2s(const l_sten &S, double dnx, double dnz)
 {
     int tsw1 = 0;
     double travm = 0.0;
-    const double slown = 1.0 / LV(G, iz, ix);
-    for (int j = ix - 1; j <= ix + 1; j += 2) {
-        if (!(j >= 1 && j <= G.nnx)) continue;
-        int swj = -1, j2;
-        if (j == ix - 1) {
-            j2 = j - 1;
-            if (j2 >= 1 && LS(G, iz, j2) == 0) swj = 0;
-        } else {
-            j2 = j + 1;
-            if (j2 <= G.nnx && LS(G, iz, j2) == 0) swj = 0;
-        }
-        if (LS(G, iz, j) == 0 && swj == 0) {
+    const double slown = 1.0 / S.v;
+    for (int sj = 0; sj < 2; sj++) {
+        if (!S.ix1[sj]) continue;
+        const double tj = S.tx1[sj], tj2 = S.tx2[sj];
+        int swj = S.ax2[sj] ? 0 : -1;
+        if (S.ax1[sj] && swj == 0) {
             swj = -1;
-            if (LT(G, iz, j) > LT(G, iz, j2)) swj = 0;
+            if (tj > tj2) swj = 0;
         } else {
             swj = -1;
         }
-        for (int k = iz - 1; k <= iz + 1; k += 2) {
-            if (!(k >= 1 && k <= G.nnz)) continue;
-            int swk = -1, k2;
-            if (k == iz - 1) {
-                k2 = k - 1;
-                if (k2 >= 1 && LS(G, k2, ix) == 0) swk = 0;
-            } else {
-                k2 = k + 1;
-                if (k2 <= G.nnz && LS(G, k2, ix) == 0) swk = 0;
-            }
-            if (LS(G, k, ix) == 0 && swk == 0) {
+        for (int sk = 0; sk < 2; sk++) {
+            if (!S.iz1[sk]) continue;
+            const double tk = S.tz1[sk], tk2 = S.tz2[sk];
+            int swk = S.az2[sk] ? 0 : -1;
+            if (S.az1[sk] && swk == 0) {
                 swk = -1;
-                if (LT(G, k, ix) > LT(G, k2, ix)) swk = 0;
+                if (tk > tk2) swk = 0;
             } else {
                 swk = -1;
             }
@@ -246,70 +286,70 @@ __device__ static void l_fouds2(lane_grid &G, int iz, int ix)
             if (swj == 0) {
                 swsol = 1;
                 if (swk == 0) {
-                    u = 2.0 * G.dnx;
-                    v = 2.0 * G.dnz;
-                    em = 4.0 * LT(G, iz, j) - LT(G, iz, j2) - 4.0 * LT(G, k, ix);
-                    em = em + LT(G, k2, ix);
+                    u = 2.0 * dnx;
+                    v = 2.0 * dnz;
+                    em = 4.0 * tj - tj2 - 4.0 * tk;
+                    em = em + tk2;
                     a = v * v + u * u;
                     b = 2.0 * em * (u * u);
                     c = (u * u) * (em * em - (slown * slown) * (v * v));
-                    tref = 4.0 * LT(G, iz, j) - LT(G, iz, j2);
+                    tref = 4.0 * tj - tj2;
                     tdiv = 3.0;
-                } else if (LS(G, k, ix) == 0) {
-                    u = G.dnz;
-                    v = 2.0 * G.dnx;
-                    em = 3.0 * LT(G, k, ix) - 4.0 * LT(G, iz, j) + LT(G, iz, j2);
+                } else if (S.az1[sk]) {
+                    u = dnz;
+                    v = 2.0 * dnx;
+                    em = 3.0 * tk - 4.0 * tj + tj2;
                     a = v * v + 9.0 * (u * u);
                     b = 6.0 * em * (u * u);
                     c = (u * u) * (em * em - (slown * slown) * (v * v));
-                    tref = LT(G, k, ix);
+                    tref = tk;
                     tdiv = 1.0;
                 } else {
-                    u = 2.0 * G.dnx;
+                    u = 2.0 * dnx;
                     a = 1.0; b = 0.0;
                     c = -((u * u) * (slown * slown));
-                    tref = 4.0 * LT(G, iz, j) - LT(G, iz, j2);
+                    tref = 4.0 * tj - tj2;
                     tdiv = 3.0;
                 }
-            } else if (LS(G, iz, j) == 0) {
+            } else if (S.ax1[sj]) {
                 swsol = 1;
                 if (swk == 0) {
-                    u = G.dnx;
-                    v = 2.0 * G.dnz;
-                    em = 3.0 * LT(G, iz, j) - 4.0 * LT(G, k, ix) + LT(G, k2, ix);
+                    u = dnx;
+                    v = 2.0 * dnz;
+                    em = 3.0 * tj - 4.0 * tk + tk2;
                     a = v * v + 9.0 * (u * u);
                     b = 6.0 * em * (u * u);
                     c = (u * u) * (em * em - (v * v) * (slown * slown));
-                    tref = LT(G, iz, j);
+                    tref = tj;
                     tdiv = 1.0;
-                } else if (LS(G, k, ix) == 0) {
-                    u = G.dnx;
-                    v = G.dnz;
-                    em = LT(G, k, ix) - LT(G, iz, j);
+                } else if (S.az1[sk]) {
+                    u = dnx;
+                    v = dnz;
+                    em = tk - tj;
                     a = u * u + v * v;
                     b = -(2.0 * (u * u) * em);
                     c = (u * u) * (em * em - (v * v) * (slown * slown));
-                    tref = LT(G, iz, j);
+                    tref = tj;
                     tdiv = 1.0;
                 } else {
                     a = 1.0; b = 0.0;
-                    c = -((slown * slown) * (G.dnx * G.dnx));
-                    tref = LT(G, iz, j);
+                    c = -((slown * slown) * (dnx * dnx));
+                    tref = tj;
                     tdiv = 1.0;
                 }
             } else {
                 if (swk == 0) {
                     swsol = 1;
-                    u = 2.0 * G.dnz;
+                    u = 2.0 * dnz;
                     a = 1.0; b = 0.0;
                     c = -((u * u) * (slown * slown));
-                    tref = 4.0 * LT(G, k, ix) - LT(G, k2, ix);
+                    tref = 4.0 * tk - tk2;
                     tdiv = 3.0;
-                } else if (LS(G, k, ix) == 0) {
+                } else if (S.az1[sk]) {
                     swsol = 1;
                     a = 1.0; b = 0.0;
-                    c = -((slown * slown) * (G.dnz * G.dnz));
-                    tref = LT(G, k, ix);
+                    c = -((slown * slown) * (dnz * dnz));
+                    tref = tk;
                     tdiv = 1.0;
                 }
             }
@@ -323,7 +363,7 @@ __device__ static void l_fouds2(lane_grid &G, int iz, int ix)
             }
         }
     }
-    LT(G, iz, ix) = travm;
+    return travm;
 }
 
 __device__ __forceinline__ double l_bilinear(const double nv[2][2], double dsx, double dsz, double dnx, double dnz)
@@ -349,7 +389,7 @@ __device__ static int l_travel(lane_grid &G, double scx, double scz, int urg)
     if (urg == 2) {
         for (int i = 1; i <= G.nnx; i++)
             for (int j = 1; j <= G.nnz; j++)
-                if (LS(G, j, i) > 0) l_addtree(G, j, i);
+                if (LS(G, j, i) > 0) l_addtree(G, j, i, LT(G, j, i));
     } else {
         for (int i = 1; i <= G.nnx; i++)
             for (int j = 1; j <= G.nnz; j++) LS(G, j, i) = -1;
@@ -363,8 +403,9 @@ __device__ static int l_travel(lane_grid &G, double scx, double scz, int urg)
             for (int j = 1; j <= 2; j++) {
                 double ex = dsx - (i - 1) * G.dnx, ez = dsz - (j - 1) * G.dnz;
                 double ds = sqrt(ex * ex + ez * ez);
-                LT(G, isz - 1 + j, isx - 1 + i) = 2.0 * ds / (vss[i - 1][j - 1] + vsrc);
-                l_addtree(G, isz - 1 + j, isx - 1 + i);
+                const double tc = 2.0 * ds / (vss[i - 1][j - 1] + vsrc);
+                LT(G, isz - 1 + j, isx - 1 + i) = tc;
+                l_addtree(G, isz - 1 + j, isx - 1 + i, tc);
             }
     }
     while (G.ntr > 0) {
@@ -379,30 +420,29 @@ __device__ static int l_travel(lane_grid &G, double scx, double scz, int urg)
             if (swrg == 1) { LS(G, iz, ix) = 0; break; }
         }
         LS(G, iz, ix) = 0;
-        l_downtree(G);
-        for (int i = ix - 1; i <= ix + 1; i += 2) {
-            if (i >= 1 && i <= G.nnx) {
-                int st = LS(G, iz, i);
-                if (st == -1) {
-                    if (G.fom == 0) l_fouds1(G, iz, i); else l_fouds2(G, iz, i);
-                    l_addtree(G, iz, i);
-                } else if (st > 0) {
-                    if (G.fom == 0) l_fouds1(G, iz, i); else l_fouds2(G, iz, i);
-                    l_updtree(G, iz, i);
-                }
-            }
+        // the four neighbours (x-1, x+1, z-1, z+1: the reference's order) and their
+        // stencils, read before the heap moves.  Processing one neighbour changes
+        // no alive flag and no alive node's time, which is all fouds reads, and
+        // far/close of the others stays as read; their heap positions are read
+        // again when updtree needs them.
+        const int nbx[4] = {ix - 1, ix + 1, ix, ix}, nbz[4] = {iz, iz, iz - 1, iz + 1};
+        int st[4];
+        l_sten S[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const bool in = nbx[k] >= 1 && nbx[k] <= G.nnx && nbz[k] >= 1 && nbz[k] <= G.nnz;
+            const int cx = in ? nbx[k] : ix, cz = in ? nbz[k] : iz;
+            st[k] = in ? LS(G, cz, cx) : 0;          // 0: alive or outside, nothing to do
+            l_load_sten(G, cz, cx, S[k]);
         }
-        for (int i = iz - 1; i <= iz + 1; i += 2) {
-            if (i >= 1 && i <= G.nnz) {
-                int st = LS(G, i, ix);
-                if (st == -1) {
-                    if (G.fom == 0) l_fouds1(G, i, ix); else l_fouds2(G, i, ix);
-                    l_addtree(G, i, ix);
-                } else if (st > 0) {
-                    if (G.fom == 0) l_fouds1(G, i, ix); else l_fouds2(G, i, ix);
-                    l_updtree(G, i, ix);
-                }
-            }
+        l_downtree(G);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (st[k] == 0) continue;
+            const double tv = G.fom == 0 ? l_fouds1s(S[k], G.dnx, G.dnz) : l_fouds2s(S[k], G.dnx, G.dnz);
+            LT(G, nbz[k], nbx[k]) = tv;
+            if (st[k] == -1) l_addtree(G, nbz[k], nbx[k], tv);
+            else l_updtree(G, nbz[k], nbx[k], tv);
         }
     }
     return 0;
@@ -434,6 +474,7 @@ __global__ __launch_bounds__(64) void k_fm2d_solve(mct_fm2d_args A)
     G.t = A.ttn + slot * wsz;
     G.s = A.nsts + slot * wsz;
     G.btg = A.heap + slot * (wsz + 8);
+    G.hk = A.heapk + slot * (wsz + 8);
     G.fom = A.fom;
     const double x = A.scx[s], z = A.scz[s];
     int isx = (int)((x - A.gox) / dnx) + 1, isz = (int)((z - A.goz) / dnz) + 1;
@@ -651,13 +692,14 @@ extern "C" int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dv
     const size_t nv = (size_t)(nvx + 2) * (nvz + 2) * np, nn = (size_t)a.nnx * a.nnz * np;
     a.nnx = (nvx - 1) * gdx + 1; a.nnz = (nvz - 1) * gdz + 1;   // for the LDS check below
     const bool fast = method == 0 && mct_fm2d_fast_ok(a);
-    void *bufs[16] = {nullptr};
-    size_t sizes[16] = {nv * 8, nn * 8, 8 * (size_t)nsrc, 8 * (size_t)nsrc, 8 * (size_t)nrc, 8 * (size_t)nrc,
+    void *bufs[17] = {nullptr};
+    size_t sizes[17] = {nv * 8, 2 * nn * 8, 8 * (size_t)nsrc, 8 * (size_t)nsrc, 8 * (size_t)nrc, 8 * (size_t)nrc,
                         4 * (size_t)np * nsrc * nrc, 8 * (size_t)(nprob * wsz), 8 * (size_t)(nprob * wsz),
                         8 * (size_t)(nprob * wsz), 4 * (size_t)(nprob * wsz), 4 * (size_t)(nprob * wsz),
-                        4 * (size_t)(nprob * (wsz + 8)), 8 * (size_t)nprob * nrc, 4, 4 * (size_t)nprob};
+                        4 * (size_t)(nprob * (wsz + 8)), 8 * (size_t)nprob * nrc, 4, 4 * (size_t)nprob,
+                        8 * (size_t)(nprob * (wsz + 8))};
     int rc = MCT_OK;
-    for (int i = 0; i < 16 && !rc; i++)
+    for (int i = 0; i < 17 && !rc; i++)
         if (hipMalloc(&bufs[i], sizes[i]) != hipSuccess) rc = MCT_ERR_NOMEM;
     if (!rc && (hipMemcpy(bufs[0], velv, nv * 8, hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(bufs[2], scx, 8 * (size_t)nsrc, hipMemcpyHostToDevice) != hipSuccess ||
@@ -669,12 +711,13 @@ extern "C" int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dv
                 hipMemset(bufs[14], 0, 4) != hipSuccess))
         rc = MCT_ERR_HIP;
     if (!rc) {
-        a.velv = (const double *)bufs[0]; a.veln = (double *)bufs[1];
+        a.velv = (const double *)bufs[0]; a.veln = (double *)bufs[1]; a.sln = (double *)bufs[1] + nn;
         a.scx = (const double *)bufs[2]; a.scz = (const double *)bufs[3];
         a.rcx = (const double *)bufs[4]; a.rcz = (const double *)bufs[5];
         a.srs = (const int32_t *)bufs[6];
         a.ttn = (double *)bufs[7]; a.ttr = (double *)bufs[8]; a.vr = (double *)bufs[9];
         a.nsts = (int32_t *)bufs[10]; a.nstr = (int32_t *)bufs[11]; a.heap = (int32_t *)bufs[12];
+        a.heapk = (double *)bufs[16];
         a.ttime = (double *)bufs[13]; a.status = (int32_t *)bufs[14];
         a.srs_pstride = (long)nsrc * nrc;
         rc = fast ? mct_launch_fm2d_fast(a, nprob, (int32_t *)bufs[15], 0) : mct_launch_fm2d(a, nprob, 0);
@@ -691,7 +734,7 @@ extern "C" int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dv
             for (long i = 0; i < nprob; i++) g_modrays_handover += pf[i] == 1;
         free(pf);
     }
-    for (int i = 0; i < 16; i++) if (bufs[i]) (void)hipFree(bufs[i]);
+    for (int i = 0; i < 17; i++) if (bufs[i]) (void)hipFree(bufs[i]);
     if (!rc && st < 0) rc = st;
     return rc;
 }

@@ -467,6 +467,9 @@ __host__ __device__ double ff_cands(const ff_sten &S, double dnx, double dnz, do
 struct ff_k0 {
     double uu, vv, a4, a2;    // dnx^2, dnz^2, 4a, 2a with a = dnx^2 + dnz^2
     double dnz;
+    double ia2;               // 1 / (2a), exact when a2p
+    bool a2p;                 // 2a is a power of two: x / (2a) == x * (1 / (2a)) bit for bit
+    bool hsame;               // dnx == dnz, a power of two: the x and z one-neighbour steps are one number
 };
 __host__ __device__ __forceinline__ ff_k0 ff_k0_make(double dnx, double dnz)
 {
@@ -477,6 +480,15 @@ __host__ __device__ __forceinline__ ff_k0 ff_k0_make(double dnx, double dnz)
     K.a4 = 4.0 * a;
     K.a2 = 2.0 * a;
     K.dnz = dnz;
+    // a correctly rounded quotient by 2^k is the correctly rounded product with
+    // 2^-k (both round the same real number), so the division can go
+    int e = 0;
+    K.a2p = K.a2 > 0.0 && frexp(K.a2, &e) == 0.5;
+    K.ia2 = K.a2p ? ldexp(1.0, 1 - e) : 0.0;
+    // with dnx == dnz == 2^k, (slown*dnz)^2 rounds to (slown^2)*dnx^2 exactly
+    // (scaling by a power of two commutes with rounding), so hz == hx
+    int ex = 0;
+    K.hsame = dnx == dnz && frexp(dnx, &ex) == 0.5 && ex > -40 && ex < 40;
     return K;
 }
 
@@ -487,7 +499,8 @@ __host__ __device__ __forceinline__ double ff_q2(double tj, double tk, const ff_
     const double c = K.uu * (em * em - K.vv * ss);
     double rd1 = b * b - K.a4 * c;
     if (rd1 < 0.0) rd1 = 0.0;
-    return tj + (-b + sqrt(rd1)) / K.a2;
+    const double num = -b + sqrt(rd1);
+    return tj + (K.a2p ? num * K.ia2 : num / K.a2);
 }
 
 __host__ __device__ double ff_cands0f(const ff_sten &S, const ff_k0 &K, double slown)
@@ -498,7 +511,7 @@ __host__ __device__ double ff_cands0f(const ff_sten &S, const ff_k0 &K, double s
     // one alive neighbour: a = 1, b = 0, so tdsh = sqrt(-4c) / 2
     const double hx = sqrt(4.0 * (ss * K.uu)) * 0.5;
     const double sdz = slown * K.dnz;
-    const double hz = sqrt(4.0 * (sdz * sdz)) * 0.5;
+    const double hz = K.hsame ? hx : sqrt(4.0 * (sdz * sdz)) * 0.5;
     double q[2][2];
     for (int sj = 0; sj < 2; sj++)
         for (int sk = 0; sk < 2; sk++)
@@ -1463,6 +1476,24 @@ __global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, int32_t *pf
 // A strip without dirty nodes is skipped.
 #define FF_PF 4
 
+// Lane i takes lane i-1's value (wave_shr:1) / lane i+1's (wave_shl:1) by DPP
+// moves, two per double; a lane without a source keeps its own value, as
+// __shfl_up / __shfl_down do.  Called with every lane of the wave active.
+__device__ __forceinline__ double ff_from_below(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double ff_from_above(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), 0x130, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 template <int FOM>
 __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, int32_t *pflag)
 {
@@ -1477,6 +1508,7 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, int32_t *pfl
     const double dnx = A.dvx / A.gdx, dnz = A.dvz / A.gdz;
     const long cp = g.prob / A.nsrc;
     const double *velnc = A.veln + cp * (long)nnx * nnz;
+    const double *slnc = A.sln ? A.sln + cp * (long)nnx * nnz : nullptr;
     double *T = A.ttn + slot * wsz;
     const int32_t *F = A.nsts + slot * wsz;
     const long nn = (long)nnx * nnz;
@@ -1595,9 +1627,11 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, int32_t *pfl
                     if (!xin || bb < 0 || bb >= nnz) return FF_BIG;
                     return T[colT + (zs > 0 ? bb : nnz - 1 - bb)];
                 };
+                // the node's slowness 1.0 / veln (fouds' slown)
                 auto vload = [&](int bb) -> double {
                     if (!xin || bb < 0 || bb >= nnz) return 1.0;
-                    return velnc[colV + (zs > 0 ? bb : nnz - 1 - bb)];
+                    const long q = colV + (zs > 0 ? bb : nnz - 1 - bb);
+                    return slnc ? slnc[q] : 1.0 / velnc[q];
                 };
                 constexpr int D = 2 + FOM;      // the stream delivers the old time D nodes ahead
                 const int b0 = tstart - lane;
@@ -1610,18 +1644,19 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, int32_t *pfl
                 // the window and the first chunk in registers before the step loop, so the
                 // loop's waits are for the chunk in flight only
                 __builtin_amdgcn_s_waitcnt(0);
-                auto step = [&](int t, double snext, double vcur) {
+                auto step = [&](int t, double snext, double scur) {
                     const int b = t - lane;
                     const bool act = xin && b >= 0 && b < nnz;
                     const int z0 = zs > 0 ? b : nnz - 1 - b;              // 0-based z
-                    double xm1 = __shfl_up(pn1, 1, 64), xp1 = __shfl_down(o1, 1, 64);
+                    // lanes 0/63 (and 1/62 for the second neighbours) take these from LDS below
+                    double xm1 = ff_from_below(pn1), xp1 = ff_from_above(o1);
                     double xm2 = 0.0, xp2 = 0.0;
-                    if (FOM) { xm2 = __shfl_up(pn2, 2, 64); xp2 = __shfl_down(o2, 2, 64); }
+                    if (FOM) { xm2 = ff_from_below(ff_from_below(pn2)); xp2 = ff_from_above(ff_from_above(o2)); }
                     double newv = c0;
                     if (act) {
                         const long q = colV + z0;
                         const uint32_t m = 1u << (q & 31);
-                        if ((DB[q >> 5] & m) && !fixed(z0 + 1, x)) {
+                        if (DB[q >> 5] & m) {        // fixed nodes are never marked
                             atomicAnd(&DB[q >> 5], ~m);
                             n_evals++;
                             if (lane == 0) xm1 = side[z0];
@@ -1650,8 +1685,8 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, int32_t *pfl
                             } else {
                                 S.tx2[0] = S.tx2[1] = S.tz2[0] = S.tz2[1] = FF_BIG;
                             }
-                            const double c = FOM == 0 ? ff_cands0f(S, K0, 1.0 / vcur)
-                                                      : ff_cands<FOM>(S, dnx, dnz, 1.0 / vcur);
+                            const double c = FOM == 0 ? ff_cands0f(S, K0, scur)
+                                                      : ff_cands<FOM>(S, dnx, dnz, scur);
                             if (c < FF_THR0 && c != c0) {
                                 if (fabs(c - c0) > 1e-14 * c) changed = 1;
                                 newv = c;

@@ -14,6 +14,13 @@
 // created only while the bytes of all lanes stay within it, and when a model
 // has waiting requests and no room, idle lanes of models with nothing pending
 // are released first (least recently used), so memory follows the mix.
+//
+// Batch fill: under steady arrivals, launching whatever is pending the moment
+// a lane frees gives small batches that share the device, and a small batch
+// costs nearly as much as a full one.  So a partial batch waits while another
+// lane is running, until the oldest of its requests has waited fill_ms (or the
+// batch is full); with no lane running it goes at once, so the device never
+// idles while requests wait.  fill_ms < 0 turns the wait off.
 #include "../../include/mctomo_hip.h"
 #include <hip/hip_runtime.h>
 
@@ -87,6 +94,7 @@ struct mct_server {
     std::thread dispatcher;
     int64_t batches = 0, evals = 0;
     int rr = 0;                      // round-robin start over models
+    double fill_ms = 25.0;           // longest wait of a partial batch while a lane runs (< 0: none)
 };
 
 static void lane_free(lane *L)
@@ -219,6 +227,12 @@ static void dispatcher_main(mct_server *S)
             const int m = (S->rr + k) % nm;
             model &M = S->models[m];
             while (!M.pending.empty()) {
+                if ((int)M.pending.size() < M.max_batch && any_busy && S->fill_ms >= 0.0) {
+                    // a partial batch waits for more requests while the device is busy
+                    const double waited = std::chrono::duration<double, std::milli>(
+                                              clk::now() - M.pending.front()->t_submit).count();
+                    if (waited < S->fill_ms) { waiting = true; break; }
+                }
                 lane *L = nullptr;
                 for (lane *c : S->lanes)
                     if (c->model == m && !c->busy) { L = c; break; }
@@ -420,6 +434,14 @@ int mct_serve_wait(mct_server *S, int64_t ticket, double *out, int32_t *status, 
     if (latency_ms) *latency_ms = r->latency_ms;
     S->all.erase(it);
     delete r;
+    return MCT_OK;
+}
+
+int mct_serve_set_fill(mct_server *S, double fill_ms)
+{
+    if (!S) return MCT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->fill_ms = fill_ms;
     return MCT_OK;
 }
 
