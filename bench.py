@@ -288,17 +288,23 @@ def run_surface(args):
         cap_s = min(B, len(reqs)) / (time.perf_counter() - t0)
         rate = 0.8 * cap_s
         gaps = np.random.default_rng(7).exponential(1.0 / rate, len(reqs))
-        tickets = []
+        tickets, t_in = [], []
         t0 = time.perf_counter()
         for i, (p_, q_) in enumerate(reqs):
             t_next = t0 + gaps[:i + 1].sum()
             now = time.perf_counter()
             if t_next > now:
                 time.sleep(t_next - now)
+            t_in.append(time.perf_counter() - t0)
             tickets.append(srv.submit(p_, q_))
+        t_sub = time.perf_counter() - t0
         res_s = [srv.wait(t) for t in tickets]
         t_all = time.perf_counter() - t0
         lat_s = np.array([r[2] for r in res_s])
+        # requests still in the server when the last one arrived: a server that keeps
+        # up holds about a batch or two; one that does not holds a backlog that grows
+        # with the run
+        backlog = int(sum(ti + li * 1e-3 > t_sub for ti, li in zip(t_in, lat_s)))
         bad = [r[0] for r in res_s if r[0] != 0]
         nb, ne, rc = srv.stats()
         srv.close()
@@ -306,7 +312,8 @@ def run_surface(args):
             raise RuntimeError("served statuses %s rc %d" % (bad[:5], rc))
         served = {"requests": len(reqs), "closed_burst_capacity_per_s": round(cap_s, 3),
                   "arrival_rate_per_s": round(rate, 3),
-                  "evals_per_s": round(len(reqs) / t_all, 3),
+                  "offered_per_s": round(len(reqs) / t_sub, 3), "evals_per_s": round(len(reqs) / t_all, 3),
+                  "backlog_at_last_arrival": backlog,
                   "p50_ms": round(float(np.median(lat_s)), 3), "p99_ms": round(float(np.quantile(lat_s, 0.99)), 3),
                   "batches": int(nb - 1), "lanes": 2, "max_batch": min(B, 32), "fill_ms": args.served_fill_ms}
         log("served:", served)
@@ -350,12 +357,13 @@ def run_surface(args):
         cap = len(burst) / (time.perf_counter() - t0)
         rate_m = 0.8 * cap
         arrive = np.cumsum(np.random.default_rng(7).exponential(1.0 / rate_m, len(live)))
-        tickets = []
+        tickets, t_in = [], []
         t0 = time.perf_counter()
         for i, (m, p_, q_) in enumerate(live):
             now = time.perf_counter()
             if t0 + arrive[i] > now:
                 time.sleep(t0 + arrive[i] - now)
+            t_in.append(time.perf_counter() - t0)
             tickets.append((m, srvm.submit(p_, q_, model=m)))
         t_sub = time.perf_counter() - t0
         res_m = [(m, srvm.wait(t)) for m, t in tickets]
@@ -366,6 +374,7 @@ def run_surface(args):
         if rc_m or any(r[0] < 0 for _, r in res_m):
             raise RuntimeError("served mix rc %d" % rc_m)
         lat_m = np.array([r[2] for _, r in res_m])
+        backlog_m = int(sum(ti + li * 1e-3 > t_sub for ti, li in zip(t_in, lat_m)))
         per = {}
         for mm, name in enumerate(("51x51x21/10", "101x101x41/20", "201x201x61/30")):
             lm = np.array([r[2] for m, r in res_m if m == mm])
@@ -377,6 +386,7 @@ def run_surface(args):
         mixres = {"chains": len(live), "closed_burst_capacity_per_s": round(cap, 3),
                   "arrival_rate_per_s": round(rate_m, 3), "offered_per_s": round(offered, 3),
                   "evals_per_s": round(sustained, 3), "sustained_over_offered": round(sustained / offered, 4),
+                  "backlog_at_last_arrival": backlog_m,
                   "p50_ms": round(float(np.median(lat_m)), 3), "p99_ms": round(float(np.quantile(lat_m, 0.99)), 3),
                   "rejected_models": int(sum(r[0] > 0 for _, r in res_m)), "per_grid": per,
                   "lanes": lanes_m, "max_batch": list(mbs), "fill_ms": args.served_fill_ms,
@@ -664,7 +674,7 @@ def main():
     ap.add_argument("--fm2d-method", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-evals", type=int, default=10)
-    ap.add_argument("--served", type=int, default=192, help="requests for the served measurement (0: skip)")
+    ap.add_argument("--served", type=int, default=512, help="requests for the served measurement (0: skip)")
     ap.add_argument("--box-steps", type=int, default=3,
                     help="timed perturbed-box batches (secondary unit; 0: skip)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")

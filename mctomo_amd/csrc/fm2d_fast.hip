@@ -1474,7 +1474,7 @@ __global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, int32_t *pf
 // and streams its column's old times and velocities from memory FF_PF to
 // 2*FF_PF steps ahead; the strip's outer neighbour columns are staged in LDS.
 // A strip without dirty nodes is skipped.
-#define FF_PF 4
+#define FF_PF 2
 
 // Lane i takes lane i-1's value (wave_shr:1) / lane i+1's (wave_shl:1) by DPP
 // moves, two per double; a lane without a source keeps its own value, as
@@ -1623,16 +1623,24 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, int32_t *pfl
                 const int tmax = nnz - 1 + last;
                 const int tstart = tl > first ? tl : first;
                 int tend = th < tmax ? th : tmax;
+                // The stream's loads have no condition: an index outside the column
+                // reads a node inside it, and the value is replaced where it is used,
+                // so the loads stay in flight until then (a load under a condition
+                // would be waited for at once).
+                const long colTs = xin ? colT : 0, colVs = xin ? colV : 0;
+                auto zrow = [&](int bb) -> int {
+                    const int c = bb < 0 ? 0 : (bb >= nnz ? nnz - 1 : bb);
+                    return zs > 0 ? c : nnz - 1 - c;
+                };
+                auto inb = [&](int bb) -> bool { return xin && bb >= 0 && bb < nnz; };
+                auto tload = [&](int bb) -> double { return T[colTs + zrow(bb)]; };
                 auto told = [&](int bb) -> double {
-                    if (!xin || bb < 0 || bb >= nnz) return FF_BIG;
-                    return T[colT + (zs > 0 ? bb : nnz - 1 - bb)];
+                    const double v = tload(bb);
+                    return inb(bb) ? v : FF_BIG;
                 };
-                // the node's slowness 1.0 / veln (fouds' slown)
-                auto vload = [&](int bb) -> double {
-                    if (!xin || bb < 0 || bb >= nnz) return 1.0;
-                    const long q = colV + (zs > 0 ? bb : nnz - 1 - bb);
-                    return slnc ? slnc[q] : 1.0 / velnc[q];
-                };
+                // the node's slowness 1.0 / veln (fouds' slown); used only where the
+                // node is in the column
+                auto vload = [&](int bb) -> double { return slnc[colVs + zrow(bb)]; };
                 constexpr int D = 2 + FOM;      // the stream delivers the old time D nodes ahead
                 const int b0 = tstart - lane;
                 // behind the start nothing has changed in this visit: the stored times
@@ -1640,12 +1648,13 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, int32_t *pfl
                 double c0 = told(b0), o1 = told(b0 + 1), o2 = FOM ? told(b0 + 2) : 0.0;
                 double sa[FF_PF], sb[FF_PF], va[FF_PF], vb[FF_PF];
 #pragma unroll
-                for (int j = 0; j < FF_PF; j++) { sa[j] = told(b0 + j + D); va[j] = vload(b0 + j); }
+                for (int j = 0; j < FF_PF; j++) { sa[j] = tload(b0 + j + D); va[j] = vload(b0 + j); }
                 // the window and the first chunk in registers before the step loop, so the
                 // loop's waits are for the chunk in flight only
                 __builtin_amdgcn_s_waitcnt(0);
-                auto step = [&](int t, double snext, double scur) {
+                auto step = [&](int t, double snext_raw, double scur) {
                     const int b = t - lane;
+                    const double snext = inb(b + D) ? snext_raw : FF_BIG;
                     const bool act = xin && b >= 0 && b < nnz;
                     const int z0 = zs > 0 ? b : nnz - 1 - b;              // 0-based z
                     // lanes 0/63 (and 1/62 for the second neighbours) take these from LDS below
@@ -1708,22 +1717,20 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, int32_t *pfl
                     pn2 = pn1; pn1 = newv; c0 = o1;
                     if (FOM) { o1 = o2; o2 = snext; } else { o1 = snext; }
                 };
-                for (int t0 = tstart; t0 <= tend; t0 += 2 * FF_PF) {
+                // each pass issues the loads of the next FF_PF steps, then runs its
+                // own FF_PF steps on the values the previous pass loaded (static
+                // register indices, so the waits count only the loads it needs)
+                for (int t0 = tstart; t0 <= tend; t0 += FF_PF) {
 #pragma unroll
                     for (int j = 0; j < FF_PF; j++) {
-                        sb[j] = told(t0 + FF_PF + j - lane + D);
+                        sb[j] = tload(t0 + FF_PF + j - lane + D);
                         vb[j] = vload(t0 + FF_PF + j - lane);
                     }
-#pragma unroll 1
-                    for (int j = 0; j < FF_PF && t0 + j <= tend; j++) step(t0 + j, sa[j], va[j]);
 #pragma unroll
-                    for (int j = 0; j < FF_PF; j++) {
-                        sa[j] = told(t0 + 2 * FF_PF + j - lane + D);
-                        va[j] = vload(t0 + 2 * FF_PF + j - lane);
-                    }
-#pragma unroll 1
-                    for (int j = 0; j < FF_PF && t0 + FF_PF + j <= tend; j++)
-                        step(t0 + FF_PF + j, sb[j], vb[j]);
+                    for (int j = 0; j < FF_PF; j++)
+                        if (t0 + j <= tend) step(t0 + j, sa[j], va[j]);
+#pragma unroll
+                    for (int j = 0; j < FF_PF; j++) { sa[j] = sb[j]; va[j] = vb[j]; }
                 }
                 if (lane == 0) n_steps += tend - tstart + 1;
                 // the flag stays set only if the strip still holds marked nodes (marks
@@ -1935,6 +1942,7 @@ int mct_fm2d_fast_ok(const mct_fm2d_args &a)
 
 int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hipStream_t s)
 {
+    if (!a.sln) return MCT_ERR_ARG;       // the sweeps read the gridder's slowness
     int region, heap_cap, win;
     size_t lds = ff_near_lds(a, region, heap_cap, win);
     int rc = mct_launch_fm2d_gridder(a, s);
