@@ -39,6 +39,7 @@ module mctomo_hip
         integer(c_int) :: max_cells, max_chains
         integer(c_int) :: straight
         integer(c_int) :: datatype
+        real(c_double) :: water_depth
     end type
 
     interface

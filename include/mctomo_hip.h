@@ -76,8 +76,10 @@ int mct_surfdisp96_batch(int nmod, int nl_max, const int32_t *nlayer, const floa
  * (1): check_model failed for some column.  mode picks the kernel the
  * likelihood uses: 1 one lane per column, 2 one wave per column, 3 the column
  * queue in layer-count order with identical columns solved once (the default
- * for large batches).  nz <= 100, np <= 60. */
-int mct_dispersion_columns(int ncol, int nz, double dz, double scaling, const double *vs,
+ * for large batches).  water: T_GRID%waterDepth (km, not scaled; 0 for no
+ * water layer), the water layer convert_to_layer puts on every column
+ * (src/likelihood_surf.F90:546-551,616-617).  nz <= 100, np <= 60. */
+int mct_dispersion_columns(int ncol, int nz, double dz, double scaling, double water, const double *vs,
                            int np, const double *periods, int iwave, int igr, double dphase,
                            int mode, double *phase, double *group, int32_t *ierr,
                            int32_t *bad_model);
@@ -303,7 +305,7 @@ int mct_set_kernel_timing(mct_ctx *ctx, int on);
 /* ---- surface-wave likelihood context (datatype 2) ------------------------- */
 /* Replaces likelihood() for datatype 2 (src/likelihood.f90:72-80 ->
  * surf_likelihood, src/likelihood_surf.F90:110-405) with phase or group
- * velocities (LIKE_SET%phaseGroup), no water layer.  The setup mirrors T_DATA /
+ * velocities (LIKE_SET%phaseGroup), with or without a water layer.  The setup mirrors T_DATA /
  * T_LIKE_SET (src/likelihood_settings.f90:19-79); arrays are copied at create
  * time.  With group velocities modrays traces a ray per used pair (rpaths,
  * fm2d/fm2dray_cartesian.f90:801-1420), the group delays along the rays
@@ -338,6 +340,9 @@ typedef struct {
                                  75-76); 3 the surface half of the joint likelihood: vp and rho
                                  are the sites' own parameters (RTI%parameters(1,:), (3,:);
                                  src/likelihood.f90:83-88) */
+    double water_depth;       /* T_GRID%waterDepth (src/settings.f90:26), km, not scaled: above
+                                 EPS every column gets a water layer on top (vp 1.5, vs 0,
+                                 rho 1; src/likelihood_surf.F90:32-33,546-551,616-617); 0: none */
 } mct_surf_setup;
 
 typedef struct mct_surf_ctx mct_surf_ctx;

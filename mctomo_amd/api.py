@@ -272,11 +272,12 @@ def surfdisp96_batch(models, periods, iwave, igr, dphase=1e-3):
     return ph, gr, ie
 
 
-def dispersion_columns(vs, dz, periods, iwave, igr, mode=3, dphase=1e-3, scaling=1.0):
+def dispersion_columns(vs, dz, periods, iwave, igr, mode=3, dphase=1e-3, scaling=1.0, water=0.0):
     """The surface likelihood's dispersion stage on explicit Vs columns
     (check_model, convert_to_layer, surfdisp96).  vs: (ncol, nz).  mode 1 lane
-    per column, 2 wave per column, 3 column queue.  Returns phase (ncol, np),
-    group (ncol, np), ierr (ncol,), bad_model."""
+    per column, 2 wave per column, 3 column queue; water: T_GRID%waterDepth
+    (0: no water layer).  Returns phase (ncol, np), group (ncol, np), ierr
+    (ncol,), bad_model."""
     vs = _f64(vs)
     ncol, nz = vs.shape
     per = _f64(periods)
@@ -284,7 +285,8 @@ def dispersion_columns(vs, dz, periods, iwave, igr, mode=3, dphase=1e-3, scaling
     gr = np.zeros((ncol, per.shape[0]))
     ie = np.zeros(ncol, np.int32)
     bad = np.zeros(1, np.int32)
-    rc = lib().mct_dispersion_columns(ncol, nz, ctypes.c_double(dz), ctypes.c_double(scaling), _p(vs),
+    rc = lib().mct_dispersion_columns(ncol, nz, ctypes.c_double(dz), ctypes.c_double(scaling),
+                                      ctypes.c_double(water), _p(vs),
                                       per.shape[0], _p(per), int(iwave), int(igr), ctypes.c_double(dphase),
                                       int(mode), _p(ph), _p(gr), _p(ie), _p(bad))
     _check(rc, "mct_dispersion_columns")
@@ -485,12 +487,13 @@ class _SurfSetup(ctypes.Structure):
                 ("nrays", ctypes.c_int), ("gridx", ctypes.c_int), ("gridy", ctypes.c_int),
                 ("sgref", ctypes.c_int), ("sgdic", ctypes.c_int), ("sgext", ctypes.c_int),
                 ("order", ctypes.c_int), ("band", ctypes.c_double), ("max_cells", ctypes.c_int),
-                ("max_chains", ctypes.c_int), ("straight", ctypes.c_int), ("datatype", ctypes.c_int)]
+                ("max_chains", ctypes.c_int), ("straight", ctypes.c_int), ("datatype", ctypes.c_int),
+                ("water_depth", ctypes.c_double)]
 
 
 def _surf_setup(grid, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3, sigdep=0, nrays=None,
                 gridx=1, gridy=1, sgref=1, sgdic=4, sgext=8, order=1, band=0.5, max_cells=1024,
-                max_chains=1, phase_group=0, straight=0, datatype=2):
+                max_chains=1, phase_group=0, straight=0, datatype=2, water_depth=0.0):
     """A filled mct_surf_setup and the arrays it points to (keep them alive)."""
     fr = _f64(freqs).reshape(-1)
     npd = fr.shape[0]
@@ -503,7 +506,7 @@ def _surf_setup(grid, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3, si
     st = _SurfSetup(int(raylov), int(phase_group), float(dphase), npd, fr.ctypes.data, sr.shape[0], rv.shape[0],
                     sr.ctypes.data, rv.ctypes.data, tt.ctypes.data, rs.ctypes.data, int(sigdep), int(nrays),
                     int(gridx), int(gridy), int(sgref), int(sgdic), int(sgext), int(order), float(band),
-                    int(max_cells), int(max_chains), int(straight), int(datatype))
+                    int(max_cells), int(max_chains), int(straight), int(datatype), float(water_depth))
     return st, (fr, sr, rv, tt, rs)
 
 
@@ -522,7 +525,7 @@ class SurfLikelihood:
 
     def __init__(self, grid, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3, sigdep=0,
                  nrays=None, gridx=1, gridy=1, sgref=1, sgdic=4, sgext=8, order=1, band=0.5,
-                 max_cells=1024, max_chains=1, phase_group=0, straight=0, datatype=2):
+                 max_cells=1024, max_chains=1, phase_group=0, straight=0, datatype=2, water_depth=0.0):
         self.grid = grid
         self.freqs = _f64(freqs).reshape(-1)
         self.np = self.freqs.shape[0]
@@ -541,7 +544,7 @@ class SurfLikelihood:
                         self.nrev, self.src.ctypes.data, self.rev.ctypes.data, self.ttime.ctypes.data,
                         self.raystat.ctypes.data, int(sigdep), int(nrays), int(gridx), int(gridy),
                         int(sgref), int(sgdic), int(sgext), int(order), float(band), int(max_cells),
-                        int(max_chains), int(straight), int(datatype))
+                        int(max_chains), int(straight), int(datatype), float(water_depth))
         self._g = grid.c()
         h = ctypes.c_void_p()
         _check(lib().mct_surf_create(ctypes.byref(self._g), ctypes.byref(st), ctypes.byref(h)),

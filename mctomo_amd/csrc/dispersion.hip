@@ -49,6 +49,53 @@ __device__ __forceinline__ void dsp_vprho(const mct_dsp_args &A, long cbase, int
     rho = (double)1.74f * pow(vp, 0.25);
 }
 
+// convert_to_layer (src/likelihood_surf.F90:523-629) for the column at cb into
+// REAL*4 layer arrays of element stride ST, as surfmodes passes them on
+// (real(thick,4) ..., surfmodes/surfmodes.f90:81-83).  With A.water > EPS a
+// water layer comes first (vp = waterVel 1.5, vs = 0, rho = waterDensity 1,
+// :32-33,546-551); a layer starts where vs changes by more than EPS and takes
+// the values of its first node; the half-space (thickness 0) takes the bottom
+// node's (either branch of :575-602).  Thicknesses are scaled, except that
+// with A.water > 0 the first layer's is A.water itself (:616-617: the water
+// layer's, or for 0 < water <= EPS the first solid layer's).  Returns the
+// layer count (at most nz + 1).
+template <int ST>
+__device__ __forceinline__ int dsp_convert(const mct_dsp_args &A, long cb, const double *vs, float *sd, float *sa,
+                                           float *sb, float *sr)
+{
+    const double EPS = (double)1.0e-10f;       // src/likelihood_surf.F90:37
+    int nl = 0, last_k = 1;
+    if (A.water > EPS) {
+        sa[0] = 1.5f; sb[0] = 0.0f; sr[0] = 1.0f; sd[0] = (float)A.water;
+        nl = 1;
+    }
+    double last_vs = vs[0];
+    for (int k = 2; k <= A.nz; k++) {
+        const double vk = vs[k - 1];
+        if (fabs(vk - last_vs) > EPS) {
+            double vpk, rhok;
+            dsp_vprho(A, cb, last_k - 1, last_vs, vpk, rhok);
+            sa[ST * nl] = (float)vpk;
+            sb[ST * nl] = (float)last_vs;
+            sr[ST * nl] = (float)rhok;
+            sd[ST * nl] = (float)(((double)(k - last_k) * A.dz) / A.scaling);
+            nl++;
+            last_vs = vk;
+            last_k = k;
+        }
+    }
+    {
+        double vpk, rhok;
+        dsp_vprho(A, cb, A.nz - 1, vs[A.nz - 1], vpk, rhok);
+        sa[ST * nl] = (float)vpk;
+        sb[ST * nl] = (float)vs[A.nz - 1];
+        sr[ST * nl] = (float)rhok;
+        sd[ST * nl] = 0.0f;
+    }
+    if (A.water > 0.0) sd[0] = (float)A.water;
+    return nl + 1;
+}
+
 // A column's layer model; ST is the element stride of the layer arrays (1 in
 // LDS, 64 when lanes interleave their models).
 template <int ST> struct dsp_model_t {
@@ -422,43 +469,16 @@ __device__ static int d_getsol(double t1, double &c1io, double clow, double dc, 
 // One wave per column: check_model, convert_to_layer, surfdisp96.
 __global__ __launch_bounds__(64) void k_dispersion(mct_dsp_args A)
 {
-    __shared__ float sd[DSP_NMAX], sa[DSP_NMAX], sb[DSP_NMAX], sr[DSP_NMAX];
+    __shared__ float sd[DSP_NMAX + 1], sa[DSP_NMAX + 1], sb[DSP_NMAX + 1], sr[DSP_NMAX + 1];
     __shared__ int s_nl;
     const int col = blockIdx.x, c = blockIdx.y, lane = threadIdx.x;
     const long cb = ((long)c * A.ncol + col) * A.nz;
     const double *vs = A.vs + cb;
-    const double EPS = (double)1.0e-10f;       // src/likelihood_surf.F90:37
     // check_model (src/likelihood_surf.F90:631-646): a deeper vs below the top one
     bool below = false;
     for (int k = 1 + lane; k < A.nz; k += 64) below |= vs[k] < vs[0];
     if (__ballot(below) != 0ull && lane == 0) A.bad_model[c] = 1;
-    // convert_to_layer (src/likelihood_surf.F90:523-629), no water layer
-    if (lane == 0) {
-        int nl = 0, last_k = 1;
-        double last_vs = vs[0];
-        for (int k = 2; k <= A.nz; k++) {
-            if (fabs(vs[k - 1] - last_vs) > EPS) {
-                double vpk, rhok;
-                dsp_vprho(A, cb, last_k - 1, last_vs, vpk, rhok);
-                sa[nl] = (float)vpk;
-                sb[nl] = (float)last_vs;
-                sr[nl] = (float)rhok;
-                sd[nl] = (float)(((double)(k - last_k) * A.dz) / A.scaling);
-                nl++;
-                last_vs = vs[k - 1];
-                last_k = k;
-            }
-        }
-        {   // the half-space: the bottom node's values, thickness 0 (either branch of :575-602)
-            double vpk, rhok;
-            dsp_vprho(A, cb, A.nz - 1, vs[A.nz - 1], vpk, rhok);
-            sa[nl] = (float)vpk;
-            sb[nl] = (float)vs[A.nz - 1];
-            sr[nl] = (float)rhok;
-            sd[nl] = 0.0f;
-        }
-        s_nl = nl + 1;
-    }
+    if (lane == 0) s_nl = dsp_convert<1>(A, cb, vs, sd, sa, sb, sr);
     __syncthreads();
     dsp_model_t<1> M;
     M.mmax = s_nl;
@@ -604,39 +624,13 @@ __global__ __launch_bounds__(64) void k_dispersion_lane(mct_dsp_args A, float *l
     const long plane = (long)nlmax * 64;
     float *base = lay + ((long)c * gridDim.x + blockIdx.x) * 4 * plane + lane;
     float *sd = base, *sa = base + plane, *sb = base + 2 * plane, *sr = base + 3 * plane;
-    const double EPS = (double)1.0e-10f;       // src/likelihood_surf.F90:37
     // check_model (src/likelihood_surf.F90:631-646)
     bool below = false;
     const double top = vs[0];
     for (int k = 1; k < A.nz; k++) below |= vs[k] < top;
     if (below) A.bad_model[c] = 1;
-    // convert_to_layer (src/likelihood_surf.F90:523-629)
-    int nl = 0, last_k = 1;
-    double last_vs = vs[0];
-    for (int k = 2; k <= A.nz; k++) {
-        double vk = vs[k - 1];
-        if (fabs(vk - last_vs) > EPS) {
-            double vpk, rhok;
-            dsp_vprho(A, cb, last_k - 1, last_vs, vpk, rhok);
-            sa[64 * nl] = (float)vpk;
-            sb[64 * nl] = (float)last_vs;
-            sr[64 * nl] = (float)rhok;
-            sd[64 * nl] = (float)(((double)(k - last_k) * A.dz) / A.scaling);
-            nl++;
-            last_vs = vk;
-            last_k = k;
-        }
-    }
-    {
-        double vpk, rhok;
-        dsp_vprho(A, cb, A.nz - 1, vs[A.nz - 1], vpk, rhok);
-        sa[64 * nl] = (float)vpk;
-        sb[64 * nl] = (float)vs[A.nz - 1];
-        sr[64 * nl] = (float)rhok;
-        sd[64 * nl] = 0.0f;
-    }
     dsp_model_t<64> M;
-    M.mmax = nl + 1;
+    M.mmax = dsp_convert<64>(A, cb, vs, sd, sa, sb, sr);
     M.d = sd; M.a = sa; M.b = sb; M.rho = sr;
     M.llw = (sb[0] <= 0.0f) ? 2 : 1;
     double *cp = A.phase + ((long)c * A.ncol + col) * A.np;
@@ -945,7 +939,6 @@ __global__ __launch_bounds__(64) void k_dispersion_sm(mct_dsp_args A, float *lay
     const long plane = (long)nlmax * 64;
     float *base = lay + (long)blockIdx.x * 4 * plane + lane;
     float *sd = base, *sa = base + plane, *sb = base + 2 * plane, *sr = base + 3 * plane;
-    const double EPS = (double)1.0e-10f;       // src/likelihood_surf.F90:37
     const double twopi = 2.0 * 3.141592653589793;
     const float sone = 1.500f, h = 0.005f;
     const float ddc = (float)A.dphase;
@@ -996,32 +989,7 @@ __global__ __launch_bounds__(64) void k_dispersion_sm(mct_dsp_args A, float *lay
                 const double top = vs[0];
                 for (int kk = 1; kk < A.nz; kk++) below |= vs[kk] < top;
                 if (below) A.bad_model[c] = 1;
-                // convert_to_layer (src/likelihood_surf.F90:523-629)
-                int nl = 0, last_k = 1;
-                double last_vs = vs[0];
-                for (int kk = 2; kk <= A.nz; kk++) {
-                    double vk = vs[kk - 1];
-                    if (fabs(vk - last_vs) > EPS) {
-                        double vpk, rhok;
-                        dsp_vprho(A, cb, last_k - 1, last_vs, vpk, rhok);
-                        sa[64 * nl] = (float)vpk;
-                        sb[64 * nl] = (float)last_vs;
-                        sr[64 * nl] = (float)rhok;
-                        sd[64 * nl] = (float)(((double)(kk - last_k) * A.dz) / A.scaling);
-                        nl++;
-                        last_vs = vk;
-                        last_k = kk;
-                    }
-                }
-                {
-                    double vpk, rhok;
-                    dsp_vprho(A, cb, A.nz - 1, vs[A.nz - 1], vpk, rhok);
-                    sa[64 * nl] = (float)vpk;
-                    sb[64 * nl] = (float)vs[A.nz - 1];
-                    sr[64 * nl] = (float)rhok;
-                    sd[64 * nl] = 0.0f;
-                }
-                M.mmax = nl + 1;
+                M.mmax = dsp_convert<64>(A, cb, vs, sd, sa, sb, sr);
                 M.llw = (sb[0] <= 0.0f) ? 2 : 1;
                 cp = A.phase + ((long)c * A.ncol + col) * A.np;
                 cg = A.group + ((long)c * A.ncol + col) * A.np;
@@ -1270,14 +1238,14 @@ int mct_launch_dispersion(const mct_dsp_args &a, hipStream_t s)
         }
         long nw = (total + 63) / 64;
         if (nw > b.lay_waves) nw = b.lay_waves;
-        hipLaunchKernelGGL(k_dispersion_sm, dim3((unsigned)nw), dim3(64), 0, s, b, b.lay, b.nz, total);
+        hipLaunchKernelGGL(k_dispersion_sm, dim3((unsigned)nw), dim3(64), 0, s, b, b.lay, b.nz + 1, total);
         if (b.dup) {
             MCT_CHECK(hipGetLastError());
             hipLaunchKernelGGL(k_dsp_dup_copy, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, b, total);
         }
     } else if (a.lay) {
         const unsigned nb = (unsigned)((a.ncol + 63) / 64);
-        hipLaunchKernelGGL(k_dispersion_lane, dim3(nb, a.nchain), dim3(64), 0, s, a, a.lay, a.nz);
+        hipLaunchKernelGGL(k_dispersion_lane, dim3(nb, a.nchain), dim3(64), 0, s, a, a.lay, a.nz + 1);
     } else {
         hipLaunchKernelGGL(k_dispersion, dim3(a.ncol, a.nchain), dim3(64), 0, s, a);
     }
@@ -1285,9 +1253,11 @@ int mct_launch_dispersion(const mct_dsp_args &a, hipStream_t s)
     return MCT_OK;
 }
 
+// a column has at most nz + 1 layers (nz - 1 interfaces, the half-space and
+// a water layer)
 long mct_dispersion_lay_floats(int ncol, int nz, int nchain)
 {
-    return (long)nchain * ((ncol + 63) / 64) * 4L * nz * 64;
+    return (long)nchain * ((ncol + 63) / 64) * 4L * (nz + 1) * 64;
 }
 
 // ---- C ABI: dispersion of explicit layer models (surfmodes' surfdisp96 call) ----
@@ -1429,14 +1399,14 @@ extern "C" int mct_surfdisp96_batch(int nmod, int nl_max, const int32_t *nlayer,
 }
 
 // ---- C ABI: the likelihood's dispersion stage on explicit Vs columns -------------
-extern "C" int mct_dispersion_columns(int ncol, int nz, double dz, double scaling, const double *vs,
-                                      int np, const double *periods, int iwave, int igr, double dphase,
-                                      int mode, double *phase, double *group, int32_t *ierr,
+extern "C" int mct_dispersion_columns(int ncol, int nz, double dz, double scaling, double water,
+                                      const double *vs, int np, const double *periods, int iwave, int igr,
+                                      double dphase, int mode, double *phase, double *group, int32_t *ierr,
                                       int32_t *bad_model)
 {
     if (ncol < 1 || nz < 1 || nz > DSP_NMAX || np < 1 || np > DSP_MAXP || !vs || !periods || !phase ||
         !group || !ierr || !bad_model || (iwave != 1 && iwave != 2) || mode < 1 || mode > 3 ||
-        !(dz > 0) || !(scaling > 0))
+        !(dz > 0) || !(scaling > 0) || !(water >= 0.0))
         return MCT_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MCT_ERR_HIP;
@@ -1462,7 +1432,7 @@ extern "C" int mct_dispersion_columns(int ncol, int nz, double dz, double scalin
         mct_dsp_args a;
         memset(&a, 0, sizeof a);
         a.ncol = ncol; a.nz = nz; a.nchain = 1; a.vs = (const double *)b[0];
-        a.dz = dz; a.scaling = scaling; a.periods = (const double *)b[1]; a.np = np;
+        a.dz = dz; a.scaling = scaling; a.water = water; a.periods = (const double *)b[1]; a.np = np;
         a.iwave = iwave; a.igr = igr; a.dphase = dphase;
         a.phase = (double *)b[2]; a.group = (double *)b[3]; a.ierr = (int32_t *)b[4];
         a.bad_model = (int32_t *)b[5];

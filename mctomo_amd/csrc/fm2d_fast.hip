@@ -1179,6 +1179,13 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
                 S.x1 = risx + 1 + FF_SEED_RINGS > R.nnx ? R.nnx : risx + 1 + FF_SEED_RINGS;
                 S.z0 = risz - FF_SEED_RINGS < 1 ? 1 : risz - FF_SEED_RINGS;
                 S.z1 = risz + 1 + FF_SEED_RINGS > R.nnz ? R.nnz : risz + 1 + FF_SEED_RINGS;
+                if (A.fom != 0) {
+                    // mixed order: the refined grid's answer depends on the heap's
+                    // history (keys rise, nodes pop out of time order), and the
+                    // heap-order check covers the coarse grid only, so the heap loop
+                    // solves the whole refined grid, up to the reference's exit
+                    S.x0 = 1; S.x1 = R.nnx; S.z0 = 1; S.z1 = R.nnz;
+                }
                 S.need = (S.x1 - S.x0 + 1) * (S.z1 - S.z0 + 1);
             } else {
                 sh_ovf = 3;   // source outside the refined grid: use the sequential solver

@@ -234,6 +234,7 @@ struct mct_dsp_args {
     int nchain;
     const double *vs;             // (nchain, nx, ny, nz) Fortran (nz,ny,nx) per chain
     double dz, scaling;           // grid%dz (scaled) and grid%scaling
+    double water;                 // T_GRID%waterDepth: a water layer on every column above it (0: none)
     const double *periods;        // (np) periods in s (1/freqs)
     int np;
     int iwave;                    // 1 Love, 2 Rayleigh

@@ -183,7 +183,7 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     mct_dsp_args da;
     memset(&da, 0, sizeof da);
     da.ncol = ncol; da.nz = G.nz; da.nchain = nchain; da.vs = vs;
-    da.dz = G.dz; da.scaling = G.scaling; da.periods = C->periods.as<double>(); da.np = np;
+    da.dz = G.dz; da.scaling = G.scaling; da.water = S.water_depth; da.periods = C->periods.as<double>(); da.np = np;
     da.iwave = S.raylov == 1 ? 2 : 1; da.igr = S.phase_group; da.dphase = S.dphase;
     da.phase = C->phase.as<double>() + (long)c0 * ncol * np;
     da.group = C->group.as<double>() + (long)c0 * ncol * np;
@@ -208,7 +208,7 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
         if ((rc = mct_fill_i32(da.ierr, (long)ncol * nchain, 0, s))) return rc;
         da.box = d_box; da.box_off = d_box_off; da.box_total = box_total; da.ny = G.ny;
     }
-    da.lay = (dmode == 1 || dmode == 3) ? C->lay.as<float>() + (long)c0 * wpc * 4 * G.nz * 64 : nullptr;
+    da.lay = (dmode == 1 || dmode == 3) ? C->lay.as<float>() + (long)c0 * wpc * 4 * (G.nz + 1) * 64 : nullptr;
     if (da.lay && dmode == 3) {
         da.queue = C->dsp_queue.as<unsigned long long>() + part;
         da.lay_waves = (long)nchain * wpc;
@@ -347,6 +347,7 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
         S.max_chains < 1 || S.gridx < 1 || S.gridy < 1 || S.sgdic < 1 || S.sgext < 1 || grid->nz > 100)
         return MCT_ERR_ARG;
     if (S.raylov != 0 && S.raylov != 1) return MCT_ERR_ARG;
+    if (!(S.water_depth >= 0.0)) return MCT_ERR_ARG;
     // group times trace rays from the refined source grid (rpaths reads it)
     if (S.phase_group != 0 && S.phase_group != 1) return MCT_ERR_ARG;
     if (S.straight != 0 && S.straight != 1) return MCT_ERR_ARG;

@@ -77,8 +77,6 @@ contains
         g%dy = like_set%grid%dy
         g%dz = like_set%grid%dz
         g%scaling = like_set%grid%scaling
-        if (like_set%grid%waterDepth > 0) &
-            call exception_raiseError('mctomo_hip: water layers are not supported')
 
         select case (like_set%datatype)
         case (0, 1)
@@ -186,6 +184,7 @@ contains
         s%max_chains = 1
         s%straight = like_set%isStraight
         s%datatype = datatype
+        s%water_depth = like_set%grid%waterDepth
         ctx = c_null_ptr
         rc = mct_surf_create(g, s, ctx)
         if (rc /= MCT_OK) call exception_raiseError('mctomo_hip: mct_surf_create failed')

@@ -123,12 +123,12 @@ int orc_body_misfit(int nsrc, int nrev, int np, const double *phase_time,
                     double *srdist, double *sigma, double *out);
 
 
-/* convert_to_layer (src/likelihood_surf.F90:523-629, no water layer) for every
- * column of vs (ncol, nz): lay (ncol, maxl, 4) = thick, vp, vs, rho with vp and
+/* convert_to_layer (src/likelihood_surf.F90:523-629; water: T_GRID%waterDepth,
+ * 0 for none) for every column of vs (ncol, nz): lay (ncol, maxl, 4) = thick, vp, vs, rho with vp and
  * rho from vs2vp_3d / vp2rho_3d, nl (ncol) layer counts.  -1 if a column has
  * more than maxl layers. */
-int orc_column_layers_all(long ncol, int nz, const double *vs, double dz, double scaling, int maxl,
-                          double *lay, int32_t *nl);
+int orc_column_layers_all(long ncol, int nz, const double *vs, double dz, double scaling, double water,
+                          int maxl, double *lay, int32_t *nl);
 
 /* Interpolate3d (src/likelihood_body.F90:976-1007): the trilinear value of a
  * traveltime field (Fortran (nz,ny,nx)) at point[3] (z unscaled), misfit.c. */
@@ -158,7 +158,7 @@ int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double
                         double band, int nthreads,
                         double *out, double *phase_time, double *srdist, double *sigma,
                         double *phase_vel, int32_t *sites_id, int phase_group,
-                        double *group_vel, int straight, int datatype);
+                        double *group_vel, int straight, int datatype, double water);
 
 #ifdef __cplusplus
 }

@@ -313,7 +313,8 @@ def ref_surf_forward(grid, points, params, freqs, src, rev, raystat, raylov=1, d
     lay = np.zeros((ncol, maxl, 4))
     nl = np.zeros(ncol, np.int32)
     if lib().orc_column_layers_all(ctypes.c_long(ncol), nz, _p(c64(vs)), ctypes.c_double(grid["dz"]),
-                                   ctypes.c_double(grid.get("scaling", 1.0)), maxl, _p(lay), _p(nl)):
+                                   ctypes.c_double(grid.get("scaling", 1.0)), ctypes.c_double(0.0), maxl,
+                                   _p(lay), _p(nl)):
         raise RuntimeError("layer count above nz")
     fr = c64(freqs).reshape(-1)
     npp = fr.shape[0]
@@ -335,8 +336,9 @@ def ref_surf_forward(grid, points, params, freqs, src, rev, raystat, raylov=1, d
 def surf_likelihood(grid, points, params, freqs, src, rev, ttime, raystat, raylov=1, dphase=1e-3,
                     sigdep=0, snoise0=None, snoise1=None, nrays=None, gridx=1, gridy=1, sgref=1,
                     sgdic=4, sgext=8, order=1, band=0.5, nthreads=0, phase_group=0, group_vel=None,
-                    straight=0, datatype=2):
-    """orc_surf_likelihood.  grid: dict nx..dz (+ scaling).  Returns (rc, out(3),
+                    straight=0, datatype=2, water=0.0):
+    """orc_surf_likelihood.  grid: dict nx..dz (+ scaling); water: T_GRID%waterDepth
+    (a water layer on every column above it, 0 for none).  Returns (rc, out(3),
     phase_time (np,nsrc,nrev), srdist, sigma (np,npair), phase_vel (nx,ny,np),
     sites_id (nx,ny,nz)).  phase_group=1: group times along rays; group_vel, an
     (nx,ny,np) array, receives the group velocities.  datatype 3: the layers take
@@ -371,5 +373,6 @@ def surf_likelihood(grid, points, params, freqs, src, rev, ttime, raystat, raylo
         _p(s0), _p(s1), ctypes.c_int(nrays), ctypes.c_int(gridx), ctypes.c_int(gridy),
         ctypes.c_int(sgref), ctypes.c_int(sgdic), ctypes.c_int(sgext), ctypes.c_int(order),
         ctypes.c_double(band), ctypes.c_int(nthreads), _p(out), _p(pt), _p(sd), _p(sg), _p(pv), _p(sid),
-        ctypes.c_int(phase_group), _p(group_vel), ctypes.c_int(straight), ctypes.c_int(datatype))
+        ctypes.c_int(phase_group), _p(group_vel), ctypes.c_int(straight), ctypes.c_int(datatype),
+        ctypes.c_double(water))
     return rc, out, pt, sd, sg, pv, sid

@@ -42,14 +42,22 @@ static void node_vprho(const double *vs, const double *gvp, const double *grho, 
     *rho = (double)1.74f * pow(*vp, 0.25);
 }
 
-/* convert_to_layer for one column (no water layer, no vbnd): a layer starts
- * where vs changes by more than EPS and takes the values of its first node;
- * the half-space (thickness 0) takes the bottom node's.  Returns the layer
- * count. */
+/* convert_to_layer for one column (no vbnd): with water > EPS a water layer
+ * first (vp = waterVel 1.5, vs = 0, rho = waterDensity 1, thickness water;
+ * src/likelihood_surf.F90:32-33,546-551); then a layer starts where vs
+ * changes by more than EPS and takes the values of its first node; the
+ * half-space (thickness 0) takes the bottom node's.  Thicknesses are scaled,
+ * except that with water > 0 the first layer's is water itself (:616-617:
+ * the water layer's, or for 0 < water <= EPS the first solid layer's).
+ * Returns the layer count. */
 static int column_layers(const double *vs, const double *gvp, const double *grho, int nz, double dz,
-                         double scaling, double *thick, double *vp, double *vsl, double *rho)
+                         double scaling, double water, double *thick, double *vp, double *vsl, double *rho)
 {
     int nl = 0, last_k = 1;
+    if (water > SURF_EPS) {
+        vp[0] = 1.5; vsl[0] = 0.0; rho[0] = 1.0; thick[0] = water;
+        nl = 1;
+    }
     double last_vs = vs[0];
     for (int k = 2; k <= nz; k++) {
         if (fabs(vs[k - 1] - last_vs) > SURF_EPS) {
@@ -64,17 +72,18 @@ static int column_layers(const double *vs, const double *gvp, const double *grho
     vsl[nl] = vs[nz - 1];
     node_vprho(vs, gvp, grho, nz - 1, &vp[nl], &rho[nl]);
     thick[nl] = 0.0;
+    if (water > 0.0) thick[0] = water;
     return nl + 1;
 }
 
-int orc_column_layers_all(long ncol, int nz, const double *vs, double dz, double scaling, int maxl,
-                          double *lay, int32_t *nl)
+int orc_column_layers_all(long ncol, int nz, const double *vs, double dz, double scaling, double water,
+                          int maxl, double *lay, int32_t *nl)
 {
     int bad = 0;
     #pragma omp parallel for schedule(static) reduction(|:bad)
     for (long c = 0; c < ncol; c++) {
-        double th[SURF_NMAX + 1], a[SURF_NMAX + 1], b[SURF_NMAX + 1], r[SURF_NMAX + 1];
-        int n = column_layers(vs + c * nz, NULL, NULL, nz, dz, scaling, th, a, b, r);
+        double th[SURF_NMAX + 2], a[SURF_NMAX + 2], b[SURF_NMAX + 2], r[SURF_NMAX + 2];
+        int n = column_layers(vs + c * nz, NULL, NULL, nz, dz, scaling, water, th, a, b, r);
         if (n > maxl) { bad = 1; continue; }
         nl[c] = n;
         for (int k = 0; k < n; k++) {
@@ -96,7 +105,7 @@ int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double
                         double band, int nthreads,
                         double *out, double *phase_time, double *srdist, double *sigma,
                         double *phase_vel, int32_t *sites_id, int phase_group, double *group_vel,
-                        int straight, int datatype)
+                        int straight, int datatype, double water)
 {
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -122,9 +131,10 @@ int orc_surf_likelihood(int nx, int ny, int nz, double xmin, double ymin, double
         /* columns in Fortran order: c = iy + ny*ix; pvel (np, ny, nx) */
         #pragma omp parallel for schedule(dynamic, 16) reduction(|:bad)
         for (long c = 0; c < C; c++) {
-            double th[SURF_NMAX + 1], a[SURF_NMAX + 1], b[SURF_NMAX + 1], r[SURF_NMAX + 1];
-            int nl = datatype == 3 ? column_layers(vs + c * nz, vp + c * nz, rho + c * nz, nz, dz, scaling, th, a, b, r)
-                                   : column_layers(vs + c * nz, NULL, NULL, nz, dz, scaling, th, a, b, r);
+            double th[SURF_NMAX + 2], a[SURF_NMAX + 2], b[SURF_NMAX + 2], r[SURF_NMAX + 2];
+            int nl = datatype == 3
+                         ? column_layers(vs + c * nz, vp + c * nz, rho + c * nz, nz, dz, scaling, water, th, a, b, r)
+                         : column_layers(vs + c * nz, NULL, NULL, nz, dz, scaling, water, th, a, b, r);
             int ierr = orc_surfmodes(th, a, b, r, nl, freqs, np, raylov, phase_group, dphase,
                                      pvel + c * np, gvel + c * np, NULL);
             if (ierr) bad = 1;

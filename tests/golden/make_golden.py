@@ -68,6 +68,37 @@ def surfdisp_golden():
     np.savez_compressed(os.path.join(OUT, "surfdisp96.npz"), n=n, **arrays)
 
 
+def surfdisp_water_golden():
+    """surfdisp96 on seeded models under a water layer (vp 1.5, vs 0, rho 1:
+    the layer convert_to_layer puts on top when T_GRID%waterDepth > EPS,
+    src/likelihood_surf.F90:32-33,546-551), Rayleigh and Love, phase and group."""
+    rng = np.random.default_rng(20261213)
+    arrays = {}
+    n = 0
+    for trial in range(24):
+        L = int(rng.integers(2, 16))
+        thk = rng.uniform(0.3, 3.0, L)
+        thk[-1] = 0
+        vs = rng.uniform(1.2, 4.6, L)
+        if trial % 2 == 0:
+            vs = np.sort(vs)
+        vp = vs * 1.73
+        rho = 1.74 * vp ** 0.25
+        w = rng.uniform(0.05, 4.0)
+        m = np.stack([np.r_[w, thk], np.r_[1.5, vp], np.r_[0.0, vs], np.r_[1.0, rho]]).astype(np.float32)
+        per = np.geomspace(0.2, 10, 20)
+        for iw in (1, 2):
+            for igr in (0, 1):
+                cp, cg, ierr, _ = po.surfdisp96(m[0], m[1], m[2], m[3], per, iw, igr, ref=True)
+                arrays["m_%d" % n] = m
+                arrays["t_%d" % n] = per
+                arrays["opt_%d" % n] = np.array([iw, igr, ierr])
+                arrays["cp_%d" % n] = cp
+                arrays["cg_%d" % n] = cg
+                n += 1
+    np.savez_compressed(os.path.join(OUT, "surfdisp96_water.npz"), n=n, **arrays)
+
+
 def fm2d_golden():
     """modrays phase times (fm2d/fm2dray_cartesian.f90) for one period on
     seeded smooth grids: first and mixed order, with and without the refined
@@ -161,9 +192,9 @@ def formats_golden():
 if __name__ == "__main__":
     if po.ref() is None:
         sys.exit("oracle/_ref/libmctref.so is not built (make -C oracle ref)")
-    kdtree_golden()
-    surfdisp_golden()
-    fm2d_golden()
-    fm2d_rays_golden()
-    formats_golden()
+    # python make_golden.py [set ...]: only the named sets (default: all)
+    sets = {"kdtree": kdtree_golden, "surfdisp": surfdisp_golden, "surfdisp_water": surfdisp_water_golden,
+            "fm2d": fm2d_golden, "fm2d_rays": fm2d_rays_golden, "formats": formats_golden}
+    for name in (sys.argv[1:] or list(sets)):
+        sets[name]()
     print("wrote golden vectors to", OUT)

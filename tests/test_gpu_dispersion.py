@@ -125,3 +125,54 @@ def test_random_columns_through_likelihood_kernels(mode, iw, igr):
     assert wp <= 1e-6 and wg <= 1e-6
     assert nexact >= 0.99 * ncol
     assert bad == 1      # the unsorted columns fail check_model
+
+
+@pytest.mark.parametrize("mode,water", [(3, 1.3), (1, 1.3), (2, 1.3), (3, 1e-12)])
+@pytest.mark.parametrize("iw,igr", [(2, 0), (1, 1), (2, 1)])
+def test_columns_under_a_water_layer(mode, water, iw, igr):
+    """T_GRID%waterDepth > EPS puts a water layer (vp 1.5, vs 0, rho 1,
+    thickness waterDepth, not scaled) on top of every column before surfdisp96
+    (src/likelihood_surf.F90:32-33,546-551); 0 < waterDepth <= EPS instead sets
+    the first layer's thickness to it (:616-617).  The likelihood's kernels
+    against the oracle's surfdisp96 (pinned to the compiled reference on
+    water models) on the layers that rule gives, bit for bit; with scaling 2
+    the solid layers' thicknesses are halved and the water's is not."""
+    rng = np.random.default_rng(900 + 10 * mode + iw * 2 + igr + (water < 1))
+    dz, nz, scaling = 0.35, 60, 2.0
+    ncol = 160
+    cols = np.zeros((ncol, nz))
+    POISSON = float(np.float32(1.73))
+    EPS = float(np.float32(1e-10))
+    for c in range(ncol):
+        if c % 5 == 4:
+            cols[c] = cols[int(rng.integers(0, c))]
+            continue
+        L = int(rng.integers(1, 10))
+        n = rng.integers(1, 8, L)
+        vs = rng.uniform(1.5, 4.8, L)
+        if c % 3:
+            vs = np.sort(vs)
+        v = np.repeat(vs, n)
+        cols[c, :min(nz, v.size)] = v[:nz]
+        cols[c, v.size:] = vs[-1]
+    per = np.geomspace(0.3, 10, 16)
+    say("water columns", mode, water, iw, igr)
+    ph, gr, ie, bad = mct.dispersion_columns(cols, dz, per, iw, igr, mode=mode, scaling=scaling, water=water)
+    nexact = 0
+    for c in range(ncol):
+        v = cols[c]
+        edges = [0] + [k for k in range(1, nz) if abs(v[k] - v[k - 1]) > EPS]
+        thk = np.array([(edges[i + 1] - edges[i]) * dz / scaling for i in range(len(edges) - 1)] + [0.0])
+        vsl = v[edges]
+        vp = vsl * POISSON
+        rho = float(np.float32(1.74)) * vp ** 0.25
+        if water > EPS:
+            thk, vp, vsl, rho = np.r_[water, thk], np.r_[1.5, vp], np.r_[0.0, vsl], np.r_[1.0, rho]
+        else:
+            thk[0] = water
+        cp, cg, e, _ = po.surfdisp96(thk, vp, vsl, rho, per, iw, igr)
+        assert ie[c] == e, (c, ie[c], e)
+        same = np.array_equal(ph[c], cp) and (not igr or np.array_equal(gr[c], cg))
+        nexact += same
+    say("water columns", mode, water, iw, igr, "exact", nexact, "/", ncol, "bad_model", bad)
+    assert nexact == ncol

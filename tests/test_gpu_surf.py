@@ -567,3 +567,56 @@ def test_identical_columns_solved_once_match_every_column(phase_group):
     say("  columns solved %d of %d" % (s1[0], s0[0]))
     assert 0 < s1[0] < s0[0] // 2 and s1[1] < w0[1]
     L.close()
+
+
+@pytest.mark.parametrize("raylov,water,batch", [(1, 0.8, 0), (1, 0.8, 1), (0, 2.5, 0)])
+def test_water_layer_matches_oracle(raylov, water, batch):
+    """T_GRID%waterDepth > EPS: every column gets a water layer on top
+    (src/likelihood_surf.F90:546-551, thickness not scaled :616-617) before
+    surfdisp96.  The plain call (and a 4-chain batch) against the oracle's
+    likelihood with the same water depth at the gates; the phase velocities
+    bit for bit.  Against a dry context: Rayleigh phase velocities change
+    (the water layer is not ignored), Love ones only within surfdisp96's root
+    tolerance (a fluid layer carries no Love wave; only the root search's
+    starting velocity moves)."""
+    seed = 20261202 if raylov == 1 else 20261207
+    w = W.surface_workload(n=41, nz=21, ncells=30, np_=10, nsrc=16, nrev=16, seed=seed, order=1)
+    w.raylov = raylov
+    say("surf water", raylov, water, batch)
+    g = w.grid
+    orc = po.surf_likelihood(gdict(g), w.points, w.params, w.freqs, w.src, w.rev, w.ttime, w.raystat,
+                             raylov=raylov, order=1, water=water)
+    if batch:
+        B = 4
+        L = ctx_for(w, water_depth=water, max_chains=B, max_cells=64)
+        P = np.zeros((B, 64, 3)); Q = np.zeros((B, 64, 3)); N = np.zeros(B, np.int32)
+        for c in range(B):
+            P[c, :w.points.shape[0]] = w.points; Q[c, :w.params.shape[0]] = w.params; N[c] = w.points.shape[0]
+        dP, dQ, dN = (mct.DeviceArray.from_numpy(a) for a in (P, Q, N))
+        dS0 = mct.DeviceArray.from_numpy(np.zeros((B, 2, w.freqs.shape[0])))
+        dO = mct.DeviceArray((B, 3), np.float64); dS = mct.DeviceArray((B,), np.int32)
+        L.likelihood_batch_dev(B, dP.ptr, dQ.ptr, dN.ptr, dS0.ptr, dO.ptr, dS.ptr)
+        L.sync()
+        out, st = dO.numpy(), dS.numpy()
+        assert np.all(st == orc[0])
+        if orc[0] == 0:
+            assert np.all(np.abs(out[:, 0] - orc[1][0]) <= 1e-6 * abs(orc[1][0]))
+        L.close()
+        return
+    L = ctx_for(w, water_depth=water)
+    rc, out, pt, sd, sg, pv = L.likelihood(w.points, w.params, arrays=True)
+    assert rc == orc[0], (rc, orc[0])
+    if rc == 0:
+        np.testing.assert_array_equal(pv, orc[5])
+        used = w.raystat[:, 0].reshape(pt.shape) == 1
+        dt = np.max(np.abs(pt[used] - orc[2][used]) / orc[2][used])
+        dl = abs(out[0] - orc[1][0]) / abs(orc[1][0])
+        say("surf water", raylov, water, "time max rel %.3g" % dt, "like rel %.3g" % dl)
+        assert dt <= 1e-4 and dl <= 1e-6
+    dry = ctx_for(w)
+    rcd, _, _, _, _, pvd = dry.likelihood(w.points, w.params, arrays=True)
+    if rc == 0 and rcd == 0:
+        d = np.max(np.abs(pvd - pv))
+        assert d > 1e-3 if raylov == 1 else d < 1e-4, d
+    dry.close()
+    L.close()

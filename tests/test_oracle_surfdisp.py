@@ -116,3 +116,63 @@ def test_normal_dispersion_and_group_below_phase():
             assert np.all(np.diff(cp) >= -1e-6)       # phase velocity grows with period
             cp1, cg, ierr, _ = po.surfdisp96(thk, vp, vs, rho, per, iw, 1)
             assert np.all(cg <= cp1 + 1e-4)
+
+
+GOLD_W = os.path.join(os.path.dirname(__file__), "golden", "surfdisp96_water.npz")
+
+
+def test_water_layer_matches_reference_golden_vectors():
+    """Models under a water layer (vs = 0 on top: surfdisp96's fluid-layer
+    branches, surfmodes/surfdisp96.f:1196-1215 among them), written by the
+    compiled reference (tests/golden/make_golden.py surfdisp_water)."""
+    g = np.load(GOLD_W)
+    for i in range(int(g["n"])):
+        thk, vp, vs, rho = g["m_%d" % i]
+        assert vs[0] == 0 and vp[0] == 1.5
+        iw, igr, ierr = g["opt_%d" % i]
+        cp, cg, e, nd = po.surfdisp96(thk, vp, vs, rho, g["t_%d" % i], int(iw), int(igr))
+        np.testing.assert_array_equal(cp, g["cp_%d" % i])
+        np.testing.assert_array_equal(cg, g["cg_%d" % i])
+        assert e == ierr
+
+
+def test_water_layer_matches_compiled_reference_live():
+    if po.ref() is None:
+        pytest.skip("oracle/_ref not built here")
+    rng = np.random.default_rng(78)
+    for trial in range(40):
+        L = int(rng.integers(2, 14))
+        thk = rng.uniform(0.2, 4.0, L); thk[-1] = 0
+        vs = np.sort(rng.uniform(1.0, 4.8, L)) if trial % 2 else rng.uniform(1.0, 4.8, L)
+        vp = vs * 1.73
+        rho = 1.74 * vp ** 0.25
+        w = rng.uniform(0.05, 4.0)
+        m = (np.r_[w, thk], np.r_[1.5, vp], np.r_[0.0, vs], np.r_[1.0, rho])
+        per = np.geomspace(0.2, 10, 25)
+        for iw in (1, 2):
+            for igr in (0, 1):
+                a = po.surfdisp96(*m, per, iw, igr)
+                b = po.surfdisp96(*m, per, iw, igr, ref=True)
+                np.testing.assert_array_equal(a[0], b[0])
+                np.testing.assert_array_equal(a[1], b[1])
+                assert a[2] == b[2]
+
+
+def test_thin_water_layer_limits():
+    """What the mathematics fixes for a water layer over a layered solid: at
+    periods long against the water depth the Rayleigh phase velocity tends to
+    the dry model's, and at periods short against it the fundamental mode is
+    trapped at the water-solid interface, slower than sound in water (a
+    Scholte wave).  A dropped water layer or a water layer read as solid fails
+    one of the two."""
+    thk = np.array([1.0, 2.0, 0.0]); vs = np.array([2.0, 3.0, 4.0])
+    vp = vs * 1.73; rho = 1.74 * vp ** 0.25
+    per_long = np.geomspace(20.0, 40.0, 5)
+    dry = po.surfdisp96(thk, vp, vs, rho, per_long, 2, 0)
+    wet = po.surfdisp96(np.r_[0.01, thk], np.r_[1.5, vp], np.r_[0.0, vs], np.r_[1.0, rho], per_long, 2, 0)
+    assert dry[2] == 0 and wet[2] == 0
+    assert np.max(np.abs(wet[0] - dry[0]) / dry[0]) < 5e-3
+    per_short = np.array([0.05, 0.08])
+    deep = po.surfdisp96(np.r_[3.0, thk], np.r_[1.5, vp], np.r_[0.0, vs], np.r_[1.0, rho], per_short, 2, 0)
+    assert deep[2] == 0
+    assert np.all(deep[0] < 1.5) and np.all(deep[0] > 1.0)
