@@ -474,7 +474,10 @@ __global__ __launch_bounds__(64) void k_fm2d_solve(mct_fm2d_args A)
     G.t = A.ttn + slot * wsz;
     G.s = A.nsts + slot * wsz;
     G.btg = A.heap + slot * (wsz + 8);
-    G.hk = A.heapk + slot * (wsz + 8);
+    // heap keys, 1-based, in arrays the stage leaves free: ttr during the refined
+    // stage (its copy of the refined grid is written after it), vr during the
+    // coarse one (the refined velocities are done with)
+    G.hk = A.vr + slot * wsz - 1;
     G.fom = A.fom;
     const double x = A.scx[s], z = A.scz[s];
     int isx = (int)((x - A.gox) / dnx) + 1, isz = (int)((z - A.goz) / dnz) + 1;
@@ -531,7 +534,9 @@ __global__ __launch_bounds__(64) void k_fm2d_solve(mct_fm2d_args A)
                 }
             }
         }
+        G.hk = A.ttr + slot * wsz - 1;
         if (l_travel(G, x, z, 1)) { atomicMin(&A.status[c], (int)MCT_ERR_OUTSIDE); return; }
+        G.hk = A.vr + slot * wsz - 1;
         // remap the refined grid onto the coarse grid
         double *tr = A.ttr + slot * wsz;
         int32_t *nr = A.nstr + slot * wsz;
@@ -692,14 +697,13 @@ extern "C" int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dv
     const size_t nv = (size_t)(nvx + 2) * (nvz + 2) * np, nn = (size_t)a.nnx * a.nnz * np;
     a.nnx = (nvx - 1) * gdx + 1; a.nnz = (nvz - 1) * gdz + 1;   // for the LDS check below
     const bool fast = method == 0 && mct_fm2d_fast_ok(a);
-    void *bufs[17] = {nullptr};
-    size_t sizes[17] = {nv * 8, 2 * nn * 8, 8 * (size_t)nsrc, 8 * (size_t)nsrc, 8 * (size_t)nrc, 8 * (size_t)nrc,
+    void *bufs[16] = {nullptr};
+    size_t sizes[16] = {nv * 8, 2 * nn * 8, 8 * (size_t)nsrc, 8 * (size_t)nsrc, 8 * (size_t)nrc, 8 * (size_t)nrc,
                         4 * (size_t)np * nsrc * nrc, 8 * (size_t)(nprob * wsz), 8 * (size_t)(nprob * wsz),
                         8 * (size_t)(nprob * wsz), 4 * (size_t)(nprob * wsz), 4 * (size_t)(nprob * wsz),
-                        4 * (size_t)(nprob * (wsz + 8)), 8 * (size_t)nprob * nrc, 4, 4 * (size_t)nprob,
-                        8 * (size_t)(nprob * (wsz + 8))};
+                        4 * (size_t)(nprob * (wsz + 8)), 8 * (size_t)nprob * nrc, 4, 4 * (size_t)nprob};
     int rc = MCT_OK;
-    for (int i = 0; i < 17 && !rc; i++)
+    for (int i = 0; i < 16 && !rc; i++)
         if (hipMalloc(&bufs[i], sizes[i]) != hipSuccess) rc = MCT_ERR_NOMEM;
     if (!rc && (hipMemcpy(bufs[0], velv, nv * 8, hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(bufs[2], scx, 8 * (size_t)nsrc, hipMemcpyHostToDevice) != hipSuccess ||
@@ -717,7 +721,6 @@ extern "C" int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dv
         a.srs = (const int32_t *)bufs[6];
         a.ttn = (double *)bufs[7]; a.ttr = (double *)bufs[8]; a.vr = (double *)bufs[9];
         a.nsts = (int32_t *)bufs[10]; a.nstr = (int32_t *)bufs[11]; a.heap = (int32_t *)bufs[12];
-        a.heapk = (double *)bufs[16];
         a.ttime = (double *)bufs[13]; a.status = (int32_t *)bufs[14];
         a.srs_pstride = (long)nsrc * nrc;
         rc = fast ? mct_launch_fm2d_fast(a, nprob, (int32_t *)bufs[15], 0) : mct_launch_fm2d(a, nprob, 0);
@@ -734,7 +737,7 @@ extern "C" int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dv
             for (long i = 0; i < nprob; i++) g_modrays_handover += pf[i] == 1;
         free(pf);
     }
-    for (int i = 0; i < 17; i++) if (bufs[i]) (void)hipFree(bufs[i]);
+    for (int i = 0; i < 16; i++) if (bufs[i]) (void)hipFree(bufs[i]);
     if (!rc && st < 0) rc = st;
     return rc;
 }

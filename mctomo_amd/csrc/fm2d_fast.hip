@@ -1481,7 +1481,7 @@ __global__ __launch_bounds__(256) void k_fm2d_sweep(mct_fm2d_args A, int32_t *pf
 // and streams its column's old times and velocities from memory FF_PF to
 // 2*FF_PF steps ahead; the strip's outer neighbour columns are staged in LDS.
 // A strip without dirty nodes is skipped.
-#define FF_PF 2
+#define FF_PF 4
 
 // Lane i takes lane i-1's value (wave_shr:1) / lane i+1's (wave_shl:1) by DPP
 // moves, two per double; a lane without a source keeps its own value, as
@@ -1630,24 +1630,29 @@ __global__ __launch_bounds__(64) void k_fm2d_strip(mct_fm2d_args A, int32_t *pfl
                 const int tmax = nnz - 1 + last;
                 const int tstart = tl > first ? tl : first;
                 int tend = th < tmax ? th : tmax;
-                // The stream's loads have no condition: an index outside the column
-                // reads a node inside it, and the value is replaced where it is used,
-                // so the loads stay in flight until then (a load under a condition
-                // would be waited for at once).
-                const long colTs = xin ? colT : 0, colVs = xin ? colV : 0;
-                auto zrow = [&](int bb) -> int {
-                    const int c = bb < 0 ? 0 : (bb >= nnz ? nnz - 1 : bb);
-                    return zs > 0 ? c : nnz - 1 - c;
-                };
+                // The stream's loads have no condition: a lane whose node is outside
+                // the grid reads the slot's first element (one line that every such
+                // lane of the instruction shares), and the value is replaced where it
+                // is used, so the loads stay in flight until then (a load under a
+                // condition would be waited for at once).
                 auto inb = [&](int bb) -> bool { return xin && bb >= 0 && bb < nnz; };
-                auto tload = [&](int bb) -> double { return T[colTs + zrow(bb)]; };
+                auto zrow = [&](int bb) -> int { return zs > 0 ? bb : nnz - 1 - bb; };
+                auto tload = [&](int bb) -> double {
+                    double v = 0.0;
+                    if (inb(bb)) v = T[colT + zrow(bb)];
+                    return v;
+                };
                 auto told = [&](int bb) -> double {
                     const double v = tload(bb);
                     return inb(bb) ? v : FF_BIG;
                 };
                 // the node's slowness 1.0 / veln (fouds' slown); used only where the
                 // node is in the column
-                auto vload = [&](int bb) -> double { return slnc[colVs + zrow(bb)]; };
+                auto vload = [&](int bb) -> double {
+                    double v = 1.0;
+                    if (inb(bb)) v = slnc[colV + zrow(bb)];
+                    return v;
+                };
                 constexpr int D = 2 + FOM;      // the stream delivers the old time D nodes ahead
                 const int b0 = tstart - lane;
                 // behind the start nothing has changed in this visit: the stored times

@@ -178,7 +178,6 @@ struct mct_fm2d_args {
     double *vr;                 // (nprob, ld*nc) refined velocity
     int32_t *nsts, *nstr;       // (nprob, ld*nc)
     int32_t *heap;              // (nprob, ld*nc+8) packed (px<<16)|pz
-    double *heapk;              // (nprob, ld*nc+8) the heap entries' keys
     double *ttime;              // (nchain, np, nsrc, nrc): phase times, receiver fastest
     int32_t *status;            // (nchain): MCT_ERR_OUTSIDE for a source or receiver outside the
                                 // grid; chains whose status is > 0 (rejected) are skipped
@@ -224,7 +223,7 @@ int mct_launch_straight_rays(const mct_fm2d_args &a, hipStream_t s);
 // Bytes of fm2d workspace per problem slot.
 static inline long mct_fm2d_slot_bytes(long ld, long nc)
 {
-    return ld * nc * (8 + 8 + 8 + 4 + 4) + (4 + 8) * (ld * nc + 8);
+    return ld * nc * (8 + 8 + 8 + 4 + 4) + 4 * (ld * nc + 8);
 }
 
 // ---- dispersion (surfdisp96 per column) ------------------------------------------

@@ -81,8 +81,8 @@ extern "C" int mct_modrays_dev_noalloc(int nvx, int nvz, double goxd, double goz
     // scratch layout: coarse velocities, then `chunk` problem slots of each array
     const size_t veln_b = up256(2 * 8 * (size_t)a.nnx * a.nnz * np);   // velocities, then slowness
     const size_t per = (size_t)mct_fm2d_slot_bytes(a.ld, a.nc) + 4;          // + the hand-over flag
-    if (veln_b + 7 * 256 + per > (size_t)MCT_NOALLOC_SCRATCH_BYTES) return MCT_ERR_SCRATCH;
-    long chunk = (long)(((size_t)MCT_NOALLOC_SCRATCH_BYTES - veln_b - 7 * 256) / per);
+    if (veln_b + 6 * 256 + per > (size_t)MCT_NOALLOC_SCRATCH_BYTES) return MCT_ERR_SCRATCH;
+    long chunk = (long)(((size_t)MCT_NOALLOC_SCRATCH_BYTES - veln_b - 6 * 256) / per);
     if (chunk > nprob) chunk = nprob;
     unsigned char *p = base;
     a.veln = (double *)p; a.sln = a.veln + (long)a.nnx * a.nnz * np; p += veln_b;
@@ -92,7 +92,6 @@ extern "C" int mct_modrays_dev_noalloc(int nvx, int nvz, double goxd, double goz
     a.nsts = (int32_t *)p; p += up256(4 * (size_t)chunk * wsz);
     a.nstr = (int32_t *)p; p += up256(4 * (size_t)chunk * wsz);
     a.heap = (int32_t *)p; p += up256(4 * (size_t)chunk * (wsz + 8));
-    a.heapk = (double *)p; p += up256(8 * (size_t)chunk * (wsz + 8));
     int32_t *pflag = (int32_t *)p; p += up256(4 * (size_t)chunk);
     if ((size_t)(p - base) > (size_t)MCT_NOALLOC_SCRATCH_BYTES) return MCT_ERR_SCRATCH;
     a.velv = velv_dev;

@@ -245,6 +245,14 @@ static void dispatcher_main(mct_server *S)
                     int rc = lane_create(M, &n);
                     lk.lock();
                     S->used -= M.lane_bytes;
+                    if (rc == MCT_ERR_NOMEM && M.nlanes > 1) {
+                        // the device is fuller than the budget says: make do with the
+                        // lanes the model has, and stop asking for more
+                        M.nlanes--;
+                        if (S->max_lanes > M.nlanes) S->max_lanes = M.nlanes;
+                        waiting = true;
+                        break;
+                    }
                     if (rc) {
                         M.nlanes--;
                         S->error = rc;

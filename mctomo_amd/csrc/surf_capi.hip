@@ -123,7 +123,7 @@ struct mct_surf_ctx {
     sbuf vs, flag_count, flag_nodes, kd_nodes, kd_ind, kd_rdata;
     sbuf phase, group, ierr, bad_model, ndltar, nlayer, lay, dsp_queue, dsp_jobs, dsp_keys, dsp_hist, velv, veln, phase_time, srdist, sigma;
     sbuf dsp_hkey, dsp_hrep, dsp_dup, dsp_njobs;
-    sbuf ttn, ttr, vr, nsts, nstr, heap, heapk, pflag, rounds;
+    sbuf ttn, ttr, vr, nsts, nstr, heap, pflag, rounds;
     // perturbed-box calls: accepted phase velocities per chain, the boxes of a call
     sbuf phase_keep, box;
     // group velocities: ray lengths (srdist) per chain and the accepted group velocities
@@ -273,7 +273,6 @@ static int surf_pipeline(mct_surf_ctx *C, int c0, int nchain, int part, int npar
     fa.vr = C->vr.as<double>() + soff * wsz;
     fa.nsts = C->nsts.as<int32_t>() + soff * wsz; fa.nstr = C->nstr.as<int32_t>() + soff * wsz;
     fa.heap = C->heap.as<int32_t>() + soff * (wsz + 8);
-    fa.heapk = C->heapk.as<double>() + soff * (wsz + 8);
     fa.ttime = phase_time; fa.status = d_status;
     fa.rounds_out = C->rounds.as<int32_t>() + 4 * soff;
     fa.ev_sweep = timing && events ? C->ev_sweep : nullptr;
@@ -385,11 +384,11 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     const long npair = (long)S.nsrc * S.nrev, gsz = (long)grid->nx * grid->ny * grid->nz;
     const long ncol = (long)grid->nx * grid->ny;
     const long nprob = (long)M * np * S.nsrc;
-    // fm2d workspace: problem slots for at most MCT_FM2D_WS_GB GiB (default 80:
+    // fm2d workspace: problem slots for at most MCT_FM2D_WS_GB GiB (default 64:
     // a 64-chain batch at 201x201 with 20 periods and 32 sources in one launch),
     // whole waves of 64; more problems run in chunks
     const long slot = mct_fm2d_slot_bytes(F.ld, F.nc);
-    long ws_gb = 80;
+    long ws_gb = 64;
     {
         const char *e = getenv("MCT_FM2D_WS_GB");
         if (e && atol(e) > 0) ws_gb = atol(e);
@@ -488,7 +487,6 @@ int mct_surf_create(const mct_grid *grid, const mct_surf_setup *setup, mct_surf_
     ALLOC(nsts, sizeof(int32_t) * wsz * ch);
     ALLOC(nstr, sizeof(int32_t) * wsz * ch);
     ALLOC(heap, sizeof(int32_t) * (wsz + 8) * ch);
-    ALLOC(heapk, sizeof(double) * (wsz + 8) * ch);
     ALLOC(pflag, sizeof(int32_t) * ch);
     ALLOC(rounds, sizeof(int32_t) * 4 * ch);
     if (!rc && hipMemset(C->rounds.p, 0, sizeof(int32_t) * 4 * ch) != hipSuccess) rc = MCT_ERR_HIP;
@@ -550,7 +548,7 @@ static std::vector<sbuf *> surf_bufs(mct_surf_ctx *C)
             &C->flag_nodes, &C->kd_nodes, &C->kd_ind, &C->kd_rdata, &C->phase, &C->group,
             &C->ierr, &C->bad_model, &C->ndltar, &C->nlayer, &C->lay, &C->dsp_queue, &C->dsp_jobs,
             &C->dsp_keys, &C->dsp_hist, &C->velv, &C->veln, &C->phase_time, &C->srdist, &C->sigma,
-            &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap, &C->heapk, &C->pflag, &C->rounds,
+            &C->ttn, &C->ttr, &C->vr, &C->nsts, &C->nstr, &C->heap, &C->pflag, &C->rounds,
             &C->phase_keep, &C->box, &C->rlen, &C->group_keep, &C->dsp_hkey, &C->dsp_hrep,
             &C->dsp_dup, &C->dsp_njobs, &C->vpg, &C->rhog};
 }
