@@ -628,13 +628,17 @@ def run_body(args):
     tb = like.timings()
     like.set_kernel_timing(False)
     G = w.grid.nx * w.grid.ny * w.grid.nz
+    # the context's eikonal precision: FP64 velocities and local solves unless the
+    # environment turns them off at create (times are FP64 either way)
+    f64 = os.environ.get("MCT_EIK_F64", "1") != "0"
     alg = B * w.nsrc * (4.0 * G + 4.0 * w.rev.shape[0])
     launches = max(1, tb["round_launches"])
     avg_ms = tb["round_kernels_ms"] / launches
     achieved = alg / launches / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-            "kernel": "k_eik_round<0,double>", "avg_launch_ms": round(avg_ms, 5), "launches": launches}
+            "kernel": "k_eik_round<%d,double,%s>" % (args.order, "double" if f64 else "float"),
+            "avg_launch_ms": round(avg_ms, 5), "launches": launches}
     cpu = None
     if not args.no_cpu:
         from oracle import pyoracle as po
@@ -650,7 +654,7 @@ def run_body(args):
     return {
         "metric": METRIC, "value": round(batched_eps, 3), "unit": "evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64" if f64 else "f32+f64",
         "data": "synthetic",
         "config": {"workload": "3-D body-wave eikonal drop-in: 100 Voronoi sites, 101^3 at 0.2 km, "
                                "50 sources x 400 receivers, P, order %d; batch of %d" % (args.order, B),
