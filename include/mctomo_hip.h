@@ -288,6 +288,19 @@ int mct_copy_d2d(void *dst_dev, const void *src_dev, int64_t bytes);
  * leave the setting unchanged. */
 int mct_set_tuning(mct_ctx *ctx, float band_tiles, int t64);
 
+/* How order 1 (LIKE_SET%order = 1, highAccuracySolveEikonal) is solved.  The
+ * reference's mixed-order times depend on the order in which its heap pops
+ * cells (fmm/fmm.hpp:92-158 over fmm/sfmm.hpp:44-52), not on the field alone.
+ * MCT_ORDER1_HEAP (the default) replays that heap loop exactly, one wave per
+ * (chain, wave type, source): the times equal the reference's SFMM.
+ * MCT_ORDER1_PARALLEL relaxes the same local update to its fixed point with
+ * the tiled solver used for order 0: much faster, within about 1e-2 relative
+ * of the reference.  The environment variable MCT_EIK_ORDER1=parallel sets the
+ * default.  The heap replay needs FP64 times (mct_set_tuning with t64 = 0
+ * selects the parallel method). */
+enum { MCT_ORDER1_HEAP = 0, MCT_ORDER1_PARALLEL = 1 };
+int mct_body_set_order1(mct_ctx *ctx, int method);
+
 /* Wait for the context's stream. */
 int mct_sync(mct_ctx *ctx);
 

@@ -44,6 +44,7 @@ template <class T> T *P(dbuf &b) { return (T *)b.p; }
 // Eikonal workspace for a set of problems on one geometry.
 struct eik_ws {
     dbuf T, flags0, flags1, key0, key1, counts, band;
+    dbuf heap, err;                  // heap replay (order 1): per-problem heaps, error bits
     uint32_t *h_count = nullptr;     // pinned
     int alloc(const mct_geom &G, long nprob)
     {
@@ -63,6 +64,7 @@ struct eik_ws {
     void release()
     {
         dfree(T); dfree(flags0); dfree(flags1); dfree(key0); dfree(key1); dfree(counts); dfree(band);
+        dfree(heap); dfree(err);
         if (h_count) (void)hipHostFree(h_count);
         h_count = nullptr;
     }
@@ -141,6 +143,29 @@ int eik_solve(const mct_eik_problem_set &Pset, eik_ws &ws, double hx, double hy,
     return MCT_OK;
 }
 
+// The reference's heap loop replayed exactly (order 1: its result depends on
+// the pop order).  T must be cleared and seeded by mct_launch_eik_init.
+int sfmm_solve(const mct_eik_problem_set &E, eik_ws &ws, const double *src_pts, int sstride, long schain,
+               double xmin, double ymin, double zmin, double lx, double ly, double lz, hipStream_t s)
+{
+    if (!E.t64 || !E.vel64) return MCT_ERR_ARG;
+    mct_sfmm_args a;
+    memset(&a, 0, sizeof a);
+    a.G = E.G; a.nprob = E.nprob; a.nsrc_eff = E.nsrc_eff; a.nwave = E.nwave; a.order = E.order;
+    a.T = (double *)E.T; a.V = E.vel64;
+    a.src = src_pts; a.src_stride = sstride; a.src_chain_stride = schain;
+    a.xmin = xmin; a.ymin = ymin; a.zmin = zmin; a.lx = lx; a.ly = ly; a.lz = lz;
+    a.hcap = mct_sfmm_heap_cap(E.G);
+    int rc;
+    if ((rc = dalloc(ws.heap, sizeof(uint32_t) * (size_t)a.hcap * (size_t)E.nprob))) return rc;
+    if ((rc = dalloc(ws.err, 4 * sizeof(uint32_t)))) return rc;
+    a.heap = (uint32_t *)ws.heap.p; a.err = (uint32_t *)ws.err.p;
+    if ((rc = mct_launch_sfmm(a, s))) return rc;
+    MCT_CHECK(hipMemcpyAsync(ws.h_count, a.err, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MCT_CHECK(hipStreamSynchronize(s));
+    return mct_sfmm_status(*ws.h_count);
+}
+
 } // namespace
 
 struct mct_ctx {
@@ -150,6 +175,7 @@ struct mct_ctx {
     float band_tiles;                // eikonal schedule: tiles the band advances per round
     int t64;                         // eikonal arrival times stored as double
     int f64;                         // eikonal velocities and local solves in FP64 (with t64)
+    int heap_order1;                 // order 1: replay the reference's heap loop (needs f64)
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double ms[4] = {0, 0, 0, 0};
@@ -221,7 +247,12 @@ static int body_pipeline(mct_ctx *C, int nchain, const double *d_points, const d
         return rc;
     int rounds = 0;
     if (C->timer) C->timer->used = 0;
-    if ((rc = eik_solve(E, C->eik, G.dx, G.dy, lz, s, &rounds, C->timer))) return rc;
+    if (E.order == 1 && C->heap_order1 && C->f64) {
+        if ((rc = sfmm_solve(E, C->eik, src_pts, sstride, schain, G.xmin, G.ymin, zmin, G.dx, G.dy, lz, s)))
+            return rc;
+    } else if ((rc = eik_solve(E, C->eik, G.dx, G.dy, lz, s, &rounds, C->timer))) {
+        return rc;
+    }
     MCT_CHECK(hipEventRecord(C->ev[2], s));
 
     // (d) picks + misfit
@@ -324,6 +355,7 @@ int mct_body_create(const mct_grid *grid, const mct_body_setup *setup, mct_ctx *
     C->band_tiles = mct_default_band_tiles();
     C->f64 = mct_default_f64();
     C->t64 = C->f64 ? 1 : mct_default_t64();
+    C->heap_order1 = mct_default_heap_order1();
     int rc = MCT_OK;
     const long npair = (long)setup->nsrc * setup->nrev;
     const int np = C->nwave, M = setup->max_chains, S = setup->max_cells;
@@ -595,6 +627,13 @@ int mct_set_tuning(mct_ctx *C, float band_tiles, int t64)
     if (band_tiles > 0.f) C->band_tiles = band_tiles;
     if (t64 >= 0) C->t64 = t64 ? 1 : 0;
     if (!C->t64) C->f64 = 0;   // FP32 arrival times: FP32 velocities and local solves too
+    return MCT_OK;
+}
+
+int mct_body_set_order1(mct_ctx *C, int method)
+{
+    if (!C || (method != MCT_ORDER1_HEAP && method != MCT_ORDER1_PARALLEL)) return MCT_ERR_ARG;
+    C->heap_order1 = method == MCT_ORDER1_HEAP;
     return MCT_OK;
 }
 

@@ -30,7 +30,7 @@ struct dropin_ws {
     std::mutex mu;
     hipStream_t s = nullptr;
     ws_buf vel_d, vel_t, T, fl0, fl1, k0, k1, band, counts, src, rev, tt, field, pts, prm,
-        sid, vp, vs, rho, pm, fcount, fnodes, kdn, kdi, kdr;
+        sid, vp, vs, rho, pm, fcount, fnodes, kdn, kdi, kdr, heap, err;
     uint32_t *h_count = nullptr;
     int init()
     {
@@ -154,7 +154,25 @@ extern "C" int fastMarching3D(int nx, int ny, int nz, double dx, double dy, doub
                                  (float *)W.band.p, s)))
         return rc;
     if ((rc = mct_launch_eik_init(E, (const double *)W.src.p, 3, 0, xmin, ymin, zmin, dx, dy, dz, s))) return rc;
-    if ((rc = eik_rounds(E, W.h_count, dx, dy, dz, s))) return rc;
+    if (E.order == 1 && f64 && mct_default_heap_order1()) {
+        // the reference's heap loop replayed (its mixed-order times depend on the pop order)
+        mct_sfmm_args a;
+        memset(&a, 0, sizeof a);
+        a.G = G; a.nprob = 1; a.nsrc_eff = 1; a.nwave = 1; a.order = 1;
+        a.T = (double *)W.T.p; a.V = (const double *)W.vel_t.p;
+        a.src = (const double *)W.src.p; a.src_stride = 3; a.src_chain_stride = 0;
+        a.xmin = xmin; a.ymin = ymin; a.zmin = zmin; a.lx = dx; a.ly = dy; a.lz = dz;
+        a.hcap = mct_sfmm_heap_cap(G);
+        if ((rc = W.heap.need(sizeof(uint32_t) * (size_t)a.hcap)) || (rc = W.err.need(4 * sizeof(uint32_t))))
+            return rc;
+        a.heap = (uint32_t *)W.heap.p; a.err = (uint32_t *)W.err.p;
+        if ((rc = mct_launch_sfmm(a, s))) return rc;
+        MCT_CHECK(hipMemcpyAsync(W.h_count, a.err, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        MCT_CHECK(hipStreamSynchronize(s));
+        if ((rc = mct_sfmm_status(*W.h_count))) return rc;
+    } else if ((rc = eik_rounds(E, W.h_count, dx, dy, dz, s))) {
+        return rc;
+    }
     if (nrev > 0) {
         mct_pick_args pa;
         memset(&pa, 0, sizeof pa);

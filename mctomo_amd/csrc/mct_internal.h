@@ -122,6 +122,32 @@ int mct_default_t64();
 // FP64 velocities and local solves: on unless the environment sets MCT_EIK_F64=0.
 int mct_default_f64();
 
+// Exact replay of the reference's SFMM heap loop (fmm/fmm.hpp:92-158), one
+// wave per problem, FP64 times and velocities.  T must hold +inf everywhere
+// before the launch (mct_launch_eik_init clears it); the kernel writes the
+// source cells itself.  heap: (nprob, hcap) scratch; err: one word of error
+// bits (read back with mct_sfmm_status); pops: (nprob) pop counts or NULL.
+struct mct_sfmm_args {
+    mct_geom G;
+    int nprob, nsrc_eff, nwave, order;
+    double *T;                     // (nprob, ntiles*64)
+    const double *V;               // (nchain*nwave, ntiles*64)
+    const double *src;             // source s of chain c at src + c*src_chain_stride + s*src_stride
+    int src_stride;
+    long src_chain_stride;
+    double xmin, ymin, zmin;       // subtracted from the source coordinates
+    double lx, ly, lz;             // leaf sizes
+    uint32_t *heap;
+    long hcap;
+    uint32_t *err;
+    uint32_t *pops;
+};
+long mct_sfmm_heap_cap(const mct_geom &G);
+int mct_launch_sfmm(const mct_sfmm_args &a, hipStream_t s);
+int mct_sfmm_status(uint32_t err_bits);
+// Order 1 replays the heap unless the environment sets MCT_EIK_ORDER1=parallel.
+int mct_default_heap_order1();
+
 // ---- picks + misfit -------------------------------------------------------------
 struct mct_pick_args {
     mct_geom G;

@@ -45,11 +45,9 @@ def test_joint_likelihood_vs_oracle(order):
     rcb, ob_o, rcs, os_o = oracle_joint(w, w.points, w.params, order)
     assert rc == 0 and rcb == 0 and rcs == 0
     say("  body like %.12g (oracle %.12g), surface like %.12g (oracle %.12g)" % (ob[0], ob_o[0], os_[0], os_o[0]))
-    # the body half at order 1 is the 3-D mixed-order SFMM, whose answer depends
-    # on the heap's history at the 1e-2 level (parity partial; order 0 is held
-    # to the gate)
-    tb = 1e-6 if order == 0 else 2e-2
-    assert np.all(np.abs(ob - ob_o) <= tb * np.abs(ob_o)), (ob, ob_o)
+    # the body half at order 1 replays the reference's SFMM heap loop, so both
+    # orders are held to the gate
+    assert np.all(np.abs(ob - ob_o) <= 1e-6 * np.abs(ob_o)), (ob, ob_o)
     assert np.all(np.abs(os_ - os_o) <= 1e-6 * np.abs(os_o)), (os_, os_o)
     assert np.all(np.abs(out - (ob + os_)) <= 1e-12 * np.abs(out))
     # the surface half depends on the sites' vp and rho: the datatype-2 layers
