@@ -16,6 +16,7 @@
 // The coarse B-spline velocity of a period is computed once by a separate
 // kernel (one thread per node) and shared by all sources of that period.
 #include "mct_internal.h"
+#include <stdlib.h>
 #include <string.h>
 
 #pragma clang fp contract(off)
@@ -635,7 +636,10 @@ int mct_launch_fm2d_gridder(const mct_fm2d_args &a, hipStream_t s)
 
 int mct_launch_fm2d_solve(const mct_fm2d_args &b, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fm2d_solve, dim3((unsigned)((b.nrun + 63) / 64)), dim3(64), 0, s, b);
+    const char *e = getenv("MCT_SOLVE_LANES");
+    int lanes = e ? atoi(e) : 64;
+    if (lanes != 8 && lanes != 16 && lanes != 32) lanes = 64;
+    hipLaunchKernelGGL(k_fm2d_solve, dim3((unsigned)((b.nrun + lanes - 1) / lanes)), dim3(lanes), 0, s, b);
     MCT_CHECK(hipGetLastError());
     return MCT_OK;
 }
