@@ -636,9 +636,12 @@ int mct_launch_fm2d_gridder(const mct_fm2d_args &a, hipStream_t s)
 
 int mct_launch_fm2d_solve(const mct_fm2d_args &b, hipStream_t s)
 {
+    // problems per wave (one per lane): 32 measured fastest for the order-1
+    // hand-over at 201x201 (5.2 s against 5.5 s with 64 per wave, 6.2 with 16);
+    // environment MCT_SOLVE_LANES = 8, 16, 32 or 64
     const char *e = getenv("MCT_SOLVE_LANES");
-    int lanes = e ? atoi(e) : 64;
-    if (lanes != 8 && lanes != 16 && lanes != 32) lanes = 64;
+    int lanes = e ? atoi(e) : 32;
+    if (lanes != 8 && lanes != 16 && lanes != 64) lanes = 32;
     hipLaunchKernelGGL(k_fm2d_solve, dim3((unsigned)((b.nrun + lanes - 1) / lanes)), dim3(lanes), 0, s, b);
     MCT_CHECK(hipGetLastError());
     return MCT_OK;
