@@ -626,6 +626,475 @@ __global__ __launch_bounds__(64) void k_fm2d_solve(mct_fm2d_args A)
     }
 }
 
+// waves per SIMD the wave solver is compiled for: 5 keeps it within 96 VGPRs
+// (measured fastest: 4.05 s for a 64-chain order-1 hand-over at 201x201,
+// against 4.29 s at 4 and 4.23 s at 6)
+#define WAVE_EU 5
+namespace {
+
+// ---- one wave per problem ----------------------------------------------------------
+// The same heap loop (travel, fm2d_ttime.f90:41-238) with the heap and its keys
+// in LDS, for the problems the parallel path hands over.  All 64 lanes run the
+// heap operations together (LDS reads broadcast, every lane holds the same
+// heap state); lane 0 alone writes global memory; lanes 0-3 load and solve the
+// popped node's four neighbours at once.  The state array keeps each heap
+// node's position, as the reference's does.  A neighbour already in the heap
+// needs that position for updtree: it arrives with the stencil, before the
+// heap moves, and every move of the pop (downtree, then the earlier
+// neighbours' addtree / updtree) is applied to it, so it is never loaded
+// twice.  A band that outgrows the LDS heap leaves the problem to the lane
+// solver.
+struct wave_grid {
+    int nnx, nnz, ld;
+    double gox, goz, dnx, dnz;
+    double *t;              // times (global)
+    int32_t *s;             // state: -1 far, 0 alive, >0 heap position (global)
+    const double *v;
+    int vld;
+    int32_t *hb;            // LDS heap, 1-based
+    double *hk;             // LDS keys, hk[p] = time of hb[p]
+    int cap, ntr, fom, full;
+    int vnl, vnr, vnt, vnb;
+};
+
+// heap positions of up to four neighbours, followed through the moves of a pop
+struct w_track {
+    int p[4];
+    __device__ __forceinline__ void move(int from, int to)
+    {
+#pragma unroll
+        for (int k = 0; k < 4; k++) p[k] = p[k] == from ? to : p[k];
+    }
+};
+
+__device__ __forceinline__ double w_rl(double v, int l)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffull), l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Lanes of one wave read what lane 0 wrote: keep the accesses in order.
+__device__ __forceinline__ void w_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void w_setpos(const wave_grid &G, int e, int pos)
+{
+    if (threadIdx.x == 0) LS(G, BPZ(e), BPX(e)) = pos;
+}
+
+// addtree (fm2d_ttime.f90:527-564)
+__device__ static void w_addtree(wave_grid &G, int iz, int ix, double tv, w_track &trk)
+{
+    if (G.ntr + 1 > G.cap) { G.full = 1; return; }
+    G.ntr = G.ntr + 1;
+    int tpc = G.ntr, tpp = tpc / 2;
+    while (tpp > 0) {
+        const int ep = G.hb[tpp];
+        const double kp = G.hk[tpp];
+        if (tv < kp) {
+            w_setpos(G, ep, tpc);
+            G.hb[tpc] = ep;
+            G.hk[tpc] = kp;
+            trk.move(tpp, tpc);
+            tpc = tpp;
+            tpp = tpc / 2;
+        } else {
+            tpp = 0;
+        }
+    }
+    const int e = (ix << 16) | iz;
+    w_setpos(G, e, tpc);
+    G.hb[tpc] = e;
+    G.hk[tpc] = tv;
+}
+
+// downtree (fm2d_ttime.f90:575-644)
+__device__ static void w_downtree(wave_grid &G, w_track &trk)
+{
+    if (G.ntr == 1) { G.ntr = G.ntr - 1; return; }
+    const int old = G.ntr;
+    const int el = G.hb[old];
+    const double kl = G.hk[old];
+    G.ntr = G.ntr - 1;
+    int tpp = 1, tpc = 2 * tpp;
+    while (tpc < G.ntr) {
+        const int e1 = G.hb[tpc], e2 = G.hb[tpc + 1];
+        const double k1 = G.hk[tpc], k2 = G.hk[tpc + 1];
+        int ec = e1;
+        double kc = k1;
+        if (k1 > k2) { tpc = tpc + 1; ec = e2; kc = k2; }
+        if (kc < kl) {
+            w_setpos(G, ec, tpp);
+            G.hb[tpp] = ec;
+            G.hk[tpp] = kc;
+            trk.move(tpc, tpp);
+            tpp = tpc;
+            tpc = 2 * tpp;
+        } else {
+            tpc = G.ntr + 1;
+        }
+    }
+    if (tpc == G.ntr) {
+        const int ec = G.hb[tpc];
+        const double kc = G.hk[tpc];
+        if (kc < kl) {
+            w_setpos(G, ec, tpp);
+            G.hb[tpp] = ec;
+            G.hk[tpp] = kc;
+            trk.move(tpc, tpp);
+            tpp = tpc;
+        }
+    }
+    w_setpos(G, el, tpp);
+    G.hb[tpp] = el;
+    G.hk[tpp] = kl;
+    trk.move(old, tpp);
+}
+
+// updtree (fm2d_ttime.f90:653-680) from heap position tpc
+__device__ static void w_updtree(wave_grid &G, int tpc, int iz, int ix, double tv, w_track &trk)
+{
+    int tpp = tpc / 2;
+    while (tpp > 0) {
+        const int ep = G.hb[tpp];
+        const double kp = G.hk[tpp];
+        if (tv < kp) {
+            w_setpos(G, ep, tpc);
+            G.hb[tpc] = ep;
+            G.hk[tpc] = kp;
+            trk.move(tpp, tpc);
+            tpc = tpp;
+            tpp = tpc / 2;
+        } else {
+            tpp = 0;
+        }
+    }
+    const int e = (ix << 16) | iz;
+    w_setpos(G, e, tpc);
+    G.hb[tpc] = e;
+    G.hk[tpc] = tv;
+}
+
+// travel (fm2d_ttime.f90:41-238) on the wave; -1 source outside, -2 heap full
+template <int FOM>
+__device__ static int w_travel(wave_grid &G, double scx, double scz, int urg)
+{
+    const int lane = threadIdx.x;
+    int isx = (int)((scx - G.gox) / G.dnx) + 1;
+    int isz = (int)((scz - G.goz) / G.dnz) + 1;
+    if (isx < 1 || isx > G.nnx || isz < 1 || isz > G.nnz) return -1;
+    if (isx == G.nnx) isx = isx - 1;
+    if (isz == G.nnz) isz = isz - 1;
+    G.ntr = 0;
+    G.full = 0;
+    w_track none;
+    none.p[0] = none.p[1] = none.p[2] = none.p[3] = 0;
+    if (urg == 2) {
+        // the band in the reference's scan order (x outer, z inner)
+        const long n = (long)G.nnx * G.nnz;
+        for (long q0 = 0; q0 < n; q0 += 64) {
+            const long q = q0 + lane;
+            int i = 1, j = 1;
+            bool band = false;
+            double tq = 0.0;
+            if (q < n) {
+                i = (int)(q / G.nnz) + 1;
+                j = (int)(q % G.nnz) + 1;
+                band = LS(G, j, i) > 0;
+                if (band) tq = LT(G, j, i);
+            }
+            unsigned long long m = __ballot(band);
+            while (m) {
+                const int b = __builtin_ctzll(m);
+                m &= m - 1;
+                const int bi = __builtin_amdgcn_readlane(i, b), bj = __builtin_amdgcn_readlane(j, b);
+                w_addtree(G, bj, bi, w_rl(tq, b), none);
+                if (G.full) return -2;
+            }
+            w_sync();
+        }
+    } else {
+        for (long q = lane; q < (long)G.nnx * G.nnz; q += 64) {
+            const int i = (int)(q / G.nnz) + 1, j = (int)(q % G.nnz) + 1;
+            LS(G, j, i) = -1;
+        }
+        w_sync();
+        double vss[2][2];
+        for (int i = 1; i <= 2; i++)
+            for (int j = 1; j <= 2; j++) vss[i - 1][j - 1] = LV(G, isz - 1 + j, isx - 1 + i);
+        double dsx = (scx - G.gox) - (isx - 1) * G.dnx;
+        double dsz = (scz - G.goz) - (isz - 1) * G.dnz;
+        double vsrc = l_bilinear(vss, dsx, dsz, G.dnx, G.dnz);
+        for (int i = 1; i <= 2; i++)
+            for (int j = 1; j <= 2; j++) {
+                double ex = dsx - (i - 1) * G.dnx, ez = dsz - (j - 1) * G.dnz;
+                double ds = sqrt(ex * ex + ez * ez);
+                const double tc = 2.0 * ds / (vss[i - 1][j - 1] + vsrc);
+                if (lane == 0) LT(G, isz - 1 + j, isx - 1 + i) = tc;
+                w_addtree(G, isz - 1 + j, isx - 1 + i, tc, none);
+            }
+        w_sync();
+    }
+    // lane k < 4 takes neighbour k (x-1, x+1, z-1, z+1: the reference's order)
+    const int kn = lane & 3;
+    const int dx = kn == 0 ? -1 : (kn == 1 ? 1 : 0), dz = kn == 2 ? -1 : (kn == 3 ? 1 : 0);
+    while (G.ntr > 0) {
+        const int e = G.hb[1];
+        const int ix = BPX(e), iz = BPZ(e);
+        if (urg == 1) {   // stop when the band reaches an inner edge of the refined grid
+            int swrg = 0;
+            if (ix == 1 && G.vnl != 1) swrg = 1;
+            if (ix == G.nnx && G.vnr != G.nnx) swrg = 1;
+            if (iz == 1 && G.vnt != 1) swrg = 1;
+            if (iz == G.nnz && G.vnb != G.nnz) swrg = 1;
+            if (swrg == 1) {
+                if (lane == 0) LS(G, iz, ix) = 0;
+                w_sync();
+                break;
+            }
+        }
+        if (lane == 0) LS(G, iz, ix) = 0;
+        w_sync();
+        // the neighbours' states (heap positions) and stencils, read before the heap moves
+        const int nx = ix + dx, nz = iz + dz;
+        const bool in = lane < 4 && nx >= 1 && nx <= G.nnx && nz >= 1 && nz <= G.nnz;
+        const int cx = in ? nx : ix, cz = in ? nz : iz;
+        const int st = in ? LS(G, cz, cx) : 0;
+        lane_grid L;
+        L.nnx = G.nnx; L.nnz = G.nnz; L.ld = G.ld; L.t = G.t; L.s = G.s; L.v = G.v; L.vld = G.vld; L.fom = FOM;
+        l_sten S;
+        l_load_sten(L, cz, cx, S);
+        const int st0 = __builtin_amdgcn_readlane(st, 0), st1 = __builtin_amdgcn_readlane(st, 1);
+        const int st2 = __builtin_amdgcn_readlane(st, 2), st3 = __builtin_amdgcn_readlane(st, 3);
+        w_track trk;
+        trk.p[0] = st0 > 0 ? st0 : 0; trk.p[1] = st1 > 0 ? st1 : 0;
+        trk.p[2] = st2 > 0 ? st2 : 0; trk.p[3] = st3 > 0 ? st3 : 0;
+        w_downtree(G, trk);
+        double tv = 0.0;
+        if (st != 0) tv = FOM == 0 ? l_fouds1s(S, G.dnx, G.dnz) : l_fouds2s(S, G.dnx, G.dnz);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int stk = k == 0 ? st0 : (k == 1 ? st1 : (k == 2 ? st2 : st3));
+            if (stk == 0) continue;
+            const double tvk = w_rl(tv, k);
+            const int xk = ix + (k == 0 ? -1 : (k == 1 ? 1 : 0)), zk = iz + (k == 2 ? -1 : (k == 3 ? 1 : 0));
+            if (lane == 0) LT(G, zk, xk) = tvk;
+            if (stk == -1) {
+                w_addtree(G, zk, xk, tvk, trk);
+            } else {
+                const int pos = trk.p[k];
+                trk.p[k] = 0;
+                w_updtree(G, pos, zk, xk, tvk, trk);
+            }
+            if (G.full) return -2;
+        }
+        w_sync();
+    }
+    return 0;
+}
+
+// One wave per handed-over problem (flag 1): modrays' loop body for that
+// source, as k_fm2d_solve runs it; the flag becomes 3 when the problem is
+// solved here and stays 1 when its band outgrew the LDS heap.
+template <int FOM>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) void k_fm2d_wave(mct_fm2d_args A, int heap_cap, int32_t *pflag)
+{
+    extern __shared__ double w_lds[];
+    const int lane = threadIdx.x;
+    const long slot = blockIdx.x;
+    if (slot >= A.nrun || pflag[slot] != 1) return;
+    const long prob = A.prob0 + slot;
+    const int s = (int)(prob % A.nsrc);
+    const long cp = prob / A.nsrc;
+    const int p = (int)(cp % A.np);
+    const int c = (int)(cp / A.np);
+    if (A.status[c] > 0) return;
+    const int32_t *srs = A.srs + (long)p * A.srs_pstride + (long)s * A.nrc;
+    const long wsz = (long)A.ld * A.nc;
+    const double *velnc = A.veln + cp * (long)A.nnx * A.nnz;
+    const double *velv = A.velv + cp * (long)(A.nvz + 2) * (A.nvx + 2);
+    const int ldv = A.nvz + 2;
+    const double dnx = A.dvx / A.gdx, dnz = A.dvz / A.gdz;
+    wave_grid G;
+    G.ld = A.ld;
+    G.t = A.ttn + slot * wsz;
+    G.s = A.nsts + slot * wsz;
+    G.hk = w_lds;
+    G.hb = (int32_t *)(w_lds + heap_cap + 1);
+    G.cap = heap_cap;
+    G.fom = A.fom;
+    const double x = A.scx[s], z = A.scz[s];
+    int isx = (int)((x - A.gox) / dnx) + 1, isz = (int)((z - A.goz) / dnz) + 1;
+    if (isx < 1 || isx > A.nnx || isz < 1 || isz > A.nnz) return;   // the lane solver reports it
+    if (isx == A.nnx) isx = isx - 1;
+    if (isz == A.nnz) isz = isz - 1;
+    int vnl = isx - A.sgs, vnr = isx + A.sgs, vnt = isz - A.sgs, vnb = isz + A.sgs;
+    if (vnl < 1) vnl = 1;
+    if (vnr > A.nnx) vnr = A.nnx;
+    if (vnt < 1) vnt = 1;
+    if (vnb > A.nnz) vnb = A.nnz;
+    if (A.asgr == 1) {
+        // refined source grid with B-spline velocities (bsplrefine), one node per lane
+        double *vr = A.vr + slot * wsz;
+        G.nnx = (vnr - vnl) * A.sgdl + 1;
+        G.nnz = (vnb - vnt) * A.sgdl + 1;
+        G.dnx = A.dvx / (double)(float)(A.gdx * A.sgdl);
+        G.dnz = A.dvz / (double)(float)(A.gdz * A.sgdl);
+        G.gox = A.gox + dnx * (vnl - 1);
+        G.goz = A.goz + dnz * (vnt - 1);
+        G.vnl = vnl; G.vnr = vnr; G.vnt = vnt; G.vnb = vnb;
+        G.v = vr; G.vld = A.ld;
+        const int nrxr = A.gdx * A.sgdl, nrzr = A.gdz * A.sgdl;
+        const int origx = (vnl - 1) * A.sgdl + 1, origz = (vnt - 1) * A.sgdl + 1;
+        const int rn = G.nnx * G.nnz;
+        for (int q = lane; q < rn; q += 64) {
+            const int idm2 = q / G.nnz + 1, idm1 = q % G.nnz + 1;
+            const int st1 = idm1 + origz - 1, st2 = idm2 + origx - 1;
+            int ii = (st1 - 1) / nrzr + 1, k = (st1 - 1) % nrzr + 1;
+            if (ii > A.nvz - 1) { ii = A.nvz - 1; k = nrzr + 1; }
+            int jj = (st2 - 1) / nrxr + 1, l = (st2 - 1) % nrxr + 1;
+            if (jj > A.nvx - 1) { jj = A.nvx - 1; l = nrxr + 1; }
+            double u = nrxr; u = (l - 1) / u;
+            double v = nrzr; v = (k - 1) / v;
+            double ui[4], vi[4];
+            bspl_w(u, ui);
+            bspl_w(v, vi);
+            double sum[4];
+            for (int i1 = 1; i1 <= 4; i1++) {
+                sum[i1 - 1] = 0.0;
+                for (int j1 = 1; j1 <= 4; j1++)
+                    sum[i1 - 1] = sum[i1 - 1] + ui[j1 - 1] * velv[(jj - 2 + j1) * ldv + (ii - 2 + i1)];
+                sum[i1 - 1] = vi[i1 - 1] * sum[i1 - 1];
+            }
+            vr[(long)(idm2 - 1) * A.ld + (idm1 - 1)] = sum[0] + sum[1] + sum[2] + sum[3];
+        }
+        w_sync();
+        const int rc = w_travel<FOM>(G, x, z, 1);
+        if (rc) return;
+        // remap the refined grid onto the coarse grid
+        double *tr = A.ttr + slot * wsz;
+        int32_t *nr = A.nstr + slot * wsz;
+        const int rnx = G.nnx, rnz = G.nnz;
+        for (int q = lane; q < rnx * rnz; q += 64) {
+            const long o = (long)(q / rnz) * A.ld + (q % rnz);
+            tr[o] = G.t[o];
+            nr[o] = G.s[o];
+        }
+        w_sync();
+        for (long q = lane; q < (long)A.nnx * A.nnz; q += 64)
+            G.s[(q / A.nnz) * A.ld + (q % A.nnz)] = -1;
+        w_sync();
+        const int wkz = (rnz - 1) / A.sgdl + 1, wkx = (rnx - 1) / A.sgdl + 1;
+        for (int q = lane; q < wkz * wkx; q += 64) {
+            const int k = (q % wkz) * A.sgdl + 1, l = (q / wkz) * A.sgdl + 1;
+            const int idm1 = vnt + (k - 1) / A.sgdl, idm2 = vnl + (l - 1) / A.sgdl;
+            const int st = nr[(long)(l - 1) * A.ld + (k - 1)];
+            G.s[(long)(idm2 - 1) * A.ld + (idm1 - 1)] = st;
+            if (st >= 0) G.t[(long)(idm2 - 1) * A.ld + (idm1 - 1)] = tr[(long)(l - 1) * A.ld + (k - 1)];
+        }
+        w_sync();
+        G.nnx = A.nnx; G.nnz = A.nnz; G.dnx = dnx; G.dnz = dnz; G.gox = A.gox; G.goz = A.goz;
+        G.v = velnc; G.vld = A.nnz;
+        // complete the narrow band: alive nodes next to far nodes become close
+        for (long q = lane; q < (long)A.nnx * A.nnz; q += 64) {
+            const int k = (int)(q / A.nnz) + 1, l = (int)(q % A.nnz) + 1;
+            if (LS(G, l, k) != 0) continue;
+            if ((l - 1 >= 1 && LS(G, l - 1, k) == -1) || (l + 1 <= A.nnz && LS(G, l + 1, k) == -1) ||
+                (k - 1 >= 1 && LS(G, l, k - 1) == -1) || (k + 1 <= A.nnx && LS(G, l, k + 1) == -1))
+                LS(G, l, k) = 1;
+        }
+        w_sync();
+        if (w_travel<FOM>(G, x, z, 2)) return;
+    } else {
+        G.nnx = A.nnx; G.nnz = A.nnz; G.dnx = dnx; G.dnz = dnz; G.gox = A.gox; G.goz = A.goz;
+        G.v = velnc; G.vld = A.nnz;
+        if (w_travel<FOM>(G, x, z, 0)) return;
+    }
+    w_sync();
+    // srtimes: receiver picks, one receiver per lane
+    double *out = A.ttime + prob * A.nrc;
+    for (int r = lane + 1; r <= A.nrc; r += 64) {
+        if (srs[r - 1] == 0) continue;
+        const double rx = A.rcx[r - 1], rz = A.rcz[r - 1];
+        int irx = (int)floor((rx - A.gox) / dnx) + 1;
+        int irz = (int)floor((rz - A.goz) / dnz) + 1;
+        if (irx < 1 || irx > A.nnx || irz < 1 || irz > A.nnz) { atomicMin(&A.status[c], (int)MCT_ERR_OUTSIDE); continue; }
+        if (irx == A.nnx) irx = irx - 1;
+        if (irz == A.nnz) irz = irz - 1;
+        int jsx = (int)floor((x - A.gox) / dnx) + 1, jsz = (int)floor((z - A.goz) / dnz) + 1;
+        double dpl = dnx, rd1 = dnz;
+        if (rd1 < dpl) dpl = rd1;
+        double sred = (x - rx) * (x - rx);
+        sred = sred + (z - rz) * (z - rz);
+        sred = sqrt(sred);
+        int sw = 0;
+        if (sred < dpl) sw = 1;
+        if (jsx == irx && jsz == irz) sw = 1;
+        double trr;
+        if (sw == 1) {
+            double vss[2][2];
+            int csx = jsx < A.nnx ? jsx : A.nnx - 1, csz = jsz < A.nnz ? jsz : A.nnz - 1;
+            csx = csx < 1 ? 1 : csx;
+            csz = csz < 1 ? 1 : csz;
+            for (int k = 1; k <= 2; k++)
+                for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = velnc[(long)(csx - 1 + k - 1) * A.nnz + (csz - 1 + l - 1)];
+            double drx = (x - A.gox) - (jsx - 1) * dnx, drz = (z - A.goz) - (jsz - 1) * dnz;
+            double vels = l_bilinear(vss, drx, drz, dnx, dnz);
+            for (int k = 1; k <= 2; k++)
+                for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = velnc[(long)(irx - 1 + k - 1) * A.nnz + (irz - 1 + l - 1)];
+            drx = (rx - A.gox) - (irx - 1) * dnx;
+            drz = (rz - A.goz) - (irz - 1) * dnz;
+            double velr = l_bilinear(vss, drx, drz, dnx, dnz);
+            trr = 2.0 * sred / (vels + velr);
+        } else {
+            double drx = (rx - A.gox) - (irx - 1) * dnx;
+            double drz = (rz - A.goz) - (irz - 1) * dnz;
+            trr = 0.0;
+            for (int k = 1; k <= 2; k++)
+                for (int l = 1; l <= 2; l++) {
+                    double produ = (1.0 - fabs(((l - 1) * dnz - drz) / dnz)) * (1.0 - fabs(((k - 1) * dnx - drx) / dnx));
+                    trr = trr + LT(G, irz - 1 + l, irx - 1 + k) * produ;
+                }
+        }
+        out[r - 1] = trr;
+    }
+    if (lane == 0) pflag[slot] = 3;
+}
+
+} // namespace
+
+// LDS entries of k_fm2d_wave's heap (environment MCT_WAVE_HEAP, default 512:
+// 6 KB per wave, so registers and not LDS bound the waves per CU; a larger
+// heap takes fewer problems back to the lane solver but was slower overall,
+// 4.33 s with 768 entries and 5.35 s with 1024)
+static int w_heap_cap()
+{
+    static int cap = 0;
+    if (!cap) {
+        const char *e = getenv("MCT_WAVE_HEAP");
+        int v = e ? atoi(e) : 512;
+        cap = v >= 256 && v <= 8192 ? v : 512;
+    }
+    return cap;
+}
+
+int mct_launch_fm2d_wave(const mct_fm2d_args &b, int32_t *pflag, hipStream_t s)
+{
+    if (!pflag || b.nrun < 1) return MCT_ERR_ARG;
+    const int cap = w_heap_cap();
+    const size_t lds = 12 * (size_t)(cap + 1);
+    if (b.fom == 0) hipLaunchKernelGGL(k_fm2d_wave<0>, dim3((unsigned)b.nrun), dim3(64), lds, s, b, cap, pflag);
+    else hipLaunchKernelGGL(k_fm2d_wave<1>, dim3((unsigned)b.nrun), dim3(64), lds, s, b, cap, pflag);
+    MCT_CHECK(hipGetLastError());
+    return MCT_OK;
+}
+
 int mct_launch_fm2d_gridder(const mct_fm2d_args &a, hipStream_t s)
 {
     const long nn = (long)a.nnx * a.nnz * a.nchain * a.np;
@@ -741,7 +1210,7 @@ extern "C" int mct_modrays(int nvx, int nvz, double goxd, double gozd, double dv
     if (!rc && fast) {
         int32_t *pf = (int32_t *)malloc(4 * (size_t)nprob);
         if (pf && hipMemcpy(pf, bufs[15], 4 * (size_t)nprob, hipMemcpyDeviceToHost) == hipSuccess)
-            for (long i = 0; i < nprob; i++) g_modrays_handover += pf[i] == 1;
+            for (long i = 0; i < nprob; i++) g_modrays_handover += pf[i] == 1 || pf[i] == 3;
         free(pf);
     }
     for (int i = 0; i < 16; i++) if (bufs[i]) (void)hipFree(bufs[i]);

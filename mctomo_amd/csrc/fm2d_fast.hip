@@ -1987,6 +1987,10 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
         const char *ev = getenv("MCT_FM2D_VERIFY");
         verify = ev ? (atoi(ev) ? 1 : -1) : (a.fom == 1 ? 1 : -1);
     }
+    // the wave-per-problem sequential solver for the handed-over problems unless
+    // the environment sets MCT_FM2D_WAVE=0 (one lane per problem for all of them)
+    const char *ew = getenv("MCT_FM2D_WAVE");
+    const int wave_off = ew && atoi(ew) == 0;
     for (long p0 = 0; p0 < nprob; p0 += chunk) {
         mct_fm2d_args b = a;
         b.verify = verify > 0 ? 1 : -1;
@@ -2015,7 +2019,9 @@ int mct_launch_fm2d_fast(const mct_fm2d_args &a, long chunk, int32_t *pflag, hip
             MCT_CHECK(hipGetLastError());
         }
         // problems whose heap loop left the LDS neighbourhood, or that the check
-        // above failed: the sequential solver
+        // above failed: the sequential heap loop, one wave per problem, and one
+        // lane per problem for the few whose band outgrows that wave's LDS heap
+        if (!wave_off && (rc = mct_launch_fm2d_wave(b, pflag, s))) return rc;
         b.only_flag = pflag;
         if ((rc = mct_launch_fm2d_solve(b, s))) return rc;
         if ((rc = mct_launch_fm2d_rays(b, s))) return rc;

@@ -234,6 +234,10 @@ struct mct_fm2d_args {
 int mct_launch_fm2d(const mct_fm2d_args &a, long chunk, hipStream_t s);
 int mct_launch_fm2d_gridder(const mct_fm2d_args &a, hipStream_t s);
 int mct_launch_fm2d_solve(const mct_fm2d_args &b, hipStream_t s);
+// The problems whose flag is 1 (handed over by the parallel path), one wave
+// each with the heap in LDS; a solved problem's flag becomes 3, one whose band
+// outgrew the LDS heap keeps 1 (for mct_launch_fm2d_solve).
+int mct_launch_fm2d_wave(const mct_fm2d_args &b, int32_t *pflag, hipStream_t s);
 // Parallel path: seed and source neighbourhood in LDS per problem, coarse
 // sweeps per problem, the sequential solver for flagged problems.  pflag
 // holds one int per workspace slot.  mct_fm2d_fast_ok tells whether the

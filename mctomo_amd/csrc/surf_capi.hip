@@ -932,7 +932,7 @@ int mct_surf_fm2d_handover(mct_surf_ctx *C, int64_t *handover2)
     for (long i = 0; i < n; i++) {
         const int b = r[4 * i + 3];
         chk += b != 0;
-        all += r[4 * n + i] == 1;
+        all += r[4 * n + i] == 1 || r[4 * n + i] == 3;
         for (int k = 0; k < 7; k++) why[k] += (b >> k) & 1;
     }
     free(r);
