@@ -173,9 +173,8 @@ def run_surface(args):
     log("2-D stage: near-source %.2f ms, sweep %.2f ms" % like.timings2d())
 
     # ---- the same batch with the mixed-order update (the reference examples'
-    # LIKE_SET%ORDER = 1): the parallel 2-D path checks each converged field
-    # against the heap loop's freeze order and hands the problems that fail to
-    # the sequential solver, so the times are the reference's
+    # LIKE_SET%ORDER = 1): its 2-D times depend on the heap loop's pop order,
+    # so the 2-D stage runs the heap loop itself, one wave per problem
     order1 = None
     if args.order == 0 and args.order1_steps > 0:
         like1 = mct.SurfLikelihood(w.grid, w.freqs, w.src, w.rev, w.ttime, w.raystat, raylov=1, order=1,
@@ -198,7 +197,7 @@ def run_surface(args):
                   "statuses": {str(k): int(v) for k, v in zip(*np.unique(st1, return_counts=True))},
                   "stage_ms": [round(v, 3) for v in like1.timings()],
                   "problems": int(B * w.freqs.shape[0] * w.nsrc),
-                  "handed_to_sequential": int(handed), "failed_heap_order_check": int(checked)}
+                  "solved_by_heap_loop": int(handed), "failed_heap_order_check": int(checked)}
         log("order 1 batched:", order1)
 
     # ---- perturbed-box proposals (the secondary unit) ----------------------------

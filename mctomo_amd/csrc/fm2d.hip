@@ -899,9 +899,10 @@ __device__ static int w_travel(wave_grid &G, double scx, double scz, int urg)
     return 0;
 }
 
-// One wave per handed-over problem (flag 1): modrays' loop body for that
-// source, as k_fm2d_solve runs it; the flag becomes 3 when the problem is
-// solved here and stays 1 when its band outgrew the LDS heap.
+// One wave per problem whose flag is 1: modrays' loop body for that source, as
+// k_fm2d_solve runs it; the flag becomes 3 when the problem is solved here, 2
+// when there is nothing to solve, and stays 1 when its band outgrew the LDS
+// heap (or its source lies outside the grid, which the lane solver reports).
 template <int FOM>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) void k_fm2d_wave(mct_fm2d_args A, int heap_cap, int32_t *pflag)
 {
@@ -914,8 +915,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) v
     const long cp = prob / A.nsrc;
     const int p = (int)(cp % A.np);
     const int c = (int)(cp / A.np);
-    if (A.status[c] > 0) return;
     const int32_t *srs = A.srs + (long)p * A.srs_pstride + (long)s * A.nrc;
+    {
+        // a rejected model, or a source without pairs (modrays skips it, :211):
+        // nothing to do
+        int any = 0;
+        for (int r = lane; r < A.nrc; r += 64) any |= srs[r];
+        if (A.status[c] > 0 || (!__ballot(any != 0) && s != 0)) {
+            if (lane == 0) pflag[slot] = 2;
+            return;
+        }
+    }
     const long wsz = (long)A.ld * A.nc;
     const double *velnc = A.veln + cp * (long)A.nnx * A.nnz;
     const double *velv = A.velv + cp * (long)(A.nvz + 2) * (A.nvx + 2);

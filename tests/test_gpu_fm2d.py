@@ -8,10 +8,12 @@ source and sweeps the rest to the fixed point "each node's update uses the
 neighbours with smaller times".  For the first-order update (fom 0) that is
 the heap loop's answer.  The mixed-order update (fom 1) can raise a node's key
 when a neighbour freezes (fouds2 overwrites, updtree only sifts up), and then
-the heap loop freezes nodes out of time order, which no fixed point follows;
-k_fm2d_verify checks every converged field against the heap loop's freeze
-order and hands the problems that fail to the sequential solver.  Both orders
-are held to 1e-9 relative (identical in practice).
+the heap loop freezes nodes out of time order, which no fixed point follows.
+So at fom 1 method 0 runs the heap loop itself, one wave per problem; with
+MCT_FM2D_ORDER1=check it sweeps instead, and k_fm2d_verify checks every
+converged field against the heap loop's freeze order and hands the problems
+that fail to the sequential solvers.  Both orders are held to 1e-9 relative
+(identical in practice).
 """
 import os
 import sys
@@ -150,6 +152,7 @@ def test_strip_sweep_matches_anti_diagonal_sweep(n, blk, monkeypatch):
     ns, nr = 5, 9
     sx, sz = rng.uniform(h, L - h, ns), rng.uniform(h, L - h, ns)
     rx, rz = rng.uniform(0, L, nr), rng.uniform(0, L, nr)
+    monkeypatch.setenv("MCT_FM2D_ORDER1", "check")   # fom 1 sweeps too
     for fom in (0, 1):
         for asgr in (0, 1):
             say("fm2d strip vs diagonal", n, blk, "fom", fom, "asgr", asgr)
@@ -166,6 +169,7 @@ def test_heap_order_check_is_what_makes_order_one_exact(monkeypatch):
     some of them (the negative control); with it on, the problems where it
     would are handed to the sequential solver and every time equals the
     oracle's."""
+    monkeypatch.setenv("MCT_FM2D_ORDER1", "check")
     worst_off = worst_on = 0.0
     handed = total = 0
     for gd, sgdl, sgs in [((1, 1), 4, 8), ((2, 2), 4, 8), ((2, 1), 2, 3), ((1, 3), 5, 2)]:
@@ -192,3 +196,30 @@ def test_heap_order_check_is_what_makes_order_one_exact(monkeypatch):
     assert worst_off > 1e-9          # the fixed point alone is not the heap loop's answer here
     assert handed > 0
     assert worst_on <= 1e-9
+
+
+def test_order_one_default_path_equals_the_checked_sweeps(monkeypatch):
+    """The default mixed-order path (the heap loop, one wave per problem, for
+    every problem) and the sweeps with the heap-order check give the same
+    times, both the oracle's, on node-scale random media with a refined source
+    grid; a lane-only run (MCT_FM2D_WAVE=0) agrees bit for bit."""
+    rng = np.random.default_rng(77)
+    nvx, nvz, h = 41, 33, 0.5
+    npd, ns, nr = 4, 8, 11
+    velv = rng.uniform(2.0, 4.5, (npd, nvx + 2, nvz + 2))
+    L, W = (nvx - 1) * h, (nvz - 1) * h
+    sx, sz = rng.uniform(h, L - h, ns), rng.uniform(h, W - h, ns)
+    rx, rz = rng.uniform(0, L, nr), rng.uniform(0, W, nr)
+    kw = dict(gdx=2, gdz=2, asgr=1, sgdl=4, sgs=8, fom=1)
+    want = np.stack([po.fm2d(velv[p], 0, 0, h, h, sx, sz, rx, rz, **kw) for p in range(npd)])
+    default = mct.modrays(velv, 0, 0, h, h, sx, sz, rx, rz, method=0, **kw)
+    monkeypatch.setenv("MCT_FM2D_WAVE", "0")
+    lanes = mct.modrays(velv, 0, 0, h, h, sx, sz, rx, rz, method=0, **kw)
+    monkeypatch.delenv("MCT_FM2D_WAVE")
+    monkeypatch.setenv("MCT_FM2D_ORDER1", "check")
+    checked = mct.modrays(velv, 0, 0, h, h, sx, sz, rx, rz, method=0, **kw)
+    say("order 1 default vs checked sweeps: identical", np.array_equal(default, checked))
+    np.testing.assert_array_equal(default, lanes)
+    agree(default, want, 0)
+    agree(checked, want, 0)
+    np.testing.assert_allclose(default, checked, rtol=1e-9, atol=0)
