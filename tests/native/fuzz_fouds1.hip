@@ -14,6 +14,7 @@ int mct_launch_fm2d_gridder(const mct_fm2d_args &, hipStream_t) { return 0; }
 int mct_launch_fm2d_solve(const mct_fm2d_args &, hipStream_t) { return 0; }
 int mct_launch_fm2d_rays(const mct_fm2d_args &, hipStream_t) { return 0; }
 int mct_launch_fm2d_wave(const mct_fm2d_args &, int32_t *, hipStream_t) { return 0; }
+int mct_fill_i32(int32_t *, long, int32_t, hipStream_t) { return 0; }
 
 int main(int argc, char **argv)
 {
