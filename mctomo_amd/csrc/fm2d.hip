@@ -626,24 +626,34 @@ __global__ __launch_bounds__(64) void k_fm2d_solve(mct_fm2d_args A)
     }
 }
 
-// waves per SIMD the wave solver is compiled for: 5 keeps it within 96 VGPRs
-// (measured fastest: 4.05 s for a 64-chain order-1 hand-over at 201x201,
-// against 4.29 s at 4 and 4.23 s at 6)
-#define WAVE_EU 5
+// Lanes per problem in the grouped heap-loop kernel (64 / GRP_LANES problems per
+// wave) and the waves per SIMD it is compiled for.  LDS bounds the problems in
+// flight (a 512-entry heap is 6.4 KB: about 24 per CU), so registers are not
+// the limit.  Measured on a 64-chain order-1 batch at 201x201 (40,960 problems):
+// 16 lanes 1.84 s, 8 lanes 1.90 s, 4 lanes 2.69 s, one problem per wave 2.31 s;
+// heaps of 384, 640 and 768 entries 2.55, 2.13 and 2.53 s (a smaller heap
+// sends more problems to the lane solver, a larger one runs fewer at once).
+#ifndef WAVE_EU
+#define WAVE_EU 2
+#endif
+#ifndef GRP_LANES
+#define GRP_LANES 16
+#endif
 namespace {
 
-// ---- one wave per problem ----------------------------------------------------------
-// The same heap loop (travel, fm2d_ttime.f90:41-238) with the heap and its keys
-// in LDS, for the problems the parallel path hands over.  All 64 lanes run the
-// heap operations together (LDS reads broadcast, every lane holds the same
-// heap state); lane 0 alone writes global memory; lanes 0-3 load and solve the
-// popped node's four neighbours at once.  The state array keeps each heap
-// node's position, as the reference's does.  A neighbour already in the heap
-// needs that position for updtree: it arrives with the stencil, before the
-// heap moves, and every move of the pop (downtree, then the earlier
-// neighbours' addtree / updtree) is applied to it, so it is never loaded
-// twice.  A band that outgrows the LDS heap leaves the problem to the lane
-// solver.
+// ---- the heap loop, several problems per wave ----------------------------------------
+// The reference's heap loop (travel, fm2d_ttime.f90:41-238) for the mixed-order
+// problems, with each problem's heap and keys in LDS.  A wave runs 64/GL
+// problems, one group of GL lanes each: every lane of a group holds its
+// problem's state and runs the heap operations (so one vector instruction
+// advances every problem of the wave), the group's first lane alone writes
+// global memory, and lanes 0-3 of a group load and solve the popped node's four
+// neighbours.  The state array keeps each heap node's position, as the
+// reference's does.  A neighbour already in the heap needs that position for
+// updtree: it arrives with the stencil, the downtree's moves are logged in LDS
+// while the loads are in flight and replayed on it, and the neighbours'
+// addtree / updtree moves are applied as they happen, so it is loaded once.  A
+// band that outgrows the LDS heap leaves the problem to the lane solver.
 struct wave_grid {
     int nnx, nnz, ld;
     double gox, goz, dnx, dnz;
@@ -651,10 +661,13 @@ struct wave_grid {
     int32_t *s;             // state: -1 far, 0 alive, >0 heap position (global)
     const double *v;
     int vld;
-    int32_t *hb;            // LDS heap, 1-based
-    double *hk;             // LDS keys, hk[p] = time of hb[p]
-    int cap, ntr, fom, full;
+    int32_t *hb;            // the group's LDS heap, 1-based
+    double *hk;             // its keys, hk[p] = time of hb[p]
+    int2 *mv;               // its log of a downtree's moves (at most 32)
+    int cap, ntr, full;
     int vnl, vnr, vnt, vnb;
+    int base;               // the group's first lane
+    bool lead;              // this lane is it
 };
 
 // heap positions of up to four neighbours, followed through the moves of a pop
@@ -667,15 +680,12 @@ struct w_track {
     }
 };
 
-__device__ __forceinline__ double w_rl(double v, int l)
+__device__ __forceinline__ double w_shfl_d(double v, int src)
 {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffull), l);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    return __shfl(v, src, 64);
 }
 
-// Lanes of one wave read what lane 0 wrote: keep the accesses in order.
+// Lanes of one wave read what a group's first lane wrote: keep the accesses in order.
 __device__ __forceinline__ void w_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -685,7 +695,7 @@ __device__ __forceinline__ void w_sync()
 
 __device__ __forceinline__ void w_setpos(const wave_grid &G, int e, int pos)
 {
-    if (threadIdx.x == 0) LS(G, BPZ(e), BPX(e)) = pos;
+    if (G.lead) LS(G, BPZ(e), BPX(e)) = pos;
 }
 
 // addtree (fm2d_ttime.f90:527-564)
@@ -714,9 +724,11 @@ __device__ static void w_addtree(wave_grid &G, int iz, int ix, double tv, w_trac
     G.hk[tpc] = tv;
 }
 
-// downtree (fm2d_ttime.f90:575-644)
-__device__ static void w_downtree(wave_grid &G, w_track &trk)
+// downtree (fm2d_ttime.f90:575-644); its moves (from, to) go to the group's
+// log, their count to nmv
+__device__ static void w_downtree(wave_grid &G, int &nmv)
 {
+    nmv = 0;
     if (G.ntr == 1) { G.ntr = G.ntr - 1; return; }
     const int old = G.ntr;
     const int el = G.hb[old];
@@ -733,7 +745,8 @@ __device__ static void w_downtree(wave_grid &G, w_track &trk)
             w_setpos(G, ec, tpp);
             G.hb[tpp] = ec;
             G.hk[tpp] = kc;
-            trk.move(tpc, tpp);
+            G.mv[nmv] = make_int2(tpc, tpp);
+            nmv++;
             tpp = tpc;
             tpc = 2 * tpp;
         } else {
@@ -747,14 +760,16 @@ __device__ static void w_downtree(wave_grid &G, w_track &trk)
             w_setpos(G, ec, tpp);
             G.hb[tpp] = ec;
             G.hk[tpp] = kc;
-            trk.move(tpc, tpp);
+            G.mv[nmv] = make_int2(tpc, tpp);
+            nmv++;
             tpp = tpc;
         }
     }
     w_setpos(G, el, tpp);
     G.hb[tpp] = el;
     G.hk[tpp] = kl;
-    trk.move(old, tpp);
+    G.mv[nmv] = make_int2(old, tpp);
+    nmv++;
 }
 
 // updtree (fm2d_ttime.f90:653-680) from heap position tpc
@@ -781,11 +796,12 @@ __device__ static void w_updtree(wave_grid &G, int tpc, int iz, int ix, double t
     G.hk[tpc] = tv;
 }
 
-// travel (fm2d_ttime.f90:41-238) on the wave; -1 source outside, -2 heap full
-template <int FOM>
+// travel (fm2d_ttime.f90:41-238) on a group; -1 source outside, -2 heap full.
+// Every lane of the group takes the same path through it.
+template <int FOM, int GL>
 __device__ static int w_travel(wave_grid &G, double scx, double scz, int urg)
 {
-    const int lane = threadIdx.x;
+    const int gl = threadIdx.x - G.base;
     int isx = (int)((scx - G.gox) / G.dnx) + 1;
     int isz = (int)((scz - G.goz) / G.dnz) + 1;
     if (isx < 1 || isx > G.nnx || isz < 1 || isz > G.nnz) return -1;
@@ -796,10 +812,11 @@ __device__ static int w_travel(wave_grid &G, double scx, double scz, int urg)
     w_track none;
     none.p[0] = none.p[1] = none.p[2] = none.p[3] = 0;
     if (urg == 2) {
-        // the band in the reference's scan order (x outer, z inner)
+        // the band in the reference's scan order (x outer, z inner), GL nodes at a time
         const long n = (long)G.nnx * G.nnz;
-        for (long q0 = 0; q0 < n; q0 += 64) {
-            const long q = q0 + lane;
+        const unsigned long long gmask = GL == 64 ? ~0ull : ((1ull << GL) - 1ull);
+        for (long q0 = 0; q0 < n; q0 += GL) {
+            const long q = q0 + gl;
             int i = 1, j = 1;
             bool band = false;
             double tq = 0.0;
@@ -809,18 +826,18 @@ __device__ static int w_travel(wave_grid &G, double scx, double scz, int urg)
                 band = LS(G, j, i) > 0;
                 if (band) tq = LT(G, j, i);
             }
-            unsigned long long m = __ballot(band);
+            unsigned long long m = (__ballot(band) >> G.base) & gmask;
             while (m) {
                 const int b = __builtin_ctzll(m);
                 m &= m - 1;
-                const int bi = __builtin_amdgcn_readlane(i, b), bj = __builtin_amdgcn_readlane(j, b);
-                w_addtree(G, bj, bi, w_rl(tq, b), none);
+                const int bi = __shfl(i, G.base + b, 64), bj = __shfl(j, G.base + b, 64);
+                w_addtree(G, bj, bi, w_shfl_d(tq, G.base + b), none);
                 if (G.full) return -2;
             }
             w_sync();
         }
     } else {
-        for (long q = lane; q < (long)G.nnx * G.nnz; q += 64) {
+        for (long q = gl; q < (long)G.nnx * G.nnz; q += GL) {
             const int i = (int)(q / G.nnz) + 1, j = (int)(q % G.nnz) + 1;
             LS(G, j, i) = -1;
         }
@@ -836,13 +853,13 @@ __device__ static int w_travel(wave_grid &G, double scx, double scz, int urg)
                 double ex = dsx - (i - 1) * G.dnx, ez = dsz - (j - 1) * G.dnz;
                 double ds = sqrt(ex * ex + ez * ez);
                 const double tc = 2.0 * ds / (vss[i - 1][j - 1] + vsrc);
-                if (lane == 0) LT(G, isz - 1 + j, isx - 1 + i) = tc;
+                if (G.lead) LT(G, isz - 1 + j, isx - 1 + i) = tc;
                 w_addtree(G, isz - 1 + j, isx - 1 + i, tc, none);
             }
         w_sync();
     }
-    // lane k < 4 takes neighbour k (x-1, x+1, z-1, z+1: the reference's order)
-    const int kn = lane & 3;
+    // lane k < 4 of the group takes neighbour k (x-1, x+1, z-1, z+1: the reference's order)
+    const int kn = gl & 3;
     const int dx = kn == 0 ? -1 : (kn == 1 ? 1 : 0), dz = kn == 2 ? -1 : (kn == 3 ? 1 : 0);
     while (G.ntr > 0) {
         const int e = G.hb[1];
@@ -854,37 +871,48 @@ __device__ static int w_travel(wave_grid &G, double scx, double scz, int urg)
             if (iz == 1 && G.vnt != 1) swrg = 1;
             if (iz == G.nnz && G.vnb != G.nnz) swrg = 1;
             if (swrg == 1) {
-                if (lane == 0) LS(G, iz, ix) = 0;
+                if (G.lead) LS(G, iz, ix) = 0;
                 w_sync();
                 break;
             }
         }
-        if (lane == 0) LS(G, iz, ix) = 0;
+        if (G.lead) LS(G, iz, ix) = 0;
         w_sync();
         // the neighbours' states (heap positions) and stencils, read before the heap moves
         const int nx = ix + dx, nz = iz + dz;
-        const bool in = lane < 4 && nx >= 1 && nx <= G.nnx && nz >= 1 && nz <= G.nnz;
+        const bool in = gl < 4 && nx >= 1 && nx <= G.nnx && nz >= 1 && nz <= G.nnz;
         const int cx = in ? nx : ix, cz = in ? nz : iz;
         const int st = in ? LS(G, cz, cx) : 0;
         lane_grid L;
         L.nnx = G.nnx; L.nnz = G.nnz; L.ld = G.ld; L.t = G.t; L.s = G.s; L.v = G.v; L.vld = G.vld; L.fom = FOM;
         l_sten S;
         l_load_sten(L, cz, cx, S);
-        const int st0 = __builtin_amdgcn_readlane(st, 0), st1 = __builtin_amdgcn_readlane(st, 1);
-        const int st2 = __builtin_amdgcn_readlane(st, 2), st3 = __builtin_amdgcn_readlane(st, 3);
+        // the heap moves while the loads are in flight; the neighbours'
+        // positions then follow the logged moves, each lane its own
+        int nmv = 0;
+        w_downtree(G, nmv);
+        w_sync();
+        int mypos = st > 0 ? st : 0;
+        for (int q = 0; q < nmv; q++) {
+            const int2 m = G.mv[q];
+            mypos = mypos == m.x ? m.y : mypos;
+        }
+        const int st0 = __shfl(st, G.base + 0, 64), st1 = __shfl(st, G.base + 1, 64);
+        const int st2 = __shfl(st, G.base + 2, 64), st3 = __shfl(st, G.base + 3, 64);
         w_track trk;
-        trk.p[0] = st0 > 0 ? st0 : 0; trk.p[1] = st1 > 0 ? st1 : 0;
-        trk.p[2] = st2 > 0 ? st2 : 0; trk.p[3] = st3 > 0 ? st3 : 0;
-        w_downtree(G, trk);
+        trk.p[0] = __shfl(mypos, G.base + 0, 64); trk.p[1] = __shfl(mypos, G.base + 1, 64);
+        trk.p[2] = __shfl(mypos, G.base + 2, 64); trk.p[3] = __shfl(mypos, G.base + 3, 64);
         double tv = 0.0;
         if (st != 0) tv = FOM == 0 ? l_fouds1s(S, G.dnx, G.dnz) : l_fouds2s(S, G.dnx, G.dnz);
+        const double tv0 = w_shfl_d(tv, G.base + 0), tv1 = w_shfl_d(tv, G.base + 1);
+        const double tv2 = w_shfl_d(tv, G.base + 2), tv3 = w_shfl_d(tv, G.base + 3);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int stk = k == 0 ? st0 : (k == 1 ? st1 : (k == 2 ? st2 : st3));
             if (stk == 0) continue;
-            const double tvk = w_rl(tv, k);
+            const double tvk = k == 0 ? tv0 : (k == 1 ? tv1 : (k == 2 ? tv2 : tv3));
             const int xk = ix + (k == 0 ? -1 : (k == 1 ? 1 : 0)), zk = iz + (k == 2 ? -1 : (k == 3 ? 1 : 0));
-            if (lane == 0) LT(G, zk, xk) = tvk;
+            if (G.lead) LT(G, zk, xk) = tvk;
             if (stk == -1) {
                 w_addtree(G, zk, xk, tvk, trk);
             } else {
@@ -892,25 +920,30 @@ __device__ static int w_travel(wave_grid &G, double scx, double scz, int urg)
                 trk.p[k] = 0;
                 w_updtree(G, pos, zk, xk, tvk, trk);
             }
-            if (G.full) return -2;
         }
+        if (G.full) return -2;
         w_sync();
     }
     return 0;
 }
 
-// One wave per problem whose flag is 1: modrays' loop body for that source, as
-// k_fm2d_solve runs it; the flag becomes 3 when the problem is solved here, 2
-// when there is nothing to solve, and stays 1 when its band outgrew the LDS
-// heap (or its source lies outside the grid, which the lane solver reports).
-template <int FOM>
+// One group of GL lanes per problem whose flag is 1: modrays' loop body for
+// that source, as k_fm2d_solve runs it; the flag becomes 3 when the problem is
+// solved here, 2 when there is nothing to solve, and stays 1 when its band
+// outgrew the LDS heap (or its source lies outside the grid, which the lane
+// solver reports).  Every branch below is taken by whole groups.
+template <int FOM, int GL>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) void k_fm2d_wave(mct_fm2d_args A, int heap_cap, int32_t *pflag)
 {
     extern __shared__ double w_lds[];
+    constexpr int NG = 64 / GL;
     const int lane = threadIdx.x;
-    const long slot = blockIdx.x;
-    if (slot >= A.nrun || pflag[slot] != 1) return;
-    const long prob = A.prob0 + slot;
+    const int g = lane / GL, gl = lane % GL;
+    const long slot = (long)blockIdx.x * NG + g;
+    // a group with nothing to do still runs with the others (its lanes mask out)
+    bool live = slot < A.nrun;
+    if (live) live = pflag[slot] == 1;
+    const long prob = A.prob0 + (live ? slot : 0);
     const int s = (int)(prob % A.nsrc);
     const long cp = prob / A.nsrc;
     const int p = (int)(cp % A.np);
@@ -920,12 +953,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) v
         // a rejected model, or a source without pairs (modrays skips it, :211):
         // nothing to do
         int any = 0;
-        for (int r = lane; r < A.nrc; r += 64) any |= srs[r];
-        if (A.status[c] > 0 || (!__ballot(any != 0) && s != 0)) {
-            if (lane == 0) pflag[slot] = 2;
-            return;
+        if (live)
+            for (int r = gl; r < A.nrc; r += GL) any |= srs[r];
+        const unsigned long long gmask = GL == 64 ? ~0ull : ((1ull << GL) - 1ull);
+        const bool anyg = ((__ballot(any != 0) >> (g * GL)) & gmask) != 0;
+        if (live && (A.status[c] > 0 || (!anyg && s != 0))) {
+            if (gl == 0) pflag[slot] = 2;
+            live = false;
         }
     }
+    if (!live) return;
     const long wsz = (long)A.ld * A.nc;
     const double *velnc = A.veln + cp * (long)A.nnx * A.nnz;
     const double *velv = A.velv + cp * (long)(A.nvz + 2) * (A.nvx + 2);
@@ -935,10 +972,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) v
     G.ld = A.ld;
     G.t = A.ttn + slot * wsz;
     G.s = A.nsts + slot * wsz;
-    G.hk = w_lds;
-    G.hb = (int32_t *)(w_lds + heap_cap + 1);
+    {
+        const int per = heap_cap + 1 + (heap_cap + 2) / 2 + 32;   // doubles per group
+        double *gb = w_lds + (long)g * per;
+        G.hk = gb;
+        G.hb = (int32_t *)(gb + heap_cap + 1);
+        G.mv = (int2 *)(gb + heap_cap + 1 + (heap_cap + 2) / 2);
+    }
     G.cap = heap_cap;
-    G.fom = A.fom;
+    G.base = g * GL;
+    G.lead = gl == 0;
     const double x = A.scx[s], z = A.scz[s];
     int isx = (int)((x - A.gox) / dnx) + 1, isz = (int)((z - A.goz) / dnz) + 1;
     if (isx < 1 || isx > A.nnx || isz < 1 || isz > A.nnz) return;   // the lane solver reports it
@@ -950,7 +993,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) v
     if (vnt < 1) vnt = 1;
     if (vnb > A.nnz) vnb = A.nnz;
     if (A.asgr == 1) {
-        // refined source grid with B-spline velocities (bsplrefine), one node per lane
+        // refined source grid with B-spline velocities (bsplrefine), a node per lane
         double *vr = A.vr + slot * wsz;
         G.nnx = (vnr - vnl) * A.sgdl + 1;
         G.nnz = (vnb - vnt) * A.sgdl + 1;
@@ -963,7 +1006,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) v
         const int nrxr = A.gdx * A.sgdl, nrzr = A.gdz * A.sgdl;
         const int origx = (vnl - 1) * A.sgdl + 1, origz = (vnt - 1) * A.sgdl + 1;
         const int rn = G.nnx * G.nnz;
-        for (int q = lane; q < rn; q += 64) {
+        for (int q = gl; q < rn; q += GL) {
             const int idm2 = q / G.nnz + 1, idm1 = q % G.nnz + 1;
             const int st1 = idm1 + origz - 1, st2 = idm2 + origx - 1;
             int ii = (st1 - 1) / nrzr + 1, k = (st1 - 1) % nrzr + 1;
@@ -985,23 +1028,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) v
             vr[(long)(idm2 - 1) * A.ld + (idm1 - 1)] = sum[0] + sum[1] + sum[2] + sum[3];
         }
         w_sync();
-        const int rc = w_travel<FOM>(G, x, z, 1);
-        if (rc) return;
+        if (w_travel<FOM, GL>(G, x, z, 1)) return;
         // remap the refined grid onto the coarse grid
         double *tr = A.ttr + slot * wsz;
         int32_t *nr = A.nstr + slot * wsz;
         const int rnx = G.nnx, rnz = G.nnz;
-        for (int q = lane; q < rnx * rnz; q += 64) {
+        for (int q = gl; q < rnx * rnz; q += GL) {
             const long o = (long)(q / rnz) * A.ld + (q % rnz);
             tr[o] = G.t[o];
             nr[o] = G.s[o];
         }
         w_sync();
-        for (long q = lane; q < (long)A.nnx * A.nnz; q += 64)
+        for (long q = gl; q < (long)A.nnx * A.nnz; q += GL)
             G.s[(q / A.nnz) * A.ld + (q % A.nnz)] = -1;
         w_sync();
         const int wkz = (rnz - 1) / A.sgdl + 1, wkx = (rnx - 1) / A.sgdl + 1;
-        for (int q = lane; q < wkz * wkx; q += 64) {
+        for (int q = gl; q < wkz * wkx; q += GL) {
             const int k = (q % wkz) * A.sgdl + 1, l = (q / wkz) * A.sgdl + 1;
             const int idm1 = vnt + (k - 1) / A.sgdl, idm2 = vnl + (l - 1) / A.sgdl;
             const int st = nr[(long)(l - 1) * A.ld + (k - 1)];
@@ -1012,7 +1054,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) v
         G.nnx = A.nnx; G.nnz = A.nnz; G.dnx = dnx; G.dnz = dnz; G.gox = A.gox; G.goz = A.goz;
         G.v = velnc; G.vld = A.nnz;
         // complete the narrow band: alive nodes next to far nodes become close
-        for (long q = lane; q < (long)A.nnx * A.nnz; q += 64) {
+        for (long q = gl; q < (long)A.nnx * A.nnz; q += GL) {
             const int k = (int)(q / A.nnz) + 1, l = (int)(q % A.nnz) + 1;
             if (LS(G, l, k) != 0) continue;
             if ((l - 1 >= 1 && LS(G, l - 1, k) == -1) || (l + 1 <= A.nnz && LS(G, l + 1, k) == -1) ||
@@ -1020,16 +1062,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) v
                 LS(G, l, k) = 1;
         }
         w_sync();
-        if (w_travel<FOM>(G, x, z, 2)) return;
+        if (w_travel<FOM, GL>(G, x, z, 2)) return;
     } else {
         G.nnx = A.nnx; G.nnz = A.nnz; G.dnx = dnx; G.dnz = dnz; G.gox = A.gox; G.goz = A.goz;
         G.v = velnc; G.vld = A.nnz;
-        if (w_travel<FOM>(G, x, z, 0)) return;
+        if (w_travel<FOM, GL>(G, x, z, 0)) return;
     }
     w_sync();
-    // srtimes: receiver picks, one receiver per lane
+    // srtimes: receiver picks, a receiver per lane
     double *out = A.ttime + prob * A.nrc;
-    for (int r = lane + 1; r <= A.nrc; r += 64) {
+    for (int r = gl + 1; r <= A.nrc; r += GL) {
         if (srs[r - 1] == 0) continue;
         const double rx = A.rcx[r - 1], rz = A.rcz[r - 1];
         int irx = (int)floor((rx - A.gox) / dnx) + 1;
@@ -1074,22 +1116,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) v
         }
         out[r - 1] = trr;
     }
-    if (lane == 0) pflag[slot] = 3;
+    if (gl == 0) pflag[slot] = 3;
 }
 
 } // namespace
 
-// LDS entries of k_fm2d_wave's heap (environment MCT_WAVE_HEAP, default 512:
-// 6 KB per wave, so registers and not LDS bound the waves per CU; a larger
-// heap takes fewer problems back to the lane solver but was slower overall,
-// 4.33 s with 768 entries and 5.35 s with 1024)
+// LDS entries of each problem's heap in k_fm2d_wave (environment MCT_WAVE_HEAP,
+// default 512)
 static int w_heap_cap()
 {
     static int cap = 0;
     if (!cap) {
         const char *e = getenv("MCT_WAVE_HEAP");
         int v = e ? atoi(e) : 512;
-        cap = v >= 256 && v <= 8192 ? v : 512;
+        cap = v >= 256 && v <= 4096 ? v : 512;
     }
     return cap;
 }
@@ -1098,9 +1138,15 @@ int mct_launch_fm2d_wave(const mct_fm2d_args &b, int32_t *pflag, hipStream_t s)
 {
     if (!pflag || b.nrun < 1) return MCT_ERR_ARG;
     const int cap = w_heap_cap();
-    const size_t lds = 12 * (size_t)(cap + 1);
-    if (b.fom == 0) hipLaunchKernelGGL(k_fm2d_wave<0>, dim3((unsigned)b.nrun), dim3(64), lds, s, b, cap, pflag);
-    else hipLaunchKernelGGL(k_fm2d_wave<1>, dim3((unsigned)b.nrun), dim3(64), lds, s, b, cap, pflag);
+    constexpr int GL = GRP_LANES, NG = 64 / GL;
+    // per problem: keys, heap entries, the downtree log (32 moves: at most 13 levels)
+    const size_t lds = (size_t)NG * 8 * (size_t)(cap + 1 + (cap + 2) / 2 + 32);
+    if (lds > 64 * 1024)
+        MCT_CHECK(hipFuncSetAttribute(b.fom == 0 ? (const void *)k_fm2d_wave<0, GL> : (const void *)k_fm2d_wave<1, GL>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const unsigned nb = (unsigned)((b.nrun + NG - 1) / NG);
+    if (b.fom == 0) hipLaunchKernelGGL((k_fm2d_wave<0, GL>), dim3(nb), dim3(64), lds, s, b, cap, pflag);
+    else hipLaunchKernelGGL((k_fm2d_wave<1, GL>), dim3(nb), dim3(64), lds, s, b, cap, pflag);
     MCT_CHECK(hipGetLastError());
     return MCT_OK;
 }
