@@ -1125,20 +1125,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) v
 // default 512)
 static int w_heap_cap()
 {
-    static int cap = 0;
-    if (!cap) {
-        const char *e = getenv("MCT_WAVE_HEAP");
-        int v = e ? atoi(e) : 512;
-        cap = v >= 256 && v <= 4096 ? v : 512;
-    }
-    return cap;
+    const char *e = getenv("MCT_WAVE_HEAP");
+    const int v = e ? atoi(e) : 512;
+    return v >= 64 && v <= 4096 ? v : 512;
 }
 
-int mct_launch_fm2d_wave(const mct_fm2d_args &b, int32_t *pflag, hipStream_t s)
+template <int GL>
+static int w_launch(const mct_fm2d_args &b, int cap, int32_t *pflag, hipStream_t s)
 {
-    if (!pflag || b.nrun < 1) return MCT_ERR_ARG;
-    const int cap = w_heap_cap();
-    constexpr int GL = GRP_LANES, NG = 64 / GL;
+    constexpr int NG = 64 / GL;
     // per problem: keys, heap entries, the downtree log (32 moves: at most 13 levels)
     const size_t lds = (size_t)NG * 8 * (size_t)(cap + 1 + (cap + 2) / 2 + 32);
     if (lds > 64 * 1024)
@@ -1149,6 +1144,16 @@ int mct_launch_fm2d_wave(const mct_fm2d_args &b, int32_t *pflag, hipStream_t s)
     else hipLaunchKernelGGL((k_fm2d_wave<1, GL>), dim3(nb), dim3(64), lds, s, b, cap, pflag);
     MCT_CHECK(hipGetLastError());
     return MCT_OK;
+}
+
+int mct_launch_fm2d_wave(const mct_fm2d_args &b, int32_t *pflag, hipStream_t s)
+{
+    if (!pflag || b.nrun < 1) return MCT_ERR_ARG;
+    // bands that outgrow the heap go to the lane solver, which keeps all of them
+    // in flight at once (a second pass with 2,048-entry heaps, four problems per
+    // CU, was slower: 9.5 s instead of 2.6 s when a 384-entry first pass left a
+    // large share of the 40,960 problems over)
+    return w_launch<GRP_LANES>(b, w_heap_cap(), pflag, s);
 }
 
 int mct_launch_fm2d_gridder(const mct_fm2d_args &a, hipStream_t s)

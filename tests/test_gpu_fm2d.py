@@ -223,3 +223,23 @@ def test_order_one_default_path_equals_the_checked_sweeps(monkeypatch):
     agree(default, want, 0)
     agree(checked, want, 0)
     np.testing.assert_allclose(default, checked, rtol=1e-9, atol=0)
+
+
+def test_order_one_bands_wider_than_the_small_heap(monkeypatch):
+    """A 64-entry heap per problem (MCT_WAVE_HEAP=64) overflows on most
+    problems: the lane solver solves them again, and the times stay the
+    oracle's (and equal to the default heap's)."""
+    rng = np.random.default_rng(91)
+    nvx, nvz, h = 61, 45, 0.5
+    ns, nr = 6, 10
+    velv = rng.uniform(2.2, 4.2, (nvx + 2, nvz + 2))
+    L, W = (nvx - 1) * h, (nvz - 1) * h
+    sx, sz = rng.uniform(h, L - h, ns), rng.uniform(h, W - h, ns)
+    rx, rz = rng.uniform(0, L, nr), rng.uniform(0, W, nr)
+    kw = dict(gdx=2, gdz=2, asgr=1, sgdl=4, sgs=8, fom=1)
+    want = po.fm2d(velv, 0, 0, h, h, sx, sz, rx, rz, **kw)
+    default = mct.modrays(velv, 0, 0, h, h, sx, sz, rx, rz, method=0, **kw)
+    monkeypatch.setenv("MCT_WAVE_HEAP", "64")
+    small = mct.modrays(velv, 0, 0, h, h, sx, sz, rx, rz, method=0, **kw)
+    np.testing.assert_array_equal(small, default)
+    agree(small, want, 0)
