@@ -440,7 +440,18 @@ def run_surface(args):
     # sweeps the coarse grid and picks, so its share is 4 C + 4 Nr
     alg_sweep = solves * (4.0 * C + 4.0 * nrev)
     r_sweep = None
-    if t_sweep > 0:
+    heap1 = args.order == 1 and os.environ.get("MCT_FM2D_ORDER1", "") != "check"
+    if t_sweep > 0 and heap1:
+        # order 1: the 2-D stage is the reference's heap loop, four problems per wave
+        a_sweep = alg_sweep / (t_sweep * 1e-3) / 1e9
+        r_sweep = {"bound": "hbm", "achieved": round(a_sweep, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(a_sweep / HBM_PEAK_GBS, 6), "traffic": None,
+                   "kernel": "k_fm2d_wave<1,16>", "launch_solves": int(solves), "launch_ms": round(t_sweep, 3),
+                   "alg_bytes": int(alg_sweep),
+                   "note": "the reference's heap loop for every 2-D problem (the mixed-order times depend on "
+                           "its pop order), four problems per wave with their heaps in LDS; algorithmic bytes "
+                           "per solve 4 C + 4 Nr; bound by the pops' dependent latency and VALU issue, not HBM"}
+    elif t_sweep > 0:
         a_sweep = alg_sweep / (t_sweep * 1e-3) / 1e9
         r_sweep = {"bound": "hbm", "achieved": round(a_sweep, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": round(a_sweep / HBM_PEAK_GBS, 6), "traffic": traffic_of("k_fm2d_strip<0>", n0),
@@ -558,7 +569,8 @@ def run_surface(args):
                    "batch": B, "grid": [w.grid.nx, w.grid.ny, w.grid.nz], "periods": npd,
                    "nsrc": w.nsrc, "nrev": int(w.rev.shape[0]), "cells": [cmin, cmax],
                    "plain_cells": int(w.points.shape[0]), "order": args.order,
-                   "fm2d_method": "parallel sweeps + heap loop near the source" if args.fm2d_method == 0
+                   "fm2d_method": ("heap loop, four problems per wave" if heap1 else
+                                   "parallel sweeps + heap loop near the source") if args.fm2d_method == 0
                    else "sequential heap loop per (period, source)"},
         "plain_evals_per_s": round(plain_eps, 4),
         "p50_ms": round(float(np.median(lat)) * 1e3, 3),
