@@ -628,8 +628,8 @@ __global__ __launch_bounds__(64) void k_fm2d_solve(mct_fm2d_args A)
 
 // Lanes per problem in the grouped heap-loop kernel (64 / GRP_LANES problems per
 // wave) and the waves per SIMD it is compiled for.  LDS bounds the problems in
-// flight (a 512-entry heap is 6.4 KB: about 24 per CU), so registers are not
-// the limit.  Measured on a 64-chain order-1 batch at 201x201 (40,960 problems):
+// flight (a 512-entry heap is 6.4 KB with 32-bit entries, 5.3 KB with 16-bit
+// ones: 24 or 30 per CU), so registers are not the limit.  Measured on a 64-chain order-1 batch at 201x201 (40,960 problems):
 // 16 lanes 1.84 s, 8 lanes 1.90 s, 4 lanes 2.69 s, one problem per wave 2.31 s;
 // heaps of 384, 640 and 768 entries 2.55, 2.13 and 2.53 s (a smaller heap
 // sends more problems to the lane solver, a larger one runs fewer at once).
@@ -661,9 +661,9 @@ struct wave_grid {
     int32_t *s;             // state: -1 far, 0 alive, >0 heap position (global)
     const double *v;
     int vld;
-    int32_t *hb;            // the group's LDS heap, 1-based
+    void *hb;               // the group's LDS heap, 1-based (entries of type ET below)
     double *hk;             // its keys, hk[p] = time of hb[p]
-    int2 *mv;               // its log of a downtree's moves (at most 32)
+    int2 *mv;               // its log of a downtree's moves (at most 16)
     int cap, ntr, full;
     int vnl, vnr, vnt, vnb;
     int base;               // the group's first lane
@@ -693,23 +693,45 @@ __device__ __forceinline__ void w_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A heap entry names its node as (ix << 16) | iz (ET = int32_t), or, where the
+// slot's arrays hold fewer than 65,536 words, as the node's offset in them
+// (ET = uint16_t: half the LDS, so more problems fit on a CU).
+template <class ET>
+__device__ __forceinline__ int w_get(const wave_grid &G, int p) { return (int)((const ET *)G.hb)[p]; }
+template <class ET>
+__device__ __forceinline__ void w_put(const wave_grid &G, int p, int e) { ((ET *)G.hb)[p] = (ET)e; }
+template <class ET>
+__device__ __forceinline__ int w_enc(const wave_grid &G, int iz, int ix)
+{
+    return sizeof(ET) == 2 ? (ix - 1) * G.ld + (iz - 1) : ((ix << 16) | iz);
+}
+template <class ET>
+__device__ __forceinline__ void w_dec(const wave_grid &G, int e, int &iz, int &ix)
+{
+    if (sizeof(ET) == 2) { ix = e / G.ld + 1; iz = e % G.ld + 1; }
+    else { ix = BPX(e); iz = BPZ(e); }
+}
+template <class ET>
 __device__ __forceinline__ void w_setpos(const wave_grid &G, int e, int pos)
 {
-    if (G.lead) LS(G, BPZ(e), BPX(e)) = pos;
+    if (!G.lead) return;
+    if (sizeof(ET) == 2) G.s[e] = pos;
+    else LS(G, BPZ(e), BPX(e)) = pos;
 }
 
 // addtree (fm2d_ttime.f90:527-564)
+template <class ET>
 __device__ static void w_addtree(wave_grid &G, int iz, int ix, double tv, w_track &trk)
 {
     if (G.ntr + 1 > G.cap) { G.full = 1; return; }
     G.ntr = G.ntr + 1;
     int tpc = G.ntr, tpp = tpc / 2;
     while (tpp > 0) {
-        const int ep = G.hb[tpp];
+        const int ep = w_get<ET>(G, tpp);
         const double kp = G.hk[tpp];
         if (tv < kp) {
-            w_setpos(G, ep, tpc);
-            G.hb[tpc] = ep;
+            w_setpos<ET>(G, ep, tpc);
+            w_put<ET>(G, tpc, ep);
             G.hk[tpc] = kp;
             trk.move(tpp, tpc);
             tpc = tpp;
@@ -718,32 +740,33 @@ __device__ static void w_addtree(wave_grid &G, int iz, int ix, double tv, w_trac
             tpp = 0;
         }
     }
-    const int e = (ix << 16) | iz;
-    w_setpos(G, e, tpc);
-    G.hb[tpc] = e;
+    const int e = w_enc<ET>(G, iz, ix);
+    w_setpos<ET>(G, e, tpc);
+    w_put<ET>(G, tpc, e);
     G.hk[tpc] = tv;
 }
 
 // downtree (fm2d_ttime.f90:575-644); its moves (from, to) go to the group's
 // log, their count to nmv
+template <class ET>
 __device__ static void w_downtree(wave_grid &G, int &nmv)
 {
     nmv = 0;
     if (G.ntr == 1) { G.ntr = G.ntr - 1; return; }
     const int old = G.ntr;
-    const int el = G.hb[old];
+    const int el = w_get<ET>(G, old);
     const double kl = G.hk[old];
     G.ntr = G.ntr - 1;
     int tpp = 1, tpc = 2 * tpp;
     while (tpc < G.ntr) {
-        const int e1 = G.hb[tpc], e2 = G.hb[tpc + 1];
+        const int e1 = w_get<ET>(G, tpc), e2 = w_get<ET>(G, tpc + 1);
         const double k1 = G.hk[tpc], k2 = G.hk[tpc + 1];
         int ec = e1;
         double kc = k1;
         if (k1 > k2) { tpc = tpc + 1; ec = e2; kc = k2; }
         if (kc < kl) {
-            w_setpos(G, ec, tpp);
-            G.hb[tpp] = ec;
+            w_setpos<ET>(G, ec, tpp);
+            w_put<ET>(G, tpp, ec);
             G.hk[tpp] = kc;
             G.mv[nmv] = make_int2(tpc, tpp);
             nmv++;
@@ -754,34 +777,35 @@ __device__ static void w_downtree(wave_grid &G, int &nmv)
         }
     }
     if (tpc == G.ntr) {
-        const int ec = G.hb[tpc];
+        const int ec = w_get<ET>(G, tpc);
         const double kc = G.hk[tpc];
         if (kc < kl) {
-            w_setpos(G, ec, tpp);
-            G.hb[tpp] = ec;
+            w_setpos<ET>(G, ec, tpp);
+            w_put<ET>(G, tpp, ec);
             G.hk[tpp] = kc;
             G.mv[nmv] = make_int2(tpc, tpp);
             nmv++;
             tpp = tpc;
         }
     }
-    w_setpos(G, el, tpp);
-    G.hb[tpp] = el;
+    w_setpos<ET>(G, el, tpp);
+    w_put<ET>(G, tpp, el);
     G.hk[tpp] = kl;
     G.mv[nmv] = make_int2(old, tpp);
     nmv++;
 }
 
 // updtree (fm2d_ttime.f90:653-680) from heap position tpc
+template <class ET>
 __device__ static void w_updtree(wave_grid &G, int tpc, int iz, int ix, double tv, w_track &trk)
 {
     int tpp = tpc / 2;
     while (tpp > 0) {
-        const int ep = G.hb[tpp];
+        const int ep = w_get<ET>(G, tpp);
         const double kp = G.hk[tpp];
         if (tv < kp) {
-            w_setpos(G, ep, tpc);
-            G.hb[tpc] = ep;
+            w_setpos<ET>(G, ep, tpc);
+            w_put<ET>(G, tpc, ep);
             G.hk[tpc] = kp;
             trk.move(tpp, tpc);
             tpc = tpp;
@@ -790,15 +814,15 @@ __device__ static void w_updtree(wave_grid &G, int tpc, int iz, int ix, double t
             tpp = 0;
         }
     }
-    const int e = (ix << 16) | iz;
-    w_setpos(G, e, tpc);
-    G.hb[tpc] = e;
+    const int e = w_enc<ET>(G, iz, ix);
+    w_setpos<ET>(G, e, tpc);
+    w_put<ET>(G, tpc, e);
     G.hk[tpc] = tv;
 }
 
 // travel (fm2d_ttime.f90:41-238) on a group; -1 source outside, -2 heap full.
 // Every lane of the group takes the same path through it.
-template <int FOM, int GL>
+template <int FOM, int GL, class ET>
 __device__ static int w_travel(wave_grid &G, double scx, double scz, int urg)
 {
     const int gl = threadIdx.x - G.base;
@@ -831,7 +855,7 @@ __device__ static int w_travel(wave_grid &G, double scx, double scz, int urg)
                 const int b = __builtin_ctzll(m);
                 m &= m - 1;
                 const int bi = __shfl(i, G.base + b, 64), bj = __shfl(j, G.base + b, 64);
-                w_addtree(G, bj, bi, w_shfl_d(tq, G.base + b), none);
+                w_addtree<ET>(G, bj, bi, w_shfl_d(tq, G.base + b), none);
                 if (G.full) return -2;
             }
             w_sync();
@@ -854,7 +878,7 @@ __device__ static int w_travel(wave_grid &G, double scx, double scz, int urg)
                 double ds = sqrt(ex * ex + ez * ez);
                 const double tc = 2.0 * ds / (vss[i - 1][j - 1] + vsrc);
                 if (G.lead) LT(G, isz - 1 + j, isx - 1 + i) = tc;
-                w_addtree(G, isz - 1 + j, isx - 1 + i, tc, none);
+                w_addtree<ET>(G, isz - 1 + j, isx - 1 + i, tc, none);
             }
         w_sync();
     }
@@ -862,8 +886,8 @@ __device__ static int w_travel(wave_grid &G, double scx, double scz, int urg)
     const int kn = gl & 3;
     const int dx = kn == 0 ? -1 : (kn == 1 ? 1 : 0), dz = kn == 2 ? -1 : (kn == 3 ? 1 : 0);
     while (G.ntr > 0) {
-        const int e = G.hb[1];
-        const int ix = BPX(e), iz = BPZ(e);
+        int ix, iz;
+        w_dec<ET>(G, w_get<ET>(G, 1), iz, ix);
         if (urg == 1) {   // stop when the band reaches an inner edge of the refined grid
             int swrg = 0;
             if (ix == 1 && G.vnl != 1) swrg = 1;
@@ -890,7 +914,7 @@ __device__ static int w_travel(wave_grid &G, double scx, double scz, int urg)
         // the heap moves while the loads are in flight; the neighbours'
         // positions then follow the logged moves, each lane its own
         int nmv = 0;
-        w_downtree(G, nmv);
+        w_downtree<ET>(G, nmv);
         w_sync();
         int mypos = st > 0 ? st : 0;
         for (int q = 0; q < nmv; q++) {
@@ -914,11 +938,11 @@ __device__ static int w_travel(wave_grid &G, double scx, double scz, int urg)
             const int xk = ix + (k == 0 ? -1 : (k == 1 ? 1 : 0)), zk = iz + (k == 2 ? -1 : (k == 3 ? 1 : 0));
             if (G.lead) LT(G, zk, xk) = tvk;
             if (stk == -1) {
-                w_addtree(G, zk, xk, tvk, trk);
+                w_addtree<ET>(G, zk, xk, tvk, trk);
             } else {
                 const int pos = trk.p[k];
                 trk.p[k] = 0;
-                w_updtree(G, pos, zk, xk, tvk, trk);
+                w_updtree<ET>(G, pos, zk, xk, tvk, trk);
             }
         }
         if (G.full) return -2;
@@ -932,7 +956,7 @@ __device__ static int w_travel(wave_grid &G, double scx, double scz, int urg)
 // solved here, 2 when there is nothing to solve, and stays 1 when its band
 // outgrew the LDS heap (or its source lies outside the grid, which the lane
 // solver reports).  Every branch below is taken by whole groups.
-template <int FOM, int GL>
+template <int FOM, int GL, class ET>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) void k_fm2d_wave(mct_fm2d_args A, int heap_cap, int32_t *pflag)
 {
     extern __shared__ double w_lds[];
@@ -973,11 +997,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) v
     G.t = A.ttn + slot * wsz;
     G.s = A.nsts + slot * wsz;
     {
-        const int per = heap_cap + 1 + (heap_cap + 2) / 2 + 32;   // doubles per group
+        const int hbw = (int)(((long)(heap_cap + 1) * sizeof(ET) + 7) / 8);   // doubles of entries
+        const int per = heap_cap + 1 + hbw + 16;                              // doubles per group
         double *gb = w_lds + (long)g * per;
         G.hk = gb;
-        G.hb = (int32_t *)(gb + heap_cap + 1);
-        G.mv = (int2 *)(gb + heap_cap + 1 + (heap_cap + 2) / 2);
+        G.hb = (void *)(gb + heap_cap + 1);
+        G.mv = (int2 *)(gb + heap_cap + 1 + hbw);
     }
     G.cap = heap_cap;
     G.base = g * GL;
@@ -1028,7 +1053,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) v
             vr[(long)(idm2 - 1) * A.ld + (idm1 - 1)] = sum[0] + sum[1] + sum[2] + sum[3];
         }
         w_sync();
-        if (w_travel<FOM, GL>(G, x, z, 1)) return;
+        if (w_travel<FOM, GL, ET>(G, x, z, 1)) return;
         // remap the refined grid onto the coarse grid
         double *tr = A.ttr + slot * wsz;
         int32_t *nr = A.nstr + slot * wsz;
@@ -1062,11 +1087,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVE_EU))) v
                 LS(G, l, k) = 1;
         }
         w_sync();
-        if (w_travel<FOM, GL>(G, x, z, 2)) return;
+        if (w_travel<FOM, GL, ET>(G, x, z, 2)) return;
     } else {
         G.nnx = A.nnx; G.nnz = A.nnz; G.dnx = dnx; G.dnz = dnz; G.gox = A.gox; G.goz = A.goz;
         G.v = velnc; G.vld = A.nnz;
-        if (w_travel<FOM, GL>(G, x, z, 0)) return;
+        if (w_travel<FOM, GL, ET>(G, x, z, 0)) return;
     }
     w_sync();
     // srtimes: receiver picks, a receiver per lane
@@ -1130,18 +1155,19 @@ static int w_heap_cap()
     return v >= 64 && v <= 4096 ? v : 512;
 }
 
-template <int GL>
+template <int GL, class ET>
 static int w_launch(const mct_fm2d_args &b, int cap, int32_t *pflag, hipStream_t s)
 {
     constexpr int NG = 64 / GL;
-    // per problem: keys, heap entries, the downtree log (32 moves: at most 13 levels)
-    const size_t lds = (size_t)NG * 8 * (size_t)(cap + 1 + (cap + 2) / 2 + 32);
+    // per problem: keys, heap entries, the downtree log (16 moves: at most 13 levels)
+    const long hbw = ((long)(cap + 1) * (long)sizeof(ET) + 7) / 8;
+    const size_t lds = (size_t)NG * 8 * (size_t)(cap + 1 + hbw + 16);
+    const void *k0 = (const void *)k_fm2d_wave<0, GL, ET>, *k1 = (const void *)k_fm2d_wave<1, GL, ET>;
     if (lds > 64 * 1024)
-        MCT_CHECK(hipFuncSetAttribute(b.fom == 0 ? (const void *)k_fm2d_wave<0, GL> : (const void *)k_fm2d_wave<1, GL>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        MCT_CHECK(hipFuncSetAttribute(b.fom == 0 ? k0 : k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const unsigned nb = (unsigned)((b.nrun + NG - 1) / NG);
-    if (b.fom == 0) hipLaunchKernelGGL((k_fm2d_wave<0, GL>), dim3(nb), dim3(64), lds, s, b, cap, pflag);
-    else hipLaunchKernelGGL((k_fm2d_wave<1, GL>), dim3(nb), dim3(64), lds, s, b, cap, pflag);
+    if (b.fom == 0) hipLaunchKernelGGL((k_fm2d_wave<0, GL, ET>), dim3(nb), dim3(64), lds, s, b, cap, pflag);
+    else hipLaunchKernelGGL((k_fm2d_wave<1, GL, ET>), dim3(nb), dim3(64), lds, s, b, cap, pflag);
     MCT_CHECK(hipGetLastError());
     return MCT_OK;
 }
@@ -1153,7 +1179,9 @@ int mct_launch_fm2d_wave(const mct_fm2d_args &b, int32_t *pflag, hipStream_t s)
     // in flight at once (a second pass with 2,048-entry heaps, four problems per
     // CU, was slower: 9.5 s instead of 2.6 s when a 384-entry first pass left a
     // large share of the 40,960 problems over)
-    return w_launch<GRP_LANES>(b, w_heap_cap(), pflag, s);
+    // 16-bit heap entries (a node's offset in the slot's arrays) where they fit
+    if ((long)b.ld * b.nc <= 65536) return w_launch<GRP_LANES, uint16_t>(b, w_heap_cap(), pflag, s);
+    return w_launch<GRP_LANES, int32_t>(b, w_heap_cap(), pflag, s);
 }
 
 int mct_launch_fm2d_gridder(const mct_fm2d_args &a, hipStream_t s)
