@@ -243,3 +243,20 @@ def test_order_one_bands_wider_than_the_small_heap(monkeypatch):
     small = mct.modrays(velv, 0, 0, h, h, sx, sz, rx, rz, method=0, **kw)
     np.testing.assert_array_equal(small, default)
     agree(small, want, 0)
+
+
+def test_order_one_grid_too_large_for_16_bit_heap_entries():
+    """A 261 x 261 coarse grid (68,121 nodes) takes the heap-loop kernel with
+    32-bit heap entries; its times are the oracle's."""
+    rng = np.random.default_rng(261)
+    n, h = 261, 0.4
+    L = (n - 1) * h
+    velv = rng.uniform(2.4, 4.0, (n + 2, n + 2))
+    ns, nr = 3, 8
+    sx, sz = rng.uniform(h, L - h, ns), rng.uniform(h, L - h, ns)
+    rx, rz = rng.uniform(0, L, nr), rng.uniform(0, L, nr)
+    for asgr in (0, 1):
+        say("fm2d order 1 at 261^2, asgr", asgr)
+        got = mct.modrays(velv, 0, 0, h, h, sx, sz, rx, rz, fom=1, asgr=asgr, method=0)
+        want = po.fm2d(velv, 0, 0, h, h, sx, sz, rx, rz, fom=1, asgr=asgr)
+        agree(got, want, 0)
