@@ -331,7 +331,7 @@ def test_perturbed_box_call_equals_the_whole_grid_call():
     say("surf perturbed box")
     with pytest.raises(mct.MCTError):     # no accepted state yet
         L.likelihood_box(w.points, w.params, [1, 1, 0, 2, 2, 1])
-    rc, outA = L.likelihood(w.points, w.params)
+    rc, outA, _, _, _, pvA = L.likelihood(w.points, w.params, arrays=True)
     assert rc == 0
     L.accept(0)
     ncol = w.grid.nx * w.grid.ny
@@ -363,9 +363,19 @@ def test_perturbed_box_call_equals_the_whole_grid_call():
             if not chg[c[0] - 1:c[1], c[2] - 1:c[3]].any():
                 far = bx
     assert far is not None
-    _, _, _, _, _, pvm = L.likelihood_box(w.points, B, far, arrays=True)
+    rm, om, _, _, _, pvm = L.likelihood_box(w.points, B, far, arrays=True)
     _, _, _, _, _, pvf = L.likelihood(w.points, B, arrays=True)
-    assert not np.array_equal(pvm, pvf)
+    assert rm == 0 and not np.array_equal(pvm, pvf)
+    # every column that box recomputes has the accepted model's Vs profile, so
+    # the call is the accepted model's evaluation: its phase velocities and
+    # likelihood, which the oracle computes for the accepted model
+    np.testing.assert_array_equal(pvm, pvA)
+    np.testing.assert_array_equal(om, outA)
+    o = po.surf_likelihood(gdict(w.grid), w.points, w.params, w.freqs, w.src, w.rev, w.ttime, w.raystat,
+                           raylov=w.raylov, order=w.order)
+    assert o[0] == 0
+    assert np.max(np.abs(pvm - o[5])) <= 1e-6
+    assert abs(om[0] - o[1][0]) <= 1e-6 * abs(o[1][0])
     check(w.params, B)                     # rejected: not accepted
     C = _bump(w.params, cells[1], 0.1)
     check(w.params, C)                     # starts from A again
