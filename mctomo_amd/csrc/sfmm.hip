@@ -450,10 +450,13 @@ __global__ __launch_bounds__(64) void k_sfmm(mct_sfmm_args A)
         if (HIGH) {
             // the plus neighbour's two-away cell on its minus side is the
             // minus neighbour: solve again with its value after the update
+            // (only when that neighbour was updated: otherwise the value the
+            // plus lane loaded is already the one the reference reads)
             const double after = upd ? nt : S.tself;
             const double from_minus = __shfl(after, lane - 1 < 0 ? 0 : lane - 1, 64);
+            const bool minus_upd = __shfl(upd ? 1 : 0, lane - 1 < 0 ? 0 : lane - 1, 64) != 0;
             const bool hm2n = ndim == 0 ? S.hm2[0] : (ndim == 1 ? S.hm2[1] : S.hm2[2]);
-            if (live && (lane & 1) && hm2n) {
+            if (live && (lane & 1) && hm2n && minus_upd) {
 #pragma unroll
                 for (int d = 0; d < 3; d++)
                     if (d == ndim) S.m2[d] = from_minus;
