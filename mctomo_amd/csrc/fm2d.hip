@@ -1179,9 +1179,18 @@ int mct_launch_fm2d_wave(const mct_fm2d_args &b, int32_t *pflag, hipStream_t s)
     // in flight at once (a second pass with 2,048-entry heaps, four problems per
     // CU, was slower: 9.5 s instead of 2.6 s when a 384-entry first pass left a
     // large share of the 40,960 problems over)
-    // 16-bit heap entries (a node's offset in the slot's arrays) where they fit
-    if ((long)b.ld * b.nc <= 65536) return w_launch<GRP_LANES, uint16_t>(b, w_heap_cap(), pflag, s);
-    return w_launch<GRP_LANES, int32_t>(b, w_heap_cap(), pflag, s);
+    // 16-bit heap entries (a node's offset in the slot's arrays) where they fit.
+    // A launch with few problems gives each a wave of its own, which shortens its
+    // chain (a plain 201x201 order-1 call, 640 problems: 194.6 ms against 242.8
+    // with four problems per wave); many problems share waves, which is the more
+    // work per instruction (40,960 problems: 1.63 s against 2.31 s)
+    const bool e16 = (long)b.ld * b.nc <= 65536;
+    const int cap = w_heap_cap();
+    const char *es = getenv("MCT_WAVE_SPLIT");      // the largest launch with a wave per problem
+    const long split = es ? atol(es) : 4096;
+    if (b.nrun <= split)
+        return e16 ? w_launch<64, uint16_t>(b, cap, pflag, s) : w_launch<64, int32_t>(b, cap, pflag, s);
+    return e16 ? w_launch<GRP_LANES, uint16_t>(b, cap, pflag, s) : w_launch<GRP_LANES, int32_t>(b, cap, pflag, s);
 }
 
 int mct_launch_fm2d_gridder(const mct_fm2d_args &a, hipStream_t s)

@@ -260,3 +260,25 @@ def test_order_one_grid_too_large_for_16_bit_heap_entries():
         got = mct.modrays(velv, 0, 0, h, h, sx, sz, rx, rz, fom=1, asgr=asgr, method=0)
         want = po.fm2d(velv, 0, 0, h, h, sx, sz, rx, rz, fom=1, asgr=asgr)
         agree(got, want, 0)
+
+
+@pytest.mark.parametrize("split", ["0", "1000000"])
+def test_order_one_problems_per_wave(split, monkeypatch):
+    """The heap-loop kernel runs four problems per wave on large launches and
+    one per wave on small ones (MCT_WAVE_SPLIT moves the boundary): both give
+    the oracle's times, the same bit for bit."""
+    rng = np.random.default_rng(5150)
+    nvx, nvz, h = 37, 29, 0.5
+    npd, ns, nr = 3, 7, 9
+    velv = rng.uniform(2.2, 4.4, (npd, nvx + 2, nvz + 2))
+    L, W = (nvx - 1) * h, (nvz - 1) * h
+    sx, sz = rng.uniform(h, L - h, ns), rng.uniform(h, W - h, ns)
+    rx, rz = rng.uniform(0, L, nr), rng.uniform(0, W, nr)
+    kw = dict(gdx=2, gdz=2, asgr=1, sgdl=4, sgs=8, fom=1)
+    want = np.stack([po.fm2d(velv[p], 0, 0, h, h, sx, sz, rx, rz, **kw) for p in range(npd)])
+    monkeypatch.setenv("MCT_WAVE_SPLIT", split)
+    got = mct.modrays(velv, 0, 0, h, h, sx, sz, rx, rz, method=0, **kw)
+    monkeypatch.setenv("MCT_FM2D_WAVE", "0")
+    lanes = mct.modrays(velv, 0, 0, h, h, sx, sz, rx, rz, method=0, **kw)
+    np.testing.assert_array_equal(got, lanes)
+    agree(got, want, 0)
