@@ -1159,9 +1159,24 @@ template <int GL, class ET>
 static int w_launch(const mct_fm2d_args &b, int cap, int32_t *pflag, hipStream_t s)
 {
     constexpr int NG = 64 / GL;
-    // per problem: keys, heap entries, the downtree log (16 moves: at most 13 levels)
-    const long hbw = ((long)(cap + 1) * (long)sizeof(ET) + 7) / 8;
-    const size_t lds = (size_t)NG * 8 * (size_t)(cap + 1 + hbw + 16);
+    static int lds_max = 0;
+    if (!lds_max) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 64 * 1024;
+        lds_max = v;
+    }
+    // per problem: keys, heap entries, the downtree log (16 moves: at most 13 levels);
+    // a heap too large for the workgroup's LDS is cut to what fits (bands that
+    // outgrow it go to the lane solver like any other overflow)
+    auto bytes = [&](int c) -> size_t {
+        const long hbw = ((long)(c + 1) * (long)sizeof(ET) + 7) / 8;
+        return (size_t)NG * 8 * (size_t)(c + 1 + hbw + 16);
+    };
+    while (cap > 64 && bytes(cap) > (size_t)lds_max) cap -= 64;
+    const size_t lds = bytes(cap);
+    if (lds > (size_t)lds_max) return MCT_ERR_ARG;
     const void *k0 = (const void *)k_fm2d_wave<0, GL, ET>, *k1 = (const void *)k_fm2d_wave<1, GL, ET>;
     if (lds > 64 * 1024)
         MCT_CHECK(hipFuncSetAttribute(b.fom == 0 ? k0 : k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

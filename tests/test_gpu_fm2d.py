@@ -282,3 +282,27 @@ def test_order_one_problems_per_wave(split, monkeypatch):
     lanes = mct.modrays(velv, 0, 0, h, h, sx, sz, rx, rz, method=0, **kw)
     np.testing.assert_array_equal(got, lanes)
     agree(got, want, 0)
+
+
+@pytest.mark.parametrize("split", ["0", "1000000"])
+def test_order_one_heap_larger_than_lds_is_cut_to_fit(split, monkeypatch):
+    """A heap of 4,096 entries asks four problems per wave for more LDS than a
+    workgroup has (164 KB of 160 KB with 16-bit entries): the launcher cuts it
+    to what fits, and the times are the oracle's, as with the default heap."""
+    rng = np.random.default_rng(7272)
+    nvx, nvz, h = 41, 33, 0.5
+    npd, ns, nr = 2, 6, 9
+    velv = rng.uniform(2.2, 4.4, (npd, nvx + 2, nvz + 2))
+    L, W = (nvx - 1) * h, (nvz - 1) * h
+    sx, sz = rng.uniform(h, L - h, ns), rng.uniform(h, W - h, ns)
+    rx, rz = rng.uniform(0, L, nr), rng.uniform(0, W, nr)
+    kw = dict(gdx=2, gdz=2, asgr=1, sgdl=4, sgs=8, fom=1)
+    want = np.stack([po.fm2d(velv[p], 0, 0, h, h, sx, sz, rx, rz, **kw) for p in range(npd)])
+    say("order 1 heap 4096, split", split)
+    monkeypatch.setenv("MCT_WAVE_SPLIT", split)
+    monkeypatch.setenv("MCT_WAVE_HEAP", "4096")
+    got = mct.modrays(velv, 0, 0, h, h, sx, sz, rx, rz, method=0, **kw)
+    monkeypatch.delenv("MCT_WAVE_HEAP")
+    dflt = mct.modrays(velv, 0, 0, h, h, sx, sz, rx, rz, method=0, **kw)
+    np.testing.assert_array_equal(got, dflt)
+    agree(got, want, 0)
