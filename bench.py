@@ -20,6 +20,9 @@ one batch.  `value` is the batched evaluations per second.
 
 --workload body101: the 3-D body-wave eikonal drop-in (100 sites, 101^3 at
 0.2 km, 50 sources x 400 receivers).
+--workload c3: the batched configuration (256 chains, 101x101x41, 20 periods
+0.5-10 s, 64 x 64 pairs, Rayleigh and Love, phase and group velocities; an
+eval is a chain's four likelihoods).
 
 Prints one JSON line.
 """
@@ -589,6 +592,106 @@ def run_surface(args):
     }
 
 
+# --------------------------------------------------------------------- batched C3
+def run_c3(args):
+    """The batched configuration: B chains (default 256) per launch on a
+    101x101x41 grid (50 x 50 x 20 km), cell counts uniform in [20,400], 20
+    periods 0.5-10 s, 64 sources x 64 receivers, Rayleigh and Love, phase and
+    group velocities.  One evaluation = one chain's four likelihoods, one
+    context per (wave type, velocity type), each a batched launch of all
+    chains.  The contexts run one after another (one holds its workspace at a
+    time); a step is the four batches, and its time is the sum of theirs."""
+    import mctomo_amd as mct
+    from mctomo_amd.workloads import surface_workload, _surface_sites, top_is_slowest
+
+    B = max(1, args.batch)
+    cmin, cmax = 20, 400
+    kinds = [(1, 0), (1, 1), (0, 0), (0, 1)]          # (raylov, phase_group)
+    names = {(1, 0): "Rayleigh phase", (1, 1): "Rayleigh group", (0, 0): "Love phase", (0, 1): "Love group"}
+    ws = {rl: surface_workload(101, 41, 50.0, 20.0, 200, 20, (0.5, 10.0), 64, 64, raylov=rl,
+                               seed=20261204, order=args.order) for rl in (1, 0)}
+    g = ws[1].grid
+    L, Z = g.dx * (g.nx - 1), g.dz * (g.nz - 1)
+    rng = np.random.default_rng(4243)
+    P = np.zeros((B, cmax, 3))
+    Q = np.zeros((B, cmax, 3))
+    N = np.zeros(B, np.int32)
+    for c in range(B):
+        while True:       # check_model rejects a column whose top node is not its slowest
+            k = int(rng.integers(cmin, cmax + 1))
+            p, q = _surface_sites(rng, k, L, Z, (2.5, 4.5))
+            if top_is_slowest(g, p, q):
+                break
+        P[c, :k], Q[c, :k], N[c] = p, q, k
+    dP, dQ, dN = (mct.DeviceArray.from_numpy(a) for a in (P, Q, N))
+    dS0 = mct.DeviceArray.from_numpy(np.zeros((B, 2, ws[1].freqs.shape[0])))
+    dO = mct.DeviceArray((B, 3), np.float64)
+    dS = mct.DeviceArray((B,), np.int32)
+    per_kind = {}
+    total_s = 0.0
+    for rl, pg in kinds:
+        w = ws[rl]
+        like = mct.SurfLikelihood(w.grid, w.freqs, w.src, w.rev, w.ttime, w.raystat, raylov=rl,
+                                  order=args.order, phase_group=pg, max_cells=cmax, max_chains=B)
+
+        def run():
+            like.likelihood_batch_dev(B, dP.ptr, dQ.ptr, dN.ptr, dS0.ptr, dO.ptr, dS.ptr)
+            like.sync()
+
+        for _ in range(max(1, args.warmup)):
+            run()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            run()
+        dt = (time.perf_counter() - t0) / args.steps
+        st = dS.numpy()
+        if (st < 0).any():
+            raise RuntimeError("%s: chain status %s" % (names[(rl, pg)], st))
+        stages = [round(float(v), 2) for v in like.timings()]
+        near_ms, sweep_ms = like.timings2d()
+        per_kind[names[(rl, pg)]] = {"ms_per_batch": round(dt * 1e3, 2), "stage_ms": stages,
+                                     "fm2d_split_ms": {"near_source": round(near_ms, 2),
+                                                       "sweep": round(sweep_ms, 2)},
+                                     "statuses": {str(k): int(v) for k, v in
+                                                  zip(*np.unique(st, return_counts=True))}}
+        log("C3 %s: %.1f ms per %d-chain batch, stages(ms) nearest %.2f dispersion %.2f 2-D %.2f "
+            "misfit %.2f (near-source %.2f, sweep %.2f)" % (names[(rl, pg)], dt * 1e3, B, *stages,
+                                                           near_ms, sweep_ms))
+        total_s += dt
+        like.close()
+    ms_step = total_s * 1e3
+    value = B / total_s
+    cpu = None
+    if not args.no_cpu:
+        # one chain's four likelihoods by the C oracle on every host core the process may use
+        from oracle import pyoracle as po
+        threads = len(os.sched_getaffinity(0))
+        k = int(N[0])
+        t0 = time.perf_counter()
+        for rl, pg in kinds:
+            w = ws[rl]
+            po.surf_likelihood(gdict(w.grid), P[0, :k], Q[0, :k], w.freqs, w.src, w.rev, w.ttime,
+                               w.raystat, raylov=rl, order=args.order, phase_group=pg, nthreads=threads)
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": round(1.0 / t_cpu, 4), "unit": "evals/s", "cores": threads, "kind": "port",
+               "sample": "one chain (%d cells), its four likelihoods by the C oracle, %.1f s" % (k, t_cpu)}
+    return {
+        "metric": METRIC, "value": round(value, 3), "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "batched surface-wave likelihood: 101x101x41 grid (50x50x20 km), 20 periods "
+                               "0.5-10 s, 64 sources x 64 receivers, Rayleigh and Love, phase and group "
+                               "velocities (four likelihoods per eval), order %d; batch of %d chains with "
+                               "20-400 Voronoi cells" % (args.order, B),
+                   "batch": B, "grid": [101, 101, 41], "periods": 20, "nsrc": 64, "nrev": 64,
+                   "order": args.order},
+        "per_kind": per_kind,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+
+
 # --------------------------------------------------------------------- body
 def run_body(args):
     import mctomo_amd as mct
@@ -696,9 +799,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=None,
+                    help="chains per launch (default 64; 256 for --workload c3)")
     ap.add_argument("--order", type=int, default=0)
-    ap.add_argument("--workload", default="surf201", choices=["surf201", "body101"])
+    ap.add_argument("--workload", default="surf201", choices=["surf201", "body101", "c3"])
     ap.add_argument("--fm2d-method", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-evals", type=int, default=10)
@@ -719,6 +823,8 @@ def main():
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
+    if args.batch is None:
+        args.batch = 256 if args.workload == "c3" else 64
     if args.gpus != 1:
         log("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU "
             "path (chains would shard across GPUs with no device traffic); use --gpus 1.")
@@ -731,7 +837,8 @@ def main():
         a, b = mct.DeviceArray((n,), np.float64), mct.DeviceArray((n,), np.float64)
         mct.lib().mct_copy_d2d(ctypes_vp(b.ptr), ctypes_vp(a.ptr), ctypes_i64(8 * n))
         a.free(); b.free()
-    res = run_surface(args) if args.workload == "surf201" else run_body(args)
+    run = {"surf201": run_surface, "body101": run_body, "c3": run_c3}[args.workload]
+    res = run(args)
     print(json.dumps(res), flush=True)
     return 0
 

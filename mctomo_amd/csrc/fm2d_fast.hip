@@ -644,6 +644,11 @@ __device__ __forceinline__ int ff_sweep(const fview &V, const FX &fixed, const V
                     int b = d - a;
                     int x = (dir & 1) ? V.nnx - a : 1 + a;
                     int z = (dir & 2) ? V.nnz - b : 1 + b;
+                    // the node's velocity is read before its work bit, so that the
+                    // load is in flight during the LDS reads below (a lane whose bit
+                    // is clear drops it)
+                    const double vz = vel(z, x);
+                    __builtin_amdgcn_sched_barrier(0);
                     if (W) {
                         long q = bit(z, x);
                         uint32_t m = 1u << (q & 31);
@@ -656,7 +661,7 @@ __device__ __forceinline__ int ff_sweep(const fview &V, const FX &fixed, const V
                     ff_sten S;
                     ff_load_sten<FOM>(V, z, x, S);
                     const double t = V.t(z, x);
-                    double slown = 1.0 / vel(z, x);
+                    double slown = 1.0 / vz;
                     double c = (FAST && FOM == 0) ? ff_cands0f(S, K0, slown) : ff_cands<FOM>(S, V.dnx, V.dnz, slown);
                     if (c < FF_THR0 && c != t) {
                         if (prune && c > __longlong_as_double((long long)*E->bound)) continue;
