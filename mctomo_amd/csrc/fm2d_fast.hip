@@ -1095,7 +1095,7 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
     RW.bits = DB;
     __shared__ int sh_flag, sh_exit, sh_ovf;
     __shared__ double sh_texit;
-    __shared__ int sh_eidx;
+    __shared__ int sh_eidx, sh_nbl;
 
     const int lane = threadIdx.x;
     near_geom g;
@@ -1248,11 +1248,15 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
             const int eidx = sh_eidx;
             for (int q = lane; q < rn; q += 64) RN[q] = (RT[q] < texit || q == eidx) ? 0 : -1;
             __syncthreads();
-            // close nodes and their tentative times
+            // close nodes and their tentative times: the open nodes with a frozen
+            // neighbour other than the exit node.  They are listed first (in the
+            // heap's storage, free between the two heap loops) and solved 64 at a
+            // time; a node beyond the list's room is solved where it is found.
+            // Only frozen nodes' times are read and only open nodes are written,
+            // so the order of the solves does not matter.
             const int ex_x = eidx >= 0 ? eidx / R.nnz + 1 : -1, ex_z = eidx >= 0 ? eidx % R.nnz + 1 : -1;
-            for (int q = lane; q < rn; q += 64) {
-                if (RN[q] == 0) continue;
-                int i = q / R.nnz + 1, j = q % R.nnz + 1;
+            auto close_tlast = [&](int q) -> double {
+                const int i = q / R.nnz + 1, j = q % R.nnz + 1;
                 double tlast = -1;
                 const int nb[4][2] = {{j, i - 1}, {j, i + 1}, {j - 1, i}, {j + 1, i}};
                 for (int w = 0; w < 4; w++) {
@@ -1260,14 +1264,35 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
                     if (zz < 1 || zz > R.nnz || xx < 1 || xx > R.nnx) continue;
                     if (R.n(zz, xx) == 0 && !(zz == ex_z && xx == ex_x) && R.t(zz, xx) > tlast) tlast = R.t(zz, xx);
                 }
+                return tlast;
+            };
+            auto close_solve = [&](int q, double tlast) {
+                const int i = q / R.nnz + 1, j = q % R.nnz + 1;
+                auto alt = [&](int zz, int xx) -> bool {
+                    return R.n(zz, xx) == 0 && !(zz == ex_z && xx == ex_x) && R.t(zz, xx) <= tlast;
+                };
+                double tv = ff_fouds(R, alt, j, i, 1.0 / rvel(j, i), A.fom);
+                RT[q] = tv;
+                RN[q] = 1;
+            };
+            int *BL = HB;
+            const int blcap = (heap_cap + 2 + 1) & ~1;
+            if (lane == 0) sh_nbl = 0;
+            __syncthreads();
+            for (int q = lane; q < rn; q += 64) {
+                if (RN[q] == 0) continue;
+                const double tlast = close_tlast(q);
                 if (tlast >= 0) {
-                    auto alt = [&](int zz, int xx) -> bool {
-                        return R.n(zz, xx) == 0 && !(zz == ex_z && xx == ex_x) && R.t(zz, xx) <= tlast;
-                    };
-                    double tv = ff_fouds(R, alt, j, i, 1.0 / rvel(j, i), A.fom);
-                    RT[q] = tv;
-                    RN[q] = 1;
+                    const int k = atomicAdd(&sh_nbl, 1);
+                    if (k < blcap) BL[k] = q;
+                    else close_solve(q, tlast);
                 }
+            }
+            __syncthreads();
+            const int nbl = sh_nbl < blcap ? sh_nbl : blcap;
+            for (int k = lane; k < nbl; k += 64) {
+                const int q = BL[k];
+                close_solve(q, close_tlast(q));
             }
             __syncthreads();
         } else {
@@ -1326,13 +1351,29 @@ __global__ __launch_bounds__(64) void k_fm2d_near(mct_fm2d_args A, int region_no
             S.gen_bits = DB;
             H.ntr = 0;
             H.full = 0;
-            for (int i = lx0; i <= lx1; i++)
-                for (int j = lz0; j <= lz1; j++)
-                    if (C.n(j, i) > 0) {
-                        ff_addtree(C, H, j, i);
-                        ff_gen_set(C, DB, j, i);
+        }
+        {
+            // the band's nodes enter the heap in scan order (x outer, z inner): the
+            // wave tests 64 nodes at a time and lane 0 adds the band nodes among
+            // them in lane order (adding a node changes only the states of nodes
+            // already in the heap, which stay positive, so the tests stand)
+            const int cz = lz1 - lz0 + 1, tot = (lx1 - lx0 + 1) * cz;
+            for (int b0 = 0; b0 < tot; b0 += 64) {
+                const int q = b0 + lane;
+                const int i = lx0 + q / cz, j = lz0 + q % cz;
+                const bool band = q < tot && C.n(j, i) > 0;
+                unsigned long long m = __ballot(band);
+                while (m) {
+                    const int l = __builtin_ctzll(m);
+                    m &= m - 1;
+                    const int il = __shfl(i, l, 64), jl = __shfl(j, l, 64);
+                    if (lane == 0) {
+                        ff_addtree(C, H, jl, il);
+                        ff_gen_set(C, DB, jl, il);
                         S.need++;
                     }
+                }
+            }
         }
         __syncthreads();
         ff_heap_loop(C, H, cvel, A.fom, S);
