@@ -719,7 +719,7 @@ __device__ __forceinline__ void ff_addtree(const fview &V, ff_heap &H, int iz, i
         if (tv < V.t(HPZ(ep), HPX(ep))) {
             V.n(iz, ix) = (int16_t)tpp;
             V.n(HPZ(ep), HPX(ep)) = (int16_t)tpc;
-            H.btg[tpp] = H.btg[tpc];
+            H.btg[tpp] = (ix << 16) | iz;      // the moving entry: the node's own
             H.btg[tpc] = ep;
             tpc = tpp;
             tpp = tpc / 2;
@@ -732,24 +732,27 @@ __device__ __forceinline__ void ff_addtree(const fview &V, ff_heap &H, int iz, i
 __device__ __forceinline__ void ff_downtree(const fview &V, ff_heap &H)
 {
     if (H.ntr == 1) { H.ntr = H.ntr - 1; return; }
-    int el = H.btg[H.ntr];
+    const int el = H.btg[H.ntr];
     V.n(HPZ(el), HPX(el)) = 1;
     H.btg[1] = el;
     H.ntr = H.ntr - 1;
+    // the entry moving down is el at every step (each comparison is against it
+    // as the parent), so its key is read once and the chosen child's entry and
+    // key are the ones just read
+    const double kel = V.t(HPZ(el), HPX(el));
     int tpp = 1, tpc = 2 * tpp;
     while (tpc < H.ntr) {
-        int ec = H.btg[tpc], ec1 = H.btg[tpc + 1];
-        double rd1 = V.t(HPZ(ec), HPX(ec));
-        double rd2 = V.t(HPZ(ec1), HPX(ec1));
-        if (rd1 > rd2) tpc = tpc + 1;
-        ec = H.btg[tpc];
-        int ep = H.btg[tpp];
-        rd1 = V.t(HPZ(ec), HPX(ec));
-        rd2 = V.t(HPZ(ep), HPX(ep));
-        if (rd1 < rd2) {
-            V.n(HPZ(ep), HPX(ep)) = (int16_t)tpc;
+        const int e0 = H.btg[tpc], e1 = H.btg[tpc + 1];
+        const double rd1 = V.t(HPZ(e0), HPX(e0));
+        const double rd2 = V.t(HPZ(e1), HPX(e1));
+        const bool right = rd1 > rd2;
+        if (right) tpc = tpc + 1;
+        const int ec = right ? e1 : e0;
+        const double kc = right ? rd2 : rd1;
+        if (kc < kel) {
+            V.n(HPZ(el), HPX(el)) = (int16_t)tpc;
             V.n(HPZ(ec), HPX(ec)) = (int16_t)tpp;
-            H.btg[tpc] = ep;
+            H.btg[tpc] = el;
             H.btg[tpp] = ec;
             tpp = tpc;
             tpc = 2 * tpp;
@@ -758,13 +761,11 @@ __device__ __forceinline__ void ff_downtree(const fview &V, ff_heap &H)
         }
     }
     if (tpc == H.ntr) {
-        int ec = H.btg[tpc], ep = H.btg[tpp];
-        double rd1 = V.t(HPZ(ec), HPX(ec));
-        double rd2 = V.t(HPZ(ep), HPX(ep));
-        if (rd1 < rd2) {
-            V.n(HPZ(ep), HPX(ep)) = (int16_t)tpc;
+        const int ec = H.btg[tpc];
+        if (V.t(HPZ(ec), HPX(ec)) < kel) {
+            V.n(HPZ(el), HPX(el)) = (int16_t)tpc;
             V.n(HPZ(ec), HPX(ec)) = (int16_t)tpp;
-            H.btg[tpc] = ep;
+            H.btg[tpc] = el;
             H.btg[tpp] = ec;
         }
     }
@@ -779,7 +780,7 @@ __device__ __forceinline__ void ff_updtree(const fview &V, ff_heap &H, int iz, i
         if (tv < V.t(HPZ(ep), HPX(ep))) {
             V.n(iz, ix) = (int16_t)tpp;
             V.n(HPZ(ep), HPX(ep)) = (int16_t)tpc;
-            H.btg[tpp] = H.btg[tpc];
+            H.btg[tpp] = (ix << 16) | iz;      // the moving entry: the node's own
             H.btg[tpc] = ep;
             tpc = tpp;
             tpp = tpc / 2;
